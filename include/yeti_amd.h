@@ -1,0 +1,144 @@
+/*
+ * yeti_amd.h — C ABI of libyeti_hip.so, the MI355X (gfx950) implementation of YETI's
+ * 3D NURBS solid (element type U1) linear-elasticity hot path.
+ *
+ * Plain pointers and sizes only. Array layouts are exactly the ones YETI's f2py routines
+ * receive (Fortran column-major, 1-based control-point / span / element ids as given).
+ * Every function returns 0 on success or one of the YETI_ERR_* codes below; the message of
+ * the last error on the calling thread is available from yeti_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the YETI source tree):
+ *   yeti_sys_linmat_lindef_static   <- stiffmtrx_elemstorage.sys_linmat_lindef_static
+ *                                      src/yeti_iga/main/build_system_elemStorage.f90:24-315
+ *   (layout YETI_LAYOUT_SLOTTED)    <- stiffmtrx_elemstorage_omp.sys_linmat_lindef_static_omp
+ *                                      src/yeti_iga/main/build_system_elemStorage_OMP.f90:26-289
+ *   yeti_asm_*                      <- the same routines, split into set-up (geometry resident in
+ *                                      HBM) and run (device-resident outputs)
+ *   yeti_csr_*                      <- scipy COO->CSC + (K + K^T) done by the callers,
+ *                                      src/yeti_iga/overlay/iga_model.py:391-395
+ *   yeti_pcg_*                      <- scipy spsolve / the matrix-free CG iga_mf_cg_3d,
+ *                                      src/yeti_iga/iga_wq_mf/fsrc/mf_iga_solver.f90:642-958
+ *   yeti_mf_*                       <- mf_iga_get_ku_3d, src/yeti_iga/iga_wq_mf/fsrc/mf_iga_solver.f90:292-555
+ *   yeti_grad_compliance            <- optim.gradadjointwork.gradlinelastwork_an,
+ *                                      src/yeti_iga/optim/gradtotalwork_fullAN.f90:17-300
+ */
+#ifndef YETI_AMD_H
+#define YETI_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---------------------------------------------------------------------- */
+#define YETI_OK                      0
+#define YETI_ERR_INVALID_ARGUMENT    1  /* inconsistent sizes, null pointers, bad layout id  */
+#define YETI_ERR_UNSUPPORTED_ELEMENT 2  /* element type other than U1 / U0                    */
+#define YETI_ERR_UNSUPPORTED_DEGREE  3  /* degree / Gauss order with no compiled kernel       */
+#define YETI_ERR_SIZE_OVERFLOW       4  /* nb_data too small, or a dof id beyond int32        */
+#define YETI_ERR_DEVICE              5  /* HIP runtime error (includes out of memory)         */
+#define YETI_ERR_NOT_CONVERGED       6  /* iterative solve hit its iteration cap              */
+#define YETI_ERR_BUDGET              7  /* request rejected by the HBM admission budget       */
+
+/* ---- COO layouts ---------------------------------------------------------------------- */
+/* PACKED: inactive elements take no COO slots (build_system_elemStorage.f90:156,276).
+ * SLOTTED: element e of a patch owns slots [count + e*nb_data_elem, ...), inactive slots stay 0
+ *          (build_system_elemStorage_OMP.f90:229). */
+#define YETI_LAYOUT_PACKED  0
+#define YETI_LAYOUT_SLOTTED 1
+
+/* All inputs of sys_linmat_lindef_static, in the reference's argument order
+ * (build_system_elemStorage.f90:24-33; marshalled by IGA_parametrization.py:1705-1716).
+ * Fortran shapes are noted as (rows, cols); storage is column-major. */
+typedef struct yeti_system_inputs {
+    const int32_t *activeElement;        /* [nb_elem] 1 = assemble this element              */
+    int64_t        nb_data;              /* length of Kdata/Krow/Kcol                        */
+    const double  *COORDS3D;             /* (3, nb_cp)                                       */
+    const int32_t *IEN;                  /* concatenated per-patch (nnode, nb_elem_patch)    */
+    int64_t        len_IEN;
+    const int32_t *nb_elem_patch;        /* [nb_patch]                                       */
+    const int32_t *Nkv;                  /* (3, nb_patch) knot-vector lengths                */
+    const double  *Ukv;                  /* concatenated knot vectors                        */
+    int64_t        len_Ukv;
+    const int32_t *Nijk;                 /* (3, nb_elem) 1-based knot-span ids               */
+    const double  *weight;               /* concatenated per-element CP weights              */
+    int64_t        len_weight;
+    const int32_t *Jpqr;                 /* (3, nb_patch) degrees                            */
+    const char    *ELT_TYPE;             /* "U1U1..." (not NUL-terminated necessarily)       */
+    int64_t        len_ELT_TYPE;
+    const double  *PROPS;
+    int64_t        len_PROPS;
+    const int32_t *JPROPS;               /* [nb_patch]                                       */
+    const double  *MATERIAL_PROPERTIES;  /* (n_mat_props_max, nb_patch): E, nu, ...          */
+    const int32_t *n_mat_props;          /* [nb_patch]                                       */
+    const double  *RHO;                  /* [nb_patch]                                       */
+    const char    *TENSOR;               /* "/THREED/THREED..."                              */
+    int64_t        len_TENSOR;
+    const int32_t *indDLoad;             /* concatenated load targets (elements or CPs)     */
+    int64_t        len_indDLoad;
+    const int32_t *JDLType;              /* [nb_load]                                        */
+    const double  *ADLMAG;               /* [nb_load]                                        */
+    const int32_t *load_target_nbelem;   /* [nb_load]                                        */
+    const double  *load_additionalInfos;
+    int64_t        len_load_additionalInfos;
+    const int32_t *nb_load_additionalInfos; /* [nb_load]                                     */
+    const double  *bc_values;            /* (2, nb_bc)                                       */
+    const int32_t *bc_target;
+    int64_t        len_bc_target;
+    const int32_t *bc_target_nbelem;     /* [nb_bc]                                          */
+    const int32_t *ind_dof_free;         /* [nb_dof_tot] 1-based, zero padded                */
+    int32_t        nb_dof_free;
+    int32_t        MCRD;                 /* must be 3                                        */
+    const int32_t *NBINT;                /* [nb_patch] Gauss points per element              */
+    const int32_t *NNODE;                /* [nb_patch] CPs per element                       */
+    const double  *nodal_dist;           /* (nb_n_dist, nb_cp_n_dist)                        */
+    int32_t        nb_bc;
+    int32_t        nb_load;
+    int32_t        nb_patch;
+    int32_t        nb_elem;
+    int32_t        nb_cp;
+    int32_t        nb_dof_tot;
+    int32_t        nb_n_dist;
+    int32_t        nb_cp_n_dist;
+    int32_t        n_mat_props_max;
+} yeti_system_inputs;
+
+/* Last error message of the calling thread ("" when none). */
+const char *yeti_last_error(void);
+
+/* Library/ABI version, e.g. 0x000100 for 0.1.0. */
+int32_t yeti_version(void);
+
+/* Number of HIP devices visible (0 when none); never fails. */
+int32_t yeti_device_count(void);
+
+/* ---- drop-in: host in, host out --------------------------------------------------------
+ * Same contract as sys_linmat_lindef_static (build_system_elemStorage.f90:24-315): fills
+ * Kdata/Krow/Kcol (nb_data entries, upper-triangle element blocks with halved diagonals,
+ * 0-based dof ids) and F (nb_dof_tot). All pointers are host pointers. layout selects the
+ * serial (PACKED) or OpenMP (SLOTTED, build_system_elemStorage_OMP.f90:229) slot rule. */
+int32_t yeti_sys_linmat_lindef_static(const yeti_system_inputs *in, int32_t layout,
+                                      double *Kdata, int32_t *Krow, int32_t *Kcol, double *F);
+
+/* ---- device-resident assembly ----------------------------------------------------------
+ * Set-up copies the geometry/load description to the current HIP device once; run writes the
+ * COO triplets and F into caller-owned device buffers on `stream` (a hipStream_t, 0 = null
+ * stream). The handle may be run any number of times; it is not thread-safe. */
+typedef struct yeti_asm yeti_asm;
+
+int32_t yeti_asm_create(const yeti_system_inputs *in, int32_t layout, yeti_asm **out);
+/* Number of COO entries the layout needs (>= what Kdata must hold). */
+int64_t yeti_asm_nb_data(const yeti_asm *h);
+int32_t yeti_asm_run_coo(yeti_asm *h, double *Kdata_dev, int32_t *Krow_dev, int32_t *Kcol_dev,
+                         double *F_dev, void *stream);
+/* Values only (indices are a pure function of IEN and can be emitted once). */
+int32_t yeti_asm_run_coo_values(yeti_asm *h, double *Kdata_dev, void *stream);
+/* Right-hand side only (concentrated, surface and body loads). */
+int32_t yeti_asm_run_rhs(yeti_asm *h, double *F_dev, void *stream);
+void    yeti_asm_destroy(yeti_asm *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YETI_AMD_H */

@@ -1,0 +1,261 @@
+"""Python entry points of the oracle. TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module, and
+only as the checker. Two independent CPU implementations are exposed:
+
+* ``oracle_sys_linmat`` -- liboracle.so, our C restatement (oracle/yeti_oracle.c);
+* ``ref_sys_linmat_omp`` / ``ref_sys_linmat_serial`` / ``ref_dersbasisfuns`` / ``ref_gauss`` /
+  ``ref_grad_linelast_work`` -- oracle/_ref/libyetiref.so, YETI's own Fortran compiled from
+  its sources by oracle/build_ref.sh (present only where the YETI sources were available at
+  build time; callers must handle ``ref_available() == False``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from yeti_amd._abi import (YETI_LAYOUT_PACKED, YETI_LAYOUT_SLOTTED,  # noqa: E402
+                           args_to_dict, pack_system_inputs)
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        _lib = C.CDLL(path)
+        _lib.oracle_sys_linmat.restype = C.c_int
+        _lib.oracle_sys_linmat.argtypes = [C.c_void_p, C.c_int, C.c_int, _f64p, _i32p, _i32p, _f64p]
+        _lib.oracle_gauss.restype = C.c_int
+        _lib.oracle_gauss.argtypes = [C.c_int, _f64p, _f64p]
+        _lib.oracle_dersbasisfuns.restype = None
+        _lib.oracle_dersbasisfuns.argtypes = [C.c_int, C.c_int, C.c_double, _f64p, _f64p, _f64p]
+        _lib.oracle_element_matrix.restype = C.c_int
+        _lib.oracle_element_matrix.argtypes = [C.c_void_p, C.c_int64, _f64p, _f64p]
+        _lib.oracle_shape_at.restype = C.c_int
+        _lib.oracle_shape_at.argtypes = [C.c_void_p, C.c_int64, _f64p, _f64p, _f64p, _f64p]
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(os.path.join(HERE, "_ref", "libyetiref.so"))
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        path = os.path.join(HERE, "_ref", "libyetiref.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/_ref/libyetiref.so is not built (needs the YETI sources)")
+        _ref = C.CDLL(path)
+    return _ref
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _nb_data_for(s, layout):
+    return int(s.nb_data)
+
+
+# ---------------------------------------------------------------------------------------
+def oracle_sys_linmat(args, layout=YETI_LAYOUT_PACKED, nthreads=None, kwargs=None):
+    """Our C restatement of sys_linmat_lindef_static -> (Kdata, Krow, Kcol, F)."""
+    d = args_to_dict(args, kwargs)
+    s, keep = pack_system_inputs(d)
+    nd = int(s.nb_data)
+    K = np.zeros(nd, np.float64)
+    r = np.zeros(nd, np.int32)
+    c = np.zeros(nd, np.int32)
+    F = np.zeros(int(s.nb_dof_tot), np.float64)
+    if nthreads is None:
+        nthreads = os.cpu_count() or 1
+    rc = lib().oracle_sys_linmat(C.byref(s), int(layout), int(nthreads), _p(K, _f64p),
+                                 _p(r, _i32p), _p(c, _i32p), _p(F, _f64p))
+    if rc != 0:
+        raise RuntimeError("oracle_sys_linmat failed with code %d" % rc)
+    return K, r, c, F
+
+
+def oracle_element_matrix(args, elem):
+    d = args_to_dict(args)
+    s, keep = pack_system_inputs(d)
+    n = int(np.asarray(d["NNODE"]).ravel()[0])
+    A = np.zeros(9 * n * (n + 1) // 2)
+    R = np.zeros(3 * n)
+    rc = lib().oracle_element_matrix(C.byref(s), int(elem), _p(A, _f64p), _p(R, _f64p))
+    if rc != 0:
+        raise RuntimeError("oracle_element_matrix failed with code %d" % rc)
+    return A.reshape(-1, 3, 3).transpose(0, 2, 1), R   # [pair][kk][ll]
+
+
+def oracle_shape_at(args, elem, pt):
+    d = args_to_dict(args)
+    s, keep = pack_system_inputs(d)
+    n = int(np.asarray(d["NNODE"]).ravel()[0])
+    R = np.zeros(n)
+    dR = np.zeros((n, 3))
+    det = np.zeros(1)
+    ptv = np.ascontiguousarray(pt, dtype=np.float64)
+    rc = lib().oracle_shape_at(C.byref(s), int(elem), _p(ptv, _f64p), _p(R, _f64p),
+                               _p(dR, _f64p), _p(det, _f64p))
+    if rc != 0:
+        raise RuntimeError("oracle_shape_at failed with code %d" % rc)
+    return R, dR, float(det[0])
+
+
+def oracle_gauss(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    if lib().oracle_gauss(int(n), _p(x, _f64p), _p(w, _f64p)):
+        raise ValueError("unsupported Gauss order %d" % n)
+    return x, w
+
+
+def oracle_dersbasisfuns(span1, p, u, U):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    N = np.zeros(p + 1)
+    dN = np.zeros(p + 1)
+    lib().oracle_dersbasisfuns(int(span1), int(p), float(u), _p(U, _f64p), _p(N, _f64p), _p(dN, _f64p))
+    return N, dN
+
+
+# ---------------------------------------------------------------------------------------
+# The compiled reference
+def ref_gauss(n, dim=1, face=0):
+    """YETI's Gauss(norder, dim, GaussWtCoord, iFace) (main/GaussLegendre.f90:27)."""
+    out = np.zeros((dim + 1, n ** dim), order="F")
+    f = ref().gauss_
+    f.restype = None
+    f(C.byref(C.c_int(n)), C.byref(C.c_int(dim)), out.ctypes.data_as(_f64p), C.byref(C.c_int(face)))
+    return out
+
+
+def ref_dersbasisfuns(span1, p, u, U):
+    """YETI's dersbasisfuns (main/dersbasisfuns.f90:30)."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    N = np.zeros(p + 1)
+    dN = np.zeros(p + 1)
+    f = ref().dersbasisfuns_
+    f.restype = None
+    f(C.byref(C.c_int(span1)), C.byref(C.c_int(p)), C.byref(C.c_int(U.size)),
+      C.byref(C.c_double(u)), _p(U, _f64p), _p(N, _f64p), _p(dN, _f64p))
+    return N, dN
+
+
+def _ival(v):
+    return C.byref(C.c_int32(int(v)))
+
+
+def ref_sys_linmat_omp(args, num_threads=1, kwargs=None):
+    """YETI's sys_linmat_lindef_static_omp (build_system_elemStorage_OMP.f90:26), called with
+    the Fortran calling convention (all by reference, hidden CHARACTER lengths appended)."""
+    d = args_to_dict(args, kwargs)
+    s, keep = pack_system_inputs(d)
+    nd = int(s.nb_data)
+    K = np.zeros(nd, np.float64)
+    r = np.zeros(nd, np.int32)
+    c = np.zeros(nd, np.int32)
+    F = np.zeros(int(s.nb_dof_tot), np.float64)
+    f = ref().sys_linmat_lindef_static_omp_
+    f.restype = None
+    nb_data64 = C.c_int64(nd)
+    keep2 = []
+
+    def iv(v):
+        x = C.c_int32(int(v)); keep2.append(x); return C.byref(x)
+
+    f(_p(K, _f64p), _p(r, _i32p), _p(c, _i32p), _p(F, _f64p),
+      s.activeElement, C.byref(nb_data64), s.COORDS3D, s.IEN, s.nb_elem_patch, s.Nkv, s.Ukv,
+      s.Nijk, s.weight, s.Jpqr, s.ELT_TYPE, s.PROPS, s.JPROPS, s.MATERIAL_PROPERTIES,
+      s.n_mat_props, s.RHO, s.TENSOR, s.indDLoad, s.JDLType, s.ADLMAG, s.load_target_nbelem,
+      s.load_additionalInfos, s.nb_load_additionalInfos, s.bc_values, iv(s.nb_bc), s.bc_target,
+      s.bc_target_nbelem, s.ind_dof_free, iv(s.nb_dof_free), iv(s.MCRD), s.NBINT, iv(s.nb_load),
+      iv(s.nb_patch), iv(s.nb_elem), s.NNODE, iv(s.nb_cp), iv(s.nb_dof_tot), s.nodal_dist,
+      iv(s.nb_n_dist), iv(s.nb_cp_n_dist), iv(s.n_mat_props_max),
+      iv(s.len_Ukv), iv(s.len_weight), iv(max(1, s.len_PROPS)), iv(s.len_IEN),
+      iv(max(1, s.len_indDLoad)), iv(max(1, s.len_load_additionalInfos)), iv(s.nb_load),
+      iv(max(1, s.len_bc_target)), iv(num_threads),
+      C.c_size_t(s.len_ELT_TYPE), C.c_size_t(s.len_TENSOR))
+    return K, r, c, F
+
+
+def ref_sys_linmat_serial(args, kwargs=None):
+    """YETI's serial sys_linmat_lindef_static (build_system_elemStorage.f90:24) through the
+    bind(C) wrapper in oracle/ref_shim.f90. nb_data must fit int32 (as in the reference)."""
+    d = args_to_dict(args, kwargs)
+    s, keep = pack_system_inputs(d)
+    nd = int(s.nb_data)
+    if nd >= 2 ** 31:
+        raise ValueError("the serial reference counts nb_data in int32")
+    K = np.zeros(nd, np.float64)
+    r = np.zeros(nd, np.int32)
+    c = np.zeros(nd, np.int32)
+    F = np.zeros(int(s.nb_dof_tot), np.float64)
+    f = ref().ref_sys_linmat_serial
+    f.restype = None
+    keep2 = []
+
+    def iv(v):
+        x = C.c_int32(int(v)); keep2.append(x); return C.byref(x)
+
+    f(_p(K, _f64p), _p(r, _i32p), _p(c, _i32p), _p(F, _f64p), s.activeElement, iv(nd),
+      s.COORDS3D, s.IEN, iv(s.len_IEN), s.nb_elem_patch, s.Nkv, s.Ukv, iv(s.len_Ukv), s.Nijk,
+      s.weight, iv(s.len_weight), s.Jpqr, s.ELT_TYPE, iv(s.len_ELT_TYPE), s.PROPS,
+      iv(max(1, s.len_PROPS)), s.JPROPS, s.MATERIAL_PROPERTIES, s.n_mat_props, s.RHO, s.TENSOR,
+      iv(s.len_TENSOR), s.indDLoad, iv(max(1, s.len_indDLoad)), s.JDLType, s.ADLMAG,
+      s.load_target_nbelem, s.load_additionalInfos, iv(max(1, s.len_load_additionalInfos)),
+      s.nb_load_additionalInfos, s.bc_values, s.bc_target, iv(max(1, s.len_bc_target)),
+      s.bc_target_nbelem, s.ind_dof_free, iv(s.nb_dof_free), iv(s.MCRD), s.NBINT, s.NNODE,
+      s.nodal_dist, iv(s.nb_bc), iv(s.nb_load), iv(s.nb_patch), iv(s.nb_elem), iv(s.nb_cp),
+      iv(s.nb_dof_tot), iv(s.nb_n_dist), iv(s.nb_cp_n_dist), iv(s.n_mat_props_max))
+    return K, r, c, F
+
+
+def ref_grad_linelast_work(args, SOL, ADJ, compute_wint=True, compute_wext=True, active_dir=(1, 1, 1)):
+    """YETI's gradLinElastWork_AN (optim/gradtotalwork_fullAN.f90:17) through
+    oracle/ref_shim.f90. SOL: (3, nb_cp); ADJ: (nadj, 3, nb_cp). Returns (gradWint, gradWext),
+    each (nadj, 3, nb_cp)."""
+    d = args_to_dict(args)
+    s, keep = pack_system_inputs(d)
+    SOLf = np.asfortranarray(SOL, dtype=np.float64)
+    ADJf = np.asfortranarray(ADJ, dtype=np.float64)
+    nadj = ADJf.shape[0]
+    nb_cp = int(s.nb_cp)
+    gi = np.zeros((nadj, 3, nb_cp), order="F")
+    ge = np.zeros((nadj, 3, nb_cp), order="F")
+    ad = np.ascontiguousarray(active_dir, dtype=np.int32)
+    matp = np.asfortranarray(np.asarray(d["MATERIAL_PROPERTIES"], np.float64)[:2])
+    f = ref().ref_grad_linelast_work
+    f.restype = None
+    keep2 = []
+
+    def iv(v):
+        x = C.c_int32(int(v)); keep2.append(x); return C.byref(x)
+
+    f(gi.ctypes.data_as(_f64p), ge.ctypes.data_as(_f64p), iv(1 if compute_wint else 0),
+      iv(1 if compute_wext else 0), SOLf.ctypes.data_as(_f64p), ADJf.ctypes.data_as(_f64p),
+      iv(nadj), s.activeElement, _p(ad, _i32p), s.COORDS3D, s.IEN, iv(s.len_IEN), s.nb_elem_patch,
+      s.Nkv, s.Ukv, iv(s.len_Ukv), s.Nijk, s.weight, iv(s.len_weight), s.Jpqr, s.ELT_TYPE,
+      iv(s.len_ELT_TYPE), s.PROPS, iv(max(1, s.len_PROPS)), s.JPROPS, matp.ctypes.data_as(_f64p),
+      s.RHO, s.TENSOR, iv(s.len_TENSOR), s.indDLoad, iv(max(1, s.len_indDLoad)), s.JDLType,
+      s.ADLMAG, s.load_target_nbelem, s.load_additionalInfos,
+      iv(max(1, s.len_load_additionalInfos)), iv(s.MCRD), s.NBINT, iv(s.nb_load),
+      iv(s.nb_patch), iv(s.nb_elem), s.NNODE, iv(nb_cp))
+    return gi, ge

@@ -1,0 +1,142 @@
+! ref_shim.f90 -- C-callable drivers around the compiled YETI reference (oracle/_ref).
+!
+! TEST INFRASTRUCTURE ONLY: linked into oracle/_ref/libyetiref.so together with the
+! reference's own Fortran sources (compiled where they lie by oracle/build_ref.sh). The
+! reference's serial assembly and gradient drivers take assumed-shape arrays, so C cannot
+! call them directly; these bind(C) wrappers give them explicit interfaces. The OpenMP
+! assembly driver (explicit-size arrays) is called from Python without a wrapper.
+
+subroutine ref_sys_linmat_serial(Kdata, Krow, Kcol, F, activeElement, nb_data, COORDS3D,    &
+    &   IEN, len_IEN, nb_elem_patch, Nkv, Ukv, len_Ukv, Nijk, weight, len_weight, Jpqr,        &
+    &   ELT_TYPE, len_elt, PROPS, len_PROPS, JPROPS, MATERIAL_PROPERTIES, n_mat_props, RHO,    &
+    &   TENSOR, len_tensor, indDLoad, len_indDLoad, JDLType, ADLMAG, load_target_nbelem,       &
+    &   load_additionalInfos, len_lai, nb_load_additionalInfos, bc_values, bc_target,          &
+    &   len_bc_target, bc_target_nbelem, ind_dof_free, nb_dof_free, MCRD, NBINT, NNODE,        &
+    &   nodal_dist, nb_bc, nb_load, nb_patch, nb_elem, nb_cp, nb_dof_tot, nb_n_dist,           &
+    &   nb_cp_n_dist, n_mat_props_max) bind(C, name='ref_sys_linmat_serial')
+    use iso_c_binding
+    implicit none
+    integer(c_int), intent(in) :: nb_data, len_IEN, len_Ukv, len_weight, len_elt, len_PROPS,  &
+        &   len_tensor, len_indDLoad, len_lai, len_bc_target, nb_dof_free, MCRD, nb_bc,         &
+        &   nb_load, nb_patch, nb_elem, nb_cp, nb_dof_tot, nb_n_dist, nb_cp_n_dist,              &
+        &   n_mat_props_max
+    real(c_double), intent(out) :: Kdata(nb_data), F(nb_dof_tot)
+    integer(c_int), intent(out) :: Krow(nb_data), Kcol(nb_data)
+    integer(c_int), intent(in) :: activeElement(nb_elem), IEN(len_IEN), nb_elem_patch(nb_patch), &
+        &   Nkv(3, nb_patch), Nijk(3, nb_elem), Jpqr(3, nb_patch), JPROPS(nb_patch),             &
+        &   n_mat_props(nb_patch), indDLoad(len_indDLoad), JDLType(nb_load),                     &
+        &   load_target_nbelem(nb_load), nb_load_additionalInfos(nb_load), bc_target(len_bc_target), &
+        &   bc_target_nbelem(nb_bc), ind_dof_free(nb_dof_tot), NBINT(nb_patch), NNODE(nb_patch)
+    real(c_double), intent(in) :: COORDS3D(3, nb_cp), Ukv(len_Ukv), weight(len_weight),          &
+        &   PROPS(len_PROPS), MATERIAL_PROPERTIES(n_mat_props_max, nb_patch), RHO(nb_patch),      &
+        &   ADLMAG(nb_load), load_additionalInfos(len_lai), bc_values(2, nb_bc),                  &
+        &   nodal_dist(nb_n_dist, nb_cp_n_dist)
+    character(kind=c_char), intent(in) :: ELT_TYPE(len_elt), TENSOR(len_tensor)
+
+    character(len=len_elt) :: elt_s
+    character(len=len_tensor) :: tensor_s
+    integer :: i
+
+    interface
+        subroutine sys_linmat_lindef_static(Kdata, Krow, Kcol, F, activeElement, nb_data,       &
+            &   COORDS3D, IEN, nb_elem_patch, Nkv, Ukv, Nijk, weight, Jpqr, ELT_TYPE, PROPS,       &
+            &   JPROPS, MATERIAL_PROPERTIES, n_mat_props, RHO, TENSOR, indDLoad, JDLType, ADLMAG,  &
+            &   load_target_nbelem, load_additionalInfos, nb_load_additionalInfos, bc_values,      &
+            &   nb_bc, bc_target, bc_target_nbelem, ind_dof_free, nb_dof_free, MCRD, NBINT,        &
+            &   nb_load, nb_patch, nb_elem, nnode, nb_cp, nb_dof_tot, nodal_dist, nb_n_dist,       &
+            &   nb_cp_n_dist, n_mat_props_max)
+            integer, intent(in) :: nb_cp, nb_patch, nb_elem, nb_load, nb_bc, nb_dof_tot,        &
+                &   nb_dof_free, MCRD, nb_n_dist, nb_cp_n_dist, n_mat_props_max, nb_data
+            double precision, intent(in) :: COORDS3D(3, nb_cp), Ukv(:), weight(:), PROPS(:),    &
+                &   MATERIAL_PROPERTIES(n_mat_props_max, nb_patch), RHO(nb_patch),               &
+                &   ADLMAG(nb_load), load_additionalInfos(:), bc_values(2, nb_bc),               &
+                &   nodal_dist(nb_n_dist, nb_cp_n_dist)
+            integer, intent(in) :: Nkv(3, nb_patch), Jpqr(3, nb_patch), Nijk(3, nb_elem),       &
+                &   NNODE(nb_patch), IEN(:), nb_elem_patch(nb_patch), JPROPS(nb_patch),          &
+                &   NBINT(nb_patch), n_mat_props(nb_patch), indDLoad(:), JDLType(nb_load),       &
+                &   load_target_nbelem(nb_load), nb_load_additionalInfos(:), bc_target(:),       &
+                &   bc_target_nbelem(nb_bc), ind_dof_free(nb_dof_tot), activeElement(nb_elem)
+            character(len=*), intent(in) :: TENSOR, ELT_TYPE
+            integer, intent(out) :: Krow(nb_data), Kcol(nb_data)
+            double precision, intent(out) :: Kdata(nb_data), F(nb_dof_tot)
+        end subroutine
+    end interface
+
+    do i = 1, len_elt
+        elt_s(i:i) = ELT_TYPE(i)
+    enddo
+    do i = 1, len_tensor
+        tensor_s(i:i) = TENSOR(i)
+    enddo
+
+    call sys_linmat_lindef_static(Kdata, Krow, Kcol, F, activeElement, nb_data, COORDS3D, IEN,  &
+        &   nb_elem_patch, Nkv, Ukv, Nijk, weight, Jpqr, elt_s, PROPS, JPROPS,                    &
+        &   MATERIAL_PROPERTIES, n_mat_props, RHO, tensor_s, indDLoad, JDLType, ADLMAG,           &
+        &   load_target_nbelem, load_additionalInfos, nb_load_additionalInfos, bc_values, nb_bc,  &
+        &   bc_target, bc_target_nbelem, ind_dof_free, nb_dof_free, MCRD, NBINT, nb_load,         &
+        &   nb_patch, nb_elem, NNODE, nb_cp, nb_dof_tot, nodal_dist, nb_n_dist, nb_cp_n_dist,     &
+        &   n_mat_props_max)
+end subroutine ref_sys_linmat_serial
+
+
+subroutine ref_grad_linelast_work(gradWint, gradWext, computeWint, computeWext, SOL, ADJ, nadj, &
+    &   activeElement, activeDir, COORDS3D, IEN, len_IEN, nb_elem_patch, Nkv, Ukv, len_Ukv,       &
+    &   Nijk, weight, len_weight, Jpqr, ELT_TYPE, len_elt, PROPS, len_PROPS, JPROPS,              &
+    &   MATERIAL_PROPERTIES, RHO, TENSOR, len_tensor, indDLoad, len_indDLoad, JDLType, ADLMAG,    &
+    &   load_target_nbelem, load_additionalInfos, len_lai, MCRD, NBINT, nb_load, nb_patch,        &
+    &   nb_elem, NNODE, nb_cp) bind(C, name='ref_grad_linelast_work')
+    use iso_c_binding
+    implicit none
+    integer(c_int), intent(in) :: computeWint, computeWext, nadj, len_IEN, len_Ukv, len_weight,  &
+        &   len_elt, len_PROPS, len_tensor, len_indDLoad, len_lai, MCRD, nb_load, nb_patch,       &
+        &   nb_elem, nb_cp
+    real(c_double), intent(out) :: gradWint(nadj, 3, nb_cp), gradWext(nadj, 3, nb_cp)
+    real(c_double), intent(in) :: SOL(MCRD, nb_cp), ADJ(nadj, MCRD, nb_cp), COORDS3D(3, nb_cp),   &
+        &   Ukv(len_Ukv), weight(len_weight), PROPS(len_PROPS), MATERIAL_PROPERTIES(2, nb_patch), &
+        &   RHO(nb_patch), ADLMAG(nb_load), load_additionalInfos(len_lai)
+    integer(c_int), intent(in) :: activeElement(nb_elem), activeDir(3), IEN(len_IEN),            &
+        &   nb_elem_patch(nb_patch), Nkv(3, nb_patch), Nijk(3, nb_elem), Jpqr(3, nb_patch),       &
+        &   JPROPS(nb_patch), indDLoad(len_indDLoad), JDLType(nb_load),                           &
+        &   load_target_nbelem(nb_load), NBINT(nb_patch), NNODE(nb_patch)
+    character(kind=c_char), intent(in) :: ELT_TYPE(len_elt), TENSOR(len_tensor)
+
+    character(len=len_elt) :: elt_s
+    character(len=len_tensor) :: tensor_s
+    logical :: cwi, cwe
+    integer :: i
+
+    interface
+        subroutine gradLinElastWork_AN(gradWint, gradWext, computeWint, computeWext, SOL, ADJ,   &
+            &   nadj, activeElement, activeDir, COORDS3D, IEN, nb_elem_patch, Nkv, Ukv, Nijk,     &
+            &   weight, Jpqr, ELT_TYPE, PROPS, JPROPS, MATERIAL_PROPERTIES, RHO, TENSOR,          &
+            &   indDLoad, JDLType, ADLMAG, load_target_nbelem, load_additionalInfos, MCRD, NBINT, &
+            &   nb_load, nb_patch, nb_elem, nnode, nb_cp)
+            integer, intent(in) :: nb_cp, nb_patch, nb_elem, nb_load, MCRD, nadj
+            double precision, intent(in) :: COORDS3D(3, nb_cp), Ukv(:), weight(:), PROPS(:),    &
+                &   MATERIAL_PROPERTIES(2, nb_patch), RHO(nb_patch), ADLMAG(nb_load),            &
+                &   load_additionalInfos(:), SOL(MCRD, nb_cp), ADJ(nadj, MCRD, nb_cp)
+            integer, intent(in) :: Nkv(3, nb_patch), Jpqr(3, nb_patch), Nijk(3, nb_elem),       &
+                &   NNODE(nb_patch), IEN(:), nb_elem_patch(nb_patch), JPROPS(nb_patch),          &
+                &   NBINT(nb_patch), indDLoad(:), JDLType(nb_load), load_target_nbelem(nb_load), &
+                &   activeElement(nb_elem), activeDir(3)
+            logical, intent(in) :: computeWint, computeWext
+            character(len=*), intent(in) :: TENSOR, ELT_TYPE
+            double precision, intent(out) :: gradWint(nadj, 3, nb_cp), gradWext(nadj, 3, nb_cp)
+        end subroutine
+    end interface
+
+    do i = 1, len_elt
+        elt_s(i:i) = ELT_TYPE(i)
+    enddo
+    do i = 1, len_tensor
+        tensor_s(i:i) = TENSOR(i)
+    enddo
+    cwi = computeWint /= 0
+    cwe = computeWext /= 0
+
+    call gradLinElastWork_AN(gradWint, gradWext, cwi, cwe, SOL, ADJ, nadj, activeElement,         &
+        &   activeDir, COORDS3D, IEN, nb_elem_patch, Nkv, Ukv, Nijk, weight, Jpqr, elt_s, PROPS,  &
+        &   JPROPS, MATERIAL_PROPERTIES, RHO, tensor_s, indDLoad, JDLType, ADLMAG,                &
+        &   load_target_nbelem, load_additionalInfos, MCRD, NBINT, nb_load, nb_patch, nb_elem,    &
+        &   NNODE, nb_cp)
+end subroutine ref_grad_linelast_work

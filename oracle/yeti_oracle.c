@@ -1,0 +1,600 @@
+/*
+ * yeti_oracle.c — CPU restatement of YETI's 3D-solid (U1) stiffness / right-hand-side
+ * assembly, written from the reference's algorithm.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker: only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load liboracle.so. The product library
+ * (yeti_amd/csrc) never links, calls or falls back to it.
+ *
+ * Each routine cites the reference file:line it restates (paths relative to the YETI
+ * source tree src/yeti_iga/). Parity pins (what checks each routine independently):
+ *   oracle_gauss           -> numpy.polynomial.legendre.leggauss + the reference's 30-digit
+ *                             table, reached through oracle/_ref (tests/test_oracle_basis.py)
+ *   oracle_dersbasisfuns   -> scipy.interpolate.BSpline, partition of unity, and the
+ *                             reference's dersbasisfuns through oracle/_ref
+ *   oracle_elem_stiffness  -> rigid-body null space, symmetry, affine-geometry closed forms,
+ *                             brute-force 1-element p=1 hex, and the compiled reference
+ *                             (oracle/_ref) on seeded patches
+ *   oracle_sys_linmat      -> COO bit-exact indices + values vs oracle/_ref and the committed
+ *                             golden fixtures (tests/golden/)
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/yeti_amd.h"
+
+#define MAXP 9          /* max degree per direction handled by the oracle   */
+#define MAXG 16         /* max Gauss points per direction                   */
+
+/* ---------------------------------------------------------------------------------------
+ * Gauss-Legendre rule of order n on [-1,1], nodes ascending.
+ * Restates the rule tabulated in main/GaussLegendre.f90:27-160 (orders 1..9); computed here
+ * by Newton iteration on P_n (three-term recurrence), which agrees with the table to ~1 ulp.
+ * ------------------------------------------------------------------------------------- */
+int oracle_gauss(int n, double *x, double *w)
+{
+    if (n < 1 || n > MAXG) return -1;
+    for (int i = 0; i < n; ++i) {
+        /* i-th root counted from the most negative one */
+        double z = -cos(M_PI * (i + 0.75) / (n + 0.5));
+        double dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                double pk = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1; p1 = pk;
+            }
+            if (n == 1) { p1 = z; p0 = 1.0; }
+            dp = n * (z * p1 - p0) / (z * z - 1.0);
+            double dz = p1 / dp;
+            z -= dz;
+            if (fabs(dz) < 1e-17) break;
+        }
+        /* recompute derivative at the converged root */
+        double p0 = 1.0, p1 = z;
+        for (int k = 2; k <= n; ++k) {
+            double pk = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+            p0 = p1; p1 = pk;
+        }
+        dp = n * (z * p1 - p0) / (z * z - 1.0);
+        x[i] = z;
+        w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+    if (n % 2 == 1) x[n / 2] = 0.0;          /* the middle node is exactly 0 */
+    for (int i = 0; i < n / 2; ++i) {        /* exact symmetry, as in the table */
+        double xs = 0.5 * (x[n - 1 - i] - x[i]);
+        double ws = 0.5 * (w[i] + w[n - 1 - i]);
+        x[i] = -xs; x[n - 1 - i] = xs; w[i] = ws; w[n - 1 - i] = ws;
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------------
+ * Non-zero B-spline basis functions of degree p and their first derivatives at u, for the
+ * knot span whose 1-based index is `span1` (U[span1-1] <= u < U[span1] in 0-based terms).
+ * Same inputs/outputs as main/dersbasisfuns.f90:30-131 (Piegl & Tiller A2.3 with one
+ * derivative); computed here by the Cox-de Boor triangle and the closed-form derivative
+ *   N'_{k,p} = p N_{k,p-1}/(U_{k+p}-U_k) - p N_{k+1,p-1}/(U_{k+p+1}-U_{k+1}).
+ * N[r], dN[r] belong to global basis index span0 - p + r (span0 = span1 - 1).
+ * ------------------------------------------------------------------------------------- */
+void oracle_dersbasisfuns(int span1, int p, double u, const double *U, double *N, double *dN)
+{
+    int i = span1 - 1;                     /* 0-based span index */
+    double left[MAXP + 2], right[MAXP + 2];
+    double Nlow[MAXP + 2];                 /* degree p-1 values */
+    double cur[MAXP + 2];
+    cur[0] = 1.0;
+    for (int j = 1; j <= p; ++j) {
+        left[j] = u - U[i + 1 - j];
+        right[j] = U[i + j] - u;
+        double saved = 0.0;
+        for (int r = 0; r < j; ++r) {
+            double tmp = cur[r] / (right[r + 1] + left[j - r]);
+            cur[r] = saved + right[r + 1] * tmp;
+            saved = left[j - r] * tmp;
+        }
+        cur[j] = saved;
+        if (j == p - 1) for (int r = 0; r < p; ++r) Nlow[r] = cur[r];
+    }
+    if (p == 1) Nlow[0] = 1.0;
+    for (int r = 0; r <= p; ++r) N[r] = cur[r];
+    /* Nlow[s] is N_{i-p+1+s, p-1}, s = 0..p-1 */
+    for (int r = 0; r <= p; ++r) {
+        int k = i - p + r;                 /* global index of N_{k,p} */
+        double d = 0.0;
+        if (r >= 1) {                      /* N_{k,p-1} = Nlow[r-1] */
+            double den = U[k + p] - U[k];
+            if (den != 0.0) d += p * Nlow[r - 1] / den;
+        }
+        if (r <= p - 1) {                  /* N_{k+1,p-1} = Nlow[r] */
+            double den = U[k + p + 1] - U[k + 1];
+            if (den != 0.0) d -= p * Nlow[r] / den;
+        }
+        dN[r] = d;
+    }
+}
+
+/* Per-patch view of the flattened inputs (main/m_nurbspatch.f90:66-170). */
+typedef struct {
+    int p[3];               /* degrees                               */
+    int nkv[3];             /* knot-vector lengths                   */
+    const double *U[3];     /* knot vectors                          */
+    int nnode;
+    int nelem;
+    int64_t elem0;          /* global index of the patch's 1st elem  */
+    const int32_t *ien;     /* (nnode, nelem)                        */
+    const double *wgt;      /* (nnode, nelem)                        */
+    const int32_t *nijk;    /* (3, nelem)                            */
+    double E, nu, rho;
+    int nbint;
+    char elt[8];
+} opatch;
+
+/* Element-type string of patch ip ("U1U1..." split at 'U'; m_nurbspatch.f90:186-201). */
+static void patch_elt_type(const char *s, int64_t len, int ip, char *out)
+{
+    int cnt = -1; int64_t k = 0;
+    out[0] = 0;
+    while (k < len) {
+        if (s[k] == 'U') {
+            ++cnt;
+            if (cnt == ip) {
+                int o = 0; out[o++] = 'U'; ++k;
+                while (k < len && s[k] != 'U' && o < 7) out[o++] = s[k++];
+                out[o] = 0;
+                return;
+            }
+        }
+        ++k;
+    }
+}
+
+static int build_patch(const yeti_system_inputs *in, int ip, opatch *P)
+{
+    int64_t kv = 0, w = 0, ie = 0, el = 0;
+    for (int q = 0; q < ip; ++q) {
+        kv += in->Nkv[3 * q] + in->Nkv[3 * q + 1] + in->Nkv[3 * q + 2];
+        int nn = (in->Jpqr[3 * q] + 1) * (in->Jpqr[3 * q + 1] + 1) * (in->Jpqr[3 * q + 2] + 1);
+        w += (int64_t)in->nb_elem_patch[q] * nn;
+        ie += (int64_t)in->NNODE[q] * in->nb_elem_patch[q];
+        el += in->nb_elem_patch[q];
+    }
+    for (int d = 0; d < 3; ++d) {
+        P->p[d] = in->Jpqr[3 * ip + d];
+        P->nkv[d] = in->Nkv[3 * ip + d];
+        P->U[d] = in->Ukv + kv;
+        kv += P->nkv[d];
+        if (P->p[d] > MAXP) return YETI_ERR_UNSUPPORTED_DEGREE;
+    }
+    P->nnode = in->NNODE[ip];
+    P->nelem = in->nb_elem_patch[ip];
+    P->elem0 = el;
+    P->ien = in->IEN + ie;
+    P->wgt = in->weight + w;
+    P->nijk = in->Nijk + 3 * el;
+    P->E = in->MATERIAL_PROPERTIES[(int64_t)in->n_mat_props_max * ip + 0];
+    P->nu = in->MATERIAL_PROPERTIES[(int64_t)in->n_mat_props_max * ip + 1];
+    P->rho = in->RHO[ip];
+    P->nbint = in->NBINT[ip];
+    patch_elt_type(in->ELT_TYPE, in->len_ELT_TYPE, ip, P->elt);
+    return 0;
+}
+
+/* Gauss points per direction from NBINT (main/UELMAT.f90:95-96). */
+static int gauss_per_dir(int nbint)
+{
+    int n = (int)pow((double)nbint, 1.0 / 3.0);
+    if (n * n * n < nbint) n += 1;
+    return n;
+}
+
+/* ---------------------------------------------------------------------------------------
+ * Rational basis, physical derivatives and Jacobian determinant at parent point pt[3] of
+ * element e of patch P. Restates main/shap.f90:21-180 / main/shap_omp.f90:21-180.
+ * Outputs: R[n], dRdx[n][3], AJ (parent->physical 3x3, row=physical), *detJ.
+ * ------------------------------------------------------------------------------------- */
+static void oracle_shap(const opatch *P, int e, const double *coords, const double *pt,
+                        double *R, double (*dRdx)[3], double AJ[3][3], double *detJ)
+{
+    const int n = P->nnode;
+    const int32_t *nijk = P->nijk + 3 * e;
+    const double *we = P->wgt + (int64_t)n * e;
+    double xi[3], half[3];
+    double B[3][MAXP + 1], dB[3][MAXP + 1];
+    for (int d = 0; d < 3; ++d) {
+        int s = nijk[d];
+        double u0 = P->U[d][s - 1], u1 = P->U[d][s];
+        xi[d] = ((u1 - u0) * pt[d] + (u1 + u0)) * 0.5;
+        half[d] = 0.5 * (u1 - u0);
+        oracle_dersbasisfuns(s, P->p[d], xi[d], P->U[d], B[d], dB[d]);
+    }
+    /* numerators, local node <-> reversed basis index (shap.f90:94-119) */
+    double W = 0.0, dW[3] = {0, 0, 0};
+    double (*dRdxi)[3] = (double (*)[3])malloc(sizeof(double) * 3 * n);
+    int loc = 0;
+    for (int k = 0; k <= P->p[2]; ++k)
+        for (int j = 0; j <= P->p[1]; ++j)
+            for (int i = 0; i <= P->p[0]; ++i, ++loc) {
+                double a = B[0][P->p[0] - i], b = B[1][P->p[1] - j], c = B[2][P->p[2] - k];
+                double da = dB[0][P->p[0] - i], db = dB[1][P->p[1] - j], dc = dB[2][P->p[2] - k];
+                R[loc] = a * b * c * we[loc];
+                dRdxi[loc][0] = da * b * c * we[loc];
+                dRdxi[loc][1] = a * db * c * we[loc];
+                dRdxi[loc][2] = a * b * dc * we[loc];
+                W += R[loc];
+                dW[0] += dRdxi[loc][0]; dW[1] += dRdxi[loc][1]; dW[2] += dRdxi[loc][2];
+            }
+    double Winv = 1.0 / W;
+    for (int a = 0; a < n; ++a) {             /* quotient rule (shap.f90:122-129) */
+        R[a] *= Winv;
+        for (int d = 0; d < 3; ++d) dRdxi[a][d] = (dRdxi[a][d] - R[a] * dW[d]) * Winv;
+    }
+    double J[3][3] = {{0}};                   /* dx/dxi (shap.f90:133-140) */
+    for (int a = 0; a < n; ++a) {
+        const double *x = coords + 3 * (int64_t)(P->ien[(int64_t)n * e + a] - 1);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) J[r][c] += x[r] * dRdxi[a][c];
+    }
+    /* inverse by adjugate (operateurs.f:385-420) */
+    double det = J[0][1] * J[1][2] * J[2][0] - J[0][2] * J[1][1] * J[2][0]
+               + J[0][2] * J[1][0] * J[2][1] - J[0][0] * J[1][2] * J[2][1]
+               + J[0][0] * J[1][1] * J[2][2] - J[0][1] * J[1][0] * J[2][2];
+    double Ji[3][3];
+    Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+    Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+    Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    for (int a = 0; a < n; ++a)               /* dR/dx (shap.f90:146-153) */
+        for (int c = 0; c < 3; ++c) {
+            double s = 0.0;
+            for (int b = 0; b < 3; ++b) s += dRdxi[a][b] * Ji[b][c];
+            dRdx[a][c] = s;
+        }
+    for (int r = 0; r < 3; ++r)               /* AJ = dxdxi * diag(half) (shap.f90:156-165) */
+        for (int c = 0; c < 3; ++c) AJ[r][c] = J[r][c] * half[c];
+    *detJ = AJ[0][0] * AJ[1][1] * AJ[2][2] + AJ[0][1] * AJ[1][2] * AJ[2][0]
+          + AJ[1][0] * AJ[2][1] * AJ[0][2] - AJ[0][2] * AJ[1][1] * AJ[2][0]
+          - AJ[0][1] * AJ[1][0] * AJ[2][2] - AJ[1][2] * AJ[2][1] * AJ[0][0];
+    free(dRdxi);
+}
+
+/* Isotropic 3D elasticity matrix, Voigt order 11,22,33,12,13,23 (main/material_lib.f90:52-87). */
+static void oracle_ddsdde(double E, double nu, double D[6][6])
+{
+    double lam = (E * nu) / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    double mu = E / (2.0 * (1.0 + nu));
+    memset(D, 0, sizeof(double) * 36);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) D[i][j] = (i == j) ? lam + 2.0 * mu : lam;
+    D[3][3] = D[4][4] = D[5][5] = mu;
+}
+
+/* Strain-displacement block of node a, 6x3 (stiffmatrix.F90:162-173). */
+static void bmat(const double *g, double B[6][3])
+{
+    memset(B, 0, sizeof(double) * 18);
+    B[0][0] = g[0]; B[1][1] = g[1]; B[2][2] = g[2];
+    B[3][0] = g[1]; B[3][1] = g[0];
+    B[4][0] = g[2]; B[4][2] = g[0];
+    B[5][1] = g[2]; B[5][2] = g[1];
+}
+
+/* Surface Gauss point (face 1..6) per main/GaussLegendre.f90:205-262. */
+static void face_point(int face, double xi_i, double xi_j, double *pt)
+{
+    switch (face) {
+    case 1: pt[0] = -1.0; pt[1] = xi_i; pt[2] = xi_j; break;
+    case 2: pt[0] = 1.0;  pt[1] = xi_i; pt[2] = xi_j; break;
+    case 3: pt[0] = xi_i; pt[1] = -1.0; pt[2] = xi_j; break;
+    case 4: pt[0] = xi_i; pt[1] = 1.0;  pt[2] = xi_j; break;
+    case 5: pt[0] = xi_j; pt[1] = xi_i; pt[2] = -1.0; break;
+    default: pt[0] = xi_j; pt[1] = xi_i; pt[2] = 1.0; break;
+    }
+}
+
+static void cross(const double *a, const double *b, double *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static int elem_in_load(const yeti_system_inputs *in, int64_t k0, int iload, int64_t jelem1)
+{
+    for (int t = 0; t < in->load_target_nbelem[iload]; ++t)
+        if (in->indDLoad[k0 + t] == jelem1) return 1;
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------------
+ * Element matrix AMATRX(3,3,n(n+1)/2) (pairs nodi<=nodj, nodj-major) and RHS(3n) of one U1
+ * element. Restates main/UELMAT.f90:22-225 (body loads 101/102 at the volume Gauss points,
+ * surface loads 10..99 through shapPress.f:18-265) and stiffmatrix_byCP
+ * (main/stiffmatrix.F90:130-225). `with_102` = 0 reproduces uelmat_byCP_omp, which has no
+ * volume-force branch (main/uelmat_solid_omp.f90:98-140).
+ * ------------------------------------------------------------------------------------- */
+int oracle_elem(const yeti_system_inputs *in, const opatch *P, int e, int with_102,
+                double *AMATRX, double *RHS)
+{
+    const int n = P->nnode;
+    const int npair = n * (n + 1) / 2;
+    const int ng = gauss_per_dir(P->nbint);
+    double gx[MAXG], gw[MAXG];
+    if (oracle_gauss(ng, gx, gw)) return YETI_ERR_UNSUPPORTED_DEGREE;
+    double D[6][6];
+    oracle_ddsdde(P->E, P->nu, D);
+    memset(AMATRX, 0, sizeof(double) * 9 * npair);
+    memset(RHS, 0, sizeof(double) * 3 * n);
+    double *R = (double *)malloc(sizeof(double) * n);
+    double (*dRdx)[3] = (double (*)[3])malloc(sizeof(double) * 3 * n);
+    const int64_t jelem1 = P->elem0 + e + 1;
+    double coordsE[3 * 729];
+    for (int a = 0; a < n && a < 729; ++a)
+        for (int d = 0; d < 3; ++d)
+            coordsE[3 * a + d] = in->COORDS3D[3 * (int64_t)(P->ien[(int64_t)n * e + a] - 1) + d];
+
+    for (int gi = 0; gi < ng; ++gi)            /* 3D rule ordering, GaussLegendre.f90:175-186 */
+        for (int gj = 0; gj < ng; ++gj)
+            for (int gk = 0; gk < ng; ++gk) {
+                double pt[3] = {gx[gi], gx[gj], gx[gk]};
+                double wq = gw[gi] * gw[gj] * gw[gk];
+                double AJ[3][3], detJ;
+                oracle_shap(P, e, in->COORDS3D, pt, R, dRdx, AJ, &detJ);
+                double dvol = wq * fabs(detJ);
+                int cnt = 0;
+                for (int b = 0; b < n; ++b) {
+                    double Bj[6][3], SBj[6][3];
+                    bmat(dRdx[b], Bj);
+                    for (int r = 0; r < 6; ++r)
+                        for (int c = 0; c < 3; ++c) {
+                            double s = 0.0;
+                            for (int m = 0; m < 6; ++m) s += D[r][m] * Bj[m][c];
+                            SBj[r][c] = s;
+                        }
+                    for (int a = 0; a <= b; ++a, ++cnt) {
+                        double Bi[6][3];
+                        bmat(dRdx[a], Bi);
+                        for (int ll = 0; ll < 3; ++ll)
+                            for (int kk = 0; kk < 3; ++kk) {
+                                double s = 0.0;
+                                for (int m = 0; m < 6; ++m) s += Bi[m][kk] * SBj[m][ll];
+                                AMATRX[9 * cnt + kk + 3 * ll] += s * dvol;
+                            }
+                    }
+                }
+                /* body loads (UELMAT.f90:121-170) */
+                int64_t kload = 0, addc = 0;
+                for (int il = 0; il < in->nb_load; ++il) {
+                    int jdl = in->JDLType[il];
+                    if ((jdl == 101 || (jdl == 102 && with_102)) && elem_in_load(in, kload, il, jelem1)) {
+                        double vr[3];
+                        double mag;
+                        if (jdl == 101) {
+                            double gp[3] = {0, 0, 0};
+                            for (int a = 0; a < n; ++a)
+                                for (int d = 0; d < 3; ++d) gp[d] += R[a] * coordsE[3 * a + d];
+                            const double *A = in->load_additionalInfos + addc;
+                            const double *Bp = A + 3;
+                            double vd[3] = {Bp[0] - A[0], Bp[1] - A[1], Bp[2] - A[2]};
+                            double nd = sqrt(vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2]);
+                            for (int d = 0; d < 3; ++d) vd[d] /= nd;
+                            double ag[3] = {gp[0] - A[0], gp[1] - A[1], gp[2] - A[2]};
+                            double sc = ag[0] * vd[0] + ag[1] * vd[1] + ag[2] * vd[2];
+                            for (int d = 0; d < 3; ++d) vr[d] = ag[d] - sc * vd[d];
+                            mag = P->rho * in->ADLMAG[il] * in->ADLMAG[il];
+                        } else {
+                            const double *A = in->load_additionalInfos + addc;
+                            for (int d = 0; d < 3; ++d) vr[d] = A[d];
+                            mag = in->ADLMAG[il];
+                        }
+                        for (int a = 0; a < n; ++a)
+                            for (int d = 0; d < 3; ++d) RHS[3 * a + d] += mag * vr[d] * R[a] * dvol;
+                    }
+                    kload += in->load_target_nbelem[il];
+                    addc += in->nb_load_additionalInfos[il];
+                }
+            }
+
+    /* surface loads (UELMAT.f90:176-223, shapPress.f:18-265) */
+    int64_t kload = 0, addc = 0;
+    for (int il = 0; il < in->nb_load; ++il) {
+        int jdl = in->JDLType[il];
+        if (jdl > 9 && jdl < 100 && elem_in_load(in, kload, il, jelem1)) {
+            int face = jdl / 10, typ = jdl % 10;
+            int field = 0;
+            if (typ == 4) field = (int)in->load_additionalInfos[addc];
+            for (int fj = 0; fj < ng; ++fj)
+                for (int fi = 0; fi < ng; ++fi) {
+                    double pt[3];
+                    face_point(face, gx[fi], gx[fj], pt);
+                    double wq = gw[fi] * gw[fj];
+                    double AJ[3][3], detJ;
+                    oracle_shap(P, e, in->COORDS3D, pt, R, dRdx, AJ, &detJ);
+                    double c1[3], c2[3], c3[3];
+                    for (int d = 0; d < 3; ++d) { c1[d] = AJ[d][0]; c2[d] = AJ[d][1]; c3[d] = AJ[d][2]; }
+                    double nrm[3];
+                    if (face <= 2) cross(c2, c3, nrm);
+                    else if (face <= 4) cross(c3, c1, nrm);
+                    else cross(c1, c2, nrm);
+                    double dS = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+                    double v[3] = {0, 0, 0};
+                    if (typ == 1) v[0] = 1.0;
+                    else if (typ == 2) v[1] = 1.0;
+                    else if (typ == 3) v[2] = 1.0;
+                    else if (typ == 0 || typ == 4) {
+                        const double *A, *B2;
+                        switch (face) {
+                        case 1: A = c2; B2 = c3; break;
+                        case 2: A = c3; B2 = c2; break;
+                        case 3: A = c3; B2 = c1; break;
+                        case 4: A = c1; B2 = c3; break;
+                        case 5: A = c1; B2 = c2; break;
+                        default: A = c2; B2 = c1; break;
+                        }
+                        cross(A, B2, v);
+                        for (int d = 0; d < 3; ++d) v[d] /= dS;
+                    }
+                    double dvol = wq * dS;
+                    double fmag;
+                    if (typ == 4) {
+                        fmag = 0.0;
+                        for (int a = 0; a < n; ++a) {
+                            int cp = P->ien[(int64_t)n * e + a];
+                            fmag += in->nodal_dist[(int64_t)in->nb_n_dist * (cp - 1) + (field - 1)] * R[a];
+                        }
+                    } else {
+                        fmag = in->ADLMAG[il];
+                    }
+                    for (int a = 0; a < n; ++a)
+                        for (int d = 0; d < 3; ++d) RHS[3 * a + d] += R[a] * v[d] * dvol * fmag;
+                }
+        }
+        kload += in->load_target_nbelem[il];
+        addc += in->nb_load_additionalInfos[il];
+    }
+    free(R);
+    free(dRdx);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------------
+ * Whole-system restatement of sys_linmat_lindef_static (main/build_system_elemStorage.f90:24-315;
+ * layout SLOTTED = build_system_elemStorage_OMP.f90:26-289). Elements are independent, so
+ * the element loop runs with `nthreads` OpenMP threads; F is accumulated afterwards in
+ * element order.
+ * ------------------------------------------------------------------------------------- */
+int oracle_sys_linmat(const yeti_system_inputs *in, int layout, int nthreads,
+                      double *Kdata, int32_t *Krow, int32_t *Kcol, double *F)
+{
+    if (in->MCRD != 3) return YETI_ERR_INVALID_ARGUMENT;
+    memset(Kdata, 0, sizeof(double) * in->nb_data);
+    memset(Krow, 0, sizeof(int32_t) * in->nb_data);
+    memset(Kcol, 0, sizeof(int32_t) * in->nb_data);
+    memset(F, 0, sizeof(double) * in->nb_dof_tot);
+    /* concentrated loads: assignment (build_system_elemStorage.f90:137-152) */
+    int64_t kk = 0;
+    for (int il = 0; il < in->nb_load; ++il) {
+        int t = in->JDLType[il];
+        if (t / 10 < 1)
+            for (int c = 0; c < in->load_target_nbelem[il]; ++c)
+                F[(int64_t)(in->indDLoad[kk + c] - 1) * 3 + t - 1] = in->ADLMAG[il];
+        kk += in->load_target_nbelem[il];
+    }
+    int64_t count = 0;
+    for (int ip = 0; ip < in->nb_patch; ++ip) {
+        opatch P;
+        int rc = build_patch(in, ip, &P);
+        if (rc) return rc;
+        const int n = P.nnode;
+        const int64_t nde = 9LL * n * (n - 1) / 2 + 6LL * n;
+        int is_u1 = strcmp(P.elt, "U1") == 0, is_u0 = strcmp(P.elt, "U0") == 0;
+        /* per-element output slot */
+        int64_t *slot = (int64_t *)malloc(sizeof(int64_t) * (P.nelem + 1));
+        int64_t c = count;
+        for (int e = 0; e < P.nelem; ++e) {
+            int act = in->activeElement[P.elem0 + e] == 1;
+            if (layout == YETI_LAYOUT_SLOTTED) slot[e] = act ? count + (int64_t)e * nde : -1;
+            else { slot[e] = act ? c : -1; if (act) c += nde; }
+        }
+        int64_t next = (layout == YETI_LAYOUT_SLOTTED) ? count + (int64_t)P.nelem * nde : c;
+        if (next > in->nb_data) { free(slot); return YETI_ERR_SIZE_OVERFLOW; }
+        if (!is_u1 && !is_u0) {
+            int any = 0;
+            for (int e = 0; e < P.nelem; ++e) any |= slot[e] >= 0;
+            if (any) { free(slot); return YETI_ERR_UNSUPPORTED_ELEMENT; }
+        }
+        double *rhs_all = (double *)calloc((size_t)P.nelem * 3 * n, sizeof(double));
+        int err = 0;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads) reduction(| : err)
+        for (int e = 0; e < P.nelem; ++e) {
+            if (slot[e] < 0) continue;
+            double *A = (double *)malloc(sizeof(double) * 9 * n * (n + 1) / 2);
+            double *rhs = rhs_all + (int64_t)e * 3 * n;
+            if (is_u1) err |= oracle_elem(in, &P, e, layout == YETI_LAYOUT_PACKED, A, rhs);
+            else memset(A, 0, sizeof(double) * 9 * n * (n + 1) / 2);
+            /* COO emission (build_system_elemStorage.f90:260-293) */
+            const int32_t *sctr = P.ien + (int64_t)n * e;
+            int64_t pos = slot[e];
+            int pi = 0;
+            for (int b = 0; b < n; ++b) {
+                int32_t dofj = (sctr[b] - 1) * 3;
+                for (int a = 0; a < b; ++a, ++pi) {
+                    int32_t dofi = (sctr[a] - 1) * 3;
+                    for (int ll = 0; ll < 3; ++ll)
+                        for (int k2 = 0; k2 < 3; ++k2, ++pos) {
+                            Kdata[pos] = A[9 * pi + k2 + 3 * ll];
+                            Krow[pos] = dofi + k2;
+                            Kcol[pos] = dofj + ll;
+                        }
+                }
+                for (int ll = 0; ll < 3; ++ll) {
+                    A[9 * pi + ll + 3 * ll] *= 0.5;
+                    for (int k2 = 0; k2 <= ll; ++k2, ++pos) {
+                        Kdata[pos] = A[9 * pi + k2 + 3 * ll];
+                        Krow[pos] = dofj + k2;
+                        Kcol[pos] = dofj + ll;
+                    }
+                }
+                ++pi;
+            }
+            free(A);
+        }
+        /* F += RHS in element order (build_system_elemStorage.f90:295-303) */
+        for (int e = 0; e < P.nelem; ++e) {
+            if (slot[e] < 0) continue;
+            const int32_t *sctr = P.ien + (int64_t)n * e;
+            for (int a = 0; a < n; ++a)
+                for (int d = 0; d < 3; ++d)
+                    F[(int64_t)(sctr[a] - 1) * 3 + d] += rhs_all[(int64_t)e * 3 * n + 3 * a + d];
+        }
+        free(rhs_all);
+        free(slot);
+        if (err) return err;
+        count = next;
+    }
+    return 0;
+}
+
+/* Element matrix of one element (global 0-based index `elem`), for element-level checks.
+ * AMATRX: 9*n(n+1)/2, RHS: 3n. */
+int oracle_element_matrix(const yeti_system_inputs *in, int64_t elem, double *AMATRX, double *RHS)
+{
+    int64_t base = 0;
+    for (int ip = 0; ip < in->nb_patch; ++ip) {
+        if (elem < base + in->nb_elem_patch[ip]) {
+            opatch P;
+            int rc = build_patch(in, ip, &P);
+            if (rc) return rc;
+            return oracle_elem(in, &P, (int)(elem - base), 1, AMATRX, RHS);
+        }
+        base += in->nb_elem_patch[ip];
+    }
+    return YETI_ERR_INVALID_ARGUMENT;
+}
+
+/* Rational basis / derivatives / detJ at a parent point of element `elem` (patch 0 indexing
+ * over all patches), exposed for basis-level tests. */
+int oracle_shape_at(const yeti_system_inputs *in, int64_t elem, const double *pt,
+                    double *R, double *dRdx, double *detJ)
+{
+    int64_t base = 0;
+    for (int ip = 0; ip < in->nb_patch; ++ip) {
+        if (elem < base + in->nb_elem_patch[ip]) {
+            opatch P;
+            int rc = build_patch(in, ip, &P);
+            if (rc) return rc;
+            double AJ[3][3];
+            oracle_shap(&P, (int)(elem - base), in->COORDS3D, pt, R, (double (*)[3])dRdx, AJ, detJ);
+            return 0;
+        }
+        base += in->nb_elem_patch[ip];
+    }
+    return YETI_ERR_INVALID_ARGUMENT;
+}

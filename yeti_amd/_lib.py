@@ -1,0 +1,68 @@
+"""Loader of libyeti_hip.so (the HIP kernels behind include/yeti_amd.h).
+
+There is no CPU fallback: every entry point of this package fails loudly when the library
+is missing or when no HIP device is visible.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from ._abi import ERROR_NAMES, YetiSystemInputs
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libyeti_hip.so")
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); every symbol include/yeti_amd.h declares
+SIGNATURES = {
+    "yeti_last_error": (C.c_char_p, []),
+    "yeti_version": (C.c_int32, []),
+    "yeti_device_count": (C.c_int32, []),
+    "yeti_sys_linmat_lindef_static": (C.c_int32, [C.POINTER(YetiSystemInputs), C.c_int32, _f64p,
+                                                  _i32p, _i32p, _f64p]),
+    "yeti_asm_create": (C.c_int32, [C.POINTER(YetiSystemInputs), C.c_int32, C.POINTER(C.c_void_p)]),
+    "yeti_asm_nb_data": (C.c_int64, [C.c_void_p]),
+    "yeti_asm_run_coo": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]),
+    "yeti_asm_run_coo_values": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_asm_run_rhs": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_asm_destroy": (None, [C.c_void_p]),
+}
+
+_lib = None
+
+
+class YetiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (code %d): %s" % (ERROR_NAMES.get(code, "error"), code, msg))
+        self.code = code
+
+
+def lib():
+    """The loaded library. Raises when it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libyeti_hip.so is not built: run `python -m yeti_amd.csrc.build` "
+                               "(or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().yeti_last_error().decode(errors="replace")
+        raise YetiError(rc, msg)
+
+
+def require_device():
+    if lib().yeti_device_count() < 1:
+        raise RuntimeError("no HIP device visible: the YETI MI355X kernels need a GPU")

@@ -1,0 +1,97 @@
+"""Device-resident assembly handle (``yeti_asm_*`` of include/yeti_amd.h) over torch-owned HBM.
+
+Mirrors what ``sys_linmat_lindef_static`` (build_system_elemStorage.f90:24-315) computes, split
+into a one-time set-up that copies the model to the GPU and runs that write the COO triplets
+and F into device tensors. torch is used for device memory and streams only.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._abi import YETI_LAYOUT_PACKED, args_to_dict, pack_system_inputs
+
+
+def _stream_handle(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+class Assembler:
+    """Holds one model resident on the current HIP device.
+
+    ``args`` is the positional f2py argument list of sys_linmat_lindef_static (what
+    ``IGAparametrization.get_inputs4system_elemStorage()`` returns) or a dict of those
+    arguments by name."""
+
+    def __init__(self, args, layout=YETI_LAYOUT_PACKED, kwargs=None):
+        _lib.require_device()
+        d = args if isinstance(args, dict) else args_to_dict(args, kwargs)
+        self._inputs, self._keep = pack_system_inputs(d)
+        self.nb_dof_tot = int(self._inputs.nb_dof_tot)
+        self.nb_cp = int(self._inputs.nb_cp)
+        h = C.c_void_p()
+        _lib.check(_lib.lib().yeti_asm_create(C.byref(self._inputs), int(layout), C.byref(h)))
+        self._h = h
+        self.nb_data = int(_lib.lib().yeti_asm_nb_data(h))
+        # the host copies are no longer needed once the model is resident
+        self._keep = None
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().yeti_asm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------------------
+    def alloc_coo(self, with_indices=True, device="cuda"):
+        n = max(self.nb_data, 1)
+        K = torch.empty(n, dtype=torch.float64, device=device)
+        if not with_indices:
+            return K, None, None
+        r = torch.empty(n, dtype=torch.int32, device=device)
+        c = torch.empty(n, dtype=torch.int32, device=device)
+        return K, r, c
+
+    def run_coo(self, Kdata=None, Krow=None, Kcol=None, F=None, stream=None):
+        if Kdata is None:
+            Kdata, Krow, Kcol = self.alloc_coo(True)
+        if F is None:
+            F = torch.empty(self.nb_dof_tot, dtype=torch.float64, device=Kdata.device)
+        self._check_out(Kdata, Krow, Kcol, F)
+        _lib.check(_lib.lib().yeti_asm_run_coo(self._h, C.c_void_p(Kdata.data_ptr()),
+                                               C.c_void_p(Krow.data_ptr()),
+                                               C.c_void_p(Kcol.data_ptr()),
+                                               C.c_void_p(F.data_ptr()), _stream_handle(stream)))
+        return Kdata, Krow, Kcol, F
+
+    def run_values(self, Kdata, stream=None):
+        self._check_out(Kdata, None, None, None)
+        _lib.check(_lib.lib().yeti_asm_run_coo_values(self._h, C.c_void_p(Kdata.data_ptr()),
+                                                      _stream_handle(stream)))
+        return Kdata
+
+    def run_rhs(self, F=None, stream=None):
+        if F is None:
+            F = torch.empty(self.nb_dof_tot, dtype=torch.float64, device="cuda")
+        if F.dtype != torch.float64 or not F.is_cuda or F.numel() < self.nb_dof_tot:
+            raise ValueError("F must be a CUDA float64 tensor of at least nb_dof_tot entries")
+        _lib.check(_lib.lib().yeti_asm_run_rhs(self._h, C.c_void_p(F.data_ptr()), _stream_handle(stream)))
+        return F
+
+    def _check_out(self, K, r, c, F):
+        if K.dtype != torch.float64 or not K.is_cuda or not K.is_contiguous() or K.numel() < self.nb_data:
+            raise ValueError("Kdata must be a contiguous CUDA float64 tensor with >= %d entries" % self.nb_data)
+        for t in (r, c):
+            if t is not None and (t.dtype != torch.int32 or not t.is_cuda or t.numel() < self.nb_data):
+                raise ValueError("Krow/Kcol must be CUDA int32 tensors with >= %d entries" % self.nb_data)
+        if F is not None and (F.dtype != torch.float64 or not F.is_cuda or F.numel() < self.nb_dof_tot):
+            raise ValueError("F must be a CUDA float64 tensor with >= %d entries" % self.nb_dof_tot)

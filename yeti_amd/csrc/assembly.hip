@@ -1,0 +1,1157 @@
+// assembly.hip -- FP64 stiffness / RHS assembly of 3D NURBS solid elements (YETI type U1) on
+// MI355X (gfx950), behind the C ABI of include/yeti_amd.h.
+//
+// Reference path (paths relative to YETI's src/yeti_iga/):
+//   sys_linmat_lindef_static         main/build_system_elemStorage.f90:24-315
+//   sys_linmat_lindef_static_omp     main/build_system_elemStorage_OMP.f90:26-289
+//   UELMAT_byCP / uelmat_byCP_omp    main/UELMAT.f90:22-225 / main/uelmat_solid_omp.f90:1-200
+//   shap, dersbasisfuns              main/shap.f90:21-180, main/dersbasisfuns.f90:30-131
+//   stiffmatrix_byCP, material_lib   main/stiffmatrix.F90:130-225, main/material_lib.f90:28-98
+//   shapPress                        main/shapPress.f:18-265
+//
+// Element kernel (one workgroup per element):
+//   1. stage the element's CP coordinates, weights, connectivity and the 1D B-spline values /
+//      derivatives at its Gauss abscissae in LDS;
+//   2. per Gauss point, reduce W, dW, sum x N, sum x (x) dN over the nodes -> J^-1, |det J| w;
+//   3. write G_i[q][a] = d R_a / d x_i (q = Gauss point, a = node) to LDS;
+//   4. S^{ij} = G_i^T diag(w|J|) G_j on v_mfma_f64_16x16x4f64: one wave owns whole 16x16 node
+//      tiles (I <= J) and all nine (i, j) component tiles of them, so the isotropic
+//      contraction K_ab[k][l] = lam S^{kl}_ab + mu S^{lk}_ab + mu d_kl tr S_ab is lane-local;
+//   5. emit the element's upper-triangle COO slice in YETI's order (nodj-major, then nodi, then
+//      the 3x3 block column-major; diagonal blocks: the 6 upper entries, diagonal halved).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace yeti {
+
+std::string &last_error() {
+    static thread_local std::string s;
+    return s;
+}
+
+int set_error(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    last_error() = buf;
+    return code;
+}
+
+// ------------------------------------------------------------------------------------------
+// Gauss-Legendre rule (same nodes/weights as main/GaussLegendre.f90:27-160, computed by Newton
+// iteration on P_n; agrees with the 30-digit table to ~1 ulp).
+static void gauss_legendre(int n, double *x, double *w) {
+    for (int i = 0; i < n; ++i) {
+        double z = -std::cos(M_PI * (i + 0.75) / (n + 0.5));
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                double pk = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = pk;
+            }
+            double dp = (n == 1) ? 1.0 : n * (z * p1 - p0) / (z * z - 1.0);
+            double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-17) break;
+        }
+        double p0 = 1.0, p1 = z;
+        for (int k = 2; k <= n; ++k) {
+            double pk = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = pk;
+        }
+        double dp = (n == 1) ? 1.0 : n * (z * p1 - p0) / (z * z - 1.0);
+        x[i] = z;
+        w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+    if (n % 2 == 1) x[n / 2] = 0.0;
+    for (int i = 0; i < n / 2; ++i) {
+        double xs = 0.5 * (x[n - 1 - i] - x[i]);
+        double ws = 0.5 * (w[i] + w[n - 1 - i]);
+        x[i] = -xs;
+        x[n - 1 - i] = xs;
+        w[i] = ws;
+        w[n - 1 - i] = ws;
+    }
+}
+
+constexpr int MAX_NG = 9;
+
+struct GaussRule {
+    double x[MAX_NG];
+    double w[MAX_NG];
+};
+
+// ------------------------------------------------------------------------------------------
+// Device: non-zero B-splines of degree P and first derivatives at u for 0-based span i
+// (Cox-de Boor triangle + closed-form derivative; the contract of dersbasisfuns.f90:30-131).
+template <int P>
+__device__ inline void bspline_ders(int i, double u, const double *__restrict__ U, double *N,
+                                    double *dN) {
+    double left[P + 1], right[P + 1], cur[P + 1], low[P + 1];
+    cur[0] = 1.0;
+    low[0] = 1.0;
+#pragma unroll
+    for (int j = 1; j <= P; ++j) {
+        left[j] = u - U[i + 1 - j];
+        right[j] = U[i + j] - u;
+        double saved = 0.0;
+#pragma unroll
+        for (int r = 0; r < j; ++r) {
+            double tmp = cur[r] / (right[r + 1] + left[j - r]);
+            cur[r] = saved + right[r + 1] * tmp;
+            saved = left[j - r] * tmp;
+        }
+        cur[j] = saved;
+        if (j == P - 1) {
+#pragma unroll
+            for (int r = 0; r < P; ++r) low[r] = cur[r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r <= P; ++r) {
+        N[r] = cur[r];
+        int k = i - P + r;
+        double d = 0.0;
+        if (r >= 1) {
+            double den = U[k + P] - U[k];
+            if (den != 0.0) d += P * low[r - 1] / den;
+        }
+        if (r <= P - 1) {
+            double den = U[k + P + 1] - U[k + 1];
+            if (den != 0.0) d -= P * low[r] / den;
+        }
+        dN[r] = d;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Compile-time shape of the element kernel for degree P (all directions) and NG Gauss points
+// per direction.
+template <int P, int NG>
+struct ElemCfg {
+    static constexpr int NP1 = P + 1;
+    static constexpr int NN = NP1 * NP1 * NP1;          // nodes (CPs) per element
+    static constexpr int NT = (NN + 15) / 16;           // 16-node tiles
+    static constexpr int NNP = NT * 16;                 // padded node count
+    static constexpr int NQ = NG * NG * NG;             // Gauss points per element
+    static constexpr int NQ4 = (NQ + 3) / 4 * 4;        // padded to the MFMA K step
+    // Gauss-point chunk so that G (3 x QC x NNP doubles) stays within 96 KiB of LDS
+    static constexpr int QCMAX = (96 * 1024 / (3 * NNP * 8)) / 4 * 4;
+    static constexpr int QC = NQ4 < QCMAX ? NQ4 : QCMAX;
+    static constexpr int NCH = (NQ4 + QC - 1) / QC;
+    static constexpr int NPAIR = NT * (NT + 1) / 2;     // node-tile pairs I <= J
+    static constexpr int WAVES = NPAIR == 1 ? 1 : NPAIR == 3 ? 3 : NPAIR == 10 ? 5 : 6;
+    static constexpr int PPW = NPAIR <= 3 ? 1 : 2;       // tile pairs per wave per pass
+    static constexpr int NPASS = (NPAIR + WAVES * PPW - 1) / (WAVES * PPW);
+    static constexpr int NTH = WAVES * 64;
+    static constexpr int64_t NDE = 9LL * NN * (NN - 1) / 2 + 6LL * NN;  // COO entries
+    static constexpr int TPG = 4;                        // lanes per Gauss point in the reductions
+};
+
+struct PatchArgs {
+    const double *coords;     // (3, nb_cp)
+    const int32_t *ien;       // (nnode, nelem_patch), 1-based
+    const double *weight;     // (nnode, nelem_patch)
+    const int32_t *nijk;      // (3, nelem_patch), 1-based spans
+    const double *U0, *U1, *U2;  // knot vectors
+    const int32_t *elems;     // active elements (patch-local 0-based)
+    const int64_t *offs;      // COO offset per active element
+    int32_t n_active;
+    double lam, mu;
+    GaussRule g;
+};
+
+template <int P, int NG, bool WRITE_IDX>
+__global__ void __launch_bounds__((ElemCfg<P, NG>::NTH))
+elem_stiffness_kernel(PatchArgs pa, double *__restrict__ Kdata, int32_t *__restrict__ Krow,
+                      int32_t *__restrict__ Kcol) {
+    using C = ElemCfg<P, NG>;
+    constexpr int NP1 = C::NP1, NN = C::NN, NNP = C::NNP, QC = C::QC;
+
+    __shared__ double sG[3][QC][NNP];          // dR_a/dx_i at the chunk's Gauss points
+    __shared__ double sJi[QC][9];              // J^{-1} (row: xi-direction k, col: x-direction i)
+    __shared__ double sGp[QC][5];              // 1/W, dW/W (3), w|det J|
+    __shared__ double sX[NN][3];
+    __shared__ double sW[NN];
+    __shared__ int32_t sIen[NN];
+    __shared__ double sB[3][NG][NP1];          // B(reversed local index) at the NG abscissae
+    __shared__ double sdB[3][NG][NP1];
+    __shared__ double sHalf[3];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int ea = blockIdx.x;
+    if (ea >= pa.n_active) return;
+    const int e = pa.elems[ea];
+    const int64_t base = pa.offs[ea];
+
+    // ---- 1. stage element data -------------------------------------------------------
+    for (int a = tid; a < NN; a += C::NTH) {
+        int cp = pa.ien[(int64_t)NN * e + a];
+        sIen[a] = cp;
+        const double *x = pa.coords + 3 * (int64_t)(cp - 1);
+        sX[a][0] = x[0];
+        sX[a][1] = x[1];
+        sX[a][2] = x[2];
+        sW[a] = pa.weight[(int64_t)NN * e + a];
+    }
+    if (tid < 3 * NG) {
+        int d = tid / NG, g = tid % NG;
+        const double *U = d == 0 ? pa.U0 : d == 1 ? pa.U1 : pa.U2;
+        int span1 = pa.nijk[3 * (int64_t)e + d];
+        double u0 = U[span1 - 1], u1 = U[span1];
+        double u = ((u1 - u0) * pa.g.x[g] + (u1 + u0)) * 0.5;
+        double N[NP1], dN[NP1];
+        bspline_ders<P>(span1 - 1, u, U, N, dN);
+#pragma unroll
+        for (int i = 0; i < NP1; ++i) {   // local index i <-> basis function P - i
+            sB[d][g][i] = N[P - i];
+            sdB[d][g][i] = dN[P - i];
+        }
+        if (g == 0) sHalf[d] = 0.5 * (u1 - u0);
+    }
+    __syncthreads();
+
+    const int nI_lane = lane & 15;
+    const int kq_lane = lane >> 4;
+
+    for (int pass = 0; pass < C::NPASS; ++pass) {
+        f64x4 acc[C::PPW][9];
+#pragma unroll
+        for (int t = 0; t < C::PPW; ++t)
+#pragma unroll
+            for (int c = 0; c < 9; ++c) acc[t][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+        int tI[C::PPW], tJ[C::PPW];
+        bool tv[C::PPW];
+#pragma unroll
+        for (int t = 0; t < C::PPW; ++t) {
+            int pi = pass * C::WAVES * C::PPW + wave * C::PPW + t;
+            tv[t] = pi < C::NPAIR;
+            int J = 0;
+            while ((J + 1) * (J + 2) / 2 <= pi) ++J;
+            tJ[t] = J;
+            tI[t] = pi - J * (J + 1) / 2;
+        }
+
+        for (int ch = 0; ch < C::NCH; ++ch) {
+            const int q0 = ch * QC;
+            // ---- 2. per-Gauss-point reductions --------------------------------------
+            for (int grp = tid / C::TPG; grp < QC; grp += C::NTH / C::TPG) {
+                const int sub = tid % C::TPG;
+                const int q = q0 + grp;
+                double s[16];
+#pragma unroll
+                for (int m = 0; m < 16; ++m) s[m] = 0.0;
+                if (q < C::NQ) {
+                    const int g0 = q % NG, g1 = (q / NG) % NG, g2 = q / (NG * NG);
+                    for (int a = sub; a < NN; a += C::TPG) {
+                        const int i = a % NP1, j = (a / NP1) % NP1, k = a / (NP1 * NP1);
+                        double b0 = sB[0][g0][i], b1 = sB[1][g1][j], b2 = sB[2][g2][k];
+                        double w = sW[a];
+                        double Na = b0 * b1 * b2 * w;
+                        double d0 = sdB[0][g0][i] * b1 * b2 * w;
+                        double d1 = b0 * sdB[1][g1][j] * b2 * w;
+                        double d2 = b0 * b1 * sdB[2][g2][k] * w;
+                        double x0 = sX[a][0], x1 = sX[a][1], x2 = sX[a][2];
+                        s[0] += Na;
+                        s[1] += d0; s[2] += d1; s[3] += d2;
+                        s[4] += x0 * Na; s[5] += x1 * Na; s[6] += x2 * Na;
+                        s[7] += x0 * d0; s[8] += x0 * d1; s[9] += x0 * d2;
+                        s[10] += x1 * d0; s[11] += x1 * d1; s[12] += x1 * d2;
+                        s[13] += x2 * d0; s[14] += x2 * d1; s[15] += x2 * d2;
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < 16; ++m) {
+                    s[m] += __shfl_xor(s[m], 1, C::TPG);
+                    s[m] += __shfl_xor(s[m], 2, C::TPG);
+                }
+                if (sub == 0) {
+                    if (q < C::NQ) {
+                        const int g0 = q % NG, g1 = (q / NG) % NG, g2 = q / (NG * NG);
+                        double Wi = 1.0 / s[0];
+                        double dw[3] = {s[1] * Wi, s[2] * Wi, s[3] * Wi};
+                        double X[3] = {s[4], s[5], s[6]};
+                        // dx/dxi (row r = physical, col c = parametric)
+                        double J[3][3];
+#pragma unroll
+                        for (int r = 0; r < 3; ++r)
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) J[r][c] = (s[7 + 3 * r + c] - X[r] * dw[c]) * Wi;
+                        double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1])
+                                   - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0])
+                                   + J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+                        double id = 1.0 / det;
+                        // J^{-1}[c][r] = d xi_c / d x_r
+                        sJi[grp][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
+                        sJi[grp][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+                        sJi[grp][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+                        sJi[grp][3] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+                        sJi[grp][4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+                        sJi[grp][5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+                        sJi[grp][6] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+                        sJi[grp][7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+                        sJi[grp][8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+                        sGp[grp][0] = Wi;
+                        sGp[grp][1] = dw[0];
+                        sGp[grp][2] = dw[1];
+                        sGp[grp][3] = dw[2];
+                        double detJ = det * sHalf[0] * sHalf[1] * sHalf[2];
+                        sGp[grp][4] = pa.g.w[g0] * pa.g.w[g1] * pa.g.w[g2] * fabs(detJ);
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < 9; ++m) sJi[grp][m] = 0.0;
+#pragma unroll
+                        for (int m = 0; m < 5; ++m) sGp[grp][m] = 0.0;
+                    }
+                }
+            }
+            __syncthreads();
+            // ---- 3. physical basis derivatives G_i[q][a] -----------------------------
+            for (int idx = tid; idx < QC * NNP; idx += C::NTH) {
+                const int qq = idx / NNP, a = idx % NNP;
+                const int q = q0 + qq;
+                double g0v = 0.0, g1v = 0.0, g2v = 0.0;
+                if (a < NN && q < C::NQ) {
+                    const int g0 = q % NG, g1 = (q / NG) % NG, g2 = q / (NG * NG);
+                    const int i = a % NP1, j = (a / NP1) % NP1, k = a / (NP1 * NP1);
+                    double b0 = sB[0][g0][i], b1 = sB[1][g1][j], b2 = sB[2][g2][k];
+                    double w = sW[a];
+                    double Wi = sGp[qq][0];
+                    double R = b0 * b1 * b2 * w * Wi;
+                    double r0 = sdB[0][g0][i] * b1 * b2 * w * Wi - R * sGp[qq][1];
+                    double r1 = b0 * sdB[1][g1][j] * b2 * w * Wi - R * sGp[qq][2];
+                    double r2 = b0 * b1 * sdB[2][g2][k] * w * Wi - R * sGp[qq][3];
+                    const double *Ji = sJi[qq];
+                    g0v = r0 * Ji[0] + r1 * Ji[3] + r2 * Ji[6];
+                    g1v = r0 * Ji[1] + r1 * Ji[4] + r2 * Ji[7];
+                    g2v = r0 * Ji[2] + r1 * Ji[5] + r2 * Ji[8];
+                }
+                sG[0][qq][a] = g0v;
+                sG[1][qq][a] = g1v;
+                sG[2][qq][a] = g2v;
+            }
+            __syncthreads();
+            // ---- 4. S^{ij} += G_i^T diag(omega) G_j on MFMA ---------------------------
+            for (int ks = 0; ks < QC / 4; ++ks) {
+                const int qr = ks * 4 + kq_lane;
+                const double om = sGp[qr][4];
+#pragma unroll
+                for (int t = 0; t < C::PPW; ++t) {
+                    if (!tv[t]) continue;
+                    const int ca = 16 * tI[t] + nI_lane;
+                    const int cb = 16 * tJ[t] + nI_lane;
+                    double a0 = sG[0][qr][ca] * om, a1 = sG[1][qr][ca] * om, a2 = sG[2][qr][ca] * om;
+                    double b0 = sG[0][qr][cb], b1 = sG[1][qr][cb], b2 = sG[2][qr][cb];
+                    acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[t][0], 0, 0, 0);
+                    acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[t][1], 0, 0, 0);
+                    acc[t][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b2, acc[t][2], 0, 0, 0);
+                    acc[t][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[t][3], 0, 0, 0);
+                    acc[t][4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[t][4], 0, 0, 0);
+                    acc[t][5] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b2, acc[t][5], 0, 0, 0);
+                    acc[t][6] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b0, acc[t][6], 0, 0, 0);
+                    acc[t][7] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b1, acc[t][7], 0, 0, 0);
+                    acc[t][8] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b2, acc[t][8], 0, 0, 0);
+                }
+            }
+            if (C::NCH > 1) __syncthreads();
+        }
+
+        // ---- 5. isotropic contraction and COO emission -------------------------------
+        const double lam = pa.lam, mu = pa.mu;
+#pragma unroll
+        for (int t = 0; t < C::PPW; ++t) {
+            if (!tv[t]) continue;
+            const int b = 16 * tJ[t] + nI_lane;
+            if (b >= NN) continue;
+            const int64_t startb = base + 9LL * b * (b - 1) / 2 + 6LL * b;
+            const int32_t dofb = (sIen[b] - 1) * 3;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int a = 16 * tI[t] + kq_lane + 4 * r;
+                if (a > b) continue;
+                double c[9];
+#pragma unroll
+                for (int m = 0; m < 9; ++m) c[m] = acc[t][m][r];
+                const double tr = mu * (c[0] + c[4] + c[8]);
+                double K[3][3];
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+                    for (int ll = 0; ll < 3; ++ll)
+                        K[kk][ll] = lam * c[3 * kk + ll] + mu * c[3 * ll + kk] + (kk == ll ? tr : 0.0);
+                const int32_t dofa = (sIen[a] - 1) * 3;
+                if (a < b) {
+                    const int64_t pos = startb + 9LL * a;
+#pragma unroll
+                    for (int ll = 0; ll < 3; ++ll)
+#pragma unroll
+                        for (int kk = 0; kk < 3; ++kk) {
+                            Kdata[pos + kk + 3 * ll] = K[kk][ll];
+                            if (WRITE_IDX) {
+                                Krow[pos + kk + 3 * ll] = dofa + kk;
+                                Kcol[pos + kk + 3 * ll] = dofb + ll;
+                            }
+                        }
+                } else {
+                    const int64_t pos = startb + 9LL * b;
+                    Kdata[pos + 0] = 0.5 * K[0][0];
+                    Kdata[pos + 1] = K[0][1];
+                    Kdata[pos + 2] = 0.5 * K[1][1];
+                    Kdata[pos + 3] = K[0][2];
+                    Kdata[pos + 4] = K[1][2];
+                    Kdata[pos + 5] = 0.5 * K[2][2];
+                    if (WRITE_IDX) {
+                        const int rr[6] = {0, 0, 1, 0, 1, 2};
+                        const int cc[6] = {0, 1, 1, 2, 2, 2};
+#pragma unroll
+                        for (int m = 0; m < 6; ++m) {
+                            Krow[pos + m] = dofb + rr[m];
+                            Kcol[pos + m] = dofb + cc[m];
+                        }
+                    }
+                }
+            }
+        }
+        if (C::NPASS > 1 && C::NCH > 1) __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Right-hand side of loaded elements (UELMAT.f90:121-223): body loads 101/102 at the volume
+// Gauss points, surface loads 10..99 at the face Gauss points. One 64-thread block per loaded
+// element; each thread owns nodes a = lane, lane + 64, ...; block-wide sums go through LDS.
+struct LoadTable {
+    const int32_t *jdltype;        // [nb_load]
+    const double *adlmag;          // [nb_load]
+    const int32_t *add_off;        // [nb_load] offset into add_infos
+    const double *add_infos;
+    const int32_t *elem_loads;     // CSR: loads acting on each loaded element
+    const int32_t *elem_loads_ptr; // [n_loaded + 1]
+    const double *nodal_dist;      // (nb_n_dist, nb_cp)
+    int32_t nb_n_dist;
+};
+
+struct RhsPatch {
+    const double *coords;
+    const int32_t *ien;
+    const double *weight;
+    const int32_t *nijk;
+    const double *U[3];
+    const int32_t *elems;          // loaded elements of the patch (patch-local)
+    int32_t n_loaded;
+    int32_t slot0;                 // index of this patch's first loaded element in the RHS buffer
+    int32_t p[3];
+    int32_t ng;
+    double rho;
+    GaussRule g;
+};
+
+template <int MAXN>
+__device__ void rhs_basis(const RhsPatch &rp, int e, const double *pt, int nn, const double *sW,
+                          const double (*sX)[3], double *R, double *Wred, double AJ[3][3],
+                          double *detJ, double *sred) {
+    // parametric point and 1D bases (general degree up to 8)
+    const int lane = threadIdx.x;
+    __shared__ double sb[3][10], sdb[3][10], shalf[3];
+    if (lane < 3) {
+        int d = lane;
+        int span1 = rp.nijk[3 * (int64_t)e + d];
+        const double *U = rp.U[d];
+        double u0 = U[span1 - 1], u1 = U[span1];
+        double u = ((u1 - u0) * pt[d] + (u1 + u0)) * 0.5;
+        shalf[d] = 0.5 * (u1 - u0);
+        int p = rp.p[d];
+        int i = span1 - 1;
+        double left[10], right[10], cur[10], low[10];
+        cur[0] = 1.0;
+        low[0] = 1.0;
+        for (int j = 1; j <= p; ++j) {
+            left[j] = u - U[i + 1 - j];
+            right[j] = U[i + j] - u;
+            double saved = 0.0;
+            for (int r = 0; r < j; ++r) {
+                double tmp = cur[r] / (right[r + 1] + left[j - r]);
+                cur[r] = saved + right[r + 1] * tmp;
+                saved = left[j - r] * tmp;
+            }
+            cur[j] = saved;
+            if (j == p - 1)
+                for (int r = 0; r < p; ++r) low[r] = cur[r];
+        }
+        for (int r = 0; r <= p; ++r) {
+            int k = i - p + r;
+            double dd = 0.0;
+            if (r >= 1) {
+                double den = U[k + p] - U[k];
+                if (den != 0.0) dd += p * low[r - 1] / den;
+            }
+            if (r <= p - 1) {
+                double den = U[k + p + 1] - U[k + 1];
+                if (den != 0.0) dd -= p * low[r] / den;
+            }
+            sb[d][p - r] = cur[r];   // reversed local index
+            sdb[d][p - r] = dd;
+        }
+    }
+    __syncthreads();
+    const int n0 = rp.p[0] + 1, n1 = rp.p[1] + 1;
+    double s[16];
+    for (int m = 0; m < 16; ++m) s[m] = 0.0;
+    double Nl[MAXN / 64 + 1], dNl[MAXN / 64 + 1][3];
+    int c = 0;
+    for (int a = lane; a < nn; a += 64, ++c) {
+        int i = a % n0, j = (a / n0) % n1, k = a / (n0 * n1);
+        double w = sW[a];
+        double Na = sb[0][i] * sb[1][j] * sb[2][k] * w;
+        double d0 = sdb[0][i] * sb[1][j] * sb[2][k] * w;
+        double d1 = sb[0][i] * sdb[1][j] * sb[2][k] * w;
+        double d2 = sb[0][i] * sb[1][j] * sdb[2][k] * w;
+        Nl[c] = Na;
+        dNl[c][0] = d0; dNl[c][1] = d1; dNl[c][2] = d2;
+        s[0] += Na; s[1] += d0; s[2] += d1; s[3] += d2;
+        for (int r = 0; r < 3; ++r) {
+            s[4 + r] += sX[a][r] * Na;
+            s[7 + 3 * r] += sX[a][r] * d0;
+            s[8 + 3 * r] += sX[a][r] * d1;
+            s[9 + 3 * r] += sX[a][r] * d2;
+        }
+    }
+    for (int m = 0; m < 16; ++m) {
+        double v = s[m];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        s[m] = v;
+    }
+    double Wi = 1.0 / s[0];
+    double dw[3] = {s[1] * Wi, s[2] * Wi, s[3] * Wi};
+    double J[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int cc = 0; cc < 3; ++cc) J[r][cc] = (s[7 + 3 * r + cc] - s[4 + r] * dw[cc]) * Wi;
+    for (int r = 0; r < 3; ++r)
+        for (int cc = 0; cc < 3; ++cc) AJ[r][cc] = J[r][cc] * shalf[cc];
+    *detJ = AJ[0][0] * (AJ[1][1] * AJ[2][2] - AJ[1][2] * AJ[2][1])
+          - AJ[0][1] * (AJ[1][0] * AJ[2][2] - AJ[1][2] * AJ[2][0])
+          + AJ[0][2] * (AJ[1][0] * AJ[2][1] - AJ[1][1] * AJ[2][0]);
+    c = 0;
+    for (int a = lane; a < nn; a += 64, ++c) R[c] = Nl[c] * Wi;
+    (void)Wred;
+    (void)sred;
+    __syncthreads();
+}
+
+static __device__ inline void cross3(const double *a, const double *b, double *c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+constexpr int RHS_MAXN = 729;   // (8+1)^3
+
+__global__ void __launch_bounds__(64) elem_rhs_kernel(RhsPatch rp, LoadTable lt, double *rhs_buf) {
+    const int le = blockIdx.x;
+    if (le >= rp.n_loaded) return;
+    const int e = rp.elems[le];
+    const int lane = threadIdx.x;
+    const int nn = (rp.p[0] + 1) * (rp.p[1] + 1) * (rp.p[2] + 1);
+    __shared__ double sX[RHS_MAXN][3];
+    __shared__ double sW[RHS_MAXN];
+    __shared__ int32_t sIen[RHS_MAXN];
+    for (int a = lane; a < nn; a += 64) {
+        int cp = rp.ien[(int64_t)nn * e + a];
+        sIen[a] = cp;
+        for (int d = 0; d < 3; ++d) sX[a][d] = rp.coords[3 * (int64_t)(cp - 1) + d];
+        sW[a] = rp.weight[(int64_t)nn * e + a];
+    }
+    __syncthreads();
+    constexpr int NPL = RHS_MAXN / 64 + 1;
+    double rhs[NPL][3];
+    for (int c = 0; c < NPL; ++c) rhs[c][0] = rhs[c][1] = rhs[c][2] = 0.0;
+    double R[NPL];
+    const int ng = rp.ng;
+    const int gle = rp.slot0 + le;
+    const int l0 = lt.elem_loads_ptr[gle], l1 = lt.elem_loads_ptr[gle + 1];
+    bool any_body = false;
+    for (int li = l0; li < l1; ++li) {
+        int t = lt.jdltype[lt.elem_loads[li]];
+        any_body |= (t == 101 || t == 102);
+    }
+    if (any_body) {
+        for (int gi = 0; gi < ng; ++gi)
+            for (int gj = 0; gj < ng; ++gj)
+                for (int gk = 0; gk < ng; ++gk) {
+                    double pt[3] = {rp.g.x[gi], rp.g.x[gj], rp.g.x[gk]};
+                    double AJ[3][3], detJ;
+                    rhs_basis<RHS_MAXN>(rp, e, pt, nn, sW, sX, R, nullptr, AJ, &detJ, nullptr);
+                    double dvol = rp.g.w[gi] * rp.g.w[gj] * rp.g.w[gk] * fabs(detJ);
+                    for (int li = l0; li < l1; ++li) {
+                        int il = lt.elem_loads[li];
+                        int t = lt.jdltype[il];
+                        if (t != 101 && t != 102) continue;
+                        const double *A = lt.add_infos + lt.add_off[il];
+                        double vr[3], mag;
+                        if (t == 101) {
+                            double gp[3] = {0, 0, 0};
+                            int c = 0;
+                            for (int a = lane; a < nn; a += 64, ++c)
+                                for (int d = 0; d < 3; ++d) gp[d] += R[c] * sX[a][d];
+                            for (int d = 0; d < 3; ++d)
+                                for (int o = 32; o > 0; o >>= 1) gp[d] += __shfl_xor(gp[d], o, 64);
+                            double vd[3] = {A[3] - A[0], A[4] - A[1], A[5] - A[2]};
+                            double nd = sqrt(vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2]);
+                            for (int d = 0; d < 3; ++d) vd[d] /= nd;
+                            double ag[3] = {gp[0] - A[0], gp[1] - A[1], gp[2] - A[2]};
+                            double sc = ag[0] * vd[0] + ag[1] * vd[1] + ag[2] * vd[2];
+                            for (int d = 0; d < 3; ++d) vr[d] = ag[d] - sc * vd[d];
+                            mag = rp.rho * lt.adlmag[il] * lt.adlmag[il];
+                        } else {
+                            for (int d = 0; d < 3; ++d) vr[d] = A[d];
+                            mag = lt.adlmag[il];
+                        }
+                        int c = 0;
+                        for (int a = lane; a < nn; a += 64, ++c)
+                            for (int d = 0; d < 3; ++d) rhs[c][d] += mag * vr[d] * R[c] * dvol;
+                    }
+                }
+    }
+    for (int li = l0; li < l1; ++li) {
+        int il = lt.elem_loads[li];
+        int t = lt.jdltype[il];
+        if (!(t > 9 && t < 100)) continue;
+        int face = t / 10, typ = t % 10;
+        int field = typ == 4 ? (int)lt.add_infos[lt.add_off[il]] : 0;
+        for (int fj = 0; fj < ng; ++fj)
+            for (int fi = 0; fi < ng; ++fi) {
+                double xi_i = rp.g.x[fi], xi_j = rp.g.x[fj];
+                double pt[3];
+                switch (face) {
+                case 1: pt[0] = -1.0; pt[1] = xi_i; pt[2] = xi_j; break;
+                case 2: pt[0] = 1.0; pt[1] = xi_i; pt[2] = xi_j; break;
+                case 3: pt[0] = xi_i; pt[1] = -1.0; pt[2] = xi_j; break;
+                case 4: pt[0] = xi_i; pt[1] = 1.0; pt[2] = xi_j; break;
+                case 5: pt[0] = xi_j; pt[1] = xi_i; pt[2] = -1.0; break;
+                default: pt[0] = xi_j; pt[1] = xi_i; pt[2] = 1.0; break;
+                }
+                double AJ[3][3], detJ;
+                rhs_basis<RHS_MAXN>(rp, e, pt, nn, sW, sX, R, nullptr, AJ, &detJ, nullptr);
+                double c1[3] = {AJ[0][0], AJ[1][0], AJ[2][0]};
+                double c2[3] = {AJ[0][1], AJ[1][1], AJ[2][1]};
+                double c3[3] = {AJ[0][2], AJ[1][2], AJ[2][2]};
+                double nrm[3];
+                if (face <= 2) cross3(c2, c3, nrm);
+                else if (face <= 4) cross3(c3, c1, nrm);
+                else cross3(c1, c2, nrm);
+                double dS = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+                double v[3] = {0.0, 0.0, 0.0};
+                if (typ == 1) v[0] = 1.0;
+                else if (typ == 2) v[1] = 1.0;
+                else if (typ == 3) v[2] = 1.0;
+                else if (typ == 0 || typ == 4) {
+                    const double *A, *B;
+                    switch (face) {
+                    case 1: A = c2; B = c3; break;
+                    case 2: A = c3; B = c2; break;
+                    case 3: A = c3; B = c1; break;
+                    case 4: A = c1; B = c3; break;
+                    case 5: A = c1; B = c2; break;
+                    default: A = c2; B = c1; break;
+                    }
+                    cross3(A, B, v);
+                    for (int d = 0; d < 3; ++d) v[d] /= dS;
+                }
+                double dvol = rp.g.w[fi] * rp.g.w[fj] * dS;
+                double fmag;
+                if (typ == 4) {
+                    double fm = 0.0;
+                    int c = 0;
+                    for (int a = lane; a < nn; a += 64, ++c)
+                        fm += lt.nodal_dist[(int64_t)lt.nb_n_dist * (sIen[a] - 1) + (field - 1)] * R[c];
+                    for (int o = 32; o > 0; o >>= 1) fm += __shfl_xor(fm, o, 64);
+                    fmag = fm;
+                } else {
+                    fmag = lt.adlmag[il];
+                }
+                int c = 0;
+                for (int a = lane; a < nn; a += 64, ++c)
+                    for (int d = 0; d < 3; ++d) rhs[c][d] += R[c] * v[d] * dvol * fmag;
+            }
+    }
+    int c = 0;
+    for (int a = lane; a < nn; a += 64, ++c)
+        for (int d = 0; d < 3; ++d) rhs_buf[(int64_t)gle * 3 * RHS_MAXN + 3 * a + d] = rhs[c][d];
+}
+
+// F = F_conc + sum over loaded elements (in element order) of their RHS (build_system_elemStorage.f90:295-303).
+__global__ void rhs_gather_kernel(int32_t nb_cp, const double *__restrict__ F_conc,
+                                  const int32_t *__restrict__ cp_ptr,
+                                  const int32_t *__restrict__ cp_ent,
+                                  const double *__restrict__ rhs_buf, double *__restrict__ F) {
+    int cp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cp >= nb_cp) return;
+    double f0 = F_conc[3 * (int64_t)cp], f1 = F_conc[3 * (int64_t)cp + 1], f2 = F_conc[3 * (int64_t)cp + 2];
+    for (int k = cp_ptr[cp]; k < cp_ptr[cp + 1]; ++k) {
+        int ent = cp_ent[k];   // slot * RHS_MAXN + local node
+        const double *r = rhs_buf + 3 * (int64_t)ent;
+        f0 += r[0];
+        f1 += r[1];
+        f2 += r[2];
+    }
+    F[3 * (int64_t)cp] = f0;
+    F[3 * (int64_t)cp + 1] = f1;
+    F[3 * (int64_t)cp + 2] = f2;
+}
+
+// Zero a range of a COO buffer (slots of inactive elements in the SLOTTED layout).
+__global__ void fill_zero_kernel(double *Kd, int32_t *Kr, int32_t *Kc, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        Kd[i] = 0.0;
+        if (Kr) Kr[i] = 0;
+        if (Kc) Kc[i] = 0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Host side
+struct PatchDev {
+    int p[3];
+    int ng;
+    int nnode;
+    int nelem;
+    double lam, mu, rho;
+    bool u1;
+    DevBuf<double> U[3];
+    DevBuf<int32_t> ien, nijk, elems;
+    DevBuf<double> weight;
+    DevBuf<int64_t> offs;
+    int32_t n_active = 0;
+    DevBuf<int32_t> loaded;
+    int32_t n_loaded = 0;
+    int32_t slot0 = 0;
+};
+
+}  // namespace yeti
+
+struct yeti_asm {
+    int layout;
+    int64_t nb_data;        // entries the layout uses
+    int64_t nb_data_cap;    // nb_data requested by the caller
+    int32_t nb_cp, nb_dof_tot;
+    std::vector<yeti::PatchDev *> patches;
+    yeti::DevBuf<double> coords;
+    // zero ranges (slotted layout, inactive elements)
+    std::vector<std::pair<int64_t, int64_t>> zero_ranges;
+    // loads
+    yeti::DevBuf<double> F_conc;
+    yeti::DevBuf<int32_t> jdl, add_off, elem_loads, elem_loads_ptr, cp_ptr, cp_ent;
+    yeti::DevBuf<double> adlmag, add_infos, nodal_dist, rhs_buf;
+    int32_t nb_n_dist = 1;
+    int32_t n_loaded_total = 0;
+    ~yeti_asm() {
+        for (auto *p : patches) delete p;
+    }
+};
+
+namespace yeti {
+
+static int parse_elt_types(const yeti_system_inputs *in, std::vector<std::string> &out) {
+    out.clear();
+    std::string s(in->ELT_TYPE, in->ELT_TYPE + in->len_ELT_TYPE);
+    size_t k = 0;
+    while (k < s.size()) {
+        if (s[k] == 'U') {
+            size_t j = k + 1;
+            while (j < s.size() && s[j] != 'U' && s[j] != ' ' && s[j] != '\0') ++j;
+            out.push_back(s.substr(k, j - k));
+            k = j;
+        } else {
+            ++k;
+        }
+    }
+    if ((int)out.size() < in->nb_patch)
+        return set_error(YETI_ERR_INVALID_ARGUMENT, "ELT_TYPE lists %d types for %d patches",
+                         (int)out.size(), in->nb_patch);
+    return 0;
+}
+
+static int gauss_per_dir(int nbint) {
+    int n = (int)std::pow((double)nbint, 1.0 / 3.0);
+    if (n * n * n < nbint) n += 1;
+    return n;
+}
+
+static bool kernel_available(int p0, int p1, int p2, int ng) {
+    if (p0 != p1 || p1 != p2) return false;
+    return (p0 == 1 && ng == 2) || (p0 == 2 && ng == 3) || (p0 == 3 && ng == 4) || (p0 == 4 && ng == 5);
+}
+
+template <int P, int NG>
+static int launch_elem(const PatchDev &pd, const double *coords, double *Kd, int32_t *Kr,
+                       int32_t *Kc, hipStream_t st) {
+    if (pd.n_active == 0) return 0;
+    PatchArgs pa;
+    pa.coords = coords;
+    pa.ien = pd.ien.p;
+    pa.weight = pd.weight.p;
+    pa.nijk = pd.nijk.p;
+    pa.U0 = pd.U[0].p;
+    pa.U1 = pd.U[1].p;
+    pa.U2 = pd.U[2].p;
+    pa.elems = pd.elems.p;
+    pa.offs = pd.offs.p;
+    pa.n_active = pd.n_active;
+    pa.lam = pd.lam;
+    pa.mu = pd.mu;
+    gauss_legendre(NG, pa.g.x, pa.g.w);
+    dim3 grid(pd.n_active), block(ElemCfg<P, NG>::NTH);
+    if (Kr)
+        hipLaunchKernelGGL((elem_stiffness_kernel<P, NG, true>), grid, block, 0, st, pa, Kd, Kr, Kc);
+    else
+        hipLaunchKernelGGL((elem_stiffness_kernel<P, NG, false>), grid, block, 0, st, pa, Kd, Kr, Kc);
+    YETI_HIP(hipGetLastError());
+    return 0;
+}
+
+static int run_elements(yeti_asm *h, double *Kd, int32_t *Kr, int32_t *Kc, hipStream_t st) {
+    for (auto &zr : h->zero_ranges) {
+        int64_t n = zr.second;
+        int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks), dim3(256), 0, st, Kd + zr.first,
+                           Kr ? Kr + zr.first : nullptr, Kc ? Kc + zr.first : nullptr, n);
+        YETI_HIP(hipGetLastError());
+    }
+    for (auto *pd : h->patches) {
+        if (!pd->u1 || pd->n_active == 0) continue;
+        int rc = 0;
+        int key = pd->p[0] * 16 + pd->ng;
+        switch (key) {
+        case 1 * 16 + 2: rc = launch_elem<1, 2>(*pd, h->coords.p, Kd, Kr, Kc, st); break;
+        case 2 * 16 + 3: rc = launch_elem<2, 3>(*pd, h->coords.p, Kd, Kr, Kc, st); break;
+        case 3 * 16 + 4: rc = launch_elem<3, 4>(*pd, h->coords.p, Kd, Kr, Kc, st); break;
+        case 4 * 16 + 5: rc = launch_elem<4, 5>(*pd, h->coords.p, Kd, Kr, Kc, st); break;
+        default: return set_error(YETI_ERR_UNSUPPORTED_DEGREE, "no kernel for p=%d ng=%d", pd->p[0], pd->ng);
+        }
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+static int run_rhs(yeti_asm *h, double *F, hipStream_t st) {
+    LoadTable lt;
+    lt.jdltype = h->jdl.p;
+    lt.adlmag = h->adlmag.p;
+    lt.add_off = h->add_off.p;
+    lt.add_infos = h->add_infos.p;
+    lt.elem_loads = h->elem_loads.p;
+    lt.elem_loads_ptr = h->elem_loads_ptr.p;
+    lt.nodal_dist = h->nodal_dist.p;
+    lt.nb_n_dist = h->nb_n_dist;
+    for (auto *pd : h->patches) {
+        if (pd->n_loaded == 0) continue;
+        RhsPatch rp;
+        rp.coords = h->coords.p;
+        rp.ien = pd->ien.p;
+        rp.weight = pd->weight.p;
+        rp.nijk = pd->nijk.p;
+        for (int d = 0; d < 3; ++d) {
+            rp.U[d] = pd->U[d].p;
+            rp.p[d] = pd->p[d];
+        }
+        rp.elems = pd->loaded.p;
+        rp.n_loaded = pd->n_loaded;
+        rp.slot0 = pd->slot0;
+        rp.ng = pd->ng;
+        rp.rho = pd->rho;
+        gauss_legendre(pd->ng, rp.g.x, rp.g.w);
+        hipLaunchKernelGGL(elem_rhs_kernel, dim3(pd->n_loaded), dim3(64), 0, st, rp, lt, h->rhs_buf.p);
+        YETI_HIP(hipGetLastError());
+    }
+    int blocks = (h->nb_cp + 255) / 256;
+    hipLaunchKernelGGL(rhs_gather_kernel, dim3(blocks), dim3(256), 0, st, h->nb_cp, h->F_conc.p,
+                       h->cp_ptr.p, h->cp_ent.p, h->rhs_buf.p, F);
+    YETI_HIP(hipGetLastError());
+    return 0;
+}
+
+static int create(const yeti_system_inputs *in, int layout, yeti_asm **out) {
+    *out = nullptr;
+    if (!in) return set_error(YETI_ERR_INVALID_ARGUMENT, "null inputs");
+    if (layout != YETI_LAYOUT_PACKED && layout != YETI_LAYOUT_SLOTTED)
+        return set_error(YETI_ERR_INVALID_ARGUMENT, "unknown layout %d", layout);
+    if (in->MCRD != 3) return set_error(YETI_ERR_INVALID_ARGUMENT, "MCRD=%d, only 3D solids are supported", in->MCRD);
+    if (in->nb_patch < 1 || in->nb_cp < 1 || in->nb_elem < 0)
+        return set_error(YETI_ERR_INVALID_ARGUMENT, "empty model");
+    if ((int64_t)in->nb_cp * 3 > INT32_MAX)
+        return set_error(YETI_ERR_SIZE_OVERFLOW, "dof ids exceed int32");
+    std::vector<std::string> types;
+    int rc = parse_elt_types(in, types);
+    if (rc) return rc;
+
+    auto *h = new yeti_asm();
+    h->layout = layout;
+    h->nb_cp = in->nb_cp;
+    h->nb_dof_tot = in->nb_dof_tot;
+    h->nb_data_cap = in->nb_data;
+    h->nb_n_dist = std::max(1, in->nb_n_dist);
+    auto fail = [&](int code) {
+        delete h;
+        return code;
+    };
+    if (hipError_t e = h->coords.upload(in->COORDS3D, 3 * (size_t)in->nb_cp); e != hipSuccess)
+        return fail(set_error(YETI_ERR_DEVICE, "upload coords: %s", hipGetErrorString(e)));
+
+    // load bookkeeping: which loads act on which (global, 1-based) element
+    std::vector<int64_t> load_off(in->nb_load + 1, 0);
+    for (int il = 0; il < in->nb_load; ++il) load_off[il + 1] = load_off[il] + in->load_target_nbelem[il];
+    std::vector<std::vector<int32_t>> loads_of_elem(in->nb_elem);
+    for (int il = 0; il < in->nb_load; ++il) {
+        int t = in->JDLType[il];
+        bool elem_load = (t > 9 && t < 100) || t == 101 || t == 102;
+        if (!elem_load) continue;
+        for (int64_t k = load_off[il]; k < load_off[il + 1]; ++k) {
+            int64_t el = in->indDLoad[k];
+            if (el >= 1 && el <= in->nb_elem) {
+                auto &v = loads_of_elem[el - 1];
+                if (v.empty() || v.back() != il) v.push_back(il);
+            }
+        }
+    }
+
+    int64_t kv = 0, woff = 0, ieoff = 0, eloff = 0, count = 0;
+    std::vector<int32_t> elem_loads, elem_loads_ptr(1, 0);
+    std::vector<std::pair<int32_t, int64_t>> cp_contrib;   // (cp0, entry)
+    int32_t slot = 0;
+    for (int ip = 0; ip < in->nb_patch; ++ip) {
+        auto *pd = new PatchDev();
+        h->patches.push_back(pd);
+        for (int d = 0; d < 3; ++d) pd->p[d] = in->Jpqr[3 * ip + d];
+        int nkv[3] = {in->Nkv[3 * ip], in->Nkv[3 * ip + 1], in->Nkv[3 * ip + 2]};
+        pd->nnode = in->NNODE[ip];
+        pd->nelem = in->nb_elem_patch[ip];
+        int nn_geo = (pd->p[0] + 1) * (pd->p[1] + 1) * (pd->p[2] + 1);
+        const std::string &ty = types[ip];
+        pd->u1 = ty == "U1";
+        bool u0 = ty == "U0";
+        pd->ng = gauss_per_dir(in->NBINT[ip]);
+        double E = in->MATERIAL_PROPERTIES[(int64_t)in->n_mat_props_max * ip];
+        double nu = in->MATERIAL_PROPERTIES[(int64_t)in->n_mat_props_max * ip + 1];
+        pd->lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+        pd->mu = E / (2.0 * (1.0 + nu));
+        pd->rho = in->RHO[ip];
+        const int64_t nde = 9LL * pd->nnode * (pd->nnode - 1) / 2 + 6LL * pd->nnode;
+        // active elements and their slots
+        std::vector<int32_t> act;
+        std::vector<int64_t> offs;
+        int64_t c = count;
+        for (int e = 0; e < pd->nelem; ++e) {
+            bool a = in->activeElement[eloff + e] == 1;
+            if (layout == YETI_LAYOUT_SLOTTED) {
+                if (a) {
+                    act.push_back(e);
+                    offs.push_back(count + (int64_t)e * nde);
+                } else {
+                    h->zero_ranges.push_back({count + (int64_t)e * nde, nde});
+                }
+            } else if (a) {
+                act.push_back(e);
+                offs.push_back(c);
+                c += nde;
+            }
+        }
+        int64_t next = layout == YETI_LAYOUT_SLOTTED ? count + (int64_t)pd->nelem * nde : c;
+        if (!act.empty() && !pd->u1 && !u0)
+            return fail(set_error(YETI_ERR_UNSUPPORTED_ELEMENT, "patch %d: element type %s is not supported",
+                                  ip + 1, ty.c_str()));
+        if (!act.empty()) {
+            if (pd->nnode != nn_geo)
+                return fail(set_error(YETI_ERR_INVALID_ARGUMENT, "patch %d: NNODE=%d but degrees give %d",
+                                      ip + 1, pd->nnode, nn_geo));
+            if (!kernel_available(pd->p[0], pd->p[1], pd->p[2], pd->ng))
+                return fail(set_error(YETI_ERR_UNSUPPORTED_DEGREE,
+                                      "patch %d: degrees (%d,%d,%d) with %d Gauss points per direction",
+                                      ip + 1, pd->p[0], pd->p[1], pd->p[2], pd->ng));
+        }
+        // void elements (U0) keep their slots with zero values and real dof ids, as the
+        // reference emits its zero AMATRX (build_system_elemStorage.f90:185-188,260-293)
+        if (u0) pd->lam = pd->mu = 0.0;
+        pd->u1 = pd->u1 || u0;
+        pd->n_active = (int32_t)act.size();
+        hipError_t e1 = hipSuccess;
+        for (int d = 0; d < 3 && e1 == hipSuccess; ++d) {
+            e1 = pd->U[d].upload(in->Ukv + kv, nkv[d]);
+            kv += nkv[d];
+        }
+        if (e1 == hipSuccess) e1 = pd->ien.upload(in->IEN + ieoff, (size_t)pd->nnode * pd->nelem);
+        if (e1 == hipSuccess) e1 = pd->weight.upload(in->weight + woff, (size_t)nn_geo * pd->nelem);
+        if (e1 == hipSuccess) e1 = pd->nijk.upload(in->Nijk + 3 * eloff, 3 * (size_t)pd->nelem);
+        if (e1 == hipSuccess) e1 = pd->elems.upload(act.data(), act.size());
+        if (e1 == hipSuccess) e1 = pd->offs.upload(offs.data(), offs.size());
+        // loaded elements of this patch
+        std::vector<int32_t> loaded;
+        pd->slot0 = slot;
+        if (pd->u1 && !u0) {
+            for (int e = 0; e < pd->nelem; ++e) {
+                if (in->activeElement[eloff + e] != 1) continue;
+                auto &ls = loads_of_elem[eloff + e];
+                if (ls.empty()) continue;
+                loaded.push_back(e);
+                for (int il : ls) elem_loads.push_back(il);
+                elem_loads_ptr.push_back((int32_t)elem_loads.size());
+                for (int a = 0; a < pd->nnode; ++a) {
+                    int32_t cp = in->IEN[ieoff + (int64_t)pd->nnode * e + a] - 1;
+                    cp_contrib.push_back({cp, (int64_t)slot * RHS_MAXN + a});
+                }
+                ++slot;
+            }
+        }
+        pd->n_loaded = (int32_t)loaded.size();
+        if (e1 == hipSuccess) e1 = pd->loaded.upload(loaded.data(), loaded.size());
+        if (e1 != hipSuccess) return fail(set_error(YETI_ERR_DEVICE, "upload patch %d: %s", ip + 1, hipGetErrorString(e1)));
+        if (pd->u1 && pd->nnode > RHS_MAXN)
+            return fail(set_error(YETI_ERR_UNSUPPORTED_DEGREE, "patch %d: too many nodes", ip + 1));
+        woff += (int64_t)nn_geo * pd->nelem;
+        ieoff += (int64_t)pd->nnode * pd->nelem;
+        eloff += pd->nelem;
+        count = next;
+    }
+    h->nb_data = count;
+    h->n_loaded_total = slot;
+    // concentrated loads: assignment in load order (build_system_elemStorage.f90:137-152)
+    std::vector<double> Fc((size_t)in->nb_dof_tot, 0.0);
+    for (int il = 0; il < in->nb_load; ++il) {
+        int t = in->JDLType[il];
+        if (t / 10 < 1)
+            for (int64_t k = load_off[il]; k < load_off[il + 1]; ++k) {
+                int64_t dof = (int64_t)(in->indDLoad[k] - 1) * 3 + t - 1;
+                if (dof < 0 || dof >= in->nb_dof_tot)
+                    return fail(set_error(YETI_ERR_INVALID_ARGUMENT, "concentrated load %d targets dof %lld",
+                                          il + 1, (long long)dof));
+                Fc[dof] = in->ADLMAG[il];
+            }
+    }
+    // CP -> RHS entries, element order kept by a stable sort on the CP id
+    std::stable_sort(cp_contrib.begin(), cp_contrib.end(),
+                     [](const std::pair<int32_t, int64_t> &a, const std::pair<int32_t, int64_t> &b) {
+                         return a.first < b.first;
+                     });
+    std::vector<int32_t> cp_ptr(in->nb_cp + 1, 0), cp_ent(cp_contrib.size());
+    for (size_t k = 0; k < cp_contrib.size(); ++k) {
+        cp_ptr[cp_contrib[k].first + 1]++;
+        cp_ent[k] = (int32_t)cp_contrib[k].second;
+    }
+    for (int i = 0; i < in->nb_cp; ++i) cp_ptr[i + 1] += cp_ptr[i];
+    std::vector<int32_t> jdl(in->JDLType, in->JDLType + in->nb_load);
+    std::vector<int32_t> add_off(in->nb_load + 1, 0);
+    for (int il = 0; il < in->nb_load; ++il) add_off[il + 1] = add_off[il] + in->nb_load_additionalInfos[il];
+    hipError_t e2 = h->F_conc.upload(Fc.data(), Fc.size());
+    if (e2 == hipSuccess) e2 = h->jdl.upload(jdl.data(), jdl.size());
+    if (e2 == hipSuccess) e2 = h->adlmag.upload(in->ADLMAG, in->nb_load);
+    if (e2 == hipSuccess) e2 = h->add_off.upload(add_off.data(), add_off.size());
+    if (e2 == hipSuccess) e2 = h->add_infos.upload(in->load_additionalInfos, std::max<int64_t>(1, in->len_load_additionalInfos));
+    if (e2 == hipSuccess) e2 = h->elem_loads.upload(elem_loads.data(), elem_loads.size());
+    if (e2 == hipSuccess) e2 = h->elem_loads_ptr.upload(elem_loads_ptr.data(), elem_loads_ptr.size());
+    if (e2 == hipSuccess) e2 = h->cp_ptr.upload(cp_ptr.data(), cp_ptr.size());
+    if (e2 == hipSuccess) e2 = h->cp_ent.upload(cp_ent.data(), cp_ent.size());
+    if (e2 == hipSuccess)
+        e2 = h->nodal_dist.upload(in->nodal_dist, (size_t)std::max(1, in->nb_n_dist) * std::max(1, in->nb_cp_n_dist));
+    if (e2 == hipSuccess && slot > 0) e2 = h->rhs_buf.alloc((size_t)slot * 3 * RHS_MAXN);
+    if (e2 != hipSuccess) return fail(set_error(YETI_ERR_DEVICE, "upload loads: %s", hipGetErrorString(e2)));
+    *out = h;
+    return 0;
+}
+
+}  // namespace yeti
+
+// ------------------------------------------------------------------------------------------
+// C ABI
+extern "C" {
+
+const char *yeti_last_error(void) { return yeti::last_error().c_str(); }
+
+int32_t yeti_version(void) { return 0x000100; }
+
+int32_t yeti_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int32_t yeti_asm_create(const yeti_system_inputs *in, int32_t layout, yeti_asm **out) {
+    if (!out) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null handle pointer");
+    yeti::last_error().clear();
+    return yeti::create(in, layout, out);
+}
+
+int64_t yeti_asm_nb_data(const yeti_asm *h) { return h ? h->nb_data : -1; }
+
+int32_t yeti_asm_run_coo(yeti_asm *h, double *Kd, int32_t *Kr, int32_t *Kc, double *F, void *stream) {
+    if (!h || !Kd || !Kr || !Kc || !F) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    int rc = yeti::run_elements(h, Kd, Kr, Kc, st);
+    if (rc) return rc;
+    return yeti::run_rhs(h, F, st);
+}
+
+int32_t yeti_asm_run_coo_values(yeti_asm *h, double *Kd, void *stream) {
+    if (!h || !Kd) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    return yeti::run_elements(h, Kd, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int32_t yeti_asm_run_rhs(yeti_asm *h, double *F, void *stream) {
+    if (!h || !F) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    return yeti::run_rhs(h, F, (hipStream_t)stream);
+}
+
+void yeti_asm_destroy(yeti_asm *h) { delete h; }
+
+int32_t yeti_sys_linmat_lindef_static(const yeti_system_inputs *in, int32_t layout, double *Kdata,
+                                      int32_t *Krow, int32_t *Kcol, double *F) {
+    if (!in || !Kdata || !Krow || !Kcol || !F) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    yeti::last_error().clear();
+    yeti_asm *h = nullptr;
+    int rc = yeti::create(in, layout, &h);
+    if (rc) return rc;
+    int64_t nd = h->nb_data;
+    if (nd > in->nb_data) {
+        yeti_asm_destroy(h);
+        return yeti::set_error(YETI_ERR_SIZE_OVERFLOW, "nb_data=%lld but the layout needs %lld",
+                               (long long)in->nb_data, (long long)nd);
+    }
+    yeti::DevBuf<double> dK, dF;
+    yeti::DevBuf<int32_t> dR, dC;
+    hipError_t e = dK.alloc(std::max<int64_t>(nd, 1));
+    if (e == hipSuccess) e = dR.alloc(std::max<int64_t>(nd, 1));
+    if (e == hipSuccess) e = dC.alloc(std::max<int64_t>(nd, 1));
+    if (e == hipSuccess) e = dF.alloc(std::max(in->nb_dof_tot, 1));
+    if (e != hipSuccess) {
+        yeti_asm_destroy(h);
+        return yeti::set_error(YETI_ERR_DEVICE, "device allocation: %s", hipGetErrorString(e));
+    }
+    rc = yeti_asm_run_coo(h, dK.p, dR.p, dC.p, dF.p, nullptr);
+    yeti_asm_destroy(h);
+    if (rc) return rc;
+    YETI_HIP(hipMemcpy(Kdata, dK.p, nd * sizeof(double), hipMemcpyDeviceToHost));
+    YETI_HIP(hipMemcpy(Krow, dR.p, nd * sizeof(int32_t), hipMemcpyDeviceToHost));
+    YETI_HIP(hipMemcpy(Kcol, dC.p, nd * sizeof(int32_t), hipMemcpyDeviceToHost));
+    YETI_HIP(hipMemcpy(F, dF.p, in->nb_dof_tot * sizeof(double), hipMemcpyDeviceToHost));
+    // entries beyond what the layout uses stay zero, as the reference's zero-initialised arrays
+    if (in->nb_data > nd) {
+        std::memset(Kdata + nd, 0, (in->nb_data - nd) * sizeof(double));
+        std::memset(Krow + nd, 0, (in->nb_data - nd) * sizeof(int32_t));
+        std::memset(Kcol + nd, 0, (in->nb_data - nd) * sizeof(int32_t));
+    }
+    return 0;
+}
+
+}  // extern "C"
