@@ -132,6 +132,9 @@ int32_t yeti_asm_create(const yeti_system_inputs *in, int32_t layout, yeti_asm *
 int64_t yeti_asm_nb_data(const yeti_asm *h);
 int32_t yeti_asm_run_coo(yeti_asm *h, double *Kdata_dev, int32_t *Krow_dev, int32_t *Kcol_dev,
                          double *F_dev, void *stream);
+/* Matrix only: Kdata + Krow + Kcol, no right-hand side. */
+int32_t yeti_asm_run_coo_matrix(yeti_asm *h, double *Kdata_dev, int32_t *Krow_dev, int32_t *Kcol_dev,
+                                void *stream);
 /* Values only (indices are a pure function of IEN and can be emitted once). */
 int32_t yeti_asm_run_coo_values(yeti_asm *h, double *Kdata_dev, void *stream);
 /* Right-hand side only (concentrated, surface and body loads). */

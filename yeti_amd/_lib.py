@@ -27,6 +27,7 @@ SIGNATURES = {
     "yeti_asm_nb_data": (C.c_int64, [C.c_void_p]),
     "yeti_asm_run_coo": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p]),
+    "yeti_asm_run_coo_matrix": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_asm_run_coo_values": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_asm_run_rhs": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_asm_destroy": (None, [C.c_void_p]),

@@ -73,6 +73,14 @@ class Assembler:
                                                C.c_void_p(F.data_ptr()), _stream_handle(stream)))
         return Kdata, Krow, Kcol, F
 
+    def run_matrix(self, Kdata, Krow, Kcol, stream=None):
+        self._check_out(Kdata, Krow, Kcol, None)
+        _lib.check(_lib.lib().yeti_asm_run_coo_matrix(self._h, C.c_void_p(Kdata.data_ptr()),
+                                                      C.c_void_p(Krow.data_ptr()),
+                                                      C.c_void_p(Kcol.data_ptr()),
+                                                      _stream_handle(stream)))
+        return Kdata, Krow, Kcol
+
     def run_values(self, Kdata, stream=None):
         self._check_out(Kdata, None, None, None)
         _lib.check(_lib.lib().yeti_asm_run_coo_values(self._h, C.c_void_p(Kdata.data_ptr()),

@@ -1103,6 +1103,11 @@ int32_t yeti_asm_run_coo(yeti_asm *h, double *Kd, int32_t *Kr, int32_t *Kc, doub
     return yeti::run_rhs(h, F, st);
 }
 
+int32_t yeti_asm_run_coo_matrix(yeti_asm *h, double *Kd, int32_t *Kr, int32_t *Kc, void *stream) {
+    if (!h || !Kd || !Kr || !Kc) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    return yeti::run_elements(h, Kd, Kr, Kc, (hipStream_t)stream);
+}
+
 int32_t yeti_asm_run_coo_values(yeti_asm *h, double *Kd, void *stream) {
     if (!h || !Kd) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
     return yeti::run_elements(h, Kd, nullptr, nullptr, (hipStream_t)stream);
