@@ -133,300 +133,7 @@ __device__ inline void bspline_ders(int i, double u, const double *__restrict__ 
     }
 }
 
-// ------------------------------------------------------------------------------------------
-// Compile-time shape of the element kernel for degree P (all directions) and NG Gauss points
-// per direction.
-template <int P, int NG>
-struct ElemCfg {
-    static constexpr int NP1 = P + 1;
-    static constexpr int NN = NP1 * NP1 * NP1;          // nodes (CPs) per element
-    static constexpr int NT = (NN + 15) / 16;           // 16-node tiles
-    static constexpr int NNP = NT * 16;                 // padded node count
-    static constexpr int NQ = NG * NG * NG;             // Gauss points per element
-    static constexpr int NQ4 = (NQ + 3) / 4 * 4;        // padded to the MFMA K step
-    // Gauss-point chunk so that G (3 x QC x NNP doubles) stays within 96 KiB of LDS
-    static constexpr int QCMAX = (96 * 1024 / (3 * NNP * 8)) / 4 * 4;
-    static constexpr int QC = NQ4 < QCMAX ? NQ4 : QCMAX;
-    static constexpr int NCH = (NQ4 + QC - 1) / QC;
-    static constexpr int NPAIR = NT * (NT + 1) / 2;     // node-tile pairs I <= J
-    static constexpr int WAVES = NPAIR == 1 ? 1 : NPAIR == 3 ? 3 : NPAIR == 10 ? 5 : 6;
-    static constexpr int PPW = NPAIR <= 3 ? 1 : 2;       // tile pairs per wave per pass
-    static constexpr int NPASS = (NPAIR + WAVES * PPW - 1) / (WAVES * PPW);
-    static constexpr int NTH = WAVES * 64;
-    static constexpr int64_t NDE = 9LL * NN * (NN - 1) / 2 + 6LL * NN;  // COO entries
-    static constexpr int TPG = 4;                        // lanes per Gauss point in the reductions
-};
-
-struct PatchArgs {
-    const double *coords;     // (3, nb_cp)
-    const int32_t *ien;       // (nnode, nelem_patch), 1-based
-    const double *weight;     // (nnode, nelem_patch)
-    const int32_t *nijk;      // (3, nelem_patch), 1-based spans
-    const double *U0, *U1, *U2;  // knot vectors
-    const int32_t *elems;     // active elements (patch-local 0-based)
-    const int64_t *offs;      // COO offset per active element
-    int32_t n_active;
-    double lam, mu;
-    GaussRule g;
-};
-
-template <int P, int NG, bool WRITE_IDX>
-__global__ void __launch_bounds__((ElemCfg<P, NG>::NTH))
-elem_stiffness_kernel(PatchArgs pa, double *__restrict__ Kdata, int32_t *__restrict__ Krow,
-                      int32_t *__restrict__ Kcol) {
-    using C = ElemCfg<P, NG>;
-    constexpr int NP1 = C::NP1, NN = C::NN, NNP = C::NNP, QC = C::QC;
-
-    __shared__ double sG[3][QC][NNP];          // dR_a/dx_i at the chunk's Gauss points
-    __shared__ double sJi[QC][9];              // J^{-1} (row: xi-direction k, col: x-direction i)
-    __shared__ double sGp[QC][5];              // 1/W, dW/W (3), w|det J|
-    __shared__ double sX[NN][3];
-    __shared__ double sW[NN];
-    __shared__ int32_t sIen[NN];
-    __shared__ double sB[3][NG][NP1];          // B(reversed local index) at the NG abscissae
-    __shared__ double sdB[3][NG][NP1];
-    __shared__ double sHalf[3];
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int ea = blockIdx.x;
-    if (ea >= pa.n_active) return;
-    const int e = pa.elems[ea];
-    const int64_t base = pa.offs[ea];
-
-    // ---- 1. stage element data -------------------------------------------------------
-    for (int a = tid; a < NN; a += C::NTH) {
-        int cp = pa.ien[(int64_t)NN * e + a];
-        sIen[a] = cp;
-        const double *x = pa.coords + 3 * (int64_t)(cp - 1);
-        sX[a][0] = x[0];
-        sX[a][1] = x[1];
-        sX[a][2] = x[2];
-        sW[a] = pa.weight[(int64_t)NN * e + a];
-    }
-    if (tid < 3 * NG) {
-        int d = tid / NG, g = tid % NG;
-        const double *U = d == 0 ? pa.U0 : d == 1 ? pa.U1 : pa.U2;
-        int span1 = pa.nijk[3 * (int64_t)e + d];
-        double u0 = U[span1 - 1], u1 = U[span1];
-        double u = ((u1 - u0) * pa.g.x[g] + (u1 + u0)) * 0.5;
-        double N[NP1], dN[NP1];
-        bspline_ders<P>(span1 - 1, u, U, N, dN);
-#pragma unroll
-        for (int i = 0; i < NP1; ++i) {   // local index i <-> basis function P - i
-            sB[d][g][i] = N[P - i];
-            sdB[d][g][i] = dN[P - i];
-        }
-        if (g == 0) sHalf[d] = 0.5 * (u1 - u0);
-    }
-    __syncthreads();
-
-    const int nI_lane = lane & 15;
-    const int kq_lane = lane >> 4;
-
-    for (int pass = 0; pass < C::NPASS; ++pass) {
-        f64x4 acc[C::PPW][9];
-#pragma unroll
-        for (int t = 0; t < C::PPW; ++t)
-#pragma unroll
-            for (int c = 0; c < 9; ++c) acc[t][c] = f64x4{0.0, 0.0, 0.0, 0.0};
-
-        int tI[C::PPW], tJ[C::PPW];
-        bool tv[C::PPW];
-#pragma unroll
-        for (int t = 0; t < C::PPW; ++t) {
-            int pi = pass * C::WAVES * C::PPW + wave * C::PPW + t;
-            tv[t] = pi < C::NPAIR;
-            int J = 0;
-            while ((J + 1) * (J + 2) / 2 <= pi) ++J;
-            tJ[t] = J;
-            tI[t] = pi - J * (J + 1) / 2;
-        }
-
-        for (int ch = 0; ch < C::NCH; ++ch) {
-            const int q0 = ch * QC;
-            // ---- 2. per-Gauss-point reductions --------------------------------------
-            for (int grp = tid / C::TPG; grp < QC; grp += C::NTH / C::TPG) {
-                const int sub = tid % C::TPG;
-                const int q = q0 + grp;
-                double s[16];
-#pragma unroll
-                for (int m = 0; m < 16; ++m) s[m] = 0.0;
-                if (q < C::NQ) {
-                    const int g0 = q % NG, g1 = (q / NG) % NG, g2 = q / (NG * NG);
-                    for (int a = sub; a < NN; a += C::TPG) {
-                        const int i = a % NP1, j = (a / NP1) % NP1, k = a / (NP1 * NP1);
-                        double b0 = sB[0][g0][i], b1 = sB[1][g1][j], b2 = sB[2][g2][k];
-                        double w = sW[a];
-                        double Na = b0 * b1 * b2 * w;
-                        double d0 = sdB[0][g0][i] * b1 * b2 * w;
-                        double d1 = b0 * sdB[1][g1][j] * b2 * w;
-                        double d2 = b0 * b1 * sdB[2][g2][k] * w;
-                        double x0 = sX[a][0], x1 = sX[a][1], x2 = sX[a][2];
-                        s[0] += Na;
-                        s[1] += d0; s[2] += d1; s[3] += d2;
-                        s[4] += x0 * Na; s[5] += x1 * Na; s[6] += x2 * Na;
-                        s[7] += x0 * d0; s[8] += x0 * d1; s[9] += x0 * d2;
-                        s[10] += x1 * d0; s[11] += x1 * d1; s[12] += x1 * d2;
-                        s[13] += x2 * d0; s[14] += x2 * d1; s[15] += x2 * d2;
-                    }
-                }
-#pragma unroll
-                for (int m = 0; m < 16; ++m) {
-                    s[m] += __shfl_xor(s[m], 1, C::TPG);
-                    s[m] += __shfl_xor(s[m], 2, C::TPG);
-                }
-                if (sub == 0) {
-                    if (q < C::NQ) {
-                        const int g0 = q % NG, g1 = (q / NG) % NG, g2 = q / (NG * NG);
-                        double Wi = 1.0 / s[0];
-                        double dw[3] = {s[1] * Wi, s[2] * Wi, s[3] * Wi};
-                        double X[3] = {s[4], s[5], s[6]};
-                        // dx/dxi (row r = physical, col c = parametric)
-                        double J[3][3];
-#pragma unroll
-                        for (int r = 0; r < 3; ++r)
-#pragma unroll
-                            for (int c = 0; c < 3; ++c) J[r][c] = (s[7 + 3 * r + c] - X[r] * dw[c]) * Wi;
-                        double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1])
-                                   - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0])
-                                   + J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-                        double id = 1.0 / det;
-                        // J^{-1}[c][r] = d xi_c / d x_r
-                        sJi[grp][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
-                        sJi[grp][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-                        sJi[grp][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-                        sJi[grp][3] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
-                        sJi[grp][4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-                        sJi[grp][5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-                        sJi[grp][6] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
-                        sJi[grp][7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-                        sJi[grp][8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-                        sGp[grp][0] = Wi;
-                        sGp[grp][1] = dw[0];
-                        sGp[grp][2] = dw[1];
-                        sGp[grp][3] = dw[2];
-                        double detJ = det * sHalf[0] * sHalf[1] * sHalf[2];
-                        sGp[grp][4] = pa.g.w[g0] * pa.g.w[g1] * pa.g.w[g2] * fabs(detJ);
-                    } else {
-#pragma unroll
-                        for (int m = 0; m < 9; ++m) sJi[grp][m] = 0.0;
-#pragma unroll
-                        for (int m = 0; m < 5; ++m) sGp[grp][m] = 0.0;
-                    }
-                }
-            }
-            __syncthreads();
-            // ---- 3. physical basis derivatives G_i[q][a] -----------------------------
-            for (int idx = tid; idx < QC * NNP; idx += C::NTH) {
-                const int qq = idx / NNP, a = idx % NNP;
-                const int q = q0 + qq;
-                double g0v = 0.0, g1v = 0.0, g2v = 0.0;
-                if (a < NN && q < C::NQ) {
-                    const int g0 = q % NG, g1 = (q / NG) % NG, g2 = q / (NG * NG);
-                    const int i = a % NP1, j = (a / NP1) % NP1, k = a / (NP1 * NP1);
-                    double b0 = sB[0][g0][i], b1 = sB[1][g1][j], b2 = sB[2][g2][k];
-                    double w = sW[a];
-                    double Wi = sGp[qq][0];
-                    double R = b0 * b1 * b2 * w * Wi;
-                    double r0 = sdB[0][g0][i] * b1 * b2 * w * Wi - R * sGp[qq][1];
-                    double r1 = b0 * sdB[1][g1][j] * b2 * w * Wi - R * sGp[qq][2];
-                    double r2 = b0 * b1 * sdB[2][g2][k] * w * Wi - R * sGp[qq][3];
-                    const double *Ji = sJi[qq];
-                    g0v = r0 * Ji[0] + r1 * Ji[3] + r2 * Ji[6];
-                    g1v = r0 * Ji[1] + r1 * Ji[4] + r2 * Ji[7];
-                    g2v = r0 * Ji[2] + r1 * Ji[5] + r2 * Ji[8];
-                }
-                sG[0][qq][a] = g0v;
-                sG[1][qq][a] = g1v;
-                sG[2][qq][a] = g2v;
-            }
-            __syncthreads();
-            // ---- 4. S^{ij} += G_i^T diag(omega) G_j on MFMA ---------------------------
-            for (int ks = 0; ks < QC / 4; ++ks) {
-                const int qr = ks * 4 + kq_lane;
-                const double om = sGp[qr][4];
-#pragma unroll
-                for (int t = 0; t < C::PPW; ++t) {
-                    if (!tv[t]) continue;
-                    const int ca = 16 * tI[t] + nI_lane;
-                    const int cb = 16 * tJ[t] + nI_lane;
-                    double a0 = sG[0][qr][ca] * om, a1 = sG[1][qr][ca] * om, a2 = sG[2][qr][ca] * om;
-                    double b0 = sG[0][qr][cb], b1 = sG[1][qr][cb], b2 = sG[2][qr][cb];
-                    acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[t][0], 0, 0, 0);
-                    acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[t][1], 0, 0, 0);
-                    acc[t][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b2, acc[t][2], 0, 0, 0);
-                    acc[t][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[t][3], 0, 0, 0);
-                    acc[t][4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[t][4], 0, 0, 0);
-                    acc[t][5] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b2, acc[t][5], 0, 0, 0);
-                    acc[t][6] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b0, acc[t][6], 0, 0, 0);
-                    acc[t][7] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b1, acc[t][7], 0, 0, 0);
-                    acc[t][8] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b2, acc[t][8], 0, 0, 0);
-                }
-            }
-            if (C::NCH > 1) __syncthreads();
-        }
-
-        // ---- 5. isotropic contraction and COO emission -------------------------------
-        const double lam = pa.lam, mu = pa.mu;
-#pragma unroll
-        for (int t = 0; t < C::PPW; ++t) {
-            if (!tv[t]) continue;
-            const int b = 16 * tJ[t] + nI_lane;
-            if (b >= NN) continue;
-            const int64_t startb = base + 9LL * b * (b - 1) / 2 + 6LL * b;
-            const int32_t dofb = (sIen[b] - 1) * 3;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int a = 16 * tI[t] + kq_lane + 4 * r;
-                if (a > b) continue;
-                double c[9];
-#pragma unroll
-                for (int m = 0; m < 9; ++m) c[m] = acc[t][m][r];
-                const double tr = mu * (c[0] + c[4] + c[8]);
-                double K[3][3];
-#pragma unroll
-                for (int kk = 0; kk < 3; ++kk)
-#pragma unroll
-                    for (int ll = 0; ll < 3; ++ll)
-                        K[kk][ll] = lam * c[3 * kk + ll] + mu * c[3 * ll + kk] + (kk == ll ? tr : 0.0);
-                const int32_t dofa = (sIen[a] - 1) * 3;
-                if (a < b) {
-                    const int64_t pos = startb + 9LL * a;
-#pragma unroll
-                    for (int ll = 0; ll < 3; ++ll)
-#pragma unroll
-                        for (int kk = 0; kk < 3; ++kk) {
-                            Kdata[pos + kk + 3 * ll] = K[kk][ll];
-                            if (WRITE_IDX) {
-                                Krow[pos + kk + 3 * ll] = dofa + kk;
-                                Kcol[pos + kk + 3 * ll] = dofb + ll;
-                            }
-                        }
-                } else {
-                    const int64_t pos = startb + 9LL * b;
-                    Kdata[pos + 0] = 0.5 * K[0][0];
-                    Kdata[pos + 1] = K[0][1];
-                    Kdata[pos + 2] = 0.5 * K[1][1];
-                    Kdata[pos + 3] = K[0][2];
-                    Kdata[pos + 4] = K[1][2];
-                    Kdata[pos + 5] = 0.5 * K[2][2];
-                    if (WRITE_IDX) {
-                        const int rr[6] = {0, 0, 1, 0, 1, 2};
-                        const int cc[6] = {0, 1, 1, 2, 2, 2};
-#pragma unroll
-                        for (int m = 0; m < 6; ++m) {
-                            Krow[pos + m] = dofb + rr[m];
-                            Kcol[pos + m] = dofb + cc[m];
-                        }
-                    }
-                }
-            }
-        }
-        if (C::NPASS > 1 && C::NCH > 1) __syncthreads();
-    }
-}
+#include "elem_kernel.hpp"
 
 // ------------------------------------------------------------------------------------------
 // Right-hand side of loaded elements (UELMAT.f90:121-223): body loads 101/102 at the volume
@@ -796,6 +503,19 @@ static bool kernel_available(int p0, int p1, int p2, int ng) {
     return (p0 == 1 && ng == 2) || (p0 == 2 && ng == 3) || (p0 == 3 && ng == 4) || (p0 == 4 && ng == 5);
 }
 
+static int device_cu_count() {
+    static thread_local int cached_dev = -1, cached_n = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (dev != cached_dev) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cached_dev = dev;
+        cached_n = n;
+    }
+    return cached_n;
+}
+
 template <int P, int NG>
 static int launch_elem(const PatchDev &pd, const double *coords, double *Kd, int32_t *Kr,
                        int32_t *Kc, hipStream_t st) {
@@ -814,11 +534,13 @@ static int launch_elem(const PatchDev &pd, const double *coords, double *Kd, int
     pa.lam = pd.lam;
     pa.mu = pd.mu;
     gauss_legendre(NG, pa.g.x, pa.g.w);
-    dim3 grid(pd.n_active), block(ElemCfg<P, NG>::NTH);
+    // persistent workgroups: one per CU (the kernel's LDS and registers allow one)
+    const int grid = std::min(pd.n_active, device_cu_count());
+    dim3 block(ElemCfg<P, NG>::NTH);
     if (Kr)
-        hipLaunchKernelGGL((elem_stiffness_kernel<P, NG, true>), grid, block, 0, st, pa, Kd, Kr, Kc);
+        hipLaunchKernelGGL((elem_stiffness_kernel<P, NG, true>), dim3(grid), block, 0, st, pa, Kd, Kr, Kc);
     else
-        hipLaunchKernelGGL((elem_stiffness_kernel<P, NG, false>), grid, block, 0, st, pa, Kd, Kr, Kc);
+        hipLaunchKernelGGL((elem_stiffness_kernel<P, NG, false>), dim3(grid), block, 0, st, pa, Kd, Kr, Kc);
     YETI_HIP(hipGetLastError());
     return 0;
 }
