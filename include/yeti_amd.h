@@ -141,6 +141,62 @@ int32_t yeti_asm_run_coo_values(yeti_asm *h, double *Kdata_dev, void *stream);
 int32_t yeti_asm_run_rhs(yeti_asm *h, double *F_dev, void *stream);
 void    yeti_asm_destroy(yeti_asm *h);
 
+/* ---- CSR of K_side + K_side^T (device) --------------------------------------------------
+ * Replaces the callers' scipy step `coo_matrix((Kdata,(Krow,Kcol))).tocsc(); K + K.T`
+ * (overlay/iga_model.py:391-395). The pattern is built once from the handle's connectivity;
+ * rows are dofs 3 I + k, each row lists the dof columns of every CP sharing an element with
+ * CP I, ascending (the structural pattern; exact zeros are kept). rowptr is int64. */
+typedef struct yeti_csr yeti_csr;
+
+int32_t yeti_csr_create(const yeti_asm *h, void *stream, yeti_csr **out);
+int64_t yeti_csr_nnz(const yeti_csr *c);
+int32_t yeti_csr_nrows(const yeti_csr *c);
+const int64_t *yeti_csr_rowptr(const yeti_csr *c);   /* device, [nrows + 1] */
+const int32_t *yeti_csr_colind(const yeti_csr *c);   /* device, [nnz]       */
+/* Copy the pattern to host buffers (rowptr[nrows + 1], colind[nnz]), e.g. to build a scipy matrix. */
+int32_t yeti_csr_pattern_to_host(const yeti_csr *c, int64_t *rowptr_host, int32_t *colind_host);
+/* vals[nnz] = CSR values of K_side + K_side^T from the COO values the handle wrote. */
+int32_t yeti_csr_values(const yeti_csr *c, const double *Kdata_dev, double *vals_dev, void *stream);
+int32_t yeti_csr_spmv(const yeti_csr *c, const double *vals_dev, const double *x_dev, double *y_dev,
+                      void *stream);
+void    yeti_csr_destroy(yeti_csr *c);
+
+/* ---- Jacobi-preconditioned CG, FP64, on a device CSR --------------------------------------
+ * Solves A[free][:, free] x[free] = b[free] (free_mask[i] = 1 for free dofs; masked entries of x
+ * are 0 on return), stopping at ||r||_2 <= rtol ||b||_2; the residual is read back every
+ * check_every iterations. Replaces the callers' spsolve on the sliced matrix
+ * (benchs/volume_force/test_volume_force_3d.py:67-79) and the CG loop of iga_mf_cg_3d
+ * (iga_wq_mf/fsrc/mf_iga_solver.f90:642-958). Returns YETI_ERR_NOT_CONVERGED at maxit. */
+int32_t yeti_pcg_csr(int64_t nrows, const int64_t *rowptr, const int32_t *colind, const double *vals,
+                     const double *b, const uint8_t *free_mask, double *x, double rtol, int32_t maxit,
+                     int32_t check_every, int32_t *iters, double *relres, void *stream);
+
+/* ---- matrix-free operator (sum-factorised, FP64 and FP32) --------------------------------
+ * y = (K_side + K_side^T) x without forming K, for patches whose elements fill the tensor grid
+ * of their knot spans. Generalises YETI's scalar matrix-free apply mf_iga_get_ku_3d
+ * (iga_wq_mf/fsrc/mf_iga_solver.f90:292-555) to 3-component NURBS elasticity. precisions: bit 0
+ * prepares FP64, bit 1 FP32 (FP64 geometry is always kept for the diagonal blocks). x, y are
+ * device arrays of 3 nb_cp values (dof 3 I + k); the apply is deterministic (no atomics). */
+typedef struct yeti_mf yeti_mf;
+
+int32_t yeti_mf_create(const yeti_system_inputs *in, int32_t precisions, yeti_mf **out);
+int32_t yeti_mf_nb_cp(const yeti_mf *m);
+int32_t yeti_mf_apply_f64(const yeti_mf *m, const double *x, double *y, void *stream);
+int32_t yeti_mf_apply_f32(const yeti_mf *m, const float *x, float *y, void *stream);
+/* 3x3 diagonal block of every CP, blocks[9 I + 3 k + l] (FP64). */
+int32_t yeti_mf_diag_blocks_f64(const yeti_mf *m, double *blocks, void *stream);
+void    yeti_mf_destroy(yeti_mf *m);
+
+/* FP64 PCG with the FP64 matrix-free apply and a 3x3 nodal block-Jacobi preconditioner on the
+ * free dofs (the CG loop of iga_mf_cg_3d, mf_iga_solver.f90:642-958). x = 0 on fixed dofs. */
+int32_t yeti_mf_pcg_f64(const yeti_mf *m, const double *b, const uint8_t *free_mask, double *x, double rtol,
+                        int32_t maxit, int32_t check_every, int32_t *iters, double *relres, void *stream);
+/* FP64 iterative refinement with FP32 inner PCG (FP32 apply + FP32 block-Jacobi, FP64 dots):
+ * stops when ||b - K x||_2 <= rtol ||b||_2 over the free dofs. */
+int32_t yeti_mf_solve_ir(const yeti_mf *m, const double *b, const uint8_t *free_mask, double *x, double rtol,
+                         double inner_rtol, int32_t max_outer, int32_t max_inner, int32_t check_every,
+                         int32_t *outer_iters, int32_t *inner_iters, double *relres, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

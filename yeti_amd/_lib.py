@@ -31,6 +31,29 @@ SIGNATURES = {
     "yeti_asm_run_coo_values": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_asm_run_rhs": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_asm_destroy": (None, [C.c_void_p]),
+    "yeti_csr_create": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "yeti_csr_nnz": (C.c_int64, [C.c_void_p]),
+    "yeti_csr_nrows": (C.c_int32, [C.c_void_p]),
+    "yeti_csr_rowptr": (C.c_void_p, [C.c_void_p]),
+    "yeti_csr_colind": (C.c_void_p, [C.c_void_p]),
+    "yeti_csr_pattern_to_host": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_csr_values": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_csr_spmv": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_csr_destroy": (None, [C.c_void_p]),
+    "yeti_mf_create": (C.c_int32, [C.POINTER(YetiSystemInputs), C.c_int32, C.POINTER(C.c_void_p)]),
+    "yeti_mf_nb_cp": (C.c_int32, [C.c_void_p]),
+    "yeti_mf_apply_f64": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_mf_apply_f32": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_mf_diag_blocks_f64": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_mf_destroy": (None, [C.c_void_p]),
+    "yeti_mf_pcg_f64": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int32,
+                                    C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p]),
+    "yeti_mf_solve_ir": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                     C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p]),
+    "yeti_pcg_csr": (C.c_int32, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_double), C.c_void_p]),
 }
 
 _lib = None

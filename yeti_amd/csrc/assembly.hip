@@ -83,12 +83,8 @@ static void gauss_legendre(int n, double *x, double *w) {
     }
 }
 
-constexpr int MAX_NG = 9;
+void gauss_rule_host(int n, double *x, double *w) { gauss_legendre(n, x, w); }
 
-struct GaussRule {
-    double x[MAX_NG];
-    double w[MAX_NG];
-};
 
 // ------------------------------------------------------------------------------------------
 // Device: non-zero B-splines of degree P and first derivatives at u for 0-based span i
@@ -451,6 +447,10 @@ struct PatchDev {
 }  // namespace yeti
 
 struct yeti_asm {
+    // host copies of the per-patch device pointers (for the CSR builder)
+    std::vector<const int32_t *> v_ien, v_elems;
+    std::vector<const int64_t *> v_offs;
+    std::vector<int32_t> v_nnode, v_nactive;
     int layout;
     int64_t nb_data;        // entries the layout uses
     int64_t nb_data_cap;    // nb_data requested by the caller
@@ -749,6 +749,13 @@ static int create(const yeti_system_inputs *in, int layout, yeti_asm **out) {
     }
     h->nb_data = count;
     h->n_loaded_total = slot;
+    for (auto *pd : h->patches) {
+        h->v_ien.push_back(pd->ien.p);
+        h->v_elems.push_back(pd->elems.p);
+        h->v_offs.push_back(pd->offs.p);
+        h->v_nnode.push_back(pd->nnode);
+        h->v_nactive.push_back(pd->u1 ? pd->n_active : 0);
+    }
     // concentrated loads: assignment in load order (build_system_elemStorage.f90:137-152)
     std::vector<double> Fc((size_t)in->nb_dof_tot, 0.0);
     for (int il = 0; il < in->nb_load; ++il) {
@@ -790,6 +797,28 @@ static int create(const yeti_system_inputs *in, int layout, yeti_asm **out) {
     if (e2 == hipSuccess && slot > 0) e2 = h->rhs_buf.alloc((size_t)slot * 3 * RHS_MAXN);
     if (e2 != hipSuccess) return fail(set_error(YETI_ERR_DEVICE, "upload loads: %s", hipGetErrorString(e2)));
     *out = h;
+    return 0;
+}
+
+struct AsmView {
+    int32_t nb_cp;
+    int32_t npatch;
+    const int32_t *const *ien;
+    const int32_t *const *elems;
+    const int64_t *const *offs;
+    const int32_t *nnode;
+    const int32_t *n_active;
+};
+
+int asm_view(const yeti_asm *h, AsmView *v) {
+    if (!h) return set_error(YETI_ERR_INVALID_ARGUMENT, "null assembly handle");
+    v->nb_cp = h->nb_cp;
+    v->npatch = (int32_t)h->patches.size();
+    v->ien = h->v_ien.data();
+    v->elems = h->v_elems.data();
+    v->offs = h->v_offs.data();
+    v->nnode = h->v_nnode.data();
+    v->n_active = h->v_nactive.data();
     return 0;
 }
 

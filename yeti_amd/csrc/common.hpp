@@ -27,6 +27,13 @@ int set_error(int code, const char *fmt, ...);
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+// Gauss-Legendre rule passed to kernels by value (orders 1..9, as YETI's table).
+constexpr int MAX_NG = 9;
+struct GaussRule {
+    double x[MAX_NG];
+    double w[MAX_NG];
+};
+
 // RAII device buffer (hipMalloc / hipFree) used by host-side set-up code.
 template <typename T>
 struct DevBuf {

@@ -1,0 +1,714 @@
+// mf.hip -- sum-factorised matrix-free apply y = K x of the NURBS linear-elasticity stiffness, in
+// FP32 or FP64, plus its 3x3 nodal block diagonal.
+//
+// Reference pattern: YETI's matrix-free operator mf_iga_get_ku_3d
+// (iga_wq_mf/fsrc/mf_iga_solver.f90:292-555) applies y = sum_ij B_i^T diag(c_ij W) B_j x with
+// tensor-product (Kronecker) 1D basis matrices (tensor_algebra.f90:369-486) for a scalar
+// B-spline operator. This generalises it to 3-component elasticity on rational (NURBS) patches:
+//   grad_xi u_l = (1/W) [ sum_b w_b x_b[l] dB_b - (dW/W) sum_b w_b x_b[l] B_b ]   (quotient rule)
+//   H = grad_xi u J^{-1},  sigma = lam tr(H) I + mu (H + H^T)
+//   y_a[k] = w_a sum_q [ sum_m T_km dB_a/dxi_m - (sum_m T_km dW_m/W) B_a ],
+//   T_km = w|det J| sum_l sigma_kl J^{-1}_ml / W
+// which equals K x for K_ab[k][l] = sum_q w|J| (lam dR_a/dx_k dR_b/dx_l + mu dR_a/dx_l dR_b/dx_k
+// + mu d_kl grad R_a . grad R_b), the matrix the assembly kernel writes.
+//
+// Per quadrature point the stored geometry is 13 numbers: A = J^{-1}/W (9), dW/W (3), w|det J|
+// (1), computed once in FP64 and kept in the apply's precision. The tensor contractions run one
+// thread per quadrature point (then per node) with LDS staging. Elements of a patch are grouped
+// in bricks of 2(p+1) elements per direction; a workgroup walks a brick colour by colour
+// ((p+1)^3 colours, elements of one colour share no node), adding element results into an LDS
+// brick accumulator in a fixed order; a gather kernel then sums the <= 8 bricks covering each
+// control point in a fixed order. No floating-point atomics anywhere: the apply is
+// bit-reproducible.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+
+namespace yeti {
+
+// -- host copy of the B-spline routine (same algorithm as the device one in assembly.hip) --
+static void host_bspline(int i, int p, double u, const double *U, double *N, double *dN) {
+    double left[16], right[16], cur[16], low[16];
+    cur[0] = 1.0;
+    low[0] = 1.0;
+    for (int j = 1; j <= p; ++j) {
+        left[j] = u - U[i + 1 - j];
+        right[j] = U[i + j] - u;
+        double saved = 0.0;
+        for (int r = 0; r < j; ++r) {
+            double tmp = cur[r] / (right[r + 1] + left[j - r]);
+            cur[r] = saved + right[r + 1] * tmp;
+            saved = left[j - r] * tmp;
+        }
+        cur[j] = saved;
+        if (j == p - 1)
+            for (int r = 0; r < p; ++r) low[r] = cur[r];
+    }
+    for (int r = 0; r <= p; ++r) {
+        N[r] = cur[r];
+        int k = i - p + r;
+        double d = 0.0;
+        if (r >= 1) {
+            double den = U[k + p] - U[k];
+            if (den != 0.0) d += p * low[r - 1] / den;
+        }
+        if (r <= p - 1) {
+            double den = U[k + p + 1] - U[k + 1];
+            if (den != 0.0) d -= p * low[r] / den;
+        }
+        dN[r] = d;
+    }
+}
+
+void gauss_rule_host(int n, double *x, double *w);   // assembly.hip
+
+template <int P, int NG>
+struct MfCfg {
+    static constexpr int NP1 = P + 1;
+    static constexpr int NQ = NG * NG * NG;
+    static constexpr int NN = NP1 * NP1 * NP1;
+    static constexpr int TPE = NQ > NN ? NQ : NN;       // threads per element
+    static constexpr int EPB = 256 / TPE;               // elements per batch
+    static constexpr int BR = 2 * NP1;                  // brick edge (elements)
+    static constexpr int BN = BR + P;                   // brick edge (nodes)
+    static constexpr int MAXD = NG > NP1 ? NG : NP1;
+    static constexpr int SLOT = 12 * MAXD * MAXD * MAXD; // LDS values per element slot per buffer
+    static_assert(EPB >= 1, "element too large for one workgroup");
+};
+
+struct MfPatch {
+    int32_t N[3];            // elements per direction
+    int32_t ncp[3];          // control points per direction
+    const int32_t *cpmap;    // patch grid CP -> global CP (0-based)
+    const double *Btab;      // [d][e_d][NG][NP1] values (basis function span - p + r)
+    const double *dBtab;     // same, derivatives
+    int32_t boff[3];         // offset of direction d in the tables (in elements)
+    double lam, mu;
+};
+
+template <typename T, int P, int NG, int NC, int BN>
+struct MfSmem {
+    using C = MfCfg<P, NG>;
+    T buf0[C::EPB][C::SLOT];
+    T buf1[C::EPB][C::SLOT];
+    T acc[BN * BN * BN * NC];
+    T B[C::EPB][3][NG][C::NP1];
+    T dB[C::EPB][3][NG][C::NP1];
+    T w[C::EPB][C::NN];
+};
+
+// geometry of one patch: geo[e][c][q], e = e1 + N1 (e2 + N2 e3), q = g0 + NG (g1 + NG g2)
+template <int P, int NG>
+__global__ void mf_geometry_kernel(MfPatch mp, const double *__restrict__ coords, const double *__restrict__ wcp,
+                                   const double *__restrict__ half, GaussRule g, double *__restrict__ geo64,
+                                   float *__restrict__ geo32) {
+    using C = MfCfg<P, NG>;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nel = (int64_t)mp.N[0] * mp.N[1] * mp.N[2];
+    if (t >= nel * C::NQ) return;
+    const int64_t e = t / C::NQ;
+    const int q = (int)(t % C::NQ);
+    const int e1 = (int)(e % mp.N[0]), e2 = (int)((e / mp.N[0]) % mp.N[1]), e3 = (int)(e / ((int64_t)mp.N[0] * mp.N[1]));
+    const int g0 = q % NG, g1 = (q / NG) % NG, g2 = q / (NG * NG);
+    const int ed[3] = {e1, e2, e3};
+    const int gg[3] = {g0, g1, g2};
+    double Bv[3][C::NP1], dBv[3][C::NP1];
+    for (int d = 0; d < 3; ++d)
+        for (int r = 0; r < C::NP1; ++r) {
+            int64_t o = ((int64_t)(mp.boff[d] + ed[d]) * NG + gg[d]) * C::NP1 + r;
+            Bv[d][r] = mp.Btab[o];
+            dBv[d][r] = mp.dBtab[o];
+        }
+    double s[16] = {0};
+    for (int k = 0; k < C::NP1; ++k)
+        for (int j = 0; j < C::NP1; ++j)
+            for (int i = 0; i < C::NP1; ++i) {
+                int64_t cpl = (e1 + i) + (int64_t)mp.ncp[0] * ((e2 + j) + (int64_t)mp.ncp[1] * (e3 + k));
+                int cp = mp.cpmap[cpl];
+                double w = wcp[cp];
+                double N = Bv[0][i] * Bv[1][j] * Bv[2][k] * w;
+                double d0 = dBv[0][i] * Bv[1][j] * Bv[2][k] * w;
+                double d1 = Bv[0][i] * dBv[1][j] * Bv[2][k] * w;
+                double d2 = Bv[0][i] * Bv[1][j] * dBv[2][k] * w;
+                const double *x = coords + 3 * (int64_t)cp;
+                s[0] += N; s[1] += d0; s[2] += d1; s[3] += d2;
+                for (int r = 0; r < 3; ++r) {
+                    s[4 + r] += x[r] * N;
+                    s[7 + 3 * r] += x[r] * d0;
+                    s[8 + 3 * r] += x[r] * d1;
+                    s[9 + 3 * r] += x[r] * d2;
+                }
+            }
+    const double W = s[0], Wi = 1.0 / W;
+    const double dw[3] = {s[1] * Wi, s[2] * Wi, s[3] * Wi};
+    double J[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) J[r][c] = (s[7 + 3 * r + c] - s[4 + r] * dw[c]) * Wi;
+    const double m00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double m01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double m02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double det = J[0][0] * m00 + J[0][1] * m01 + J[0][2] * m02;
+    const double id = 1.0 / det;
+    double Ji[9];   // Ji[3 m + k] = d xi_m / d x_k
+    Ji[0] = m00 * id;
+    Ji[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    Ji[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    Ji[3] = m01 * id;
+    Ji[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    Ji[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    Ji[6] = m02 * id;
+    Ji[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    Ji[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    const double hdet = half[3 * e1 + 0] * half[3 * e2 + 1] * half[3 * e3 + 2];
+    // half[] is [e_d][3] per direction with the direction's element index
+    const double om = g.w[g0] * g.w[g1] * g.w[g2] * fabs(det * hdet);
+    double out[13];
+    for (int c = 0; c < 9; ++c) out[c] = Ji[c] * Wi;
+    out[9] = dw[0];
+    out[10] = dw[1];
+    out[11] = dw[2];
+    out[12] = om;
+    for (int c = 0; c < 13; ++c) {
+        int64_t o = (e * 13 + c) * C::NQ + q;
+        if (geo64) geo64[o] = out[c];
+        if (geo32) geo32[o] = (float)out[c];
+    }
+}
+
+// Brick edge in elements: 2(p+1) for the apply (8 elements per colour), p+1 for the diagonal
+// blocks (1 element per colour, keeps the 9-value accumulator within LDS).
+template <int P, int MODE>
+struct MfBrick {
+    static constexpr int BR = MODE == 0 ? 2 * (P + 1) : (P + 1);
+    static constexpr int BN = BR + P;
+    static constexpr int K = BR / (P + 1);            // elements per colour per direction
+    static constexpr int EPC = K * K * K;
+};
+
+// MODE 0: y = K x (3 values per node); MODE 1: 3x3 diagonal blocks (9 values per node).
+template <typename T, int P, int NG, int MODE>
+__global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__restrict__ geo, const T *__restrict__ x,
+                                                       const T *__restrict__ wcp, const int32_t *__restrict__ nbrick,
+                                                       T *__restrict__ partial) {
+    using C = MfCfg<P, NG>;
+    using Bk = MfBrick<P, MODE>;
+    constexpr int NP1 = C::NP1, NQ = C::NQ, NN = C::NN, BN = Bk::BN, BR = Bk::BR, NC = MODE == 0 ? 3 : 9;
+    __shared__ MfSmem<T, P, NG, NC, BN> sm;
+    const int tid = threadIdx.x;
+    const int slot = tid / C::TPE, r = tid % C::TPE;
+    const int64_t brick = blockIdx.x;
+    const int b1 = (int)(brick % nbrick[0]), b2 = (int)((brick / nbrick[0]) % nbrick[1]),
+              b3 = (int)(brick / ((int64_t)nbrick[0] * nbrick[1]));
+    for (int i = tid; i < BN * BN * BN * NC; i += 256) sm.acc[i] = T(0);
+    const T lam = (T)mp.lam, mu = (T)mp.mu;
+    // elements of colour c: e_d = b_d BR + c_d + NP1 t_d, t_d in [0, K): EPC per colour
+    for (int color = 0; color < NP1 * NP1 * NP1; ++color) {
+        const int c1 = color % NP1, c2 = (color / NP1) % NP1, c3 = color / (NP1 * NP1);
+        for (int bt = 0; bt < Bk::EPC; bt += C::EPB) {
+            __syncthreads();   // previous batch finished with the staging buffers
+            const int tt = bt + slot;
+            const bool sl = slot < C::EPB && tt < Bk::EPC;
+            const int t1 = tt % Bk::K, t2 = (tt / Bk::K) % Bk::K, t3 = tt / (Bk::K * Bk::K);
+            const int e1 = b1 * BR + c1 + NP1 * t1, e2 = b2 * BR + c2 + NP1 * t2, e3 = b3 * BR + c3 + NP1 * t3;
+            const bool live = sl && e1 < mp.N[0] && e2 < mp.N[1] && e3 < mp.N[2];
+            const int64_t e = e1 + (int64_t)mp.N[0] * (e2 + (int64_t)mp.N[1] * e3);
+            T *b0 = sm.buf0[slot < C::EPB ? slot : 0];
+            T *b1v = sm.buf1[slot < C::EPB ? slot : 0];
+            // stage nodal data and 1D tables
+            if (live && r < NN) {
+                const int i = r % NP1, j = (r / NP1) % NP1, k = r / (NP1 * NP1);
+                int64_t cpl = (e1 + i) + (int64_t)mp.ncp[0] * ((e2 + j) + (int64_t)mp.ncp[1] * (e3 + k));
+                int cp = mp.cpmap[cpl];
+                T w = wcp[cp];
+                sm.w[slot][r] = w;
+                if (MODE == 0)
+                    for (int l = 0; l < 3; ++l) b0[l * NN + r] = w * x[3 * (int64_t)cp + l];
+            }
+            if (live)
+                for (int idx = r; idx < 3 * NG * NP1; idx += C::TPE) {
+                    const int d = idx / (NG * NP1), rem = idx % (NG * NP1), gq = rem / NP1, rr = rem % NP1;
+                    const int ed = d == 0 ? e1 : d == 1 ? e2 : e3;
+                    int64_t o = ((int64_t)(mp.boff[d] + ed) * NG + gq) * NP1 + rr;
+                    sm.B[slot][d][gq][rr] = (T)mp.Btab[o];
+                    sm.dB[slot][d][gq][rr] = (T)mp.dBtab[o];
+                }
+            __syncthreads();
+            T yv[NC];
+            for (int c = 0; c < NC; ++c) yv[c] = T(0);
+            if (MODE == 0) {
+                const auto &B = sm.B[slot < C::EPB ? slot : 0];
+                const auto &dB = sm.dB[slot < C::EPB ? slot : 0];
+                // stage 1: contract i -> g0. b1v[(f*2+v)*NG*NP1*NP1 + (k*NP1+j)*NG + g0]
+                if (live && r < NG * NP1 * NP1) {
+                    const int g0 = r % NG, jk = r / NG;
+                    for (int f = 0; f < 3; ++f) {
+                        T v = 0, d = 0;
+                        for (int i = 0; i < NP1; ++i) {
+                            T fv = b0[f * NN + jk * NP1 + i];
+                            v += fv * B[0][g0][i];
+                            d += fv * dB[0][g0][i];
+                        }
+                        b1v[(f * 2 + 0) * NG * NP1 * NP1 + r] = v;
+                        b1v[(f * 2 + 1) * NG * NP1 * NP1 + r] = d;
+                    }
+                }
+                __syncthreads();
+                // stage 2: contract j -> g1. b0[(f*3+v)*NG*NG*NP1 + (k*NG+g1)*NG + g0], v: val, d0, d1
+                if (live && r < NG * NG * NP1) {
+                    const int g0 = r % NG, g1 = (r / NG) % NG, k = r / (NG * NG);
+                    for (int f = 0; f < 3; ++f) {
+                        T v = 0, d0 = 0, d1 = 0;
+                        for (int j = 0; j < NP1; ++j) {
+                            T sv = b1v[(f * 2 + 0) * NG * NP1 * NP1 + (k * NP1 + j) * NG + g0];
+                            T sd = b1v[(f * 2 + 1) * NG * NP1 * NP1 + (k * NP1 + j) * NG + g0];
+                            v += sv * B[1][g1][j];
+                            d1 += sv * dB[1][g1][j];
+                            d0 += sd * B[1][g1][j];
+                        }
+                        b0[(f * 3 + 0) * NG * NG * NP1 + r] = v;
+                        b0[(f * 3 + 1) * NG * NG * NP1 + r] = d0;
+                        b0[(f * 3 + 2) * NG * NG * NP1 + r] = d1;
+                    }
+                }
+                __syncthreads();
+                // stage 3 (k -> g2) and the quadrature-point work
+                if (live && r < NQ) {
+                    const int g0 = r % NG, g1 = (r / NG) % NG, g2 = r / (NG * NG);
+                    T V[3], G[3][3];    // G[m][l] = sum w x_l dB/dxi_m
+                    for (int f = 0; f < 3; ++f) {
+                        T v = 0, d0 = 0, d1 = 0, d2 = 0;
+                        for (int k = 0; k < NP1; ++k) {
+                            const int o = (k * NG + g1) * NG + g0;
+                            T sv = b0[(f * 3 + 0) * NG * NG * NP1 + o];
+                            v += sv * B[2][g2][k];
+                            d2 += sv * dB[2][g2][k];
+                            d0 += b0[(f * 3 + 1) * NG * NG * NP1 + o] * B[2][g2][k];
+                            d1 += b0[(f * 3 + 2) * NG * NG * NP1 + o] * B[2][g2][k];
+                        }
+                        V[f] = v;
+                        G[0][f] = d0;
+                        G[1][f] = d1;
+                        G[2][f] = d2;
+                    }
+                    const T *gq = geo + (e * 13) * NQ + r;
+                    T A[9], gw[3];
+                    for (int c = 0; c < 9; ++c) A[c] = gq[c * NQ];
+                    gw[0] = gq[9 * NQ];
+                    gw[1] = gq[10 * NQ];
+                    gw[2] = gq[11 * NQ];
+                    const T om = gq[12 * NQ];
+                    // H[l][k] = d u_l / d x_k = sum_m (G[m][l] - gw[m] V[l]) A[m][k]
+                    T H[3][3];
+                    for (int l = 0; l < 3; ++l)
+                        for (int k = 0; k < 3; ++k) {
+                            T s = 0;
+                            for (int m = 0; m < 3; ++m) s += (G[m][l] - gw[m] * V[l]) * A[3 * m + k];
+                            H[l][k] = s;
+                        }
+                    const T tr = lam * (H[0][0] + H[1][1] + H[2][2]);
+                    T S[3][3];
+                    for (int k = 0; k < 3; ++k)
+                        for (int l = 0; l < 3; ++l) S[k][l] = mu * (H[k][l] + H[l][k]) + (k == l ? tr : T(0));
+                    // coefficients of dB/dxi_m (Tm) and of B (Tb) for each component k
+                    for (int k = 0; k < 3; ++k) {
+                        T tb = 0;
+                        for (int m = 0; m < 3; ++m) {
+                            T s = 0;
+                            for (int l = 0; l < 3; ++l) s += S[k][l] * A[3 * m + l];
+                            s *= om;
+                            b1v[(k * 4 + 1 + m) * NQ + r] = s;
+                            tb -= s * gw[m];
+                        }
+                        b1v[(k * 4 + 0) * NQ + r] = tb;
+                    }
+                }
+                __syncthreads();
+                // backward stage 3^T (g2 -> k): P, R0, R1 per component into b0
+                if (live && r < NG * NG * NP1) {
+                    const int g0 = r % NG, g1 = (r / NG) % NG, k = r / (NG * NG);
+                    for (int c = 0; c < 3; ++c) {
+                        T Pv = 0, R0 = 0, R1 = 0;
+                        for (int g2 = 0; g2 < NG; ++g2) {
+                            const int q = g0 + NG * (g1 + NG * g2);
+                            Pv += b1v[(c * 4 + 0) * NQ + q] * B[2][g2][k] + b1v[(c * 4 + 3) * NQ + q] * dB[2][g2][k];
+                            R0 += b1v[(c * 4 + 1) * NQ + q] * B[2][g2][k];
+                            R1 += b1v[(c * 4 + 2) * NQ + q] * B[2][g2][k];
+                        }
+                        b0[(c * 3 + 0) * NG * NG * NP1 + r] = Pv;
+                        b0[(c * 3 + 1) * NG * NG * NP1 + r] = R0;
+                        b0[(c * 3 + 2) * NG * NG * NP1 + r] = R1;
+                    }
+                }
+                __syncthreads();
+                // backward stage 2^T (g1 -> j): P', R0' into b1v
+                if (live && r < NG * NP1 * NP1) {
+                    const int g0 = r % NG, j = (r / NG) % NP1, k = r / (NG * NP1);
+                    for (int c = 0; c < 3; ++c) {
+                        T Pv = 0, R0 = 0;
+                        for (int g1 = 0; g1 < NG; ++g1) {
+                            const int o = (k * NG + g1) * NG + g0;
+                            Pv += b0[(c * 3 + 0) * NG * NG * NP1 + o] * B[1][g1][j] +
+                                  b0[(c * 3 + 2) * NG * NG * NP1 + o] * dB[1][g1][j];
+                            R0 += b0[(c * 3 + 1) * NG * NG * NP1 + o] * B[1][g1][j];
+                        }
+                        b1v[(c * 2 + 0) * NG * NP1 * NP1 + r] = Pv;
+                        b1v[(c * 2 + 1) * NG * NP1 * NP1 + r] = R0;
+                    }
+                }
+                __syncthreads();
+                // backward stage 1^T (g0 -> i): node values
+                if (live && r < NN) {
+                    const int i = r % NP1, jk = r / NP1;
+                    for (int c = 0; c < 3; ++c) {
+                        T s = 0;
+                        for (int g0 = 0; g0 < NG; ++g0) {
+                            const int o = jk * NG + g0;
+                            s += b1v[(c * 2 + 0) * NG * NP1 * NP1 + o] * B[0][g0][i] +
+                                 b1v[(c * 2 + 1) * NG * NP1 * NP1 + o] * dB[0][g0][i];
+                        }
+                        yv[c] = s * sm.w[slot][r];
+                    }
+                }
+            } else {
+                // diagonal blocks: thread = node a, loop over quadrature points
+                const auto &B = sm.B[slot < C::EPB ? slot : 0];
+                const auto &dB = sm.dB[slot < C::EPB ? slot : 0];
+                if (live && r < NN) {
+                    const int i = r % NP1, j = (r / NP1) % NP1, k = r / (NP1 * NP1);
+                    const T w = sm.w[slot][r];
+                    for (int q = 0; q < NQ; ++q) {
+                        const int g0 = q % NG, g1 = (q / NG) % NG, g2 = q / (NG * NG);
+                        const T *gq = geo + (e * 13) * NQ + q;
+                        const T Bq = B[0][g0][i] * B[1][g1][j] * B[2][g2][k];
+                        const T dq[3] = {dB[0][g0][i] * B[1][g1][j] * B[2][g2][k], B[0][g0][i] * dB[1][g1][j] * B[2][g2][k],
+                                         B[0][g0][i] * B[1][g1][j] * dB[2][g2][k]};
+                        T gx[3];
+                        for (int kk = 0; kk < 3; ++kk) {
+                            T s = 0;
+                            for (int m = 0; m < 3; ++m) s += (dq[m] - Bq * gq[(9 + m) * NQ]) * gq[(3 * m + kk) * NQ];
+                            gx[kk] = w * s;
+                        }
+                        const T om = gq[12 * NQ];
+                        const T nn2 = gx[0] * gx[0] + gx[1] * gx[1] + gx[2] * gx[2];
+                        for (int kk = 0; kk < 3; ++kk)
+                            for (int ll = 0; ll < 3; ++ll)
+                                yv[3 * kk + ll] += om * ((lam + mu) * gx[kk] * gx[ll] + (kk == ll ? mu * nn2 : T(0)));
+                    }
+                }
+            }
+            // accumulate into the brick: elements of one colour share no node
+            if (live && r < NN) {
+                const int i = r % NP1, j = (r / NP1) % NP1, k = r / (NP1 * NP1);
+                const int n1 = e1 - b1 * BR + i, n2 = e2 - b2 * BR + j, n3 = e3 - b3 * BR + k;
+                T *dst = sm.acc + ((n3 * BN + n2) * BN + n1) * NC;
+                for (int c = 0; c < NC; ++c) dst[c] += yv[c];
+            }
+        }
+    }
+    __syncthreads();
+    T *out = partial + brick * (int64_t)(BN * BN * BN * NC);
+    for (int i = tid; i < BN * BN * BN * NC; i += 256) out[i] = sm.acc[i];
+}
+
+// y[global cp] (+)= sum over the bricks covering patch CP c (fixed order)
+template <typename T, int NC>
+__global__ void mf_gather_kernel(MfPatch mp, int32_t br, int32_t bn, int32_t nb0, int32_t nb1, int32_t nb2,
+                                 const T *__restrict__ partial, T *__restrict__ y, int accumulate) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ncp = (int64_t)mp.ncp[0] * mp.ncp[1] * mp.ncp[2];
+    if (t >= ncp) return;
+    const int c1 = (int)(t % mp.ncp[0]), c2 = (int)((t / mp.ncp[0]) % mp.ncp[1]),
+              c3 = (int)(t / ((int64_t)mp.ncp[0] * mp.ncp[1]));
+    const int cc[3] = {c1, c2, c3};
+    const int nb[3] = {nb0, nb1, nb2};
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        // brick b covers CPs [b br, b br + bn)
+        hi[d] = std::min(nb[d] - 1, cc[d] / br);
+        int l = (cc[d] - bn) / br + 1;
+        if (cc[d] - bn < 0) l = 0;
+        lo[d] = std::max(0, l);
+    }
+    T s[NC];
+    for (int c = 0; c < NC; ++c) s[c] = T(0);
+    for (int b3 = lo[2]; b3 <= hi[2]; ++b3)
+        for (int b2 = lo[1]; b2 <= hi[1]; ++b2)
+            for (int b1 = lo[0]; b1 <= hi[0]; ++b1) {
+                const int64_t brick = b1 + (int64_t)nb0 * (b2 + (int64_t)nb1 * b3);
+                const int n1 = c1 - b1 * br, n2 = c2 - b2 * br, n3 = c3 - b3 * br;
+                const T *src = partial + brick * (int64_t)(bn * bn * bn * NC) + ((n3 * bn + n2) * bn + n1) * NC;
+                for (int c = 0; c < NC; ++c) s[c] += src[c];
+            }
+    const int cp = mp.cpmap[t];
+    for (int c = 0; c < NC; ++c) {
+        if (accumulate) y[(int64_t)NC * cp + c] += s[c];
+        else y[(int64_t)NC * cp + c] = s[c];
+    }
+}
+
+template <typename T>
+__global__ void fill_kernel(T *p, int64_t n, T v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace yeti
+
+// ------------------------------------------------------------------------------------------
+namespace yeti {
+struct MfPatchDev {
+    int p, ng;
+    MfPatch mp;
+    int32_t nbrick[2][3];                 // per mode (apply, diagonal)
+    DevBuf<int32_t> cpmap, nbrick_dev;    // [2][3]
+    DevBuf<double> Btab, dBtab, half, geo64;
+    DevBuf<float> geo32;
+    DevBuf<double> part64;
+    DevBuf<float> part32;
+    GaussRule g;
+};
+}  // namespace yeti
+
+struct yeti_mf {
+    int32_t nb_cp;
+    int precisions;     // bit 0: FP64, bit 1: FP32
+    std::vector<yeti::MfPatchDev *> patches;
+    yeti::DevBuf<double> coords, w64;
+    yeti::DevBuf<float> w32;
+    ~yeti_mf() {
+        for (auto *p : patches) delete p;
+    }
+};
+
+namespace yeti {
+
+template <typename T, int P, int NG, int MODE>
+static int launch_brick(const MfPatchDev &pd, const T *geo, const T *x, const T *w, T *part, hipStream_t st) {
+    const int32_t *nbk = pd.nbrick[MODE];
+    int64_t nb = (int64_t)nbk[0] * nbk[1] * nbk[2];
+    hipLaunchKernelGGL((mf_brick_kernel<T, P, NG, MODE>), dim3((unsigned)nb), dim3(256), 0, st, pd.mp, geo, x, w,
+                       pd.nbrick_dev.p + 3 * MODE, part);
+    YETI_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T, int MODE>
+static int dispatch_brick(const MfPatchDev &pd, const T *geo, const T *x, const T *w, T *part, hipStream_t st) {
+    switch (pd.p * 16 + pd.ng) {
+    case 1 * 16 + 2: return launch_brick<T, 1, 2, MODE>(pd, geo, x, w, part, st);
+    case 2 * 16 + 3: return launch_brick<T, 2, 3, MODE>(pd, geo, x, w, part, st);
+    case 3 * 16 + 4: return launch_brick<T, 3, 4, MODE>(pd, geo, x, w, part, st);
+    case 4 * 16 + 5: return launch_brick<T, 4, 5, MODE>(pd, geo, x, w, part, st);
+    default: return set_error(YETI_ERR_UNSUPPORTED_DEGREE, "matrix-free: no kernel for p=%d ng=%d", pd.p, pd.ng);
+    }
+}
+
+template <typename T, int MODE>
+static int mf_run(const yeti_mf *m, const T *x, T *y, hipStream_t st) {
+    constexpr int NC = MODE == 0 ? 3 : 9;
+    for (size_t ip = 0; ip < m->patches.size(); ++ip) {
+        const MfPatchDev &pd = *m->patches[ip];
+        const T *geo;
+        const T *w;
+        T *part;
+        if constexpr (sizeof(T) == 8) {
+            geo = (const T *)pd.geo64.p;
+            w = (const T *)m->w64.p;
+            part = (T *)pd.part64.p;
+        } else {
+            geo = (const T *)pd.geo32.p;
+            w = (const T *)m->w32.p;
+            part = (T *)pd.part32.p;
+        }
+        if (!geo || !part) return set_error(YETI_ERR_INVALID_ARGUMENT, "matrix-free: precision not prepared");
+        int rc = dispatch_brick<T, MODE>(pd, geo, x, w, part, st);
+        if (rc) return rc;
+        const int br = MODE == 0 ? 2 * (pd.p + 1) : pd.p + 1, bn = br + pd.p;
+        const int32_t *nbk = pd.nbrick[MODE];
+        int64_t ncp = (int64_t)pd.mp.ncp[0] * pd.mp.ncp[1] * pd.mp.ncp[2];
+        hipLaunchKernelGGL((mf_gather_kernel<T, NC>), dim3((unsigned)((ncp + 255) / 256)), dim3(256), 0, st, pd.mp, br,
+                           bn, nbk[0], nbk[1], nbk[2], part, y, ip > 0 ? 1 : 0);
+        YETI_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+}  // namespace yeti
+
+using namespace yeti;
+
+extern "C" {
+
+int32_t yeti_mf_create(const yeti_system_inputs *in, int32_t precisions, yeti_mf **out) {
+    if (!in || !out) return set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (in->MCRD != 3) return set_error(YETI_ERR_INVALID_ARGUMENT, "MCRD must be 3");
+    if (!(precisions & 3)) return set_error(YETI_ERR_INVALID_ARGUMENT, "no precision requested");
+    auto *m = new yeti_mf();
+    auto fail = [&](int code) {
+        delete m;
+        return code;
+    };
+    m->nb_cp = in->nb_cp;
+    m->precisions = precisions;
+    // per-CP weights (the inputs repeat them per element) and coordinates
+    std::vector<double> wcp((size_t)in->nb_cp, 1.0);
+    int64_t kv = 0, woff = 0, ieoff = 0, eloff = 0;
+    for (int ip = 0; ip < in->nb_patch; ++ip) {
+        int nn = in->NNODE[ip];
+        for (int64_t e = 0; e < in->nb_elem_patch[ip]; ++e)
+            for (int a = 0; a < nn; ++a) wcp[in->IEN[ieoff + nn * e + a] - 1] = in->weight[woff + nn * e + a];
+        woff += (int64_t)nn * in->nb_elem_patch[ip];
+        ieoff += (int64_t)nn * in->nb_elem_patch[ip];
+    }
+    if (m->coords.upload(in->COORDS3D, 3 * (size_t)in->nb_cp) != hipSuccess ||
+        m->w64.upload(wcp.data(), wcp.size()) != hipSuccess)
+        return fail(set_error(YETI_ERR_DEVICE, "matrix-free: upload failed"));
+    if (precisions & 2) {
+        std::vector<float> w32(wcp.begin(), wcp.end());
+        if (m->w32.upload(w32.data(), w32.size()) != hipSuccess)
+            return fail(set_error(YETI_ERR_DEVICE, "matrix-free: upload failed"));
+    }
+    woff = ieoff = 0;
+    for (int ip = 0; ip < in->nb_patch; ++ip) {
+        auto *pd = new MfPatchDev();
+        m->patches.push_back(pd);
+        const int p0 = in->Jpqr[3 * ip], p1 = in->Jpqr[3 * ip + 1], p2 = in->Jpqr[3 * ip + 2];
+        const int nn = in->NNODE[ip];
+        const int ne = in->nb_elem_patch[ip];
+        int nkv[3] = {in->Nkv[3 * ip], in->Nkv[3 * ip + 1], in->Nkv[3 * ip + 2]};
+        const double *U[3] = {in->Ukv + kv, in->Ukv + kv + nkv[0], in->Ukv + kv + nkv[0] + nkv[1]};
+        kv += nkv[0] + nkv[1] + nkv[2];
+        if (p0 != p1 || p1 != p2) return fail(set_error(YETI_ERR_UNSUPPORTED_DEGREE, "matrix-free: degrees differ per direction"));
+        int nbint = in->NBINT[ip];
+        int ng = (int)std::pow((double)nbint, 1.0 / 3.0);
+        if (ng * ng * ng < nbint) ++ng;
+        pd->p = p0;
+        pd->ng = ng;
+        if (!((p0 == 1 && ng == 2) || (p0 == 2 && ng == 3) || (p0 == 3 && ng == 4) || (p0 == 4 && ng == 5)))
+            return fail(set_error(YETI_ERR_UNSUPPORTED_DEGREE, "matrix-free: p=%d with %d Gauss points", p0, ng));
+        // element grid from the spans; require every span between distinct knots to hold one element
+        int N[3], ncp[3];
+        for (int d = 0; d < 3; ++d) {
+            ncp[d] = nkv[d] - p0 - 1;
+            N[d] = ncp[d] - p0;
+            for (int s = p0; s < p0 + N[d]; ++s)
+                if (!(U[d][s + 1] > U[d][s]))
+                    return fail(set_error(YETI_ERR_INVALID_ARGUMENT, "matrix-free: repeated interior knots are not supported"));
+        }
+        if ((int64_t)N[0] * N[1] * N[2] != ne)
+            return fail(set_error(YETI_ERR_INVALID_ARGUMENT, "matrix-free: patch %d is not a full element grid", ip + 1));
+        std::vector<int32_t> cpmap((size_t)ncp[0] * ncp[1] * ncp[2], -1);
+        for (int e = 0; e < ne; ++e) {
+            int ed[3];
+            for (int d = 0; d < 3; ++d) ed[d] = in->Nijk[3 * (int64_t)(eloff + e) + d] - 1 - p0;
+            for (int a = 0; a < nn; ++a) {
+                int i = a % (p0 + 1), j = (a / (p0 + 1)) % (p0 + 1), k = a / ((p0 + 1) * (p0 + 1));
+                // local node a <-> basis index p - i in each direction (shap.f90:94-119)
+                int64_t c = (ed[0] + p0 - i) + (int64_t)ncp[0] * ((ed[1] + p0 - j) + (int64_t)ncp[1] * (ed[2] + p0 - k));
+                cpmap[c] = in->IEN[ieoff + (int64_t)nn * e + a] - 1;
+            }
+        }
+        for (int32_t v : cpmap)
+            if (v < 0) return fail(set_error(YETI_ERR_INVALID_ARGUMENT, "matrix-free: connectivity is not a tensor grid"));
+        // 1D tables per direction and element, and half spans
+        double gx[MAX_NG], gwt[MAX_NG];
+        gauss_rule_host(ng, gx, gwt);
+        for (int i = 0; i < ng; ++i) {
+            pd->g.x[i] = gx[i];
+            pd->g.w[i] = gwt[i];
+        }
+        const int np1 = p0 + 1;
+        std::vector<double> Bt, dBt;
+        int nmax = std::max(N[0], std::max(N[1], N[2]));
+        std::vector<double> half((size_t)nmax * 3, 0.0);
+        for (int d = 0; d < 3; ++d) {
+            pd->mp.boff[d] = (int32_t)(Bt.size() / (ng * np1));
+            for (int e = 0; e < N[d]; ++e) {
+                int s = p0 + e;
+                double u0 = U[d][s], u1 = U[d][s + 1];
+                half[3 * e + d] = 0.5 * (u1 - u0);
+                for (int g = 0; g < ng; ++g) {
+                    double u = ((u1 - u0) * gx[g] + (u1 + u0)) * 0.5;
+                    double Nv[16], dNv[16];
+                    host_bspline(s, p0, u, U[d], Nv, dNv);
+                    for (int r = 0; r < np1; ++r) {
+                        Bt.push_back(Nv[r]);
+                        dBt.push_back(dNv[r]);
+                    }
+                }
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            pd->mp.N[d] = N[d];
+            pd->mp.ncp[d] = ncp[d];
+            pd->nbrick[0][d] = (N[d] + 2 * np1 - 1) / (2 * np1);
+            pd->nbrick[1][d] = (N[d] + np1 - 1) / np1;
+        }
+        double E = in->MATERIAL_PROPERTIES[(int64_t)in->n_mat_props_max * ip];
+        double nu = in->MATERIAL_PROPERTIES[(int64_t)in->n_mat_props_max * ip + 1];
+        pd->mp.lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+        pd->mp.mu = E / (2.0 * (1.0 + nu));
+        hipError_t er = pd->cpmap.upload(cpmap.data(), cpmap.size());
+        if (er == hipSuccess) er = pd->Btab.upload(Bt.data(), Bt.size());
+        if (er == hipSuccess) er = pd->dBtab.upload(dBt.data(), dBt.size());
+        if (er == hipSuccess) er = pd->half.upload(half.data(), half.size());
+        if (er == hipSuccess) er = pd->nbrick_dev.upload(&pd->nbrick[0][0], 6);
+        if (er != hipSuccess) return fail(set_error(YETI_ERR_DEVICE, "matrix-free: upload failed"));
+        pd->mp.cpmap = pd->cpmap.p;
+        pd->mp.Btab = pd->Btab.p;
+        pd->mp.dBtab = pd->dBtab.p;
+        const int64_t nel = (int64_t)ne;
+        const int nq = ng * ng * ng;
+        const int bn0 = 2 * np1 + p0, bn1 = np1 + p0;
+        const int64_t nb0 = (int64_t)pd->nbrick[0][0] * pd->nbrick[0][1] * pd->nbrick[0][2];
+        const int64_t nb1 = (int64_t)pd->nbrick[1][0] * pd->nbrick[1][1] * pd->nbrick[1][2];
+        const int64_t part_n = std::max(nb0 * bn0 * bn0 * bn0 * 3, nb1 * bn1 * bn1 * bn1 * 9);
+        // geometry in FP64 always (it seeds the FP32 copy), brick partials per precision
+        er = pd->geo64.alloc(nel * 13 * nq);
+        if (er == hipSuccess && (precisions & 2)) er = pd->geo32.alloc(nel * 13 * nq);
+        // the FP64 partials also serve the (FP64) diagonal-block pass
+        if (er == hipSuccess) er = pd->part64.alloc(part_n);
+        if (er == hipSuccess && (precisions & 2)) er = pd->part32.alloc(nb0 * bn0 * bn0 * bn0 * 3);
+        if (er != hipSuccess) return fail(set_error(YETI_ERR_DEVICE, "matrix-free: allocation failed (out of memory?)"));
+        const int64_t nt = nel * nq;
+        const unsigned blocks = (unsigned)((nt + 255) / 256);
+        switch (p0) {
+        case 1: hipLaunchKernelGGL((mf_geometry_kernel<1, 2>), dim3(blocks), dim3(256), 0, 0, pd->mp, m->coords.p, m->w64.p, pd->half.p, pd->g, pd->geo64.p, pd->geo32.p); break;
+        case 2: hipLaunchKernelGGL((mf_geometry_kernel<2, 3>), dim3(blocks), dim3(256), 0, 0, pd->mp, m->coords.p, m->w64.p, pd->half.p, pd->g, pd->geo64.p, pd->geo32.p); break;
+        case 3: hipLaunchKernelGGL((mf_geometry_kernel<3, 4>), dim3(blocks), dim3(256), 0, 0, pd->mp, m->coords.p, m->w64.p, pd->half.p, pd->g, pd->geo64.p, pd->geo32.p); break;
+        default: hipLaunchKernelGGL((mf_geometry_kernel<4, 5>), dim3(blocks), dim3(256), 0, 0, pd->mp, m->coords.p, m->w64.p, pd->half.p, pd->g, pd->geo64.p, pd->geo32.p); break;
+        }
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+            return fail(set_error(YETI_ERR_DEVICE, "matrix-free: geometry kernel failed"));
+        // the FP64 geometry is kept: the diagonal blocks and the FP64 apply read it
+        woff += (int64_t)nn * ne;
+        ieoff += (int64_t)nn * ne;
+        eloff += ne;
+    }
+    *out = m;
+    return 0;
+}
+
+int32_t yeti_mf_apply_f64(const yeti_mf *m, const double *x, double *y, void *stream) {
+    if (!m || !x || !y) return set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    return mf_run<double, 0>(m, x, y, (hipStream_t)stream);
+}
+
+int32_t yeti_mf_apply_f32(const yeti_mf *m, const float *x, float *y, void *stream) {
+    if (!m || !x || !y) return set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    return mf_run<float, 0>(m, x, y, (hipStream_t)stream);
+}
+
+int32_t yeti_mf_diag_blocks_f64(const yeti_mf *m, double *blocks, void *stream) {
+    if (!m || !blocks) return set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    return mf_run<double, 1>(m, nullptr, blocks, (hipStream_t)stream);
+}
+
+int32_t yeti_mf_nb_cp(const yeti_mf *m) { return m ? m->nb_cp : -1; }
+
+void yeti_mf_destroy(yeti_mf *m) { delete m; }
+
+}  // extern "C"

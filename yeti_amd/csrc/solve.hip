@@ -1,0 +1,208 @@
+// solve.hip -- device-resident Jacobi-preconditioned conjugate gradients on the assembled CSR
+// (FP64), with fixed-order reductions so a solve is bit-reproducible.
+//
+// Reference behaviour replaced: the callers solve (K + K^T)[free][:, free] u = F[free] with
+// scipy (spsolve, benchs/volume_force/test_volume_force_3d.py:67-79), and YETI's matrix-free CG
+// iga_mf_cg_3d (iga_wq_mf/fsrc/mf_iga_solver.f90:642-958) iterates alpha = rsold / (p . Ap).
+// Here the free-dof restriction is a 0/1 mask on the full vector: masked entries of x, r, z and
+// p stay exactly zero, which is the reduced system. Stopping rule: ||r||_2 <= rtol ||b||_2.
+#include <cmath>
+
+#include "common.hpp"
+
+namespace yeti {
+
+constexpr int RED_BLOCKS = 1024;     // partial sums per reduction (fixed -> fixed summation order)
+constexpr int RED_THREADS = 256;
+
+__device__ inline double block_sum(double v, double *sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
+    __syncthreads();
+    return s;
+}
+
+// y = mask .* (A x) ; partial[b] = sum over the block's rows of x . y
+__global__ void __launch_bounds__(256) spmv_dot_kernel(int64_t nrows, const int64_t *__restrict__ rowptr,
+                                                       const int32_t *__restrict__ colind,
+                                                       const double *__restrict__ vals,
+                                                       const double *__restrict__ x,
+                                                       const uint8_t *__restrict__ mask,
+                                                       double *__restrict__ y, double *__restrict__ partial) {
+    __shared__ double sh[4];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double acc = 0.0;
+    for (int64_t row = (int64_t)blockIdx.x * 4 + w; row < nrows; row += (int64_t)gridDim.x * 4) {
+        double s = 0.0;
+        if (mask[row]) {
+            for (int64_t k = rowptr[row] + lane; k < rowptr[row + 1]; k += 64) s += vals[k] * x[colind[k]];
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        }
+        if (lane == 0) {
+            y[row] = s;
+            acc += s * x[row];
+        }
+    }
+    double t = block_sum(lane == 0 ? acc : 0.0, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ void finalize_kernel(const double *partial, int n, double *out) {
+    __shared__ double sh[4];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += partial[i];
+    double t = block_sum(v, sh);
+    if (threadIdx.x == 0) *out = t;
+}
+
+// scalars: [0] rz, [1] pq, [2] rr, [3] bb, [4] rz_new
+__global__ void __launch_bounds__(256) update_xr_kernel(int64_t n, const double *__restrict__ sc,
+                                                        double *__restrict__ x, double *__restrict__ r,
+                                                        double *__restrict__ z, const double *__restrict__ p,
+                                                        const double *__restrict__ q,
+                                                        const double *__restrict__ dinv,
+                                                        double *__restrict__ part_rz, double *__restrict__ part_rr) {
+    __shared__ double sh[4];
+    const double alpha = sc[0] / sc[1];
+    double a_rz = 0.0, a_rr = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        double zi = dinv[i] * ri;
+        z[i] = zi;
+        a_rz += ri * zi;
+        a_rr += ri * ri;
+    }
+    double t1 = block_sum(a_rz, sh);
+    double t2 = block_sum(a_rr, sh);
+    if (threadIdx.x == 0) {
+        part_rz[blockIdx.x] = t1;
+        part_rr[blockIdx.x] = t2;
+    }
+}
+
+__global__ void __launch_bounds__(256) update_p_kernel(int64_t n, double *__restrict__ sc, double *__restrict__ p,
+                                                       const double *__restrict__ z) {
+    const double beta = sc[4] / sc[0];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = z[i] + beta * p[i];
+}
+
+__global__ void roll_scalars_kernel(double *sc) { sc[0] = sc[4]; }
+
+// r = mask b, z = dinv r, p = z; partials of r.z and b.b
+__global__ void __launch_bounds__(256) init_kernel(int64_t n, const double *__restrict__ b, const uint8_t *mask,
+                                                   const double *__restrict__ dinv, double *__restrict__ x,
+                                                   double *__restrict__ r, double *__restrict__ z,
+                                                   double *__restrict__ p, double *__restrict__ part_rz,
+                                                   double *__restrict__ part_bb) {
+    __shared__ double sh[4];
+    double a_rz = 0.0, a_bb = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double bi = mask[i] ? b[i] : 0.0;
+        x[i] = 0.0;
+        r[i] = bi;
+        double zi = dinv[i] * bi;
+        z[i] = zi;
+        p[i] = zi;
+        a_rz += bi * zi;
+        a_bb += bi * bi;
+    }
+    double t1 = block_sum(a_rz, sh);
+    double t2 = block_sum(a_bb, sh);
+    if (threadIdx.x == 0) {
+        part_rz[blockIdx.x] = t1;
+        part_bb[blockIdx.x] = t2;
+    }
+}
+
+// dinv[i] = mask[i] / A[i][i]
+__global__ void jacobi_kernel(int64_t nrows, const int64_t *rowptr, const int32_t *colind, const double *vals,
+                              const uint8_t *mask, double *dinv) {
+    int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nrows) return;
+    int64_t lo = rowptr[row], hi = rowptr[row + 1] - 1;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (colind[mid] < row) lo = mid + 1; else hi = mid;
+    }
+    double d = (lo < rowptr[row + 1] && colind[lo] == row) ? vals[lo] : 0.0;
+    dinv[row] = (mask[row] && d != 0.0) ? 1.0 / d : 0.0;
+}
+
+}  // namespace yeti
+
+using namespace yeti;
+
+extern "C" {
+
+int32_t yeti_pcg_csr(int64_t nrows, const int64_t *rowptr, const int32_t *colind, const double *vals,
+                     const double *b, const uint8_t *free_mask, double *x, double rtol, int32_t maxit,
+                     int32_t check_every, int32_t *iters_out, double *relres_out, void *stream) {
+    if (!rowptr || !colind || !vals || !b || !free_mask || !x || nrows <= 0 || maxit < 0)
+        return set_error(YETI_ERR_INVALID_ARGUMENT, "pcg: bad argument");
+    if (check_every <= 0) check_every = 10;
+    hipStream_t st = (hipStream_t)stream;
+    DevBuf<double> r, z, p, q, dinv, part1, part2, sc;
+    YETI_HIP(r.alloc(nrows));
+    YETI_HIP(z.alloc(nrows));
+    YETI_HIP(p.alloc(nrows));
+    YETI_HIP(q.alloc(nrows));
+    YETI_HIP(dinv.alloc(nrows));
+    YETI_HIP(part1.alloc(RED_BLOCKS));
+    YETI_HIP(part2.alloc(RED_BLOCKS));
+    YETI_HIP(sc.alloc(8));
+    const unsigned nb = (unsigned)std::min<int64_t>(RED_BLOCKS, (nrows + 255) / 256);
+    hipLaunchKernelGGL(jacobi_kernel, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, st, nrows, rowptr,
+                       colind, vals, free_mask, dinv.p);
+    hipLaunchKernelGGL(init_kernel, dim3(nb), dim3(256), 0, st, nrows, b, free_mask, dinv.p, x, r.p, z.p, p.p,
+                       part1.p, part2.p);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, part1.p, (int)nb, sc.p + 0);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, part2.p, (int)nb, sc.p + 3);
+    YETI_HIP(hipGetLastError());
+    double hs[8] = {0};
+    YETI_HIP(hipMemcpyAsync(hs, sc.p, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+    YETI_HIP(hipStreamSynchronize(st));
+    const double bb = hs[3];
+    if (bb == 0.0) {
+        if (iters_out) *iters_out = 0;
+        if (relres_out) *relres_out = 0.0;
+        return 0;
+    }
+    const unsigned nspmv = (unsigned)std::min<int64_t>(RED_BLOCKS, (nrows + 3) / 4);
+    int it = 0;
+    double rel = 1.0;
+    while (it < maxit) {
+        hipLaunchKernelGGL(spmv_dot_kernel, dim3(nspmv), dim3(256), 0, st, nrows, rowptr, colind, vals, p.p,
+                           free_mask, q.p, part1.p);
+        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, part1.p, (int)nspmv, sc.p + 1);
+        hipLaunchKernelGGL(update_xr_kernel, dim3(nb), dim3(256), 0, st, nrows, sc.p, x, r.p, z.p, p.p, q.p,
+                           dinv.p, part1.p, part2.p);
+        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, part1.p, (int)nb, sc.p + 4);
+        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, part2.p, (int)nb, sc.p + 2);
+        hipLaunchKernelGGL(update_p_kernel, dim3(nb), dim3(256), 0, st, nrows, sc.p, p.p, z.p);
+        hipLaunchKernelGGL(roll_scalars_kernel, dim3(1), dim3(1), 0, st, sc.p);
+        YETI_HIP(hipGetLastError());
+        ++it;
+        if (it % check_every == 0 || it == maxit) {
+            double rr = 0.0;
+            YETI_HIP(hipMemcpyAsync(&rr, sc.p + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+            YETI_HIP(hipStreamSynchronize(st));
+            rel = std::sqrt(rr / bb);
+            if (!(rel > rtol)) break;     // also stops on NaN
+        }
+    }
+    if (iters_out) *iters_out = it;
+    if (relres_out) *relres_out = rel;
+    if (std::isnan(rel)) return set_error(YETI_ERR_NOT_CONVERGED, "pcg: residual is NaN");
+    if (rel > rtol) return set_error(YETI_ERR_NOT_CONVERGED, "pcg: %d iterations, relative residual %.3e > %.3e", it, rel, rtol);
+    return 0;
+}
+
+}  // extern "C"
