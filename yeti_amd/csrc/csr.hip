@@ -117,10 +117,12 @@ static int exclusive_scan(const T *in, int64_t n, int64_t *out, hipStream_t st) 
     YETI_HIP(boff.alloc(nb + 1));
     hipLaunchKernelGGL(scan_block_kernel<T>, dim3((unsigned)nb), dim3(1024), 0, st, in, n, out, bsum.p);
     YETI_HIP(hipGetLastError());
-    int rc = exclusive_scan<int64_t>(bsum.p, nb, boff.p, st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)nb), dim3(1024), 0, st, out, n, boff.p);
-    YETI_HIP(hipGetLastError());
+    if (nb > 1) {   // one block needs no carry; recursing on it would never end
+        int rc = exclusive_scan<int64_t>(bsum.p, nb, boff.p, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)nb), dim3(1024), 0, st, out, n, boff.p);
+        YETI_HIP(hipGetLastError());
+    }
     // total = last offset + last count
     int64_t last_off = 0;
     T last_in{};

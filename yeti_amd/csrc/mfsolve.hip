@@ -153,9 +153,10 @@ __global__ void __launch_bounds__(256) xr_kernel(int64_t n, const double *sc, T 
 template <typename T>
 __global__ void __launch_bounds__(256) p_kernel(int64_t n, const double *sc, T *__restrict__ p, const T *__restrict__ z,
                                                 int first) {
+    // first direction: p = z without reading p (fresh buffers may hold NaN bit patterns)
     const double beta = first ? 0.0 : sc[4] / sc[0];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = (T)((double)z[i] + beta * (double)p[i]);
+        p[i] = first ? z[i] : (T)((double)z[i] + beta * (double)p[i]);
 }
 
 __global__ void roll_kernel(double *sc) { sc[0] = sc[4]; }
