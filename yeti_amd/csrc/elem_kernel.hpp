@@ -7,8 +7,9 @@
 //     sum x (x) dN over the element's nodes -> J^{-1}, w|det J| (shap.f90:94-178), then
 //     G_c[a][q] = dR_a/dx_c into one half of a double-buffered LDS array;
 //   * 8 consumer waves accumulate S^{kl}_{ab} = sum_q w|J| dR_a/dx_k dR_b/dx_l over step s's
-//     chunk on v_mfma_f64_16x16x4f64 and, after an element's last chunk, contract and emit
-//     its COO entries;
+//     chunk on v_mfma_f64_16x16x4f64 and, after an element's last chunk, contract its COO
+//     values into an LDS stage, from which all 12 waves write them (and the row / column
+//     indices, computed from the position) with coalesced stores;
 // one __syncthreads() ends each step. Producers never need each other: each producer wave owns
 // a quarter of every chunk's Gauss points and reduces them inside the wave.
 //
@@ -30,7 +31,8 @@ struct ElemCfg {
     static constexpr int NNP = NT * 16;                 // padded node count
     static constexpr int NQ = NG * NG * NG;             // Gauss points per element
     static constexpr int NQ8 = (NQ + 7) / 8 * 8;        // padded: two MFMA K steps per LDS read
-    static constexpr int QC = NNP >= 128 ? 16 : (NQ8 < 32 ? NQ8 : 32);   // Gauss points per chunk
+    static constexpr int QCW = NNP >= 128 ? 8 : 16;     // Gauss points per chunk (LDS budget)
+    static constexpr int QC = NQ8 < QCW ? NQ8 : QCW;
     static constexpr int NCH = (NQ8 + QC - 1) / QC;
     static constexpr int QCP = QC + 2;                  // padded G row (doubles)
     static constexpr int NPAIR = NT * (NT + 1) / 2;     // node-tile pairs I <= J
@@ -40,61 +42,108 @@ struct ElemCfg {
     static constexpr int DCAP = 2;                      // D units per consumer wave per pass
     static constexpr int PCAP = 4;                      // P units per consumer wave per pass
     static constexpr int MAXPASS = 8;
+    static constexpr int MAXR = 8;                      // staging rounds per pass
     static constexpr int GPW = QC / PW;                 // Gauss points per producer wave per chunk
     static constexpr int LPG = 64 / GPW;                // lanes per Gauss point in the reductions
     static constexpr int NPL = (NN + LPG - 1) / LPG;    // nodes per lane in the reductions
+    static constexpr int64_t NDE = 9LL * NN * (NN - 1) / 2 + 6LL * NN;   // COO entries per element
+    // LDS staging of the COO values (coalesced stores): what is left of ~150 KiB after G
+    static constexpr int64_t GBYTES = 2LL * 3 * NNP * QCP * 8;
+    static constexpr int64_t STGMAX = (150 * 1024 - GBYTES - 6 * 1024) / 8;
+    static constexpr int STG = (int)(NDE < STGMAX ? NDE : STGMAX);
     static_assert(QC % PW == 0 && 64 % GPW == 0, "chunk must split over producer waves");
 };
 
-// Longest-first static schedule of D and P units over consumer waves and passes.
-template <int NT, int WAVES, int DCAP, int PCAP, int MAXPASS>
+__host__ __device__ constexpr int64_t coo_start(int b) { return (9LL * b * b + 3LL * b) / 2; }
+
+// Static schedule of D and P units over consumer waves and passes. A pass takes whole tile
+// columns J (all pairs I <= J), so the COO entries of b in [16 J0, 16 J1) are complete after it;
+// inside a pass the units go longest-first to the least-loaded wave. Each pass's b-range is
+// emitted in rounds whose entries fit the LDS staging buffer.
+template <int NT, int WAVES, int DCAP, int PCAP, int MAXPASS, int MAXR>
 struct UnitSchedule {
     int npass;
     int nd[MAXPASS][WAVES];
     int np[MAXPASS][WAVES];
     int d[MAXPASS][WAVES][DCAP];      // tile-pair index
     int p[MAXPASS][WAVES][PCAP];      // tile-pair index * 3 + (0: (0,1), 1: (0,2), 2: (1,2))
+    int nround[MAXPASS];
+    int rb[MAXPASS][MAXR + 1];        // round boundaries in b
+    bool ok;
 };
 
-template <int NT, int WAVES, int DCAP, int PCAP, int MAXPASS>
-constexpr UnitSchedule<NT, WAVES, DCAP, PCAP, MAXPASS> make_schedule() {
-    UnitSchedule<NT, WAVES, DCAP, PCAP, MAXPASS> s{};
-    constexpr int NPAIR = NT * (NT + 1) / 2;
-    int next_d = 0, next_p = 0;
-    int pass = 0;
-    while ((next_d < NPAIR || next_p < 3 * NPAIR) && pass < MAXPASS) {
-        int load[WAVES] = {};
-        for (int w = 0; w < WAVES; ++w) {
-            s.nd[pass][w] = 0;
-            s.np[pass][w] = 0;
+template <int NT, int WAVES, int DCAP, int PCAP, int MAXPASS, int MAXR>
+constexpr UnitSchedule<NT, WAVES, DCAP, PCAP, MAXPASS, MAXR> make_schedule(int nn, int stg) {
+    UnitSchedule<NT, WAVES, DCAP, PCAP, MAXPASS, MAXR> s{};
+    s.ok = true;
+    int J = 0, pass = 0;
+    while (J < NT && pass < MAXPASS) {
+        // largest run of columns starting at J whose units fit the per-pass capacity
+        int J1 = J, nd = 0, np = 0;
+        while (J1 < NT) {
+            int add = J1 + 1;   // pairs in column J1
+            if (nd + add > DCAP * WAVES || np + 3 * add > PCAP * WAVES) break;
+            nd += add;
+            np += 3 * add;
+            ++J1;
         }
-        while (next_d < NPAIR) {    // D units (3 tiles) first
+        if (J1 == J) {
+            s.ok = false;
+            return s;
+        }
+        int load[WAVES] = {};
+        for (int w = 0; w < WAVES; ++w) s.nd[pass][w] = s.np[pass][w] = 0;
+        for (int pi = J * (J + 1) / 2; pi < J1 * (J1 + 1) / 2; ++pi) {
             int best = -1;
             for (int w = 0; w < WAVES; ++w)
                 if (s.nd[pass][w] < DCAP && (best < 0 || load[w] < load[best])) best = w;
-            if (best < 0) break;
-            s.d[pass][best][s.nd[pass][best]++] = next_d++;
+            if (best < 0) {
+                s.ok = false;
+                return s;
+            }
+            s.d[pass][best][s.nd[pass][best]++] = pi;
             load[best] += 3;
         }
-        while (next_p < 3 * NPAIR) {
-            int best = -1;
-            for (int w = 0; w < WAVES; ++w)
-                if (s.np[pass][w] < PCAP && (best < 0 || load[w] < load[best])) best = w;
-            if (best < 0) break;
-            s.p[pass][best][s.np[pass][best]++] = next_p++;
-            load[best] += 2;
+        for (int pi = J * (J + 1) / 2; pi < J1 * (J1 + 1) / 2; ++pi)
+            for (int which = 0; which < 3; ++which) {
+                int best = -1;
+                for (int w = 0; w < WAVES; ++w)
+                    if (s.np[pass][w] < PCAP && (best < 0 || load[w] < load[best])) best = w;
+                if (best < 0) {
+                    s.ok = false;
+                    return s;
+                }
+                s.p[pass][best][s.np[pass][best]++] = 3 * pi + which;
+                load[best] += 2;
+            }
+        // rounds over b in [16 J, min(16 J1, nn))
+        int b0 = 16 * J, bend = 16 * J1 < nn ? 16 * J1 : nn, r = 0;
+        s.rb[pass][0] = b0;
+        while (b0 < bend) {
+            int b1 = b0 + 1;
+            while (b1 < bend && coo_start(b1 + 1) - coo_start(b0) <= stg) ++b1;
+            if (coo_start(b1) - coo_start(b0) > stg || r >= MAXR) {
+                s.ok = false;
+                return s;
+            }
+            s.rb[pass][++r] = b1;
+            b0 = b1;
         }
+        s.nround[pass] = r;
+        J = J1;
         ++pass;
     }
     s.npass = pass;
+    if (J < NT) s.ok = false;
     return s;
 }
 
 template <int P, int NG>
 struct Sched {
     using C = ElemCfg<P, NG>;
-    static constexpr auto S = make_schedule<C::NT, C::CW, C::DCAP, C::PCAP, C::MAXPASS>();
+    static constexpr auto S = make_schedule<C::NT, C::CW, C::DCAP, C::PCAP, C::MAXPASS, C::MAXR>(C::NN, C::STG);
     static constexpr int NPASS = S.npass;
+    static_assert(S.ok, "unit schedule does not fit the wave / staging capacities");
 };
 
 struct PatchArgs {
@@ -118,6 +167,8 @@ struct ElemSmem {
     double B[C::PW][3][NG][C::NP1];     // per producer wave: B(reversed local index) at the abscissae
     double dB[C::PW][3][NG][C::NP1];
     double half[C::PW][3];
+    double stage[C::STG];             // COO values of one round, element-local order
+    int32_t dof[C::NN];               // 3 (IEN - 1) of the element being emitted
 };
 
 __device__ constexpr int p_k(int which) { return which == 2 ? 1 : 0; }
@@ -361,64 +412,46 @@ __device__ __forceinline__ void mfma_chunk(const ElemSmem<P, NG> &sm, int buf,
     }
 }
 
-// Consumer epilogue: isotropic contraction and COO emission for wave W of pass PASS.
+// Consumer epilogue, first half: isotropic contraction of the accumulators whose column node b
+// lies in round R of pass PASS, written to the LDS stage in element-local COO order.
 // Accumulator lane layout (v_mfma_f64_16x16x4f64): register r of lane l holds row (l >> 4) + 4 r
 // and column l & 15 of the tile; rows are nodes a of tile I, columns nodes b of tile J. COO
 // position of block (a, b), a < b: start(b) + 9 a + kk + 3 ll with start(b) = 9 b (b-1)/2 + 6 b
 // (build_system_elemStorage.f90:260-293); diagonal block: start(b) + 9 b + {0..5} holding
 // (0,0) (0,1) (1,1) (0,2) (1,2) (2,2), diagonal halved.
-template <int P, int NG, bool WRITE_IDX, int PASS, int W>
-__device__ __forceinline__ void epilogue(const PatchArgs &pa, int e, int64_t base,
-                                         const f64x4 (&accD)[ElemCfg<P, NG>::DCAP][3],
-                                         const f64x4 (&accP)[ElemCfg<P, NG>::PCAP][2],
-                                         double *__restrict__ Kdata, int32_t *__restrict__ Krow,
-                                         int32_t *__restrict__ Kcol) {
-    using C = ElemCfg<P, NG>;
-    constexpr int NN = C::NN;
+template <int P, int NG, int PASS, int W, int R>
+__device__ __forceinline__ void stage_round(ElemSmem<P, NG> &sm, double lam, double mu,
+                                            const f64x4 (&accD)[ElemCfg<P, NG>::DCAP][3],
+                                            const f64x4 (&accP)[ElemCfg<P, NG>::PCAP][2]) {
     constexpr auto &S = Sched<P, NG>::S;
     constexpr int nd = S.nd[PASS][W], np = S.np[PASS][W];
+    constexpr int b0 = S.rb[PASS][R], b1 = S.rb[PASS][R + 1];
+    constexpr int s0 = (int)coo_start(b0);
     const int lane = threadIdx.x & 63;
     const int nl = lane & 15, kg = lane >> 4;
-    const double lam = pa.lam, mu = pa.mu;
-    const int32_t *ien = pa.ien + (int64_t)NN * e;
 #pragma unroll
     for (int u = 0; u < nd; ++u) {
         const int pi = S.d[PASS][W][u];
         const int I = pair_I(pi), J = pair_J(pi);
+        if (16 * J + 15 < b0 || 16 * J >= b1) continue;      // resolved at compile time
         const int b = 16 * J + nl;
-        if (b >= NN) continue;
-        const int64_t startb = base + 9LL * b * (b - 1) / 2 + 6LL * b;
-        const int32_t dofb = WRITE_IDX ? (ien[b] - 1) * 3 : 0;
+        if (b < b0 || b >= b1) continue;
+        double *st = sm.stage + ((int)coo_start(b) - s0);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int a = 16 * I + kg + 4 * r;
-            if (a > b) continue;
+            if (I == J && a > b) continue;
             const double c0 = accD[u][0][r], c1 = accD[u][1][r], c2 = accD[u][2][r];
             const double tr = mu * (c0 + c1 + c2);
             const double k00 = (lam + mu) * c0 + tr, k11 = (lam + mu) * c1 + tr, k22 = (lam + mu) * c2 + tr;
             if (a < b) {
-                const int64_t pos = startb + 9LL * a;
-                Kdata[pos + 0] = k00;
-                Kdata[pos + 4] = k11;
-                Kdata[pos + 8] = k22;
-                if (WRITE_IDX) {
-                    const int32_t dofa = (ien[a] - 1) * 3;
-#pragma unroll
-                    for (int m = 0; m < 3; ++m) {
-                        Krow[pos + 4 * m] = dofa + m;
-                        Kcol[pos + 4 * m] = dofb + m;
-                    }
-                }
+                st[9 * a + 0] = k00;
+                st[9 * a + 4] = k11;
+                st[9 * a + 8] = k22;
             } else {
-                const int64_t pos = startb + 9LL * b;
-                Kdata[pos + 0] = 0.5 * k00;
-                Kdata[pos + 2] = 0.5 * k11;
-                Kdata[pos + 5] = 0.5 * k22;
-                if (WRITE_IDX) {
-                    Krow[pos + 0] = dofb;     Kcol[pos + 0] = dofb;
-                    Krow[pos + 2] = dofb + 1; Kcol[pos + 2] = dofb + 1;
-                    Krow[pos + 5] = dofb + 2; Kcol[pos + 5] = dofb + 2;
-                }
+                st[9 * b + 0] = 0.5 * k00;
+                st[9 * b + 2] = 0.5 * k11;
+                st[9 * b + 5] = 0.5 * k22;
             }
         }
     }
@@ -427,40 +460,108 @@ __device__ __forceinline__ void epilogue(const PatchArgs &pa, int e, int64_t bas
         const int code = S.p[PASS][W][u];
         const int pi = code / 3, which = code % 3;
         const int I = pair_I(pi), J = pair_J(pi);
+        if (16 * J + 15 < b0 || 16 * J >= b1) continue;
         const int k = p_k(which), l = p_l(which);
         const int b = 16 * J + nl;
-        if (b >= NN) continue;
-        const int64_t startb = base + 9LL * b * (b - 1) / 2 + 6LL * b;
-        const int32_t dofb = WRITE_IDX ? (ien[b] - 1) * 3 : 0;
+        if (b < b0 || b >= b1) continue;
+        double *st = sm.stage + ((int)coo_start(b) - s0);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int a = 16 * I + kg + 4 * r;
-            if (a > b) continue;
+            if (I == J && a > b) continue;
             const double ckl = accP[u][0][r], clk = accP[u][1][r];
             const double kkl = lam * ckl + mu * clk;     // K_ab[k][l]
             const double klk = lam * clk + mu * ckl;     // K_ab[l][k]
             if (a < b) {
-                const int64_t pos = startb + 9LL * a;
-                Kdata[pos + k + 3 * l] = kkl;
-                Kdata[pos + l + 3 * k] = klk;
-                if (WRITE_IDX) {
-                    const int32_t dofa = (ien[a] - 1) * 3;
-                    Krow[pos + k + 3 * l] = dofa + k;
-                    Kcol[pos + k + 3 * l] = dofb + l;
-                    Krow[pos + l + 3 * k] = dofa + l;
-                    Kcol[pos + l + 3 * k] = dofb + k;
-                }
+                st[9 * a + k + 3 * l] = kkl;
+                st[9 * a + l + 3 * k] = klk;
             } else {
                 // diagonal block keeps the upper entry (k, l), k < l
-                const int64_t pos = startb + 9LL * b;
-                const int m = (k == 0 && l == 1) ? 1 : (k == 0 && l == 2) ? 3 : 4;
-                Kdata[pos + m] = kkl;
-                if (WRITE_IDX) {
-                    Krow[pos + m] = dofb + k;
-                    Kcol[pos + m] = dofb + l;
-                }
+                st[9 * b + ((k == 0 && l == 1) ? 1 : (k == 0 && l == 2) ? 3 : 4)] = kkl;
             }
         }
+    }
+}
+
+// Epilogue, second half (all 768 threads): round R of pass PASS leaves the LDS stage with
+// consecutive lanes on consecutive COO entries. The (row, col) of an entry follow from its
+// element-local position L alone: b = largest node with start(b) <= L, then the 9-entry block
+// index a and (kk, ll) inside it, or the 6-entry diagonal pattern.
+template <int P, int NG, bool WRITE_IDX, int PASS, int R>
+__device__ __forceinline__ void copy_round(const ElemSmem<P, NG> &sm, int64_t base, double *__restrict__ Kdata,
+                                           int32_t *__restrict__ Krow, int32_t *__restrict__ Kcol) {
+    using C = ElemCfg<P, NG>;
+    constexpr auto &S = Sched<P, NG>::S;
+    constexpr int b0 = S.rb[PASS][R], b1 = S.rb[PASS][R + 1];
+    constexpr int s0 = (int)coo_start(b0);
+    constexpr int n = (int)coo_start(b1) - s0;
+    const int64_t g0 = base + s0;
+    for (int j = threadIdx.x; j < n; j += C::NTH) {
+        Kdata[g0 + j] = sm.stage[j];
+        if (WRITE_IDX) {
+            const int L = s0 + j;
+            int b = (int)((__builtin_sqrtf((float)(72 * L + 9)) - 3.0f) * (1.0f / 18.0f));
+            b = b < b0 ? b0 : b >= b1 ? b1 - 1 : b;
+            if ((int)coo_start(b + 1) <= L) ++b;
+            if ((int)coo_start(b) > L) --b;
+            const int off = L - (int)coo_start(b);
+            int a, kk, ll;
+            if (off < 9 * b) {
+                a = off / 9;
+                const int m = off - 9 * a;
+                ll = m / 3;
+                kk = m - 3 * ll;
+            } else {
+                const int d = off - 9 * b;
+                a = b;
+                kk = (0x210100 >> (4 * d)) & 15;
+                ll = (0x222110 >> (4 * d)) & 15;
+            }
+            Krow[g0 + j] = sm.dof[a] + kk;
+            Kcol[g0 + j] = sm.dof[b] + ll;
+        }
+    }
+}
+
+// Rounds of pass PASS for consumer wave W: stage, barrier, copy, (barrier before the next round).
+template <int P, int NG, bool WRITE_IDX, int PASS, int W, int R = 0>
+__device__ __forceinline__ void consumer_epilogue(ElemSmem<P, NG> &sm, double lam, double mu, int64_t base,
+                                                  const f64x4 (&accD)[ElemCfg<P, NG>::DCAP][3],
+                                                  const f64x4 (&accP)[ElemCfg<P, NG>::PCAP][2],
+                                                  double *__restrict__ Kdata, int32_t *__restrict__ Krow,
+                                                  int32_t *__restrict__ Kcol) {
+    constexpr int NR = Sched<P, NG>::S.nround[PASS];
+    stage_round<P, NG, PASS, W, R>(sm, lam, mu, accD, accP);
+    __syncthreads();
+    copy_round<P, NG, WRITE_IDX, PASS, R>(sm, base, Kdata, Krow, Kcol);
+    if constexpr (R + 1 < NR) {
+        __syncthreads();
+        consumer_epilogue<P, NG, WRITE_IDX, PASS, W, R + 1>(sm, lam, mu, base, accD, accP, Kdata, Krow, Kcol);
+    }
+}
+
+// The producers' side of the same rounds (same barriers, they help with the copies).
+template <int P, int NG, bool WRITE_IDX, int PASS, int R = 0>
+__device__ __forceinline__ void producer_epilogue_pass(const ElemSmem<P, NG> &sm, int64_t base,
+                                                       double *__restrict__ Kdata, int32_t *__restrict__ Krow,
+                                                       int32_t *__restrict__ Kcol) {
+    constexpr int NR = Sched<P, NG>::S.nround[PASS];
+    __syncthreads();
+    copy_round<P, NG, WRITE_IDX, PASS, R>(sm, base, Kdata, Krow, Kcol);
+    if constexpr (R + 1 < NR) {
+        __syncthreads();
+        producer_epilogue_pass<P, NG, WRITE_IDX, PASS, R + 1>(sm, base, Kdata, Krow, Kcol);
+    }
+}
+
+template <int P, int NG, bool WRITE_IDX, int PASS = 0>
+__device__ __forceinline__ void producer_epilogue(const ElemSmem<P, NG> &sm, int pass, int64_t base,
+                                                  double *__restrict__ Kdata, int32_t *__restrict__ Krow,
+                                                  int32_t *__restrict__ Kcol) {
+    if (pass == PASS) {
+        producer_epilogue_pass<P, NG, WRITE_IDX, PASS>(sm, base, Kdata, Krow, Kcol);
+    } else if constexpr (PASS + 1 < Sched<P, NG>::NPASS) {
+        producer_epilogue<P, NG, WRITE_IDX, PASS + 1>(sm, pass, base, Kdata, Krow, Kcol);
     }
 }
 
@@ -483,26 +584,32 @@ struct ConsumerAcc {
 
 // One consumer step for wave W: the pass is dispatched to compile-time code.
 template <int P, int NG, bool WRITE_IDX, int W, int PASS = 0>
-__device__ __forceinline__ void consumer_step(const ElemSmem<P, NG> &sm, const PatchArgs &pa, int buf,
-                                              int pass, int ch, int e, int64_t base, ConsumerAcc<P, NG> &acc,
+__device__ __forceinline__ void consumer_step(ElemSmem<P, NG> &sm, const PatchArgs &pa, int buf, int pass, int ch,
+                                              int e, int64_t base, ConsumerAcc<P, NG> &acc,
                                               double *__restrict__ Kdata, int32_t *__restrict__ Krow,
                                               int32_t *__restrict__ Kcol) {
     using C = ElemCfg<P, NG>;
     if (pass == PASS) {
         if (ch == 0) acc.zero();
         mfma_chunk<P, NG, PASS, W>(sm, buf, acc.D, acc.Pp);
-        if (ch == C::NCH - 1) epilogue<P, NG, WRITE_IDX, PASS, W>(pa, e, base, acc.D, acc.Pp, Kdata, Krow, Kcol);
+        if (ch == C::NCH - 1) {
+            if (WRITE_IDX && W < 2) {
+                const int t = threadIdx.x;
+                if (t < C::NN) sm.dof[t] = 3 * (pa.ien[(int64_t)C::NN * e + t] - 1);
+            }
+            consumer_epilogue<P, NG, WRITE_IDX, PASS, W>(sm, pa.lam, pa.mu, base, acc.D, acc.Pp, Kdata, Krow, Kcol);
+        }
     } else if constexpr (PASS + 1 < Sched<P, NG>::NPASS) {
         consumer_step<P, NG, WRITE_IDX, W, PASS + 1>(sm, pa, buf, pass, ch, e, base, acc, Kdata, Krow, Kcol);
     }
 }
 
 // The whole step loop of consumer wave W (one compile-time specialisation per wave, so each wave
-// keeps only its own accumulators live). Executes exactly one barrier per step.
+// keeps only its own accumulators live). One barrier per step, plus the epilogue's rounds.
 template <int P, int NG, bool WRITE_IDX, int W>
-__device__ __forceinline__ void consumer_loop(const ElemSmem<P, NG> &sm, const PatchArgs &pa, int nsteps,
-                                           double *__restrict__ Kdata, int32_t *__restrict__ Krow,
-                                           int32_t *__restrict__ Kcol) {
+__device__ __forceinline__ void consumer_loop(ElemSmem<P, NG> &sm, const PatchArgs &pa, int nsteps,
+                                              double *__restrict__ Kdata, int32_t *__restrict__ Krow,
+                                              int32_t *__restrict__ Kcol) {
     using C = ElemCfg<P, NG>;
     constexpr int SPE = Sched<P, NG>::NPASS * C::NCH;
     ConsumerAcc<P, NG> acc;
@@ -554,6 +661,11 @@ elem_stiffness_kernel(PatchArgs pa, double *__restrict__ Kdata, int32_t *__restr
                     pe_elem = el;
                 }
                 producer_chunk<P, NG>(sm, pa, pw, s1 & 1, ch * C::QC, pe);
+            }
+            const int rem = s % SPE;
+            if (rem % C::NCH == C::NCH - 1) {
+                const int64_t gi = blockIdx.x + (int64_t)(s / SPE) * gridDim.x;
+                producer_epilogue<P, NG, WRITE_IDX>(sm, rem / C::NCH, pa.offs[gi], Kdata, Krow, Kcol);
             }
             __syncthreads();
         }
