@@ -139,6 +139,14 @@ int32_t yeti_asm_run_coo_matrix(yeti_asm *h, double *Kdata_dev, int32_t *Krow_de
 int32_t yeti_asm_run_coo_values(yeti_asm *h, double *Kdata_dev, void *stream);
 /* Right-hand side only (concentrated, surface and body loads). */
 int32_t yeti_asm_run_rhs(yeti_asm *h, double *F_dev, void *stream);
+/* Replace the control-point coordinates (device, 3 nb_cp values, COORDS3D order) used by later
+ * runs; the design-update step of the reference's shape optimisation
+ * (OPT_modelling.py:268-312 re-marshals COORDS3D before each assembly). */
+int32_t yeti_asm_update_coords(yeti_asm *h, const double *coords_dev, void *stream);
+/* g[3 I + j] = 2 u^T dF/dX_Ij: shape derivative of the face-load work for a displacement u
+ * (device, 3 nb_cp). Face loads 10..99 only (YETI's own gradient leaves this term as a stub,
+ * optim/gradUELMAT_solid.f90:463-477); body loads return YETI_ERR_UNSUPPORTED_ELEMENT. */
+int32_t yeti_asm_load_shape_grad(yeti_asm *h, const double *u_dev, double *g_dev, void *stream);
 void    yeti_asm_destroy(yeti_asm *h);
 
 /* ---- CSR of K_side + K_side^T (device) --------------------------------------------------
@@ -185,6 +193,12 @@ int32_t yeti_mf_apply_f64(const yeti_mf *m, const double *x, double *y, void *st
 int32_t yeti_mf_apply_f32(const yeti_mf *m, const float *x, float *y, void *stream);
 /* 3x3 diagonal block of every CP, blocks[9 I + 3 k + l] (FP64). */
 int32_t yeti_mf_diag_blocks_f64(const yeti_mf *m, double *blocks, void *stream);
+/* g[3 I + j] = -u^T (dK/dX_Ij) u, the shape derivative of the stiffness part of the compliance
+ * (FP64, deterministic). Twice YETI's gradLinElastWork_AN internal term for the adjoint 1/2 u
+ * (optim/gradtotalwork_fullAN.f90:17-300, OPT_modelling.py:447-481). */
+int32_t yeti_mf_strain_energy_grad_f64(const yeti_mf *m, const double *u_dev, double *g_dev, void *stream);
+/* Replace the control-point coordinates (device, 3 nb_cp) and recompute the stored geometry. */
+int32_t yeti_mf_update_coords(yeti_mf *m, const double *coords_dev, void *stream);
 void    yeti_mf_destroy(yeti_mf *m);
 
 /* FP64 PCG with the FP64 matrix-free apply and a 3x3 nodal block-Jacobi preconditioner on the

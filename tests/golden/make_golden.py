@@ -15,6 +15,11 @@ Each fixture is data only (inputs and expected outputs):
 * box_1x1x1_p3.npz: one p=3 element (full COO + F);
 * all_loads_3x2x2_p2.npz: every surface-load code plus U101/U102 body loads on a 3x2x2 p=2
   patch: COO hashes, |Kdata|_F, full F.
+* grad_4x2x2_p2.npz: a 4x2x2 p=2 cantilever (seed 11, traction U23 at x=10): the state u of
+  the direct solve, gradLinElastWork_AN for the adjoints (1/2 u, a seeded random field), the
+  compliance J = F . u, and central finite differences (steps 1e-6 and 1e-4) of J at 24
+  seeded (control point, direction) pairs, each J evaluated with the compiled serial assembly
+  and a direct solve.
 """
 from __future__ import annotations
 
@@ -116,7 +121,53 @@ def main():
     K, r, c, F = O.ref_sys_linmat_serial((), kwargs=al)
     np.savez_compressed(os.path.join(HERE, "all_loads_3x2x2_p2.npz"), krow_sha=sha(r),
                         kcol_sha=sha(c), kfro=np.linalg.norm(K), F=F)
+    grad_fixture()
     print("wrote fixtures to", HERE)
+
+
+def compliance_ref(bp, X):
+    """J = F . u with the compiled serial assembly at control points X (3, nb_cp)."""
+    a = dict(bp.args)
+    a["COORDS3D"] = np.asfortranarray(X)
+    args = [a[n] for n in __import__("yeti_amd._abi", fromlist=["F2PY_ARGS"]).F2PY_ARGS]
+    args += [a["nb_bc"], a["nb_load"], a["nb_patch"], a["nb_elem"], a["nb_cp"], a["nb_dof_tot"]]
+    K, r, c, F = O.ref_sys_linmat_serial(args)
+    free = bp.ind_dof_free0
+    uf = solve(K, r, c, F, free, bp.nb_dof_tot)
+    u = np.zeros(bp.nb_dof_tot)
+    u[free] = uf
+    return float(F @ u), u, F
+
+
+def grad_fixture():
+    bp = make_box_patch(nel=(4, 2, 2), p=2, seed=11)
+    X0 = np.array(bp.args["COORDS3D"], dtype=np.float64)
+    J, u, F = compliance_ref(bp, X0)
+    SOL = u.reshape(-1, 3).T.copy()
+    rng = np.random.default_rng(12)
+    ADJ = np.stack([0.5 * SOL, rng.uniform(-1e-3, 1e-3, size=SOL.shape)])
+    gi, ge = O.ref_grad_linelast_work(bp.f2py_args(), SOL, ADJ)
+    n = bp.nb_cp
+    cps = rng.choice(n, 24, replace=False)
+    dirs = rng.integers(0, 3, 24)
+    # make sure the loaded face x = 10 and the clamped face x = 0 are both represented
+    xs = X0[0]
+    cps[:4] = np.flatnonzero(np.isclose(xs, 10.0))[rng.choice(int(np.isclose(xs, 10.0).sum()), 4, replace=False)]
+    cps[4:6] = np.flatnonzero(np.isclose(xs, 0.0))[:2]
+    # step 1e-6 as the reference's _sensitivityFD default (OPT_modelling.py:411-437); at that step
+    # the ~1e-14 noise of J (direct solve) gives ~1e-8 absolute error, so 1e-4 is kept as well
+    fds = {}
+    for h in (1e-6, 1e-4):
+        fd = np.zeros(24)
+        for t in range(24):
+            Xp = X0.copy(); Xp[dirs[t], cps[t]] += h
+            Xm = X0.copy(); Xm[dirs[t], cps[t]] -= h
+            fd[t] = (compliance_ref(bp, Xp)[0] - compliance_ref(bp, Xm)[0]) / (2 * h)
+        fds[h] = fd
+    geo = sha(np.concatenate([bp.args["COORDS3D"].ravel(order="F"), bp.args["weight"]])) + sha(bp.args["IEN"])
+    np.savez_compressed(os.path.join(HERE, "grad_4x2x2_p2.npz"), SOL=SOL, ADJ=ADJ, gradWint=gi, gradWext=ge,
+                        J=J, F=F, fd_cp=cps, fd_dir=dirs, fd_val=fds[1e-6], fd_val_h1e4=fds[1e-4],
+                        geometry_sha=geo)
 
 
 if __name__ == "__main__":

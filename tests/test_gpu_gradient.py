@@ -1,0 +1,112 @@
+"""GPU parity of the compliance shape gradient (MatrixFree.strain_energy_grad,
+Assembler.load_shape_grad and the torch.autograd.Function around them).
+
+Oracles (tests/golden/grad_4x2x2_p2.npz and unit_cube_p1.npz, written from YETI's compiled
+Fortran by tests/golden/make_golden.py):
+* -a^T (dK/dX) u is bilinear; the kernel computes q(v) = -v^T (dK/dX) v, so
+  (q(u + a) - q(u - a)) / 4 must equal gradLinElastWork_AN's gradWint for (SOL = u, ADJ = a)
+  (optim/gradtotalwork_fullAN.f90:17-300), within 1e-10 of max |gradWint|;
+* the full dJ/dX, J = F . u, against central differences of the reference's own J (serial
+  assembly + direct solve): within 1e-9 + 1e-6 |g| at step 1e-4 and 1e-7 + 1e-4 |g| at the
+  1e-6 step (where the ~1e-14 noise of J dominates);
+* invariants: translation (sum over control points of dJ/dX_j = 0) and homogeneity
+  (sum X . dJ/dX = 3 J: K ~ s, traction F ~ s^2, u ~ s).
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from yeti_amd.geometry import make_box_patch
+
+pytestmark = pytest.mark.gpu
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def fixture():
+    d = np.load("tests/golden/grad_4x2x2_p2.npz")
+    bp = make_box_patch(nel=(4, 2, 2), p=2, seed=11)
+    geo = sha(np.concatenate([bp.args["COORDS3D"].ravel(order="F"), bp.args["weight"]])) + sha(bp.args["IEN"])
+    assert geo == str(d["geometry_sha"]), "generator changed: regenerate the fixture"
+    return d, bp
+
+
+def q_grad(mf, v):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(v.T.reshape(-1))).cuda()
+    return mf.strain_energy_grad(t).cpu().numpy().reshape(-1, 3).T
+
+
+@pytest.mark.parametrize("name", ["grad_4x2x2_p2", "unit_cube_p1"])
+def test_strain_energy_grad_matches_reference(gpu, name):
+    from yeti_amd.matfree import F64, MatrixFree
+    print("[case] strain energy gradient vs gradLinElastWork_AN", name, flush=True)
+    if name == "unit_cube_p1":
+        from tests.golden.make_golden import unit_cube_args
+        args = unit_cube_args()
+        d = np.load("tests/golden/unit_cube_p1.npz")
+    else:
+        d, bp = fixture()
+        args = bp.args
+    mf = MatrixFree(args, precisions=F64)
+    SOL, ADJ, gi = d["SOL"], d["ADJ"], d["gradWint"]
+    for k in range(ADJ.shape[0]):
+        g = 0.25 * (q_grad(mf, SOL + ADJ[k]) - q_grad(mf, SOL - ADJ[k]))
+        err = np.abs(g - gi[k]).max()
+        print("  adjoint %d: max |gradWint| %.3e, max err %.3e" % (k, np.abs(gi[k]).max(), err))
+        assert err <= 1e-10 * np.abs(gi[k]).max()
+
+
+def test_compliance_gradient_matches_finite_differences(gpu):
+    import torch
+    from yeti_amd.gradient import CompliancePatch, compliance
+    print("[case] compliance gradient vs central differences", flush=True)
+    d, bp = fixture()
+    for solver in ("csr", "mf"):
+        patch = CompliancePatch(bp.args, solver=solver, rtol=1e-13)
+        X = patch.coords0.clone().cuda().requires_grad_(True)
+        J = compliance(X, patch)
+        J.backward()
+        g = X.grad.detach().cpu().numpy().reshape(-1, 3).T
+        print("  %s: J %.15e (reference %.15e), %d iterations" % (solver, J.item(), float(d["J"]), patch.iters))
+        assert abs(J.item() - float(d["J"])) <= 1e-10 * abs(float(d["J"]))
+        gsel = g[d["fd_dir"], d["fd_cp"]]
+        e4 = np.abs(gsel - d["fd_val_h1e4"])
+        e6 = np.abs(gsel - d["fd_val"])
+        print("  max |fd(1e-4) - g| %.2e, max |fd(1e-6) - g| %.2e, max |g| %.2e" % (e4.max(), e6.max(), np.abs(g).max()))
+        assert np.all(e4 <= 1e-9 + 1e-6 * np.abs(gsel))
+        assert np.all(e6 <= 1e-7 + 1e-4 * np.abs(gsel))
+        # the stiffness part is twice YETI's internal term for the adjoint u / 2
+        u = torch.from_numpy(np.ascontiguousarray(d["SOL"].T.reshape(-1))).cuda()
+        gint = patch.mf.strain_energy_grad(u).cpu().numpy().reshape(-1, 3).T
+        assert np.abs(gint - 2 * d["gradWint"][0]).max() <= 1e-10 * np.abs(d["gradWint"][0]).max() * 2
+        # invariants
+        X0 = patch.coords0.numpy().reshape(-1, 3).T
+        assert np.abs(g.sum(axis=1)).max() <= 1e-9 * np.abs(g).max() * g.shape[1]
+        hom = float((X0 * g).sum())
+        assert abs(hom - 3 * J.item()) <= 1e-8 * abs(J.item())
+
+
+def test_coordinate_update_round_trip(gpu):
+    """Moving the control points and moving them back reproduces J and the gradient bit for bit."""
+    import torch
+    from yeti_amd.gradient import CompliancePatch, compliance
+    print("[case] coordinate update round trip", flush=True)
+    d, bp = fixture()
+    patch = CompliancePatch(bp.args, solver="csr", rtol=1e-13)
+    X0 = patch.coords0.clone().cuda()
+    X = X0.clone().requires_grad_(True)
+    J0 = compliance(X, patch)
+    J0.backward()
+    g0 = X.grad.clone()
+    Xs = (X0 * 1.5).requires_grad_(True)
+    Js = compliance(Xs, patch)
+    assert abs(Js.item() - 1.5 ** 3 * J0.item()) <= 1e-9 * abs(Js.item())
+    X1 = X0.clone().requires_grad_(True)
+    J1 = compliance(X1, patch)
+    J1.backward()
+    assert J1.item() == J0.item()
+    assert torch.equal(X1.grad, g0)

@@ -30,6 +30,8 @@ SIGNATURES = {
     "yeti_asm_run_coo_matrix": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_asm_run_coo_values": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_asm_run_rhs": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_asm_update_coords": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_asm_load_shape_grad": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_asm_destroy": (None, [C.c_void_p]),
     "yeti_csr_create": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "yeti_csr_nnz": (C.c_int64, [C.c_void_p]),
@@ -45,6 +47,8 @@ SIGNATURES = {
     "yeti_mf_apply_f64": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_mf_apply_f32": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_mf_diag_blocks_f64": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_mf_strain_energy_grad_f64": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_mf_update_coords": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_mf_destroy": (None, [C.c_void_p]),
     "yeti_mf_pcg_f64": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int32,
                                     C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p]),

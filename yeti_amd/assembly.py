@@ -52,6 +52,24 @@ class Assembler:
             pass
 
     # ------------------------------------------------------------------------------
+    def update_coords(self, coords, stream=None):
+        """New control-point coordinates (CUDA float64, nb_cp x 3 or 3 nb_cp, COORDS3D order)."""
+        c = coords.reshape(-1)
+        if c.dtype != torch.float64 or c.numel() != 3 * self.nb_cp or not c.is_cuda or not c.is_contiguous():
+            raise ValueError("coords must be a contiguous CUDA float64 tensor of %d entries" % (3 * self.nb_cp))
+        _lib.check(_lib.lib().yeti_asm_update_coords(self._h, C.c_void_p(c.data_ptr()), _stream_handle(stream)))
+
+    def load_shape_grad(self, u, out=None, stream=None):
+        """g[3 I + j] = 2 u^T dF/dX_Ij for the face loads (FP64)."""
+        if u.dtype != torch.float64 or u.numel() != 3 * self.nb_cp or not u.is_cuda:
+            raise ValueError("u must be a CUDA float64 tensor of %d entries" % (3 * self.nb_cp))
+        if out is None:
+            out = torch.empty_like(u)
+        _lib.check(_lib.lib().yeti_asm_load_shape_grad(self._h, C.c_void_p(u.data_ptr()),
+                                                       C.c_void_p(out.data_ptr()), _stream_handle(stream)))
+        return out
+
+    # ------------------------------------------------------------------------------
     def alloc_coo(self, with_indices=True, device="cuda"):
         n = max(self.nb_data, 1)
         K = torch.empty(n, dtype=torch.float64, device=device)

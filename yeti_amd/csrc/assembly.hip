@@ -163,8 +163,8 @@ struct RhsPatch {
 
 template <int MAXN>
 __device__ void rhs_basis(const RhsPatch &rp, int e, const double *pt, int nn, const double *sW,
-                          const double (*sX)[3], double *R, double *Wred, double AJ[3][3],
-                          double *detJ, double *sred) {
+                          const double (*sX)[3], double *R, double (*dR)[3], double AJ[3][3], double *detJ) {
+    // R[c] (and, if dR is given, dR[c][m] = dR/d(parent coordinate m)) for the nodes a = lane + 64 c
     // parametric point and 1D bases (general degree up to 8)
     const int lane = threadIdx.x;
     __shared__ double sb[3][10], sdb[3][10], shalf[3];
@@ -247,9 +247,11 @@ __device__ void rhs_basis(const RhsPatch &rp, int e, const double *pt, int nn, c
           - AJ[0][1] * (AJ[1][0] * AJ[2][2] - AJ[1][2] * AJ[2][0])
           + AJ[0][2] * (AJ[1][0] * AJ[2][1] - AJ[1][1] * AJ[2][0]);
     c = 0;
-    for (int a = lane; a < nn; a += 64, ++c) R[c] = Nl[c] * Wi;
-    (void)Wred;
-    (void)sred;
+    for (int a = lane; a < nn; a += 64, ++c) {
+        R[c] = Nl[c] * Wi;
+        if (dR)
+            for (int m = 0; m < 3; ++m) dR[c][m] = (dNl[c][m] - Nl[c] * dw[m]) * Wi * shalf[m];
+    }
     __syncthreads();
 }
 
@@ -295,7 +297,7 @@ __global__ void __launch_bounds__(64) elem_rhs_kernel(RhsPatch rp, LoadTable lt,
                 for (int gk = 0; gk < ng; ++gk) {
                     double pt[3] = {rp.g.x[gi], rp.g.x[gj], rp.g.x[gk]};
                     double AJ[3][3], detJ;
-                    rhs_basis<RHS_MAXN>(rp, e, pt, nn, sW, sX, R, nullptr, AJ, &detJ, nullptr);
+                    rhs_basis<RHS_MAXN>(rp, e, pt, nn, sW, sX, R, nullptr, AJ, &detJ);
                     double dvol = rp.g.w[gi] * rp.g.w[gj] * rp.g.w[gk] * fabs(detJ);
                     for (int li = l0; li < l1; ++li) {
                         int il = lt.elem_loads[li];
@@ -346,7 +348,7 @@ __global__ void __launch_bounds__(64) elem_rhs_kernel(RhsPatch rp, LoadTable lt,
                 default: pt[0] = xi_j; pt[1] = xi_i; pt[2] = 1.0; break;
                 }
                 double AJ[3][3], detJ;
-                rhs_basis<RHS_MAXN>(rp, e, pt, nn, sW, sX, R, nullptr, AJ, &detJ, nullptr);
+                rhs_basis<RHS_MAXN>(rp, e, pt, nn, sW, sX, R, nullptr, AJ, &detJ);
                 double c1[3] = {AJ[0][0], AJ[1][0], AJ[2][0]};
                 double c2[3] = {AJ[0][1], AJ[1][1], AJ[2][1]};
                 double c3[3] = {AJ[0][2], AJ[1][2], AJ[2][2]};
@@ -394,14 +396,136 @@ __global__ void __launch_bounds__(64) elem_rhs_kernel(RhsPatch rp, LoadTable lt,
         for (int d = 0; d < 3; ++d) rhs_buf[(int64_t)gle * 3 * RHS_MAXN + 3 * a + d] = rhs[c][d];
 }
 
-// F = F_conc + sum over loaded elements (in element order) of their RHS (build_system_elemStorage.f90:295-303).
+// Shape derivative of the surface-load work, per loaded element and node: g_cj = 2 u^T dF/dX_cj
+// for the face loads of elem_rhs_kernel (same faces, quadrature and load types). With the face
+// tangents c_m = dx/dxi_m (columns of AJ), dc_m/dX_cj = e_j dR_c/dxi_m, so
+//   fixed direction v (types 1-3), F_a = R_a v w dS f, dS = |cA x cB|:
+//     d dS/dX_cj = dR_c/dxi_A (cB x n)_j + dR_c/dxi_B (n x cA)_j,  n = (cA x cB)/dS
+//   normal load (types 0 and 4), F_a = R_a (A x B) w f:
+//     u . d(A x B)/dX_cj = dR_c/dxi_A (B x u)_j + dR_c/dxi_B (u x A)_j
+// with u = u(xi_q) = sum_a R_a u_a. Body loads are not handled here (the caller rejects them).
+__global__ void __launch_bounds__(64) elem_load_grad_kernel(RhsPatch rp, LoadTable lt, const double *__restrict__ u,
+                                                            double *grad_buf) {
+    const int le = blockIdx.x;
+    if (le >= rp.n_loaded) return;
+    const int e = rp.elems[le];
+    const int lane = threadIdx.x;
+    const int nn = (rp.p[0] + 1) * (rp.p[1] + 1) * (rp.p[2] + 1);
+    __shared__ double sX[RHS_MAXN][3];
+    __shared__ double sW[RHS_MAXN];
+    __shared__ double sU[RHS_MAXN][3];
+    __shared__ int32_t sIen[RHS_MAXN];
+    for (int a = lane; a < nn; a += 64) {
+        int cp = rp.ien[(int64_t)nn * e + a];
+        sIen[a] = cp;
+        for (int d = 0; d < 3; ++d) {
+            sX[a][d] = rp.coords[3 * (int64_t)(cp - 1) + d];
+            sU[a][d] = u[3 * (int64_t)(cp - 1) + d];
+        }
+        sW[a] = rp.weight[(int64_t)nn * e + a];
+    }
+    __syncthreads();
+    constexpr int NPL = RHS_MAXN / 64 + 1;
+    double g[NPL][3];
+    for (int c = 0; c < NPL; ++c) g[c][0] = g[c][1] = g[c][2] = 0.0;
+    double R[NPL], dR[NPL][3];
+    const int ng = rp.ng;
+    const int gle = rp.slot0 + le;
+    const int l0 = lt.elem_loads_ptr[gle], l1 = lt.elem_loads_ptr[gle + 1];
+    for (int li = l0; li < l1; ++li) {
+        int il = lt.elem_loads[li];
+        int t = lt.jdltype[il];
+        if (!(t > 9 && t < 100)) continue;
+        int face = t / 10, typ = t % 10;
+        int field = typ == 4 ? (int)lt.add_infos[lt.add_off[il]] : 0;
+        // columns spanning the face, in the order of elem_rhs_kernel's normal
+        const int nA = face <= 2 ? 1 : face <= 4 ? 2 : 0, nB = face <= 2 ? 2 : face <= 4 ? 0 : 1;
+        // (A, B) of the normal load per face
+        int mA, mB;
+        switch (face) {
+        case 1: mA = 1; mB = 2; break;
+        case 2: mA = 2; mB = 1; break;
+        case 3: mA = 2; mB = 0; break;
+        case 4: mA = 0; mB = 2; break;
+        case 5: mA = 0; mB = 1; break;
+        default: mA = 1; mB = 0; break;
+        }
+        for (int fj = 0; fj < ng; ++fj)
+            for (int fi = 0; fi < ng; ++fi) {
+                double xi_i = rp.g.x[fi], xi_j = rp.g.x[fj];
+                double pt[3];
+                switch (face) {
+                case 1: pt[0] = -1.0; pt[1] = xi_i; pt[2] = xi_j; break;
+                case 2: pt[0] = 1.0; pt[1] = xi_i; pt[2] = xi_j; break;
+                case 3: pt[0] = xi_i; pt[1] = -1.0; pt[2] = xi_j; break;
+                case 4: pt[0] = xi_i; pt[1] = 1.0; pt[2] = xi_j; break;
+                case 5: pt[0] = xi_j; pt[1] = xi_i; pt[2] = -1.0; break;
+                default: pt[0] = xi_j; pt[1] = xi_i; pt[2] = 1.0; break;
+                }
+                double AJ[3][3], detJ;
+                rhs_basis<RHS_MAXN>(rp, e, pt, nn, sW, sX, R, dR, AJ, &detJ);
+                double col[3][3];
+                for (int m = 0; m < 3; ++m)
+                    for (int d = 0; d < 3; ++d) col[m][d] = AJ[d][m];
+                double nrm[3];
+                cross3(col[nA], col[nB], nrm);
+                const double dS = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+                // u and the load field at the face point
+                double uq[3] = {0.0, 0.0, 0.0}, fm = 0.0;
+                int c = 0;
+                for (int a = lane; a < nn; a += 64, ++c) {
+                    for (int d = 0; d < 3; ++d) uq[d] += R[c] * sU[a][d];
+                    if (typ == 4) fm += lt.nodal_dist[(int64_t)lt.nb_n_dist * (sIen[a] - 1) + (field - 1)] * R[c];
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    for (int d = 0; d < 3; ++d) uq[d] += __shfl_xor(uq[d], o, 64);
+                    fm += __shfl_xor(fm, o, 64);
+                }
+                const double fmag = typ == 4 ? fm : lt.adlmag[il];
+                const double wq = 2.0 * rp.g.w[fi] * rp.g.w[fj] * fmag;
+                double tA[3], tB[3];   // g_c = wq (dR_c/dxi_A tA + dR_c/dxi_B tB)
+                int cA, cB;
+                if (typ >= 1 && typ <= 3) {
+                    const double un = uq[typ - 1];   // u . v
+                    double n[3] = {nrm[0] / dS, nrm[1] / dS, nrm[2] / dS};
+                    cross3(col[nB], n, tA);
+                    cross3(n, col[nA], tB);
+                    for (int d = 0; d < 3; ++d) {
+                        tA[d] *= un;
+                        tB[d] *= un;
+                    }
+                    cA = nA;
+                    cB = nB;
+                } else {
+                    cross3(col[mB], uq, tA);
+                    cross3(uq, col[mA], tB);
+                    cA = mA;
+                    cB = mB;
+                }
+                c = 0;
+                for (int a = lane; a < nn; a += 64, ++c)
+                    for (int d = 0; d < 3; ++d) g[c][d] += wq * (dR[c][cA] * tA[d] + dR[c][cB] * tB[d]);
+            }
+    }
+    int c = 0;
+    for (int a = lane; a < nn; a += 64, ++c)
+        for (int d = 0; d < 3; ++d) grad_buf[(int64_t)gle * 3 * RHS_MAXN + 3 * a + d] = g[c][d];
+}
+
+// F = F_conc + sum over loaded elements (in element order) of their RHS (build_system_elemStorage.f90:295-303);
+// F_conc may be null (then F starts from zero).
 __global__ void rhs_gather_kernel(int32_t nb_cp, const double *__restrict__ F_conc,
                                   const int32_t *__restrict__ cp_ptr,
                                   const int32_t *__restrict__ cp_ent,
                                   const double *__restrict__ rhs_buf, double *__restrict__ F) {
     int cp = blockIdx.x * blockDim.x + threadIdx.x;
     if (cp >= nb_cp) return;
-    double f0 = F_conc[3 * (int64_t)cp], f1 = F_conc[3 * (int64_t)cp + 1], f2 = F_conc[3 * (int64_t)cp + 2];
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    if (F_conc) {
+        f0 = F_conc[3 * (int64_t)cp];
+        f1 = F_conc[3 * (int64_t)cp + 1];
+        f2 = F_conc[3 * (int64_t)cp + 2];
+    }
     for (int k = cp_ptr[cp]; k < cp_ptr[cp + 1]; ++k) {
         int ent = cp_ent[k];   // slot * RHS_MAXN + local node
         const double *r = rhs_buf + 3 * (int64_t)ent;
@@ -462,8 +586,9 @@ struct yeti_asm {
     // loads
     yeti::DevBuf<double> F_conc;
     yeti::DevBuf<int32_t> jdl, add_off, elem_loads, elem_loads_ptr, cp_ptr, cp_ent;
-    yeti::DevBuf<double> adlmag, add_infos, nodal_dist, rhs_buf;
+    yeti::DevBuf<double> adlmag, add_infos, nodal_dist, rhs_buf, grad_buf;
     int32_t nb_n_dist = 1;
+    bool body_loads = false;   // any element carries a body load (101 / 102)
     int32_t n_loaded_total = 0;
     ~yeti_asm() {
         for (auto *p : patches) delete p;
@@ -569,6 +694,47 @@ static int run_elements(yeti_asm *h, double *Kd, int32_t *Kr, int32_t *Kc, hipSt
     return 0;
 }
 
+// u^T F shape derivative of the face loads: g = 2 u^T dF/dX (3 per CP), gathered like F.
+static int run_load_grad(yeti_asm *h, const double *u, double *g, hipStream_t st) {
+    if (h->body_loads)
+        return set_error(YETI_ERR_UNSUPPORTED_ELEMENT, "load shape gradient: body loads (101/102) are not supported");
+    if (h->n_loaded_total > 0 && !h->grad_buf.p) YETI_HIP(h->grad_buf.alloc((size_t)h->n_loaded_total * 3 * RHS_MAXN));
+    LoadTable lt;
+    lt.jdltype = h->jdl.p;
+    lt.adlmag = h->adlmag.p;
+    lt.add_off = h->add_off.p;
+    lt.add_infos = h->add_infos.p;
+    lt.elem_loads = h->elem_loads.p;
+    lt.elem_loads_ptr = h->elem_loads_ptr.p;
+    lt.nodal_dist = h->nodal_dist.p;
+    lt.nb_n_dist = h->nb_n_dist;
+    for (auto *pd : h->patches) {
+        if (pd->n_loaded == 0) continue;
+        RhsPatch rp;
+        rp.coords = h->coords.p;
+        rp.ien = pd->ien.p;
+        rp.weight = pd->weight.p;
+        rp.nijk = pd->nijk.p;
+        for (int d = 0; d < 3; ++d) {
+            rp.U[d] = pd->U[d].p;
+            rp.p[d] = pd->p[d];
+        }
+        rp.elems = pd->loaded.p;
+        rp.n_loaded = pd->n_loaded;
+        rp.slot0 = pd->slot0;
+        rp.ng = pd->ng;
+        rp.rho = pd->rho;
+        gauss_legendre(pd->ng, rp.g.x, rp.g.w);
+        hipLaunchKernelGGL(elem_load_grad_kernel, dim3(pd->n_loaded), dim3(64), 0, st, rp, lt, u, h->grad_buf.p);
+        YETI_HIP(hipGetLastError());
+    }
+    int blocks = (h->nb_cp + 255) / 256;
+    hipLaunchKernelGGL(rhs_gather_kernel, dim3(blocks), dim3(256), 0, st, h->nb_cp, (const double *)nullptr,
+                       h->cp_ptr.p, h->cp_ent.p, (const double *)h->grad_buf.p, g);
+    YETI_HIP(hipGetLastError());
+    return 0;
+}
+
 static int run_rhs(yeti_asm *h, double *F, hipStream_t st) {
     LoadTable lt;
     lt.jdltype = h->jdl.p;
@@ -644,6 +810,7 @@ static int create(const yeti_system_inputs *in, int layout, yeti_asm **out) {
         for (int64_t k = load_off[il]; k < load_off[il + 1]; ++k) {
             int64_t el = in->indDLoad[k];
             if (el >= 1 && el <= in->nb_elem) {
+                if (t == 101 || t == 102) h->body_loads = true;
                 auto &v = loads_of_elem[el - 1];
                 if (v.empty() || v.back() != il) v.push_back(il);
             }
@@ -867,6 +1034,18 @@ int32_t yeti_asm_run_coo_values(yeti_asm *h, double *Kd, void *stream) {
 int32_t yeti_asm_run_rhs(yeti_asm *h, double *F, void *stream) {
     if (!h || !F) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
     return yeti::run_rhs(h, F, (hipStream_t)stream);
+}
+
+int32_t yeti_asm_load_shape_grad(yeti_asm *h, const double *u, double *g, void *stream) {
+    if (!h || !u || !g) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    return yeti::run_load_grad(h, u, g, (hipStream_t)stream);
+}
+
+int32_t yeti_asm_update_coords(yeti_asm *h, const double *coords, void *stream) {
+    if (!h || !coords) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    YETI_HIP(hipMemcpyAsync(h->coords.p, coords, 3 * sizeof(double) * (size_t)h->nb_cp, hipMemcpyDeviceToDevice,
+                            (hipStream_t)stream));
+    return 0;
 }
 
 void yeti_asm_destroy(yeti_asm *h) { delete h; }

@@ -5,7 +5,7 @@
 // patch's elements. Each element is cut into steps (pass, Gauss-point chunk). In step s
 //   * 4 producer waves build step s+1: per Gauss point the reductions W, dW, sum x N,
 //     sum x (x) dN over the element's nodes -> J^{-1}, w|det J| (shap.f90:94-178), then
-//     G_c[a][q] = dR_a/dx_c into one half of a double-buffered LDS array;
+//     G_c[a][q] = sqrt(w|det J|) dR_a/dx_c into one half of a double-buffered LDS array;
 //   * 8 consumer waves accumulate S^{kl}_{ab} = sum_q w|J| dR_a/dx_k dR_b/dx_l over step s's
 //     chunk on v_mfma_f64_16x16x4f64 and, after an element's last chunk, contract its COO
 //     values into an LDS stage, from which all 12 waves write them (and the row / column
@@ -21,7 +21,20 @@
 // schedule deals the units to the consumer waves so every SIMD gets the same MFMA work.
 #pragma once
 
+#include <utility>
+
 typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// Compile-time loop: f(std::integral_constant<int, 0>{}), ..., f(std::integral_constant<int, N-1>{}),
+// so schedule tables indexed by the loop variable fold to constants.
+template <typename F, int... Is>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, Is...>) {
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 template <int P, int NG>
 struct ElemCfg {
@@ -162,8 +175,8 @@ struct PatchArgs {
 template <int P, int NG>
 struct ElemSmem {
     using C = ElemCfg<P, NG>;
-    __attribute__((aligned(16))) double G[2][3][C::NNP][C::QCP];  // dR_a/dx_c, double buffered
-    __attribute__((aligned(16))) double om[2][C::QC];             // w |det J|
+    // sqrt(w |det J|) dR_a/dx_c at the chunk's Gauss points, double buffered
+    __attribute__((aligned(16))) double G[2][3][C::NNP][C::QCP];
     double B[C::PW][3][NG][C::NP1];     // per producer wave: B(reversed local index) at the abscissae
     double dB[C::PW][3][NG][C::NP1];
     double half[C::PW][3];
@@ -291,8 +304,7 @@ __device__ __forceinline__ void producer_chunk(ElemSmem<P, NG> &sm, const PatchA
 #pragma unroll
         for (int o = 1; o < C::LPG; o <<= 1) s[m] += __shfl_xor(s[m], o, C::LPG);
     // every lane of the group now holds the sums; form 1/W, dW/W, J^{-1}, w|det J|
-    double gpv[13];      // Wi, dw0..2, Ji[c][r] (9)
-    double omv = 0.0;
+    double gpv[14];      // Wi, dw0..2, Ji[c][r] (9), sqrt(w |det J|)
     {
         const double Wi = live ? 1.0 / s[0] : 0.0;
         const double dw[3] = {s[1] * Wi, s[2] * Wi, s[3] * Wi};
@@ -319,12 +331,12 @@ __device__ __forceinline__ void producer_chunk(ElemSmem<P, NG> &sm, const PatchA
         gpv[10] = m02 * id;
         gpv[11] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
         gpv[12] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+        gpv[13] = 0.0;
         if (live) {
             const double detJ = det * sm.half[pw][0] * sm.half[pw][1] * sm.half[pw][2];
-            omv = pa.g.w[g0] * pa.g.w[g1] * pa.g.w[g2] * fabs(detJ);
+            gpv[13] = sqrt(pa.g.w[g0] * pa.g.w[g1] * pa.g.w[g2] * fabs(detJ));
         }
     }
-    if (sub == 0) sm.om[buf][qw + gq] = omv;
     // ---- B. G_c[a][q]: lane = node, loop over the wave's Gauss points ------------------
 #pragma unroll
     for (int m = 0; m < (NNP + 63) / 64; ++m) {
@@ -335,9 +347,9 @@ __device__ __forceinline__ void producer_chunk(ElemSmem<P, NG> &sm, const PatchA
         for (int t = 0; t < C::GPW; ++t) {
             const int src = t * C::LPG;                  // a lane of Gauss point t's group
             const int qt = q0 + qw + t;
-            double v[13];
+            double v[14];
 #pragma unroll
-            for (int c = 0; c < 13; ++c) v[c] = bcast(gpv[c], src);
+            for (int c = 0; c < 14; ++c) v[c] = bcast(gpv[c], src);
             double g0v = 0.0, g1v = 0.0, g2v = 0.0;
             if (a < NN && qt < C::NQ) {
                 const int h0 = qt % NG, h1 = (qt / NG) % NG, h2 = qt / (NG * NG);
@@ -348,9 +360,9 @@ __device__ __forceinline__ void producer_chunk(ElemSmem<P, NG> &sm, const PatchA
                 const double r0 = sm.dB[pw][0][h0][i] * b12 - R * v[1];
                 const double r1 = b0 * sm.dB[pw][1][h1][j] * b2 * w - R * v[2];
                 const double r2 = b0 * b1 * sm.dB[pw][2][h2][k] * w - R * v[3];
-                g0v = r0 * v[4] + r1 * v[7] + r2 * v[10];
-                g1v = r0 * v[5] + r1 * v[8] + r2 * v[11];
-                g2v = r0 * v[6] + r1 * v[9] + r2 * v[12];
+                g0v = (r0 * v[4] + r1 * v[7] + r2 * v[10]) * v[13];
+                g1v = (r0 * v[5] + r1 * v[8] + r2 * v[11]) * v[13];
+                g2v = (r0 * v[6] + r1 * v[9] + r2 * v[12]) * v[13];
             }
             if (a < NNP) {
                 sm.G[buf][0][a][qw + t] = g0v;
@@ -370,22 +382,21 @@ __device__ __forceinline__ void mfma_chunk(const ElemSmem<P, NG> &sm, int buf,
                                            f64x4 (&accD)[ElemCfg<P, NG>::DCAP][3],
                                            f64x4 (&accP)[ElemCfg<P, NG>::PCAP][2]) {
     using C = ElemCfg<P, NG>;
-    constexpr auto &S = Sched<P, NG>::S;
-    constexpr int nd = S.nd[PASS][W], np = S.np[PASS][W];
+    constexpr int nd = Sched<P, NG>::S.nd[PASS][W], np = Sched<P, NG>::S.np[PASS][W];
     const int lane = threadIdx.x & 63;
     const int nl = lane & 15, kg = lane >> 4;
 #pragma unroll 1
     for (int t = 0; t < C::QC / 8; ++t) {
         const int qo = 8 * t + 2 * kg;
-        const f64x2 om = *reinterpret_cast<const f64x2 *>(&sm.om[buf][qo]);
-#pragma unroll
-        for (int u = 0; u < nd; ++u) {
-            const int pi = S.d[PASS][W][u];
-            const int ra = 16 * pair_I(pi) + nl, rb = 16 * pair_J(pi) + nl;
+        static_for<nd>([&](auto ic) {
+            constexpr int u = decltype(ic)::value;
+            constexpr int pi = Sched<P, NG>::S.d[PASS][W][u];
+            constexpr int I = pair_I(pi), J = pair_J(pi);
+            const int ra = 16 * I + nl, rb = 16 * J + nl;
             f64x2 A[3], B[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                A[c] = *reinterpret_cast<const f64x2 *>(&sm.G[buf][c][ra][qo]) * om;
+                A[c] = *reinterpret_cast<const f64x2 *>(&sm.G[buf][c][ra][qo]);
                 B[c] = *reinterpret_cast<const f64x2 *>(&sm.G[buf][c][rb][qo]);
             }
 #pragma unroll
@@ -393,22 +404,23 @@ __device__ __forceinline__ void mfma_chunk(const ElemSmem<P, NG> &sm, int buf,
                 accD[u][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c].x, B[c].x, accD[u][c], 0, 0, 0);
                 accD[u][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c].y, B[c].y, accD[u][c], 0, 0, 0);
             }
-        }
-#pragma unroll
-        for (int u = 0; u < np; ++u) {
-            const int code = S.p[PASS][W][u];
-            const int pi = code / 3, which = code % 3;
-            const int k = p_k(which), l = p_l(which);
-            const int ra = 16 * pair_I(pi) + nl, rb = 16 * pair_J(pi) + nl;
-            const f64x2 Ak = *reinterpret_cast<const f64x2 *>(&sm.G[buf][k][ra][qo]) * om;
-            const f64x2 Al = *reinterpret_cast<const f64x2 *>(&sm.G[buf][l][ra][qo]) * om;
+        });
+        static_for<np>([&](auto ic) {
+            constexpr int u = decltype(ic)::value;
+            constexpr int code = Sched<P, NG>::S.p[PASS][W][u];
+            constexpr int pi = code / 3, which = code % 3;
+            constexpr int k = p_k(which), l = p_l(which);
+            constexpr int I = pair_I(pi), J = pair_J(pi);
+            const int ra = 16 * I + nl, rb = 16 * J + nl;
+            const f64x2 Ak = *reinterpret_cast<const f64x2 *>(&sm.G[buf][k][ra][qo]);
+            const f64x2 Al = *reinterpret_cast<const f64x2 *>(&sm.G[buf][l][ra][qo]);
             const f64x2 Bk = *reinterpret_cast<const f64x2 *>(&sm.G[buf][k][rb][qo]);
             const f64x2 Bl = *reinterpret_cast<const f64x2 *>(&sm.G[buf][l][rb][qo]);
             accP[u][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ak.x, Bl.x, accP[u][0], 0, 0, 0);
             accP[u][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ak.y, Bl.y, accP[u][0], 0, 0, 0);
             accP[u][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(Al.x, Bk.x, accP[u][1], 0, 0, 0);
             accP[u][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(Al.y, Bk.y, accP[u][1], 0, 0, 0);
-        }
+        });
     }
 }
 
@@ -426,61 +438,66 @@ __device__ __forceinline__ void stage_round(ElemSmem<P, NG> &sm, double lam, dou
     constexpr auto &S = Sched<P, NG>::S;
     constexpr int nd = S.nd[PASS][W], np = S.np[PASS][W];
     constexpr int b0 = S.rb[PASS][R], b1 = S.rb[PASS][R + 1];
+    (void)S;
     constexpr int s0 = (int)coo_start(b0);
     const int lane = threadIdx.x & 63;
     const int nl = lane & 15, kg = lane >> 4;
+    static_for<nd>([&](auto ic) {
+        constexpr int u = decltype(ic)::value;
+        constexpr int pi = Sched<P, NG>::S.d[PASS][W][u];
+        constexpr int I = pair_I(pi), J = pair_J(pi);
+        if constexpr (16 * J + 15 >= b0 && 16 * J < b1) {
+            const int b = 16 * J + nl;
+            if (b >= b0 && b < b1) {
+                double *st = sm.stage + ((int)coo_start(b) - s0);
 #pragma unroll
-    for (int u = 0; u < nd; ++u) {
-        const int pi = S.d[PASS][W][u];
-        const int I = pair_I(pi), J = pair_J(pi);
-        if (16 * J + 15 < b0 || 16 * J >= b1) continue;      // resolved at compile time
-        const int b = 16 * J + nl;
-        if (b < b0 || b >= b1) continue;
-        double *st = sm.stage + ((int)coo_start(b) - s0);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int a = 16 * I + kg + 4 * r;
-            if (I == J && a > b) continue;
-            const double c0 = accD[u][0][r], c1 = accD[u][1][r], c2 = accD[u][2][r];
-            const double tr = mu * (c0 + c1 + c2);
-            const double k00 = (lam + mu) * c0 + tr, k11 = (lam + mu) * c1 + tr, k22 = (lam + mu) * c2 + tr;
-            if (a < b) {
-                st[9 * a + 0] = k00;
-                st[9 * a + 4] = k11;
-                st[9 * a + 8] = k22;
-            } else {
-                st[9 * b + 0] = 0.5 * k00;
-                st[9 * b + 2] = 0.5 * k11;
-                st[9 * b + 5] = 0.5 * k22;
+                for (int r = 0; r < 4; ++r) {
+                    const int a = 16 * I + kg + 4 * r;
+                    if (I == J && a > b) continue;
+                    const double c0 = accD[u][0][r], c1 = accD[u][1][r], c2 = accD[u][2][r];
+                    const double tr = mu * (c0 + c1 + c2);
+                    const double k00 = (lam + mu) * c0 + tr, k11 = (lam + mu) * c1 + tr, k22 = (lam + mu) * c2 + tr;
+                    if (a < b) {
+                        st[9 * a + 0] = k00;
+                        st[9 * a + 4] = k11;
+                        st[9 * a + 8] = k22;
+                    } else {
+                        st[9 * b + 0] = 0.5 * k00;
+                        st[9 * b + 2] = 0.5 * k11;
+                        st[9 * b + 5] = 0.5 * k22;
+                    }
+                }
             }
         }
-    }
+    });
+    static_for<np>([&](auto ic) {
+        constexpr int u = decltype(ic)::value;
+        constexpr int code = Sched<P, NG>::S.p[PASS][W][u];
+        constexpr int pi = code / 3, which = code % 3;
+        constexpr int k = p_k(which), l = p_l(which);
+        constexpr int I = pair_I(pi), J = pair_J(pi);
+        constexpr int mdiag = (k == 0 && l == 1) ? 1 : (k == 0 && l == 2) ? 3 : 4;
+        if constexpr (16 * J + 15 >= b0 && 16 * J < b1) {
+            const int b = 16 * J + nl;
+            if (b >= b0 && b < b1) {
+                double *st = sm.stage + ((int)coo_start(b) - s0);
 #pragma unroll
-    for (int u = 0; u < np; ++u) {
-        const int code = S.p[PASS][W][u];
-        const int pi = code / 3, which = code % 3;
-        const int I = pair_I(pi), J = pair_J(pi);
-        if (16 * J + 15 < b0 || 16 * J >= b1) continue;
-        const int k = p_k(which), l = p_l(which);
-        const int b = 16 * J + nl;
-        if (b < b0 || b >= b1) continue;
-        double *st = sm.stage + ((int)coo_start(b) - s0);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int a = 16 * I + kg + 4 * r;
-            if (I == J && a > b) continue;
-            const double ckl = accP[u][0][r], clk = accP[u][1][r];
-            const double kkl = lam * ckl + mu * clk;     // K_ab[k][l]
-            const double klk = lam * clk + mu * ckl;     // K_ab[l][k]
-            if (a < b) {
-                st[9 * a + k + 3 * l] = kkl;
-                st[9 * a + l + 3 * k] = klk;
-            } else {
-                // diagonal block keeps the upper entry (k, l), k < l
-                st[9 * b + ((k == 0 && l == 1) ? 1 : (k == 0 && l == 2) ? 3 : 4)] = kkl;
+                for (int r = 0; r < 4; ++r) {
+                    const int a = 16 * I + kg + 4 * r;
+                    if (I == J && a > b) continue;
+                    const double ckl = accP[u][0][r], clk = accP[u][1][r];
+                    const double kkl = lam * ckl + mu * clk;     // K_ab[k][l]
+                    const double klk = lam * clk + mu * ckl;     // K_ab[l][k]
+                    if (a < b) {
+                        st[9 * a + k + 3 * l] = kkl;
+                        st[9 * a + l + 3 * k] = klk;
+                    } else {
+                        st[9 * b + mdiag] = kkl;   // diagonal block keeps the upper entry (k, l), k < l
+                    }
+                }
             }
         }
-    }
+    });
 }
 
 // Epilogue, second half (all 768 threads): round R of pass PASS leaves the LDS stage with

@@ -177,24 +177,28 @@ __global__ void mf_geometry_kernel(MfPatch mp, const double *__restrict__ coords
     }
 }
 
-// Brick edge in elements: 2(p+1) for the apply (8 elements per colour), p+1 for the diagonal
-// blocks (1 element per colour, keeps the 9-value accumulator within LDS).
+// Brick edge in elements: 2(p+1) for the apply and the shape gradient (8 elements per colour),
+// p+1 for the diagonal blocks (1 element per colour, keeps the 9-value accumulator within LDS).
 template <int P, int MODE>
 struct MfBrick {
-    static constexpr int BR = MODE == 0 ? 2 * (P + 1) : (P + 1);
+    static constexpr int BR = MODE == 1 ? (P + 1) : 2 * (P + 1);
     static constexpr int BN = BR + P;
     static constexpr int K = BR / (P + 1);            // elements per colour per direction
     static constexpr int EPC = K * K * K;
 };
 
-// MODE 0: y = K x (3 values per node); MODE 1: 3x3 diagonal blocks (9 values per node).
+// MODE 0: y = K x (3 values per node); MODE 1: 3x3 diagonal blocks (9 values per node);
+// MODE 2: shape derivative of the strain energy, y_cj = -x^T (dK / dX_cj) x (3 values per node).
+// For MODE 2 the quadrature-point tensor is M = 2 H^T sigma - (sigma : H) I in place of sigma:
+// with d(dR_a/dx_k)/dX_cj = -dR_a/dx_j dR_c/dx_k and d(det J)/dX_cj = det J dR_c/dx_j,
+// -x^T dK_e/dX_cj x = sum_q w|det J| sum_k M_jk dR_c/dx_k, the same backward contraction as K x.
 template <typename T, int P, int NG, int MODE>
 __global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__restrict__ geo, const T *__restrict__ x,
                                                        const T *__restrict__ wcp, const int32_t *__restrict__ nbrick,
                                                        T *__restrict__ partial) {
     using C = MfCfg<P, NG>;
     using Bk = MfBrick<P, MODE>;
-    constexpr int NP1 = C::NP1, NQ = C::NQ, NN = C::NN, BN = Bk::BN, BR = Bk::BR, NC = MODE == 0 ? 3 : 9;
+    constexpr int NP1 = C::NP1, NQ = C::NQ, NN = C::NN, BN = Bk::BN, BR = Bk::BR, NC = MODE == 1 ? 9 : 3;
     __shared__ MfSmem<T, P, NG, NC, BN> sm;
     const int tid = threadIdx.x;
     const int slot = tid / C::TPE, r = tid % C::TPE;
@@ -223,7 +227,7 @@ __global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__re
                 int cp = mp.cpmap[cpl];
                 T w = wcp[cp];
                 sm.w[slot][r] = w;
-                if (MODE == 0)
+                if (MODE != 1)
                     for (int l = 0; l < 3; ++l) b0[l * NN + r] = w * x[3 * (int64_t)cp + l];
             }
             if (live)
@@ -237,7 +241,7 @@ __global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__re
             __syncthreads();
             T yv[NC];
             for (int c = 0; c < NC; ++c) yv[c] = T(0);
-            if (MODE == 0) {
+            if (MODE != 1) {
                 const auto &B = sm.B[slot < C::EPB ? slot : 0];
                 const auto &dB = sm.dB[slot < C::EPB ? slot : 0];
                 // stage 1: contract i -> g0. b1v[(f*2+v)*NG*NP1*NP1 + (k*NP1+j)*NG + g0]
@@ -311,6 +315,20 @@ __global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__re
                     T S[3][3];
                     for (int k = 0; k < 3; ++k)
                         for (int l = 0; l < 3; ++l) S[k][l] = mu * (H[k][l] + H[l][k]) + (k == l ? tr : T(0));
+                    if constexpr (MODE == 2) {
+                        T en = 0;   // sigma : H
+                        for (int k = 0; k < 3; ++k)
+                            for (int l = 0; l < 3; ++l) en += S[k][l] * H[k][l];
+                        T M[3][3];
+                        for (int j = 0; j < 3; ++j)
+                            for (int k = 0; k < 3; ++k) {
+                                T m = 0;
+                                for (int i = 0; i < 3; ++i) m += H[i][j] * S[i][k];
+                                M[j][k] = 2 * m - (j == k ? en : T(0));
+                            }
+                        for (int j = 0; j < 3; ++j)
+                            for (int k = 0; k < 3; ++k) S[j][k] = M[j][k];
+                    }
                     // coefficients of dB/dxi_m (Tm) and of B (Tb) for each component k
                     for (int k = 0; k < 3; ++k) {
                         T tb = 0;
@@ -486,10 +504,11 @@ namespace yeti {
 
 template <typename T, int P, int NG, int MODE>
 static int launch_brick(const MfPatchDev &pd, const T *geo, const T *x, const T *w, T *part, hipStream_t st) {
-    const int32_t *nbk = pd.nbrick[MODE];
+    constexpr int BM = MODE == 1 ? 1 : 0;   // brick grid: apply-sized or diagonal-sized
+    const int32_t *nbk = pd.nbrick[BM];
     int64_t nb = (int64_t)nbk[0] * nbk[1] * nbk[2];
     hipLaunchKernelGGL((mf_brick_kernel<T, P, NG, MODE>), dim3((unsigned)nb), dim3(256), 0, st, pd.mp, geo, x, w,
-                       pd.nbrick_dev.p + 3 * MODE, part);
+                       pd.nbrick_dev.p + 3 * BM, part);
     YETI_HIP(hipGetLastError());
     return 0;
 }
@@ -507,7 +526,7 @@ static int dispatch_brick(const MfPatchDev &pd, const T *geo, const T *x, const 
 
 template <typename T, int MODE>
 static int mf_run(const yeti_mf *m, const T *x, T *y, hipStream_t st) {
-    constexpr int NC = MODE == 0 ? 3 : 9;
+    constexpr int NC = MODE == 1 ? 9 : 3;
     for (size_t ip = 0; ip < m->patches.size(); ++ip) {
         const MfPatchDev &pd = *m->patches[ip];
         const T *geo;
@@ -525,13 +544,29 @@ static int mf_run(const yeti_mf *m, const T *x, T *y, hipStream_t st) {
         if (!geo || !part) return set_error(YETI_ERR_INVALID_ARGUMENT, "matrix-free: precision not prepared");
         int rc = dispatch_brick<T, MODE>(pd, geo, x, w, part, st);
         if (rc) return rc;
-        const int br = MODE == 0 ? 2 * (pd.p + 1) : pd.p + 1, bn = br + pd.p;
-        const int32_t *nbk = pd.nbrick[MODE];
+        const int br = MODE == 1 ? pd.p + 1 : 2 * (pd.p + 1), bn = br + pd.p;
+        const int32_t *nbk = pd.nbrick[MODE == 1 ? 1 : 0];
         int64_t ncp = (int64_t)pd.mp.ncp[0] * pd.mp.ncp[1] * pd.mp.ncp[2];
         hipLaunchKernelGGL((mf_gather_kernel<T, NC>), dim3((unsigned)((ncp + 255) / 256)), dim3(256), 0, st, pd.mp, br,
                            bn, nbk[0], nbk[1], nbk[2], part, y, ip > 0 ? 1 : 0);
         YETI_HIP(hipGetLastError());
     }
+    return 0;
+}
+
+// Quadrature-point geometry of one patch from the handle's current coordinates.
+static int mf_geometry(const yeti_mf *m, const MfPatchDev &pd, hipStream_t st) {
+    const int64_t nel = (int64_t)pd.mp.N[0] * pd.mp.N[1] * pd.mp.N[2];
+    const int64_t nt = nel * pd.ng * pd.ng * pd.ng;
+    const unsigned blocks = (unsigned)((nt + 255) / 256);
+    if (nt == 0) return 0;
+    switch (pd.p) {
+    case 1: hipLaunchKernelGGL((mf_geometry_kernel<1, 2>), dim3(blocks), dim3(256), 0, st, pd.mp, m->coords.p, m->w64.p, pd.half.p, pd.g, pd.geo64.p, pd.geo32.p); break;
+    case 2: hipLaunchKernelGGL((mf_geometry_kernel<2, 3>), dim3(blocks), dim3(256), 0, st, pd.mp, m->coords.p, m->w64.p, pd.half.p, pd.g, pd.geo64.p, pd.geo32.p); break;
+    case 3: hipLaunchKernelGGL((mf_geometry_kernel<3, 4>), dim3(blocks), dim3(256), 0, st, pd.mp, m->coords.p, m->w64.p, pd.half.p, pd.g, pd.geo64.p, pd.geo32.p); break;
+    default: hipLaunchKernelGGL((mf_geometry_kernel<4, 5>), dim3(blocks), dim3(256), 0, st, pd.mp, m->coords.p, m->w64.p, pd.half.p, pd.g, pd.geo64.p, pd.geo32.p); break;
+    }
+    YETI_HIP(hipGetLastError());
     return 0;
 }
 
@@ -673,15 +708,7 @@ int32_t yeti_mf_create(const yeti_system_inputs *in, int32_t precisions, yeti_mf
         if (er == hipSuccess) er = pd->part64.alloc(part_n);
         if (er == hipSuccess && (precisions & 2)) er = pd->part32.alloc(nb0 * bn0 * bn0 * bn0 * 3);
         if (er != hipSuccess) return fail(set_error(YETI_ERR_DEVICE, "matrix-free: allocation failed (out of memory?)"));
-        const int64_t nt = nel * nq;
-        const unsigned blocks = (unsigned)((nt + 255) / 256);
-        switch (p0) {
-        case 1: hipLaunchKernelGGL((mf_geometry_kernel<1, 2>), dim3(blocks), dim3(256), 0, 0, pd->mp, m->coords.p, m->w64.p, pd->half.p, pd->g, pd->geo64.p, pd->geo32.p); break;
-        case 2: hipLaunchKernelGGL((mf_geometry_kernel<2, 3>), dim3(blocks), dim3(256), 0, 0, pd->mp, m->coords.p, m->w64.p, pd->half.p, pd->g, pd->geo64.p, pd->geo32.p); break;
-        case 3: hipLaunchKernelGGL((mf_geometry_kernel<3, 4>), dim3(blocks), dim3(256), 0, 0, pd->mp, m->coords.p, m->w64.p, pd->half.p, pd->g, pd->geo64.p, pd->geo32.p); break;
-        default: hipLaunchKernelGGL((mf_geometry_kernel<4, 5>), dim3(blocks), dim3(256), 0, 0, pd->mp, m->coords.p, m->w64.p, pd->half.p, pd->g, pd->geo64.p, pd->geo32.p); break;
-        }
-        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        if (mf_geometry(m, *pd, 0) != 0 || hipDeviceSynchronize() != hipSuccess)
             return fail(set_error(YETI_ERR_DEVICE, "matrix-free: geometry kernel failed"));
         // the FP64 geometry is kept: the diagonal blocks and the FP64 apply read it
         woff += (int64_t)nn * ne;
@@ -705,6 +732,23 @@ int32_t yeti_mf_apply_f32(const yeti_mf *m, const float *x, float *y, void *stre
 int32_t yeti_mf_diag_blocks_f64(const yeti_mf *m, double *blocks, void *stream) {
     if (!m || !blocks) return set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
     return mf_run<double, 1>(m, nullptr, blocks, (hipStream_t)stream);
+}
+
+int32_t yeti_mf_strain_energy_grad_f64(const yeti_mf *m, const double *u, double *g, void *stream) {
+    if (!m || !u || !g) return set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    if (!(m->precisions & 1)) return set_error(YETI_ERR_INVALID_ARGUMENT, "matrix-free: FP64 not prepared");
+    return mf_run<double, 2>(m, u, g, (hipStream_t)stream);
+}
+
+int32_t yeti_mf_update_coords(yeti_mf *m, const double *coords, void *stream) {
+    if (!m || !coords) return set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    YETI_HIP(hipMemcpyAsync(m->coords.p, coords, 3 * sizeof(double) * (size_t)m->nb_cp, hipMemcpyDeviceToDevice, st));
+    for (auto *pd : m->patches) {
+        int rc = mf_geometry(m, *pd, st);
+        if (rc) return rc;
+    }
+    return 0;
 }
 
 int32_t yeti_mf_nb_cp(const yeti_mf *m) { return m ? m->nb_cp : -1; }

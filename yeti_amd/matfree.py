@@ -61,6 +61,24 @@ class MatrixFree:
         _lib.check(_lib.lib().yeti_mf_diag_blocks_f64(self._h, C.c_void_p(out.data_ptr()), _stream_handle(stream)))
         return out.view(self.nb_cp, 3, 3)
 
+    def strain_energy_grad(self, u, out=None, stream=None):
+        """g[3 I + j] = -u^T (dK/dX_Ij) u (FP64), the stiffness part of the compliance shape
+        gradient (optim/gradtotalwork_fullAN.f90:17-300 with adjoint u)."""
+        if u.dtype != torch.float64 or u.numel() != self.ndof or not u.is_cuda:
+            raise ValueError("u must be a CUDA float64 tensor of %d entries" % self.ndof)
+        if out is None:
+            out = torch.empty_like(u)
+        _lib.check(_lib.lib().yeti_mf_strain_energy_grad_f64(self._h, C.c_void_p(u.data_ptr()),
+                                                              C.c_void_p(out.data_ptr()), _stream_handle(stream)))
+        return out
+
+    def update_coords(self, coords, stream=None):
+        """New control-point coordinates (CUDA float64, nb_cp x 3 or 3 nb_cp, COORDS3D order)."""
+        c = coords.reshape(-1)
+        if c.dtype != torch.float64 or c.numel() != self.ndof or not c.is_cuda or not c.is_contiguous():
+            raise ValueError("coords must be a contiguous CUDA float64 tensor of %d entries" % self.ndof)
+        _lib.check(_lib.lib().yeti_mf_update_coords(self._h, C.c_void_p(c.data_ptr()), _stream_handle(stream)))
+
     def pcg_f64(self, b, free_mask, rtol=1e-10, maxit=200000, check_every=20, stream=None):
         x = torch.empty(self.ndof, dtype=torch.float64, device=b.device)
         it, rel = C.c_int32(0), C.c_double(0.0)
