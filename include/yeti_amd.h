@@ -25,6 +25,7 @@
 #ifndef YETI_AMD_H
 #define YETI_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -210,6 +211,47 @@ int32_t yeti_mf_pcg_f64(const yeti_mf *m, const double *b, const uint8_t *free_m
 int32_t yeti_mf_solve_ir(const yeti_mf *m, const double *b, const uint8_t *free_mask, double *x, double rtol,
                          double inner_rtol, int32_t max_outer, int32_t max_inner, int32_t check_every,
                          int32_t *outer_iters, int32_t *inner_iters, double *relres, void *stream);
+
+/* ---- set-up inside caller-owned device memory ------------------------------------------
+ * Same as yeti_mf_create / yeti_asm_create, but every device buffer of the handle is carved from
+ * [arena, arena + arena_bytes) and the set-up's copies and kernels run on `stream` (only that
+ * stream is synchronised, no hipMalloc / hipFree / device-wide synchronisation). With
+ * arena == NULL nothing touches the device: *used returns the bytes the set-up needs and *out
+ * stays NULL. The arena must outlive the handle; destroying the handle frees nothing in it.
+ * Used by the serving loop to admit requests against an HBM budget allocated once. */
+int32_t yeti_mf_create_in(const yeti_system_inputs *in, int32_t precisions, void *arena, size_t arena_bytes,
+                          void *stream, yeti_mf **out, size_t *used);
+int32_t yeti_asm_create_in(const yeti_system_inputs *in, int32_t layout, void *arena, size_t arena_bytes,
+                           void *stream, yeti_asm **out, size_t *used);
+
+/* ---- steppable, device-controlled solve (serving) ---------------------------------------
+ * Solves K[free][:, free] x = b[free] with the matrix-free operator m: PCG with 3x3 block-Jacobi,
+ * vectors in FP32 (inner_precision 32: FP64 iterative refinement, stopping at
+ * ||b - K x||_2 <= rtol ||b||_2 with the FP64 apply, inner solves to inner_rtol of the current
+ * residual) or FP64 (64). Control state lives on the device: yeti_ir_step enqueues `chunk` inner
+ * iterations and one refinement step, all predicated on device flags, so a step is a fixed launch
+ * sequence; with use_graph it is captured once into a hipGraph (on a non-null stream) and
+ * replayed. The iterates do not depend on how steps of different sessions interleave. All
+ * device memory is taken at creation (arena semantics as above; arena == NULL counts).
+ * No reference counterpart: the reference solves one system per call
+ * (iga_wq_mf/fsrc/mf_iga_solver.f90:642-958; scipy spsolve in its callers). */
+typedef struct yeti_ir yeti_ir;
+typedef struct yeti_ir_status {
+    int32_t active, outer_pending, restart, done, failed, inner_iters, inner_total, outer_iters;
+    double rz, pq, rr_inner, rz_new, bb, rr_outer, tol_inner, tol_outer;
+    double reserved[8];
+} yeti_ir_status;
+
+int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask_dev, int32_t inner_precision, double rtol,
+                       double inner_rtol, int32_t chunk, int32_t max_inner, int32_t max_outer, void *arena,
+                       size_t arena_bytes, void *stream, yeti_ir **out, size_t *used);
+/* x = 0 and the first inner solve set up for right-hand side b (device buffers kept by the session). */
+int32_t yeti_ir_begin(yeti_ir *s, const double *b_dev, double *x_dev, void *stream);
+int32_t yeti_ir_step(yeti_ir *s, int32_t use_graph, void *stream);
+/* Enqueue a copy of the status into host memory (pinned for an asynchronous copy); read it once
+ * the stream has reached this point. Relative residual = sqrt(rr_outer / bb). */
+int32_t yeti_ir_status_async(const yeti_ir *s, yeti_ir_status *host_status, void *stream);
+void    yeti_ir_destroy(yeti_ir *s);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,17 @@ SIGNATURES = {
     "yeti_mf_solve_ir": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                      C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                      C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p]),
+    "yeti_mf_create_in": (C.c_int32, [C.POINTER(YetiSystemInputs), C.c_int32, C.c_void_p, C.c_size_t, C.c_void_p,
+                                      C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]),
+    "yeti_asm_create_in": (C.c_int32, [C.POINTER(YetiSystemInputs), C.c_int32, C.c_void_p, C.c_size_t, C.c_void_p,
+                                       C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]),
+    "yeti_ir_create": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                   C.c_int32, C.c_void_p, C.c_size_t, C.c_void_p, C.POINTER(C.c_void_p),
+                                   C.POINTER(C.c_size_t)]),
+    "yeti_ir_begin": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_ir_step": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "yeti_ir_status_async": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_ir_destroy": (None, [C.c_void_p]),
     "yeti_pcg_csr": (C.c_int32, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                  C.POINTER(C.c_double), C.c_void_p]),

@@ -34,6 +34,11 @@ std::string &last_error() {
     return s;
 }
 
+AllocCtx &alloc_ctx() {
+    static thread_local AllocCtx c;
+    return c;
+}
+
 int set_error(int code, const char *fmt, ...) {
     char buf[1024];
     va_list ap;
@@ -1009,6 +1014,27 @@ int32_t yeti_asm_create(const yeti_system_inputs *in, int32_t layout, yeti_asm *
     if (!out) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null handle pointer");
     yeti::last_error().clear();
     return yeti::create(in, layout, out);
+}
+
+int32_t yeti_asm_create_in(const yeti_system_inputs *in, int32_t layout, void *arena, size_t arena_bytes,
+                           void *stream, yeti_asm **out, size_t *used) {
+    if (!out) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null handle pointer");
+    *out = nullptr;
+    yeti::last_error().clear();
+    yeti::Arena a;
+    a.base = (char *)arena;
+    a.size = arena ? arena_bytes : 0;
+    yeti::AllocScope scope(&a, (hipStream_t)stream);
+    yeti_asm *h = nullptr;
+    int rc = yeti::create(in, layout, &h);
+    if (used) *used = a.used;
+    if (rc) return rc;
+    if (!arena) {
+        delete h;   // counting only
+        return 0;
+    }
+    *out = h;
+    return 0;
 }
 
 int64_t yeti_asm_nb_data(const yeti_asm *h) { return h ? h->nb_data : -1; }

@@ -34,29 +34,77 @@ struct GaussRule {
     double w[MAX_NG];
 };
 
-// RAII device buffer (hipMalloc / hipFree) used by host-side set-up code.
+// Device memory of a set-up can come from a caller-owned arena instead of hipMalloc: while an
+// AllocScope is alive on a thread, DevBuf::alloc bump-allocates from its arena (256-byte
+// aligned) and uploads are hipMemcpyAsync on the scope's stream. An arena with a null base only
+// counts the bytes a set-up would take (no device memory is touched; set-up code skips its
+// kernel launches and copies while counting()).
+struct Arena {
+    char *base = nullptr;   // null: count only
+    size_t size = 0;
+    size_t used = 0;
+};
+
+struct AllocCtx {
+    Arena *arena = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+AllocCtx &alloc_ctx();   // per thread
+
+struct AllocScope {
+    AllocCtx saved;
+    AllocScope(Arena *a, hipStream_t st) : saved(alloc_ctx()) {
+        alloc_ctx().arena = a;
+        alloc_ctx().stream = st;
+    }
+    ~AllocScope() { alloc_ctx() = saved; }
+    AllocScope(const AllocScope &) = delete;
+    AllocScope &operator=(const AllocScope &) = delete;
+};
+
+inline bool counting() { return alloc_ctx().arena && !alloc_ctx().arena->base; }
+
+// Device buffer used by host-side set-up code: owns hipMalloc memory, or borrows arena memory.
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    bool owned = false;
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        owned = false;
     }
     hipError_t alloc(size_t count) {
         release();
         n = count;
         if (count == 0) return hipSuccess;
-        return hipMalloc((void **)&p, count * sizeof(T));
+        const size_t bytes = count * sizeof(T);
+        if (Arena *a = alloc_ctx().arena) {
+            const size_t off = (a->used + 255) & ~size_t(255);
+            if (a->base && (off > a->size || bytes > a->size - off)) return hipErrorOutOfMemory;
+            a->used = off + bytes;
+            // counting: a distinct non-null placeholder that is never dereferenced
+            p = a->base ? reinterpret_cast<T *>(a->base + off) : reinterpret_cast<T *>(uintptr_t(off + 256));
+            return hipSuccess;
+        }
+        owned = true;
+        return hipMalloc((void **)&p, bytes);
     }
     hipError_t upload(const T *h, size_t count) {
         hipError_t e = alloc(count);
-        if (e != hipSuccess || count == 0) return e;
+        if (e != hipSuccess || count == 0 || counting()) return e;
+        if (alloc_ctx().arena) {
+            // the source may be a temporary of the set-up: wait for this stream only
+            e = hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, alloc_ctx().stream);
+            return e == hipSuccess ? hipStreamSynchronize(alloc_ctx().stream) : e;
+        }
         return hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice);
     }
 };

@@ -195,7 +195,8 @@ struct MfBrick {
 template <typename T, int P, int NG, int MODE>
 __global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__restrict__ geo, const T *__restrict__ x,
                                                        const T *__restrict__ wcp, const int32_t *__restrict__ nbrick,
-                                                       T *__restrict__ partial) {
+                                                       T *__restrict__ partial, const int32_t *__restrict__ pred) {
+    if (pred && *pred == 0) return;   // predicated launch (device-side solver control)
     using C = MfCfg<P, NG>;
     using Bk = MfBrick<P, MODE>;
     constexpr int NP1 = C::NP1, NQ = C::NQ, NN = C::NN, BN = Bk::BN, BR = Bk::BR, NC = MODE == 1 ? 9 : 3;
@@ -433,7 +434,9 @@ __global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__re
 // y[global cp] (+)= sum over the bricks covering patch CP c (fixed order)
 template <typename T, int NC>
 __global__ void mf_gather_kernel(MfPatch mp, int32_t br, int32_t bn, int32_t nb0, int32_t nb1, int32_t nb2,
-                                 const T *__restrict__ partial, T *__restrict__ y, int accumulate) {
+                                 const T *__restrict__ partial, T *__restrict__ y, int accumulate,
+                                 const int32_t *__restrict__ pred) {
+    if (pred && *pred == 0) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t ncp = (int64_t)mp.ncp[0] * mp.ncp[1] * mp.ncp[2];
     if (t >= ncp) return;
@@ -503,29 +506,31 @@ struct yeti_mf {
 namespace yeti {
 
 template <typename T, int P, int NG, int MODE>
-static int launch_brick(const MfPatchDev &pd, const T *geo, const T *x, const T *w, T *part, hipStream_t st) {
+static int launch_brick(const MfPatchDev &pd, const T *geo, const T *x, const T *w, T *part, hipStream_t st,
+                        const int32_t *pred) {
     constexpr int BM = MODE == 1 ? 1 : 0;   // brick grid: apply-sized or diagonal-sized
     const int32_t *nbk = pd.nbrick[BM];
     int64_t nb = (int64_t)nbk[0] * nbk[1] * nbk[2];
     hipLaunchKernelGGL((mf_brick_kernel<T, P, NG, MODE>), dim3((unsigned)nb), dim3(256), 0, st, pd.mp, geo, x, w,
-                       pd.nbrick_dev.p + 3 * BM, part);
+                       pd.nbrick_dev.p + 3 * BM, part, pred);
     YETI_HIP(hipGetLastError());
     return 0;
 }
 
 template <typename T, int MODE>
-static int dispatch_brick(const MfPatchDev &pd, const T *geo, const T *x, const T *w, T *part, hipStream_t st) {
+static int dispatch_brick(const MfPatchDev &pd, const T *geo, const T *x, const T *w, T *part, hipStream_t st,
+                          const int32_t *pred) {
     switch (pd.p * 16 + pd.ng) {
-    case 1 * 16 + 2: return launch_brick<T, 1, 2, MODE>(pd, geo, x, w, part, st);
-    case 2 * 16 + 3: return launch_brick<T, 2, 3, MODE>(pd, geo, x, w, part, st);
-    case 3 * 16 + 4: return launch_brick<T, 3, 4, MODE>(pd, geo, x, w, part, st);
-    case 4 * 16 + 5: return launch_brick<T, 4, 5, MODE>(pd, geo, x, w, part, st);
+    case 1 * 16 + 2: return launch_brick<T, 1, 2, MODE>(pd, geo, x, w, part, st, pred);
+    case 2 * 16 + 3: return launch_brick<T, 2, 3, MODE>(pd, geo, x, w, part, st, pred);
+    case 3 * 16 + 4: return launch_brick<T, 3, 4, MODE>(pd, geo, x, w, part, st, pred);
+    case 4 * 16 + 5: return launch_brick<T, 4, 5, MODE>(pd, geo, x, w, part, st, pred);
     default: return set_error(YETI_ERR_UNSUPPORTED_DEGREE, "matrix-free: no kernel for p=%d ng=%d", pd.p, pd.ng);
     }
 }
 
 template <typename T, int MODE>
-static int mf_run(const yeti_mf *m, const T *x, T *y, hipStream_t st) {
+static int mf_run(const yeti_mf *m, const T *x, T *y, hipStream_t st, const int32_t *pred = nullptr) {
     constexpr int NC = MODE == 1 ? 9 : 3;
     for (size_t ip = 0; ip < m->patches.size(); ++ip) {
         const MfPatchDev &pd = *m->patches[ip];
@@ -542,16 +547,24 @@ static int mf_run(const yeti_mf *m, const T *x, T *y, hipStream_t st) {
             part = (T *)pd.part32.p;
         }
         if (!geo || !part) return set_error(YETI_ERR_INVALID_ARGUMENT, "matrix-free: precision not prepared");
-        int rc = dispatch_brick<T, MODE>(pd, geo, x, w, part, st);
+        int rc = dispatch_brick<T, MODE>(pd, geo, x, w, part, st, pred);
         if (rc) return rc;
         const int br = MODE == 1 ? pd.p + 1 : 2 * (pd.p + 1), bn = br + pd.p;
         const int32_t *nbk = pd.nbrick[MODE == 1 ? 1 : 0];
         int64_t ncp = (int64_t)pd.mp.ncp[0] * pd.mp.ncp[1] * pd.mp.ncp[2];
         hipLaunchKernelGGL((mf_gather_kernel<T, NC>), dim3((unsigned)((ncp + 255) / 256)), dim3(256), 0, st, pd.mp, br,
-                           bn, nbk[0], nbk[1], nbk[2], part, y, ip > 0 ? 1 : 0);
+                           bn, nbk[0], nbk[1], nbk[2], part, y, ip > 0 ? 1 : 0, pred);
         YETI_HIP(hipGetLastError());
     }
     return 0;
+}
+
+// Predicated applies for the device-controlled solvers (nothing runs while *pred == 0).
+int mf_apply_pred(const yeti_mf *m, const double *x, double *y, hipStream_t st, const int32_t *pred) {
+    return mf_run<double, 0>(m, x, y, st, pred);
+}
+int mf_apply_pred(const yeti_mf *m, const float *x, float *y, hipStream_t st, const int32_t *pred) {
+    return mf_run<float, 0>(m, x, y, st, pred);
 }
 
 // Quadrature-point geometry of one patch from the handle's current coordinates.
@@ -576,7 +589,7 @@ using namespace yeti;
 
 extern "C" {
 
-int32_t yeti_mf_create(const yeti_system_inputs *in, int32_t precisions, yeti_mf **out) {
+static int32_t mf_create(const yeti_system_inputs *in, int32_t precisions, hipStream_t st, yeti_mf **out) {
     if (!in || !out) return set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     if (in->MCRD != 3) return set_error(YETI_ERR_INVALID_ARGUMENT, "MCRD must be 3");
@@ -708,12 +721,40 @@ int32_t yeti_mf_create(const yeti_system_inputs *in, int32_t precisions, yeti_mf
         if (er == hipSuccess) er = pd->part64.alloc(part_n);
         if (er == hipSuccess && (precisions & 2)) er = pd->part32.alloc(nb0 * bn0 * bn0 * bn0 * 3);
         if (er != hipSuccess) return fail(set_error(YETI_ERR_DEVICE, "matrix-free: allocation failed (out of memory?)"));
-        if (mf_geometry(m, *pd, 0) != 0 || hipDeviceSynchronize() != hipSuccess)
-            return fail(set_error(YETI_ERR_DEVICE, "matrix-free: geometry kernel failed"));
+        if (!counting()) {
+            const bool arena = alloc_ctx().arena != nullptr;
+            if (mf_geometry(m, *pd, st) != 0 ||
+                (arena ? hipStreamSynchronize(st) : hipDeviceSynchronize()) != hipSuccess)
+                return fail(set_error(YETI_ERR_DEVICE, "matrix-free: geometry kernel failed"));
+        }
         // the FP64 geometry is kept: the diagonal blocks and the FP64 apply read it
         woff += (int64_t)nn * ne;
         ieoff += (int64_t)nn * ne;
         eloff += ne;
+    }
+    *out = m;
+    return 0;
+}
+
+int32_t yeti_mf_create(const yeti_system_inputs *in, int32_t precisions, yeti_mf **out) {
+    return mf_create(in, precisions, nullptr, out);
+}
+
+int32_t yeti_mf_create_in(const yeti_system_inputs *in, int32_t precisions, void *arena, size_t arena_bytes,
+                          void *stream, yeti_mf **out, size_t *used) {
+    if (!out) return set_error(YETI_ERR_INVALID_ARGUMENT, "null handle pointer");
+    *out = nullptr;
+    Arena a;
+    a.base = (char *)arena;
+    a.size = arena ? arena_bytes : 0;
+    AllocScope scope(&a, (hipStream_t)stream);
+    yeti_mf *m = nullptr;
+    int rc = mf_create(in, precisions, (hipStream_t)stream, &m);
+    if (used) *used = a.used;
+    if (rc) return rc;
+    if (!arena) {
+        delete m;   // counting only
+        return 0;
     }
     *out = m;
     return 0;

@@ -8,7 +8,9 @@
 // alpha = rsold / (p . Ap), with diagonal-type preconditioners). Stopping rule here:
 // ||r||_2 <= rtol ||b||_2 over the free dofs. Reductions use a fixed grid and a fixed-order final
 // sum, so every solve is bit-reproducible.
+#include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <vector>
 
 #include "common.hpp"
@@ -358,5 +360,420 @@ int32_t yeti_mf_solve_ir(const yeti_mf *m, const double *b, const uint8_t *free_
                          inner_total, rel);
     return 0;
 }
+
+}  // extern "C"
+
+// ==========================================================================================
+// Steppable, device-controlled solver for serving (yeti_ir_*).
+//
+// One session solves K[free][:, free] x = b[free] for one handle. Its control state lives in
+// device memory: every kernel of a step is predicated on device flags, so a step is a fixed
+// launch sequence that can be captured once in a hipGraph and replayed until the session
+// reports done, with the host only reading a small status block. A step is
+//   `chunk` inner PCG iterations (FP32 or FP64 vectors, FP64 dots; an iteration runs only while
+//   the inner solve is active, and the iteration that meets the inner tolerance deactivates
+//   it), then one refinement step (predicated on the inner solve having just finished):
+//   x += d, r = mask (b - K x) with the FP64 apply, and either done (||r|| <= rtol ||b||) or a
+//   restart of the inner solve on r with tolerance inner_rtol ||r||.
+// The iterates depend only on the session's own launch sequence, never on how steps are
+// interleaved with other sessions: a request gives bit-identical results alone or in a mix.
+// All device memory is allocated at creation (optionally inside a caller-owned arena).
+namespace yeti {
+
+int mf_apply_pred(const yeti_mf *m, const double *x, double *y, hipStream_t st, const int32_t *pred);
+int mf_apply_pred(const yeti_mf *m, const float *x, float *y, hipStream_t st, const int32_t *pred);
+
+// sc[] slots
+enum : int { S_RZ = 0, S_PQ, S_RR, S_RZN, S_BB, S_RR64, S_TOLIN, S_TOLOUT, S_NSC = 16 };
+// flags[] slots
+enum : int { F_ACTIVE = 0, F_OUTER_NOW, F_RESTART, F_DONE, F_FAILED, F_INNER_IT, F_INNER_TOTAL, F_OUTER, F_NFLAGS = 8 };
+
+__global__ void pred_sum_partials_kernel(const double *part, int n, double *out, const int32_t *pred) {
+    if (pred && *pred == 0) return;
+    __shared__ double sh[4];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
+    double t = block_sum256(v, sh);
+    if (threadIdx.x == 0) *out = t;
+}
+
+// r_in = (T) r64, d = 0 (restart of the inner solve), predicated on F_RESTART
+template <typename T>
+__global__ void __launch_bounds__(256) ir_restart_copy_kernel(int64_t n, const double *r64, T *r, T *d,
+                                                              const int32_t *flags) {
+    if (flags[F_RESTART] == 0) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        r[i] = (T)r64[i];
+        d[i] = T(0);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) pred_precond_kernel(int32_t ncp, const T *__restrict__ inv, const T *__restrict__ r,
+                                                           T *__restrict__ z, double *part, const int32_t *pred) {
+    if (*pred == 0) return;
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (int64_t cp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; cp < ncp; cp += (int64_t)gridDim.x * blockDim.x) {
+        const T *M = inv + 9 * cp;
+        T r0 = r[3 * cp], r1 = r[3 * cp + 1], r2 = r[3 * cp + 2];
+        T z0 = M[0] * r0 + M[1] * r1 + M[2] * r2;
+        T z1 = M[3] * r0 + M[4] * r1 + M[5] * r2;
+        T z2 = M[6] * r0 + M[7] * r1 + M[8] * r2;
+        z[3 * cp] = z0;
+        z[3 * cp + 1] = z1;
+        z[3 * cp + 2] = z2;
+        acc += (double)r0 * (double)z0 + (double)r1 * (double)z1 + (double)r2 * (double)z2;
+    }
+    double t = block_sum256(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// p = z (first direction of a restart), predicated on F_RESTART; then the inner solve is active
+template <typename T>
+__global__ void __launch_bounds__(256) ir_restart_p_kernel(int64_t n, const T *z, T *p, const int32_t *flags) {
+    if (flags[F_RESTART] == 0) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = z[i];
+}
+
+__global__ void ir_restart_done_kernel(int32_t *flags) {
+    if (flags[F_RESTART] == 0) return;
+    flags[F_RESTART] = 0;
+    flags[F_ACTIVE] = 1;
+    flags[F_INNER_IT] = 0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) pred_mask_dot_kernel(int64_t n, const uint8_t *mask, const T *__restrict__ p,
+                                                            T *__restrict__ q, double *part, const int32_t *pred) {
+    if (*pred == 0) return;
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        T qi = mask[i] ? q[i] : T(0);
+        q[i] = qi;
+        acc += (double)p[i] * (double)qi;
+    }
+    double t = block_sum256(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) pred_xr_kernel(int64_t n, const double *sc, T *__restrict__ x, T *__restrict__ r,
+                                                      const T *__restrict__ p, const T *__restrict__ q, double *part,
+                                                      const int32_t *pred) {
+    if (*pred == 0) return;
+    __shared__ double sh[4];
+    const double alpha = sc[S_RZ] / sc[S_PQ];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] = (T)((double)x[i] + alpha * (double)p[i]);
+        T ri = (T)((double)r[i] - alpha * (double)q[i]);
+        r[i] = ri;
+        acc += (double)ri * (double)ri;
+    }
+    double t = block_sum256(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) pred_p_kernel(int64_t n, const double *sc, T *__restrict__ p, const T *__restrict__ z,
+                                                     const int32_t *pred) {
+    if (*pred == 0) return;
+    const double beta = sc[S_RZN] / sc[S_RZ];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = (T)((double)z[i] + beta * (double)p[i]);
+}
+
+// end of an inner iteration: roll rz, count, and stop the inner solve at its tolerance (or cap)
+__global__ void ir_inner_check_kernel(double *sc, int32_t *flags, int32_t max_inner) {
+    if (flags[F_ACTIVE] == 0) return;
+    sc[S_RZ] = sc[S_RZN];
+    flags[F_INNER_IT] += 1;
+    flags[F_INNER_TOTAL] += 1;
+    if (!(sc[S_RR] > sc[S_TOLIN]) || flags[F_INNER_IT] >= max_inner) {
+        flags[F_ACTIVE] = 0;
+        flags[F_OUTER_NOW] = 1;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) ir_x_update_kernel(int64_t n, double *x, const T *d, const int32_t *flags) {
+    if (flags[F_OUTER_NOW] == 0) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] += (double)d[i];
+}
+
+__global__ void __launch_bounds__(256) pred_residual_kernel(int64_t n, const double *b, const double *y, const uint8_t *mask,
+                                                            double *r, double *part, const int32_t *pred) {
+    if (*pred == 0) return;
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double v = mask[i] ? b[i] - y[i] : 0.0;
+        r[i] = v;
+        acc += v * v;
+    }
+    double t = block_sum256(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void ir_outer_check_kernel(double *sc, int32_t *flags, double inner_rtol2, int32_t max_outer) {
+    if (flags[F_OUTER_NOW] == 0) return;
+    flags[F_OUTER_NOW] = 0;
+    flags[F_OUTER] += 1;
+    if (!(sc[S_RR64] > sc[S_TOLOUT])) {
+        flags[F_DONE] = 1;
+    } else if (flags[F_OUTER] >= max_outer) {
+        flags[F_DONE] = 1;
+        flags[F_FAILED] = 1;
+    } else {
+        sc[S_TOLIN] = inner_rtol2 * sc[S_RR64];
+        flags[F_RESTART] = 1;
+    }
+}
+
+// start of a solve: x = 0, r64 = mask b, thresholds; restart requested unless b = 0
+__global__ void __launch_bounds__(256) ir_begin_copy_kernel(int64_t n, const double *b, const uint8_t *mask, double *x,
+                                                            double *r, double *part) {
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double v = mask[i] ? b[i] : 0.0;
+        r[i] = v;
+        x[i] = 0.0;
+        acc += v * v;
+    }
+    double t = block_sum256(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void ir_begin_flags_kernel(double *sc, int32_t *flags, double rtol2, double inner_rtol2) {
+    const double bb = sc[S_BB];
+    sc[S_RR64] = bb;
+    sc[S_TOLOUT] = rtol2 * bb;
+    sc[S_TOLIN] = inner_rtol2 * bb;
+    for (int i = 0; i < F_NFLAGS; ++i) flags[i] = 0;
+    if (bb > 0.0) flags[F_RESTART] = 1;
+    else flags[F_DONE] = 1;
+}
+
+}  // namespace yeti
+
+struct yeti_ir {
+    const yeti_mf *m = nullptr;
+    const uint8_t *mask = nullptr;
+    int64_t n = 0;
+    int32_t ncp = 0;
+    int inner64 = 0;          // 1: FP64 inner vectors / apply (plain FP64 PCG), 0: FP32
+    double rtol = 0, inner_rtol = 0;
+    int32_t chunk = 8, max_inner = 0, max_outer = 0;
+    unsigned nb = 1, nbc = 1;
+    yeti::DevBuf<double> blk, inv64, r64, y64, part, sc, status;
+    yeti::DevBuf<float> inv32;
+    yeti::DevBuf<double> vr64, vd64, vz64, vp64, vq64;   // inner vectors, FP64
+    yeti::DevBuf<float> vr32, vd32, vz32, vp32, vq32;    // inner vectors, FP32
+    yeti::DevBuf<int32_t> flags;
+    const double *b = nullptr;
+    double *x = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    hipStream_t gstream = nullptr;
+    ~yeti_ir() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+namespace yeti {
+
+template <typename T>
+static int ir_restart(yeti_ir *s, T *r, T *d, T *z, T *p, const T *inv, hipStream_t st) {
+    hipLaunchKernelGGL(ir_restart_copy_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->r64.p, r, d, s->flags.p);
+    hipLaunchKernelGGL(pred_precond_kernel<T>, dim3(s->nbc), dim3(256), 0, st, s->ncp, inv, r, z, s->part.p,
+                       s->flags.p + F_RESTART);
+    hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nbc, s->sc.p + S_RZ,
+                       s->flags.p + F_RESTART);
+    hipLaunchKernelGGL(ir_restart_p_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, z, p, s->flags.p);
+    hipLaunchKernelGGL(ir_restart_done_kernel, dim3(1), dim3(1), 0, st, s->flags.p);
+    YETI_HIP(hipGetLastError());
+    return 0;
+}
+
+// the launch sequence of one step (captured into the session's graph)
+template <typename T>
+static int ir_step_launches(yeti_ir *s, T *r, T *d, T *z, T *p, T *q, const T *inv, hipStream_t st) {
+    const int32_t *act = s->flags.p + F_ACTIVE;
+    for (int k = 0; k < s->chunk; ++k) {
+        int rc = mf_apply_pred(s->m, p, q, st, act);
+        if (rc) return rc;
+        hipLaunchKernelGGL(pred_mask_dot_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->mask, p, q, s->part.p, act);
+        hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_PQ, act);
+        hipLaunchKernelGGL(pred_xr_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->sc.p, d, r, p, q, s->part.p, act);
+        hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_RR, act);
+        hipLaunchKernelGGL(pred_precond_kernel<T>, dim3(s->nbc), dim3(256), 0, st, s->ncp, inv, r, z, s->part.p, act);
+        hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nbc, s->sc.p + S_RZN, act);
+        hipLaunchKernelGGL(pred_p_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->sc.p, p, z, act);
+        hipLaunchKernelGGL(ir_inner_check_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->max_inner);
+        YETI_HIP(hipGetLastError());
+    }
+    // refinement step, predicated on the inner solve having just finished
+    const int32_t *now = s->flags.p + F_OUTER_NOW;
+    hipLaunchKernelGGL(ir_x_update_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->x, (const T *)d, s->flags.p);
+    int rc = mf_apply_pred(s->m, (const double *)s->x, s->y64.p, st, now);
+    if (rc) return rc;
+    hipLaunchKernelGGL(pred_residual_kernel, dim3(s->nb), dim3(256), 0, st, s->n, s->b, s->y64.p, s->mask, s->r64.p,
+                       s->part.p, now);
+    hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_RR64, now);
+    hipLaunchKernelGGL(ir_outer_check_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->inner_rtol * s->inner_rtol,
+                       s->max_outer);
+    YETI_HIP(hipGetLastError());
+    rc = ir_restart<T>(s, r, d, z, p, inv, st);
+    if (rc) return rc;
+    return 0;
+}
+
+static int ir_step_any(yeti_ir *s, hipStream_t st) {
+    if (s->inner64)
+        return ir_step_launches<double>(s, s->vr64.p, s->vd64.p, s->vz64.p, s->vp64.p, s->vq64.p, s->inv64.p, st);
+    return ir_step_launches<float>(s, s->vr32.p, s->vd32.p, s->vz32.p, s->vp32.p, s->vq32.p, s->inv32.p, st);
+}
+
+static int ir_restart_any(yeti_ir *s, hipStream_t st) {
+    if (s->inner64) return ir_restart<double>(s, s->vr64.p, s->vd64.p, s->vz64.p, s->vp64.p, s->inv64.p, st);
+    return ir_restart<float>(s, s->vr32.p, s->vd32.p, s->vz32.p, s->vp32.p, s->inv32.p, st);
+}
+
+}  // namespace yeti
+
+extern "C" {
+
+int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask, int32_t inner_precision, double rtol,
+                       double inner_rtol, int32_t chunk, int32_t max_inner, int32_t max_outer, void *arena,
+                       size_t arena_bytes, void *stream, yeti_ir **out, size_t *used) {
+    if (!m || !out) return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: null argument");
+    *out = nullptr;
+    if (inner_precision != 32 && inner_precision != 64)
+        return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: inner precision must be 32 or 64");
+    if (!(rtol > 0.0) || !(inner_rtol > 0.0) || inner_rtol >= 1.0 || chunk < 1 || max_inner < 1 || max_outer < 1)
+        return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: bad tolerance or iteration limits");
+    if (arena && !free_mask) return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: null free mask");
+    hipStream_t st = (hipStream_t)stream;
+    Arena a;
+    a.base = (char *)arena;
+    a.size = arena ? arena_bytes : 0;
+    AllocScope scope(&a, st);
+    auto *s = new yeti_ir();
+    auto fail = [&](int code) {
+        delete s;
+        return code;
+    };
+    s->m = m;
+    s->mask = free_mask;
+    s->ncp = yeti_mf_nb_cp(m);
+    s->n = 3 * (int64_t)s->ncp;
+    s->inner64 = inner_precision == 64;
+    s->rtol = rtol;
+    s->inner_rtol = inner_rtol;
+    s->chunk = chunk;
+    s->max_inner = max_inner;
+    s->max_outer = max_outer;
+    s->nb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(NRED, (s->n + 255) / 256));
+    s->nbc = (unsigned)std::max<int64_t>(1, std::min<int64_t>(NRED, (s->ncp + 255) / 256));
+    const size_t n = (size_t)s->n;
+    hipError_t e = s->blk.alloc(9 * (size_t)s->ncp);
+    if (e == hipSuccess) e = s->r64.alloc(n);
+    if (e == hipSuccess) e = s->y64.alloc(n);
+    if (e == hipSuccess) e = s->part.alloc(NRED);
+    if (e == hipSuccess) e = s->sc.alloc(S_NSC);
+    if (e == hipSuccess) e = s->flags.alloc(F_NFLAGS);
+    if (s->inner64) {
+        if (e == hipSuccess) e = s->inv64.alloc(9 * (size_t)s->ncp);
+        for (auto *v : {&s->vr64, &s->vd64, &s->vz64, &s->vp64, &s->vq64})
+            if (e == hipSuccess) e = v->alloc(n);
+    } else {
+        if (e == hipSuccess) e = s->inv32.alloc(9 * (size_t)s->ncp);
+        for (auto *v : {&s->vr32, &s->vd32, &s->vz32, &s->vp32, &s->vq32})
+            if (e == hipSuccess) e = v->alloc(n);
+    }
+    if (e != hipSuccess) return fail(set_error(YETI_ERR_BUDGET, "ir: device memory: %s", hipGetErrorString(e)));
+    if (used) *used = a.used;
+    if (!arena) {
+        delete s;   // counting only
+        return 0;
+    }
+    // block-Jacobi preconditioner of the free dofs
+    int rc = yeti_mf_diag_blocks_f64(m, s->blk.p, st);
+    if (rc) return fail(rc);
+    hipLaunchKernelGGL(block_inverse_kernel, dim3((s->ncp + 255) / 256), dim3(256), 0, st, s->ncp, s->blk.p, free_mask,
+                       s->inner64 ? s->inv64.p : nullptr, s->inner64 ? nullptr : s->inv32.p);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+        return fail(set_error(YETI_ERR_DEVICE, "ir: preconditioner set-up failed"));
+    *out = s;
+    return 0;
+}
+
+int32_t yeti_ir_begin(yeti_ir *s, const double *b, double *x, void *stream) {
+    if (!s || !b || !x) return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    if ((s->b != b || s->x != x) && s->gexec) {
+        // the captured step refers to b and x: capture again for new buffers
+        (void)hipGraphExecDestroy(s->gexec);
+        (void)hipGraphDestroy(s->graph);
+        s->gexec = nullptr;
+        s->graph = nullptr;
+    }
+    s->b = b;
+    s->x = x;
+    hipLaunchKernelGGL(ir_begin_copy_kernel, dim3(s->nb), dim3(256), 0, st, s->n, b, s->mask, x, s->r64.p, s->part.p);
+    hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_BB,
+                       (const int32_t *)nullptr);
+    hipLaunchKernelGGL(ir_begin_flags_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->rtol * s->rtol,
+                       s->inner_rtol * s->inner_rtol);
+    YETI_HIP(hipGetLastError());
+    return ir_restart_any(s, st);
+}
+
+int32_t yeti_ir_step(yeti_ir *s, int32_t use_graph, void *stream) {
+    if (!s || !s->b) return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: step before begin");
+    hipStream_t st = (hipStream_t)stream;
+    if (!use_graph) return ir_step_any(s, st);
+    if (!st) return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: graph steps need a non-null stream");
+    if (!s->gexec) {
+        YETI_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = ir_step_any(s, st);
+        hipGraph_t g = nullptr;
+        hipError_t ec = hipStreamEndCapture(st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess) return set_error(YETI_ERR_DEVICE, "ir: capture failed: %s", hipGetErrorString(ec));
+        hipGraphExec_t ex = nullptr;
+        hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return set_error(YETI_ERR_DEVICE, "ir: graph instantiation failed: %s", hipGetErrorString(ei));
+        }
+        s->graph = g;
+        s->gexec = ex;
+    }
+    YETI_HIP(hipGraphLaunch(s->gexec, st));
+    return 0;
+}
+
+int32_t yeti_ir_status_async(const yeti_ir *s, yeti_ir_status *host_status, void *stream) {
+    if (!s || !host_status) return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    // flags then scalars, in the layout of yeti_ir_status (the caller waits on the stream)
+    static_assert(offsetof(yeti_ir_status, rz) == sizeof(int32_t) * F_NFLAGS, "status layout");
+    static_assert(sizeof(yeti_ir_status) == sizeof(int32_t) * F_NFLAGS + sizeof(double) * S_NSC, "status layout");
+    YETI_HIP(hipMemcpyAsync(&host_status->active, s->flags.p, sizeof(int32_t) * F_NFLAGS, hipMemcpyDeviceToHost, st));
+    YETI_HIP(hipMemcpyAsync(&host_status->rz, s->sc.p, sizeof(double) * S_NSC, hipMemcpyDeviceToHost, st));
+    return 0;
+}
+
+void yeti_ir_destroy(yeti_ir *s) { delete s; }
 
 }  // extern "C"

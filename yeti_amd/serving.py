@@ -1,0 +1,382 @@
+"""Device-resident request loop: assembly + solve requests for NURBS solid patches of different
+sizes and degrees, admitted against an HBM budget, solved with graph-captured, device-controlled
+solve steps, reporting per-request latency next to throughput.
+
+No reference counterpart exists: YETI builds and solves one model per call
+(build_system_elemStorage.f90:24-315, then scipy or iga_mf_cg_3d in its callers). What a request
+computes is the reference's pipeline: F from the load description
+(build_system_elemStorage.f90:143-152,295-303) and u from K[free][:, free] u = F[free]
+(benchs/volume_force/test_volume_force_3d.py:67-79), with K applied matrix-free.
+
+Memory: one device arena of `budget_bytes` is allocated when the server starts. A request is
+admitted (first come, first served) when its footprint -- measured exactly by the set-up
+routines' counting mode, plus the solve session and the request's vectors -- fits in a free
+range of the arena; its handles are created inside that range (no hipMalloc/hipFree while
+serving) and the range is returned when the request completes.
+
+Solves: ``precision="mixed"`` is FP64 iterative refinement over FP32 PCG (FP32 apply and
+preconditioner, FP64 dots and residuals); ``precision="fp64"`` runs the same PCG in FP64.
+Each step (``chunk`` inner iterations and one refinement step) is one hipGraph launch per
+request; steps of different requests run on a pool of streams and never change each other's
+iterates, so a request's solution is bit-identical alone or in a mix.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import threading
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._abi import YETI_LAYOUT_PACKED, pack_system_inputs
+from .geometry import make_box_patch
+
+ALIGN = 256
+NRED = 1024
+STATUS_BYTES = 8 * 4 + 16 * 8     # yeti_ir_status
+
+
+def _align(n):
+    return (int(n) + ALIGN - 1) // ALIGN * ALIGN
+
+
+# ------------------------------------------------------------------------------------------
+class ArenaAllocator:
+    """First-fit allocator of byte ranges of [0, size), ALIGN-aligned, with coalescing frees.
+    Host bookkeeping only; the device memory is one buffer allocated up front."""
+
+    def __init__(self, size):
+        self.size = int(size)
+        self.free = [(0, self.size)]      # sorted, disjoint, non-adjacent (offset, length)
+        self.used = 0
+        self.peak = 0
+
+    def alloc(self, nbytes):
+        n = _align(max(int(nbytes), 1))
+        for i, (off, ln) in enumerate(self.free):
+            if ln >= n:
+                if ln == n:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (off + n, ln - n)
+                self.used += n
+                self.peak = max(self.peak, self.used)
+                return off
+        return None
+
+    def release(self, off, nbytes):
+        n = _align(max(int(nbytes), 1))
+        if off < 0 or off + n > self.size:
+            raise ValueError("range outside the arena")
+        i = 0
+        while i < len(self.free) and self.free[i][0] < off:
+            i += 1
+        if i > 0 and self.free[i - 1][0] + self.free[i - 1][1] > off:
+            raise ValueError("double free")
+        if i < len(self.free) and off + n > self.free[i][0]:
+            raise ValueError("double free")
+        self.free.insert(i, (off, n))
+        # coalesce with the right, then the left neighbour
+        if i + 1 < len(self.free) and off + n == self.free[i + 1][0]:
+            self.free[i] = (off, n + self.free[i + 1][1])
+            self.free.pop(i + 1)
+        if i > 0 and self.free[i - 1][0] + self.free[i - 1][1] == off:
+            self.free[i - 1] = (self.free[i - 1][0], self.free[i - 1][1] + self.free[i][1])
+            self.free.pop(i)
+        self.used -= n
+
+
+def ir_session_bytes(nb_cp, inner_precision):
+    """Device bytes of yeti_ir_create (mirrors its allocations, each ALIGN-aligned)."""
+    n = 3 * nb_cp
+    w = 8 if inner_precision == 64 else 4
+    sizes = [9 * nb_cp * 8, n * 8, n * 8, NRED * 8, 16 * 8, 8 * 4, 9 * nb_cp * w] + [n * w] * 5
+    return sum(_align(s) for s in sizes)
+
+
+# ------------------------------------------------------------------------------------------
+@dataclass
+class Request:
+    rid: int
+    arrival: float            # seconds after the start of the run
+    nel: tuple
+    p: int
+    E: float
+    nu: float
+    traction: float
+    seed: int
+    args: dict = field(default=None, repr=False)
+
+    def model(self):
+        if self.args is None:
+            self.args = make_box_patch(nel=self.nel, p=self.p, E=self.E, nu=self.nu, traction=self.traction,
+                                       seed=self.seed).args
+        return self.args
+
+    @property
+    def nb_cp(self):
+        return (self.nel[0] + self.p) * (self.nel[1] + self.p) * (self.nel[2] + self.p)
+
+
+def request_stream(n=2000, rate=200.0, seed=20261202, nel_range=(4, 48), degrees=(2, 3, 4)):
+    """Poisson arrivals at `rate` per second; elements per direction round(exp(U[ln a, ln b]));
+    degree uniform over `degrees`; E ~ U[1e3, 1e5], nu ~ U[0.2, 0.4], traction ~ U[0.1, 10]
+    (downward, on the face x = 10 of a 10 x 1 x 1 box clamped at x = 0; interior control
+    points jittered by 5 % of an element)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    lo, hi = math.log(nel_range[0]), math.log(nel_range[1])
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        nel = tuple(int(round(math.exp(rng.uniform(lo, hi)))) for _ in range(3))
+        p = int(rng.choice(degrees))
+        E = float(rng.uniform(1e3, 1e5))
+        nu = float(rng.uniform(0.2, 0.4))
+        tr = -float(rng.uniform(0.1, 10.0))
+        out.append(Request(i, t, nel, p, E, nu, tr, int(rng.integers(1 << 31))))
+    return out
+
+
+def latency_summary(lat):
+    a = np.sort(np.asarray(lat, dtype=np.float64))
+    if a.size == 0:
+        return {"p50": None, "p99": None, "mean": None, "max": None}
+    def pct(q):
+        # nearest-rank percentile
+        k = max(0, min(a.size - 1, int(math.ceil(q / 100.0 * a.size)) - 1))
+        return float(a[k])
+    return {"p50": pct(50), "p99": pct(99), "mean": float(a.mean()), "max": float(a[-1])}
+
+
+# ------------------------------------------------------------------------------------------
+class _Active:
+    __slots__ = ("req", "off", "nbytes", "asm", "mf", "ir", "F", "x", "mask", "stream", "event", "status",
+                 "t_admit", "steps", "keep")
+
+
+class SolveServer:
+    """Runs a list of requests against one arena of `budget_bytes`."""
+
+    def __init__(self, budget_bytes, precision="mixed", nstreams=8, chunk=8, rtol=1e-10, inner_rtol=1e-4,
+                 max_inner=20000, max_outer=60, use_graph=True, prep_threads=2):
+        import torch
+        _lib.require_device()
+        if precision not in ("mixed", "fp64"):
+            raise ValueError("precision must be 'mixed' or 'fp64'")
+        self.torch = torch
+        self.inner = 32 if precision == "mixed" else 64
+        self.precision = precision
+        self.chunk, self.rtol, self.inner_rtol = int(chunk), float(rtol), float(inner_rtol)
+        self.max_inner, self.max_outer = int(max_inner), int(max_outer)
+        self.use_graph = bool(use_graph)
+        self.budget = int(budget_bytes)
+        self.arena_buf = torch.empty(self.budget, dtype=torch.uint8, device="cuda")
+        self.base = self.arena_buf.data_ptr()
+        self.alloc = ArenaAllocator(self.budget)
+        self.streams = [torch.cuda.Stream() for _ in range(int(nstreams))]
+        self.prep_threads = int(prep_threads)
+
+    # -- footprint ---------------------------------------------------------------------------
+    def footprint(self, req, inputs):
+        L = _lib.lib()
+        used_a, used_m = C.c_size_t(0), C.c_size_t(0)
+        h = C.c_void_p()
+        _lib.check(L.yeti_asm_create_in(C.byref(inputs), YETI_LAYOUT_PACKED, None, 0, None, C.byref(h),
+                                        C.byref(used_a)))
+        _lib.check(L.yeti_mf_create_in(C.byref(inputs), 1 | 2 if self.inner == 32 else 1, None, 0, None, C.byref(h),
+                                       C.byref(used_m)))
+        n = 3 * req.nb_cp
+        parts = [_align(used_a.value) + ALIGN, _align(used_m.value) + ALIGN, ir_session_bytes(req.nb_cp, self.inner),
+                 _align(8 * n), _align(8 * n), _align(n)]
+        return parts
+
+    # -- admission -----------------------------------------------------------------------
+    def _admit(self, req, stream, keep):
+        torch = self.torch
+        L = _lib.lib()
+        args = req.model()
+        inputs, hold = pack_system_inputs(args)
+        parts = self.footprint(req, inputs)
+        total = sum(parts)
+        off = self.alloc.alloc(total)
+        if off is None:
+            return None
+        a = _Active()
+        a.req, a.off, a.nbytes, a.stream, a.keep = req, off, total, stream, keep
+        a.t_admit = time.perf_counter()
+        a.steps = 0
+        sh = C.c_void_p(stream.cuda_stream)
+        o = off
+        ptr = lambda k: C.c_void_p(self.base + k)   # noqa: E731
+        used = C.c_size_t(0)
+        try:
+            h = C.c_void_p()
+            _lib.check(L.yeti_asm_create_in(C.byref(inputs), YETI_LAYOUT_PACKED, ptr(o), parts[0], sh, C.byref(h),
+                                            C.byref(used)))
+            a.asm = h
+            o += parts[0]
+            h = C.c_void_p()
+            _lib.check(L.yeti_mf_create_in(C.byref(inputs), 1 | 2 if self.inner == 32 else 1, ptr(o), parts[1], sh,
+                                           C.byref(h), C.byref(used)))
+            a.mf = h
+            o += parts[1]
+            o_ir = o
+            o += parts[2]
+            n = 3 * req.nb_cp
+            a.F = self.arena_buf[o:o + 8 * n].view(torch.float64)
+            o += parts[3]
+            a.x = self.arena_buf[o:o + 8 * n].view(torch.float64)
+            o += parts[4]
+            a.mask = self.arena_buf[o:o + n]
+            idf = np.asarray(args["ind_dof_free"]).astype(np.int64)
+            m = np.zeros(n, np.uint8)
+            m[idf[: int(args["nb_dof_free"])] - 1] = 1
+            with torch.cuda.stream(stream):
+                a.mask.copy_(torch.from_numpy(m), non_blocking=False)
+            _lib.check(L.yeti_asm_run_rhs(a.asm, C.c_void_p(a.F.data_ptr()), sh))
+            h = C.c_void_p()
+            _lib.check(L.yeti_ir_create(a.mf, C.c_void_p(a.mask.data_ptr()), self.inner, self.rtol, self.inner_rtol,
+                                        self.chunk, self.max_inner, self.max_outer, ptr(o_ir), parts[2], sh,
+                                        C.byref(h), C.byref(used)))
+            a.ir = h
+            _lib.check(L.yeti_ir_begin(a.ir, C.c_void_p(a.F.data_ptr()), C.c_void_p(a.x.data_ptr()), sh))
+        except Exception:
+            self._release(a)
+            raise
+        a.status = torch.empty(STATUS_BYTES, dtype=torch.uint8, pin_memory=True)
+        a.event = torch.cuda.Event()
+        self._step(a)
+        return a
+
+    def _step(self, a):
+        L = _lib.lib()
+        sh = C.c_void_p(a.stream.cuda_stream)
+        _lib.check(L.yeti_ir_step(a.ir, 1 if self.use_graph else 0, sh))
+        _lib.check(L.yeti_ir_status_async(a.ir, C.c_void_p(a.status.data_ptr()), sh))
+        a.event.record(a.stream)
+        a.steps += 1
+
+    def _read_status(self, a):
+        buf = a.status.numpy()
+        fl = np.frombuffer(buf[:32].tobytes(), dtype=np.int32)
+        sc = np.frombuffer(buf[32:].tobytes(), dtype=np.float64)
+        return {"done": int(fl[3]), "failed": int(fl[4]), "inner_total": int(fl[6]), "outer": int(fl[7]),
+                "relres": math.sqrt(sc[5] / sc[4]) if sc[4] > 0 else 0.0}
+
+    def _release(self, a):
+        L = _lib.lib()
+        # the handles live in the arena: deleting them frees no device memory
+        if getattr(a, "ir", None) is not None:
+            L.yeti_ir_destroy(a.ir)
+            a.ir = None
+        if getattr(a, "mf", None) is not None:
+            L.yeti_mf_destroy(a.mf)
+            a.mf = None
+        if getattr(a, "asm", None) is not None:
+            L.yeti_asm_destroy(a.asm)
+            a.asm = None
+        self.alloc.release(a.off, a.nbytes)
+
+    # -- the loop ------------------------------------------------------------------------
+    def run(self, requests, keep_solutions=False, realtime=True, progress=None):
+        """Serve `requests` (sorted by arrival). realtime=False admits every request at t=0
+        (closed-loop throughput). Returns a report dict; with keep_solutions, report["x"][rid]
+        holds each solution on the host."""
+        torch = self.torch
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        ready = {}
+        lock = threading.Lock()
+        stop = [False]
+
+        def prep(chunk):
+            for r in chunk:
+                if stop[0]:
+                    return
+                r.model()
+                with lock:
+                    ready[r.rid] = True
+
+        # clients prepare their models ahead of time, in arrival order
+        threads = []
+        for k in range(max(1, self.prep_threads)):
+            th = threading.Thread(target=prep, args=(reqs[k::max(1, self.prep_threads)],), daemon=True)
+            th.start()
+            threads.append(th)
+        pending = deque(reqs)
+        active = []
+        done = {}
+        xs = {}
+        t0 = time.perf_counter()
+        si = 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        try:
+            while pending or active:
+                now = time.perf_counter() - t0
+                progressed = False
+                # admission, first come first served
+                while pending and (not realtime or pending[0].arrival <= now):
+                    r = pending[0]
+                    with lock:
+                        ok = ready.get(r.rid, False)
+                    if not ok:
+                        break
+                    a = self._admit(r, self.streams[si % len(self.streams)], keep_solutions)
+                    if a is None:
+                        if not active:
+                            raise RuntimeError("request %d needs more than the whole budget" % r.rid)
+                        break
+                    si += 1
+                    pending.popleft()
+                    active.append(a)
+                    progressed = True
+                # progress
+                still = []
+                for a in active:
+                    if not a.event.query():
+                        still.append(a)
+                        continue
+                    st = self._read_status(a)
+                    if st["done"]:
+                        t_end = time.perf_counter() - t0
+                        arr = a.req.arrival if realtime else 0.0
+                        done[a.req.rid] = dict(latency=t_end - arr, finish=t_end, steps=a.steps, p=a.req.p,
+                                               nel=a.req.nel, ndof=3 * a.req.nb_cp, **st)
+                        if a.keep:
+                            xs[a.req.rid] = a.x.cpu().numpy().copy()
+                        self._release(a)
+                        if progress:
+                            progress(len(done), len(reqs))
+                    else:
+                        self._step(a)
+                        still.append(a)
+                    progressed = True
+                active = still
+                if not progressed:
+                    time.sleep(20e-6)
+            torch.cuda.synchronize()
+            makespan = time.perf_counter() - t0
+        finally:
+            stop[0] = True
+            for a in active:
+                try:
+                    a.event.synchronize()
+                    self._release(a)
+                except Exception:
+                    pass
+        lat = [d["latency"] for d in done.values()]
+        rep = {"completed": len(done), "makespan_s": makespan,
+               "solves_per_s": len(done) / makespan if makespan > 0 else None,
+               "latency_s": latency_summary(lat), "failed": sum(1 for d in done.values() if d["failed"]),
+               "peak_arena_bytes": self.alloc.peak, "budget_bytes": self.budget, "requests": done}
+        if keep_solutions:
+            rep["x"] = xs
+        return rep
+
+    def close(self):
+        self.arena_buf = None
