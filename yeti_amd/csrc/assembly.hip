@@ -571,6 +571,7 @@ struct PatchDev {
     DevBuf<int32_t> loaded;
     int32_t n_loaded = 0;
     int32_t slot0 = 0;
+    DevBuf<double> geo;     // per-Gauss-point geometry of one element batch (allocated on first run)
 };
 
 }  // namespace yeti
@@ -646,10 +647,17 @@ static int device_cu_count() {
     return cached_n;
 }
 
+// Elements per launch pair (geometry pass + element kernel): bounds the geometry buffer
+// (12 doubles per Gauss point: 400 MB at p = 3) while keeping every CU busy.
+constexpr int32_t ELEM_BATCH = 65536;
+
 template <int P, int NG>
-static int launch_elem(const PatchDev &pd, const double *coords, double *Kd, int32_t *Kr,
-                       int32_t *Kc, hipStream_t st) {
+static int launch_elem(PatchDev &pd, const double *coords, double *Kd, int32_t *Kr, int32_t *Kc, hipStream_t st) {
+    using C = ElemCfg<P, NG>;
     if (pd.n_active == 0) return 0;
+    const int32_t batch = std::min(pd.n_active, ELEM_BATCH);
+    const size_t geo_n = (size_t)batch * ELEM_NGEO * C::NQ;
+    if (pd.geo.n < geo_n) YETI_HIP(pd.geo.alloc(geo_n));
     PatchArgs pa;
     pa.coords = coords;
     pa.ien = pd.ien.p;
@@ -658,20 +666,26 @@ static int launch_elem(const PatchDev &pd, const double *coords, double *Kd, int
     pa.U0 = pd.U[0].p;
     pa.U1 = pd.U[1].p;
     pa.U2 = pd.U[2].p;
-    pa.elems = pd.elems.p;
-    pa.offs = pd.offs.p;
-    pa.n_active = pd.n_active;
     pa.lam = pd.lam;
     pa.mu = pd.mu;
     gauss_legendre(NG, pa.g.x, pa.g.w);
-    // persistent workgroups: one per CU (the kernel's LDS and registers allow one)
-    const int grid = std::min(pd.n_active, device_cu_count());
-    dim3 block(ElemCfg<P, NG>::NTH);
-    if (Kr)
-        hipLaunchKernelGGL((elem_stiffness_kernel<P, NG, true>), dim3(grid), block, 0, st, pa, Kd, Kr, Kc);
-    else
-        hipLaunchKernelGGL((elem_stiffness_kernel<P, NG, false>), dim3(grid), block, 0, st, pa, Kd, Kr, Kc);
-    YETI_HIP(hipGetLastError());
+    const int cus = device_cu_count();
+    for (int32_t e0 = 0; e0 < pd.n_active; e0 += batch) {
+        pa.elems = pd.elems.p + e0;
+        pa.offs = pd.offs.p + e0;
+        pa.n_active = std::min(batch, pd.n_active - e0);
+        const int64_t nt = (int64_t)pa.n_active * C::NQ;
+        hipLaunchKernelGGL((elem_geo_kernel<P, NG>), dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, pa, pd.geo.p);
+        // persistent workgroups: one per CU (the kernel's LDS and registers allow one)
+        const int grid = std::min(pa.n_active, cus);
+        if (Kr)
+            hipLaunchKernelGGL((elem_stiffness_kernel<P, NG, true>), dim3(grid), dim3(C::NTH), 0, st, pa,
+                               (const double *)pd.geo.p, Kd, Kr, Kc);
+        else
+            hipLaunchKernelGGL((elem_stiffness_kernel<P, NG, false>), dim3(grid), dim3(C::NTH), 0, st, pa,
+                               (const double *)pd.geo.p, Kd, Kr, Kc);
+        YETI_HIP(hipGetLastError());
+    }
     return 0;
 }
 
