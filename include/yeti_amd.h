@@ -225,31 +225,35 @@ int32_t yeti_asm_create_in(const yeti_system_inputs *in, int32_t layout, void *a
                            void *stream, yeti_asm **out, size_t *used);
 
 /* ---- steppable, device-controlled solve (serving) ---------------------------------------
- * Solves K[free][:, free] x = b[free] with the matrix-free operator m: PCG with 3x3 block-Jacobi,
- * vectors in FP32 (inner_precision 32: FP64 iterative refinement, stopping at
- * ||b - K x||_2 <= rtol ||b||_2 with the FP64 apply, inner solves to inner_rtol of the current
- * residual) or FP64 (64). Control state lives on the device: yeti_ir_step enqueues `chunk` inner
- * iterations and one refinement step, all predicated on device flags, so a step is a fixed launch
- * sequence; with use_graph it is captured once into a hipGraph (on a non-null stream) and
- * replayed. The iterates do not depend on how steps of different sessions interleave. All
- * device memory is taken at creation (arena semantics as above; arena == NULL counts).
- * No reference counterpart: the reference solves one system per call
+ * Solves K[free][:, free] x = b[free] with the matrix-free operator m by 3x3 block-Jacobi PCG.
+ * precision 32: vectors, operator and preconditioner in FP32 (FP64 dot products), with the
+ * solution and the true residual kept in FP64 and the residual replaced by mask (b - K x) from
+ * the FP64 apply whenever the iterated residual has fallen by `delta` since the last replacement
+ * (mixed-precision refinement that keeps the Krylov space); precision 64: the same in FP64.
+ * Stops at ||b - K x||_2 <= rtol ||b||_2 on the FP64 residual; `failed` is set at max_iters,
+ * max_updates, or when four replacements in a row do not halve the true residual (the FP64
+ * rounding floor of an ill-conditioned system). Control state lives on the device:
+ * yeti_ir_step enqueues `chunk` iterations, every kernel predicated on device flags, so a step is
+ * a fixed launch sequence; with use_graph it is captured once into a hipGraph (on a non-null
+ * stream) and replayed. The iterates do not depend on how steps of different sessions
+ * interleave. All device memory is taken at creation (arena semantics as above; arena == NULL
+ * counts). No reference counterpart: the reference solves one system per call
  * (iga_wq_mf/fsrc/mf_iga_solver.f90:642-958; scipy spsolve in its callers). */
 typedef struct yeti_ir yeti_ir;
 typedef struct yeti_ir_status {
-    int32_t active, outer_pending, restart, done, failed, inner_iters, inner_total, outer_iters;
-    double rz, pq, rr_inner, rz_new, bb, rr_outer, tol_inner, tol_outer;
-    double reserved[8];
+    int32_t active, update_pending, stagnant, done, failed, iters, spare, updates;
+    double rz, pq, rr, rz_new, bb, rr_true, rr_max, tol2, rr_true_best;
+    double reserved[7];
 } yeti_ir_status;
 
-int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask_dev, int32_t inner_precision, double rtol,
-                       double inner_rtol, int32_t chunk, int32_t max_inner, int32_t max_outer, void *arena,
-                       size_t arena_bytes, void *stream, yeti_ir **out, size_t *used);
-/* x = 0 and the first inner solve set up for right-hand side b (device buffers kept by the session). */
+int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask_dev, int32_t precision, double rtol, double delta,
+                       int32_t chunk, int32_t max_iters, int32_t max_updates, void *arena, size_t arena_bytes,
+                       void *stream, yeti_ir **out, size_t *used);
+/* x = 0 and the first direction set up for right-hand side b (device buffers kept by the session). */
 int32_t yeti_ir_begin(yeti_ir *s, const double *b_dev, double *x_dev, void *stream);
 int32_t yeti_ir_step(yeti_ir *s, int32_t use_graph, void *stream);
 /* Enqueue a copy of the status into host memory (pinned for an asynchronous copy); read it once
- * the stream has reached this point. Relative residual = sqrt(rr_outer / bb). */
+ * the stream has reached this point. Relative residual = sqrt(rr_true / bb). */
 int32_t yeti_ir_status_async(const yeti_ir *s, yeti_ir_status *host_status, void *stream);
 void    yeti_ir_destroy(yeti_ir *s);
 

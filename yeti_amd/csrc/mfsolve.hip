@@ -366,27 +366,28 @@ int32_t yeti_mf_solve_ir(const yeti_mf *m, const double *b, const uint8_t *free_
 // ==========================================================================================
 // Steppable, device-controlled solver for serving (yeti_ir_*).
 //
-// One session solves K[free][:, free] x = b[free] for one handle. Its control state lives in
-// device memory: every kernel of a step is predicated on device flags, so a step is a fixed
-// launch sequence that can be captured once in a hipGraph and replayed until the session
-// reports done, with the host only reading a small status block. A step is
-//   `chunk` inner PCG iterations (FP32 or FP64 vectors, FP64 dots; an iteration runs only while
-//   the inner solve is active, and the iteration that meets the inner tolerance deactivates
-//   it), then one refinement step (predicated on the inner solve having just finished):
-//   x += d, r = mask (b - K x) with the FP64 apply, and either done (||r|| <= rtol ||b||) or a
-//   restart of the inner solve on r with tolerance inner_rtol ||r||.
-// The iterates depend only on the session's own launch sequence, never on how steps are
-// interleaved with other sessions: a request gives bit-identical results alone or in a mix.
-// All device memory is allocated at creation (optionally inside a caller-owned arena).
+// One session solves K[free][:, free] x = b[free] for one handle with preconditioned CG whose
+// vectors and operator run in FP32 (or FP64) while the solution and the true residual are kept
+// in FP64 and refreshed by residual replacement ("reliable updates"): the FP32 iteration
+// accumulates its update y; whenever its recurred residual has fallen by `delta` since the last
+// replacement (or meets the tolerance) x += y, r = mask (b - K x) with the FP64 apply, and the
+// FP32 residual restarts from r while the search direction is kept. The Krylov space is not
+// restarted, so an ill-conditioned system converges in about the iterations of FP64 CG, and the
+// stopping test ||b - K x||_2 <= rtol ||b||_2 is always made on the FP64 residual.
+// Control state lives in device memory: every kernel of a step is predicated on device flags,
+// so a step (`chunk` iterations) is a fixed launch sequence that is captured once in a hipGraph
+// and replayed until the session reports done; the host reads a small status block. The
+// iterates depend only on the session's own launch sequence: a request gives bit-identical
+// results alone or in a mix. All device memory is taken at creation (optionally in an arena).
 namespace yeti {
 
 int mf_apply_pred(const yeti_mf *m, const double *x, double *y, hipStream_t st, const int32_t *pred);
 int mf_apply_pred(const yeti_mf *m, const float *x, float *y, hipStream_t st, const int32_t *pred);
 
-// sc[] slots
-enum : int { S_RZ = 0, S_PQ, S_RR, S_RZN, S_BB, S_RR64, S_TOLIN, S_TOLOUT, S_NSC = 16 };
-// flags[] slots
-enum : int { F_ACTIVE = 0, F_OUTER_NOW, F_RESTART, F_DONE, F_FAILED, F_INNER_IT, F_INNER_TOTAL, F_OUTER, F_NFLAGS = 8 };
+// sc[] slots (doubles), in the order of yeti_ir_status
+enum : int { S_RZ = 0, S_PQ, S_RR, S_RZN, S_BB, S_RRT, S_RRMAX, S_TOL2, S_RRT_PREV, S_NSC = 16 };
+// flags[] slots (int32), in the order of yeti_ir_status
+enum : int { F_ACTIVE = 0, F_UPDATE, F_STAGNANT, F_DONE, F_FAILED, F_ITERS, F_SPARE, F_UPDATES, F_NFLAGS = 8 };
 
 __global__ void pred_sum_partials_kernel(const double *part, int n, double *out, const int32_t *pred) {
     if (pred && *pred == 0) return;
@@ -395,17 +396,6 @@ __global__ void pred_sum_partials_kernel(const double *part, int n, double *out,
     for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
     double t = block_sum256(v, sh);
     if (threadIdx.x == 0) *out = t;
-}
-
-// r_in = (T) r64, d = 0 (restart of the inner solve), predicated on F_RESTART
-template <typename T>
-__global__ void __launch_bounds__(256) ir_restart_copy_kernel(int64_t n, const double *r64, T *r, T *d,
-                                                              const int32_t *flags) {
-    if (flags[F_RESTART] == 0) return;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        r[i] = (T)r64[i];
-        d[i] = T(0);
-    }
 }
 
 template <typename T>
@@ -429,21 +419,6 @@ __global__ void __launch_bounds__(256) pred_precond_kernel(int32_t ncp, const T 
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// p = z (first direction of a restart), predicated on F_RESTART; then the inner solve is active
-template <typename T>
-__global__ void __launch_bounds__(256) ir_restart_p_kernel(int64_t n, const T *z, T *p, const int32_t *flags) {
-    if (flags[F_RESTART] == 0) return;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = z[i];
-}
-
-__global__ void ir_restart_done_kernel(int32_t *flags) {
-    if (flags[F_RESTART] == 0) return;
-    flags[F_RESTART] = 0;
-    flags[F_ACTIVE] = 1;
-    flags[F_INNER_IT] = 0;
-}
-
 template <typename T>
 __global__ void __launch_bounds__(256) pred_mask_dot_kernel(int64_t n, const uint8_t *mask, const T *__restrict__ p,
                                                             T *__restrict__ q, double *part, const int32_t *pred) {
@@ -459,8 +434,9 @@ __global__ void __launch_bounds__(256) pred_mask_dot_kernel(int64_t n, const uin
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// y += alpha p, r -= alpha q; partial r.r
 template <typename T>
-__global__ void __launch_bounds__(256) pred_xr_kernel(int64_t n, const double *sc, T *__restrict__ x, T *__restrict__ r,
+__global__ void __launch_bounds__(256) pred_xr_kernel(int64_t n, const double *sc, T *__restrict__ y, T *__restrict__ r,
                                                       const T *__restrict__ p, const T *__restrict__ q, double *part,
                                                       const int32_t *pred) {
     if (*pred == 0) return;
@@ -468,7 +444,7 @@ __global__ void __launch_bounds__(256) pred_xr_kernel(int64_t n, const double *s
     const double alpha = sc[S_RZ] / sc[S_PQ];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        x[i] = (T)((double)x[i] + alpha * (double)p[i]);
+        y[i] = (T)((double)y[i] + alpha * (double)p[i]);
         T ri = (T)((double)r[i] - alpha * (double)q[i]);
         r[i] = ri;
         acc += (double)ri * (double)ri;
@@ -486,77 +462,118 @@ __global__ void __launch_bounds__(256) pred_p_kernel(int64_t n, const double *sc
         p[i] = (T)((double)z[i] + beta * (double)p[i]);
 }
 
-// end of an inner iteration: roll rz, count, and stop the inner solve at its tolerance (or cap)
-__global__ void ir_inner_check_kernel(double *sc, int32_t *flags, int32_t max_inner) {
+// after the residual update of an iteration: replace the residual when it has fallen by delta
+// since the last replacement, or when it meets the tolerance
+__global__ void ir_check_kernel(double *sc, int32_t *flags, double delta2) {
     if (flags[F_ACTIVE] == 0) return;
-    sc[S_RZ] = sc[S_RZN];
-    flags[F_INNER_IT] += 1;
-    flags[F_INNER_TOTAL] += 1;
-    if (!(sc[S_RR] > sc[S_TOLIN]) || flags[F_INNER_IT] >= max_inner) {
-        flags[F_ACTIVE] = 0;
-        flags[F_OUTER_NOW] = 1;
+    const double rr = sc[S_RR];
+    if (rr > sc[S_RRMAX]) sc[S_RRMAX] = rr;
+    if (!(rr > delta2 * sc[S_RRMAX]) || !(rr > sc[S_TOL2])) flags[F_UPDATE] = 1;
+}
+
+// x += y, y = 0 (FP64 accumulation of the FP32/FP64 update), predicated on F_UPDATE
+template <typename T>
+__global__ void __launch_bounds__(256) ir_accumulate_kernel(int64_t n, double *x, T *y, const int32_t *flags) {
+    if (flags[F_UPDATE] == 0) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] += (double)y[i];
+        y[i] = T(0);
     }
 }
 
+// r64 = mask (b - K x); r = (T) r64; partial r64.r64 (predicated)
 template <typename T>
-__global__ void __launch_bounds__(256) ir_x_update_kernel(int64_t n, double *x, const T *d, const int32_t *flags) {
-    if (flags[F_OUTER_NOW] == 0) return;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        x[i] += (double)d[i];
-}
-
-__global__ void __launch_bounds__(256) pred_residual_kernel(int64_t n, const double *b, const double *y, const uint8_t *mask,
-                                                            double *r, double *part, const int32_t *pred) {
+__global__ void __launch_bounds__(256) ir_true_residual_kernel(int64_t n, const double *b, const double *Kx,
+                                                               const uint8_t *mask, T *r, double *part,
+                                                               const int32_t *pred) {
     if (*pred == 0) return;
     __shared__ double sh[4];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double v = mask[i] ? b[i] - y[i] : 0.0;
-        r[i] = v;
+        double v = mask[i] ? b[i] - Kx[i] : 0.0;
+        r[i] = (T)v;
         acc += v * v;
     }
     double t = block_sum256(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-__global__ void ir_outer_check_kernel(double *sc, int32_t *flags, double inner_rtol2, int32_t max_outer) {
-    if (flags[F_OUTER_NOW] == 0) return;
-    flags[F_OUTER_NOW] = 0;
-    flags[F_OUTER] += 1;
-    if (!(sc[S_RR64] > sc[S_TOLOUT])) {
+// end of a replacement: stop at the tolerance; count replacements that do not improve the true
+// residual (stagnation at the FP64 rounding floor of ill-conditioned systems)
+__global__ void ir_update_done_kernel(double *sc, int32_t *flags, double rtol2, int32_t max_updates,
+                                      int32_t max_stagnant) {
+    if (flags[F_UPDATE] == 0) return;
+    flags[F_UPDATE] = 0;
+    flags[F_UPDATES] += 1;
+    const double rrt = sc[S_RRT];
+    sc[S_RR] = rrt;
+    sc[S_RRMAX] = rrt;
+    if (!(rrt > rtol2 * sc[S_BB])) {
         flags[F_DONE] = 1;
-    } else if (flags[F_OUTER] >= max_outer) {
+        flags[F_ACTIVE] = 0;
+        return;
+    }
+    if (!(rrt < 0.25 * sc[S_RRT_PREV])) flags[F_STAGNANT] += 1;
+    else flags[F_STAGNANT] = 0;
+    sc[S_RRT_PREV] = rrt < sc[S_RRT_PREV] ? rrt : sc[S_RRT_PREV];
+    if (flags[F_UPDATES] >= max_updates || flags[F_STAGNANT] >= max_stagnant) {
         flags[F_DONE] = 1;
         flags[F_FAILED] = 1;
-    } else {
-        sc[S_TOLIN] = inner_rtol2 * sc[S_RR64];
-        flags[F_RESTART] = 1;
+        flags[F_ACTIVE] = 0;
     }
 }
 
-// start of a solve: x = 0, r64 = mask b, thresholds; restart requested unless b = 0
+// end of an iteration: roll rz, count, cap
+__global__ void ir_roll_kernel(double *sc, int32_t *flags, int32_t max_iters) {
+    if (flags[F_ACTIVE] == 0) return;
+    sc[S_RZ] = sc[S_RZN];
+    flags[F_ITERS] += 1;
+    if (flags[F_ITERS] >= max_iters) {
+        flags[F_DONE] = 1;
+        flags[F_FAILED] = 1;
+        flags[F_ACTIVE] = 0;
+    }
+}
+
+template <typename T>
 __global__ void __launch_bounds__(256) ir_begin_copy_kernel(int64_t n, const double *b, const uint8_t *mask, double *x,
-                                                            double *r, double *part) {
+                                                            T *r, T *y, double *part) {
     __shared__ double sh[4];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         double v = mask[i] ? b[i] : 0.0;
-        r[i] = v;
+        r[i] = (T)v;
         x[i] = 0.0;
+        y[i] = T(0);
         acc += v * v;
     }
     double t = block_sum256(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-__global__ void ir_begin_flags_kernel(double *sc, int32_t *flags, double rtol2, double inner_rtol2) {
+__global__ void ir_begin_flags_kernel(double *sc, int32_t *flags, double rtol2) {
     const double bb = sc[S_BB];
-    sc[S_RR64] = bb;
-    sc[S_TOLOUT] = rtol2 * bb;
-    sc[S_TOLIN] = inner_rtol2 * bb;
+    sc[S_RR] = bb;
+    sc[S_RRT] = bb;
+    sc[S_RRMAX] = bb;
+    sc[S_RRT_PREV] = bb;
+    sc[S_TOL2] = rtol2 * bb;
     for (int i = 0; i < F_NFLAGS; ++i) flags[i] = 0;
-    if (bb > 0.0) flags[F_RESTART] = 1;
+    if (bb > 0.0) flags[F_ACTIVE] = 1;
     else flags[F_DONE] = 1;
+}
+
+// p = z (first direction), rz, predicated on F_ACTIVE
+template <typename T>
+__global__ void __launch_bounds__(256) ir_first_p_kernel(int64_t n, const T *z, T *p, const int32_t *flags) {
+    if (flags[F_ACTIVE] == 0) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = z[i];
+}
+
+__global__ void ir_first_rz_kernel(double *sc, const int32_t *flags) {
+    if (flags[F_ACTIVE] == 0) return;
+    sc[S_RZ] = sc[S_RZN];
 }
 
 }  // namespace yeti
@@ -566,20 +583,19 @@ struct yeti_ir {
     const uint8_t *mask = nullptr;
     int64_t n = 0;
     int32_t ncp = 0;
-    int inner64 = 0;          // 1: FP64 inner vectors / apply (plain FP64 PCG), 0: FP32
-    double rtol = 0, inner_rtol = 0;
-    int32_t chunk = 8, max_inner = 0, max_outer = 0;
+    int inner64 = 0;          // 1: FP64 vectors / apply (plain FP64 PCG), 0: FP32
+    double rtol = 0, delta = 0;
+    int32_t chunk = 8, max_iters = 0, max_updates = 0;
     unsigned nb = 1, nbc = 1;
-    yeti::DevBuf<double> blk, inv64, r64, y64, part, sc, status;
+    yeti::DevBuf<double> blk, inv64, y64, part, sc;
     yeti::DevBuf<float> inv32;
-    yeti::DevBuf<double> vr64, vd64, vz64, vp64, vq64;   // inner vectors, FP64
-    yeti::DevBuf<float> vr32, vd32, vz32, vp32, vq32;    // inner vectors, FP32
+    yeti::DevBuf<double> vr64, vy64, vz64, vp64, vq64;   // iteration vectors, FP64
+    yeti::DevBuf<float> vr32, vy32, vz32, vp32, vq32;    // iteration vectors, FP32
     yeti::DevBuf<int32_t> flags;
     const double *b = nullptr;
     double *x = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
-    hipStream_t gstream = nullptr;
     ~yeti_ir() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -588,75 +604,73 @@ struct yeti_ir {
 
 namespace yeti {
 
-template <typename T>
-static int ir_restart(yeti_ir *s, T *r, T *d, T *z, T *p, const T *inv, hipStream_t st) {
-    hipLaunchKernelGGL(ir_restart_copy_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->r64.p, r, d, s->flags.p);
-    hipLaunchKernelGGL(pred_precond_kernel<T>, dim3(s->nbc), dim3(256), 0, st, s->ncp, inv, r, z, s->part.p,
-                       s->flags.p + F_RESTART);
-    hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nbc, s->sc.p + S_RZ,
-                       s->flags.p + F_RESTART);
-    hipLaunchKernelGGL(ir_restart_p_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, z, p, s->flags.p);
-    hipLaunchKernelGGL(ir_restart_done_kernel, dim3(1), dim3(1), 0, st, s->flags.p);
-    YETI_HIP(hipGetLastError());
-    return 0;
-}
+constexpr int32_t IR_MAX_STAGNANT = 4;
 
 // the launch sequence of one step (captured into the session's graph)
 template <typename T>
-static int ir_step_launches(yeti_ir *s, T *r, T *d, T *z, T *p, T *q, const T *inv, hipStream_t st) {
+static int ir_step_launches(yeti_ir *s, T *r, T *y, T *z, T *p, T *q, const T *inv, hipStream_t st) {
     const int32_t *act = s->flags.p + F_ACTIVE;
+    const int32_t *upd = s->flags.p + F_UPDATE;
     for (int k = 0; k < s->chunk; ++k) {
         int rc = mf_apply_pred(s->m, p, q, st, act);
         if (rc) return rc;
         hipLaunchKernelGGL(pred_mask_dot_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->mask, p, q, s->part.p, act);
         hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_PQ, act);
-        hipLaunchKernelGGL(pred_xr_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->sc.p, d, r, p, q, s->part.p, act);
+        hipLaunchKernelGGL(pred_xr_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->sc.p, y, r, p, q, s->part.p, act);
         hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_RR, act);
+        hipLaunchKernelGGL(ir_check_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->delta * s->delta);
+        // residual replacement (predicated)
+        hipLaunchKernelGGL(ir_accumulate_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->x, y, s->flags.p);
+        rc = mf_apply_pred(s->m, (const double *)s->x, s->y64.p, st, upd);
+        if (rc) return rc;
+        hipLaunchKernelGGL(ir_true_residual_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->b, s->y64.p, s->mask, r,
+                           s->part.p, upd);
+        hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_RRT, upd);
+        hipLaunchKernelGGL(ir_update_done_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->rtol * s->rtol,
+                           s->max_updates, IR_MAX_STAGNANT);
+        // preconditioner and direction
         hipLaunchKernelGGL(pred_precond_kernel<T>, dim3(s->nbc), dim3(256), 0, st, s->ncp, inv, r, z, s->part.p, act);
         hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nbc, s->sc.p + S_RZN, act);
         hipLaunchKernelGGL(pred_p_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->sc.p, p, z, act);
-        hipLaunchKernelGGL(ir_inner_check_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->max_inner);
+        hipLaunchKernelGGL(ir_roll_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->max_iters);
         YETI_HIP(hipGetLastError());
     }
-    // refinement step, predicated on the inner solve having just finished
-    const int32_t *now = s->flags.p + F_OUTER_NOW;
-    hipLaunchKernelGGL(ir_x_update_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->x, (const T *)d, s->flags.p);
-    int rc = mf_apply_pred(s->m, (const double *)s->x, s->y64.p, st, now);
-    if (rc) return rc;
-    hipLaunchKernelGGL(pred_residual_kernel, dim3(s->nb), dim3(256), 0, st, s->n, s->b, s->y64.p, s->mask, s->r64.p,
-                       s->part.p, now);
-    hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_RR64, now);
-    hipLaunchKernelGGL(ir_outer_check_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->inner_rtol * s->inner_rtol,
-                       s->max_outer);
-    YETI_HIP(hipGetLastError());
-    rc = ir_restart<T>(s, r, d, z, p, inv, st);
-    if (rc) return rc;
     return 0;
 }
 
 static int ir_step_any(yeti_ir *s, hipStream_t st) {
     if (s->inner64)
-        return ir_step_launches<double>(s, s->vr64.p, s->vd64.p, s->vz64.p, s->vp64.p, s->vq64.p, s->inv64.p, st);
-    return ir_step_launches<float>(s, s->vr32.p, s->vd32.p, s->vz32.p, s->vp32.p, s->vq32.p, s->inv32.p, st);
+        return ir_step_launches<double>(s, s->vr64.p, s->vy64.p, s->vz64.p, s->vp64.p, s->vq64.p, s->inv64.p, st);
+    return ir_step_launches<float>(s, s->vr32.p, s->vy32.p, s->vz32.p, s->vp32.p, s->vq32.p, s->inv32.p, st);
 }
 
-static int ir_restart_any(yeti_ir *s, hipStream_t st) {
-    if (s->inner64) return ir_restart<double>(s, s->vr64.p, s->vd64.p, s->vz64.p, s->vp64.p, s->inv64.p, st);
-    return ir_restart<float>(s, s->vr32.p, s->vd32.p, s->vz32.p, s->vp32.p, s->inv32.p, st);
+template <typename T>
+static int ir_begin_launches(yeti_ir *s, T *r, T *y, T *z, T *p, const T *inv, hipStream_t st) {
+    const int32_t *act = s->flags.p + F_ACTIVE;
+    hipLaunchKernelGGL(ir_begin_copy_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->b, s->mask, s->x, r, y, s->part.p);
+    hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_BB,
+                       (const int32_t *)nullptr);
+    hipLaunchKernelGGL(ir_begin_flags_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->rtol * s->rtol);
+    hipLaunchKernelGGL(pred_precond_kernel<T>, dim3(s->nbc), dim3(256), 0, st, s->ncp, inv, r, z, s->part.p, act);
+    hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nbc, s->sc.p + S_RZN, act);
+    hipLaunchKernelGGL(ir_first_p_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, z, p, s->flags.p);
+    hipLaunchKernelGGL(ir_first_rz_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p);
+    YETI_HIP(hipGetLastError());
+    return 0;
 }
 
 }  // namespace yeti
 
 extern "C" {
 
-int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask, int32_t inner_precision, double rtol,
-                       double inner_rtol, int32_t chunk, int32_t max_inner, int32_t max_outer, void *arena,
-                       size_t arena_bytes, void *stream, yeti_ir **out, size_t *used) {
+int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask, int32_t precision, double rtol, double delta,
+                       int32_t chunk, int32_t max_iters, int32_t max_updates, void *arena, size_t arena_bytes,
+                       void *stream, yeti_ir **out, size_t *used) {
     if (!m || !out) return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: null argument");
     *out = nullptr;
-    if (inner_precision != 32 && inner_precision != 64)
-        return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: inner precision must be 32 or 64");
-    if (!(rtol > 0.0) || !(inner_rtol > 0.0) || inner_rtol >= 1.0 || chunk < 1 || max_inner < 1 || max_outer < 1)
+    if (precision != 32 && precision != 64)
+        return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: precision must be 32 or 64");
+    if (!(rtol > 0.0) || !(delta > 0.0) || delta >= 1.0 || chunk < 1 || max_iters < 1 || max_updates < 1)
         return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: bad tolerance or iteration limits");
     if (arena && !free_mask) return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: null free mask");
     hipStream_t st = (hipStream_t)stream;
@@ -673,28 +687,27 @@ int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask, int32_t inner
     s->mask = free_mask;
     s->ncp = yeti_mf_nb_cp(m);
     s->n = 3 * (int64_t)s->ncp;
-    s->inner64 = inner_precision == 64;
+    s->inner64 = precision == 64;
     s->rtol = rtol;
-    s->inner_rtol = inner_rtol;
+    s->delta = delta;
     s->chunk = chunk;
-    s->max_inner = max_inner;
-    s->max_outer = max_outer;
+    s->max_iters = max_iters;
+    s->max_updates = max_updates;
     s->nb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(NRED, (s->n + 255) / 256));
     s->nbc = (unsigned)std::max<int64_t>(1, std::min<int64_t>(NRED, (s->ncp + 255) / 256));
     const size_t n = (size_t)s->n;
     hipError_t e = s->blk.alloc(9 * (size_t)s->ncp);
-    if (e == hipSuccess) e = s->r64.alloc(n);
     if (e == hipSuccess) e = s->y64.alloc(n);
     if (e == hipSuccess) e = s->part.alloc(NRED);
     if (e == hipSuccess) e = s->sc.alloc(S_NSC);
     if (e == hipSuccess) e = s->flags.alloc(F_NFLAGS);
     if (s->inner64) {
         if (e == hipSuccess) e = s->inv64.alloc(9 * (size_t)s->ncp);
-        for (auto *v : {&s->vr64, &s->vd64, &s->vz64, &s->vp64, &s->vq64})
+        for (auto *v : {&s->vr64, &s->vy64, &s->vz64, &s->vp64, &s->vq64})
             if (e == hipSuccess) e = v->alloc(n);
     } else {
         if (e == hipSuccess) e = s->inv32.alloc(9 * (size_t)s->ncp);
-        for (auto *v : {&s->vr32, &s->vd32, &s->vz32, &s->vp32, &s->vq32})
+        for (auto *v : {&s->vr32, &s->vy32, &s->vz32, &s->vp32, &s->vq32})
             if (e == hipSuccess) e = v->alloc(n);
     }
     if (e != hipSuccess) return fail(set_error(YETI_ERR_BUDGET, "ir: device memory: %s", hipGetErrorString(e)));
@@ -726,13 +739,8 @@ int32_t yeti_ir_begin(yeti_ir *s, const double *b, double *x, void *stream) {
     }
     s->b = b;
     s->x = x;
-    hipLaunchKernelGGL(ir_begin_copy_kernel, dim3(s->nb), dim3(256), 0, st, s->n, b, s->mask, x, s->r64.p, s->part.p);
-    hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_BB,
-                       (const int32_t *)nullptr);
-    hipLaunchKernelGGL(ir_begin_flags_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->rtol * s->rtol,
-                       s->inner_rtol * s->inner_rtol);
-    YETI_HIP(hipGetLastError());
-    return ir_restart_any(s, st);
+    if (s->inner64) return ir_begin_launches<double>(s, s->vr64.p, s->vy64.p, s->vz64.p, s->vp64.p, s->inv64.p, st);
+    return ir_begin_launches<float>(s, s->vr32.p, s->vy32.p, s->vz32.p, s->vp32.p, s->inv32.p, st);
 }
 
 int32_t yeti_ir_step(yeti_ir *s, int32_t use_graph, void *stream) {

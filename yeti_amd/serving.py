@@ -14,9 +14,11 @@ routines' counting mode, plus the solve session and the request's vectors -- fit
 range of the arena; its handles are created inside that range (no hipMalloc/hipFree while
 serving) and the range is returned when the request completes.
 
-Solves: ``precision="mixed"`` is FP64 iterative refinement over FP32 PCG (FP32 apply and
-preconditioner, FP64 dots and residuals); ``precision="fp64"`` runs the same PCG in FP64.
-Each step (``chunk`` inner iterations and one refinement step) is one hipGraph launch per
+Solves: ``precision="mixed"`` is block-Jacobi PCG with FP32 vectors, operator and
+preconditioner (FP64 dots) whose FP64 solution and true residual are refreshed by residual
+replacement with the FP64 apply each time the FP32 residual has fallen by ``delta``
+(refinement that keeps the Krylov space); ``precision="fp64"`` runs the same PCG in FP64.
+Both stop on the FP64 residual. Each step (``chunk`` iterations) is one hipGraph launch per
 request; steps of different requests run on a pool of streams and never change each other's
 iterates, so a request's solution is bit-identical alone or in a mix.
 """
@@ -90,11 +92,11 @@ class ArenaAllocator:
         self.used -= n
 
 
-def ir_session_bytes(nb_cp, inner_precision):
+def ir_session_bytes(nb_cp, precision):
     """Device bytes of yeti_ir_create (mirrors its allocations, each ALIGN-aligned)."""
     n = 3 * nb_cp
-    w = 8 if inner_precision == 64 else 4
-    sizes = [9 * nb_cp * 8, n * 8, n * 8, NRED * 8, 16 * 8, 8 * 4, 9 * nb_cp * w] + [n * w] * 5
+    w = 8 if precision == 64 else 4
+    sizes = [9 * nb_cp * 8, n * 8, NRED * 8, 16 * 8, 8 * 4, 9 * nb_cp * w] + [n * w] * 5
     return sum(_align(s) for s in sizes)
 
 
@@ -162,8 +164,8 @@ class _Active:
 class SolveServer:
     """Runs a list of requests against one arena of `budget_bytes`."""
 
-    def __init__(self, budget_bytes, precision="mixed", nstreams=8, chunk=8, rtol=1e-10, inner_rtol=1e-4,
-                 max_inner=20000, max_outer=60, use_graph=True, prep_threads=2):
+    def __init__(self, budget_bytes, precision="mixed", nstreams=8, chunk=8, rtol=1e-10, delta=0.1,
+                 max_iters=100000, max_updates=400, use_graph=True, prep_threads=2):
         import torch
         _lib.require_device()
         if precision not in ("mixed", "fp64"):
@@ -171,8 +173,8 @@ class SolveServer:
         self.torch = torch
         self.inner = 32 if precision == "mixed" else 64
         self.precision = precision
-        self.chunk, self.rtol, self.inner_rtol = int(chunk), float(rtol), float(inner_rtol)
-        self.max_inner, self.max_outer = int(max_inner), int(max_outer)
+        self.chunk, self.rtol, self.delta = int(chunk), float(rtol), float(delta)
+        self.max_iters, self.max_updates = int(max_iters), int(max_updates)
         self.use_graph = bool(use_graph)
         self.budget = int(budget_bytes)
         self.arena_buf = torch.empty(self.budget, dtype=torch.uint8, device="cuda")
@@ -240,8 +242,8 @@ class SolveServer:
                 a.mask.copy_(torch.from_numpy(m), non_blocking=False)
             _lib.check(L.yeti_asm_run_rhs(a.asm, C.c_void_p(a.F.data_ptr()), sh))
             h = C.c_void_p()
-            _lib.check(L.yeti_ir_create(a.mf, C.c_void_p(a.mask.data_ptr()), self.inner, self.rtol, self.inner_rtol,
-                                        self.chunk, self.max_inner, self.max_outer, ptr(o_ir), parts[2], sh,
+            _lib.check(L.yeti_ir_create(a.mf, C.c_void_p(a.mask.data_ptr()), self.inner, self.rtol, self.delta,
+                                        self.chunk, self.max_iters, self.max_updates, ptr(o_ir), parts[2], sh,
                                         C.byref(h), C.byref(used)))
             a.ir = h
             _lib.check(L.yeti_ir_begin(a.ir, C.c_void_p(a.F.data_ptr()), C.c_void_p(a.x.data_ptr()), sh))
@@ -265,7 +267,7 @@ class SolveServer:
         buf = a.status.numpy()
         fl = np.frombuffer(buf[:32].tobytes(), dtype=np.int32)
         sc = np.frombuffer(buf[32:].tobytes(), dtype=np.float64)
-        return {"done": int(fl[3]), "failed": int(fl[4]), "inner_total": int(fl[6]), "outer": int(fl[7]),
+        return {"done": int(fl[3]), "failed": int(fl[4]), "iters": int(fl[5]), "updates": int(fl[7]),
                 "relres": math.sqrt(sc[5] / sc[4]) if sc[4] > 0 else 0.0}
 
     def _release(self, a):
