@@ -240,15 +240,21 @@ int32_t yeti_asm_create_in(const yeti_system_inputs *in, int32_t layout, void *a
  * counts). No reference counterpart: the reference solves one system per call
  * (iga_wq_mf/fsrc/mf_iga_solver.f90:642-958; scipy spsolve in its callers). */
 typedef struct yeti_ir yeti_ir;
+typedef struct yeti_fdm yeti_fdm;   /* fast diagonalisation data of a session (opaque) */
 typedef struct yeti_ir_status {
     int32_t active, update_pending, stagnant, done, failed, iters, spare, updates;
     double rz, pq, rr, rz_new, bb, rr_true, rr_max, tol2, rr_true_best;
     double reserved[7];
 } yeti_ir_status;
 
-int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask_dev, int32_t precision, double rtol, double delta,
-                       int32_t chunk, int32_t max_iters, int32_t max_updates, void *arena, size_t arena_bytes,
-                       void *stream, yeti_ir **out, size_t *used);
+/* precond 0: 3x3 nodal block-Jacobi; 1: fast diagonalisation (single-patch handles whose fixed
+ * dofs are whole patch faces per component), the preconditioner family of iga_mf_cg_3d
+ * (iga_wq_mf/fsrc/tensor_algebra.f90:992-1161): per component the separable operator
+ * sum_d alpha_d K_d (x) M (x) M built from 1D B-spline Gram matrices and the patch-mean
+ * Jacobian, inverted exactly by 1D generalized eigendecompositions. */
+int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask_dev, int32_t precision, int32_t precond, double rtol,
+                       double delta, int32_t chunk, int32_t max_iters, int32_t max_updates, void *arena,
+                       size_t arena_bytes, void *stream, yeti_ir **out, size_t *used);
 /* x = 0 and the first direction set up for right-hand side b (device buffers kept by the session). */
 int32_t yeti_ir_begin(yeti_ir *s, const double *b_dev, double *x_dev, void *stream);
 int32_t yeti_ir_step(yeti_ir *s, int32_t use_graph, void *stream);

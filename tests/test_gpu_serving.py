@@ -38,12 +38,12 @@ def small_requests():
     return out
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_served_solutions_match_direct_solve(gpu, precision):
+@pytest.mark.parametrize("precision,precond", [("mixed", "fdm"), ("fp64", "fdm"), ("mixed", "jacobi")])
+def test_served_solutions_match_direct_solve(gpu, precision, precond):
     from yeti_amd.serving import SolveServer
-    print("[case] served solutions vs direct solve", precision, flush=True)
+    print("[case] served solutions vs direct solve", precision, precond, flush=True)
     reqs = small_requests()[:6]
-    srv = SolveServer(budget_bytes=1 << 30, precision=precision, nstreams=3, chunk=6)
+    srv = SolveServer(budget_bytes=1 << 30, precision=precision, nstreams=3, chunk=6, rtol=1e-10, precond=precond)
     rep = srv.run(reqs, keep_solutions=True, realtime=False)
     assert rep["completed"] == len(reqs) and rep["failed"] == 0
     for r in reqs:
@@ -100,22 +100,27 @@ def test_budget_admission_limits_footprint(gpu):
     print("  peak arena %.1f MiB of %.1f MiB" % (rep["peak_arena_bytes"] / 2**20, budget / 2**20))
 
 
-def test_session_footprint_formula(gpu):
+@pytest.mark.parametrize("precond", [0, 1])
+def test_session_footprint_formula(gpu, precond):
     import ctypes as C
     import torch
     from yeti_amd import _lib
     from yeti_amd.matfree import MatrixFree
     from yeti_amd.serving import ir_session_bytes
     from yeti_amd.solver import free_mask_from_args
-    print("[case] session footprint", flush=True)
+    print("[case] session footprint", precond, flush=True)
     bp = make_box_patch(nel=(5, 3, 2), p=3)
     mf = MatrixFree(bp.args)
     mask = free_mask_from_args(bp.args)
+    dims = (5 + 3, 3 + 3, 2 + 3)
     for prec in (32, 64):
         arena = torch.empty(64 << 20, dtype=torch.uint8, device="cuda")
-        h, used = C.c_void_p(), C.c_size_t(0)
-        _lib.check(_lib.lib().yeti_ir_create(mf._h, C.c_void_p(mask.data_ptr()), prec, 1e-10, 1e-4, 4, 1000, 20,
-                                             C.c_void_p(arena.data_ptr()), arena.numel(), None, C.byref(h),
+        h, used, counted = C.c_void_p(), C.c_size_t(0), C.c_size_t(0)
+        _lib.check(_lib.lib().yeti_ir_create(mf._h, C.c_void_p(mask.data_ptr()), prec, precond, 1e-10, 0.1, 4, 1000,
+                                             20, None, 0, None, C.byref(h), C.byref(counted)))
+        _lib.check(_lib.lib().yeti_ir_create(mf._h, C.c_void_p(mask.data_ptr()), prec, precond, 1e-10, 0.1, 4, 1000,
+                                             20, C.c_void_p(arena.data_ptr()), arena.numel(), None, C.byref(h),
                                              C.byref(used)))
-        assert used.value <= ir_session_bytes(bp.nb_cp, prec)
+        bound = ir_session_bytes(dims, prec, "fdm" if precond else "jacobi")
+        assert used.value <= counted.value <= bound
         _lib.lib().yeti_ir_destroy(h)

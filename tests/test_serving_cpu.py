@@ -81,5 +81,7 @@ def test_latency_percentiles_nearest_rank():
 
 
 def test_ir_session_bytes_grows_with_size_and_precision():
-    assert ir_session_bytes(1000, 64) > ir_session_bytes(1000, 32) > ir_session_bytes(100, 32)
-    assert ir_session_bytes(1000, 32) % ALIGN == 0
+    d, d2 = (10, 10, 10), (5, 4, 5)
+    assert ir_session_bytes(d, 64) > ir_session_bytes(d, 32) > ir_session_bytes(d2, 32)
+    assert ir_session_bytes(d, 32, "fdm") > ir_session_bytes(d, 32, "jacobi")
+    assert ir_session_bytes(d, 32) % ALIGN == 0

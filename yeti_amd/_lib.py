@@ -59,8 +59,8 @@ SIGNATURES = {
                                       C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]),
     "yeti_asm_create_in": (C.c_int32, [C.POINTER(YetiSystemInputs), C.c_int32, C.c_void_p, C.c_size_t, C.c_void_p,
                                        C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]),
-    "yeti_ir_create": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,
-                                   C.c_int32, C.c_void_p, C.c_size_t, C.c_void_p, C.POINTER(C.c_void_p),
+    "yeti_ir_create": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
+                                   C.c_int32, C.c_int32, C.c_void_p, C.c_size_t, C.c_void_p, C.POINTER(C.c_void_p),
                                    C.POINTER(C.c_size_t)]),
     "yeti_ir_begin": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_ir_step": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p]),

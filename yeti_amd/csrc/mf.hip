@@ -489,6 +489,10 @@ struct MfPatchDev {
     DevBuf<double> part64;
     DevBuf<float> part32;
     GaussRule g;
+    // host copies for the fast-diagonalisation set-up
+    std::vector<double> hB, hdB, hhalf, hgw;
+    std::vector<int32_t> hcpmap;
+    std::vector<double> hU[3];
 };
 }  // namespace yeti
 
@@ -582,6 +586,419 @@ static int mf_geometry(const yeti_mf *m, const MfPatchDev &pd, hipStream_t st) {
     YETI_HIP(hipGetLastError());
     return 0;
 }
+
+}  // namespace yeti
+
+
+// ==========================================================================================
+// Fast diagonalisation preconditioner (single-patch handles), the preconditioner family of
+// YETI's matrix-free CG (iga_wq_mf/fsrc/tensor_algebra.f90:992-1161, after Sangalli and Tani,
+// "Isogeometric preconditioners based on fast solvers for the Sylvester equation", 2016).
+// Component c of the displacement is preconditioned by the separable operator
+//   A_c = sum_d alpha_cd K_d (x) M_(other directions),
+// K_d / M_d the 1D B-spline stiffness / mass Gram matrices of direction d on the free index range
+// of component c, alpha_cd = |det J| (mu g_dd + (lam + mu) (J^-1)_dc^2) from the patch-mean
+// Jacobian J (the d = e terms of the diagonal block K_cc of isotropic elasticity). With
+// K_d U_d = M_d U_d Lambda_d and U_d^T M_d U_d = I,
+//   A_c^-1 = (U_2 (x) U_1 (x) U_0) diag(1 / (alpha_c0 l0_i + alpha_c1 l1_j + alpha_c2 l2_k))
+//            (U_2 (x) U_1 (x) U_0)^T,
+// applied as three mode products, a scaling and three mode products per component. Fixed dofs
+// must be whole patch faces per component (the clamped face of the solids here).
+namespace yeti {
+
+struct FdmArgs {
+    int32_t n[3];            // patch CP grid
+    int32_t lo[3][3];        // [component][direction] first free index
+    int32_t m[3][3];         // [component][direction] free count
+    int64_t tot[3];          // m0 m1 m2 per component
+    int64_t maxtot;
+    int64_t uoff[3][3];      // offset of U_cd (m x m, row-major) in the matrix buffer
+    int64_t doff[3];         // offset of 1/D_c in the scaling buffer
+    const int32_t *cpmap;
+};
+
+}  // namespace yeti
+
+struct yeti_fdm {
+    yeti::FdmArgs fa;
+    int32_t ncp = 0;
+    yeti::DevBuf<double> U64, Ut64, invD64, wa64, wb64;
+    yeti::DevBuf<float> U32, Ut32, invD32, wa32, wb32;
+};
+
+namespace yeti {
+
+// cyclic Jacobi eigensolver of a symmetric n x n matrix a (row-major, destroyed): eigenvalues
+// in w, eigenvectors in the columns of v
+static void jacobi_eigen(int n, std::vector<double> &a, std::vector<double> &w, std::vector<double> &v) {
+    v.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, dia = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) (i == j ? dia : off) += a[(size_t)i * n + j] * a[(size_t)i * n + j];
+        if (off <= 1e-30 * dia) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = a[(size_t)p * n + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                const double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < n; ++k) {   // columns p, q
+                    const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+                    a[(size_t)k * n + p] = c * akp - sn * akq;
+                    a[(size_t)k * n + q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {   // rows p, q
+                    const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+                    a[(size_t)p * n + k] = c * apk - sn * aqk;
+                    a[(size_t)q * n + k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
+                    v[(size_t)k * n + p] = c * vkp - sn * vkq;
+                    v[(size_t)k * n + q] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    w.resize(n);
+    for (int i = 0; i < n; ++i) w[i] = a[(size_t)i * n + i];
+}
+
+// K U = M U Lambda, U^T M U = I (M = L L^T, C = L^-1 K L^-T = Q Lambda Q^T, U = L^-T Q)
+static bool gen_eigen(int n, const std::vector<double> &K, const std::vector<double> &M, std::vector<double> &lam,
+                      std::vector<double> &U) {
+    std::vector<double> L((size_t)n * n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        double d = M[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) d -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+        if (!(d > 0.0)) return false;
+        L[(size_t)j * n + j] = std::sqrt(d);
+        for (int i = j + 1; i < n; ++i) {
+            double s = M[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) s -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+            L[(size_t)i * n + j] = s / L[(size_t)j * n + j];
+        }
+    }
+    // Y = L^-1 K (forward substitution on columns), C = Y L^-T (on rows)
+    std::vector<double> Y = K;
+    for (int c = 0; c < n; ++c)
+        for (int i = 0; i < n; ++i) {
+            double s = Y[(size_t)i * n + c];
+            for (int k = 0; k < i; ++k) s -= L[(size_t)i * n + k] * Y[(size_t)k * n + c];
+            Y[(size_t)i * n + c] = s / L[(size_t)i * n + i];
+        }
+    std::vector<double> Cm((size_t)n * n);
+    for (int r = 0; r < n; ++r)
+        for (int j = 0; j < n; ++j) {
+            double s = Y[(size_t)r * n + j];
+            for (int k = 0; k < j; ++k) s -= Cm[(size_t)r * n + k] * L[(size_t)j * n + k];
+            Cm[(size_t)r * n + j] = s / L[(size_t)j * n + j];
+        }
+    for (int i = 0; i < n; ++i)   // symmetrise rounding
+        for (int j = i + 1; j < n; ++j) {
+            const double a = 0.5 * (Cm[(size_t)i * n + j] + Cm[(size_t)j * n + i]);
+            Cm[(size_t)i * n + j] = Cm[(size_t)j * n + i] = a;
+        }
+    std::vector<double> Q;
+    jacobi_eigen(n, Cm, lam, Q);
+    // U = L^-T Q (back substitution on columns)
+    U.assign((size_t)n * n, 0.0);
+    for (int c = 0; c < n; ++c)
+        for (int i = n - 1; i >= 0; --i) {
+            double s = Q[(size_t)i * n + c];
+            for (int k = i + 1; k < n; ++k) s -= L[(size_t)k * n + i] * U[(size_t)k * n + c];
+            U[(size_t)i * n + c] = s / L[(size_t)i * n + i];
+        }
+    return true;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) fdm_gather_kernel(FdmArgs fa, const T *__restrict__ r, T *__restrict__ w,
+                                                         const int32_t *__restrict__ pred) {
+    if (pred && *pred == 0) return;
+    const int c = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= fa.tot[c]) return;
+    const int m0 = fa.m[c][0], m1 = fa.m[c][1];
+    const int i = (int)(t % m0), j = (int)((t / m0) % m1), k = (int)(t / ((int64_t)m0 * m1));
+    const int64_t g = (fa.lo[c][0] + i) + (int64_t)fa.n[0] * ((fa.lo[c][1] + j) + (int64_t)fa.n[1] * (fa.lo[c][2] + k));
+    w[c * fa.maxtot + t] = r[3 * (int64_t)fa.cpmap[g] + c];
+}
+
+// out = (mode-`axis` product with A) in, A[a][b] read from P as P[b m + a] (axis 0) or P[a m + b]
+template <typename T>
+__global__ void __launch_bounds__(256) fdm_mode_kernel(FdmArgs fa, const T *__restrict__ in, T *__restrict__ out,
+                                                       const T *__restrict__ P, int axis, int use_transpose_store,
+                                                       const int32_t *__restrict__ pred) {
+    if (pred && *pred == 0) return;
+    const int c = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= fa.tot[c]) return;
+    const int m0 = fa.m[c][0], m1 = fa.m[c][1], m2 = fa.m[c][2];
+    const int i = (int)(t % m0), j = (int)((t / m0) % m1), k = (int)(t / ((int64_t)m0 * m1));
+    const T *src = in + c * fa.maxtot;
+    const T *A = P + fa.uoff[c][axis];
+    double s = 0.0;
+    if (axis == 0) {
+        const T *line = src + (int64_t)m0 * (j + (int64_t)m1 * k);
+        for (int b = 0; b < m0; ++b) s += (double)A[use_transpose_store ? b * m0 + i : i * m0 + b] * (double)line[b];
+    } else if (axis == 1) {
+        const T *line = src + i + (int64_t)m0 * m1 * k;
+        for (int b = 0; b < m1; ++b) s += (double)A[use_transpose_store ? b * m1 + j : j * m1 + b] * (double)line[(int64_t)m0 * b];
+    } else {
+        const T *line = src + i + (int64_t)m0 * j;
+        for (int b = 0; b < m2; ++b)
+            s += (double)A[use_transpose_store ? b * m2 + k : k * m2 + b] * (double)line[(int64_t)m0 * m1 * b];
+    }
+    out[c * fa.maxtot + t] = (T)s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) fdm_scale_kernel(FdmArgs fa, T *__restrict__ w, const T *__restrict__ invD,
+                                                        const int32_t *__restrict__ pred) {
+    if (pred && *pred == 0) return;
+    const int c = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= fa.tot[c]) return;
+    w[c * fa.maxtot + t] *= invD[fa.doff[c] + t];
+}
+
+// z on every CP (0 outside the free boxes and on masked dofs); partial r.z per block
+template <typename T>
+__global__ void __launch_bounds__(256) fdm_scatter_kernel(FdmArgs fa, int64_t ncp, const T *__restrict__ w,
+                                                          const T *__restrict__ r, const uint8_t *__restrict__ mask,
+                                                          T *__restrict__ z, double *__restrict__ part,
+                                                          const int32_t *__restrict__ pred) {
+    if (pred && *pred == 0) return;
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ncp; g += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(g % fa.n[0]), j = (int)((g / fa.n[0]) % fa.n[1]), k = (int)(g / ((int64_t)fa.n[0] * fa.n[1]));
+        const int64_t cp = fa.cpmap[g];
+        for (int c = 0; c < 3; ++c) {
+            const int ii = i - fa.lo[c][0], jj = j - fa.lo[c][1], kk = k - fa.lo[c][2];
+            T v = T(0);
+            if (ii >= 0 && ii < fa.m[c][0] && jj >= 0 && jj < fa.m[c][1] && kk >= 0 && kk < fa.m[c][2] && mask[3 * cp + c])
+                v = w[c * fa.maxtot + ii + (int64_t)fa.m[c][0] * (jj + (int64_t)fa.m[c][1] * kk)];
+            z[3 * cp + c] = v;
+            acc += (double)r[3 * cp + c] * (double)v;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+template <typename T>
+static int fdm_apply_t(const yeti_fdm *f, const T *U, const T *Ut, const T *invD, T *wa, T *wb, const T *r,
+                       const uint8_t *mask, T *z, double *part, int nblocks, hipStream_t st, const int32_t *pred) {
+    const FdmArgs &fa = f->fa;
+    dim3 grid((unsigned)((fa.maxtot + 255) / 256), 3);
+    hipLaunchKernelGGL(fdm_gather_kernel<T>, grid, dim3(256), 0, st, fa, r, wa, pred);
+    // forward: (U^T) along axes 0, 1, 2; A = U^T: axis 0 reads U[b][a], axes 1, 2 read Ut[a][b]
+    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wa, wb, Ut, 0, 0, pred);
+    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, Ut, 1, 0, pred);
+    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wa, wb, Ut, 2, 0, pred);
+    hipLaunchKernelGGL(fdm_scale_kernel<T>, grid, dim3(256), 0, st, fa, wb, invD, pred);
+    // backward: U along axes 0, 1, 2
+    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, U, 0, 0, pred);
+    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wa, wb, U, 1, 0, pred);
+    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, U, 2, 0, pred);
+    hipLaunchKernelGGL(fdm_scatter_kernel<T>, dim3(nblocks), dim3(256), 0, st, fa, (int64_t)f->ncp, wa, r, mask, z,
+                       part, pred);
+    YETI_HIP(hipGetLastError());
+    return 0;
+}
+
+int fdm_apply(const yeti_fdm *f, const double *r, const uint8_t *mask, double *z, double *part, int nblocks,
+              hipStream_t st, const int32_t *pred) {
+    return fdm_apply_t<double>(f, f->U64.p, f->Ut64.p, f->invD64.p, f->wa64.p, f->wb64.p, r, mask, z, part, nblocks,
+                               st, pred);
+}
+int fdm_apply(const yeti_fdm *f, const float *r, const uint8_t *mask, float *z, double *part, int nblocks,
+              hipStream_t st, const int32_t *pred) {
+    return fdm_apply_t<float>(f, f->U32.p, f->Ut32.p, f->invD32.p, f->wa32.p, f->wb32.p, r, mask, z, part, nblocks,
+                              st, pred);
+}
+
+// Host set-up. mask_host: free-dof mask (3 nb_cp). precision 32 or 64. Device memory comes from
+// the current allocation scope.
+// Upper bound of the device bytes of fdm_create (no face removed), for counting set-ups.
+size_t fdm_bytes_bound(const yeti_mf *m, int precision) {
+    if (m->patches.size() != 1) return 0;
+    const MfPatchDev &pd = *m->patches[0];
+    const size_t w = precision == 64 ? 8 : 4;
+    const size_t n0 = pd.mp.ncp[0], n1 = pd.mp.ncp[1], n2 = pd.mp.ncp[2], tot = n0 * n1 * n2;
+    auto al = [](size_t b) { return (b + 255) / 256 * 256 + 256; };
+    const size_t mats = 3 * (n0 * n0 + n1 * n1 + n2 * n2) * w;
+    return 2 * al(mats) + al(3 * tot * w) + 2 * al(3 * tot * w);
+}
+
+int fdm_create(const yeti_mf *m, const uint8_t *mask_host, int precision, hipStream_t st, yeti_fdm **out) {
+    *out = nullptr;
+    if (m->patches.size() != 1)
+        return set_error(YETI_ERR_UNSUPPORTED_ELEMENT, "fast diagonalisation: single-patch handles only");
+    const MfPatchDev &pd = *m->patches[0];
+    const int p = pd.p, ng = pd.ng, np1 = p + 1;
+    auto *f = new yeti_fdm();
+    auto fail = [&](int code) {
+        delete f;
+        return code;
+    };
+    FdmArgs &fa = f->fa;
+    for (int d = 0; d < 3; ++d) fa.n[d] = pd.mp.ncp[d];
+    const int64_t ngrid = (int64_t)fa.n[0] * fa.n[1] * fa.n[2];
+    f->ncp = (int32_t)ngrid;
+    // fixed dofs: whole faces per component
+    for (int c = 0; c < 3; ++c) {
+        bool face_fixed[3][2];
+        for (int d = 0; d < 3; ++d)
+            for (int side = 0; side < 2; ++side) {
+                bool all = true;
+                for (int64_t g = 0; g < ngrid && all; ++g) {
+                    const int idx[3] = {(int)(g % fa.n[0]), (int)((g / fa.n[0]) % fa.n[1]), (int)(g / ((int64_t)fa.n[0] * fa.n[1]))};
+                    if (idx[d] != (side ? fa.n[d] - 1 : 0)) continue;
+                    if (mask_host[3 * (int64_t)pd.hcpmap[g] + c]) all = false;
+                }
+                face_fixed[d][side] = all;
+            }
+        for (int d = 0; d < 3; ++d) {
+            fa.lo[c][d] = face_fixed[d][0] ? 1 : 0;
+            fa.m[c][d] = fa.n[d] - fa.lo[c][d] - (face_fixed[d][1] ? 1 : 0);
+            if (fa.m[c][d] < 1) return fail(set_error(YETI_ERR_UNSUPPORTED_ELEMENT, "fast diagonalisation: no free dofs"));
+        }
+        for (int64_t g = 0; g < ngrid; ++g) {   // every fixed dof must lie on a fixed face
+            const int idx[3] = {(int)(g % fa.n[0]), (int)((g / fa.n[0]) % fa.n[1]), (int)(g / ((int64_t)fa.n[0] * fa.n[1]))};
+            bool inside = true;
+            for (int d = 0; d < 3; ++d) inside &= idx[d] >= fa.lo[c][d] && idx[d] < fa.lo[c][d] + fa.m[c][d];
+            if (inside && !mask_host[3 * (int64_t)pd.hcpmap[g] + c])
+                return fail(set_error(YETI_ERR_UNSUPPORTED_ELEMENT,
+                                      "fast diagonalisation: fixed dofs are not whole patch faces"));
+        }
+        fa.tot[c] = (int64_t)fa.m[c][0] * fa.m[c][1] * fa.m[c][2];
+    }
+    fa.maxtot = std::max(fa.tot[0], std::max(fa.tot[1], fa.tot[2]));
+    // 1D Gram matrices per direction on the whole index range
+    std::vector<double> K1[3], M1[3];
+    for (int d = 0; d < 3; ++d) {
+        const int n = fa.n[d], N = pd.mp.N[d];
+        K1[d].assign((size_t)n * n, 0.0);
+        M1[d].assign((size_t)n * n, 0.0);
+        for (int e = 0; e < N; ++e) {
+            const double h = pd.hhalf[3 * e + d];
+            for (int g = 0; g < ng; ++g) {
+                const size_t o = ((size_t)(pd.mp.boff[d] + e) * ng + g) * np1;
+                for (int r1 = 0; r1 < np1; ++r1)
+                    for (int r2 = 0; r2 < np1; ++r2) {
+                        const int i = e + r1, j = e + r2;   // basis span - p + r, span = p + e
+                        M1[d][(size_t)i * n + j] += h * pd.hgw[g] * pd.hB[o + r1] * pd.hB[o + r2];
+                        K1[d][(size_t)i * n + j] += h * pd.hgw[g] * pd.hdB[o + r1] * pd.hdB[o + r2];
+                    }
+            }
+        }
+    }
+    // patch-mean Jacobian from the face centroids of the control net
+    const double *X = nullptr;
+    std::vector<double> Xh((size_t)3 * m->nb_cp);
+    if (hipMemcpyAsync(Xh.data(), m->coords.p, Xh.size() * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return fail(set_error(YETI_ERR_DEVICE, "fast diagonalisation: coordinate copy failed"));
+    X = Xh.data();
+    double J[3][3];
+    for (int d = 0; d < 3; ++d) {
+        double c0[3] = {0, 0, 0}, c1[3] = {0, 0, 0};
+        int64_t n0 = 0, n1 = 0;
+        for (int64_t g = 0; g < ngrid; ++g) {
+            const int idx[3] = {(int)(g % fa.n[0]), (int)((g / fa.n[0]) % fa.n[1]), (int)(g / ((int64_t)fa.n[0] * fa.n[1]))};
+            const double *x = X + 3 * (int64_t)pd.hcpmap[g];
+            if (idx[d] == 0) {
+                for (int k = 0; k < 3; ++k) c0[k] += x[k];
+                ++n0;
+            }
+            if (idx[d] == fa.n[d] - 1) {
+                for (int k = 0; k < 3; ++k) c1[k] += x[k];
+                ++n1;
+            }
+        }
+        const double plen = pd.hU[d].back() - pd.hU[d].front();
+        for (int k = 0; k < 3; ++k) J[k][d] = (c1[k] / n1 - c0[k] / n0) / plen;
+    }
+    const double m00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], m01 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
+                 m02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double det = J[0][0] * m00 + J[0][1] * m01 + J[0][2] * m02;
+    if (!(std::fabs(det) > 0.0)) return fail(set_error(YETI_ERR_INVALID_ARGUMENT, "fast diagonalisation: degenerate patch"));
+    double Ji[3][3];   // Ji[d][k] = d u_d / d x_k
+    Ji[0][0] = m00 / det;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+    Ji[1][0] = m01 / det;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+    Ji[2][0] = m02 / det;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    const double lam = pd.mp.lam, mu = pd.mp.mu;
+    // per (component, direction): restricted Gram matrices, generalized eigenpairs
+    std::vector<double> Uall, Utall, Dall;
+    std::vector<double> eig[3][3];
+    for (int c = 0; c < 3; ++c)
+        for (int d = 0; d < 3; ++d) {
+            const int n = fa.n[d], mm = fa.m[c][d], lo = fa.lo[c][d];
+            std::vector<double> Ks((size_t)mm * mm), Ms((size_t)mm * mm);
+            for (int i = 0; i < mm; ++i)
+                for (int j = 0; j < mm; ++j) {
+                    Ks[(size_t)i * mm + j] = K1[d][(size_t)(lo + i) * n + lo + j];
+                    Ms[(size_t)i * mm + j] = M1[d][(size_t)(lo + i) * n + lo + j];
+                }
+            std::vector<double> U;
+            if (!gen_eigen(mm, Ks, Ms, eig[c][d], U))
+                return fail(set_error(YETI_ERR_INVALID_ARGUMENT, "fast diagonalisation: singular mass matrix"));
+            fa.uoff[c][d] = (int64_t)Uall.size();
+            Uall.insert(Uall.end(), U.begin(), U.end());
+            for (int i = 0; i < mm; ++i)
+                for (int j = 0; j < mm; ++j) Utall.push_back(U[(size_t)j * mm + i]);
+        }
+    for (int c = 0; c < 3; ++c) {
+        double alpha[3];
+        for (int d = 0; d < 3; ++d) {
+            const double gdd = Ji[d][0] * Ji[d][0] + Ji[d][1] * Ji[d][1] + Ji[d][2] * Ji[d][2];
+            alpha[d] = std::fabs(det) * (mu * gdd + (lam + mu) * Ji[d][c] * Ji[d][c]);
+        }
+        fa.doff[c] = (int64_t)Dall.size();
+        for (int k = 0; k < fa.m[c][2]; ++k)
+            for (int j = 0; j < fa.m[c][1]; ++j)
+                for (int i = 0; i < fa.m[c][0]; ++i)
+                {
+                    const double D = alpha[0] * eig[c][0][i] + alpha[1] * eig[c][1][j] + alpha[2] * eig[c][2][k];
+                    Dall.push_back(D > 1e-300 ? 1.0 / D : 0.0);   // 0 on a rigid mode of an unclamped component
+                }
+    }
+    fa.cpmap = pd.cpmap.p;
+    hipError_t e = hipSuccess;
+    if (precision == 64) {
+        e = f->U64.upload(Uall.data(), Uall.size());
+        if (e == hipSuccess) e = f->Ut64.upload(Utall.data(), Utall.size());
+        if (e == hipSuccess) e = f->invD64.upload(Dall.data(), Dall.size());
+        if (e == hipSuccess) e = f->wa64.alloc(3 * (size_t)fa.maxtot);
+        if (e == hipSuccess) e = f->wb64.alloc(3 * (size_t)fa.maxtot);
+    } else {
+        std::vector<float> u32(Uall.begin(), Uall.end()), ut32(Utall.begin(), Utall.end()), d32(Dall.begin(), Dall.end());
+        e = f->U32.upload(u32.data(), u32.size());
+        if (e == hipSuccess) e = f->Ut32.upload(ut32.data(), ut32.size());
+        if (e == hipSuccess) e = f->invD32.upload(d32.data(), d32.size());
+        if (e == hipSuccess) e = f->wa32.alloc(3 * (size_t)fa.maxtot);
+        if (e == hipSuccess) e = f->wb32.alloc(3 * (size_t)fa.maxtot);
+    }
+    if (e != hipSuccess) return fail(set_error(YETI_ERR_BUDGET, "fast diagonalisation: device memory"));
+    *out = f;
+    return 0;
+}
+
+void fdm_destroy(yeti_fdm *f) { delete f; }
 
 }  // namespace yeti
 
@@ -689,6 +1106,12 @@ static int32_t mf_create(const yeti_system_inputs *in, int32_t precisions, hipSt
                 }
             }
         }
+        pd->hB = Bt;
+        pd->hdB = dBt;
+        pd->hhalf = half;
+        pd->hgw.assign(gwt, gwt + ng);
+        pd->hcpmap = cpmap;
+        for (int d = 0; d < 3; ++d) pd->hU[d].assign(U[d], U[d] + nkv[d]);
         for (int d = 0; d < 3; ++d) {
             pd->mp.N[d] = N[d];
             pd->mp.ncp[d] = ncp[d];

@@ -383,6 +383,13 @@ namespace yeti {
 
 int mf_apply_pred(const yeti_mf *m, const double *x, double *y, hipStream_t st, const int32_t *pred);
 int mf_apply_pred(const yeti_mf *m, const float *x, float *y, hipStream_t st, const int32_t *pred);
+int fdm_create(const yeti_mf *m, const uint8_t *mask_host, int precision, hipStream_t st, yeti_fdm **out);
+void fdm_destroy(yeti_fdm *f);
+size_t fdm_bytes_bound(const yeti_mf *m, int precision);
+int fdm_apply(const yeti_fdm *f, const double *r, const uint8_t *mask, double *z, double *part, int nblocks,
+              hipStream_t st, const int32_t *pred);
+int fdm_apply(const yeti_fdm *f, const float *r, const uint8_t *mask, float *z, double *part, int nblocks,
+              hipStream_t st, const int32_t *pred);
 
 // sc[] slots (doubles), in the order of yeti_ir_status
 enum : int { S_RZ = 0, S_PQ, S_RR, S_RZN, S_BB, S_RRT, S_RRMAX, S_TOL2, S_RRT_PREV, S_NSC = 16 };
@@ -592,6 +599,7 @@ struct yeti_ir {
     yeti::DevBuf<double> vr64, vy64, vz64, vp64, vq64;   // iteration vectors, FP64
     yeti::DevBuf<float> vr32, vy32, vz32, vp32, vq32;    // iteration vectors, FP32
     yeti::DevBuf<int32_t> flags;
+    yeti_fdm *fdm = nullptr;  // fast diagonalisation (else 3x3 block-Jacobi)
     const double *b = nullptr;
     double *x = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -599,12 +607,22 @@ struct yeti_ir {
     ~yeti_ir() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (fdm) yeti::fdm_destroy(fdm);
     }
 };
 
 namespace yeti {
 
 constexpr int32_t IR_MAX_STAGNANT = 4;
+
+// z = M^-1 r and the partials of r.z (nbc blocks), predicated
+template <typename T>
+static int ir_precond(yeti_ir *s, const T *inv, const T *r, T *z, hipStream_t st, const int32_t *pred) {
+    if (s->fdm) return fdm_apply(s->fdm, r, s->mask, z, s->part.p, (int)s->nbc, st, pred);
+    hipLaunchKernelGGL(pred_precond_kernel<T>, dim3(s->nbc), dim3(256), 0, st, s->ncp, inv, r, z, s->part.p, pred);
+    YETI_HIP(hipGetLastError());
+    return 0;
+}
 
 // the launch sequence of one step (captured into the session's graph)
 template <typename T>
@@ -629,7 +647,8 @@ static int ir_step_launches(yeti_ir *s, T *r, T *y, T *z, T *p, T *q, const T *i
         hipLaunchKernelGGL(ir_update_done_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->rtol * s->rtol,
                            s->max_updates, IR_MAX_STAGNANT);
         // preconditioner and direction
-        hipLaunchKernelGGL(pred_precond_kernel<T>, dim3(s->nbc), dim3(256), 0, st, s->ncp, inv, r, z, s->part.p, act);
+        rc = ir_precond<T>(s, inv, r, z, st, act);
+        if (rc) return rc;
         hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nbc, s->sc.p + S_RZN, act);
         hipLaunchKernelGGL(pred_p_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->sc.p, p, z, act);
         hipLaunchKernelGGL(ir_roll_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->max_iters);
@@ -651,7 +670,8 @@ static int ir_begin_launches(yeti_ir *s, T *r, T *y, T *z, T *p, const T *inv, h
     hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_BB,
                        (const int32_t *)nullptr);
     hipLaunchKernelGGL(ir_begin_flags_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->rtol * s->rtol);
-    hipLaunchKernelGGL(pred_precond_kernel<T>, dim3(s->nbc), dim3(256), 0, st, s->ncp, inv, r, z, s->part.p, act);
+    int rc = ir_precond<T>(s, inv, r, z, st, act);
+    if (rc) return rc;
     hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nbc, s->sc.p + S_RZN, act);
     hipLaunchKernelGGL(ir_first_p_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, z, p, s->flags.p);
     hipLaunchKernelGGL(ir_first_rz_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p);
@@ -663,9 +683,9 @@ static int ir_begin_launches(yeti_ir *s, T *r, T *y, T *z, T *p, const T *inv, h
 
 extern "C" {
 
-int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask, int32_t precision, double rtol, double delta,
-                       int32_t chunk, int32_t max_iters, int32_t max_updates, void *arena, size_t arena_bytes,
-                       void *stream, yeti_ir **out, size_t *used) {
+int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask, int32_t precision, int32_t precond, double rtol,
+                       double delta, int32_t chunk, int32_t max_iters, int32_t max_updates, void *arena,
+                       size_t arena_bytes, void *stream, yeti_ir **out, size_t *used) {
     if (!m || !out) return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: null argument");
     *out = nullptr;
     if (precision != 32 && precision != 64)
@@ -711,12 +731,22 @@ int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask, int32_t preci
             if (e == hipSuccess) e = v->alloc(n);
     }
     if (e != hipSuccess) return fail(set_error(YETI_ERR_BUDGET, "ir: device memory: %s", hipGetErrorString(e)));
-    if (used) *used = a.used;
+    if (precond != 0 && precond != 1) return fail(set_error(YETI_ERR_INVALID_ARGUMENT, "ir: unknown preconditioner"));
     if (!arena) {
+        if (used) *used = a.used + (precond == 1 ? fdm_bytes_bound(m, precision) : 0);
         delete s;   // counting only
         return 0;
     }
-    // block-Jacobi preconditioner of the free dofs
+    if (precond == 1) {
+        std::vector<uint8_t> mh(n);
+        if (hipMemcpyAsync(mh.data(), free_mask, n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return fail(set_error(YETI_ERR_DEVICE, "ir: mask copy failed"));
+        int rcf = fdm_create(m, mh.data(), precision, st, &s->fdm);
+        if (rcf) return fail(rcf);
+    }
+    if (used) *used = a.used;
+    // block-Jacobi preconditioner of the free dofs (also kept with fast diagonalisation: cheap)
     int rc = yeti_mf_diag_blocks_f64(m, s->blk.p, st);
     if (rc) return fail(rc);
     hipLaunchKernelGGL(block_inverse_kernel, dim3((s->ncp + 255) / 256), dim3(256), 0, st, s->ncp, s->blk.p, free_mask,

@@ -14,8 +14,8 @@ routines' counting mode, plus the solve session and the request's vectors -- fit
 range of the arena; its handles are created inside that range (no hipMalloc/hipFree while
 serving) and the range is returned when the request completes.
 
-Solves: ``precision="mixed"`` is block-Jacobi PCG with FP32 vectors, operator and
-preconditioner (FP64 dots) whose FP64 solution and true residual are refreshed by residual
+Solves: ``precision="mixed"`` is PCG (fast-diagonalisation or block-Jacobi preconditioner)
+with FP32 vectors, operator and preconditioner (FP64 dots) whose FP64 solution and true residual are refreshed by residual
 replacement with the FP64 apply each time the FP32 residual has fallen by ``delta``
 (refinement that keeps the Krylov space); ``precision="fp64"`` runs the same PCG in FP64.
 Both stop on the FP64 residual. Each step (``chunk`` iterations) is one hipGraph launch per
@@ -92,12 +92,19 @@ class ArenaAllocator:
         self.used -= n
 
 
-def ir_session_bytes(nb_cp, precision):
-    """Device bytes of yeti_ir_create (mirrors its allocations, each ALIGN-aligned)."""
+def ir_session_bytes(ncp_dims, precision, precond="fdm"):
+    """Device bytes of yeti_ir_create (mirrors its allocations, each ALIGN-aligned; the fast
+    diagonalisation part is the bound of fdm_bytes_bound, no face removed)."""
+    nb_cp = int(np.prod(ncp_dims))
     n = 3 * nb_cp
     w = 8 if precision == 64 else 4
     sizes = [9 * nb_cp * 8, n * 8, NRED * 8, 16 * 8, 8 * 4, 9 * nb_cp * w] + [n * w] * 5
-    return sum(_align(s) for s in sizes)
+    total = sum(_align(s) for s in sizes)
+    if precond == "fdm":
+        al = lambda b: _align(b) + ALIGN   # noqa: E731
+        mats = 3 * sum(int(d) * int(d) for d in ncp_dims) * w
+        total += 2 * al(mats) + al(3 * nb_cp * w) + 2 * al(3 * nb_cp * w)
+    return total
 
 
 # ------------------------------------------------------------------------------------------
@@ -118,6 +125,10 @@ class Request:
             self.args = make_box_patch(nel=self.nel, p=self.p, E=self.E, nu=self.nu, traction=self.traction,
                                        seed=self.seed).args
         return self.args
+
+    @property
+    def ncp_dims(self):
+        return tuple(self.nel[d] + self.p for d in range(3))
 
     @property
     def nb_cp(self):
@@ -164,8 +175,8 @@ class _Active:
 class SolveServer:
     """Runs a list of requests against one arena of `budget_bytes`."""
 
-    def __init__(self, budget_bytes, precision="mixed", nstreams=8, chunk=8, rtol=1e-10, delta=0.1,
-                 max_iters=100000, max_updates=400, use_graph=True, prep_threads=2):
+    def __init__(self, budget_bytes, precision="mixed", nstreams=8, chunk=8, rtol=1e-8, delta=0.1,
+                 max_iters=100000, max_updates=400, use_graph=True, prep_threads=2, precond="fdm"):
         import torch
         _lib.require_device()
         if precision not in ("mixed", "fp64"):
@@ -176,6 +187,9 @@ class SolveServer:
         self.chunk, self.rtol, self.delta = int(chunk), float(rtol), float(delta)
         self.max_iters, self.max_updates = int(max_iters), int(max_updates)
         self.use_graph = bool(use_graph)
+        if precond not in ("fdm", "jacobi"):
+            raise ValueError("precond must be 'fdm' or 'jacobi'")
+        self.precond = precond
         self.budget = int(budget_bytes)
         self.arena_buf = torch.empty(self.budget, dtype=torch.uint8, device="cuda")
         self.base = self.arena_buf.data_ptr()
@@ -193,7 +207,8 @@ class SolveServer:
         _lib.check(L.yeti_mf_create_in(C.byref(inputs), 1 | 2 if self.inner == 32 else 1, None, 0, None, C.byref(h),
                                        C.byref(used_m)))
         n = 3 * req.nb_cp
-        parts = [_align(used_a.value) + ALIGN, _align(used_m.value) + ALIGN, ir_session_bytes(req.nb_cp, self.inner),
+        parts = [_align(used_a.value) + ALIGN, _align(used_m.value) + ALIGN,
+                 ir_session_bytes(req.ncp_dims, self.inner, self.precond) + ALIGN,
                  _align(8 * n), _align(8 * n), _align(n)]
         return parts
 
@@ -242,7 +257,8 @@ class SolveServer:
                 a.mask.copy_(torch.from_numpy(m), non_blocking=False)
             _lib.check(L.yeti_asm_run_rhs(a.asm, C.c_void_p(a.F.data_ptr()), sh))
             h = C.c_void_p()
-            _lib.check(L.yeti_ir_create(a.mf, C.c_void_p(a.mask.data_ptr()), self.inner, self.rtol, self.delta,
+            _lib.check(L.yeti_ir_create(a.mf, C.c_void_p(a.mask.data_ptr()), self.inner,
+                                        1 if self.precond == "fdm" else 0, self.rtol, self.delta,
                                         self.chunk, self.max_iters, self.max_updates, ptr(o_ir), parts[2], sh,
                                         C.byref(h), C.byref(used)))
             a.ir = h
