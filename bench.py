@@ -9,9 +9,13 @@ right-hand side F, written to HBM. Inputs (control points, connectivity, knots, 
 resident in HBM before the timed region starts.
 
 Prints one JSON line on stdout; progress goes to stderr. ``roofline`` describes the dominant
-kernel (the element kernel with index emission), timed with HIP events on the stream it runs
-on; ``cpu_baseline`` is YETI's own OpenMP Fortran routine (oracle/_ref) on a bounded sample of
-the same workload, on this host's cores.
+kernel (the element kernel with index emission), each launch timed with HIP events on the
+stream it runs on; ``cpu_baseline`` is YETI's own OpenMP Fortran routine (oracle/_ref) on a
+bounded sample of the same workload, on this host's cores. Extra objects report the other
+parts of the metric on the same GPU: ``serving`` (a Poisson sample of the serving workload:
+solves/s and p50/p99 latency, mixed precision vs FP64, vs plain one-at-a-time FP64 solves),
+``gradient`` (compliance shape gradient of a 32^3 p=2 patch against central differences) and
+``mixed_precision`` (128^3 p=2 matrix-free FP32 solve with FP64 refinement vs FP64).
 """
 from __future__ import annotations
 
@@ -54,6 +58,147 @@ def cpu_baseline(p, sample_nel, threads):
                       % (sample_nel[0], sample_nel[1], sample_nel[2], p, ne, dt)}
 
 
+def read_traffic(path):
+    """HBM bytes per launch of the element kernel from a committed rocprofv3 PMC summary
+    (line 'traffic_bytes_per_launch: N'), or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
+def bench_serving(n, rate, seed, budget, plain_n):
+    import numpy as np
+    import torch
+    from yeti_amd.serving import SolveServer, request_stream
+    reqs = request_stream(n=n, rate=rate, seed=seed)
+    t0 = time.perf_counter()
+    for r in reqs:
+        r.model()
+    log("serving: %d requests generated in %.1f s (mean %.0f dofs)"
+        % (n, time.perf_counter() - t0, np.mean([3 * r.nb_cp for r in reqs])))
+    out = {}
+    xs = {}
+    for prec in ("mixed", "fp64"):
+        srv = SolveServer(budget_bytes=budget, precision=prec, nstreams=8, chunk=16, rtol=1e-8)
+        rep = srv.run(reqs, keep_solutions=True, realtime=True)
+        xs[prec] = rep.pop("x")
+        out[prec] = {"solves_per_s": rep["solves_per_s"], "latency_s": rep["latency_s"],
+                     "completed": rep["completed"], "failed": rep["failed"],
+                     "mean_iterations": float(np.mean([d["iters"] for d in rep["requests"].values()])),
+                     "peak_arena_gb": rep["peak_arena_bytes"] / 1e9}
+        log("serving %s: %.2f solves/s, p50 %.3f s, p99 %.3f s, failed %d"
+            % (prec, rep["solves_per_s"], rep["latency_s"]["p50"], rep["latency_s"]["p99"], rep["failed"]))
+        srv.close()
+        del srv
+        torch.cuda.empty_cache()
+    err = max(float(np.linalg.norm(xs["mixed"][k] - xs["fp64"][k]) / np.linalg.norm(xs["fp64"][k])) for k in xs["fp64"])
+    # plain FP64 GPU path: each request alone, one after the other, eager launches, one stream
+    sub = [r for r in reqs[:plain_n]]
+    lat = []
+    t0 = time.perf_counter()
+    for r in sub:
+        srv = SolveServer(budget_bytes=budget // 4, precision="fp64", nstreams=1, chunk=16, rtol=1e-8, use_graph=False)
+        ts = time.perf_counter()
+        srv.run([r], realtime=False)
+        lat.append(time.perf_counter() - ts)
+        srv.close()
+        del srv
+    dt = time.perf_counter() - t0
+    out["plain_fp64"] = {"solves_per_s": len(sub) / dt, "requests": len(sub),
+                         "latency_s": {"p50": float(np.median(lat)), "max": float(max(lat))}}
+    log("plain fp64 (one at a time, %d requests): %.2f solves/s" % (len(sub), len(sub) / dt))
+    out["max_rel_err_mixed_vs_fp64"] = err
+    out["speedup_mixed_vs_plain_fp64"] = out["mixed"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]
+    out["speedup_mixed_vs_served_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    out["workload"] = ("%d requests, Poisson arrivals at %.1f/s, elements per direction log-uniform in [4,48], "
+                       "p in {2,3,4}, E~U[1e3,1e5], nu~U[0.2,0.4], traction~U[0.1,10], solved to 1e-8 "
+                       "(FP64 residual), fast-diagonalisation PCG, arena %.0f GB; plain FP64: the first %d "
+                       "requests solved alone, eager launches" % (n, rate, budget / 1e9, len(sub)))
+    return out
+
+
+def bench_gradient(nfd, seed=7):
+    import numpy as np
+    import torch
+    from yeti_amd.geometry import make_box_patch
+    from yeti_amd.gradient import CompliancePatch, compliance
+    bp = make_box_patch(nel=(32, 32, 32), p=2)
+    patch = CompliancePatch(bp.args, solver="csr", rtol=1e-12)
+    X = patch.coords0.clone().cuda()
+    Xg = X.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = compliance(Xg, patch)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    J.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    g = Xg.grad.detach().cpu().numpy()
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(X.numel(), nfd, replace=False)
+    h = 1e-6
+    errs = []
+    for i in idx:
+        Xp = X.clone(); Xp[i] += h
+        Xm = X.clone(); Xm[i] -= h
+        fd = (compliance(Xp, patch).item() - compliance(Xm, patch).item()) / (2 * h)
+        errs.append(abs(fd - g[i]) / max(abs(g).max(), 1e-300))
+    compliance(X, patch)
+    log("gradient: forward %.3f s, backward %.3f s, max |fd - g| / max|g| %.2e over %d coordinates"
+        % (t1 - t0, t2 - t1, max(errs), nfd))
+    return {"workload": "32x32x32 p=2 cantilever (117,912 dofs), J = F.u, FP64 assembly + CSR Jacobi PCG to 1e-12",
+            "forward_s": t1 - t0, "backward_s": t2 - t1, "J": J.item(), "fd_step": h, "fd_coordinates": nfd,
+            "max_abs_err_vs_fd_rel_to_max_grad": max(errs), "pcg_iterations": patch.iters}
+
+
+def bench_mixed_precision(nel):
+    import numpy as np
+    import torch
+    from yeti_amd.assembly import Assembler
+    from yeti_amd.geometry import make_box_patch
+    from yeti_amd.serving import SolveServer, Request
+    bp = make_box_patch(nel=(nel, nel, nel), p=2)
+    req = Request(0, 0.0, (nel, nel, nel), 2, 210000.0, 0.3, -1.0, 20261202)
+    req.args = bp.args
+    out = {}
+    xs = {}
+    log("%d^3 p=2: model generated (%d elements, %d dofs)" % (nel, bp.nb_elem, bp.nb_dof_tot))
+    for prec in ("fp64", "mixed"):
+        srv = SolveServer(budget_bytes=64 << 30, precision=prec, nstreams=1, chunk=32, rtol=1e-10, max_iters=20000)
+        t0 = time.perf_counter()
+        rep = srv.run([req], keep_solutions=True, realtime=False,
+                      step_log=lambda rid, st: log("  %s: %d iterations, relres %.2e, %.1f s"
+                                                   % (prec, st["iters"], st["relres"], time.perf_counter() - t0)),
+                      step_every=10)
+        dt = time.perf_counter() - t0
+        d = rep["requests"][0]
+        xs[prec] = rep["x"][0]
+        out[prec] = {"seconds": dt, "iterations": d["iters"], "replacements": d["updates"], "relres": d["relres"],
+                     "failed": d["failed"], "stopped_at_fp64_floor": d["floor"]}
+        log("%d^3 p=2 %s: %.2f s, %d iterations, relres %.2e" % (nel, prec, dt, d["iters"], d["relres"]))
+        srv.close()
+        del srv
+        torch.cuda.empty_cache()
+    asm = Assembler(bp.args)
+    F = asm.run_rhs().cpu().numpy()
+    u64, u32 = xs["fp64"], xs["mixed"]
+    J64, J32 = float(F @ u64), float(F @ u32)
+    out["displacement_rel_err"] = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
+    out["compliance_rel_err"] = abs(J32 - J64) / abs(J64)
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    out["workload"] = ("%d^3 p=2 cantilever (%d dofs), matrix-free apply, fast-diagonalisation PCG; mixed: FP32 "
+                       "vectors/apply/preconditioner with FP64 residual replacement to 1e-10; errors against the "
+                       "FP64 solve of the same operator (itself within 1e-12 of the assembled K)"
+                       % (nel, bp.nb_dof_tot))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +209,14 @@ def main():
     ap.add_argument("--cpu-sample", default="32,16,16", help="elements per direction of the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="offered requests per second")
+    ap.add_argument("--serve-plain", type=int, default=12, help="requests solved one at a time (plain FP64)")
+    ap.add_argument("--grad-fd", type=int, default=10, help="coordinates checked by central differences")
+    ap.add_argument("--mp-nel", type=int, default=128, help="elements per direction of the mixed-precision solve")
+    ap.add_argument("--only-assembly", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="run only a 2 GiB device copy and one element-kernel run (for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the assembly shards by element "
@@ -80,6 +233,20 @@ def main():
         sys.exit(1)
     p = args.p
     nel = (args.nel, args.nel, args.nel)
+    if args.pmc_probe:
+        # calibration copy of known size (2 GiB read + 2 GiB written), then one run of the
+        # element kernels on the bench workload; counters per dispatch come from rocprofv3
+        a = torch.ones(2 << 30, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        bp = make_box_patch(nel=nel, p=p)
+        asm = Assembler(bp.f2py_args())
+        K, r, c = asm.alloc_coo(True)
+        asm.run_matrix(K, r, c)
+        torch.cuda.synchronize()
+        log("pmc probe done: copy of %d bytes, %d elements" % (a.numel(), bp.nb_elem))
+        return
     t0 = time.perf_counter()
     bp = make_box_patch(nel=nel, p=p)
     log("generated %s p=%d in %.1f s: %d elements, %d dofs, nb_data %d"
@@ -108,28 +275,31 @@ def main():
     nq = (p + 1) ** 3
     e_ent = 9 * n * (n - 1) // 2 + 6 * n
     bytes_elem = 16 * e_ent + 12 * n + 12            # values + row + col out; IEN + weights + spans in
-    bytes_launch = bp.nb_elem * bytes_elem + 24 * bp.nb_cp
     flops_elem = 9 * n * (n + 1) * nq
     stream = torch.cuda.current_stream()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     nrep = max(3, args.steps)
-    ev0.record(stream)
+    asm.set_timing(True)
+    launch_ms = []
     for _ in range(nrep):
         asm.run_matrix(K, r, c)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    kern_ms = ev0.elapsed_time(ev1) / nrep
+        launch_ms += asm.kernel_times()
+    nlaunch = len(asm.kernel_times())
+    kern_ms = sum(launch_ms) / len(launch_ms)                    # average launch duration
+    elems_per_launch = bp.nb_elem / nlaunch
+    bytes_launch = elems_per_launch * bytes_elem + 24 * bp.nb_cp / nlaunch
     achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
-    # values-only launch: the MFMA-bound variant (indices are static and emitted once)
-    ev0.record(stream)
+    launch_ms = []
     for _ in range(nrep):
         asm.run_values(K)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    val_ms = ev0.elapsed_time(ev1) / nrep
-    tflops = bp.nb_elem * flops_elem / (val_ms * 1e-3) / 1e12
-    log("element kernel: %.3f ms (%.0f GB/s); values-only %.3f ms (%.1f TFLOP/s FP64)"
-        % (kern_ms, achieved, val_ms, tflops))
+        launch_ms += asm.kernel_times()
+    val_ms = sum(launch_ms) / len(launch_ms)
+    asm.set_timing(False)
+    tflops = elems_per_launch * flops_elem / (val_ms * 1e-3) / 1e12
+    log("element kernel: %d launches of %d elements, %.3f ms each (%.0f GB/s); values-only %.3f ms (%.1f TFLOP/s FP64)"
+        % (nlaunch, elems_per_launch, kern_ms, achieved, val_ms, tflops))
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "elem_stiffness_64cube_p%d_traffic.md" % p))
+    del K, r, c, F
+    torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -160,17 +330,22 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "elem_stiffness_kernel<%d,%d,true>" % (p, p + 1),
-            "kernel_ms": kern_ms, "algorithmic_bytes_per_launch": bytes_launch,
+            "kernel_ms": kern_ms, "launches_per_step": nlaunch, "elements_per_launch": elems_per_launch,
+            "algorithmic_bytes_per_launch": bytes_launch,
         },
         "mfma_values_only": {
             "kernel": "elem_stiffness_kernel<%d,%d,false>" % (p, p + 1), "kernel_ms": val_ms,
             "achieved": tflops, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
-            "frac": tflops / FP64_PEAK_TFS, "flops_per_launch": bp.nb_elem * flops_elem,
+            "frac": tflops / FP64_PEAK_TFS, "flops_per_launch": elems_per_launch * flops_elem,
         },
         "cpu_baseline": cpu,
     }
+    if not args.only_assembly:
+        out["serving"] = bench_serving(args.serve_requests, args.serve_rate, 20261202, 200 << 30, args.serve_plain)
+        out["gradient"] = bench_gradient(args.grad_fd)
+        out["mixed_precision"] = bench_mixed_precision(args.mp_nel)
     print(json.dumps(out), flush=True)
 
 

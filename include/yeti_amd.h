@@ -148,6 +148,11 @@ int32_t yeti_asm_update_coords(yeti_asm *h, const double *coords_dev, void *stre
  * (device, 3 nb_cp). Face loads 10..99 only (YETI's own gradient leaves this term as a stub,
  * optim/gradUELMAT_solid.f90:463-477); body loads return YETI_ERR_UNSUPPORTED_ELEMENT. */
 int32_t yeti_asm_load_shape_grad(yeti_asm *h, const double *u_dev, double *g_dev, void *stream);
+/* Measurement: with timing on, each element-kernel launch of the next runs is bracketed by HIP
+ * events on its stream; yeti_asm_kernel_times waits for them and returns the durations (ms) of
+ * the last run's launches (up to cap) and their count. */
+int32_t yeti_asm_set_timing(yeti_asm *h, int32_t on);
+int32_t yeti_asm_kernel_times(yeti_asm *h, float *ms, int32_t cap, int32_t *n);
 void    yeti_asm_destroy(yeti_asm *h);
 
 /* ---- CSR of K_side + K_side^T (device) --------------------------------------------------
@@ -230,9 +235,10 @@ int32_t yeti_asm_create_in(const yeti_system_inputs *in, int32_t layout, void *a
  * solution and the true residual kept in FP64 and the residual replaced by mask (b - K x) from
  * the FP64 apply whenever the iterated residual has fallen by `delta` since the last replacement
  * (mixed-precision refinement that keeps the Krylov space); precision 64: the same in FP64.
- * Stops at ||b - K x||_2 <= rtol ||b||_2 on the FP64 residual; `failed` is set at max_iters,
- * max_updates, or when four replacements in a row do not halve the true residual (the FP64
- * rounding floor of an ill-conditioned system). Control state lives on the device:
+ * Stops at ||b - K x||_2 <= rtol ||b||_2 on the FP64 residual. A replacement is also forced every
+ * 256 iterations; when four replacements in a row do not halve the true residual the session
+ * stops with `floor` set (the FP64 rounding floor of an ill-conditioned system lies above rtol);
+ * `failed` is set at max_iters or max_updates. Control state lives on the device:
  * yeti_ir_step enqueues `chunk` iterations, every kernel predicated on device flags, so a step is
  * a fixed launch sequence; with use_graph it is captured once into a hipGraph (on a non-null
  * stream) and replayed. The iterates do not depend on how steps of different sessions
@@ -242,9 +248,9 @@ int32_t yeti_asm_create_in(const yeti_system_inputs *in, int32_t layout, void *a
 typedef struct yeti_ir yeti_ir;
 typedef struct yeti_fdm yeti_fdm;   /* fast diagonalisation data of a session (opaque) */
 typedef struct yeti_ir_status {
-    int32_t active, update_pending, stagnant, done, failed, iters, spare, updates;
-    double rz, pq, rr, rz_new, bb, rr_true, rr_max, tol2, rr_true_best;
-    double reserved[7];
+    int32_t active, update_pending, stagnant, done, failed, iters, floor, updates;
+    double rz, pq, rr, rz_new, bb, rr_true, rr_max, tol2, rr_true_best, last_update_iter;
+    double reserved[6];
 } yeti_ir_status;
 
 /* precond 0: 3x3 nodal block-Jacobi; 1: fast diagonalisation (single-patch handles whose fixed

@@ -59,6 +59,17 @@ class Assembler:
             raise ValueError("coords must be a contiguous CUDA float64 tensor of %d entries" % (3 * self.nb_cp))
         _lib.check(_lib.lib().yeti_asm_update_coords(self._h, C.c_void_p(c.data_ptr()), _stream_handle(stream)))
 
+    def set_timing(self, on=True):
+        """Bracket each element-kernel launch of later runs with HIP events on its stream."""
+        _lib.check(_lib.lib().yeti_asm_set_timing(self._h, 1 if on else 0))
+
+    def kernel_times(self, cap=4096):
+        """Durations (ms) of the element-kernel launches of the last run (timing on)."""
+        buf = (C.c_float * cap)()
+        n = C.c_int32(0)
+        _lib.check(_lib.lib().yeti_asm_kernel_times(self._h, buf, cap, C.byref(n)))
+        return [float(buf[i]) for i in range(min(n.value, cap))]
+
     def load_shape_grad(self, u, out=None, stream=None):
         """g[3 I + j] = 2 u^T dF/dX_Ij for the face loads (FP64)."""
         if u.dtype != torch.float64 or u.numel() != 3 * self.nb_cp or not u.is_cuda:

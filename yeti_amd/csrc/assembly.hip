@@ -595,9 +595,14 @@ struct yeti_asm {
     yeti::DevBuf<double> adlmag, add_infos, nodal_dist, rhs_buf, grad_buf;
     int32_t nb_n_dist = 1;
     bool body_loads = false;   // any element carries a body load (101 / 102)
+    // optional per-launch timing of the element kernel (HIP events on the launch stream)
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    int32_t ev_used = 0;
     int32_t n_loaded_total = 0;
     ~yeti_asm() {
         for (auto *p : patches) delete p;
+        for (auto e : ev) (void)hipEventDestroy(e);
     }
 };
 
@@ -651,8 +656,21 @@ static int device_cu_count() {
 // (12 doubles per Gauss point: 400 MB at p = 3) while keeping every CU busy.
 constexpr int32_t ELEM_BATCH = 65536;
 
+// event pair k of the handle's timing record (created on first use)
+static int timing_events(yeti_asm *h, int k, hipEvent_t *a, hipEvent_t *b) {
+    while ((int)h->ev.size() < 2 * (k + 1)) {
+        hipEvent_t e;
+        YETI_HIP(hipEventCreate(&e));
+        h->ev.push_back(e);
+    }
+    *a = h->ev[2 * k];
+    *b = h->ev[2 * k + 1];
+    return 0;
+}
+
 template <int P, int NG>
-static int launch_elem(PatchDev &pd, const double *coords, double *Kd, int32_t *Kr, int32_t *Kc, hipStream_t st) {
+static int launch_elem(yeti_asm *h, PatchDev &pd, const double *coords, double *Kd, int32_t *Kr, int32_t *Kc,
+                       hipStream_t st) {
     using C = ElemCfg<P, NG>;
     if (pd.n_active == 0) return 0;
     const int32_t batch = std::min(pd.n_active, ELEM_BATCH);
@@ -678,6 +696,12 @@ static int launch_elem(PatchDev &pd, const double *coords, double *Kd, int32_t *
         hipLaunchKernelGGL((elem_geo_kernel<P, NG>), dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, pa, pd.geo.p);
         // persistent workgroups: one per CU (the kernel's LDS and registers allow one)
         const int grid = std::min(pa.n_active, cus);
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (h->timing) {
+            int rc = timing_events(h, h->ev_used++, &ea, &eb);
+            if (rc) return rc;
+            YETI_HIP(hipEventRecord(ea, st));
+        }
         if (Kr)
             hipLaunchKernelGGL((elem_stiffness_kernel<P, NG, true>), dim3(grid), dim3(C::NTH), 0, st, pa,
                                (const double *)pd.geo.p, Kd, Kr, Kc);
@@ -685,11 +709,13 @@ static int launch_elem(PatchDev &pd, const double *coords, double *Kd, int32_t *
             hipLaunchKernelGGL((elem_stiffness_kernel<P, NG, false>), dim3(grid), dim3(C::NTH), 0, st, pa,
                                (const double *)pd.geo.p, Kd, Kr, Kc);
         YETI_HIP(hipGetLastError());
+        if (h->timing) YETI_HIP(hipEventRecord(eb, st));
     }
     return 0;
 }
 
 static int run_elements(yeti_asm *h, double *Kd, int32_t *Kr, int32_t *Kc, hipStream_t st) {
+    h->ev_used = 0;
     for (auto &zr : h->zero_ranges) {
         int64_t n = zr.second;
         int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
@@ -702,10 +728,10 @@ static int run_elements(yeti_asm *h, double *Kd, int32_t *Kr, int32_t *Kc, hipSt
         int rc = 0;
         int key = pd->p[0] * 16 + pd->ng;
         switch (key) {
-        case 1 * 16 + 2: rc = launch_elem<1, 2>(*pd, h->coords.p, Kd, Kr, Kc, st); break;
-        case 2 * 16 + 3: rc = launch_elem<2, 3>(*pd, h->coords.p, Kd, Kr, Kc, st); break;
-        case 3 * 16 + 4: rc = launch_elem<3, 4>(*pd, h->coords.p, Kd, Kr, Kc, st); break;
-        case 4 * 16 + 5: rc = launch_elem<4, 5>(*pd, h->coords.p, Kd, Kr, Kc, st); break;
+        case 1 * 16 + 2: rc = launch_elem<1, 2>(h, *pd, h->coords.p, Kd, Kr, Kc, st); break;
+        case 2 * 16 + 3: rc = launch_elem<2, 3>(h, *pd, h->coords.p, Kd, Kr, Kc, st); break;
+        case 3 * 16 + 4: rc = launch_elem<3, 4>(h, *pd, h->coords.p, Kd, Kr, Kc, st); break;
+        case 4 * 16 + 5: rc = launch_elem<4, 5>(h, *pd, h->coords.p, Kd, Kr, Kc, st); break;
         default: return set_error(YETI_ERR_UNSUPPORTED_DEGREE, "no kernel for p=%d ng=%d", pd->p[0], pd->ng);
         }
         if (rc) return rc;
@@ -1085,6 +1111,22 @@ int32_t yeti_asm_update_coords(yeti_asm *h, const double *coords, void *stream) 
     if (!h || !coords) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
     YETI_HIP(hipMemcpyAsync(h->coords.p, coords, 3 * sizeof(double) * (size_t)h->nb_cp, hipMemcpyDeviceToDevice,
                             (hipStream_t)stream));
+    return 0;
+}
+
+int32_t yeti_asm_set_timing(yeti_asm *h, int32_t on) {
+    if (!h) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null handle");
+    h->timing = on != 0;
+    return 0;
+}
+
+int32_t yeti_asm_kernel_times(yeti_asm *h, float *ms, int32_t cap, int32_t *n) {
+    if (!h || !n) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    *n = h->ev_used;
+    for (int k = 0; k < h->ev_used && k < cap; ++k) {
+        YETI_HIP(hipEventSynchronize(h->ev[2 * k + 1]));
+        YETI_HIP(hipEventElapsedTime(&ms[k], h->ev[2 * k], h->ev[2 * k + 1]));
+    }
     return 0;
 }
 

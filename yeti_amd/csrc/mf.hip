@@ -799,13 +799,14 @@ static int fdm_apply_t(const yeti_fdm *f, const T *U, const T *Ut, const T *invD
     const FdmArgs &fa = f->fa;
     dim3 grid((unsigned)((fa.maxtot + 255) / 256), 3);
     hipLaunchKernelGGL(fdm_gather_kernel<T>, grid, dim3(256), 0, st, fa, r, wa, pred);
-    // forward: (U^T) along axes 0, 1, 2; A = U^T: axis 0 reads U[b][a], axes 1, 2 read Ut[a][b]
-    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wa, wb, Ut, 0, 0, pred);
+    // forward: A = U^T along axes 0, 1, 2. Along axis 0 the lanes run over a, so A[a][b] is read
+    // as U[b][a] (consecutive addresses); along axes 1, 2 a is wave-uniform: Ut[a][b].
+    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wa, wb, U, 0, 1, pred);
     hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, Ut, 1, 0, pred);
     hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wa, wb, Ut, 2, 0, pred);
     hipLaunchKernelGGL(fdm_scale_kernel<T>, grid, dim3(256), 0, st, fa, wb, invD, pred);
-    // backward: U along axes 0, 1, 2
-    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, U, 0, 0, pred);
+    // backward: A = U (axis 0 reads Ut[b][a])
+    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, Ut, 0, 1, pred);
     hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wa, wb, U, 1, 0, pred);
     hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, U, 2, 0, pred);
     hipLaunchKernelGGL(fdm_scatter_kernel<T>, dim3(nblocks), dim3(256), 0, st, fa, (int64_t)f->ncp, wa, r, mask, z,

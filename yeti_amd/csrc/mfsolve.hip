@@ -392,9 +392,13 @@ int fdm_apply(const yeti_fdm *f, const float *r, const uint8_t *mask, float *z, 
               hipStream_t st, const int32_t *pred);
 
 // sc[] slots (doubles), in the order of yeti_ir_status
-enum : int { S_RZ = 0, S_PQ, S_RR, S_RZN, S_BB, S_RRT, S_RRMAX, S_TOL2, S_RRT_PREV, S_NSC = 16 };
+enum : int { S_RZ = 0, S_PQ, S_RR, S_RZN, S_BB, S_RRT, S_RRMAX, S_TOL2, S_RRT_PREV, S_LASTUPD, S_NSC = 16 };
 // flags[] slots (int32), in the order of yeti_ir_status
-enum : int { F_ACTIVE = 0, F_UPDATE, F_STAGNANT, F_DONE, F_FAILED, F_ITERS, F_SPARE, F_UPDATES, F_NFLAGS = 8 };
+enum : int { F_ACTIVE = 0, F_UPDATE, F_STAGNANT, F_DONE, F_FAILED, F_ITERS, F_FLOOR, F_UPDATES, F_NFLAGS = 8 };
+
+// iterations after which a residual replacement is forced (so a solve at the FP64 rounding
+// floor is recognised within a few such periods)
+constexpr int32_t IR_FORCE_EVERY = 256;
 
 __global__ void pred_sum_partials_kernel(const double *part, int n, double *out, const int32_t *pred) {
     if (pred && *pred == 0) return;
@@ -475,7 +479,8 @@ __global__ void ir_check_kernel(double *sc, int32_t *flags, double delta2) {
     if (flags[F_ACTIVE] == 0) return;
     const double rr = sc[S_RR];
     if (rr > sc[S_RRMAX]) sc[S_RRMAX] = rr;
-    if (!(rr > delta2 * sc[S_RRMAX]) || !(rr > sc[S_TOL2])) flags[F_UPDATE] = 1;
+    if (!(rr > delta2 * sc[S_RRMAX]) || !(rr > sc[S_TOL2]) || flags[F_ITERS] + 1 - (int32_t)sc[S_LASTUPD] >= IR_FORCE_EVERY)
+        flags[F_UPDATE] = 1;
 }
 
 // x += y, y = 0 (FP64 accumulation of the FP32/FP64 update), predicated on F_UPDATE
@@ -512,6 +517,7 @@ __global__ void ir_update_done_kernel(double *sc, int32_t *flags, double rtol2, 
     if (flags[F_UPDATE] == 0) return;
     flags[F_UPDATE] = 0;
     flags[F_UPDATES] += 1;
+    sc[S_LASTUPD] = (double)(flags[F_ITERS] + 1);
     const double rrt = sc[S_RRT];
     sc[S_RR] = rrt;
     sc[S_RRMAX] = rrt;
@@ -523,7 +529,11 @@ __global__ void ir_update_done_kernel(double *sc, int32_t *flags, double rtol2, 
     if (!(rrt < 0.25 * sc[S_RRT_PREV])) flags[F_STAGNANT] += 1;
     else flags[F_STAGNANT] = 0;
     sc[S_RRT_PREV] = rrt < sc[S_RRT_PREV] ? rrt : sc[S_RRT_PREV];
-    if (flags[F_UPDATES] >= max_updates || flags[F_STAGNANT] >= max_stagnant) {
+    if (flags[F_STAGNANT] >= max_stagnant) {   // FP64 rounding floor above rtol: stop, flagged
+        flags[F_DONE] = 1;
+        flags[F_FLOOR] = 1;
+        flags[F_ACTIVE] = 0;
+    } else if (flags[F_UPDATES] >= max_updates) {
         flags[F_DONE] = 1;
         flags[F_FAILED] = 1;
         flags[F_ACTIVE] = 0;
@@ -565,6 +575,7 @@ __global__ void ir_begin_flags_kernel(double *sc, int32_t *flags, double rtol2) 
     sc[S_RRMAX] = bb;
     sc[S_RRT_PREV] = bb;
     sc[S_TOL2] = rtol2 * bb;
+    sc[S_LASTUPD] = 0.0;
     for (int i = 0; i < F_NFLAGS; ++i) flags[i] = 0;
     if (bb > 0.0) flags[F_ACTIVE] = 1;
     else flags[F_DONE] = 1;

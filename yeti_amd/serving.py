@@ -283,8 +283,8 @@ class SolveServer:
         buf = a.status.numpy()
         fl = np.frombuffer(buf[:32].tobytes(), dtype=np.int32)
         sc = np.frombuffer(buf[32:].tobytes(), dtype=np.float64)
-        return {"done": int(fl[3]), "failed": int(fl[4]), "iters": int(fl[5]), "updates": int(fl[7]),
-                "relres": math.sqrt(sc[5] / sc[4]) if sc[4] > 0 else 0.0}
+        return {"done": int(fl[3]), "failed": int(fl[4]), "floor": int(fl[6]), "iters": int(fl[5]),
+                "updates": int(fl[7]), "relres": math.sqrt(sc[5] / sc[4]) if sc[4] > 0 else 0.0}
 
     def _release(self, a):
         L = _lib.lib()
@@ -301,10 +301,11 @@ class SolveServer:
         self.alloc.release(a.off, a.nbytes)
 
     # -- the loop ------------------------------------------------------------------------
-    def run(self, requests, keep_solutions=False, realtime=True, progress=None):
+    def run(self, requests, keep_solutions=False, realtime=True, progress=None, step_log=None, step_every=0):
         """Serve `requests` (sorted by arrival). realtime=False admits every request at t=0
         (closed-loop throughput). Returns a report dict; with keep_solutions, report["x"][rid]
-        holds each solution on the host."""
+        holds each solution on the host. progress(done, total) is called as requests complete;
+        step_log(rid, status) every `step_every` steps of a request."""
         torch = self.torch
         reqs = sorted(requests, key=lambda r: r.arrival)
         ready = {}
@@ -360,6 +361,8 @@ class SolveServer:
                         still.append(a)
                         continue
                     st = self._read_status(a)
+                    if step_log and step_every and a.steps % step_every == 0:
+                        step_log(a.req.rid, st)
                     if st["done"]:
                         t_end = time.perf_counter() - t0
                         arr = a.req.arrival if realtime else 0.0
@@ -391,6 +394,7 @@ class SolveServer:
         rep = {"completed": len(done), "makespan_s": makespan,
                "solves_per_s": len(done) / makespan if makespan > 0 else None,
                "latency_s": latency_summary(lat), "failed": sum(1 for d in done.values() if d["failed"]),
+               "at_fp64_floor": sum(1 for d in done.values() if d["floor"]),
                "peak_arena_bytes": self.alloc.peak, "budget_bytes": self.budget, "requests": done}
         if keep_solutions:
             rep["x"] = xs
