@@ -692,8 +692,9 @@ static int launch_elem(yeti_asm *h, PatchDev &pd, const double *coords, double *
         pa.elems = pd.elems.p + e0;
         pa.offs = pd.offs.p + e0;
         pa.n_active = std::min(batch, pd.n_active - e0);
-        const int64_t nt = (int64_t)pa.n_active * C::NQ;
-        hipLaunchKernelGGL((elem_geo_kernel<P, NG>), dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, pa, pd.geo.p);
+        constexpr int GEO_EPB = 4 * geo_epw<C::NQ>();
+        hipLaunchKernelGGL((elem_geo_kernel<P, NG>), dim3((unsigned)((pa.n_active + GEO_EPB - 1) / GEO_EPB)), dim3(256), 0,
+                           st, pa, pd.geo.p);
         // persistent workgroups: one per CU (the kernel's LDS and registers allow one)
         const int grid = std::min(pa.n_active, cus);
         hipEvent_t ea = nullptr, eb = nullptr;

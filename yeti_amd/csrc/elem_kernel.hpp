@@ -1,7 +1,7 @@
 // elem_kernel.hpp -- the FP64 element stiffness kernels (included by assembly.hip, which defines
 // GaussRule and bspline_ders before including it).
 //
-// 1. elem_geo_kernel, one thread per (element, Gauss point): the Jacobian of the NURBS map
+// 1. elem_geo_kernel, one lane per (element, Gauss point): the Jacobian of the NURBS map
 //    (shap.f90:94-178) reduced to 12 numbers per point, A[m][c] = sqrt(w |det J|) / W
 //    dxi_m/dx_c and dW/dxi_m / W, written to HBM (12 doubles per Gauss point).
 // 2. elem_stiffness_kernel, persistent and warp-specialised: one workgroup of 12 waves per CU
@@ -11,14 +11,18 @@
 //        G_c[a][q] = sqrt(w|det J|) dR_a/dx_c = w_a sum_m (dN_a/dxi_m - N_a dW_m/W) A[m][c]
 //        into one half of a double-buffered LDS array;
 //      * 8 consumer waves accumulate S^{kl}_{ab} = sum_q G_k[a][q] G_l[b][q] over the chunk on
-//        v_mfma_f64_16x16x4f64. A consumer task is (16-node tile pair (I, J), k): the three
-//        16x16 tiles S^{k0}, S^{k1}, S^{k2} of rows I, columns J; a wave keeps 3-4 tasks and
-//        loads each distinct operand fragment once per K step (about 0.8 fragment per MFMA);
+//        v_mfma_f64_16x16x4f64. The 16x16 tiles of a 16-node tile pair (I, J) are grouped in
+//        units that contract independently: D = {S^00, S^11, S^22} -> K_ab[k][k], and
+//        P(k,l) = {S^kl, S^lk} -> K_ab[k][l], K_ab[l][k] (k < l); units are dealt to the waves
+//        in pair-major runs (about 11 tiles per wave), so a wave loads each distinct operand
+//        fragment once per two K steps and most of its units share fragments;
 //    one __syncthreads() ends each step. After an element's last chunk of a pass the consumers
-//    write the raw S tiles of the pass's columns into an LDS stage; all 12 waves then emit
-//    K_ab[k][l] = lam S^{kl}_ab + mu S^{lk}_ab + mu d_kl tr S_ab (the isotropic form of
-//    stiffmatrix_byCP's BoI^T D BoJ, stiffmatrix.F90:175-216) with coalesced stores, the row
-//    and column indices computed from the COO position.
+//    contract their units in registers, K_ab[k][l] = lam S^{kl}_ab + mu S^{lk}_ab +
+//    mu d_kl tr S_ab (the isotropic form of stiffmatrix_byCP's BoI^T D BoJ,
+//    stiffmatrix.F90:175-216), into an LDS stage laid out as the COO entries, and the waves copy
+//    it out with coalesced stores, computing the row and column indices from the position. The
+//    stage holds one round of node columns b at a time; the last (largest) round of a pass is
+//    copied in slices during the following steps, beside the MFMA work and the chunk builds.
 #pragma once
 
 #include <utility>
@@ -54,37 +58,44 @@ struct ElemCfg {
     static constexpr int CW = 8;                        // consumer waves
     static constexpr int PW = 4;                        // producer waves
     static constexpr int NTH = (CW + PW) * 64;
-    static constexpr int MAXT = NNP >= 128 ? 3 : 4;     // tasks per consumer wave per pass (registers)
+    static constexpr int MAXTILE = NNP >= 128 ? 9 : 12; // accumulator tiles per consumer wave per pass
+    static constexpr int MAXU = 8;                      // units per consumer wave per pass
+    static constexpr int MAXF = 12;                     // distinct A (or B) fragments per wave
     static constexpr int MAXPASS = 8;
     static constexpr int MAXR = 8;                      // staging rounds per pass
     static constexpr int GPW = QC / PW;                 // Gauss points per producer wave per chunk
     static constexpr int64_t NDE = 9LL * NN * (NN - 1) / 2 + 6LL * NN;   // COO entries per element
-    // LDS stage of raw S values (9 per node pair a <= b): what is left of ~150 KiB after G
-    static constexpr int64_t SSTN = 9LL * NN * (NN + 1) / 2;
+    // LDS stage of one round of COO values: what is left of ~150 KiB after G
     static constexpr int64_t GBYTES = 2LL * 3 * NNP * QCP * 8;
     static constexpr int64_t STGMAX = (150 * 1024 - GBYTES - 4 * 1024) / 8;
-    static constexpr int STG = (int)(SSTN < STGMAX ? SSTN : STGMAX);
+    static constexpr int STG = (int)(NDE < STGMAX ? NDE : STGMAX);
     static_assert(QC % PW == 0, "chunk must split over producer waves");
 };
 
-__host__ __device__ constexpr int64_t coo_start(int b) { return (9LL * b * b + 3LL * b) / 2; }   // COO
-__host__ __device__ constexpr int64_t sst_start(int b) { return 9LL * b * (b + 1) / 2; }          // stage
+__host__ __device__ constexpr int64_t coo_start(int b) { return (9LL * b * b + 3LL * b) / 2; }
 
-// Static schedule of consumer tasks (tile pair, k) over waves and passes. A pass takes whole tile
-// columns J (all pairs I <= J), so the S values of nodes b in [16 J0, 16 J1) are complete after
-// it; its tasks are dealt to the waves in pair-major order in contiguous runs (sizes differing by
-// at most one), so a wave's tasks share operand fragments. Each pass's b-range is emitted in
-// rounds whose raw S values fit the LDS stage.
-template <int NT, int WAVES, int MAXT, int MAXPASS, int MAXR>
-struct TaskSchedule {
+// Static schedule of consumer units over waves and passes. Unit code = pair * 4 + kind, kind 0:
+// D (tiles S^00, S^11, S^22), 1: P(0,1), 2: P(0,2), 3: P(1,2) (tiles S^kl, S^lk). A pass takes
+// whole tile columns J (all pairs I <= J), so the COO entries of nodes b in [16 J0, 16 J1) are
+// complete after it; its units are dealt in pair-major order to consecutive waves, each filled
+// up to T tiles with the smallest T that places every unit (so the waves of a SIMD get nearly
+// equal MFMA work). Each pass's b-range is emitted in rounds that fit the LDS stage.
+__host__ __device__ constexpr int unit_tiles(int kind) { return kind == 0 ? 3 : 2; }
+__host__ __device__ constexpr int unit_k(int kind, int t) { return kind == 0 ? t : t == 0 ? (kind == 3 ? 1 : 0) : (kind == 1 ? 1 : 2); }
+__host__ __device__ constexpr int unit_l(int kind, int t) { return kind == 0 ? t : t == 1 ? (kind == 3 ? 1 : 0) : (kind == 1 ? 1 : 2); }
+
+template <int NT, int WAVES, int MAXU, int MAXTILE, int MAXF, int MAXPASS, int MAXR>
+struct UnitSchedule {
     int npass;
-    int nt[MAXPASS][WAVES];                 // tasks of a wave
-    int task[MAXPASS][WAVES][MAXT];         // pair * 3 + k
+    int nu[MAXPASS][WAVES];                 // units of a wave
+    int unit[MAXPASS][WAVES][MAXU];         // pair * 4 + kind
+    int toff[MAXPASS][WAVES][MAXU];         // accumulator slot of the unit's first tile
+    int nt[MAXPASS][WAVES];                 // tiles of a wave
     int na[MAXPASS][WAVES], nb[MAXPASS][WAVES];
-    int afr[MAXPASS][WAVES][MAXT];          // distinct A fragments: I * 3 + k
-    int bfr[MAXPASS][WAVES][3 * MAXT];      // distinct B fragments: J * 3 + l
-    int aidx[MAXPASS][WAVES][MAXT];         // task -> A fragment slot
-    int bidx[MAXPASS][WAVES][MAXT][3];      // (task, l) -> B fragment slot
+    int afr[MAXPASS][WAVES][MAXF];          // distinct A fragments: I * 3 + k
+    int bfr[MAXPASS][WAVES][MAXF];          // distinct B fragments: J * 3 + l
+    int aidx[MAXPASS][WAVES][MAXTILE];      // tile -> A fragment slot
+    int bidx[MAXPASS][WAVES][MAXTILE];      // tile -> B fragment slot
     int nround[MAXPASS];
     int rb[MAXPASS][MAXR + 1];              // round boundaries in b
     bool ok;
@@ -97,65 +108,100 @@ __host__ __device__ constexpr int pair_J(int pi) {
 }
 __host__ __device__ constexpr int pair_I(int pi) { return pi - pair_J(pi) * (pair_J(pi) + 1) / 2; }
 
-template <int NT, int WAVES, int MAXT, int MAXPASS, int MAXR>
-constexpr TaskSchedule<NT, WAVES, MAXT, MAXPASS, MAXR> make_schedule(int nn, int stg) {
-    TaskSchedule<NT, WAVES, MAXT, MAXPASS, MAXR> s{};
+template <int NT, int WAVES, int MAXU, int MAXTILE, int MAXF, int MAXPASS, int MAXR>
+constexpr UnitSchedule<NT, WAVES, MAXU, MAXTILE, MAXF, MAXPASS, MAXR> make_schedule(int nn, int stg) {
+    UnitSchedule<NT, WAVES, MAXU, MAXTILE, MAXF, MAXPASS, MAXR> s{};
     s.ok = true;
     int J = 0, pass = 0;
     while (J < NT && pass < MAXPASS) {
-        int J1 = J, ntask = 0;
-        while (J1 < NT && ntask + 3 * (J1 + 1) <= MAXT * WAVES) {
-            ntask += 3 * (J1 + 1);
+        int J1 = J, tiles = 0;
+        while (J1 < NT && tiles + 9 * (J1 + 1) <= MAXTILE * WAVES) {
+            tiles += 9 * (J1 + 1);
             ++J1;
         }
         if (J1 == J) {
             s.ok = false;
             return s;
         }
-        const int t0 = 3 * (J * (J + 1) / 2);
-        int t = t0;
-        for (int w = 0; w < WAVES; ++w) {
-            const int cnt = ntask / WAVES + (w < ntask % WAVES ? 1 : 0);
-            s.nt[pass][w] = cnt;
-            s.na[pass][w] = s.nb[pass][w] = 0;
-            for (int u = 0; u < cnt; ++u, ++t) {
-                s.task[pass][w][u] = t;
-                const int pi = t / 3, k = t % 3;
-                const int akey = pair_I(pi) * 3 + k;
-                int ai = -1;
-                for (int x = 0; x < s.na[pass][w]; ++x)
-                    if (s.afr[pass][w][x] == akey) ai = x;
-                if (ai < 0) {
-                    ai = s.na[pass][w]++;
-                    s.afr[pass][w][ai] = akey;
+        const int u0 = 4 * (J * (J + 1) / 2), u1 = 4 * (J1 * (J1 + 1) / 2);
+        bool placed = false;
+        for (int T = (tiles + WAVES - 1) / WAVES; T <= MAXTILE && !placed; ++T) {
+            // greedy run: fill wave w up to T tiles, then move on
+            int w = 0, fill = 0, cnt = 0;
+            bool fits = true;
+            for (int w2 = 0; w2 < WAVES; ++w2) s.nu[pass][w2] = s.nt[pass][w2] = 0;
+            for (int u = u0; u < u1 && fits; ++u) {
+                const int nt = unit_tiles(u % 4);
+                if (fill + nt > T || cnt == MAXU) {
+                    ++w;
+                    fill = cnt = 0;
                 }
-                s.aidx[pass][w][u] = ai;
-                for (int l = 0; l < 3; ++l) {
-                    const int bkey = pair_J(pi) * 3 + l;
-                    int bi = -1;
-                    for (int x = 0; x < s.nb[pass][w]; ++x)
-                        if (s.bfr[pass][w][x] == bkey) bi = x;
+                if (w >= WAVES || nt > T) {
+                    fits = false;
+                    break;
+                }
+                s.unit[pass][w][cnt] = u;
+                s.toff[pass][w][cnt] = fill;
+                ++cnt;
+                fill += nt;
+                s.nu[pass][w] = cnt;
+                s.nt[pass][w] = fill;
+            }
+            placed = fits;
+        }
+        if (!placed) {
+            s.ok = false;
+            return s;
+        }
+        // distinct fragments per wave
+        for (int w = 0; w < WAVES; ++w) {
+            s.na[pass][w] = s.nb[pass][w] = 0;
+            for (int x = 0; x < s.nu[pass][w]; ++x) {
+                const int u = s.unit[pass][w][x], pi = u / 4, kind = u % 4;
+                for (int t = 0; t < unit_tiles(kind); ++t) {
+                    const int akey = pair_I(pi) * 3 + unit_k(kind, t), bkey = pair_J(pi) * 3 + unit_l(kind, t);
+                    int ai = -1, bi = -1;
+                    for (int y = 0; y < s.na[pass][w]; ++y)
+                        if (s.afr[pass][w][y] == akey) ai = y;
+                    for (int y = 0; y < s.nb[pass][w]; ++y)
+                        if (s.bfr[pass][w][y] == bkey) bi = y;
+                    if (ai < 0) {
+                        if (s.na[pass][w] == MAXF) {
+                            s.ok = false;
+                            return s;
+                        }
+                        ai = s.na[pass][w]++;
+                        s.afr[pass][w][ai] = akey;
+                    }
                     if (bi < 0) {
+                        if (s.nb[pass][w] == MAXF) {
+                            s.ok = false;
+                            return s;
+                        }
                         bi = s.nb[pass][w]++;
                         s.bfr[pass][w][bi] = bkey;
                     }
-                    s.bidx[pass][w][u][l] = bi;
+                    s.aidx[pass][w][s.toff[pass][w][x] + t] = ai;
+                    s.bidx[pass][w][s.toff[pass][w][x] + t] = bi;
                 }
             }
         }
-        // rounds over b in [16 J, min(16 J1, nn))
-        int b0 = 16 * J, bend = 16 * J1 < nn ? 16 * J1 : nn, r = 0;
-        s.rb[pass][0] = b0;
-        while (b0 < bend) {
-            int b1 = b0 + 1;
-            while (b1 < bend && sst_start(b1 + 1) - sst_start(b0) <= stg) ++b1;
-            if (sst_start(b1) - sst_start(b0) > stg || r >= MAXR) {
+        // rounds over b in [16 J, min(16 J1, nn)), built from the end so that the last round
+        // (whose copy-out is deferred into the next step) is the largest that fits
+        const int bstart = 16 * J, bend = 16 * J1 < nn ? 16 * J1 : nn;
+        int cuts[MAXR + 1] = {}, r = 0, b1 = bend;
+        cuts[0] = bend;
+        while (b1 > bstart) {
+            int b0 = b1 - 1;
+            while (b0 > bstart && coo_start(b1) - coo_start(b0 - 1) <= stg) --b0;
+            if (coo_start(b1) - coo_start(b0) > stg || r >= MAXR) {
                 s.ok = false;
                 return s;
             }
-            s.rb[pass][++r] = b1;
-            b0 = b1;
+            cuts[++r] = b0;
+            b1 = b0;
         }
+        for (int x = 0; x <= r; ++x) s.rb[pass][x] = cuts[r - x];
         s.nround[pass] = r;
         J = J1;
         ++pass;
@@ -168,9 +214,10 @@ constexpr TaskSchedule<NT, WAVES, MAXT, MAXPASS, MAXR> make_schedule(int nn, int
 template <int P, int NG>
 struct Sched {
     using C = ElemCfg<P, NG>;
-    static constexpr auto S = make_schedule<C::NT, C::CW, C::MAXT, C::MAXPASS, C::MAXR>(C::NN, C::STG);
+    static constexpr auto S =
+        make_schedule<C::NT, C::CW, C::MAXU, C::MAXTILE, C::MAXF, C::MAXPASS, C::MAXR>(C::NN, C::STG);
     static constexpr int NPASS = S.npass;
-    static_assert(S.ok, "task schedule does not fit the wave / staging capacities");
+    static_assert(S.ok, "unit schedule does not fit the wave / staging capacities");
 };
 
 struct PatchArgs {
@@ -191,82 +238,110 @@ struct PatchArgs {
 // v = 3 m + c: A[m][c] = sqrt(w |det J|) / W dxi_m/dx_c; v = 9 + m: dW/dxi_m / W. J is the
 // derivative with respect to the knot parameters (shap.f90:133-153); w |det J| includes the
 // parent-to-parameter factors (shap.f90:156-178) and the Gauss weights (UELMAT.f90:118-119).
+// A workgroup of 4 waves takes 4 GEO_EPW elements: their control points (x, y, z, w) are
+// gathered once into LDS with one load per node, then each lane evaluates one Gauss point
+// (several when NQ > 64) from LDS broadcasts.
+template <int NQ>
+constexpr int geo_epw() { return NQ >= 64 ? 1 : 64 / NQ; }   // elements per wave
+
 template <int P, int NG>
 __global__ void __launch_bounds__(256) elem_geo_kernel(PatchArgs pa, double *__restrict__ geo) {
     using C = ElemCfg<P, NG>;
     constexpr int NP1 = C::NP1, NN = C::NN, NQ = C::NQ;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)pa.n_active * NQ) return;
-    const int64_t gi = t / NQ;
-    const int q = (int)(t % NQ);
-    const int e = pa.elems[gi];
-    const int gq[3] = {q % NG, (q / NG) % NG, q / (NG * NG)};
-    double B[3][NP1], dB[3][NP1], half[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double *U = d == 0 ? pa.U0 : d == 1 ? pa.U1 : pa.U2;
-        const int span1 = pa.nijk[3 * (int64_t)e + d];
-        const double u0 = U[span1 - 1], u1 = U[span1];
-        const double u = ((u1 - u0) * pa.g.x[gq[d]] + (u1 + u0)) * 0.5;
-        double N[NP1], dN[NP1];
-        bspline_ders<P>(span1 - 1, u, U, N, dN);
-#pragma unroll
-        for (int i = 0; i < NP1; ++i) {   // local index i <-> basis function P - i
-            B[d][i] = N[P - i];
-            dB[d][i] = dN[P - i];
+    constexpr int EPW = geo_epw<NQ>(), EPB = 4 * EPW;
+    constexpr int QPL = (NQ + 63) / 64;              // Gauss points per lane
+    __shared__ double cp[EPB][NN][4];
+    const int64_t gb0 = (int64_t)blockIdx.x * EPB;
+    for (int x = threadIdx.x; x < EPB * NN; x += 256) {
+        const int le = x / NN, a = x - le * NN;
+        const int64_t gi = gb0 + le;
+        if (gi < pa.n_active) {
+            const int e = pa.elems[gi];
+            const double *xp = pa.coords + 3 * (int64_t)(pa.ien[(int64_t)NN * e + a] - 1);
+            cp[le][a][0] = xp[0];
+            cp[le][a][1] = xp[1];
+            cp[le][a][2] = xp[2];
+            cp[le][a][3] = pa.weight[(int64_t)NN * e + a];
         }
-        half[d] = 0.5 * (u1 - u0);
     }
-    const int32_t *ien = pa.ien + (int64_t)NN * e;
-    const double *wt = pa.weight + (int64_t)NN * e;
-    double s[16];
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int le = wv * EPW + (EPW > 1 ? lane / NQ : 0);
+    const int64_t gi = gb0 + le;
+    if ((EPW > 1 && lane >= EPW * NQ) || gi >= pa.n_active) return;
+    const int e = pa.elems[gi];
+    for (int r = 0; r < QPL; ++r) {
+        const int q = EPW > 1 ? lane % NQ : lane + 64 * r;
+        if (q >= NQ) break;
+        const int gq[3] = {q % NG, (q / NG) % NG, q / (NG * NG)};
+        double B[3][NP1], dB[3][NP1], half[3];
 #pragma unroll
-    for (int m = 0; m < 16; ++m) s[m] = 0.0;
-    for (int a = 0; a < NN; ++a) {
-        const int i = a % NP1, j = (a / NP1) % NP1, k = a / (NP1 * NP1);
-        const double w = wt[a];
-        const double *xp = pa.coords + 3 * (int64_t)(ien[a] - 1);
-        const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
-        const double b12 = B[1][j] * B[2][k] * w;
-        const double Na = B[0][i] * b12;
-        const double d0 = dB[0][i] * b12;
-        const double d1 = B[0][i] * dB[1][j] * B[2][k] * w;
-        const double d2 = B[0][i] * B[1][j] * dB[2][k] * w;
-        s[0] += Na;
-        s[1] += d0; s[2] += d1; s[3] += d2;
-        s[4] += x0 * Na; s[5] += x1 * Na; s[6] += x2 * Na;
-        s[7] += x0 * d0; s[8] += x0 * d1; s[9] += x0 * d2;
-        s[10] += x1 * d0; s[11] += x1 * d1; s[12] += x1 * d2;
-        s[13] += x2 * d0; s[14] += x2 * d1; s[15] += x2 * d2;
+        for (int d = 0; d < 3; ++d) {
+            const double *U = d == 0 ? pa.U0 : d == 1 ? pa.U1 : pa.U2;
+            const int span1 = pa.nijk[3 * (int64_t)e + d];
+            const double u0 = U[span1 - 1], u1 = U[span1];
+            const double u = ((u1 - u0) * pa.g.x[gq[d]] + (u1 + u0)) * 0.5;
+            double N[NP1], dN[NP1];
+            bspline_ders<P>(span1 - 1, u, U, N, dN);
+#pragma unroll
+            for (int i = 0; i < NP1; ++i) {   // local index i <-> basis function P - i
+                B[d][i] = N[P - i];
+                dB[d][i] = dN[P - i];
+            }
+            half[d] = 0.5 * (u1 - u0);
+        }
+        double s[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) s[m] = 0.0;
+#pragma unroll
+        for (int k = 0; k < NP1; ++k)
+#pragma unroll
+            for (int j = 0; j < NP1; ++j)
+#pragma unroll
+                for (int i = 0; i < NP1; ++i) {
+                    const int a = i + NP1 * (j + NP1 * k);
+                    const double x0 = cp[le][a][0], x1 = cp[le][a][1], x2 = cp[le][a][2], w = cp[le][a][3];
+                    const double b12 = B[1][j] * B[2][k] * w;
+                    const double Na = B[0][i] * b12;
+                    const double d0 = dB[0][i] * b12;
+                    const double d1 = B[0][i] * dB[1][j] * B[2][k] * w;
+                    const double d2 = B[0][i] * B[1][j] * dB[2][k] * w;
+                    s[0] += Na;
+                    s[1] += d0; s[2] += d1; s[3] += d2;
+                    s[4] += x0 * Na; s[5] += x1 * Na; s[6] += x2 * Na;
+                    s[7] += x0 * d0; s[8] += x0 * d1; s[9] += x0 * d2;
+                    s[10] += x1 * d0; s[11] += x1 * d1; s[12] += x1 * d2;
+                    s[13] += x2 * d0; s[14] += x2 * d1; s[15] += x2 * d2;
+                }
+        const double Wi = 1.0 / s[0];
+        const double dw[3] = {s[1] * Wi, s[2] * Wi, s[3] * Wi};
+        double J[3][3];
+#pragma unroll
+        for (int r2 = 0; r2 < 3; ++r2)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) J[r2][c] = (s[7 + 3 * r2 + c] - s[4 + r2] * dw[c]) * Wi;
+        const double m00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double m01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const double m02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * m00 + J[0][1] * m01 + J[0][2] * m02;
+        const double om = pa.g.w[gq[0]] * pa.g.w[gq[1]] * pa.g.w[gq[2]] * fabs(det * half[0] * half[1] * half[2]);
+        const double f = sqrt(om) * Wi / det;
+        double A[9];   // A[3 m + c] = f * (adj J)[m][c]
+        A[0] = m00 * f;
+        A[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * f;
+        A[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * f;
+        A[3] = m01 * f;
+        A[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * f;
+        A[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * f;
+        A[6] = m02 * f;
+        A[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * f;
+        A[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * f;
+        double *out = geo + gi * (ELEM_NGEO * NQ) + q;
+#pragma unroll
+        for (int v = 0; v < 9; ++v) out[v * NQ] = A[v];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) out[(9 + m) * NQ] = dw[m];
     }
-    const double Wi = 1.0 / s[0];
-    const double dw[3] = {s[1] * Wi, s[2] * Wi, s[3] * Wi};
-    double J[3][3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) J[r][c] = (s[7 + 3 * r + c] - s[4 + r] * dw[c]) * Wi;
-    const double m00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-    const double m01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-    const double m02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-    const double det = J[0][0] * m00 + J[0][1] * m01 + J[0][2] * m02;
-    const double om = pa.g.w[gq[0]] * pa.g.w[gq[1]] * pa.g.w[gq[2]] * fabs(det * half[0] * half[1] * half[2]);
-    const double f = sqrt(om) * Wi / det;
-    double A[9];   // A[3 m + c] = f * (adj J)[m][c]
-    A[0] = m00 * f;
-    A[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * f;
-    A[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * f;
-    A[3] = m01 * f;
-    A[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * f;
-    A[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * f;
-    A[6] = m02 * f;
-    A[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * f;
-    A[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * f;
-    double *out = geo + gi * (ELEM_NGEO * NQ) + q;
-#pragma unroll
-    for (int v = 0; v < 9; ++v) out[v * NQ] = A[v];
-#pragma unroll
-    for (int m = 0; m < 3; ++m) out[(9 + m) * NQ] = dw[m];
 }
 
 __device__ __forceinline__ double bcast(double v, int src) {
@@ -284,7 +359,7 @@ struct ElemSmem {
     __attribute__((aligned(16))) double G[2][3][C::NNP][C::QCP];
     double B[C::PW][3][NG][C::NP1];     // per producer wave: B(reversed local index) at the abscissae
     double dB[C::PW][3][NG][C::NP1];
-    double stage[C::STG];               // raw S of one round: stage[sst(b) + 9 a + k + 3 l] = S^{kl}_{ab}
+    double stage[C::STG];               // COO values of one round, element-local order
     int32_t dof[C::NN];                 // 3 (IEN - 1) of the element being emitted
 };
 
@@ -389,12 +464,10 @@ __device__ __forceinline__ void producer_chunk(ElemSmem<P, NG> &sm, double gv, i
 template <int P, int NG>
 struct ConsumerAcc {
     using C = ElemCfg<P, NG>;
-    f64x4 S[C::MAXT][3];
+    f64x4 S[C::MAXTILE];
     __device__ __forceinline__ void zero() {
 #pragma unroll
-        for (int u = 0; u < C::MAXT; ++u)
-#pragma unroll
-            for (int l = 0; l < 3; ++l) S[u][l] = f64x4{0.0, 0.0, 0.0, 0.0};
+        for (int u = 0; u < C::MAXTILE; ++u) S[u] = f64x4{0.0, 0.0, 0.0, 0.0};
     }
 };
 
@@ -422,127 +495,120 @@ __device__ __forceinline__ void mfma_chunk(const ElemSmem<P, NG> &sm, int buf, C
             });
             static_for<nt>([&](auto ic) {
                 constexpr int u = decltype(ic)::value;
-                constexpr int ai = Sched<P, NG>::S.aidx[PASS][W][u];
-                static_for<3>([&](auto lc) {
-                    constexpr int l = decltype(lc)::value;
-                    constexpr int bi = Sched<P, NG>::S.bidx[PASS][W][u][l];
-                    acc.S[u][l] = __builtin_amdgcn_mfma_f64_16x16x4f64(Af[ai].x, Bf[bi].x, acc.S[u][l], 0, 0, 0);
-                });
+                constexpr int ai = Sched<P, NG>::S.aidx[PASS][W][u], bi = Sched<P, NG>::S.bidx[PASS][W][u];
+                acc.S[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(Af[ai].x, Bf[bi].x, acc.S[u], 0, 0, 0);
             });
             static_for<nt>([&](auto ic) {
                 constexpr int u = decltype(ic)::value;
-                constexpr int ai = Sched<P, NG>::S.aidx[PASS][W][u];
-                static_for<3>([&](auto lc) {
-                    constexpr int l = decltype(lc)::value;
-                    constexpr int bi = Sched<P, NG>::S.bidx[PASS][W][u][l];
-                    acc.S[u][l] = __builtin_amdgcn_mfma_f64_16x16x4f64(Af[ai].y, Bf[bi].y, acc.S[u][l], 0, 0, 0);
-                });
+                constexpr int ai = Sched<P, NG>::S.aidx[PASS][W][u], bi = Sched<P, NG>::S.bidx[PASS][W][u];
+                acc.S[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(Af[ai].y, Bf[bi].y, acc.S[u], 0, 0, 0);
             });
         }
     }
 }
 
-// Epilogue, first half: the raw S tiles of wave W whose column node b lies in round R of pass
-// PASS, into the stage. Accumulator lane layout (v_mfma_f64_16x16x4f64): register r of lane l
-// holds row (l >> 4) + 4 r and column l & 15 of the tile; rows are nodes a of tile I, columns
-// nodes b of tile J.
+// Epilogue, first half: wave W contracts its units whose column node b lies in round R of pass
+// PASS and writes the K values at their COO positions (relative to the round) in the stage.
+// Accumulator lane layout (v_mfma_f64_16x16x4f64): register r of lane l holds row (l >> 4) + 4 r
+// and column l & 15 of the tile; rows are nodes a of tile I, columns nodes b of tile J. COO
+// position of block (a, b), a < b: start(b) + 9 a + kk + 3 ll with start(b) = 9 b (b-1)/2 + 6 b;
+// diagonal block: start(b) + 9 b + {0..5} holding (0,0) (0,1) (1,1) (0,2) (1,2) (2,2), diagonal
+// halved (build_system_elemStorage.f90:260-293).
 template <int P, int NG, int PASS, int W, int R>
-__device__ __forceinline__ void stage_round(ElemSmem<P, NG> &sm, const ConsumerAcc<P, NG> &acc) {
+__device__ __forceinline__ void stage_round(ElemSmem<P, NG> &sm, double lam, double mu, const ConsumerAcc<P, NG> &acc) {
     constexpr auto &S = Sched<P, NG>::S;
-    constexpr int nt = S.nt[PASS][W];
+    constexpr int nu = S.nu[PASS][W];
     constexpr int b0 = S.rb[PASS][R], b1 = S.rb[PASS][R + 1];
-    constexpr int s0 = (int)sst_start(b0);
+    constexpr int c0 = (int)coo_start(b0);
     (void)S;
     const int lane = threadIdx.x & 63;
     const int nl = lane & 15, kg = lane >> 4;
-    static_for<nt>([&](auto ic) {
-        constexpr int u = decltype(ic)::value;
-        constexpr int code = Sched<P, NG>::S.task[PASS][W][u];
-        constexpr int pi = code / 3, k = code % 3;
+    static_for<nu>([&](auto ic) {
+        constexpr int x = decltype(ic)::value;
+        constexpr int code = Sched<P, NG>::S.unit[PASS][W][x];
+        constexpr int off = Sched<P, NG>::S.toff[PASS][W][x];
+        constexpr int pi = code / 4, kind = code % 4;
         constexpr int I = pair_I(pi), J = pair_J(pi);
         if constexpr (16 * J + 15 >= b0 && 16 * J < b1) {
             const int b = 16 * J + nl;
             if (b >= b0 && b < b1) {
-                double *st = sm.stage + ((int)sst_start(b) - s0) + k;
+                double *st = sm.stage + ((int)coo_start(b) - c0);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int a = 16 * I + kg + 4 * r;
                     if (I == J && a > b) continue;
-#pragma unroll
-                    for (int l = 0; l < 3; ++l) st[9 * a + 3 * l] = acc.S[u][l][r];
+                    if constexpr (kind == 0) {
+                        const double s0 = acc.S[off][r], s1 = acc.S[off + 1][r], s2 = acc.S[off + 2][r];
+                        const double tr = mu * (s0 + s1 + s2);
+                        const double k00 = (lam + mu) * s0 + tr, k11 = (lam + mu) * s1 + tr, k22 = (lam + mu) * s2 + tr;
+                        if (a < b) {
+                            st[9 * a + 0] = k00;
+                            st[9 * a + 4] = k11;
+                            st[9 * a + 8] = k22;
+                        } else {
+                            st[9 * b + 0] = 0.5 * k00;
+                            st[9 * b + 2] = 0.5 * k11;
+                            st[9 * b + 5] = 0.5 * k22;
+                        }
+                    } else {
+                        constexpr int k = kind == 3 ? 1 : 0, l = kind == 1 ? 1 : 2;
+                        constexpr int mdiag = kind == 1 ? 1 : kind == 2 ? 3 : 4;
+                        const double skl = acc.S[off][r], slk = acc.S[off + 1][r];
+                        const double kkl = lam * skl + mu * slk;   // K_ab[k][l]
+                        if (a < b) {
+                            st[9 * a + k + 3 * l] = kkl;
+                            st[9 * a + l + 3 * k] = lam * slk + mu * skl;   // K_ab[l][k]
+                        } else {
+                            st[9 * b + mdiag] = kkl;   // diagonal block keeps the upper entry (k, l)
+                        }
+                    }
                 }
             }
         }
     });
 }
 
-// Epilogue, second half (all 768 threads): the COO entries of round R of pass PASS. Node b's
-// entries are the contiguous range [start(b), start(b+1)), start(b) = 9 b (b-1)/2 + 6 b: first
-// the 9-entry blocks a < b with kk fastest, then the 6-entry diagonal block (0,0) (0,1) (1,1)
-// (0,2) (1,2) (2,2) with the diagonal halved (build_system_elemStorage.f90:260-293). Wave w
-// takes rows b = b0 + w, b0 + w + 12, ...: its lanes first contract the row's blocks in place
-// (lane = block a), then copy the row with consecutive lanes on consecutive entries; entry
-// j < 9 b of the row sits at stage offset 9 a + kk + 3 ll = j. row = 3 (IEN_a - 1) + kk,
-// col = 3 (IEN_b - 1) + ll.
-template <int P, int NG, bool WRITE_IDX, int PASS, int R>
-__device__ __forceinline__ void copy_round(ElemSmem<P, NG> &sm, double lam, double mu, int64_t base,
-                                           double *__restrict__ Kdata, int32_t *__restrict__ Krow,
-                                           int32_t *__restrict__ Kcol) {
-    using C = ElemCfg<P, NG>;
+// Epilogue, second half (waves 0 .. NW-1): the COO entries of round R of pass PASS. Wave w takes
+// rows b = b0 + w + NW part, then every NW NPART-th row (part selects one of NPART slices) (row b = the contiguous entries [start(b), start(b+1))) and
+// copies them with consecutive lanes on consecutive entries. Entry j of a row: j < 9 b is block
+// a = j / 9, (kk, ll) = (j mod 3, j mod 9 / 3); the last 6 are the diagonal block.
+// row = 3 (IEN_a - 1) + kk, col = 3 (IEN_b - 1) + ll.
+template <int P, int NG, bool WRITE_IDX, int PASS, int R, int NW, int NPART = 1>
+__device__ __forceinline__ void copy_round(const ElemSmem<P, NG> &sm, int64_t base, double *__restrict__ Kdata,
+                                           int32_t *__restrict__ Krow, int32_t *__restrict__ Kcol, int part = 0) {
     constexpr auto &S = Sched<P, NG>::S;
     constexpr int b0 = S.rb[PASS][R], b1 = S.rb[PASS][R + 1];
-    constexpr int s0 = (int)sst_start(b0);
-    constexpr int NW = C::NTH / 64;
+    constexpr int c0 = (int)coo_start(b0);
     (void)S;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    for (int b = b0 + wave; b < b1; b += NW) {
-        double *st_b = sm.stage + ((int)sst_start(b) - s0);
-        // K_ab[kk][ll] = lam S^{kk ll} + mu S^{ll kk} + mu d_kk,ll tr S, in place
-        for (int a = lane; a <= b; a += 64) {
-            double *s9 = st_b + 9 * a;
-            double Sv[9], Kv[9];
-#pragma unroll
-            for (int i = 0; i < 9; ++i) Sv[i] = s9[i];
-            const double tr = mu * (Sv[0] + Sv[4] + Sv[8]);
-            const double half = a == b ? 0.5 : 1.0;
-#pragma unroll
-            for (int ll = 0; ll < 3; ++ll)
-#pragma unroll
-                for (int kk = 0; kk < 3; ++kk) {
-                    if (kk == ll) Kv[kk + 3 * ll] = ((lam + mu) * Sv[kk + 3 * ll] + tr) * half;
-                    else Kv[kk + 3 * ll] = lam * Sv[kk + 3 * ll] + mu * Sv[ll + 3 * kk];
-                }
-#pragma unroll
-            for (int i = 0; i < 9; ++i) s9[i] = Kv[i];
-        }
-        // the copy reads values other lanes of this wave wrote
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int b = b0 + wave + NW * part; b < b1; b += NW * NPART) {
         const int len = 9 * b + 6;
         const int64_t gb = base + coo_start(b);
+        const double *st_b = sm.stage + ((int)coo_start(b) - c0);
         const int32_t dofb = WRITE_IDX ? sm.dof[b] : 0;
-        // four entries per lane per trip, their LDS reads issued together
-        for (int j0 = 0; j0 < len; j0 += 256) {
-            double v[4];
-            int pos[4];
+        for (int j0 = 0; j0 < len; j0 += 128) {
+            double v[2];
 #pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                const int j = min(j0 + 64 * h + lane, len - 1);
-                const int d = j - 9 * b;
-                pos[h] = d < 0 ? j : 9 * b + ((0x210100 >> (4 * d)) & 15) + 3 * ((0x222110 >> (4 * d)) & 15);
-                v[h] = st_b[pos[h]];
-            }
+            for (int h = 0; h < 2; ++h) v[h] = st_b[min(j0 + 64 * h + lane, len - 1)];
 #pragma unroll
-            for (int h = 0; h < 4; ++h) {
+            for (int h = 0; h < 2; ++h) {
                 const int j = j0 + 64 * h + lane;
                 if (j < len) {
                     Kdata[gb + j] = v[h];
                     if (WRITE_IDX) {
-                        const int a = pos[h] / 9;
-                        const int m = pos[h] - 9 * a;   // kk + 3 ll within the block
-                        const int ll = m / 3, kk = m - 3 * ll;
+                        int a, kk, ll;
+                        if (j < 9 * b) {
+                            a = j / 9;
+                            const int m = j - 9 * a;
+                            ll = m / 3;
+                            kk = m - 3 * ll;
+                        } else {
+                            const int d = j - 9 * b;
+                            a = b;
+                            kk = (0x210100 >> (4 * d)) & 15;
+                            ll = (0x222110 >> (4 * d)) & 15;
+                        }
                         Krow[gb + j] = sm.dof[a] + kk;
                         Kcol[gb + j] = dofb + ll;
                     }
@@ -552,15 +618,24 @@ __device__ __forceinline__ void copy_round(ElemSmem<P, NG> &sm, double lam, doub
     }
 }
 
-// Rounds of pass PASS for consumer wave W: stage, barrier, copy, (barrier before the next round).
+// Whether the last round's copy-out is deferred: it is then split into NCH - 1 slices of rows that
+// all waves copy during the following NCH - 1 steps (which have no epilogue), beside the MFMA work
+// and the chunk builds, instead of stalling the epilogue step. Needs at least two chunks per pass.
+template <int P, int NG>
+constexpr bool elem_defer_last() { return ElemCfg<P, NG>::NCH >= 2; }
+
+// Rounds of pass PASS for consumer wave W: stage, barrier, copy (all waves), barrier before the
+// next round; the last round is only staged when its copy is deferred.
 template <int P, int NG, bool WRITE_IDX, int PASS, int W, int R = 0>
 __device__ __forceinline__ void consumer_epilogue(ElemSmem<P, NG> &sm, double lam, double mu, int64_t base,
                                                   const ConsumerAcc<P, NG> &acc, double *__restrict__ Kdata,
                                                   int32_t *__restrict__ Krow, int32_t *__restrict__ Kcol) {
+    using C = ElemCfg<P, NG>;
     constexpr int NR = Sched<P, NG>::S.nround[PASS];
-    stage_round<P, NG, PASS, W, R>(sm, acc);
+    stage_round<P, NG, PASS, W, R>(sm, lam, mu, acc);
+    if constexpr (R + 1 == NR && elem_defer_last<P, NG>()) return;
     __syncthreads();
-    copy_round<P, NG, WRITE_IDX, PASS, R>(sm, lam, mu, base, Kdata, Krow, Kcol);
+    copy_round<P, NG, WRITE_IDX, PASS, R, C::NTH / 64>(sm, base, Kdata, Krow, Kcol);
     if constexpr (R + 1 < NR) {
         __syncthreads();
         consumer_epilogue<P, NG, WRITE_IDX, PASS, W, R + 1>(sm, lam, mu, base, acc, Kdata, Krow, Kcol);
@@ -569,26 +644,46 @@ __device__ __forceinline__ void consumer_epilogue(ElemSmem<P, NG> &sm, double la
 
 // The producers' side of the same rounds (same barriers, they help with the copies).
 template <int P, int NG, bool WRITE_IDX, int PASS, int R = 0>
-__device__ __forceinline__ void producer_epilogue_pass(ElemSmem<P, NG> &sm, double lam, double mu, int64_t base,
-                                                       double *__restrict__ Kdata, int32_t *__restrict__ Krow,
-                                                       int32_t *__restrict__ Kcol) {
+__device__ __forceinline__ void producer_epilogue_pass(ElemSmem<P, NG> &sm, int64_t base, double *__restrict__ Kdata,
+                                                       int32_t *__restrict__ Krow, int32_t *__restrict__ Kcol) {
+    using C = ElemCfg<P, NG>;
     constexpr int NR = Sched<P, NG>::S.nround[PASS];
+    if constexpr (R + 1 == NR && elem_defer_last<P, NG>()) return;
     __syncthreads();
-    copy_round<P, NG, WRITE_IDX, PASS, R>(sm, lam, mu, base, Kdata, Krow, Kcol);
+    copy_round<P, NG, WRITE_IDX, PASS, R, C::NTH / 64>(sm, base, Kdata, Krow, Kcol);
     if constexpr (R + 1 < NR) {
         __syncthreads();
-        producer_epilogue_pass<P, NG, WRITE_IDX, PASS, R + 1>(sm, lam, mu, base, Kdata, Krow, Kcol);
+        producer_epilogue_pass<P, NG, WRITE_IDX, PASS, R + 1>(sm, base, Kdata, Krow, Kcol);
     }
 }
 
 template <int P, int NG, bool WRITE_IDX, int PASS = 0>
-__device__ __forceinline__ void producer_epilogue(ElemSmem<P, NG> &sm, int pass, double lam, double mu,
-                                                  int64_t base, double *__restrict__ Kdata, int32_t *__restrict__ Krow,
+__device__ __forceinline__ void producer_epilogue(ElemSmem<P, NG> &sm, int pass, int64_t base,
+                                                  double *__restrict__ Kdata, int32_t *__restrict__ Krow,
                                                   int32_t *__restrict__ Kcol) {
     if (pass == PASS) {
-        producer_epilogue_pass<P, NG, WRITE_IDX, PASS>(sm, lam, mu, base, Kdata, Krow, Kcol);
+        producer_epilogue_pass<P, NG, WRITE_IDX, PASS>(sm, base, Kdata, Krow, Kcol);
     } else if constexpr (PASS + 1 < Sched<P, NG>::NPASS) {
-        producer_epilogue<P, NG, WRITE_IDX, PASS + 1>(sm, pass, lam, mu, base, Kdata, Krow, Kcol);
+        producer_epilogue<P, NG, WRITE_IDX, PASS + 1>(sm, pass, base, Kdata, Krow, Kcol);
+    }
+}
+
+// Slice `part` of the deferred copy-out of the last round of pass `pass` (all waves take part;
+// part = -1 copies the whole round).
+template <int P, int NG, bool WRITE_IDX, int PASS = 0>
+__device__ __forceinline__ void deferred_copy(const ElemSmem<P, NG> &sm, int pass, int part, int64_t base,
+                                              double *__restrict__ Kdata, int32_t *__restrict__ Krow,
+                                              int32_t *__restrict__ Kcol) {
+    using C = ElemCfg<P, NG>;
+    constexpr int NW = C::NTH / 64;
+    if (pass == PASS) {
+        constexpr int R = Sched<P, NG>::S.nround[PASS] - 1;
+        if (part < 0)
+            copy_round<P, NG, WRITE_IDX, PASS, R, NW>(sm, base, Kdata, Krow, Kcol);
+        else
+            copy_round<P, NG, WRITE_IDX, PASS, R, NW, C::NCH - 1>(sm, base, Kdata, Krow, Kcol, part);
+    } else if constexpr (PASS + 1 < Sched<P, NG>::NPASS) {
+        deferred_copy<P, NG, WRITE_IDX, PASS + 1>(sm, pass, part, base, Kdata, Krow, Kcol);
     }
 }
 
@@ -623,6 +718,8 @@ __device__ __forceinline__ void consumer_loop(ElemSmem<P, NG> &sm, const PatchAr
     using C = ElemCfg<P, NG>;
     constexpr int SPE = Sched<P, NG>::NPASS * C::NCH;
     ConsumerAcc<P, NG> acc;
+    int pend_pass = -1;     // pass whose last round waits in the stage for its copy-out
+    int64_t pend_base = 0;
     for (int s = 0; s < nsteps; ++s) {
         const int el = s / SPE, rem = s % SPE;
         const int pass = rem / C::NCH, ch = rem % C::NCH;
@@ -630,8 +727,19 @@ __device__ __forceinline__ void consumer_loop(ElemSmem<P, NG> &sm, const PatchAr
         const int e = pa.elems[gi];
         const int64_t base = pa.offs[gi];
         consumer_step<P, NG, WRITE_IDX, W>(sm, pa, s & 1, pass, ch, e, base, acc, Kdata, Krow, Kcol);
+        if constexpr (elem_defer_last<P, NG>()) {
+            if (ch == C::NCH - 1) {
+                pend_pass = pass;
+                pend_base = base;
+            } else if (pend_pass >= 0) {
+                deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, ch, pend_base, Kdata, Krow, Kcol);
+            }
+        }
         __syncthreads();
     }
+    // an unfinished deferred round (the stage is stable after the last barrier)
+    if constexpr (elem_defer_last<P, NG>())
+        if (pend_pass >= 0) deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, -1, pend_base, Kdata, Krow, Kcol);
 }
 
 template <int P, int NG, bool WRITE_IDX>
@@ -671,6 +779,8 @@ elem_stiffness_kernel(PatchArgs pa, const double *__restrict__ geo, double *__re
             gv = gnext;
         }
         __syncthreads();
+        int pend_pass = -1;     // pass whose last round waits in the stage for its copy-out
+        int64_t pend_base = 0;
         for (int s = 0; s < nsteps; ++s) {
             const int s1 = s + 1;
             if (s1 < nsteps) {
@@ -683,13 +793,22 @@ elem_stiffness_kernel(PatchArgs pa, const double *__restrict__ geo, double *__re
                 producer_chunk<P, NG>(sm, gv, pw, s1 & 1, ch * C::QC, pe);
                 gv = gnext;
             }
-            const int rem = s % SPE;
-            if (rem % C::NCH == C::NCH - 1) {
+            const int rem = s % SPE, ch = rem % C::NCH;
+            if (ch == C::NCH - 1) {
                 const int64_t gi = blockIdx.x + (int64_t)(s / SPE) * gridDim.x;
-                producer_epilogue<P, NG, WRITE_IDX>(sm, rem / C::NCH, pa.lam, pa.mu, pa.offs[gi], Kdata, Krow, Kcol);
+                producer_epilogue<P, NG, WRITE_IDX>(sm, rem / C::NCH, pa.offs[gi], Kdata, Krow, Kcol);
+                if constexpr (elem_defer_last<P, NG>()) {
+                    pend_pass = rem / C::NCH;
+                    pend_base = pa.offs[gi];
+                }
+            } else if (elem_defer_last<P, NG>() && pend_pass >= 0) {
+                deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, ch, pend_base, Kdata, Krow, Kcol);
             }
             __syncthreads();
         }
+        // an unfinished deferred round (the stage is stable after the last barrier)
+        if constexpr (elem_defer_last<P, NG>())
+            if (pend_pass >= 0) deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, -1, pend_base, Kdata, Krow, Kcol);
     } else {
         __syncthreads();
         switch (wave) {

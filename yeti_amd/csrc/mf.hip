@@ -24,7 +24,7 @@
 #include <cmath>
 #include <vector>
 
-#include "common.hpp"
+#include "session.hpp"
 
 namespace yeti {
 
@@ -715,23 +715,14 @@ static bool gen_eigen(int n, const std::vector<double> &K, const std::vector<dou
     return true;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) fdm_gather_kernel(FdmArgs fa, const T *__restrict__ r, T *__restrict__ w,
-                                                         const int32_t *__restrict__ pred) {
-    if (pred && *pred == 0) return;
-    const int c = blockIdx.y;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= fa.tot[c]) return;
-    const int m0 = fa.m[c][0], m1 = fa.m[c][1];
-    const int i = (int)(t % m0), j = (int)((t / m0) % m1), k = (int)(t / ((int64_t)m0 * m1));
-    const int64_t g = (fa.lo[c][0] + i) + (int64_t)fa.n[0] * ((fa.lo[c][1] + j) + (int64_t)fa.n[1] * (fa.lo[c][2] + k));
-    w[c * fa.maxtot + t] = r[3 * (int64_t)fa.cpmap[g] + c];
-}
-
-// out = (mode-`axis` product with A) in, A[a][b] read from P as P[b m + a] (axis 0) or P[a m + b]
+// out = (mode-`axis` product with A) in, A[a][b] read from P as P[b m + a] (use_transpose_store,
+// for axis 0, where the lanes run over a) or P[a m + b]. gather_r (axis 0 only): the input is
+// read from the DOF vector r through the CP map (the free box of component c) instead of a
+// tensor; scale: the output is multiplied by 1 / D_c.
 template <typename T>
 __global__ void __launch_bounds__(256) fdm_mode_kernel(FdmArgs fa, const T *__restrict__ in, T *__restrict__ out,
                                                        const T *__restrict__ P, int axis, int use_transpose_store,
+                                                       const T *__restrict__ r, const T *__restrict__ invD,
                                                        const int32_t *__restrict__ pred) {
     if (pred && *pred == 0) return;
     const int c = blockIdx.y;
@@ -743,8 +734,14 @@ __global__ void __launch_bounds__(256) fdm_mode_kernel(FdmArgs fa, const T *__re
     const T *A = P + fa.uoff[c][axis];
     double s = 0.0;
     if (axis == 0) {
-        const T *line = src + (int64_t)m0 * (j + (int64_t)m1 * k);
-        for (int b = 0; b < m0; ++b) s += (double)A[use_transpose_store ? b * m0 + i : i * m0 + b] * (double)line[b];
+        if (r) {
+            const int64_t g0 = fa.lo[c][0] + (int64_t)fa.n[0] * ((fa.lo[c][1] + j) + (int64_t)fa.n[1] * (fa.lo[c][2] + k));
+            for (int b = 0; b < m0; ++b)
+                s += (double)A[use_transpose_store ? b * m0 + i : i * m0 + b] * (double)r[3 * (int64_t)fa.cpmap[g0 + b] + c];
+        } else {
+            const T *line = src + (int64_t)m0 * (j + (int64_t)m1 * k);
+            for (int b = 0; b < m0; ++b) s += (double)A[use_transpose_store ? b * m0 + i : i * m0 + b] * (double)line[b];
+        }
     } else if (axis == 1) {
         const T *line = src + i + (int64_t)m0 * m1 * k;
         for (int b = 0; b < m1; ++b) s += (double)A[use_transpose_store ? b * m1 + j : j * m1 + b] * (double)line[(int64_t)m0 * b];
@@ -753,24 +750,17 @@ __global__ void __launch_bounds__(256) fdm_mode_kernel(FdmArgs fa, const T *__re
         for (int b = 0; b < m2; ++b)
             s += (double)A[use_transpose_store ? b * m2 + k : k * m2 + b] * (double)line[(int64_t)m0 * m1 * b];
     }
-    out[c * fa.maxtot + t] = (T)s;
+    T v = (T)s;
+    if (invD) v *= invD[fa.doff[c] + t];
+    out[c * fa.maxtot + t] = v;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) fdm_scale_kernel(FdmArgs fa, T *__restrict__ w, const T *__restrict__ invD,
-                                                        const int32_t *__restrict__ pred) {
-    if (pred && *pred == 0) return;
-    const int c = blockIdx.y;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= fa.tot[c]) return;
-    w[c * fa.maxtot + t] *= invD[fa.doff[c] + t];
-}
-
-// z on every CP (0 outside the free boxes and on masked dofs); partial r.z per block
+// z on every CP (0 outside the free boxes and on masked dofs); r.z reduced in the same launch
+// and handed to rz_tail (the session's scalars: first direction, or beta and the iteration count)
 template <typename T>
 __global__ void __launch_bounds__(256) fdm_scatter_kernel(FdmArgs fa, int64_t ncp, const T *__restrict__ w,
                                                           const T *__restrict__ r, const uint8_t *__restrict__ mask,
-                                                          T *__restrict__ z, double *__restrict__ part,
+                                                          T *__restrict__ z, double *__restrict__ part, RzTail tail,
                                                           const int32_t *__restrict__ pred) {
     if (pred && *pred == 0) return;
     __shared__ double sh[4];
@@ -787,43 +777,44 @@ __global__ void __launch_bounds__(256) fdm_scatter_kernel(FdmArgs fa, int64_t nc
             acc += (double)r[3 * cp + c] * (double)v;
         }
     }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    double tot;
+    if (last_block_total(block_sum256(acc, sh), part, tail.flags + F_COUNTER, sh, tot) && threadIdx.x == 0)
+        rz_tail(tail, tot);
 }
 
+// 7 launches: three forward mode products (the first gathers r, the third scales by 1/D), three
+// backward ones and the scatter with r.z.
 template <typename T>
 static int fdm_apply_t(const yeti_fdm *f, const T *U, const T *Ut, const T *invD, T *wa, T *wb, const T *r,
-                       const uint8_t *mask, T *z, double *part, int nblocks, hipStream_t st, const int32_t *pred) {
+                       const uint8_t *mask, T *z, double *part, int nblocks, const RzTail &tail, hipStream_t st,
+                       const int32_t *pred) {
     const FdmArgs &fa = f->fa;
     dim3 grid((unsigned)((fa.maxtot + 255) / 256), 3);
-    hipLaunchKernelGGL(fdm_gather_kernel<T>, grid, dim3(256), 0, st, fa, r, wa, pred);
+    const T *nul = nullptr;
     // forward: A = U^T along axes 0, 1, 2. Along axis 0 the lanes run over a, so A[a][b] is read
     // as U[b][a] (consecutive addresses); along axes 1, 2 a is wave-uniform: Ut[a][b].
-    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wa, wb, U, 0, 1, pred);
-    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, Ut, 1, 0, pred);
-    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wa, wb, Ut, 2, 0, pred);
-    hipLaunchKernelGGL(fdm_scale_kernel<T>, grid, dim3(256), 0, st, fa, wb, invD, pred);
+    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, nul, wb, U, 0, 1, r, nul, pred);
+    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, Ut, 1, 0, nul, nul, pred);
+    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wa, wb, Ut, 2, 0, nul, invD, pred);
     // backward: A = U (axis 0 reads Ut[b][a])
-    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, Ut, 0, 1, pred);
-    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wa, wb, U, 1, 0, pred);
-    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, U, 2, 0, pred);
+    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, Ut, 0, 1, nul, nul, pred);
+    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wa, wb, U, 1, 0, nul, nul, pred);
+    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, U, 2, 0, nul, nul, pred);
     hipLaunchKernelGGL(fdm_scatter_kernel<T>, dim3(nblocks), dim3(256), 0, st, fa, (int64_t)f->ncp, wa, r, mask, z,
-                       part, pred);
+                       part, tail, pred);
     YETI_HIP(hipGetLastError());
     return 0;
 }
 
 int fdm_apply(const yeti_fdm *f, const double *r, const uint8_t *mask, double *z, double *part, int nblocks,
-              hipStream_t st, const int32_t *pred) {
+              const RzTail &tail, hipStream_t st, const int32_t *pred) {
     return fdm_apply_t<double>(f, f->U64.p, f->Ut64.p, f->invD64.p, f->wa64.p, f->wb64.p, r, mask, z, part, nblocks,
-                               st, pred);
+                               tail, st, pred);
 }
 int fdm_apply(const yeti_fdm *f, const float *r, const uint8_t *mask, float *z, double *part, int nblocks,
-              hipStream_t st, const int32_t *pred) {
+              const RzTail &tail, hipStream_t st, const int32_t *pred) {
     return fdm_apply_t<float>(f, f->U32.p, f->Ut32.p, f->invD32.p, f->wa32.p, f->wb32.p, r, mask, z, part, nblocks,
-                              st, pred);
+                              tail, st, pred);
 }
 
 // Host set-up. mask_host: free-dof mask (3 nb_cp). precision 32 or 64. Device memory comes from

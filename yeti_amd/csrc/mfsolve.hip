@@ -13,7 +13,7 @@
 #include <cstddef>
 #include <vector>
 
-#include "common.hpp"
+#include "session.hpp"
 
 extern "C" {
 int32_t yeti_mf_apply_f64(const yeti_mf *m, const double *x, double *y, void *stream);
@@ -25,18 +25,6 @@ int32_t yeti_mf_nb_cp(const yeti_mf *m);
 namespace yeti {
 
 constexpr int NRED = 1024;
-
-__device__ inline double block_sum256(double v, double *sh) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) sh[w] = v;
-    __syncthreads();
-    double s = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
-    __syncthreads();
-    return s;
-}
 
 __global__ void sum_partials_kernel(const double *part, int n, double *out) {
     __shared__ double sh[4];
@@ -369,14 +357,17 @@ int32_t yeti_mf_solve_ir(const yeti_mf *m, const double *b, const uint8_t *free_
 // One session solves K[free][:, free] x = b[free] for one handle with preconditioned CG whose
 // vectors and operator run in FP32 (or FP64) while the solution and the true residual are kept
 // in FP64 and refreshed by residual replacement ("reliable updates"): the FP32 iteration
-// accumulates its update y; whenever its recurred residual has fallen by `delta` since the last
-// replacement (or meets the tolerance) x += y, r = mask (b - K x) with the FP64 apply, and the
-// FP32 residual restarts from r while the search direction is kept. The Krylov space is not
+// accumulates its update y; when its recurred residual has fallen by `delta` since the last
+// replacement (or meets the tolerance), then at the end of the current step x += y,
+// r = mask (b - K x) with the FP64 apply, and the FP32 residual restarts from r while the search
+// direction is kept. The Krylov space is not
 // restarted, so an ill-conditioned system converges in about the iterations of FP64 CG, and the
 // stopping test ||b - K x||_2 <= rtol ||b||_2 is always made on the FP64 residual.
 // Control state lives in device memory: every kernel of a step is predicated on device flags,
 // so a step (`chunk` iterations) is a fixed launch sequence that is captured once in a hipGraph
-// and replayed until the session reports done; the host reads a small status block. The
+// and replayed until the session reports done; the host reads a small status block. Dot
+// products are finished inside their producing launch (the last block to arrive adds the
+// per-block partials in block order), so an iteration is 6 launches plus the preconditioner's. The
 // iterates depend only on the session's own launch sequence: a request gives bit-identical
 // results alone or in a mix. All device memory is taken at creation (optionally in an arena).
 namespace yeti {
@@ -387,14 +378,9 @@ int fdm_create(const yeti_mf *m, const uint8_t *mask_host, int precision, hipStr
 void fdm_destroy(yeti_fdm *f);
 size_t fdm_bytes_bound(const yeti_mf *m, int precision);
 int fdm_apply(const yeti_fdm *f, const double *r, const uint8_t *mask, double *z, double *part, int nblocks,
-              hipStream_t st, const int32_t *pred);
+              const RzTail &tail, hipStream_t st, const int32_t *pred);
 int fdm_apply(const yeti_fdm *f, const float *r, const uint8_t *mask, float *z, double *part, int nblocks,
-              hipStream_t st, const int32_t *pred);
-
-// sc[] slots (doubles), in the order of yeti_ir_status
-enum : int { S_RZ = 0, S_PQ, S_RR, S_RZN, S_BB, S_RRT, S_RRMAX, S_TOL2, S_RRT_PREV, S_LASTUPD, S_NSC = 16 };
-// flags[] slots (int32), in the order of yeti_ir_status
-enum : int { F_ACTIVE = 0, F_UPDATE, F_STAGNANT, F_DONE, F_FAILED, F_ITERS, F_FLOOR, F_UPDATES, F_NFLAGS = 8 };
+              const RzTail &tail, hipStream_t st, const int32_t *pred);
 
 // iterations after which a residual replacement is forced (so a solve at the FP64 rounding
 // floor is recognised within a few such periods)
@@ -409,9 +395,11 @@ __global__ void pred_sum_partials_kernel(const double *part, int n, double *out,
     if (threadIdx.x == 0) *out = t;
 }
 
+// 3x3 block-Jacobi z = M^-1 r; r.z reduced in the same launch and handed to rz_tail
 template <typename T>
 __global__ void __launch_bounds__(256) pred_precond_kernel(int32_t ncp, const T *__restrict__ inv, const T *__restrict__ r,
-                                                           T *__restrict__ z, double *part, const int32_t *pred) {
+                                                           T *__restrict__ z, double *part, RzTail tail,
+                                                           const int32_t *pred) {
     if (*pred == 0) return;
     __shared__ double sh[4];
     double acc = 0.0;
@@ -426,14 +414,17 @@ __global__ void __launch_bounds__(256) pred_precond_kernel(int32_t ncp, const T 
         z[3 * cp + 2] = z2;
         acc += (double)r0 * (double)z0 + (double)r1 * (double)z1 + (double)r2 * (double)z2;
     }
-    double t = block_sum256(acc, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    double tot;
+    if (last_block_total(block_sum256(acc, sh), part, tail.flags + F_COUNTER, sh, tot) && threadIdx.x == 0)
+        rz_tail(tail, tot);
 }
 
+// q = mask q; sc[S_PQ] = p.q
 template <typename T>
 __global__ void __launch_bounds__(256) pred_mask_dot_kernel(int64_t n, const uint8_t *mask, const T *__restrict__ p,
-                                                            T *__restrict__ q, double *part, const int32_t *pred) {
-    if (*pred == 0) return;
+                                                            T *__restrict__ q, double *part, double *sc,
+                                                            int32_t *flags) {
+    if (flags[F_ACTIVE] == 0) return;
     __shared__ double sh[4];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -441,16 +432,18 @@ __global__ void __launch_bounds__(256) pred_mask_dot_kernel(int64_t n, const uin
         q[i] = qi;
         acc += (double)p[i] * (double)qi;
     }
-    double t = block_sum256(acc, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    double tot;
+    if (last_block_total(block_sum256(acc, sh), part, flags + F_COUNTER, sh, tot) && threadIdx.x == 0) sc[S_PQ] = tot;
 }
 
-// y += alpha p, r -= alpha q; partial r.r
+// y += alpha p, r -= alpha q; sc[S_RR] = r.r, then the replacement test: a residual replacement
+// is due when r.r has fallen by delta^2 since the last one, meets the tolerance, or after
+// IR_FORCE_EVERY iterations (it is carried out at the end of the step).
 template <typename T>
-__global__ void __launch_bounds__(256) pred_xr_kernel(int64_t n, const double *sc, T *__restrict__ y, T *__restrict__ r,
+__global__ void __launch_bounds__(256) pred_xr_kernel(int64_t n, double *sc, T *__restrict__ y, T *__restrict__ r,
                                                       const T *__restrict__ p, const T *__restrict__ q, double *part,
-                                                      const int32_t *pred) {
-    if (*pred == 0) return;
+                                                      int32_t *flags, double delta2) {
+    if (flags[F_ACTIVE] == 0) return;
     __shared__ double sh[4];
     const double alpha = sc[S_RZ] / sc[S_PQ];
     double acc = 0.0;
@@ -460,27 +453,23 @@ __global__ void __launch_bounds__(256) pred_xr_kernel(int64_t n, const double *s
         r[i] = ri;
         acc += (double)ri * (double)ri;
     }
-    double t = block_sum256(acc, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    double rr;
+    if (last_block_total(block_sum256(acc, sh), part, flags + F_COUNTER, sh, rr) && threadIdx.x == 0) {
+        sc[S_RR] = rr;
+        if (rr > sc[S_RRMAX]) sc[S_RRMAX] = rr;
+        if (!(rr > delta2 * sc[S_RRMAX]) || !(rr > sc[S_TOL2]) ||
+            flags[F_ITERS] + 1 - (int32_t)sc[S_LASTUPD] >= IR_FORCE_EVERY)
+            flags[F_UPDATE] = 1;
+    }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) pred_p_kernel(int64_t n, const double *sc, T *__restrict__ p, const T *__restrict__ z,
                                                      const int32_t *pred) {
     if (*pred == 0) return;
-    const double beta = sc[S_RZN] / sc[S_RZ];
+    const double beta = sc[S_BETA];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = (T)((double)z[i] + beta * (double)p[i]);
-}
-
-// after the residual update of an iteration: replace the residual when it has fallen by delta
-// since the last replacement, or when it meets the tolerance
-__global__ void ir_check_kernel(double *sc, int32_t *flags, double delta2) {
-    if (flags[F_ACTIVE] == 0) return;
-    const double rr = sc[S_RR];
-    if (rr > sc[S_RRMAX]) sc[S_RRMAX] = rr;
-    if (!(rr > delta2 * sc[S_RRMAX]) || !(rr > sc[S_TOL2]) || flags[F_ITERS] + 1 - (int32_t)sc[S_LASTUPD] >= IR_FORCE_EVERY)
-        flags[F_UPDATE] = 1;
 }
 
 // x += y, y = 0 (FP64 accumulation of the FP32/FP64 update), predicated on F_UPDATE
@@ -493,12 +482,15 @@ __global__ void __launch_bounds__(256) ir_accumulate_kernel(int64_t n, double *x
     }
 }
 
-// r64 = mask (b - K x); r = (T) r64; partial r64.r64 (predicated)
+// Residual replacement, predicated on F_UPDATE: r64 = mask (b - K x); r = (T) r64; then, in the
+// last block, the end of the replacement: stop at the tolerance, and count replacements that do
+// not improve the true residual (stagnation at the FP64 rounding floor of ill-conditioned systems).
 template <typename T>
 __global__ void __launch_bounds__(256) ir_true_residual_kernel(int64_t n, const double *b, const double *Kx,
-                                                               const uint8_t *mask, T *r, double *part,
-                                                               const int32_t *pred) {
-    if (*pred == 0) return;
+                                                               const uint8_t *mask, T *r, double *part, double *sc,
+                                                               int32_t *flags, double rtol2, int32_t max_updates,
+                                                               int32_t max_stagnant) {
+    if (flags[F_UPDATE] == 0) return;
     __shared__ double sh[4];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -506,19 +498,12 @@ __global__ void __launch_bounds__(256) ir_true_residual_kernel(int64_t n, const 
         r[i] = (T)v;
         acc += v * v;
     }
-    double t = block_sum256(acc, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
-}
-
-// end of a replacement: stop at the tolerance; count replacements that do not improve the true
-// residual (stagnation at the FP64 rounding floor of ill-conditioned systems)
-__global__ void ir_update_done_kernel(double *sc, int32_t *flags, double rtol2, int32_t max_updates,
-                                      int32_t max_stagnant) {
-    if (flags[F_UPDATE] == 0) return;
+    double rrt;
+    if (!last_block_total(block_sum256(acc, sh), part, flags + F_COUNTER, sh, rrt) || threadIdx.x != 0) return;
+    sc[S_RRT] = rrt;
     flags[F_UPDATE] = 0;
     flags[F_UPDATES] += 1;
     sc[S_LASTUPD] = (double)(flags[F_ITERS] + 1);
-    const double rrt = sc[S_RRT];
     sc[S_RR] = rrt;
     sc[S_RRMAX] = rrt;
     if (!(rrt > rtol2 * sc[S_BB])) {
@@ -534,18 +519,6 @@ __global__ void ir_update_done_kernel(double *sc, int32_t *flags, double rtol2, 
         flags[F_FLOOR] = 1;
         flags[F_ACTIVE] = 0;
     } else if (flags[F_UPDATES] >= max_updates) {
-        flags[F_DONE] = 1;
-        flags[F_FAILED] = 1;
-        flags[F_ACTIVE] = 0;
-    }
-}
-
-// end of an iteration: roll rz, count, cap
-__global__ void ir_roll_kernel(double *sc, int32_t *flags, int32_t max_iters) {
-    if (flags[F_ACTIVE] == 0) return;
-    sc[S_RZ] = sc[S_RZN];
-    flags[F_ITERS] += 1;
-    if (flags[F_ITERS] >= max_iters) {
         flags[F_DONE] = 1;
         flags[F_FAILED] = 1;
         flags[F_ACTIVE] = 0;
@@ -576,7 +549,8 @@ __global__ void ir_begin_flags_kernel(double *sc, int32_t *flags, double rtol2) 
     sc[S_RRT_PREV] = bb;
     sc[S_TOL2] = rtol2 * bb;
     sc[S_LASTUPD] = 0.0;
-    for (int i = 0; i < F_NFLAGS; ++i) flags[i] = 0;
+    sc[S_BETA] = 0.0;
+    for (int i = 0; i < F_ALLOC; ++i) flags[i] = 0;   // includes the reduction counter
     if (bb > 0.0) flags[F_ACTIVE] = 1;
     else flags[F_DONE] = 1;
 }
@@ -587,11 +561,6 @@ __global__ void __launch_bounds__(256) ir_first_p_kernel(int64_t n, const T *z, 
     if (flags[F_ACTIVE] == 0) return;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = z[i];
-}
-
-__global__ void ir_first_rz_kernel(double *sc, const int32_t *flags) {
-    if (flags[F_ACTIVE] == 0) return;
-    sc[S_RZ] = sc[S_RZN];
 }
 
 }  // namespace yeti
@@ -626,43 +595,51 @@ namespace yeti {
 
 constexpr int32_t IR_MAX_STAGNANT = 4;
 
-// z = M^-1 r and the partials of r.z (nbc blocks), predicated
+// z = M^-1 r; r.z goes to rz_tail in the same launch sequence (mode 0: first direction, 1: end
+// of an iteration), predicated
 template <typename T>
-static int ir_precond(yeti_ir *s, const T *inv, const T *r, T *z, hipStream_t st, const int32_t *pred) {
-    if (s->fdm) return fdm_apply(s->fdm, r, s->mask, z, s->part.p, (int)s->nbc, st, pred);
-    hipLaunchKernelGGL(pred_precond_kernel<T>, dim3(s->nbc), dim3(256), 0, st, s->ncp, inv, r, z, s->part.p, pred);
+static int ir_precond(yeti_ir *s, const T *inv, const T *r, T *z, int mode, hipStream_t st, const int32_t *pred) {
+    RzTail tail;
+    tail.sc = s->sc.p;
+    tail.flags = s->flags.p;
+    tail.mode = mode;
+    tail.max_iters = s->max_iters;
+    if (s->fdm) return fdm_apply(s->fdm, r, s->mask, z, s->part.p, (int)s->nbc, tail, st, pred);
+    hipLaunchKernelGGL(pred_precond_kernel<T>, dim3(s->nbc), dim3(256), 0, st, s->ncp, inv, r, z, s->part.p, tail, pred);
     YETI_HIP(hipGetLastError());
     return 0;
 }
 
-// the launch sequence of one step (captured into the session's graph)
+// The launch sequence of one step (captured into the session's graph): `chunk` PCG iterations,
+// the residual replacement when one fell due during them, and the new direction. Per iteration:
+// the apply (2 launches per patch), p.q, the x/r update with r.r and the replacement test, the
+// preconditioner with r.z, beta and the iteration count, and p = z + beta p. The replacement
+// (x += y, the FP64 apply, the true residual and the stopping tests) is placed once per step,
+// before the last preconditioner, so it costs 4 launches per step instead of per iteration.
 template <typename T>
 static int ir_step_launches(yeti_ir *s, T *r, T *y, T *z, T *p, T *q, const T *inv, hipStream_t st) {
     const int32_t *act = s->flags.p + F_ACTIVE;
     const int32_t *upd = s->flags.p + F_UPDATE;
+    const double delta2 = s->delta * s->delta;
     for (int k = 0; k < s->chunk; ++k) {
         int rc = mf_apply_pred(s->m, p, q, st, act);
         if (rc) return rc;
-        hipLaunchKernelGGL(pred_mask_dot_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->mask, p, q, s->part.p, act);
-        hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_PQ, act);
-        hipLaunchKernelGGL(pred_xr_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->sc.p, y, r, p, q, s->part.p, act);
-        hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_RR, act);
-        hipLaunchKernelGGL(ir_check_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->delta * s->delta);
-        // residual replacement (predicated)
-        hipLaunchKernelGGL(ir_accumulate_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->x, y, s->flags.p);
-        rc = mf_apply_pred(s->m, (const double *)s->x, s->y64.p, st, upd);
-        if (rc) return rc;
-        hipLaunchKernelGGL(ir_true_residual_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->b, s->y64.p, s->mask, r,
-                           s->part.p, upd);
-        hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_RRT, upd);
-        hipLaunchKernelGGL(ir_update_done_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->rtol * s->rtol,
-                           s->max_updates, IR_MAX_STAGNANT);
+        hipLaunchKernelGGL(pred_mask_dot_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->mask, p, q, s->part.p,
+                           s->sc.p, s->flags.p);
+        hipLaunchKernelGGL(pred_xr_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->sc.p, y, r, p, q, s->part.p,
+                           s->flags.p, delta2);
+        if (k == s->chunk - 1) {
+            // residual replacement (predicated on F_UPDATE)
+            hipLaunchKernelGGL(ir_accumulate_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->x, y, s->flags.p);
+            rc = mf_apply_pred(s->m, (const double *)s->x, s->y64.p, st, upd);
+            if (rc) return rc;
+            hipLaunchKernelGGL(ir_true_residual_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->b, s->y64.p, s->mask,
+                               r, s->part.p, s->sc.p, s->flags.p, s->rtol * s->rtol, s->max_updates, IR_MAX_STAGNANT);
+        }
         // preconditioner and direction
-        rc = ir_precond<T>(s, inv, r, z, st, act);
+        rc = ir_precond<T>(s, inv, r, z, 1, st, act);
         if (rc) return rc;
-        hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nbc, s->sc.p + S_RZN, act);
         hipLaunchKernelGGL(pred_p_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->sc.p, p, z, act);
-        hipLaunchKernelGGL(ir_roll_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->max_iters);
         YETI_HIP(hipGetLastError());
     }
     return 0;
@@ -681,11 +658,9 @@ static int ir_begin_launches(yeti_ir *s, T *r, T *y, T *z, T *p, const T *inv, h
     hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_BB,
                        (const int32_t *)nullptr);
     hipLaunchKernelGGL(ir_begin_flags_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->rtol * s->rtol);
-    int rc = ir_precond<T>(s, inv, r, z, st, act);
+    int rc = ir_precond<T>(s, inv, r, z, 0, st, act);
     if (rc) return rc;
-    hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nbc, s->sc.p + S_RZN, act);
     hipLaunchKernelGGL(ir_first_p_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, z, p, s->flags.p);
-    hipLaunchKernelGGL(ir_first_rz_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p);
     YETI_HIP(hipGetLastError());
     return 0;
 }
@@ -731,7 +706,7 @@ int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask, int32_t preci
     if (e == hipSuccess) e = s->y64.alloc(n);
     if (e == hipSuccess) e = s->part.alloc(NRED);
     if (e == hipSuccess) e = s->sc.alloc(S_NSC);
-    if (e == hipSuccess) e = s->flags.alloc(F_NFLAGS);
+    if (e == hipSuccess) e = s->flags.alloc(F_ALLOC);
     if (s->inner64) {
         if (e == hipSuccess) e = s->inv64.alloc(9 * (size_t)s->ncp);
         for (auto *v : {&s->vr64, &s->vy64, &s->vz64, &s->vp64, &s->vq64})

@@ -173,9 +173,13 @@ class _Active:
 
 
 class SolveServer:
-    """Runs a list of requests against one arena of `budget_bytes`."""
+    """Runs a list of requests against one arena of `budget_bytes`.
 
-    def __init__(self, budget_bytes, precision="mixed", nstreams=8, chunk=8, rtol=1e-8, delta=0.1,
+    Each active request is a solve session on one of `nstreams` streams; HIP spreads streams over
+    GPU_MAX_HW_QUEUES hardware queues (default 4), so a serving process should set that variable
+    to about `nstreams` before the HIP runtime starts for the sessions to overlap."""
+
+    def __init__(self, budget_bytes, precision="mixed", nstreams=16, chunk=16, rtol=1e-8, delta=0.1,
                  max_iters=100000, max_updates=400, use_graph=True, prep_threads=2, precond="fdm"):
         import torch
         _lib.require_device()
