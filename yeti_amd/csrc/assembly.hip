@@ -674,7 +674,7 @@ static int launch_elem(yeti_asm *h, PatchDev &pd, const double *coords, double *
     using C = ElemCfg<P, NG>;
     if (pd.n_active == 0) return 0;
     const int32_t batch = std::min(pd.n_active, ELEM_BATCH);
-    const size_t geo_n = (size_t)batch * ELEM_NGEO * C::NQ;
+    const size_t geo_n = (size_t)batch * C::REC;
     if (pd.geo.n < geo_n) YETI_HIP(pd.geo.alloc(geo_n));
     PatchArgs pa;
     pa.coords = coords;

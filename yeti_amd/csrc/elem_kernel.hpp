@@ -13,9 +13,10 @@
 //      * 8 consumer waves accumulate S^{kl}_{ab} = sum_q G_k[a][q] G_l[b][q] over the chunk on
 //        v_mfma_f64_16x16x4f64. The 16x16 tiles of a 16-node tile pair (I, J) are grouped in
 //        units that contract independently: D = {S^00, S^11, S^22} -> K_ab[k][k], and
-//        P(k,l) = {S^kl, S^lk} -> K_ab[k][l], K_ab[l][k] (k < l); units are dealt to the waves
-//        in pair-major runs (about 11 tiles per wave), so a wave loads each distinct operand
-//        fragment once per two K steps and most of its units share fragments;
+//        P(k,l) = {S^kl, S^lk} -> K_ab[k][l], K_ab[l][k] (k < l; on a diagonal pair only S^kl,
+//        whose transpose is S^lk); units are dealt to the waves in pair-major runs (about 10
+//        tiles per wave at p = 3), so a wave loads each distinct operand fragment once per two K
+//        steps and most of its units share fragments;
 //    one __syncthreads() ends each step. After an element's last chunk of a pass the consumers
 //    contract their units in registers, K_ab[k][l] = lam S^{kl}_ab + mu S^{lk}_ab +
 //    mu d_kl tr S_ab (the isotropic form of stiffmatrix_byCP's BoI^T D BoJ,
@@ -58,12 +59,21 @@ struct ElemCfg {
     static constexpr int CW = 8;                        // consumer waves
     static constexpr int PW = 4;                        // producer waves
     static constexpr int NTH = (CW + PW) * 64;
+    // waves that copy the staged COO values out: the consumers only. The producers' loads then
+    // never queue behind stores of their own (the vector memory counter covers both), so their
+    // next chunk's geometry arrives while they build the current one.
+    static constexpr bool PROD_COPY = false;
+    static constexpr int CPW = PROD_COPY ? CW + PW : CW;
     static constexpr int MAXTILE = NNP >= 128 ? 9 : 12; // accumulator tiles per consumer wave per pass
     static constexpr int MAXU = 8;                      // units per consumer wave per pass
     static constexpr int MAXF = 12;                     // distinct A (or B) fragments per wave
     static constexpr int MAXPASS = 8;
     static constexpr int MAXR = 8;                      // staging rounds per pass
     static constexpr int GPW = QC / PW;                 // Gauss points per producer wave per chunk
+    // per-element record written by the geometry pass: 12 geometry values per Gauss point
+    // ([v][q]), the 1D basis tables B, dB ([which][d][g][i]) and the node weights
+    static constexpr int NTAB = 2 * 3 * NG * NP1;
+    static constexpr int64_t REC = (int64_t)ELEM_NGEO * NQ + NTAB + NN;
     static constexpr int64_t NDE = 9LL * NN * (NN - 1) / 2 + 6LL * NN;   // COO entries per element
     // LDS stage of one round of COO values: what is left of ~150 KiB after G
     static constexpr int64_t GBYTES = 2LL * 3 * NNP * QCP * 8;
@@ -75,12 +85,21 @@ struct ElemCfg {
 __host__ __device__ constexpr int64_t coo_start(int b) { return (9LL * b * b + 3LL * b) / 2; }
 
 // Static schedule of consumer units over waves and passes. Unit code = pair * 4 + kind, kind 0:
-// D (tiles S^00, S^11, S^22), 1: P(0,1), 2: P(0,2), 3: P(1,2) (tiles S^kl, S^lk). A pass takes
+// D (tiles S^00, S^11, S^22), 1: P(0,1), 2: P(0,2), 3: P(1,2) (tiles S^kl, S^lk; on a diagonal
+// pair I = J only S^kl, since S^lk_ab = S^kl_ba lies in the same tile). A pass takes
 // whole tile columns J (all pairs I <= J), so the COO entries of nodes b in [16 J0, 16 J1) are
-// complete after it; its units are dealt in pair-major order to consecutive waves, each filled
-// up to T tiles with the smallest T that places every unit (so the waves of a SIMD get nearly
-// equal MFMA work). Each pass's b-range is emitted in rounds that fit the LDS stage.
-__host__ __device__ constexpr int unit_tiles(int kind) { return kind == 0 ? 3 : 2; }
+// complete after it; its units are dealt in pair-major runs, balanced over the 4 SIMDs (waves w
+// and w + 4 of the workgroup share one) and then over the two waves of each SIMD. Each pass's
+// b-range is emitted in rounds that fit the LDS stage.
+__host__ __device__ constexpr int pair_J(int pi) {
+    int j = 0;
+    while ((j + 1) * (j + 2) / 2 <= pi) ++j;
+    return j;
+}
+__host__ __device__ constexpr int pair_I(int pi) { return pi - pair_J(pi) * (pair_J(pi) + 1) / 2; }
+__host__ __device__ constexpr int unit_tiles(int kind, int pi) {
+    return kind == 0 ? 3 : pair_I(pi) == pair_J(pi) ? 1 : 2;
+}
 __host__ __device__ constexpr int unit_k(int kind, int t) { return kind == 0 ? t : t == 0 ? (kind == 3 ? 1 : 0) : (kind == 1 ? 1 : 2); }
 __host__ __device__ constexpr int unit_l(int kind, int t) { return kind == 0 ? t : t == 1 ? (kind == 3 ? 1 : 0) : (kind == 1 ? 1 : 2); }
 
@@ -101,13 +120,6 @@ struct UnitSchedule {
     bool ok;
 };
 
-__host__ __device__ constexpr int pair_J(int pi) {
-    int j = 0;
-    while ((j + 1) * (j + 2) / 2 <= pi) ++j;
-    return j;
-}
-__host__ __device__ constexpr int pair_I(int pi) { return pi - pair_J(pi) * (pair_J(pi) + 1) / 2; }
-
 template <int NT, int WAVES, int MAXU, int MAXTILE, int MAXF, int MAXPASS, int MAXR>
 constexpr UnitSchedule<NT, WAVES, MAXU, MAXTILE, MAXF, MAXPASS, MAXR> make_schedule(int nn, int stg) {
     UnitSchedule<NT, WAVES, MAXU, MAXTILE, MAXF, MAXPASS, MAXR> s{};
@@ -115,8 +127,8 @@ constexpr UnitSchedule<NT, WAVES, MAXU, MAXTILE, MAXF, MAXPASS, MAXR> make_sched
     int J = 0, pass = 0;
     while (J < NT && pass < MAXPASS) {
         int J1 = J, tiles = 0;
-        while (J1 < NT && tiles + 9 * (J1 + 1) <= MAXTILE * WAVES) {
-            tiles += 9 * (J1 + 1);
+        while (J1 < NT && tiles + 9 * J1 + 6 <= MAXTILE * WAVES) {   // column J1: J1 pairs of 9 tiles, one of 6
+            tiles += 9 * J1 + 6;
             ++J1;
         }
         if (J1 == J) {
@@ -124,28 +136,60 @@ constexpr UnitSchedule<NT, WAVES, MAXU, MAXTILE, MAXF, MAXPASS, MAXR> make_sched
             return s;
         }
         const int u0 = 4 * (J * (J + 1) / 2), u1 = 4 * (J1 * (J1 + 1) / 2);
+        // Waves w and w + 4 share a SIMD's matrix core, so the units are first cut into 4 runs of
+        // at most T tiles (one per SIMD, smallest T that places all), then each run is split
+        // between its two waves at the unit boundary that balances them best.
         bool placed = false;
-        for (int T = (tiles + WAVES - 1) / WAVES; T <= MAXTILE && !placed; ++T) {
-            // greedy run: fill wave w up to T tiles, then move on
-            int w = 0, fill = 0, cnt = 0;
-            bool fits = true;
-            for (int w2 = 0; w2 < WAVES; ++w2) s.nu[pass][w2] = s.nt[pass][w2] = 0;
+        for (int T = (tiles + 3) / 4; T <= 2 * MAXTILE && !placed; ++T) {
+            int r0[4] = {}, r1[4] = {}, sidx = 0, fill = 0;
+            bool fits = WAVES == 8;
+            r0[0] = u0;
             for (int u = u0; u < u1 && fits; ++u) {
-                const int nt = unit_tiles(u % 4);
-                if (fill + nt > T || cnt == MAXU) {
-                    ++w;
-                    fill = cnt = 0;
+                const int nt = unit_tiles(u % 4, u / 4);
+                if (fill + nt > T) {
+                    r1[sidx] = u;
+                    if (++sidx >= 4) {
+                        fits = false;
+                        break;
+                    }
+                    r0[sidx] = u;
+                    fill = 0;
                 }
-                if (w >= WAVES || nt > T) {
+                fill += nt;
+            }
+            if (!fits) continue;
+            r1[sidx] = u1;
+            for (int x = sidx + 1; x < 4; ++x) r0[x] = r1[x] = u1;
+            for (int w2 = 0; w2 < WAVES; ++w2) s.nu[pass][w2] = s.nt[pass][w2] = 0;
+            for (int sm = 0; sm < 4 && fits; ++sm) {
+                int best = -1, bestmax = 1 << 30;
+                for (int m = r0[sm]; m <= r1[sm]; ++m) {
+                    int ta = 0, tb = 0;
+                    for (int u = r0[sm]; u < m; ++u) ta += unit_tiles(u % 4, u / 4);
+                    for (int u = m; u < r1[sm]; ++u) tb += unit_tiles(u % 4, u / 4);
+                    if (ta > MAXTILE || tb > MAXTILE || m - r0[sm] > MAXU || r1[sm] - m > MAXU) continue;
+                    const int mx = ta > tb ? ta : tb;
+                    if (mx < bestmax) {
+                        bestmax = mx;
+                        best = m;
+                    }
+                }
+                if (best < 0) {
                     fits = false;
                     break;
                 }
-                s.unit[pass][w][cnt] = u;
-                s.toff[pass][w][cnt] = fill;
-                ++cnt;
-                fill += nt;
-                s.nu[pass][w] = cnt;
-                s.nt[pass][w] = fill;
+                const int wv[2] = {sm, sm + 4}, ua[2] = {r0[sm], best}, ub[2] = {best, r1[sm]};
+                for (int h = 0; h < 2; ++h) {
+                    int cnt = 0, f = 0;
+                    for (int u = ua[h]; u < ub[h]; ++u) {
+                        s.unit[pass][wv[h]][cnt] = u;
+                        s.toff[pass][wv[h]][cnt] = f;
+                        ++cnt;
+                        f += unit_tiles(u % 4, u / 4);
+                    }
+                    s.nu[pass][wv[h]] = cnt;
+                    s.nt[pass][wv[h]] = f;
+                }
             }
             placed = fits;
         }
@@ -158,7 +202,7 @@ constexpr UnitSchedule<NT, WAVES, MAXU, MAXTILE, MAXF, MAXPASS, MAXR> make_sched
             s.na[pass][w] = s.nb[pass][w] = 0;
             for (int x = 0; x < s.nu[pass][w]; ++x) {
                 const int u = s.unit[pass][w][x], pi = u / 4, kind = u % 4;
-                for (int t = 0; t < unit_tiles(kind); ++t) {
+                for (int t = 0; t < unit_tiles(kind, pi); ++t) {
                     const int akey = pair_I(pi) * 3 + unit_k(kind, t), bkey = pair_J(pi) * 3 + unit_l(kind, t);
                     int ai = -1, bi = -1;
                     for (int y = 0; y < s.na[pass][w]; ++y)
@@ -234,8 +278,11 @@ struct PatchArgs {
 };
 
 // ---------------------------------------------------------------------------------------
-// Geometry pass: per active element gi and Gauss point q, geo[(gi * 12 + v) * NQ + q] with
-// v = 3 m + c: A[m][c] = sqrt(w |det J|) / W dxi_m/dx_c; v = 9 + m: dW/dxi_m / W. J is the
+// Geometry pass: per active element gi a record of REC doubles at geo + gi REC: for Gauss point
+// q, [v NQ + q] with v = 3 m + c: A[m][c] = sqrt(w |det J|) / W dxi_m/dx_c; v = 9 + m:
+// dW/dxi_m / W; then the element's 1D basis tables (values and derivatives at the abscissae,
+// reversed local index) and its node weights, which the element kernel's producers copy with
+// independent loads. J is the
 // derivative with respect to the knot parameters (shap.f90:133-153); w |det J| includes the
 // parent-to-parameter factors (shap.f90:156-178) and the Gauss weights (UELMAT.f90:118-119).
 // A workgroup of 4 waves takes 4 GEO_EPW elements: their control points (x, y, z, w) are
@@ -290,6 +337,18 @@ __global__ void __launch_bounds__(256) elem_geo_kernel(PatchArgs pa, double *__r
             }
             half[d] = 0.5 * (u1 - u0);
         }
+        double *rec = geo + gi * C::REC;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)   // each direction's table from the lane whose other two indices are 0
+            if (gq[(d + 1) % 3] == 0 && gq[(d + 2) % 3] == 0) {
+                double *tb = rec + ELEM_NGEO * NQ + (d * NG + gq[d]) * NP1;
+#pragma unroll
+                for (int i = 0; i < NP1; ++i) {
+                    tb[i] = B[d][i];
+                    tb[3 * NG * NP1 + i] = dB[d][i];
+                }
+            }
+        for (int a = q; a < NN; a += NQ) rec[ELEM_NGEO * NQ + C::NTAB + a] = cp[le][a][3];
         double s[16];
 #pragma unroll
         for (int m = 0; m < 16; ++m) s[m] = 0.0;
@@ -336,7 +395,7 @@ __global__ void __launch_bounds__(256) elem_geo_kernel(PatchArgs pa, double *__r
         A[6] = m02 * f;
         A[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * f;
         A[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * f;
-        double *out = geo + gi * (ELEM_NGEO * NQ) + q;
+        double *out = rec + q;
 #pragma unroll
         for (int v = 0; v < 9; ++v) out[v * NQ] = A[v];
 #pragma unroll
@@ -363,7 +422,9 @@ struct ElemSmem {
     int32_t dof[C::NN];                 // 3 (IEN - 1) of the element being emitted
 };
 
-// Producer: per-element set-up (1D bases in the wave's own LDS table, node weights in registers).
+// Producer: per-element set-up (1D bases in the wave's own LDS table, node weights in registers),
+// fetched from the element's record into registers one step before the element starts, so the
+// loads overlap the step in between.
 template <int P, int NG>
 struct ProducerElem {
     using C = ElemCfg<P, NG>;
@@ -371,31 +432,49 @@ struct ProducerElem {
 };
 
 template <int P, int NG>
-__device__ __forceinline__ void producer_load_element(ElemSmem<P, NG> &sm, const PatchArgs &pa, int e, int pw,
-                                                      ProducerElem<P, NG> &pe) {
+struct ProducerFetch {
     using C = ElemCfg<P, NG>;
-    constexpr int NP1 = C::NP1, NN = C::NN;
+    double tab[(C::NTAB + 63) / 64];   // table entry lane + 64 m
+    double w[(C::NNP + 63) / 64];
+    int64_t base;                      // COO offset of the element
+};
+
+template <int P, int NG>
+__device__ __forceinline__ void producer_fetch_element(const double *__restrict__ rec, const int64_t *offs_gi,
+                                                       ProducerFetch<P, NG> &f) {
+    using C = ElemCfg<P, NG>;
     const int lane = threadIdx.x & 63;
-    const double *wt = pa.weight + (int64_t)NN * e;
+#pragma unroll
+    for (int m = 0; m < (C::NTAB + 63) / 64; ++m) {
+        const int t = lane + 64 * m;
+        f.tab[m] = t < C::NTAB ? rec[ELEM_NGEO * C::NQ + t] : 0.0;
+    }
 #pragma unroll
     for (int m = 0; m < (C::NNP + 63) / 64; ++m) {
         const int a = lane + 64 * m;
-        pe.wb[m] = a < NN ? wt[a] : 0.0;
+        f.w[m] = a < C::NN ? rec[ELEM_NGEO * C::NQ + C::NTAB + a] : 0.0;
     }
-    if (lane < 3 * NG) {
-        const int d = lane / NG, g = lane % NG;
-        const double *U = d == 0 ? pa.U0 : d == 1 ? pa.U1 : pa.U2;
-        const int span1 = pa.nijk[3 * (int64_t)e + d];
-        const double u0 = U[span1 - 1], u1 = U[span1];
-        const double u = ((u1 - u0) * pa.g.x[g] + (u1 + u0)) * 0.5;
-        double N[NP1], dN[NP1];
-        bspline_ders<P>(span1 - 1, u, U, N, dN);
+    f.base = *offs_gi;
+}
+
+template <int P, int NG>
+__device__ __forceinline__ void producer_store_element(ElemSmem<P, NG> &sm, int pw, const ProducerFetch<P, NG> &f,
+                                                       ProducerElem<P, NG> &pe) {
+    using C = ElemCfg<P, NG>;
+    constexpr int HALF = 3 * NG * C::NP1;
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-        for (int i = 0; i < NP1; ++i) {   // local index i <-> basis function P - i
-            sm.B[pw][d][g][i] = N[P - i];
-            sm.dB[pw][d][g][i] = dN[P - i];
+    for (int m = 0; m < (C::NTAB + 63) / 64; ++m) {
+        const int t = lane + 64 * m;
+        if (t < C::NTAB) {
+            const int u = t < HALF ? t : t - HALF;
+            const int d = u / (NG * C::NP1), g = (u / C::NP1) % NG, i = u % C::NP1;
+            if (t < HALF) sm.B[pw][d][g][i] = f.tab[m];
+            else sm.dB[pw][d][g][i] = f.tab[m];
         }
     }
+#pragma unroll
+    for (int m = 0; m < (C::NNP + 63) / 64; ++m) pe.wb[m] = f.w[m];
     // the table is read back by this wave only: order the LDS writes before the reads
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -550,17 +629,43 @@ __device__ __forceinline__ void stage_round(ElemSmem<P, NG> &sm, double lam, dou
                             st[9 * b + 2] = 0.5 * k11;
                             st[9 * b + 5] = 0.5 * k22;
                         }
+                    } else if constexpr (I != J) {
+                        constexpr int k = kind == 3 ? 1 : 0, l = kind == 1 ? 1 : 2;
+                        const double skl = acc.S[off][r], slk = acc.S[off + 1][r];
+                        st[9 * a + k + 3 * l] = lam * skl + mu * slk;   // K_ab[k][l]
+                        st[9 * a + l + 3 * k] = lam * slk + mu * skl;   // K_ab[l][k]
                     } else {
+                        // diagonal pair, tile s = S^kl: K_ab[k][l] = lam s_ab + mu s_ba and
+                        // K_ab[l][k] = lam s_ba + mu s_ab; here the s_ab (a <= b) parts, the s_ba
+                        // parts are added below by the lanes holding the lower triangle
                         constexpr int k = kind == 3 ? 1 : 0, l = kind == 1 ? 1 : 2;
                         constexpr int mdiag = kind == 1 ? 1 : kind == 2 ? 3 : 4;
-                        const double skl = acc.S[off][r], slk = acc.S[off + 1][r];
-                        const double kkl = lam * skl + mu * slk;   // K_ab[k][l]
+                        const double sab = acc.S[off][r];
                         if (a < b) {
-                            st[9 * a + k + 3 * l] = kkl;
-                            st[9 * a + l + 3 * k] = lam * slk + mu * skl;   // K_ab[l][k]
+                            st[9 * a + k + 3 * l] = lam * sab;
+                            st[9 * a + l + 3 * k] = mu * sab;
                         } else {
-                            st[9 * b + mdiag] = kkl;   // diagonal block keeps the upper entry (k, l)
+                            st[9 * b + mdiag] = (lam + mu) * sab;   // diagonal block keeps the upper entry (k, l)
                         }
+                    }
+                }
+            }
+            if constexpr (kind != 0 && I == J) {
+                // lower triangle of the diagonal-pair tile: lane (a, b), a > b, holds s_ab = S^kl_ab,
+                // the s_ba part of block (b, a), whose row in the COO belongs to node a
+                constexpr int k = kind == 3 ? 1 : 0, l = kind == 1 ? 1 : 2;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const int bb = 16 * J + nl;   // the lane's column node (the block's row node here)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int a = 16 * I + kg + 4 * r;
+                    if (a > bb && a >= b0 && a < b1) {
+                        double *sta = sm.stage + ((int)coo_start(a) - c0);
+                        const double s_ab = acc.S[off][r];
+                        sta[9 * bb + k + 3 * l] += mu * s_ab;
+                        sta[9 * bb + l + 3 * k] += lam * s_ab;
                     }
                 }
             }
@@ -568,9 +673,10 @@ __device__ __forceinline__ void stage_round(ElemSmem<P, NG> &sm, double lam, dou
     });
 }
 
-// Epilogue, second half (waves 0 .. NW-1): the COO entries of round R of pass PASS. Wave w takes
-// rows b = b0 + w + NW part, then every NW NPART-th row (part selects one of NPART slices) (row b = the contiguous entries [start(b), start(b+1))) and
-// copies them with consecutive lanes on consecutive entries. Entry j of a row: j < 9 b is block
+// Epilogue, second half (waves 0 .. NW-1): the COO entries of round R of pass PASS, row by row
+// (row b = the contiguous entries [start(b), start(b+1))). The rows are cut into NPART slices;
+// in slice `part` wave w takes rows b0 + w + NW part, then every (NW NPART)-th row, and copies
+// them with consecutive lanes on consecutive entries. Entry j of a row: j < 9 b is block
 // a = j / 9, (kk, ll) = (j mod 3, j mod 9 / 3); the last 6 are the diagonal block.
 // row = 3 (IEN_a - 1) + kk, col = 3 (IEN_b - 1) + ll.
 template <int P, int NG, bool WRITE_IDX, int PASS, int R, int NW, int NPART = 1>
@@ -635,14 +741,15 @@ __device__ __forceinline__ void consumer_epilogue(ElemSmem<P, NG> &sm, double la
     stage_round<P, NG, PASS, W, R>(sm, lam, mu, acc);
     if constexpr (R + 1 == NR && elem_defer_last<P, NG>()) return;
     __syncthreads();
-    copy_round<P, NG, WRITE_IDX, PASS, R, C::NTH / 64>(sm, base, Kdata, Krow, Kcol);
+    copy_round<P, NG, WRITE_IDX, PASS, R, C::CPW>(sm, base, Kdata, Krow, Kcol);
     if constexpr (R + 1 < NR) {
         __syncthreads();
         consumer_epilogue<P, NG, WRITE_IDX, PASS, W, R + 1>(sm, lam, mu, base, acc, Kdata, Krow, Kcol);
     }
 }
 
-// The producers' side of the same rounds (same barriers, they help with the copies).
+// The producers' side of the same rounds (same barriers; they help with the copies when
+// PROD_COPY).
 template <int P, int NG, bool WRITE_IDX, int PASS, int R = 0>
 __device__ __forceinline__ void producer_epilogue_pass(ElemSmem<P, NG> &sm, int64_t base, double *__restrict__ Kdata,
                                                        int32_t *__restrict__ Krow, int32_t *__restrict__ Kcol) {
@@ -650,7 +757,7 @@ __device__ __forceinline__ void producer_epilogue_pass(ElemSmem<P, NG> &sm, int6
     constexpr int NR = Sched<P, NG>::S.nround[PASS];
     if constexpr (R + 1 == NR && elem_defer_last<P, NG>()) return;
     __syncthreads();
-    copy_round<P, NG, WRITE_IDX, PASS, R, C::NTH / 64>(sm, base, Kdata, Krow, Kcol);
+    if constexpr (C::PROD_COPY) copy_round<P, NG, WRITE_IDX, PASS, R, C::CPW>(sm, base, Kdata, Krow, Kcol);
     if constexpr (R + 1 < NR) {
         __syncthreads();
         producer_epilogue_pass<P, NG, WRITE_IDX, PASS, R + 1>(sm, base, Kdata, Krow, Kcol);
@@ -668,14 +775,14 @@ __device__ __forceinline__ void producer_epilogue(ElemSmem<P, NG> &sm, int pass,
     }
 }
 
-// Slice `part` of the deferred copy-out of the last round of pass `pass` (all waves take part;
+// Slice `part` of the deferred copy-out of the last round of pass `pass` (by the copying waves;
 // part = -1 copies the whole round).
 template <int P, int NG, bool WRITE_IDX, int PASS = 0>
 __device__ __forceinline__ void deferred_copy(const ElemSmem<P, NG> &sm, int pass, int part, int64_t base,
                                               double *__restrict__ Kdata, int32_t *__restrict__ Krow,
                                               int32_t *__restrict__ Kcol) {
     using C = ElemCfg<P, NG>;
-    constexpr int NW = C::NTH / 64;
+    constexpr int NW = C::CPW;
     if (pass == PASS) {
         constexpr int R = Sched<P, NG>::S.nround[PASS] - 1;
         if (part < 0)
@@ -763,51 +870,62 @@ elem_stiffness_kernel(PatchArgs pa, const double *__restrict__ geo, double *__re
     // the first step, one after each step, and the epilogue rounds), so each role keeps only its
     // own registers live.
     if (producer) {
+        // Step s: issue the geometry load for step s + 2, build the chunk of step s + 1 from the
+        // load of step s - 1, then (at an element start) fetch the tables of step s + 2's
+        // element, so each load has a whole step to arrive.
         ProducerElem<P, NG> pe;
-        int pe_elem = -1;
-        auto step_geo = [&](int st) {   // element batch index and first Gauss point of step st
-            const int64_t gi = blockIdx.x + (int64_t)(st / SPE) * gridDim.x;
-            return producer_prefetch<P, NG>(geo + gi * (ELEM_NGEO * C::NQ), pw, ((st % SPE) % C::NCH) * C::QC);
+        ProducerFetch<P, NG> fe;
+        const double *geo_b = geo + (int64_t)blockIdx.x * C::REC;
+        const int64_t rec_stride = (int64_t)gridDim.x * C::REC;
+        auto step_geo = [&](int st) {   // geometry of step st for this wave's lanes
+            return producer_prefetch<P, NG>(geo_b + (int64_t)(st / SPE) * rec_stride, pw, ((st % SPE) % C::NCH) * C::QC);
         };
-        double gv = 0.0;   // geometry of the chunk built next
+        auto fetch = [&](int el) {
+            producer_fetch_element<P, NG>(geo_b + (int64_t)el * rec_stride, pa.offs + blockIdx.x + (int64_t)el * gridDim.x,
+                                          fe);
+        };
+        double gv = 0.0;         // geometry of the chunk built next
+        int64_t base = 0;        // COO offset of the element of step s
+        int64_t next_base = 0;
         if (nsteps > 0) {
+            fetch(0);
             gv = step_geo(0);
-            const double gnext = nsteps > 1 ? step_geo(1) : 0.0;
-            producer_load_element<P, NG>(sm, pa, pa.elems[blockIdx.x], pw, pe);
-            pe_elem = 0;
+            producer_store_element<P, NG>(sm, pw, fe, pe);
+            base = next_base = fe.base;
             producer_chunk<P, NG>(sm, gv, pw, 0, 0, pe);
-            gv = gnext;
+            if (nsteps > 1) {
+                gv = step_geo(1);
+                if (SPE == 1) fetch(1);
+            }
         }
         __syncthreads();
         int pend_pass = -1;     // pass whose last round waits in the stage for its copy-out
         int64_t pend_base = 0;
         for (int s = 0; s < nsteps; ++s) {
-            const int s1 = s + 1;
-            if (s1 < nsteps) {
-                const double gnext = s1 + 1 < nsteps ? step_geo(s1 + 1) : 0.0;   // one step ahead
-                const int el = s1 / SPE, ch = (s1 % SPE) % C::NCH;
-                if (el != pe_elem) {
-                    producer_load_element<P, NG>(sm, pa, pa.elems[blockIdx.x + (int64_t)el * gridDim.x], pw, pe);
-                    pe_elem = el;
-                }
-                producer_chunk<P, NG>(sm, gv, pw, s1 & 1, ch * C::QC, pe);
-                gv = gnext;
+            const int s1 = s + 1, s2 = s + 2;
+            const double gcur = gv;
+            if (s1 < nsteps && s1 % SPE == 0) {   // the element of step s1 starts
+                producer_store_element<P, NG>(sm, pw, fe, pe);
+                next_base = fe.base;
             }
+            if (s2 < nsteps) gv = step_geo(s2);   // geometry for step s2, before this step's chunk
+            if (s1 < nsteps) producer_chunk<P, NG>(sm, gcur, pw, s1 & 1, ((s1 % SPE) % C::NCH) * C::QC, pe);
+            if (s2 < nsteps && s2 % SPE == 0) fetch(s2 / SPE);   // after the chunk's wait on gcur
             const int rem = s % SPE, ch = rem % C::NCH;
             if (ch == C::NCH - 1) {
-                const int64_t gi = blockIdx.x + (int64_t)(s / SPE) * gridDim.x;
-                producer_epilogue<P, NG, WRITE_IDX>(sm, rem / C::NCH, pa.offs[gi], Kdata, Krow, Kcol);
-                if constexpr (elem_defer_last<P, NG>()) {
+                producer_epilogue<P, NG, WRITE_IDX>(sm, rem / C::NCH, base, Kdata, Krow, Kcol);
+                if constexpr (elem_defer_last<P, NG>() && C::PROD_COPY) {
                     pend_pass = rem / C::NCH;
-                    pend_base = pa.offs[gi];
+                    pend_base = base;
                 }
-            } else if (elem_defer_last<P, NG>() && pend_pass >= 0) {
+            } else if (elem_defer_last<P, NG>() && C::PROD_COPY && pend_pass >= 0) {
                 deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, ch, pend_base, Kdata, Krow, Kcol);
             }
+            if (s1 % SPE == 0) base = next_base;
             __syncthreads();
         }
         // an unfinished deferred round (the stage is stable after the last barrier)
-        if constexpr (elem_defer_last<P, NG>())
+        if constexpr (elem_defer_last<P, NG>() && C::PROD_COPY)
             if (pend_pass >= 0) deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, -1, pend_base, Kdata, Krow, Kcol);
     } else {
         __syncthreads();
