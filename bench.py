@@ -15,7 +15,8 @@ bounded sample of the same workload, on this host's cores. Extra objects report 
 parts of the metric on the same GPU: ``serving`` (a Poisson sample of the serving workload:
 solves/s and p50/p99 latency, mixed precision vs FP64, vs plain one-at-a-time FP64 solves),
 ``gradient`` (compliance shape gradient of a 32^3 p=2 patch against central differences) and
-``mixed_precision`` (128^3 p=2 matrix-free FP32 solve with FP64 refinement vs FP64).
+``mixed_precision`` (128^3 p=2 matrix-free FP32 solve with FP64 refinement vs FP64, and the
+matrix-free apply alone in DOF-applies/s against the HBM roofline).
 """
 from __future__ import annotations
 
@@ -28,9 +29,11 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
-# The serving section keeps 16 request streams busy; HIP maps streams onto this many hardware
-# queues (its default is 4). Must be set before the HIP runtime starts.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# The serving section keeps 16 request streams busy; HIP maps streams onto GPU_MAX_HW_QUEUES
+# hardware queues (4 by default), so this process asks for 16. Must be set before the HIP
+# runtime starts.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 METRIC = "assembled elem/s + served solves/s, p99 latency vs plain FP64 at rel err ≤1e-8"
 HBM_PEAK_GBS = 8000.0
@@ -152,6 +155,8 @@ def bench_gradient(nfd, seed=7):
         Xm = X.clone(); Xm[i] -= h
         fd = (compliance(Xp, patch).item() - compliance(Xm, patch).item()) / (2 * h)
         errs.append(abs(fd - g[i]) / max(abs(g).max(), 1e-300))
+        if len(errs) % 10 == 0:
+            log("gradient: %d / %d central differences done" % (len(errs), nfd))
     compliance(X, patch)
     log("gradient: forward %.3f s, backward %.3f s, max |fd - g| / max|g| %.2e over %d coordinates"
         % (t1 - t0, t2 - t1, max(errs), nfd))
@@ -188,6 +193,36 @@ def bench_mixed_precision(nel):
         srv.close()
         del srv
         torch.cuda.empty_cache()
+    # the matrix-free apply alone: DOF-applies/s and HBM roofline (algorithmic bytes: the 13
+    # stored geometry values per Gauss point + x read + y written, in the apply's precision)
+    from yeti_amd.matfree import MatrixFree
+    mf = MatrixFree(bp.args)
+    nqp = bp.nb_elem * 27
+    stream = torch.cuda.current_stream()
+    for name, dtype, sz in (("fp32", torch.float32, 4), ("fp64", torch.float64, 8)):
+        x = torch.randn(mf.ndof, dtype=dtype, device="cuda")
+        y = torch.empty_like(x)
+        mf.apply(x, y)
+        torch.cuda.synchronize()
+        nrep = 20
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(nrep):
+            mf.apply(x, y)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / nrep
+        byts = 13 * sz * nqp + 2 * sz * mf.ndof
+        gbs = byts / (ms * 1e-3) / 1e9
+        out["apply_" + name] = {"ms": ms, "dof_applies_per_s": mf.ndof / (ms * 1e-3),
+                                "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                             "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": byts}}
+        log("%d^3 p=2 apply %s: %.3f ms, %.3e DOF-applies/s, %.0f GB/s algorithmic"
+            % (nel, name, ms, mf.ndof / (ms * 1e-3), gbs))
+        del x, y
+    mf.close()
+    del mf
+    torch.cuda.empty_cache()
     asm = Assembler(bp.args)
     F = asm.run_rhs().cpu().numpy()
     u64, u32 = xs["fp64"], xs["mixed"]
@@ -216,7 +251,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="offered requests per second")
     ap.add_argument("--serve-plain", type=int, default=0,
                     help="requests solved one at a time (plain FP64); 0: all of them")
-    ap.add_argument("--grad-fd", type=int, default=10, help="coordinates checked by central differences")
+    ap.add_argument("--grad-fd", type=int, default=50, help="coordinates checked by central differences")
     ap.add_argument("--mp-nel", type=int, default=128, help="elements per direction of the mixed-precision solve")
     ap.add_argument("--only-assembly", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true",
