@@ -715,44 +715,145 @@ static bool gen_eigen(int n, const std::vector<double> &K, const std::vector<dou
     return true;
 }
 
-// out = (mode-`axis` product with A) in, A[a][b] read from P as P[b m + a] (use_transpose_store,
-// for axis 0, where the lanes run over a) or P[a m + b]. gather_r (axis 0 only): the input is
-// read from the DOF vector r through the CP map (the free box of component c) instead of a
-// tensor; scale: the output is multiplied by 1 / D_c.
+// One mode product of the preconditioner, out(.., a, ..) = sum_b A[a][b] in(.., b, ..) along one
+// axis for the three components, written as a batched strided GEMM C = L R: along axis 0,
+// L = A (m0 x m0) and R = the tensor seen as m0 x (m1 m2); along axis 1, per k, L = the tensor
+// slice (m0 x m1) and R = A^T; along axis 2, L = the tensor seen as (m0 m1) x m2 and R = A^T.
+// Element (row, col) of operand X of component c, batch z, is at X[off[c] + z bs[c] + row rs[c]
+// + col cs[c]].
+struct FdmGemm {
+    int32_t M[3], N[3], K[3], nbatch[3];
+    int64_t loff[3], lrs[3], lcs[3], lbs[3];
+    int64_t roff[3], rrs[3], rcs[3], rbs[3];
+    int64_t coff[3], crs[3], ccs[3], cbs[3];
+    int32_t gather;   // R is read from the DOF vector r through the CP map (first forward product)
+    int32_t scale;    // C is multiplied by 1 / D_c (last forward product)
+};
+
+// 64 x 64 tile of C per 256-thread workgroup, K in steps of 16 through LDS, 4 x 4 outputs per
+// thread (rows tx + 16 i, columns ty + 16 j), accumulated in T. grid = (N tiles, M tiles,
+// 3 x batches).
 template <typename T>
-__global__ void __launch_bounds__(256) fdm_mode_kernel(FdmArgs fa, const T *__restrict__ in, T *__restrict__ out,
-                                                       const T *__restrict__ P, int axis, int use_transpose_store,
-                                                       const T *__restrict__ r, const T *__restrict__ invD,
+__global__ void __launch_bounds__(256) fdm_gemm_kernel(FdmGemm g, FdmArgs fa, const T *__restrict__ L,
+                                                       const T *__restrict__ R, T *__restrict__ Cout,
+                                                       const T *__restrict__ invD, const T *__restrict__ r,
                                                        const int32_t *__restrict__ pred) {
     if (pred && *pred == 0) return;
-    const int c = blockIdx.y;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= fa.tot[c]) return;
-    const int m0 = fa.m[c][0], m1 = fa.m[c][1], m2 = fa.m[c][2];
-    const int i = (int)(t % m0), j = (int)((t / m0) % m1), k = (int)(t / ((int64_t)m0 * m1));
-    const T *src = in + c * fa.maxtot;
-    const T *A = P + fa.uoff[c][axis];
-    double s = 0.0;
-    if (axis == 0) {
-        if (r) {
-            const int64_t g0 = fa.lo[c][0] + (int64_t)fa.n[0] * ((fa.lo[c][1] + j) + (int64_t)fa.n[1] * (fa.lo[c][2] + k));
-            for (int b = 0; b < m0; ++b)
-                s += (double)A[use_transpose_store ? b * m0 + i : i * m0 + b] * (double)r[3 * (int64_t)fa.cpmap[g0 + b] + c];
-        } else {
-            const T *line = src + (int64_t)m0 * (j + (int64_t)m1 * k);
-            for (int b = 0; b < m0; ++b) s += (double)A[use_transpose_store ? b * m0 + i : i * m0 + b] * (double)line[b];
+    const int c = blockIdx.z % 3, z = blockIdx.z / 3;
+    if (z >= g.nbatch[c]) return;
+    const int M = g.M[c], N = g.N[c], K = g.K[c];
+    const int mt = blockIdx.y * 64, nt = blockIdx.x * 64;
+    if (mt >= M || nt >= N) return;
+    __shared__ T Ls[16][65];
+    __shared__ T Rs[16][65];
+    const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    const T *Lb = L + g.loff[c] + z * g.lbs[c];
+    const T *Rb = R + g.roff[c] + z * g.rbs[c];
+    T acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+    const int m1c = fa.m[c][1];
+    for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {   // L tile, 64 rows x 16: lanes along the rows
+            const int mm = t & 63, kk = (t >> 6) + 4 * q;
+            const int row = mt + mm, col = k0 + kk;
+            Ls[kk][mm] = (row < M && col < K) ? Lb[row * g.lrs[c] + col * g.lcs[c]] : T(0);
         }
-    } else if (axis == 1) {
-        const T *line = src + i + (int64_t)m0 * m1 * k;
-        for (int b = 0; b < m1; ++b) s += (double)A[use_transpose_store ? b * m1 + j : j * m1 + b] * (double)line[(int64_t)m0 * b];
-    } else {
-        const T *line = src + i + (int64_t)m0 * j;
-        for (int b = 0; b < m2; ++b)
-            s += (double)A[use_transpose_store ? b * m2 + k : k * m2 + b] * (double)line[(int64_t)m0 * m1 * b];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {   // R tile, 16 x 64 columns: lanes along K
+            const int kk = t & 15, nn = (t >> 4) + 16 * q;
+            const int row = k0 + kk, col = nt + nn;
+            T v = T(0);
+            if (row < K && col < N) {
+                if (g.gather) {
+                    const int j = col % m1c, k = col / m1c;
+                    const int64_t gp = (fa.lo[c][0] + row) +
+                                       (int64_t)fa.n[0] * ((fa.lo[c][1] + j) + (int64_t)fa.n[1] * (fa.lo[c][2] + k));
+                    v = r[3 * (int64_t)fa.cpmap[gp] + c];
+                } else {
+                    v = Rb[row * g.rrs[c] + col * g.rcs[c]];
+                }
+            }
+            Rs[kk][nn] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            T a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = Ls[kk][tx + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Rs[kk][ty + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+        }
+        __syncthreads();
     }
-    T v = (T)s;
-    if (invD) v *= invD[fa.doff[c] + t];
-    out[c * fa.maxtot + t] = v;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = mt + tx + 16 * i, col = nt + ty + 16 * j;
+            if (row < M && col < N) {
+                const int64_t lin = z * g.cbs[c] + row * g.crs[c] + col * g.ccs[c];   // index in the component
+                T v = acc[i][j];
+                if (g.scale) v *= invD[fa.doff[c] + lin];
+                Cout[g.coff[c] + lin] = v;
+            }
+        }
+}
+
+// GEMM descriptor of the mode product along `axis` with A[a][b] = P[b m + a] (transposed_store)
+// or P[a m + b]; the tensor operands of component c start at c maxtot.
+static FdmGemm fdm_gemm_op(const FdmArgs &fa, int axis, bool transposed_store) {
+    FdmGemm g{};
+    for (int c = 0; c < 3; ++c) {
+        const int64_t m0 = fa.m[c][0], m1 = fa.m[c][1], m2 = fa.m[c][2], m = fa.m[c][axis];
+        const int64_t toff = c * fa.maxtot, moff = fa.uoff[c][axis];
+        const int64_t ars = transposed_store ? 1 : m, acs = transposed_store ? m : 1;   // A(a, b)
+        g.coff[c] = toff;
+        if (axis == 0) {
+            g.M[c] = (int32_t)m0; g.N[c] = (int32_t)(m1 * m2); g.K[c] = (int32_t)m0; g.nbatch[c] = 1;
+            g.loff[c] = moff; g.lrs[c] = ars; g.lcs[c] = acs; g.lbs[c] = 0;
+            g.roff[c] = toff; g.rrs[c] = 1; g.rcs[c] = m0; g.rbs[c] = 0;
+            g.crs[c] = 1; g.ccs[c] = m0; g.cbs[c] = 0;
+        } else if (axis == 1) {
+            g.M[c] = (int32_t)m0; g.N[c] = (int32_t)m1; g.K[c] = (int32_t)m1; g.nbatch[c] = (int32_t)m2;
+            g.loff[c] = toff; g.lrs[c] = 1; g.lcs[c] = m0; g.lbs[c] = m0 * m1;
+            g.roff[c] = moff; g.rrs[c] = acs; g.rcs[c] = ars; g.rbs[c] = 0;   // R(b, j) = A(j, b)
+            g.crs[c] = 1; g.ccs[c] = m0; g.cbs[c] = m0 * m1;
+        } else {
+            g.M[c] = (int32_t)(m0 * m1); g.N[c] = (int32_t)m2; g.K[c] = (int32_t)m2; g.nbatch[c] = 1;
+            g.loff[c] = toff; g.lrs[c] = 1; g.lcs[c] = m0 * m1; g.lbs[c] = 0;
+            g.roff[c] = moff; g.rrs[c] = acs; g.rcs[c] = ars; g.rbs[c] = 0;
+            g.crs[c] = 1; g.ccs[c] = m0 * m1; g.cbs[c] = 0;
+        }
+        (void)m;
+    }
+    return g;
+}
+
+template <typename T>
+static void fdm_product(const FdmArgs &fa, int axis, const T *P, bool transposed_store, const T *in, T *out,
+                        const T *r, const T *invD, hipStream_t st, const int32_t *pred) {
+    FdmGemm g = fdm_gemm_op(fa, axis, transposed_store);
+    g.gather = r != nullptr;
+    g.scale = invD != nullptr;
+    int maxM = 0, maxN = 0, maxB = 0;
+    for (int c = 0; c < 3; ++c) {
+        maxM = std::max(maxM, g.M[c]);
+        maxN = std::max(maxN, g.N[c]);
+        maxB = std::max(maxB, g.nbatch[c]);
+    }
+    const T *Lp = axis == 0 ? P : in;
+    const T *Rp = axis == 0 ? in : P;
+    dim3 grid((unsigned)((maxN + 63) / 64), (unsigned)((maxM + 63) / 64), (unsigned)(3 * maxB));
+    hipLaunchKernelGGL(fdm_gemm_kernel<T>, grid, dim3(256), 0, st, g, fa, Lp, Rp, out, invD, r, pred);
 }
 
 // z on every CP (0 outside the free boxes and on masked dofs); r.z reduced in the same launch
@@ -789,17 +890,15 @@ static int fdm_apply_t(const yeti_fdm *f, const T *U, const T *Ut, const T *invD
                        const uint8_t *mask, T *z, double *part, int nblocks, const RzTail &tail, hipStream_t st,
                        const int32_t *pred) {
     const FdmArgs &fa = f->fa;
-    dim3 grid((unsigned)((fa.maxtot + 255) / 256), 3);
     const T *nul = nullptr;
-    // forward: A = U^T along axes 0, 1, 2. Along axis 0 the lanes run over a, so A[a][b] is read
-    // as U[b][a] (consecutive addresses); along axes 1, 2 a is wave-uniform: Ut[a][b].
-    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, nul, wb, U, 0, 1, r, nul, pred);
-    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, Ut, 1, 0, nul, nul, pred);
-    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wa, wb, Ut, 2, 0, nul, invD, pred);
-    // backward: A = U (axis 0 reads Ut[b][a])
-    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, Ut, 0, 1, nul, nul, pred);
-    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wa, wb, U, 1, 0, nul, nul, pred);
-    hipLaunchKernelGGL(fdm_mode_kernel<T>, grid, dim3(256), 0, st, fa, wb, wa, U, 2, 0, nul, nul, pred);
+    // forward: A = U^T along axes 0, 1, 2 (U read transposed, or Ut read as stored)
+    fdm_product<T>(fa, 0, U, true, nul, wb, r, nul, st, pred);
+    fdm_product<T>(fa, 1, Ut, false, wb, wa, nul, nul, st, pred);
+    fdm_product<T>(fa, 2, Ut, false, wa, wb, nul, invD, st, pred);
+    // backward: A = U
+    fdm_product<T>(fa, 0, Ut, true, wb, wa, nul, nul, st, pred);
+    fdm_product<T>(fa, 1, U, false, wa, wb, nul, nul, st, pred);
+    fdm_product<T>(fa, 2, U, false, wb, wa, nul, nul, st, pred);
     hipLaunchKernelGGL(fdm_scatter_kernel<T>, dim3(nblocks), dim3(256), 0, st, fa, (int64_t)f->ncp, wa, r, mask, z,
                        part, tail, pred);
     YETI_HIP(hipGetLastError());

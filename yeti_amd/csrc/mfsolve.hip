@@ -732,7 +732,11 @@ int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask, int32_t preci
         if (rcf) return fail(rcf);
     }
     if (used) *used = a.used;
-    // block-Jacobi preconditioner of the free dofs (also kept with fast diagonalisation: cheap)
+    if (precond == 1) {   // fast diagonalisation: the block-Jacobi arrays stay unused
+        *out = s;
+        return 0;
+    }
+    // block-Jacobi preconditioner of the free dofs
     int rc = yeti_mf_diag_blocks_f64(m, s->blk.p, st);
     if (rc) return fail(rc);
     hipLaunchKernelGGL(block_inverse_kernel, dim3((s->ncp + 255) / 256), dim3(256), 0, st, s->ncp, s->blk.p, free_mask,
