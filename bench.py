@@ -223,6 +223,39 @@ def bench_mixed_precision(nel):
     mf.close()
     del mf
     torch.cuda.empty_cache()
+    # displacement reconstruction (reconstruction.f:22) of the mixed solution, on the device
+    import ctypes as C
+    from yeti_amd import _lib
+    a = bp.args
+    nfree, ncp = int(a["nb_dof_free"]), int(a["nb_cp"])
+    idf = np.asarray(a["ind_dof_free"])[:nfree].astype(np.int32)
+    u_inv = torch.as_tensor(xs["mixed"][idf - 1], device="cuda")
+    ind = torch.as_tensor(idf, device="cuda")
+    tgt = torch.as_tensor(np.asarray(a["bc_target"]).reshape(-1).astype(np.int32), device="cuda")
+    bn = np.ascontiguousarray(np.asarray(a["bc_target_nbelem"]).reshape(-1)[: int(a["nb_bc"])], dtype=np.int32)
+    bv = np.ascontiguousarray(np.asarray(a["bc_values"], np.float64).reshape(2, -1, order="F")[:, : bn.size].T)
+    sol = torch.empty(3 * ncp, dtype=torch.float64, device="cuda")
+    U = torch.empty_like(sol)
+
+    def recon():
+        _lib.check(_lib.lib().yeti_reconstruction(
+            C.c_void_p(u_inv.data_ptr()), C.c_void_p(ind.data_ptr()), nfree, C.c_void_p(tgt.data_ptr()),
+            bn.ctypes.data_as(C.c_void_p), bv.ctypes.data_as(C.c_void_p), int(bn.size), 3, ncp,
+            C.c_void_p(sol.data_ptr()), C.c_void_p(U.data_ptr()), C.c_void_p(stream.cuda_stream)))
+    recon()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(20):
+        recon()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 20
+    byts = 12 * nfree + 16 * 3 * ncp     # u_inv + index read, U and sol written
+    out["reconstruction"] = {"ms": ms, "dofs": 3 * ncp, "achieved_gbs": byts / (ms * 1e-3) / 1e9,
+                             "algorithmic_bytes": byts,
+                             "max_abs_diff_vs_solution": float(np.abs(U.cpu().numpy() - xs["mixed"]).max())}
+    log("%d^3 p=2 reconstruction: %.3f ms (%.0f GB/s algorithmic)" % (nel, ms, byts / (ms * 1e-3) / 1e9))
     asm = Assembler(bp.args)
     F = asm.run_rhs().cpu().numpy()
     u64, u32 = xs["fp64"], xs["mixed"]

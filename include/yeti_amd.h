@@ -269,6 +269,21 @@ int32_t yeti_ir_step(yeti_ir *s, int32_t use_graph, void *stream);
 int32_t yeti_ir_status_async(const yeti_ir *s, yeti_ir_status *host_status, void *stream);
 void    yeti_ir_destroy(yeti_ir *s);
 
+/* ---- Displacement reconstruction ------------------------------------------------------------
+ * Replaces f2py reconstructionSOL.reconstruction (src/yeti_iga/main/reconstruction.f:22-135,
+ * called as rsol.reconstruction(**get_inputs4solution(x)) at overlay/iga_model.py:504 and
+ * OPT_modelling.py:456,571): U (nb_cp*mcrd, dof-interleaved) = 0, then U[ind_dof_free[d]-1] =
+ * u_inv[d] for d < nb_dof_free, then, boundary condition by boundary condition (later ones
+ * overwrite earlier ones), U[(bc_target[.]-1)*mcrd + dir-1] = value for every condition whose
+ * direction bc_values[2b] <= 3 (value bc_values[2b+1]); sol (nb_cp x mcrd, column-major) = U
+ * reshaped. Device pointers: u_inv, ind_dof_free, bc_target (concatenated targets, 1-based CP
+ * ids), sol, U. Host pointers: bc_target_nbelem[nb_bc], bc_values[2*nb_bc]. Out-of-range dof
+ * indices are skipped on the device; callers validate them (the Python mirror does). */
+int32_t yeti_reconstruction(const double *u_inv, const int32_t *ind_dof_free, int32_t nb_dof_free,
+                            const int32_t *bc_target, const int32_t *bc_target_nbelem_host,
+                            const double *bc_values_host, int32_t nb_bc, int32_t mcrd, int32_t nb_cp,
+                            double *sol, double *U, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,8 +4,9 @@ Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
 only as the checker. Two independent CPU implementations are exposed:
 
 * ``oracle_sys_linmat`` -- liboracle.so, our C restatement (oracle/yeti_oracle.c);
+* ``oracle_reconstruction`` -- the same library's restatement of the displacement scatter;
 * ``ref_sys_linmat_omp`` / ``ref_sys_linmat_serial`` / ``ref_dersbasisfuns`` / ``ref_gauss`` /
-  ``ref_grad_linelast_work`` -- oracle/_ref/libyetiref.so, YETI's own Fortran compiled from
+  ``ref_grad_linelast_work`` / ``ref_reconstruction`` -- oracle/_ref/libyetiref.so, YETI's own Fortran compiled from
   its sources by oracle/build_ref.sh (present only where the YETI sources were available at
   build time; callers must handle ``ref_available() == False``).
 """
@@ -49,6 +50,9 @@ def lib():
         _lib.oracle_element_matrix.argtypes = [C.c_void_p, C.c_int64, _f64p, _f64p]
         _lib.oracle_shape_at.restype = C.c_int
         _lib.oracle_shape_at.argtypes = [C.c_void_p, C.c_int64, _f64p, _f64p, _f64p, _f64p]
+        _lib.oracle_reconstruction.restype = C.c_int
+        _lib.oracle_reconstruction.argtypes = [_f64p, _i32p, C.c_int32, _i32p, _i32p, _f64p, C.c_int32, C.c_int32,
+                                               C.c_int32, _f64p, _f64p]
     return _lib
 
 
@@ -259,3 +263,49 @@ def ref_grad_linelast_work(args, SOL, ADJ, compute_wint=True, compute_wext=True,
       iv(max(1, s.len_load_additionalInfos)), iv(s.MCRD), s.NBINT, iv(s.nb_load),
       iv(s.nb_patch), iv(s.nb_elem), s.NNODE, iv(nb_cp))
     return gi, ge
+
+
+def _recon_arrays(u_inv, ind_dof_free, bc_target, bc_target_nbelem, bc_values, mcrd, nb_cp):
+    u = np.ascontiguousarray(u_inv, dtype=np.float64)
+    idf = np.ascontiguousarray(ind_dof_free, dtype=np.int32)
+    bt = np.ascontiguousarray(bc_target, dtype=np.int32).reshape(-1)
+    if bt.size == 0:
+        bt = np.zeros(1, np.int32)
+    bn = np.ascontiguousarray(bc_target_nbelem, dtype=np.int32).reshape(-1)
+    bv = np.asfortranarray(np.asarray(bc_values, dtype=np.float64).reshape(2, -1, order="F"))
+    return u, idf, bt, bn, bv
+
+
+def oracle_reconstruction(u_inv, ind_dof_free, bc_target, bc_target_nbelem, bc_values, mcrd, nb_cp):
+    """liboracle's restatement of reconstruction.f:22-135. Returns (sol (nb_cp, mcrd) Fortran
+    order, U (nb_cp mcrd))."""
+    u, idf, bt, bn, bv = _recon_arrays(u_inv, ind_dof_free, bc_target, bc_target_nbelem, bc_values, mcrd, nb_cp)
+    sol = np.zeros((nb_cp, mcrd), order="F")
+    U = np.zeros(nb_cp * mcrd)
+    rc = lib().oracle_reconstruction(_p(u, _f64p), _p(idf, _i32p), int(u.size), _p(bt, _i32p), _p(bn, _i32p),
+                                     _p(bv, _f64p), int(bn.size), int(mcrd), int(nb_cp), _p(sol, _f64p),
+                                     _p(U, _f64p))
+    if rc:
+        raise ValueError("oracle_reconstruction: invalid dof index (code %d)" % rc)
+    return sol, U
+
+
+def ref_reconstruction(u_inv, coords, ind_dof_free, bc_target, bc_target_nbelem, bc_values, mcrd, nb_cp):
+    """YETI's reconstruction (main/reconstruction.f:22) through oracle/ref_shim.f90; the output
+    buffers start at zero."""
+    u, idf, bt, bn, bv = _recon_arrays(u_inv, ind_dof_free, bc_target, bc_target_nbelem, bc_values, mcrd, nb_cp)
+    X = np.asfortranarray(np.asarray(coords, dtype=np.float64).reshape(3, -1, order="F"))
+    idf_full = np.zeros(nb_cp * mcrd, np.int32)
+    idf_full[: idf.size] = idf
+    sol = np.zeros((nb_cp, mcrd), order="F")
+    U = np.zeros(nb_cp * mcrd)
+    keep = []
+
+    def iv(v):
+        x = C.c_int32(int(v)); keep.append(x); return C.byref(x)
+
+    f = ref().ref_reconstruction
+    f.restype = None
+    f(_p(sol, _f64p), _p(U, _f64p), _p(u, _f64p), _p(X, _f64p), _p(idf_full, _i32p), _p(bt, _i32p), iv(bt.size),
+      _p(bn, _i32p), _p(bv, _f64p), iv(bn.size), iv(mcrd), iv(nb_cp), iv(u.size))
+    return sol, U

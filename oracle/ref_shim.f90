@@ -140,3 +140,32 @@ subroutine ref_grad_linelast_work(gradWint, gradWext, computeWint, computeWext, 
         &   load_target_nbelem, load_additionalInfos, MCRD, NBINT, nb_load, nb_patch, nb_elem,    &
         &   NNODE, nb_cp)
 end subroutine ref_grad_linelast_work
+
+! reconstruction (main/reconstruction.f:22) takes bc_target(:) assumed-shape.
+subroutine ref_reconstruction(sol, U, U_inv, COORDS, ind_dof_free, bc_target, len_bc_target,  &
+    &   bc_target_nbelem, bc_values, nb_bc, MCRD, nb_cp, nb_dof_free)                          &
+    &   bind(C, name='ref_reconstruction')
+    use iso_c_binding
+    implicit none
+    integer(c_int), intent(in) :: len_bc_target, nb_bc, MCRD, nb_cp, nb_dof_free
+    integer(c_int), intent(in) :: ind_dof_free(nb_cp * MCRD), bc_target(len_bc_target),        &
+        &   bc_target_nbelem(nb_bc)
+    real(c_double), intent(in) :: U_inv(nb_dof_free), COORDS(3, nb_cp), bc_values(2, nb_bc)
+    real(c_double), intent(inout) :: sol(nb_cp, MCRD), U(nb_cp * MCRD)
+    logical :: cpl
+
+    interface
+        subroutine reconstruction(sol, U, U_inv, COORDS, ind_dof_free, bc_target,               &
+            &   bc_target_nbelem, bc_values, COUPLG_flag, nb_bc, MCRD, nb_cp, nb_dof_free)
+            integer, intent(in) :: nb_cp, MCRD, nb_dof_free, nb_bc
+            integer, intent(in) :: ind_dof_free(nb_cp * MCRD), bc_target(:), bc_target_nbelem(nb_bc)
+            double precision, intent(in) :: U_inv(nb_dof_free), COORDS(3, nb_cp), bc_values(2, nb_bc)
+            logical, intent(in) :: COUPLG_flag
+            double precision, intent(out) :: sol(nb_cp, MCRD), U(nb_cp * MCRD)
+        end subroutine
+    end interface
+
+    cpl = .false.
+    call reconstruction(sol, U, U_inv, COORDS, ind_dof_free, bc_target, bc_target_nbelem,       &
+        &   bc_values, cpl, nb_bc, MCRD, nb_cp, nb_dof_free)
+end subroutine ref_reconstruction

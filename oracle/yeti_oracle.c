@@ -598,3 +598,37 @@ int oracle_shape_at(const yeti_system_inputs *in, int64_t elem, const double *pt
     }
     return YETI_ERR_INVALID_ARGUMENT;
 }
+
+/* reconstruction.f:22-135: the full displacement from the free-dof solution. U (nb_cp MCRD,
+ * dof-interleaved) takes U_inv at ind_dof_free (1-based, :60-62); then, in boundary-condition
+ * order, every target CP of a condition whose direction bc_values(1, .) <= 3 gets the prescribed
+ * value bc_values(2, .) (:64-74; a later condition overwrites an earlier one); sol(nb_cp, MCRD)
+ * (column-major) is U reshaped (:131-135). Entries neither free nor prescribed are 0 here (the
+ * reference leaves them as the caller's output buffer holds them). */
+int oracle_reconstruction(const double *U_inv, const int32_t *ind_dof_free, int32_t nb_dof_free,
+                          const int32_t *bc_target, const int32_t *bc_target_nbelem, const double *bc_values,
+                          int32_t nb_bc, int32_t MCRD, int32_t nb_cp, double *sol, double *U)
+{
+    const int64_t ndof = (int64_t)nb_cp * MCRD;
+    for (int64_t i = 0; i < ndof; ++i) U[i] = 0.0;
+    for (int32_t d = 0; d < nb_dof_free; ++d) {
+        const int64_t g = (int64_t)ind_dof_free[d] - 1;
+        if (g < 0 || g >= ndof) return YETI_ERR_INVALID_ARGUMENT;
+        U[g] = U_inv[d];
+    }
+    int64_t count = 0;
+    for (int32_t b = 0; b < nb_bc; ++b) {
+        if (bc_values[2 * b] <= 3.0) {
+            const int dir = (int)bc_values[2 * b];
+            for (int32_t k = 0; k < bc_target_nbelem[b]; ++k) {
+                const int64_t g = ((int64_t)bc_target[count + k] - 1) * MCRD + dir - 1;
+                if (g < 0 || g >= ndof) return YETI_ERR_INVALID_ARGUMENT;
+                U[g] = bc_values[2 * b + 1];
+            }
+        }
+        count += bc_target_nbelem[b];
+    }
+    for (int i = 0; i < MCRD; ++i)
+        for (int32_t cp = 0; cp < nb_cp; ++cp) sol[(int64_t)i * nb_cp + cp] = U[(int64_t)cp * MCRD + i];
+    return 0;
+}
