@@ -128,6 +128,86 @@ def bench_serving(n, rate, seed, budget, plain_n):
     return out
 
 
+def bench_c2_solve(nel, p, jacobi_iters):
+    """The C2 solve after the assembly: device CSR of K + K^T, its SpMV against the HBM roofline,
+    Jacobi PCG for a bounded number of iterations (residual reached), and the fast-diagonalisation
+    PCG session (FP64, matrix-free) to 1e-10, its solution checked with the assembled CSR."""
+    import numpy as np
+    import torch
+    from yeti_amd.assembly import Assembler
+    from yeti_amd.geometry import make_box_patch
+    from yeti_amd.serving import SolveServer, Request
+    from yeti_amd.solver import StiffnessCSR, free_mask_from_args
+    bp = make_box_patch(nel=(nel, nel, nel), p=p)
+    asm = Assembler(bp.args)
+    K = torch.empty(asm.nb_data, dtype=torch.float64, device="cuda")
+    asm.run_values(K)
+    F = asm.run_rhs()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    csr = StiffnessCSR(asm)
+    torch.cuda.synchronize()
+    t_pat = time.perf_counter() - t0
+    vals = csr.values(K)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        csr.values(K, out=vals)
+    torch.cuda.synchronize()
+    t_val = (time.perf_counter() - t0) / 3
+    del K
+    torch.cuda.empty_cache()
+    x = torch.randn(csr.nrows, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    csr.spmv(vals, x, y)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(10):
+        csr.spmv(vals, x, y)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    spmv_ms = e0.elapsed_time(e1) / 10
+    spmv_bytes = 12 * csr.nnz + 8 * (csr.nrows + 1) + 16 * csr.nrows
+    log("C2 %d^3 p=%d: CSR pattern %.2f s (nnz %d), values %.1f ms, SpMV %.3f ms (%.0f GB/s)"
+        % (nel, p, t_pat, csr.nnz, t_val * 1e3, spmv_ms, spmv_bytes / (spmv_ms * 1e-3) / 1e9))
+    mask = free_mask_from_args(bp.args)
+    t0 = time.perf_counter()
+    try:
+        _, it, rel = csr.pcg(vals, F, mask, rtol=1e-10, maxit=jacobi_iters)
+    except Exception as ex:   # not converged within the bound: read the reached residual from the message
+        it, rel = jacobi_iters, float(str(ex).split("relative residual")[1].split()[0])
+    torch.cuda.synchronize()
+    t_jac = time.perf_counter() - t0
+    log("C2 Jacobi PCG: %d iterations in %.2f s, relative residual %.3e" % (it, t_jac, rel))
+    req = Request(0, 0.0, (nel, nel, nel), p, 210000.0, 0.3, -1.0, 20261202)
+    req.args = bp.args
+    srv = SolveServer(budget_bytes=64 << 30, precision="fp64", nstreams=1, chunk=32, rtol=1e-10, max_iters=20000)
+    t0 = time.perf_counter()
+    rep = srv.run([req], keep_solutions=True, realtime=False)
+    t_fdm = time.perf_counter() - t0
+    d = rep["requests"][0]
+    u = torch.as_tensor(rep["x"][0], device="cuda")
+    srv.close()
+    r = (F - csr.spmv(vals, u)) * mask
+    rel_csr = float(r.norm() / (F * mask).norm())
+    log("C2 fast-diagonalisation PCG (FP64, matrix-free): %d iterations in %.2f s, residual %.3e "
+        "(assembled CSR: %.3e)" % (d["iters"], t_fdm, d["relres"], rel_csr))
+    out = {"workload": "%d^3 p=%d cantilever (%d dofs): device CSR of K + K^T from the FP64 COO values, "
+                       "Jacobi PCG (bounded) and fast-diagonalisation PCG to 1e-10" % (nel, p, csr.nrows),
+           "csr_pattern_s": t_pat, "csr_values_ms": t_val * 1e3, "nnz": csr.nnz,
+           "spmv": {"ms": spmv_ms, "roofline": {"bound": "hbm", "achieved": spmv_bytes / (spmv_ms * 1e-3) / 1e9,
+                                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                                "frac": spmv_bytes / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}},
+           "jacobi_pcg": {"iterations": it, "seconds": t_jac, "relres": rel, "converged": rel <= 1e-10},
+           "fdm_pcg_fp64": {"iterations": d["iters"], "seconds": t_fdm, "relres": d["relres"],
+                            "relres_assembled_csr": rel_csr, "stopped_at_fp64_floor": d["floor"],
+                            "failed": d["failed"]}}
+    del csr, vals, asm
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_gradient(nfd, seed=7):
     import numpy as np
     import torch
@@ -285,6 +365,7 @@ def main():
     ap.add_argument("--serve-plain", type=int, default=0,
                     help="requests solved one at a time (plain FP64); 0: all of them")
     ap.add_argument("--grad-fd", type=int, default=50, help="coordinates checked by central differences")
+    ap.add_argument("--jacobi-iters", type=int, default=1000, help="bound on the C2 Jacobi PCG iterations")
     ap.add_argument("--mp-nel", type=int, default=128, help="elements per direction of the mixed-precision solve")
     ap.add_argument("--only-assembly", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -415,6 +496,7 @@ def main():
         "cpu_baseline": cpu,
     }
     if not args.only_assembly:
+        out["c2_solve"] = bench_c2_solve(args.nel, p, args.jacobi_iters)
         out["serving"] = bench_serving(args.serve_requests, args.serve_rate, 20261202, 200 << 30, args.serve_plain)
         out["gradient"] = bench_gradient(args.grad_fd)
         out["mixed_precision"] = bench_mixed_precision(args.mp_nel)
