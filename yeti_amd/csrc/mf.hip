@@ -69,7 +69,11 @@ struct MfCfg {
     static constexpr int NP1 = P + 1;
     static constexpr int NQ = NG * NG * NG;
     static constexpr int NN = NP1 * NP1 * NP1;
-    static constexpr int TPE = NQ > NN ? NQ : NN;       // threads per element
+    static constexpr int TPE0 = NQ > NN ? NQ : NN;
+    // threads per element: padded to a power of two up to a wave, so that an element's threads
+    // lie in one wave and its contraction stages need only wave-level ordering
+    static constexpr int TPE = TPE0 <= 8 ? 8 : TPE0 <= 16 ? 16 : TPE0 <= 32 ? 32 : TPE0 <= 64 ? 64 : TPE0;
+    static constexpr bool WAVE_LOCAL = 64 % TPE == 0;
     static constexpr int EPB = 256 / TPE;               // elements per batch
     static constexpr int BR = 2 * NP1;                  // brick edge (elements)
     static constexpr int BN = BR + P;                   // brick edge (nodes)
@@ -179,6 +183,19 @@ __global__ void mf_geometry_kernel(MfPatch mp, const double *__restrict__ coords
 
 // Brick edge in elements: 2(p+1) for the apply and the shape gradient (8 elements per colour),
 // p+1 for the diagonal blocks (1 element per colour, keeps the 9-value accumulator within LDS).
+// Orders an element's LDS stage writes before the next stage's reads: within a wave when the
+// element's threads share one, else across the workgroup.
+template <bool WAVE_LOCAL>
+__device__ __forceinline__ void mf_stage_sync() {
+    if constexpr (WAVE_LOCAL) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 template <int P, int MODE>
 struct MfBrick {
     static constexpr int BR = MODE == 1 ? (P + 1) : 2 * (P + 1);
@@ -212,7 +229,14 @@ __global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__re
     for (int color = 0; color < NP1 * NP1 * NP1; ++color) {
         const int c1 = color % NP1, c2 = (color / NP1) % NP1, c3 = color / (NP1 * NP1);
         for (int bt = 0; bt < Bk::EPC; bt += C::EPB) {
-            __syncthreads();   // previous batch finished with the staging buffers
+            // the previous colour's accumulation is complete (elements of one colour share no node;
+            // the staging buffers of a slot are used by its own wave only)
+            if constexpr (C::WAVE_LOCAL) {
+                if (bt == 0) __syncthreads();
+                else mf_stage_sync<true>();
+            } else {
+                __syncthreads();
+            }
             const int tt = bt + slot;
             const bool sl = slot < C::EPB && tt < Bk::EPC;
             const int t1 = tt % Bk::K, t2 = (tt / Bk::K) % Bk::K, t3 = tt / (Bk::K * Bk::K);
@@ -239,7 +263,7 @@ __global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__re
                     sm.B[slot][d][gq][rr] = (T)mp.Btab[o];
                     sm.dB[slot][d][gq][rr] = (T)mp.dBtab[o];
                 }
-            __syncthreads();
+            mf_stage_sync<C::WAVE_LOCAL>();
             T yv[NC];
             for (int c = 0; c < NC; ++c) yv[c] = T(0);
             if (MODE != 1) {
@@ -259,7 +283,7 @@ __global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__re
                         b1v[(f * 2 + 1) * NG * NP1 * NP1 + r] = d;
                     }
                 }
-                __syncthreads();
+                mf_stage_sync<C::WAVE_LOCAL>();
                 // stage 2: contract j -> g1. b0[(f*3+v)*NG*NG*NP1 + (k*NG+g1)*NG + g0], v: val, d0, d1
                 if (live && r < NG * NG * NP1) {
                     const int g0 = r % NG, g1 = (r / NG) % NG, k = r / (NG * NG);
@@ -277,7 +301,7 @@ __global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__re
                         b0[(f * 3 + 2) * NG * NG * NP1 + r] = d1;
                     }
                 }
-                __syncthreads();
+                mf_stage_sync<C::WAVE_LOCAL>();
                 // stage 3 (k -> g2) and the quadrature-point work
                 if (live && r < NQ) {
                     const int g0 = r % NG, g1 = (r / NG) % NG, g2 = r / (NG * NG);
@@ -343,7 +367,7 @@ __global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__re
                         b1v[(k * 4 + 0) * NQ + r] = tb;
                     }
                 }
-                __syncthreads();
+                mf_stage_sync<C::WAVE_LOCAL>();
                 // backward stage 3^T (g2 -> k): P, R0, R1 per component into b0
                 if (live && r < NG * NG * NP1) {
                     const int g0 = r % NG, g1 = (r / NG) % NG, k = r / (NG * NG);
@@ -360,7 +384,7 @@ __global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__re
                         b0[(c * 3 + 2) * NG * NG * NP1 + r] = R1;
                     }
                 }
-                __syncthreads();
+                mf_stage_sync<C::WAVE_LOCAL>();
                 // backward stage 2^T (g1 -> j): P', R0' into b1v
                 if (live && r < NG * NP1 * NP1) {
                     const int g0 = r % NG, j = (r / NG) % NP1, k = r / (NG * NP1);
@@ -376,7 +400,7 @@ __global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__re
                         b1v[(c * 2 + 1) * NG * NP1 * NP1 + r] = R0;
                     }
                 }
-                __syncthreads();
+                mf_stage_sync<C::WAVE_LOCAL>();
                 // backward stage 1^T (g0 -> i): node values
                 if (live && r < NN) {
                     const int i = r % NP1, jk = r / NP1;
