@@ -174,6 +174,12 @@ int32_t yeti_csr_values(const yeti_csr *c, const double *Kdata_dev, double *vals
 int32_t yeti_csr_spmv(const yeti_csr *c, const double *vals_dev, const double *x_dev, double *y_dev,
                       void *stream);
 void    yeti_csr_destroy(yeti_csr *c);
+/* scipy's canonical K_side + K_side^T: the structural CSR with every entry whose sum is exactly 0
+ * removed (scipy's sparse add drops them; overlay/iga_model.py:391-395 returns that matrix).
+ * Outputs (device): rowptr_out[3 nb_cp + 1], colind_out / vals_out with room for yeti_csr_nnz
+ * entries; *nnz_out (host) receives the kept count. Order within rows is kept (ascending). */
+int32_t yeti_csr_drop_zeros(const yeti_csr *c, const double *vals_dev, int64_t *rowptr_out, int32_t *colind_out,
+                            double *vals_out, int64_t *nnz_out, void *stream);
 
 /* ---- Jacobi-preconditioned CG, FP64, on a device CSR --------------------------------------
  * Solves A[free][:, free] x[free] = b[free] (free_mask[i] = 1 for free dofs; masked entries of x

@@ -92,3 +92,47 @@ def test_pcg_reports_non_convergence(gpu):
     with pytest.raises(YetiError) as ei:
         csr.pcg(vals, F, free_mask_from_args(bp.args), rtol=1e-14, maxit=3)
     assert ei.value.code == 6
+
+
+def test_canonical_drops_exact_zeros_like_scipy(gpu):
+    """The canonical form (scipy's K_side + K_side^T drops entries that sum to exactly 0): with
+    exact zeros planted in the device values, the compacted CSR equals scipy's eliminate_zeros of
+    the same matrix (indptr, indices and values bit-exact, row order kept)."""
+    import torch
+    from yeti_amd.assembly import Assembler
+    from yeti_amd.solver import StiffnessCSR
+    print("[case] canonical with planted zeros", flush=True)
+    bp = make_box_patch(nel=(4, 3, 2), p=2, seed=11)
+    asm = Assembler(bp.args)
+    K, r, c, F = asm.run_coo()
+    csr = StiffnessCSR(asm)
+    vals = csr.values(K)
+    rng = np.random.default_rng(5)
+    kill = torch.from_numpy(rng.choice(csr.nnz, csr.nnz // 7, replace=False)).cuda()
+    vals[kill] = 0.0
+    rp, ci, v = (t.cpu().numpy() for t in csr.canonical(vals))
+    rp0, ci0 = csr.pattern_host()
+    B = sp.csr_matrix((vals[: csr.nnz].cpu().numpy(), ci0, rp0), shape=(csr.nrows, csr.nrows))
+    B.eliminate_zeros()
+    assert np.array_equal(rp, B.indptr) and np.array_equal(ci, B.indices) and np.array_equal(v, B.data)
+    assert v.size == csr.nnz - kill.numel()
+
+
+@pytest.mark.parametrize("nel,p", [((16, 4, 4), 2), ((3, 2, 2), 3)])
+def test_build_stiffness_matrix_dropin(gpu, nel, p):
+    """yeti_amd.solver.build_stiffness_matrix against the reference's own steps on the oracle COO:
+    (coo_matrix(...).tocsc() + transpose) (overlay/iga_model.py:389-395): same type (CSC), same
+    canonical pattern, values within 1e-12 of max |A|, same rhs."""
+    from yeti_amd.solver import build_stiffness_matrix
+    print("[case] build_stiffness_matrix %s p=%d" % (nel, p), flush=True)
+    bp = make_box_patch(nel=nel, p=p, seed=2 + p)
+    A, rhs = build_stiffness_matrix(bp.args)
+    K0, r0, c0, F0 = O.oracle_sys_linmat((), kwargs=bp.args)
+    n = int(bp.args["nb_dof_tot"])
+    S = sp.coo_matrix((K0, (r0, c0)), shape=(n, n), dtype="float64").tocsc()
+    B = S + S.transpose()
+    assert A.format == B.format == "csc"
+    B.sort_indices()
+    assert np.array_equal(A.indptr, B.indptr) and np.array_equal(A.indices, B.indices)
+    assert np.abs(A.data - B.data).max() <= 1e-12 * np.abs(B.data).max()
+    assert np.abs(rhs - F0).max() <= 1e-12 * max(np.abs(F0).max(), 1e-300)

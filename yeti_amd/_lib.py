@@ -68,6 +68,8 @@ SIGNATURES = {
     "yeti_ir_step": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p]),
     "yeti_ir_status_async": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_ir_destroy": (None, [C.c_void_p]),
+    "yeti_csr_drop_zeros": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]),
     "yeti_reconstruction": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_pcg_csr": (C.c_int32, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

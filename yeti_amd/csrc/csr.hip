@@ -351,7 +351,61 @@ __global__ void __launch_bounds__(256) spmv_kernel(int64_t nrows, const int64_t 
 
 using namespace yeti;
 
+namespace yeti {
+// ---- canonical form: scipy's K_side + K_side^T drops the entries that sum to exactly 0 ----
+__global__ void __launch_bounds__(256) nz_count_kernel(int64_t nrows, const int64_t *rowptr, const double *vals,
+                                                       int32_t *cnt) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    int32_t n = 0;
+    for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) n += vals[k] != 0.0;
+    cnt[r] = n;
+}
+
+__global__ void __launch_bounds__(256) nz_compact_kernel(int64_t nrows, const int64_t *rowptr, const int32_t *colind,
+                                                         const double *vals, const int64_t *newptr,
+                                                         int32_t *colind_out, double *vals_out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    int64_t o = newptr[r];
+    for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+        const double v = vals[k];
+        if (v != 0.0) {
+            colind_out[o] = colind[k];
+            vals_out[o] = v;
+            ++o;
+        }
+    }
+}
+
+}  // namespace yeti
+
 extern "C" {
+
+int32_t yeti_csr_drop_zeros(const yeti_csr *c, const double *vals_dev, int64_t *rowptr_out, int32_t *colind_out,
+                            double *vals_out, int64_t *nnz_out, void *stream) {
+    if (!c || !vals_dev || !rowptr_out || !colind_out || !vals_out || !nnz_out)
+        return set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t nrows = 3 * (int64_t)c->nb_cp;
+    DevBuf<int32_t> cnt;
+    YETI_HIP(cnt.alloc(std::max<int64_t>(nrows, 1)));
+    const unsigned g = (unsigned)((nrows + 255) / 256);
+    if (nrows > 0) {
+        hipLaunchKernelGGL(nz_count_kernel, dim3(g), dim3(256), 0, st, nrows, c->rowptr.p, vals_dev, cnt.p);
+        YETI_HIP(hipGetLastError());
+    }
+    int rc = exclusive_scan<int32_t>(cnt.p, nrows, rowptr_out, st);   // ends with a stream sync
+    if (rc) return rc;
+    if (nrows > 0) {
+        hipLaunchKernelGGL(nz_compact_kernel, dim3(g), dim3(256), 0, st, nrows, c->rowptr.p, c->colind.p, vals_dev,
+                           (const int64_t *)rowptr_out, colind_out, vals_out);
+        YETI_HIP(hipGetLastError());
+    }
+    YETI_HIP(hipMemcpyAsync(nnz_out, rowptr_out + nrows, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    YETI_HIP(hipStreamSynchronize(st));
+    return 0;
+}
 
 int32_t yeti_csr_create(const yeti_asm *h, void *stream, yeti_csr **out) {
     if (!h || !out) return set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");

@@ -60,11 +60,32 @@ class StiffnessCSR:
                                                        C.c_void_p(ci.ctypes.data)))
         return rp, ci[: self.nnz]
 
-    def to_scipy(self, vals):
+    def canonical(self, vals, stream=None):
+        """scipy's canonical K_side + K_side^T on the device: (rowptr int64, colind int32, values)
+        with the entries that sum to exactly 0 removed, as scipy's sparse add does."""
+        rp = torch.empty(self.nrows + 1, dtype=torch.int64, device=vals.device)
+        ci = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=vals.device)
+        v = torch.empty(max(self.nnz, 1), dtype=torch.float64, device=vals.device)
+        n = C.c_int64(0)
+        _lib.check(_lib.lib().yeti_csr_drop_zeros(self._h, C.c_void_p(vals.data_ptr()), C.c_void_p(rp.data_ptr()),
+                                                  C.c_void_p(ci.data_ptr()), C.c_void_p(v.data_ptr()), C.byref(n),
+                                                  _stream_handle(stream)))
+        return rp, ci[: n.value], v[: n.value]
+
+    def to_scipy(self, vals, canonical=False, fmt="csr"):
+        """The matrix as a scipy sparse matrix; canonical=True drops exact zeros (scipy's form of
+        K_side + K_side^T); fmt "csc" gives the reference's type (the matrix is symmetric, so its
+        CSC arrays are the CSR ones)."""
         import scipy.sparse as sp
-        rp, ci = self.pattern_host()
-        v = vals[: self.nnz].cpu().numpy()
+        if canonical:
+            rp, ci, v = (t.cpu().numpy() for t in self.canonical(vals))
+        else:
+            rp, ci = self.pattern_host()
+            v = vals[: self.nnz].cpu().numpy()
+        if fmt == "csc":
+            return sp.csc_matrix((v, ci, rp), shape=(self.nrows, self.nrows))
         return sp.csr_matrix((v, ci, rp), shape=(self.nrows, self.nrows))
+
 
     def pcg(self, vals, b, free_mask, rtol=1e-10, maxit=100000, check_every=20, stream=None):
         """Jacobi-PCG on A[free][:, free]; returns (x, iterations, relative residual)."""
@@ -101,3 +122,17 @@ def build_and_solve(args, rtol=1e-10, maxit=200000):
     mask = free_mask_from_args(d)
     x, it, rel = csr.pcg(vals, F, mask, rtol=rtol, maxit=maxit)
     return x, it, rel, (asm, csr, K, vals, F, mask)
+
+
+def build_stiffness_matrix(args):
+    """Drop-in for IgaModel.build_stiffness_matrix (overlay/iga_model.py:375-395): the assembly
+    and K_side + K_side^T on the GPU, returned as the reference returns them (scipy CSC in
+    canonical form, rhs as numpy)."""
+    from ._abi import args_to_dict
+    d = args if isinstance(args, dict) else args_to_dict(args)
+    asm = Assembler(d)
+    K, r, c, F = asm.run_coo()
+    csr = StiffnessCSR(asm)
+    vals = csr.values(K)
+    A = csr.to_scipy(vals, canonical=True, fmt="csc")
+    return A, F.cpu().numpy()
