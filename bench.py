@@ -336,6 +336,19 @@ def bench_mixed_precision(nel):
                              "algorithmic_bytes": byts,
                              "max_abs_diff_vs_solution": float(np.abs(U.cpu().numpy() - xs["mixed"]).max())}
     log("%d^3 p=2 reconstruction: %.3f ms (%.0f GB/s algorithmic)" % (nel, ms, byts / (ms * 1e-3) / 1e9))
+    # displacement at 2^20 random parametric points (evaluate_disp.f), host arrays in and out
+    from yeti_amd.postproc import evaldisp_points
+    pts = np.random.default_rng(1).uniform(0.0, 1.0, (1 << 20, 3))
+    solT = U.cpu().numpy().reshape(-1, 3).T
+    kw = dict(IEN=a["IEN"], elementsByPatch=a["nb_elem_patch"], Nkv=a["Nkv"], Ukv=a["Ukv"], Nijk=a["Nijk"],
+              weight=a["weight"], Jpqr=a["Jpqr"], NNODE=a["NNODE"])
+    evaldisp_points(1, pts[:1024], solT, **kw)
+    t0 = time.perf_counter()
+    evaldisp_points(1, pts, solT, **kw)
+    t_ev = time.perf_counter() - t0
+    out["evaldisp"] = {"points": pts.shape[0], "seconds_incl_transfers": t_ev,
+                       "points_per_s": pts.shape[0] / t_ev}
+    log("%d^3 p=2 evaldisp: %d points in %.3f s (host arrays in and out)" % (nel, pts.shape[0], t_ev))
     asm = Assembler(bp.args)
     F = asm.run_rhs().cpu().numpy()
     u64, u32 = xs["fp64"], xs["mixed"]

@@ -285,6 +285,26 @@ void    yeti_ir_destroy(yeti_ir *s);
  * reshaped. Device pointers: u_inv, ind_dof_free, bc_target (concatenated targets, 1-based CP
  * ids), sol, U. Host pointers: bc_target_nbelem[nb_bc], bc_values[2*nb_bc]. Out-of-range dof
  * indices are skipped on the device; callers validate them (the Python mirror does). */
+/* ---- Displacement at parametric points ------------------------------------------------------
+ * Replaces f2py postproc.evaldisp / evaldispmulti (src/yeti_iga/postprocessing/evaluate_disp.f:
+ * 20-103, 105-189; called at OPT_modelling.py:573): for each point xi (3 per point) of one patch,
+ * the element containing it (igaparmetrization.f90:321-372: per direction a binary search over
+ * the interior knots, intervals [left, right), then the element whose spans match, element 1 when
+ * none or several do), the rational basis there (evaluatenurbsfcts.f:94-201) and
+ * disp = sum_a R_a sol(:, IEN(a, e)); disp has 3 values per point (beyond mcrd: 0). All pointers
+ * are host memory; the evaluation runs on the device. Degrees up to 8 per direction. */
+typedef struct yeti_nurbs_patch {
+    int32_t p[3];                 /* degrees */
+    int32_t nkv[3];               /* knot counts */
+    const double *U[3];           /* knot vectors */
+    int32_t nelem, nnode;
+    const int32_t *nijk;          /* (3, nelem), 1-based spans */
+    const int32_t *ien;           /* (nnode, nelem), 1-based CP ids */
+    const double *weight;         /* (nnode, nelem) */
+} yeti_nurbs_patch;
+int32_t yeti_evaldisp(const yeti_nurbs_patch *patch, int32_t nb_cp, int32_t mcrd, const double *sol, int64_t npts,
+                      const double *xi, double *disp);
+
 int32_t yeti_reconstruction(const double *u_inv, const int32_t *ind_dof_free, int32_t nb_dof_free,
                             const int32_t *bc_target, const int32_t *bc_target_nbelem_host,
                             const double *bc_values_host, int32_t nb_bc, int32_t mcrd, int32_t nb_cp,

@@ -4,9 +4,10 @@ Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
 only as the checker. Two independent CPU implementations are exposed:
 
 * ``oracle_sys_linmat`` -- liboracle.so, our C restatement (oracle/yeti_oracle.c);
-* ``oracle_reconstruction`` -- the same library's restatement of the displacement scatter;
+* ``oracle_reconstruction`` / ``oracle_evaldisp`` -- the same library's restatements of the
+  displacement scatter and of the displacement at parametric points;
 * ``ref_sys_linmat_omp`` / ``ref_sys_linmat_serial`` / ``ref_dersbasisfuns`` / ``ref_gauss`` /
-  ``ref_grad_linelast_work`` / ``ref_reconstruction`` -- oracle/_ref/libyetiref.so, YETI's own Fortran compiled from
+  ``ref_grad_linelast_work`` / ``ref_reconstruction`` / ``ref_evaldisp`` -- oracle/_ref/libyetiref.so, YETI's own Fortran compiled from
   its sources by oracle/build_ref.sh (present only where the YETI sources were available at
   build time; callers must handle ``ref_available() == False``).
 """
@@ -50,6 +51,9 @@ def lib():
         _lib.oracle_element_matrix.argtypes = [C.c_void_p, C.c_int64, _f64p, _f64p]
         _lib.oracle_shape_at.restype = C.c_int
         _lib.oracle_shape_at.argtypes = [C.c_void_p, C.c_int64, _f64p, _f64p, _f64p, _f64p]
+        _lib.oracle_evaldisp.restype = C.c_int
+        _lib.oracle_evaldisp.argtypes = [_i32p, _i32p, _f64p, _f64p, _f64p, C.c_int32, C.c_int32, _i32p, _i32p, _f64p,
+                                         C.c_int32, _f64p, C.c_int64, _f64p, _f64p]
         _lib.oracle_reconstruction.restype = C.c_int
         _lib.oracle_reconstruction.argtypes = [_f64p, _i32p, C.c_int32, _i32p, _i32p, _f64p, C.c_int32, C.c_int32,
                                                C.c_int32, _f64p, _f64p]
@@ -309,3 +313,45 @@ def ref_reconstruction(u_inv, coords, ind_dof_free, bc_target, bc_target_nbelem,
     f(_p(sol, _f64p), _p(U, _f64p), _p(u, _f64p), _p(X, _f64p), _p(idf_full, _i32p), _p(bt, _i32p), iv(bt.size),
       _p(bn, _i32p), _p(bv, _f64p), iv(bn.size), iv(mcrd), iv(nb_cp), iv(u.size))
     return sol, U
+
+
+def oracle_evaldisp(numpatch, xi_points, sol, IEN, elementsByPatch, Nkv, Ukv, Nijk, weight, Jpqr, NNODE, mcrd=3):
+    """liboracle's restatement of evalDisp (evaluate_disp.f:20-103) at many points:
+    xi_points (npts, 3) -> (npts, 3); sol (mcrd, nb_cp)."""
+    from yeti_amd.postproc import _patch
+    s, keep = _patch(numpatch, IEN, elementsByPatch, Nkv, Ukv, Nijk, weight, Jpqr, NNODE)
+    p = np.array(list(s.p), np.int32)
+    nkv = np.array(list(s.nkv), np.int32)
+    S = np.asfortranarray(np.asarray(sol, np.float64).reshape(mcrd, -1, order="F"))
+    X = np.ascontiguousarray(np.asarray(xi_points, np.float64).reshape(-1, 3))
+    out = np.zeros((X.shape[0], 3))
+    U = keep["U"]
+    rc = lib().oracle_evaldisp(_p(p, _i32p), _p(nkv, _i32p), _p(U[0], _f64p), _p(U[1], _f64p), _p(U[2], _f64p),
+                               int(s.nelem), int(s.nnode), _p(keep["nijk"], _i32p), _p(keep["ien"], _i32p),
+                               _p(keep["w"], _f64p), int(mcrd), _p(S, _f64p), X.shape[0], _p(X, _f64p),
+                               _p(out, _f64p))
+    if rc:
+        raise ValueError("oracle_evaldisp failed (%d)" % rc)
+    return out
+
+
+def ref_evaldisp(d, numpatch, xi, sol):
+    """YETI's evalDisp (postprocessing/evaluate_disp.f:20) through oracle/ref_shim.f90 for one point
+    xi (3,), with the flattened model inputs of the dict d; sol (3, nb_cp)."""
+    s, keep = pack_system_inputs(d)
+    S = np.asfortranarray(np.asarray(sol, np.float64).reshape(3, -1, order="F"))
+    X = np.ascontiguousarray(xi, np.float64)
+    out = np.zeros(3)
+    matp = np.asfortranarray(np.asarray(d["MATERIAL_PROPERTIES"], np.float64)[:2])
+    keep2 = []
+
+    def iv(v):
+        x = C.c_int32(int(v)); keep2.append(x); return C.byref(x)
+
+    f = ref().ref_evaldisp
+    f.restype = None
+    f(_p(out, _f64p), iv(numpatch), _p(X, _f64p), _p(S, _f64p), s.COORDS3D, s.IEN, iv(s.len_IEN), s.nb_elem_patch,
+      s.Nkv, s.Ukv, iv(s.len_Ukv), s.Nijk, s.weight, iv(s.len_weight), s.Jpqr, s.ELT_TYPE, iv(s.len_ELT_TYPE),
+      matp.ctypes.data_as(_f64p), s.TENSOR, iv(s.len_TENSOR), s.PROPS, iv(max(1, s.len_PROPS)), s.JPROPS, s.NNODE,
+      iv(s.nb_patch), iv(s.nb_elem), iv(s.nb_cp), iv(3))
+    return out

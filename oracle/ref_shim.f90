@@ -169,3 +169,48 @@ subroutine ref_reconstruction(sol, U, U_inv, COORDS, ind_dof_free, bc_target, le
     call reconstruction(sol, U, U_inv, COORDS, ind_dof_free, bc_target, bc_target_nbelem,       &
         &   bc_values, cpl, nb_bc, MCRD, nb_cp, nb_dof_free)
 end subroutine ref_reconstruction
+
+! evalDisp (postprocessing/evaluate_disp.f:20) takes assumed-shape IEN, Ukv, weight and PROPS.
+subroutine ref_evaldisp(disp, NumPatch, Xi, SOL, COORDS3D, IEN, len_IEN, nb_elem_patch, Nkv, Ukv,  &
+    &   len_Ukv, Nijk, weight, len_weight, Jpqr, ELT_TYPE, len_elt, MATERIAL_PROPERTIES, TENSOR,    &
+    &   len_tensor, PROPS, len_PROPS, JPROPS, NNODE, nb_patch, nb_elem, nb_cp, MCRD)               &
+    &   bind(C, name='ref_evaldisp')
+    use iso_c_binding
+    implicit none
+    integer(c_int), intent(in) :: NumPatch, len_IEN, len_Ukv, len_weight, len_elt, len_tensor,       &
+        &   len_PROPS, nb_patch, nb_elem, nb_cp, MCRD
+    integer(c_int), intent(in) :: IEN(len_IEN), nb_elem_patch(nb_patch), Nkv(3, nb_patch),          &
+        &   Nijk(3, nb_elem), Jpqr(3, nb_patch), JPROPS(nb_patch), NNODE(nb_patch)
+    real(c_double), intent(in) :: Xi(3), SOL(MCRD, nb_cp), COORDS3D(3, nb_cp), Ukv(len_Ukv),         &
+        &   weight(len_weight), MATERIAL_PROPERTIES(2, nb_patch), PROPS(len_PROPS)
+    character(kind=c_char), intent(in) :: ELT_TYPE(len_elt), TENSOR(len_tensor)
+    real(c_double), intent(out) :: disp(3)
+    character(len=len_elt) :: elt_s
+    character(len=len_tensor) :: tensor_s
+    integer :: i
+
+    interface
+        subroutine evalDisp(disp, NumPatch, Xi, SOL, COORDS3D, IEN, nb_elem_patch, Nkv, Ukv, Nijk,   &
+            &   weight, Jpqr, ELT_TYPE, MATERIAL_PROPERTIES, TENSOR, PROPS, JPROPS, NNODE, nb_patch,  &
+            &   nb_elem, nb_cp, MCRD)
+            integer, intent(in) :: nb_cp, MCRD, NNODE(nb_patch), nb_patch, nb_elem, IEN(:),           &
+                &   nb_elem_patch(nb_patch), JPROPS(nb_patch), Nkv(3, nb_patch), Jpqr(3, nb_patch),   &
+                &   Nijk(3, nb_elem), NumPatch
+            double precision, intent(in) :: COORDS3D(3, nb_cp), Ukv(:), weight(:),                  &
+                &   MATERIAL_PROPERTIES(2, nb_patch), PROPS(:), SOL(MCRD, nb_cp), Xi(3)
+            character(len=*), intent(in) :: TENSOR, ELT_TYPE
+            double precision, intent(out) :: disp(3)
+        end subroutine
+    end interface
+
+    do i = 1, len_elt
+        elt_s(i:i) = ELT_TYPE(i)
+    enddo
+    do i = 1, len_tensor
+        tensor_s(i:i) = TENSOR(i)
+    enddo
+    disp = 0.0d0
+    call evalDisp(disp, NumPatch, Xi, SOL, COORDS3D, IEN, nb_elem_patch, Nkv, Ukv, Nijk, weight,     &
+        &   Jpqr, elt_s, MATERIAL_PROPERTIES, tensor_s, PROPS, JPROPS, NNODE, nb_patch, nb_elem, nb_cp, &
+        &   MCRD)
+end subroutine ref_evaldisp

@@ -632,3 +632,65 @@ int oracle_reconstruction(const double *U_inv, const int32_t *ind_dof_free, int3
         for (int32_t cp = 0; cp < nb_cp; ++cp) sol[(int64_t)i * nb_cp + cp] = U[(int64_t)cp * MCRD + i];
     return 0;
 }
+
+/* evaluate_disp.f:20-103 (evalDisp) for one patch: for each parametric point, the element that
+ * contains it (igaparmetrization.f90:321-372: per direction a binary search over the interior
+ * knots ukv(p+1 : nkv-p), BINARYSEARCH :291-319 with INRANGE [left, right) :282-289, span =
+ * index + p; then the element whose Nijk equals the spans, element 1 when none or several do),
+ * the rational basis there (evaluatenurbsfcts.f:94-201: reversed local numbering, element
+ * weights, divided by the sum) and disp = sum_a R_a SOL(:, IEN(a, e)); components beyond mcrd
+ * are 0. nijk (3, nelem), ien (nnode, nelem) 1-based, weight (nnode, nelem), sol (mcrd, nb_cp),
+ * xi (3, npts), disp (3, npts). */
+static int ref_binary_search(double val, const double *uk, int n)
+{
+    int low = 1, high = n, found = 0, middle = 1;   /* 1-based as the reference */
+    while (low != high && !found) {
+        middle = (low + high) / 2;
+        if (val >= uk[middle - 1] && val < uk[middle]) found = 1;
+        else if (val < uk[middle - 1]) high = middle;
+        else low = middle + 1;
+    }
+    return middle;
+}
+
+int oracle_evaldisp(const int32_t *p, const int32_t *nkv, const double *U0, const double *U1, const double *U2,
+                    int32_t nelem, int32_t nnode, const int32_t *nijk, const int32_t *ien, const double *weight,
+                    int32_t mcrd, const double *sol, int64_t npts, const double *xi, double *disp)
+{
+    const double *U[3] = {U0, U1, U2};
+    for (int64_t t = 0; t < npts; ++t) {
+        int span[3];
+        for (int d = 0; d < 3; ++d)
+            span[d] = ref_binary_search(xi[3 * t + d], U[d] + p[d], nkv[d] - 2 * p[d]) + p[d];
+        int e = -1, nfound = 0;
+        for (int k = 0; k < nelem; ++k)
+            if (nijk[3 * k] == span[0] && nijk[3 * k + 1] == span[1] && nijk[3 * k + 2] == span[2]) {
+                if (e < 0) e = k;
+                ++nfound;
+            }
+        if (nfound != 1) e = 0;
+        double B[3][MAXP + 2], dB[3][MAXP + 2];
+        for (int d = 0; d < 3; ++d) {
+            const int s = nijk[3 * e + d];
+            oracle_dersbasisfuns(s, p[d], xi[3 * t + d], U[d], B[d], dB[d]);
+        }
+        double R[(MAXP + 1) * (MAXP + 1) * (MAXP + 1)];
+        double sum = 0.0;
+        int a = 0;
+        for (int k = 0; k <= p[2]; ++k)
+            for (int j = 0; j <= p[1]; ++j)
+                for (int i = 0; i <= p[0]; ++i, ++a) {
+                    R[a] = B[0][p[0] - i] * B[1][p[1] - j] * B[2][p[2] - k] * weight[(int64_t)nnode * e + a];
+                    sum += R[a];
+                }
+        const double inv = 1.0 / sum;
+        double u[3] = {0.0, 0.0, 0.0};
+        for (a = 0; a < nnode; ++a) {
+            const double r = R[a] * inv;
+            const int64_t cp = (int64_t)ien[(int64_t)nnode * e + a] - 1;
+            for (int c = 0; c < mcrd && c < 3; ++c) u[c] += r * sol[cp * mcrd + c];
+        }
+        for (int c = 0; c < 3; ++c) disp[3 * t + c] = u[c];
+    }
+    return 0;
+}

@@ -70,6 +70,7 @@ SIGNATURES = {
     "yeti_ir_destroy": (None, [C.c_void_p]),
     "yeti_csr_drop_zeros": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p]),
+    "yeti_evaldisp": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "yeti_reconstruction": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_pcg_csr": (C.c_int32, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -90,6 +91,9 @@ def lib():
     """The loaded library. Raises when it is not built."""
     global _lib
     if _lib is None:
+        # torch first: its HIP runtime (same soname as the system's) is then the one this library
+        # binds to, so the process holds one runtime whichever module the caller touches first
+        import torch  # noqa: F401
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libyeti_hip.so is not built: run `python -m yeti_amd.csrc.build` "
                                "(or __graft_entry__.build())")
