@@ -464,6 +464,17 @@ def main():
     log("element kernel: %d launches of %d elements, %.3f ms each (%.0f GB/s); values-only %.3f ms (%.1f TFLOP/s FP64)"
         % (nlaunch, elems_per_launch, kern_ms, achieved, val_ms, tflops))
     traffic = read_traffic(os.path.join(ROOT, "profiles", "elem_stiffness_64cube_p%d_traffic.md" % p))
+    # consistent mass matrix of the same model in the same COO layout (compute_CMASSMatrix.f),
+    # written into the same buffers: HBM-bound (values + indices written)
+    asm.run_mass(K, r, c)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        asm.run_mass(K, r, c)
+    torch.cuda.synchronize()
+    mass_ms = (time.perf_counter() - t0) / 3 * 1e3
+    mass_bytes = bp.nb_elem * bytes_elem + 24 * bp.nb_cp
+    log("mass matrix (COO values + indices): %.2f ms, %.0f GB/s" % (mass_ms, mass_bytes / (mass_ms * 1e-3) / 1e9))
     del K, r, c, F
     torch.cuda.empty_cache()
 
@@ -507,6 +518,10 @@ def main():
             "frac": tflops / FP64_PEAK_TFS, "flops_per_launch": elems_per_launch * flops_elem,
         },
         "cpu_baseline": cpu,
+        "mass_matrix": {"ms": mass_ms, "elements_per_s": bp.nb_elem / (mass_ms * 1e-3),
+                        "roofline": {"bound": "hbm", "achieved": mass_bytes / (mass_ms * 1e-3) / 1e9,
+                                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": mass_bytes / (mass_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}},
     }
     if not args.only_assembly:
         out["c2_solve"] = bench_c2_solve(args.nel, p, args.jacobi_iters)

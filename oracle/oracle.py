@@ -4,10 +4,12 @@ Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
 only as the checker. Two independent CPU implementations are exposed:
 
 * ``oracle_sys_linmat`` -- liboracle.so, our C restatement (oracle/yeti_oracle.c);
-* ``oracle_reconstruction`` / ``oracle_evaldisp`` -- the same library's restatements of the
-  displacement scatter and of the displacement at parametric points;
+* ``oracle_reconstruction`` / ``oracle_evaldisp`` / ``oracle_mass_coo`` -- the same library's
+  restatements of the displacement scatter, the displacement at parametric points and the
+  consistent mass matrix;
 * ``ref_sys_linmat_omp`` / ``ref_sys_linmat_serial`` / ``ref_dersbasisfuns`` / ``ref_gauss`` /
-  ``ref_grad_linelast_work`` / ``ref_reconstruction`` / ``ref_evaldisp`` -- oracle/_ref/libyetiref.so, YETI's own Fortran compiled from
+  ``ref_grad_linelast_work`` / ``ref_reconstruction`` / ``ref_evaldisp`` / ``ref_build_cmassmatrix``
+  -- oracle/_ref/libyetiref.so, YETI's own Fortran compiled from
   its sources by oracle/build_ref.sh (present only where the YETI sources were available at
   build time; callers must handle ``ref_available() == False``).
 """
@@ -51,6 +53,8 @@ def lib():
         _lib.oracle_element_matrix.argtypes = [C.c_void_p, C.c_int64, _f64p, _f64p]
         _lib.oracle_shape_at.restype = C.c_int
         _lib.oracle_shape_at.argtypes = [C.c_void_p, C.c_int64, _f64p, _f64p, _f64p, _f64p]
+        _lib.oracle_mass_coo.restype = C.c_int
+        _lib.oracle_mass_coo.argtypes = [C.c_void_p, _f64p, _i32p, _i32p]
         _lib.oracle_evaldisp.restype = C.c_int
         _lib.oracle_evaldisp.argtypes = [_i32p, _i32p, _f64p, _f64p, _f64p, C.c_int32, C.c_int32, _i32p, _i32p, _f64p,
                                          C.c_int32, _f64p, C.c_int64, _f64p, _f64p]
@@ -355,3 +359,42 @@ def ref_evaldisp(d, numpatch, xi, sol):
       matp.ctypes.data_as(_f64p), s.TENSOR, iv(s.len_TENSOR), s.PROPS, iv(max(1, s.len_PROPS)), s.JPROPS, s.NNODE,
       iv(s.nb_patch), iv(s.nb_elem), iv(s.nb_cp), iv(3))
     return out
+
+
+def oracle_mass_coo(args):
+    """liboracle's restatement of build_CMASSMatrix (compute_CMASSMatrix.f:22) on the system
+    inputs (dict or f2py positional list): (Mdata, Mrow, Mcol) in the packed layout."""
+    d = args_to_dict(args) if not isinstance(args, dict) else args
+    s, keep = pack_system_inputs(d)
+    n = int(s.nb_data)
+    Md = np.zeros(n)
+    Mr = np.zeros(n, np.int32)
+    Mc = np.zeros(n, np.int32)
+    rc = lib().oracle_mass_coo(C.byref(s), _p(Md, _f64p), _p(Mr, _i32p), _p(Mc, _i32p))
+    if rc:
+        raise ValueError("oracle_mass_coo failed (%d)" % rc)
+    return Md, Mr, Mc
+
+
+def ref_build_cmassmatrix(args):
+    """YETI's build_CMASSMatrix through oracle/ref_shim.f90 (every element active, as
+    get_inputs4massmat gives for solid patches)."""
+    d = args_to_dict(args) if not isinstance(args, dict) else args
+    s, keep = pack_system_inputs(d)
+    n = int(s.nb_data)
+    Md = np.zeros(n)
+    Mr = np.zeros(n, np.int32)
+    Mc = np.zeros(n, np.int32)
+    keep2 = []
+
+    def iv(v):
+        x = C.c_int32(int(v)); keep2.append(x); return C.byref(x)
+
+    f = ref().ref_build_cmassmatrix
+    f.restype = None
+    f(_p(Md, _f64p), _p(Mr, _i32p), _p(Mc, _i32p), s.activeElement, iv(n), s.COORDS3D, s.IEN, iv(s.len_IEN),
+      s.nb_elem_patch, s.Nkv, s.Ukv, iv(s.len_Ukv), s.Nijk, s.weight, iv(s.len_weight), s.Jpqr, s.ELT_TYPE,
+      iv(s.len_ELT_TYPE), s.PROPS, iv(max(1, s.len_PROPS)), s.JPROPS, s.RHO, s.TENSOR, iv(s.len_TENSOR),
+      s.ind_dof_free, iv(s.nb_dof_free), iv(s.MCRD), s.NBINT, iv(s.nb_patch), iv(s.nb_elem), s.NNODE, iv(s.nb_cp),
+      iv(s.nb_dof_tot))
+    return Md, Mr, Mc

@@ -214,3 +214,51 @@ subroutine ref_evaldisp(disp, NumPatch, Xi, SOL, COORDS3D, IEN, len_IEN, nb_elem
         &   Jpqr, elt_s, MATERIAL_PROPERTIES, tensor_s, PROPS, JPROPS, NNODE, nb_patch, nb_elem, nb_cp, &
         &   MCRD)
 end subroutine ref_evaldisp
+
+! build_CMASSMatrix (main/compute_CMASSMatrix.f:22) takes assumed-shape IEN, Ukv, weight, PROPS.
+subroutine ref_build_cmassmatrix(Mdata, Mrow, Mcol, activeElement, nb_data, COORDS3D, IEN, len_IEN,  &
+    &   nb_elem_patch, Nkv, Ukv, len_Ukv, Nijk, weight, len_weight, Jpqr, ELT_TYPE, len_elt, PROPS,   &
+    &   len_PROPS, JPROPS, RHO, TENSOR, len_tensor, ind_dof_free, nb_dof_free, MCRD, NBINT, nb_patch,  &
+    &   nb_elem, NNODE, nb_cp, nb_dof_tot) bind(C, name='ref_build_cmassmatrix')
+    use iso_c_binding
+    implicit none
+    integer(c_int), intent(in) :: nb_data, len_IEN, len_Ukv, len_weight, len_elt, len_PROPS,         &
+        &   len_tensor, nb_dof_free, MCRD, nb_patch, nb_elem, nb_cp, nb_dof_tot
+    integer(c_int), intent(in) :: activeElement(nb_elem), IEN(len_IEN), nb_elem_patch(nb_patch),      &
+        &   Nkv(3, nb_patch), Nijk(3, nb_elem), Jpqr(3, nb_patch), JPROPS(nb_patch),                  &
+        &   ind_dof_free(nb_dof_tot), NBINT(nb_patch), NNODE(nb_patch)
+    real(c_double), intent(in) :: COORDS3D(3, nb_cp), Ukv(len_Ukv), weight(len_weight),             &
+        &   PROPS(len_PROPS), RHO(nb_patch)
+    character(kind=c_char), intent(in) :: ELT_TYPE(len_elt), TENSOR(len_tensor)
+    real(c_double), intent(out) :: Mdata(nb_data)
+    integer(c_int), intent(out) :: Mrow(nb_data), Mcol(nb_data)
+    character(len=len_elt) :: elt_s
+    character(len=len_tensor) :: tensor_s
+    integer :: i
+
+    interface
+        subroutine build_CMASSMatrix(Mdata, Mrow, Mcol, activeElement, nb_data, COORDS3D, IEN,       &
+            &   nb_elem_patch, Nkv, Ukv, Nijk, weight, Jpqr, ELT_TYPE, PROPS, JPROPS, RHO, TENSOR,    &
+            &   ind_dof_free, nb_dof_free, MCRD, NBINT, nb_patch, nb_elem, nnode, nb_cp, nb_dof_tot)
+            integer, intent(in) :: nb_cp, MCRD, NNODE(nb_patch), nb_patch, NBINT(nb_patch), nb_elem,  &
+                &   nb_elem_patch(nb_patch), IEN(:), JPROPS(nb_patch), nb_dof_tot, nb_dof_free,         &
+                &   ind_dof_free(nb_dof_tot), nb_data, activeElement(nb_elem), Nkv(3, nb_patch),       &
+                &   Jpqr(3, nb_patch), Nijk(3, nb_elem)
+            double precision, intent(in) :: COORDS3D(3, nb_cp), Ukv(:), weight(:), RHO(nb_patch),    &
+                &   PROPS(:)
+            character(len=*), intent(in) :: TENSOR, ELT_TYPE
+            double precision, intent(out) :: Mdata(nb_data)
+            integer, intent(out) :: Mrow(nb_data), Mcol(nb_data)
+        end subroutine
+    end interface
+
+    do i = 1, len_elt
+        elt_s(i:i) = ELT_TYPE(i)
+    enddo
+    do i = 1, len_tensor
+        tensor_s(i:i) = TENSOR(i)
+    enddo
+    call build_CMASSMatrix(Mdata, Mrow, Mcol, activeElement, nb_data, COORDS3D, IEN, nb_elem_patch,   &
+        &   Nkv, Ukv, Nijk, weight, Jpqr, elt_s, PROPS, JPROPS, RHO, tensor_s, ind_dof_free,           &
+        &   nb_dof_free, MCRD, NBINT, nb_patch, nb_elem, NNODE, nb_cp, nb_dof_tot)
+end subroutine ref_build_cmassmatrix

@@ -694,3 +694,78 @@ int oracle_evaldisp(const int32_t *p, const int32_t *nkv, const double *U0, cons
     }
     return 0;
 }
+
+/* compute_CMASSMatrix.f:22-215 with UMASSMAT_byCP (UMASSMAT.f:220-330, solid branch): the
+ * consistent mass matrix in the stiffness COO layout, packed over the active elements. Per
+ * element, Gauss points in the reference's 3D order; dvol = rho w DetJac (signed, :311-314);
+ * for nodj, nodi <= nodj: temp1 = dvol R(nodj), temp2 = temp1 R(nodi), added to the three
+ * diagonal entries (:315-325); emission as the stiffness (:137-175), the diagonal block's
+ * diagonal halved. */
+int oracle_mass_coo(const yeti_system_inputs *in, double *Md, int32_t *Mr, int32_t *Mc)
+{
+    if (in->MCRD != 3) return YETI_ERR_INVALID_ARGUMENT;
+    memset(Md, 0, sizeof(double) * in->nb_data);
+    memset(Mr, 0, sizeof(int32_t) * in->nb_data);
+    memset(Mc, 0, sizeof(int32_t) * in->nb_data);
+    int64_t pos = 0;
+    for (int ip = 0; ip < in->nb_patch; ++ip) {
+        opatch P;
+        int rc = build_patch(in, ip, &P);
+        if (rc) return rc;
+        const int n = P.nnode, npair = n * (n + 1) / 2;
+        const int ng = gauss_per_dir(P.nbint);
+        double gx[MAXG], gw[MAXG];
+        if (oracle_gauss(ng, gx, gw)) return YETI_ERR_UNSUPPORTED_DEGREE;
+        double *CM = (double *)malloc(sizeof(double) * 9 * npair);
+        double *R = (double *)malloc(sizeof(double) * n);
+        double (*dRdx)[3] = (double (*)[3])malloc(sizeof(double) * 3 * n);
+        for (int e = 0; e < P.nelem; ++e) {
+            if (in->activeElement[P.elem0 + e] != 1) continue;
+            if (pos + 9LL * n * (n - 1) / 2 + 6LL * n > in->nb_data) { free(CM); free(R); free(dRdx); return YETI_ERR_SIZE_OVERFLOW; }
+            memset(CM, 0, sizeof(double) * 9 * npair);
+            for (int gi = 0; gi < ng; ++gi)
+                for (int gj = 0; gj < ng; ++gj)
+                    for (int gk = 0; gk < ng; ++gk) {
+                        double pt[3] = {gx[gi], gx[gj], gx[gk]};
+                        double AJ[3][3], detJ;
+                        oracle_shap(&P, e, in->COORDS3D, pt, R, dRdx, AJ, &detJ);
+                        const double dvol = P.rho * (gw[gi] * gw[gj] * gw[gk] * detJ);
+                        int cnt = 0;
+                        for (int b = 0; b < n; ++b) {
+                            const double temp1 = dvol * R[b];
+                            for (int a = 0; a <= b; ++a, ++cnt) {
+                                const double temp2 = temp1 * R[a];
+                                for (int k = 0; k < 3; ++k) CM[9 * cnt + k + 3 * k] += temp2;
+                            }
+                        }
+                    }
+            const int32_t *sctr = P.ien + (int64_t)n * e;
+            int pi = 0;
+            for (int b = 0; b < n; ++b) {
+                const int32_t dofj = (sctr[b] - 1) * 3;
+                for (int a = 0; a < b; ++a, ++pi) {
+                    const int32_t dofi = (sctr[a] - 1) * 3;
+                    for (int ll = 0; ll < 3; ++ll)
+                        for (int kk = 0; kk < 3; ++kk, ++pos) {
+                            Md[pos] = CM[9 * pi + kk + 3 * ll];
+                            Mr[pos] = dofi + kk;
+                            Mc[pos] = dofj + ll;
+                        }
+                }
+                for (int ll = 0; ll < 3; ++ll) {
+                    CM[9 * pi + ll + 3 * ll] *= 0.5;
+                    for (int kk = 0; kk <= ll; ++kk, ++pos) {
+                        Md[pos] = CM[9 * pi + kk + 3 * ll];
+                        Mr[pos] = dofj + kk;
+                        Mc[pos] = dofj + ll;
+                    }
+                }
+                ++pi;
+            }
+        }
+        free(CM);
+        free(R);
+        free(dRdx);
+    }
+    return 0;
+}

@@ -124,6 +124,20 @@ class Assembler:
         _lib.check(_lib.lib().yeti_asm_run_rhs(self._h, C.c_void_p(F.data_ptr()), _stream_handle(stream)))
         return F
 
+    def run_mass(self, M=None, r=None, c=None, with_indices=True, stream=None):
+        """Consistent mass matrix (compute_CMASSMatrix.f) in the stiffness COO layout:
+        (Mdata, Mrow, Mcol) device tensors (Mrow/Mcol None when with_indices is False)."""
+        if M is None:
+            M, r0, c0 = self.alloc_coo(with_indices)
+            r = r if r is not None else r0
+            c = c if c is not None else c0
+        self._check_out(M, r, c, None)
+        _lib.check(_lib.lib().yeti_asm_run_mass_coo(self._h, C.c_void_p(M.data_ptr()),
+                                                    C.c_void_p(r.data_ptr()) if r is not None else None,
+                                                    C.c_void_p(c.data_ptr()) if c is not None else None,
+                                                    _stream_handle(stream)))
+        return M, r, c
+
     def _check_out(self, K, r, c, F):
         if K.dtype != torch.float64 or not K.is_cuda or not K.is_contiguous() or K.numel() < self.nb_data:
             raise ValueError("Kdata must be a contiguous CUDA float64 tensor with >= %d entries" % self.nb_data)

@@ -740,6 +740,219 @@ static int run_elements(yeti_asm *h, double *Kd, int32_t *Kr, int32_t *Kc, hipSt
     return 0;
 }
 
+// ---------------------------------------------------------------------------------------
+// Consistent mass matrix (compute_CMASSMatrix.f:22-215 with UMASSMAT_byCP, UMASSMAT.f:220-330,
+// solid branch): M_ab[k][k] = sum_q rho w_q DetJac_q R_a(q) R_b(q) (DetJac signed, parent ->
+// physical), off-diagonal k != l entries 0, emitted in the stiffness COO layout (a < b: 3 x 3
+// block, kk fastest; a = b: the upper 6 with the diagonal halved) at the same element offsets.
+// One workgroup per element (grid-stride over the active elements): Gauss points in chunks of
+// MQC, their geometry by one thread each, R_a(q) for the chunk in LDS, then every thread adds
+// the chunk to its node pairs (pair t, t + 256, ... in COO order, so consecutive threads write
+// consecutive 72-byte blocks at the end).
+constexpr int MQC = 32;
+
+template <int P, int NG>
+struct MassCfg {
+    static constexpr int NP1 = P + 1, NN = NP1 * NP1 * NP1, NQ = NG * NG * NG;
+    static constexpr int NPAIR = NN * (NN + 1) / 2;
+    static constexpr int PPT = (NPAIR + 255) / 256;   // pairs per thread
+};
+
+template <int P, int NG, bool WRITE_IDX>
+__global__ void __launch_bounds__(256) mass_kernel(PatchArgs pa, double rho, double *__restrict__ Md,
+                                                   int32_t *__restrict__ Mr, int32_t *__restrict__ Mc) {
+    using C = MassCfg<P, NG>;
+    constexpr int NP1 = C::NP1, NN = C::NN, NQ = C::NQ;
+    __shared__ double xw[NN][4];            // control point (x, y, z) and weight
+    __shared__ double Bq[MQC][3][NP1];      // 1D bases at the chunk's points (local index reversed)
+    __shared__ double dBq[MQC][3][NP1];     // and their derivatives
+    __shared__ double dvq[MQC], wiq[MQC];   // rho w DetJac, 1 / W
+    __shared__ double Rq[NN][MQC + 1];
+    __shared__ int32_t dof[NN];
+    const int t = threadIdx.x;
+    for (int64_t gi = blockIdx.x; gi < pa.n_active; gi += gridDim.x) {
+        const int e = pa.elems[gi];
+        __syncthreads();   // the previous element is done with the LDS arrays
+        for (int a = t; a < NN; a += 256) {
+            const int32_t cp = pa.ien[(int64_t)NN * e + a] - 1;
+            xw[a][0] = pa.coords[3 * (int64_t)cp];
+            xw[a][1] = pa.coords[3 * (int64_t)cp + 1];
+            xw[a][2] = pa.coords[3 * (int64_t)cp + 2];
+            xw[a][3] = pa.weight[(int64_t)NN * e + a];
+            dof[a] = 3 * cp;
+        }
+        double m[C::PPT];
+#pragma unroll
+        for (int k = 0; k < C::PPT; ++k) m[k] = 0.0;
+        for (int q0 = 0; q0 < NQ; q0 += MQC) {
+            __syncthreads();
+            if (t < MQC && q0 + t < NQ) {   // geometry of point q (shap.f90:75-178)
+                const int q = q0 + t;
+                // the reference's 3D rule: i (xi) slowest, k (zeta) fastest (GaussLegendre.f90:175-186)
+                const int gq[3] = {q / (NG * NG), (q / NG) % NG, q % NG};
+                double half[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double *U = d == 0 ? pa.U0 : d == 1 ? pa.U1 : pa.U2;
+                    const int span1 = pa.nijk[3 * (int64_t)e + d];
+                    const double u0 = U[span1 - 1], u1 = U[span1];
+                    const double u = ((u1 - u0) * pa.g.x[gq[d]] + (u1 + u0)) * 0.5;
+                    double N[NP1], dN[NP1];
+                    bspline_ders<P>(span1 - 1, u, U, N, dN);
+#pragma unroll
+                    for (int i = 0; i < NP1; ++i) {
+                        Bq[t][d][i] = N[P - i];
+                        dBq[t][d][i] = dN[P - i];
+                    }
+                    half[d] = 0.5 * (u1 - u0);
+                }
+                double s[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) s[k] = 0.0;
+                // the tables are read back by this thread only
+#pragma unroll 1
+                for (int k = 0; k < NP1; ++k)
+#pragma unroll 1
+                    for (int j = 0; j < NP1; ++j)
+#pragma unroll
+                        for (int i = 0; i < NP1; ++i) {
+                            const int a = i + NP1 * (j + NP1 * k);
+                            const double w = xw[a][3], x0 = xw[a][0], x1 = xw[a][1], x2 = xw[a][2];
+                            const double B0 = Bq[t][0][i], B1 = Bq[t][1][j], B2 = Bq[t][2][k];
+                            const double b12 = B1 * B2 * w;
+                            const double Na = B0 * b12;
+                            const double d0 = dBq[t][0][i] * b12;
+                            const double d1 = B0 * dBq[t][1][j] * B2 * w;
+                            const double d2 = B0 * B1 * dBq[t][2][k] * w;
+                            s[0] += Na;
+                            s[1] += d0; s[2] += d1; s[3] += d2;
+                            s[4] += x0 * Na; s[5] += x1 * Na; s[6] += x2 * Na;
+                            s[7] += x0 * d0; s[8] += x0 * d1; s[9] += x0 * d2;
+                            s[10] += x1 * d0; s[11] += x1 * d1; s[12] += x1 * d2;
+                            s[13] += x2 * d0; s[14] += x2 * d1; s[15] += x2 * d2;
+                        }
+                const double Wi = 1.0 / s[0];
+                const double dw[3] = {s[1] * Wi, s[2] * Wi, s[3] * Wi};
+                double J[3][3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) J[r][c] = (s[7 + 3 * r + c] - s[4 + r] * dw[c]) * Wi;
+                const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) +
+                                   J[0][1] * (J[1][2] * J[2][0] - J[1][0] * J[2][2]) +
+                                   J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+                dvq[t] = rho * (pa.g.w[gq[0]] * pa.g.w[gq[1]] * pa.g.w[gq[2]]) * (det * half[0] * half[1] * half[2]);
+                wiq[t] = Wi;
+            }
+            __syncthreads();
+            const int nq = NQ - q0 < MQC ? NQ - q0 : MQC;
+            for (int x = t; x < NN * MQC; x += 256) {   // R_a(q) = N_a w_a / W
+                const int a = x / MQC, qq = x % MQC;
+                if (qq < nq) {
+                    const int i = a % NP1, j = (a / NP1) % NP1, k = a / (NP1 * NP1);
+                    Rq[a][qq] = Bq[qq][0][i] * Bq[qq][1][j] * Bq[qq][2][k] * xw[a][3] * wiq[qq];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < C::PPT; ++k) {   // (a loop over k with m[k] in registers)
+                const int pi = t + 256 * k;
+                if (pi < C::NPAIR) {
+                    int b = (int)((sqrt(8.0 * pi + 1.0) - 1.0) * 0.5);   // pi = b (b + 1) / 2 + a
+                    while (b * (b + 1) / 2 > pi) --b;
+                    while ((b + 1) * (b + 2) / 2 <= pi) ++b;
+                    const int a = pi - b * (b + 1) / 2;
+                    double acc = m[k];
+#pragma unroll 4
+                    for (int qq = 0; qq < nq; ++qq) acc += dvq[qq] * Rq[b][qq] * Rq[a][qq];
+                    m[k] = acc;
+                }
+            }
+        }
+        // emission at the element's COO offset
+        const int64_t base = pa.offs[gi];
+#pragma unroll
+        for (int k = 0; k < C::PPT; ++k) {
+            const int pi = t + 256 * k;
+            if (pi >= C::NPAIR) continue;
+            int b = (int)((sqrt(8.0 * pi + 1.0) - 1.0) * 0.5);
+            while (b * (b + 1) / 2 > pi) --b;
+            while ((b + 1) * (b + 2) / 2 <= pi) ++b;
+            const int a = pi - b * (b + 1) / 2;
+            const int64_t o = base + coo_start(b) + (a < b ? 9 * a : 9 * b);
+            if (a < b) {
+#pragma unroll
+                for (int x = 0; x < 9; ++x) {
+                    const int kk = x % 3, ll = x / 3;
+                    Md[o + x] = kk == ll ? m[k] : 0.0;
+                    if (WRITE_IDX) {
+                        Mr[o + x] = dof[a] + kk;
+                        Mc[o + x] = dof[b] + ll;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int x = 0; x < 6; ++x) {   // (0,0) (0,1) (1,1) (0,2) (1,2) (2,2)
+                    const int kk = (0x210100 >> (4 * x)) & 15, ll = (0x222110 >> (4 * x)) & 15;
+                    Md[o + x] = kk == ll ? 0.5 * m[k] : 0.0;
+                    if (WRITE_IDX) {
+                        Mr[o + x] = dof[b] + kk;
+                        Mc[o + x] = dof[b] + ll;
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int P, int NG>
+static int launch_mass(yeti_asm *h, PatchDev &pd, double *Md, int32_t *Mr, int32_t *Mc, hipStream_t st) {
+    PatchArgs pa{};
+    pa.coords = h->coords.p;
+    pa.ien = pd.ien.p;
+    pa.weight = pd.weight.p;
+    pa.nijk = pd.nijk.p;
+    pa.U0 = pd.U[0].p;
+    pa.U1 = pd.U[1].p;
+    pa.U2 = pd.U[2].p;
+    pa.elems = pd.elems.p;
+    pa.offs = pd.offs.p;
+    pa.n_active = pd.n_active;
+    gauss_legendre(NG, pa.g.x, pa.g.w);
+    const unsigned grid = (unsigned)std::min<int64_t>(pd.n_active, 8 * (int64_t)device_cu_count());
+    if (Mr)
+        hipLaunchKernelGGL((mass_kernel<P, NG, true>), dim3(grid), dim3(256), 0, st, pa, pd.rho, Md, Mr, Mc);
+    else
+        hipLaunchKernelGGL((mass_kernel<P, NG, false>), dim3(grid), dim3(256), 0, st, pa, pd.rho, Md, Mr, Mc);
+    YETI_HIP(hipGetLastError());
+    return 0;
+}
+
+static int run_mass(yeti_asm *h, double *Md, int32_t *Mr, int32_t *Mc, hipStream_t st) {
+    for (auto &zr : h->zero_ranges) {
+        int64_t n = zr.second;
+        int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks), dim3(256), 0, st, Md + zr.first,
+                           Mr ? Mr + zr.first : nullptr, Mc ? Mc + zr.first : nullptr, n);
+        YETI_HIP(hipGetLastError());
+    }
+    for (auto *pd : h->patches) {
+        if (!pd->u1 || pd->n_active == 0) continue;
+        if (pd->p[1] != pd->p[0] || pd->p[2] != pd->p[0])
+            return set_error(YETI_ERR_UNSUPPORTED_DEGREE, "mass: degrees must be equal per direction");
+        int rc = 0;
+        switch (pd->p[0] * 16 + pd->ng) {
+        case 1 * 16 + 2: rc = launch_mass<1, 2>(h, *pd, Md, Mr, Mc, st); break;
+        case 2 * 16 + 3: rc = launch_mass<2, 3>(h, *pd, Md, Mr, Mc, st); break;
+        case 3 * 16 + 4: rc = launch_mass<3, 4>(h, *pd, Md, Mr, Mc, st); break;
+        case 4 * 16 + 5: rc = launch_mass<4, 5>(h, *pd, Md, Mr, Mc, st); break;
+        default: return set_error(YETI_ERR_UNSUPPORTED_DEGREE, "mass: no kernel for p=%d ng=%d", pd->p[0], pd->ng);
+        }
+        if (rc) return rc;
+    }
+    return 0;
+}
+
 // u^T F shape derivative of the face loads: g = 2 u^T dF/dX (3 per CP), gathered like F.
 static int run_load_grad(yeti_asm *h, const double *u, double *g, hipStream_t st) {
     if (h->body_loads)
@@ -1096,6 +1309,11 @@ int32_t yeti_asm_run_coo_matrix(yeti_asm *h, double *Kd, int32_t *Kr, int32_t *K
 int32_t yeti_asm_run_coo_values(yeti_asm *h, double *Kd, void *stream) {
     if (!h || !Kd) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
     return yeti::run_elements(h, Kd, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int32_t yeti_asm_run_mass_coo(yeti_asm *h, double *Md, int32_t *Mr, int32_t *Mc, void *stream) {
+    if (!h || !Md || (!Mr) != (!Mc)) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    return yeti::run_mass(h, Md, Mr, Mc, (hipStream_t)stream);
 }
 
 int32_t yeti_asm_run_rhs(yeti_asm *h, double *F, void *stream) {
