@@ -745,31 +745,61 @@ static int run_elements(yeti_asm *h, double *Kd, int32_t *Kr, int32_t *Kc, hipSt
 // solid branch): M_ab[k][k] = sum_q rho w_q DetJac_q R_a(q) R_b(q) (DetJac signed, parent ->
 // physical), off-diagonal k != l entries 0, emitted in the stiffness COO layout (a < b: 3 x 3
 // block, kk fastest; a = b: the upper 6 with the diagonal halved) at the same element offsets.
-// One workgroup per element (grid-stride over the active elements): Gauss points in chunks of
-// MQC, their geometry by one thread each, R_a(q) for the chunk in LDS, then every thread adds
-// the chunk to its node pairs (pair t, t + 256, ... in COO order, so consecutive threads write
-// consecutive 72-byte blocks at the end).
-constexpr int MQC = 32;
-
+// One 256-thread workgroup per element (grid-stride over the active elements), Gauss points in
+// chunks of MQC:
+//   1. the 1D bases of the chunk's points, one thread per (point, direction);
+//   2. W, dW and the Jacobian of each point from G = 256 / MQC threads that split its nodes and
+//      add their sums with lane shuffles;
+//   3. R_a(q) = N_a w_a / W for the chunk in LDS (node-major, points contiguous);
+//   4. the node-pair sums as 16 x 16 tiles (a tile, b tile) of (dv R)^T R on
+//      v_mfma_f64_16x16x4f64, upper tiles spread over the four waves, accumulated over chunks;
+// then the upper triangle of M goes to LDS (packed by column, b (b + 1) / 2 + a) and all threads
+// write the element's COO range front to back, so every store of a wave is one contiguous run.
 template <int P, int NG>
 struct MassCfg {
     static constexpr int NP1 = P + 1, NN = NP1 * NP1 * NP1, NQ = NG * NG * NG;
+    static constexpr int NNP = (NN + 15) / 16 * 16, NT = NNP / 16, NTILE = NT * (NT + 1) / 2;
+    static constexpr int MAXT = (NTILE + 3) / 4;                       // tiles per wave
+    static constexpr int MQC = NQ <= 64 ? (NQ + 7) / 8 * 8 : 32;       // points per chunk
+    static constexpr int G = 256 / MQC;                                // threads per point
+    static constexpr int RS = MQC + 2;                                 // R row stride (doubles)
     static constexpr int NPAIR = NN * (NN + 1) / 2;
-    static constexpr int PPT = (NPAIR + 255) / 256;   // pairs per thread
+    // LDS (doubles): xw, dvq, then R + bases, which the packed M reuses after the last chunk
+    static constexpr int O_DV = 4 * NN, O_U = O_DV + MQC;
+    static constexpr int O_B = O_U + NNP * RS, O_DB = O_B + MQC * 3 * NP1;
+    static constexpr int N_U = NNP * RS + 2 * MQC * 3 * NP1;
+    static constexpr int NDBL = O_U + (N_U > NPAIR ? N_U : NPAIR);
+    static constexpr int64_t NCOO = coo_start(NN);
+    static_assert(256 % MQC == 0 && MQC % 8 == 0 && G <= 64, "mass chunking");
 };
 
 template <int P, int NG, bool WRITE_IDX>
 __global__ void __launch_bounds__(256) mass_kernel(PatchArgs pa, double rho, double *__restrict__ Md,
                                                    int32_t *__restrict__ Mr, int32_t *__restrict__ Mc) {
     using C = MassCfg<P, NG>;
-    constexpr int NP1 = C::NP1, NN = C::NN, NQ = C::NQ;
-    __shared__ double xw[NN][4];            // control point (x, y, z) and weight
-    __shared__ double Bq[MQC][3][NP1];      // 1D bases at the chunk's points (local index reversed)
-    __shared__ double dBq[MQC][3][NP1];     // and their derivatives
-    __shared__ double dvq[MQC], wiq[MQC];   // rho w DetJac, 1 / W
-    __shared__ double Rq[NN][MQC + 1];
+    constexpr int NP1 = C::NP1, NN = C::NN, NQ = C::NQ, MQC = C::MQC, G = C::G, RS = C::RS;
+    __shared__ __attribute__((aligned(16))) double sm[C::NDBL];
+    __shared__ double hel[3], wiq[MQC];
     __shared__ int32_t dof[NN];
-    const int t = threadIdx.x;
+    double(*xw)[4] = reinterpret_cast<double(*)[4]>(sm);
+    double *dvq = sm + C::O_DV;
+    double *Rq = sm + C::O_U;                                              // [NNP][RS]
+    double(*Bq)[3][NP1] = reinterpret_cast<double(*)[3][NP1]>(sm + C::O_B);
+    double(*dBq)[3][NP1] = reinterpret_cast<double(*)[3][NP1]>(sm + C::O_DB);
+    double *mp = sm + C::O_U;                                              // packed upper M
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int nl = lane & 15, kg = lane >> 4;
+    const int qq = t / G, part = t % G;
+    // this wave's tiles u = wv + 4 s, numbered column by column: u = tb (tb + 1) / 2 + ta
+    int ta[C::MAXT], tb[C::MAXT];
+#pragma unroll
+    for (int s = 0; s < C::MAXT; ++s) {
+        const int u = wv + 4 * s;
+        int j = 0;
+        while ((j + 1) * (j + 2) / 2 <= u) ++j;
+        tb[s] = j;
+        ta[s] = u - j * (j + 1) / 2;
+    }
     for (int64_t gi = blockIdx.x; gi < pa.n_active; gi += gridDim.x) {
         const int e = pa.elems[gi];
         __syncthreads();   // the previous element is done with the LDS arrays
@@ -781,125 +811,151 @@ __global__ void __launch_bounds__(256) mass_kernel(PatchArgs pa, double rho, dou
             xw[a][3] = pa.weight[(int64_t)NN * e + a];
             dof[a] = 3 * cp;
         }
-        double m[C::PPT];
+        f64x4 acc[C::MAXT];
 #pragma unroll
-        for (int k = 0; k < C::PPT; ++k) m[k] = 0.0;
+        for (int s = 0; s < C::MAXT; ++s) acc[s] = f64x4{0.0, 0.0, 0.0, 0.0};
         for (int q0 = 0; q0 < NQ; q0 += MQC) {
+            const int nq = NQ - q0 < MQC ? NQ - q0 : MQC;
             __syncthreads();
-            if (t < MQC && q0 + t < NQ) {   // geometry of point q (shap.f90:75-178)
-                const int q = q0 + t;
-                // the reference's 3D rule: i (xi) slowest, k (zeta) fastest (GaussLegendre.f90:175-186)
-                const int gq[3] = {q / (NG * NG), (q / NG) % NG, q % NG};
-                double half[3];
+            // 1. bases: direction `part` of point qq (shap.f90:75-178); the reference's 3D rule
+            // runs i (xi) slowest and k (zeta) fastest (GaussLegendre.f90:175-186)
+            if (part < 3 && qq < nq) {
+                const int q = q0 + qq, d = part;
+                const int gd = d == 0 ? q / (NG * NG) : d == 1 ? (q / NG) % NG : q % NG;
+                const double *U = d == 0 ? pa.U0 : d == 1 ? pa.U1 : pa.U2;
+                const int span1 = pa.nijk[3 * (int64_t)e + d];
+                const double u0 = U[span1 - 1], u1 = U[span1];
+                const double u = ((u1 - u0) * pa.g.x[gd] + (u1 + u0)) * 0.5;
+                double N[NP1], dN[NP1];
+                bspline_ders<P>(span1 - 1, u, U, N, dN);
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    const double *U = d == 0 ? pa.U0 : d == 1 ? pa.U1 : pa.U2;
-                    const int span1 = pa.nijk[3 * (int64_t)e + d];
-                    const double u0 = U[span1 - 1], u1 = U[span1];
-                    const double u = ((u1 - u0) * pa.g.x[gq[d]] + (u1 + u0)) * 0.5;
-                    double N[NP1], dN[NP1];
-                    bspline_ders<P>(span1 - 1, u, U, N, dN);
-#pragma unroll
-                    for (int i = 0; i < NP1; ++i) {
-                        Bq[t][d][i] = N[P - i];
-                        dBq[t][d][i] = dN[P - i];
-                    }
-                    half[d] = 0.5 * (u1 - u0);
+                for (int i = 0; i < NP1; ++i) {
+                    Bq[qq][d][i] = N[P - i];
+                    dBq[qq][d][i] = dN[P - i];
                 }
+                if (qq == 0) hel[d] = 0.5 * (u1 - u0);
+            }
+            __syncthreads();
+            // 2. W, dW/dxi and dX/dxi of point qq: nodes part, part + G, ... then a shuffle sum
+            {
                 double s[16];
 #pragma unroll
                 for (int k = 0; k < 16; ++k) s[k] = 0.0;
-                // the tables are read back by this thread only
-#pragma unroll 1
-                for (int k = 0; k < NP1; ++k)
-#pragma unroll 1
-                    for (int j = 0; j < NP1; ++j)
-#pragma unroll
-                        for (int i = 0; i < NP1; ++i) {
-                            const int a = i + NP1 * (j + NP1 * k);
-                            const double w = xw[a][3], x0 = xw[a][0], x1 = xw[a][1], x2 = xw[a][2];
-                            const double B0 = Bq[t][0][i], B1 = Bq[t][1][j], B2 = Bq[t][2][k];
-                            const double b12 = B1 * B2 * w;
-                            const double Na = B0 * b12;
-                            const double d0 = dBq[t][0][i] * b12;
-                            const double d1 = B0 * dBq[t][1][j] * B2 * w;
-                            const double d2 = B0 * B1 * dBq[t][2][k] * w;
-                            s[0] += Na;
-                            s[1] += d0; s[2] += d1; s[3] += d2;
-                            s[4] += x0 * Na; s[5] += x1 * Na; s[6] += x2 * Na;
-                            s[7] += x0 * d0; s[8] += x0 * d1; s[9] += x0 * d2;
-                            s[10] += x1 * d0; s[11] += x1 * d1; s[12] += x1 * d2;
-                            s[13] += x2 * d0; s[14] += x2 * d1; s[15] += x2 * d2;
-                        }
-                const double Wi = 1.0 / s[0];
-                const double dw[3] = {s[1] * Wi, s[2] * Wi, s[3] * Wi};
-                double J[3][3];
-#pragma unroll
-                for (int r = 0; r < 3; ++r)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) J[r][c] = (s[7 + 3 * r + c] - s[4 + r] * dw[c]) * Wi;
-                const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) +
-                                   J[0][1] * (J[1][2] * J[2][0] - J[1][0] * J[2][2]) +
-                                   J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-                dvq[t] = rho * (pa.g.w[gq[0]] * pa.g.w[gq[1]] * pa.g.w[gq[2]]) * (det * half[0] * half[1] * half[2]);
-                wiq[t] = Wi;
-            }
-            __syncthreads();
-            const int nq = NQ - q0 < MQC ? NQ - q0 : MQC;
-            for (int x = t; x < NN * MQC; x += 256) {   // R_a(q) = N_a w_a / W
-                const int a = x / MQC, qq = x % MQC;
                 if (qq < nq) {
-                    const int i = a % NP1, j = (a / NP1) % NP1, k = a / (NP1 * NP1);
-                    Rq[a][qq] = Bq[qq][0][i] * Bq[qq][1][j] * Bq[qq][2][k] * xw[a][3] * wiq[qq];
+#pragma unroll 1
+                    for (int a = part; a < NN; a += G) {
+                        const int i = a % NP1, j = (a / NP1) % NP1, k = a / (NP1 * NP1);
+                        const double w = xw[a][3], x0 = xw[a][0], x1 = xw[a][1], x2 = xw[a][2];
+                        const double B0 = Bq[qq][0][i], B1 = Bq[qq][1][j], B2 = Bq[qq][2][k];
+                        const double b12 = B1 * B2 * w;
+                        const double Na = B0 * b12;
+                        const double d0 = dBq[qq][0][i] * b12;
+                        const double d1 = B0 * dBq[qq][1][j] * B2 * w;
+                        const double d2 = B0 * B1 * dBq[qq][2][k] * w;
+                        s[0] += Na;
+                        s[1] += d0; s[2] += d1; s[3] += d2;
+                        s[4] += x0 * Na; s[5] += x1 * Na; s[6] += x2 * Na;
+                        s[7] += x0 * d0; s[8] += x0 * d1; s[9] += x0 * d2;
+                        s[10] += x1 * d0; s[11] += x1 * d1; s[12] += x1 * d2;
+                        s[13] += x2 * d0; s[14] += x2 * d1; s[15] += x2 * d2;
+                    }
+                }
+#pragma unroll
+                for (int o = 1; o < G; o <<= 1)   // the G threads of a point are adjacent lanes
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) s[k] += __shfl_xor(s[k], o, 64);
+                if (part == 0) {
+                    double dv = 0.0, Wi = 0.0;
+                    if (qq < nq) {
+                        const int q = q0 + qq;
+                        Wi = 1.0 / s[0];
+                        const double dw[3] = {s[1] * Wi, s[2] * Wi, s[3] * Wi};
+                        double J[3][3];
+#pragma unroll
+                        for (int r = 0; r < 3; ++r)
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) J[r][c] = (s[7 + 3 * r + c] - s[4 + r] * dw[c]) * Wi;
+                        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) +
+                                           J[0][1] * (J[1][2] * J[2][0] - J[1][0] * J[2][2]) +
+                                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+                        const int g0 = q / (NG * NG), g1 = (q / NG) % NG, g2 = q % NG;
+                        dv = rho * (pa.g.w[g0] * pa.g.w[g1] * pa.g.w[g2]) * (det * hel[0] * hel[1] * hel[2]);
+                    }
+                    dvq[qq] = dv;   // 0 at the padding points of the last chunk
+                    wiq[qq] = Wi;
                 }
             }
             __syncthreads();
+            // 3. R_a(q) = N_a w_a / W; zero rows past NN and columns past nq
+            for (int x = t; x < C::NNP * MQC; x += 256) {
+                const int a = x / MQC, c = x % MQC;
+                double r = 0.0;
+                if (a < NN && c < nq) {
+                    const int i = a % NP1, j = (a / NP1) % NP1, k = a / (NP1 * NP1);
+                    r = Bq[c][0][i] * Bq[c][1][j] * Bq[c][2][k] * xw[a][3] * wiq[c];
+                }
+                Rq[a * RS + c] = r;
+            }
+            __syncthreads();
+            // 4. tile sums; K step 2v uses point 8v + 2 kg, step 2v + 1 point 8v + 2 kg + 1 (A and
+            // B with the same permutation of the K sum)
+#pragma unroll 1
+            for (int v = 0; v < MQC / 8; ++v) {
+                const int qo = 8 * v + 2 * kg;
+                const f64x2 dv2 = *reinterpret_cast<const f64x2 *>(dvq + qo);
 #pragma unroll
-            for (int k = 0; k < C::PPT; ++k) {   // (a loop over k with m[k] in registers)
-                const int pi = t + 256 * k;
-                if (pi < C::NPAIR) {
-                    int b = (int)((sqrt(8.0 * pi + 1.0) - 1.0) * 0.5);   // pi = b (b + 1) / 2 + a
-                    while (b * (b + 1) / 2 > pi) --b;
-                    while ((b + 1) * (b + 2) / 2 <= pi) ++b;
-                    const int a = pi - b * (b + 1) / 2;
-                    double acc = m[k];
-#pragma unroll 4
-                    for (int qq = 0; qq < nq; ++qq) acc += dvq[qq] * Rq[b][qq] * Rq[a][qq];
-                    m[k] = acc;
+                for (int s = 0; s < C::MAXT; ++s) {
+                    if (wv + 4 * s < C::NTILE) {
+                        f64x2 A = *reinterpret_cast<const f64x2 *>(Rq + (16 * ta[s] + nl) * RS + qo);
+                        const f64x2 B = *reinterpret_cast<const f64x2 *>(Rq + (16 * tb[s] + nl) * RS + qo);
+                        A.x *= dv2.x;
+                        A.y *= dv2.y;
+                        acc[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(A.x, B.x, acc[s], 0, 0, 0);
+                        acc[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(A.y, B.y, acc[s], 0, 0, 0);
+                    }
                 }
             }
         }
-        // emission at the element's COO offset
+        __syncthreads();   // every wave is done with R before M overwrites it
+        // accumulator lane layout: register r of lane l holds row (l >> 4) + 4 r, column l & 15
+#pragma unroll
+        for (int s = 0; s < C::MAXT; ++s) {
+            if (wv + 4 * s < C::NTILE) {
+                const int b = 16 * tb[s] + nl;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int a = 16 * ta[s] + kg + 4 * r;
+                    if (a <= b && b < NN) mp[b * (b + 1) / 2 + a] = acc[s][r];
+                }
+            }
+        }
+        __syncthreads();
+        // emission: entry x of the element's range lies in column b with start(b) <= x < start(b+1)
         const int64_t base = pa.offs[gi];
-#pragma unroll
-        for (int k = 0; k < C::PPT; ++k) {
-            const int pi = t + 256 * k;
-            if (pi >= C::NPAIR) continue;
-            int b = (int)((sqrt(8.0 * pi + 1.0) - 1.0) * 0.5);
-            while (b * (b + 1) / 2 > pi) --b;
-            while ((b + 1) * (b + 2) / 2 <= pi) ++b;
-            const int a = pi - b * (b + 1) / 2;
-            const int64_t o = base + coo_start(b) + (a < b ? 9 * a : 9 * b);
-            if (a < b) {
-#pragma unroll
-                for (int x = 0; x < 9; ++x) {
-                    const int kk = x % 3, ll = x / 3;
-                    Md[o + x] = kk == ll ? m[k] : 0.0;
-                    if (WRITE_IDX) {
-                        Mr[o + x] = dof[a] + kk;
-                        Mc[o + x] = dof[b] + ll;
-                    }
-                }
+        int b = 0;
+        for (int x = t; x < (int)C::NCOO; x += 256) {
+            while (b + 1 < NN && (int)coo_start(b + 1) <= x) ++b;
+            const unsigned r = (unsigned)(x - (int)coo_start(b));
+            const int mb = b * (b + 1) / 2;
+            double v;
+            int32_t row, col;
+            if (r < 9u * (unsigned)b) {
+                const unsigned a = r / 9u, y = r - 9u * a, kk = y % 3u, ll = y / 3u;
+                v = kk == ll ? mp[mb + (int)a] : 0.0;
+                row = dof[a] + (int)kk;
+                col = dof[b] + (int)ll;
             } else {
-#pragma unroll
-                for (int x = 0; x < 6; ++x) {   // (0,0) (0,1) (1,1) (0,2) (1,2) (2,2)
-                    const int kk = (0x210100 >> (4 * x)) & 15, ll = (0x222110 >> (4 * x)) & 15;
-                    Md[o + x] = kk == ll ? 0.5 * m[k] : 0.0;
-                    if (WRITE_IDX) {
-                        Mr[o + x] = dof[b] + kk;
-                        Mc[o + x] = dof[b] + ll;
-                    }
-                }
+                const unsigned y = r - 9u * (unsigned)b;   // (0,0) (0,1) (1,1) (0,2) (1,2) (2,2)
+                const int kk = (0x210100 >> (4 * y)) & 15, ll = (0x222110 >> (4 * y)) & 15;
+                v = kk == ll ? 0.5 * mp[mb + b] : 0.0;
+                row = dof[b] + kk;
+                col = dof[b] + ll;
+            }
+            // streaming output, read back by no kernel of this launch
+            __builtin_nontemporal_store(v, Md + base + x);
+            if (WRITE_IDX) {
+                __builtin_nontemporal_store(row, Mr + base + x);
+                __builtin_nontemporal_store(col, Mc + base + x);
             }
         }
     }
@@ -919,7 +975,7 @@ static int launch_mass(yeti_asm *h, PatchDev &pd, double *Md, int32_t *Mr, int32
     pa.offs = pd.offs.p;
     pa.n_active = pd.n_active;
     gauss_legendre(NG, pa.g.x, pa.g.w);
-    const unsigned grid = (unsigned)std::min<int64_t>(pd.n_active, 8 * (int64_t)device_cu_count());
+    const unsigned grid = (unsigned)std::min<int64_t>(pd.n_active, 1 << 20);
     if (Mr)
         hipLaunchKernelGGL((mass_kernel<P, NG, true>), dim3(grid), dim3(256), 0, st, pa, pd.rho, Md, Mr, Mc);
     else
