@@ -349,6 +349,31 @@ def bench_mixed_precision(nel):
     out["evaldisp"] = {"points": pts.shape[0], "seconds_incl_transfers": t_ev,
                        "points_per_s": pts.shape[0] / t_ev}
     log("%d^3 p=2 evaldisp: %d points in %.3f s (host arrays in and out)" % (nel, pts.shape[0], t_ev))
+    # VTU output (postproc_vtu.f): vertex fields of every element on the GPU, then the native
+    # writer on a 16384-element slice
+    from yeti_amd.postproc import svars_solid
+    vflags, vref = (True, True, True), (1, 1, 1)
+    vargs = (solT, a["COORDS3D"], a["IEN"], a["nb_elem_patch"], a["Nkv"], a["Ukv"], a["Nijk"], a["weight"], a["Jpqr"],
+             a["NNODE"], a["MATERIAL_PROPERTIES"])
+    t0 = time.perf_counter()
+    sv = svars_solid(1, vref, vflags, *vargs)
+    t_sv = time.perf_counter() - t0
+    import tempfile
+    sl = np.ascontiguousarray(sv[:16384])
+    with tempfile.TemporaryDirectory() as td:
+        vpath = os.path.join(td, "slice.vtu")
+        t0 = time.perf_counter()
+        _lib.check(_lib.lib().yeti_vtu_write_piece(
+            vpath.encode(), sl.ctypes.data_as(C.c_void_p), sl.shape[0], np.array(vref, np.int32).ctypes.data_as(
+                C.c_void_p), np.array(vflags, np.int32).ctypes.data_as(C.c_void_p)))
+        t_wr = time.perf_counter() - t0
+        vbytes = os.path.getsize(vpath)
+    out["vtu"] = {"elements": sv.shape[0], "vertices": sv.shape[0] * sv.shape[1],
+                  "svars_seconds_incl_transfers": t_sv, "writer_elements": sl.shape[0], "writer_seconds": t_wr,
+                  "writer_mb_per_s": vbytes / t_wr / 1e6}
+    log("%d^3 p=2 VTU: vertex fields of %d elements in %.3f s (host arrays in and out); writer %.1f MB/s"
+        % (nel, sv.shape[0], t_sv, vbytes / t_wr / 1e6))
+    del sv, sl
     asm = Assembler(bp.args)
     F = asm.run_rhs().cpu().numpy()
     u64, u32 = xs["fp64"], xs["mixed"]

@@ -316,6 +316,25 @@ int32_t yeti_reconstruction(const double *u_inv, const int32_t *ind_dof_free, in
                             const double *bc_values_host, int32_t nb_bc, int32_t mcrd, int32_t nb_cp,
                             double *sol, double *U, void *stream);
 
+/* VTU output of solid (U1) patches: generateVTU, postprocessing/postproc_vtu.f:22-297, called as
+ * postproc.generatevtu(*IGAparametrization.get_inputs4postprocVTU(...))
+ * (IGA_parametrization.py:717-752).
+ *
+ * yeti_svars_solid replaces compute_svars_Q1 (postprocessing/compute_svars_solid.f:24-188) over
+ * the elements of one patch, on the GPU: host arrays in and out; coords3d and sol are (3, nb_cp)
+ * column-major; nb_ref[d] in 1..8 gives 2^(nb_ref[d]-1)+1 vertices per direction; flags =
+ * (disp, stress, strain) as FieldOutput_flag; E, nu the patch's MATERIAL_PROPERTIES. svars
+ * receives nelem x nvertex x nsvint doubles (nsvint = 3 + 3 disp + 6 stress + 6 strain), the
+ * reference's svars(nsvint*nb_vertice, nb_elem).
+ *
+ * yeti_vtu_write_piece replaces writePatch (postprocessing/writepatch.f) for a 3D patch: it
+ * appends the <Piece> of those svars (points, hexahedron cells, disp, stress, von Mises,
+ * strain) to the file at path. Host only. */
+int32_t yeti_svars_solid(const yeti_nurbs_patch *patch, int32_t nb_cp, const double *coords3d, const double *sol,
+                         const int32_t *nb_ref, const int32_t *flags, double E, double nu, double *svars);
+int32_t yeti_vtu_write_piece(const char *path, const double *svars, int32_t nb_elem, const int32_t *nb_ref,
+                             const int32_t *flags);
+
 #ifdef __cplusplus
 }
 #endif

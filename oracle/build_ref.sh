@@ -4,7 +4,8 @@
 #
 # The file list is the reference's own `base` library (src/yeti_iga/CMakeLists.txt:21-58),
 # the `optim` library (src/yeti_iga/optim/CMakeLists.txt:20-27), the two assembly drivers,
-# the displacement reconstruction and evaluation, the mass-matrix driver and the gradient
+# the displacement reconstruction and evaluation, the mass-matrix driver, the VTU writer
+# (generateVTU with its per-element-type field routines and writePatch) and the gradient
 # driver, compiled with the
 # reference's flags (-O2, -fopenmp, -fPIC;
 # CMakeLists.txt:69, src/CMakeLists.txt:42-48) by amdflang (LLVM flang), the only Fortran
@@ -37,7 +38,9 @@ main/embedded/mapEntities2parametricSpaces.f main/embedded/UELMAT_embeddedshell.
 main/embedded/UELMAT_embeddedvol.f90 main/m_nurbspatch.f90 main/shapPress_omp.f90
 main/shap_omp.f90 main/uelmat_solid_omp.f90 main/build_system_elemStorage.f90
 main/build_system_elemStorage_OMP.f90 main/reconstruction.f postprocessing/evaluate_disp.f
-main/compute_CMASSMatrix.f optim/gradUELMAT_solid.f90 optim/gradUELMAT_shell.f
+main/compute_CMASSMatrix.f postprocessing/compute_svars_solid.f postprocessing/compute_svars_shell.f
+postprocessing/compute_svars_embdedshell.f postprocessing/compute_svars_embdedsolid.f
+postprocessing/writepatch.f postprocessing/postproc_vtu.f optim/gradUELMAT_solid.f90 optim/gradUELMAT_shell.f
 optim/gradUELMAT_embeddedshell.f optim/gradUELMAT_embeddedsolid.f90 optim/gradtotalwork_fullAN.f90"
 TMP="$(mktemp -d /tmp/yetiref.XXXXXX)"
 trap 'rm -rf "$TMP"' EXIT

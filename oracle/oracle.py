@@ -4,11 +4,12 @@ Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
 only as the checker. Two independent CPU implementations are exposed:
 
 * ``oracle_sys_linmat`` -- liboracle.so, our C restatement (oracle/yeti_oracle.c);
-* ``oracle_reconstruction`` / ``oracle_evaldisp`` / ``oracle_mass_coo`` -- the same library's
-  restatements of the displacement scatter, the displacement at parametric points and the
-  consistent mass matrix;
+* ``oracle_reconstruction`` / ``oracle_evaldisp`` / ``oracle_mass_coo`` / ``oracle_svars_solid``
+  -- the same library's restatements of the displacement scatter, the displacement at
+  parametric points, the consistent mass matrix and the VTU vertex fields;
 * ``ref_sys_linmat_omp`` / ``ref_sys_linmat_serial`` / ``ref_dersbasisfuns`` / ``ref_gauss`` /
-  ``ref_grad_linelast_work`` / ``ref_reconstruction`` / ``ref_evaldisp`` / ``ref_build_cmassmatrix``
+  ``ref_grad_linelast_work`` / ``ref_reconstruction`` / ``ref_evaldisp`` / ``ref_build_cmassmatrix`` /
+  ``ref_generate_vtu``
   -- oracle/_ref/libyetiref.so, YETI's own Fortran compiled from
   its sources by oracle/build_ref.sh (present only where the YETI sources were available at
   build time; callers must handle ``ref_available() == False``).
@@ -398,3 +399,73 @@ def ref_build_cmassmatrix(args):
       s.ind_dof_free, iv(s.nb_dof_free), iv(s.MCRD), s.NBINT, iv(s.nb_patch), iv(s.nb_elem), s.NNODE, iv(s.nb_cp),
       iv(s.nb_dof_tot))
     return Md, Mr, Mc
+
+
+def _vtu_counts(nb_ref, flags):
+    nb = [2 ** max(int(r) - 1, 0) + 1 for r in nb_ref]
+    nsv = 3 + 3 * bool(flags[0]) + 6 * bool(flags[1]) + 6 * bool(flags[2])
+    return nb, nb[0] * nb[1] * nb[2], nsv
+
+
+def oracle_svars_solid(d, ip, sol, nb_ref=(1, 1, 1), flags=(True, True, True)):
+    """liboracle's restatement of compute_svars_Q1 (postprocessing/compute_svars_solid.f:24) over
+    every element of solid patch ip (0-based); sol (nb_cp, 3). Returns (nelem, nvertex, nsvint)."""
+    s, keep = pack_system_inputs(d)
+    S = np.ascontiguousarray(np.asarray(sol, np.float64).reshape(-1, 3))
+    ref_ = np.asarray(nb_ref, np.int32)
+    fl = np.asarray([int(bool(f)) for f in flags], np.int32)
+    _, nv, nsv = _vtu_counts(ref_, fl)
+    nel = int(np.asarray(d["nb_elem_patch"]).reshape(-1)[ip])
+    out = np.zeros((nel, nv, nsv))
+    f = lib().oracle_svars_solid
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p, C.c_int, _f64p, _i32p, _i32p, _f64p]
+    rc = f(C.byref(s), int(ip), _p(S, _f64p), _p(ref_, _i32p), _p(fl, _i32p), _p(out, _f64p))
+    if rc:
+        raise ValueError("oracle_svars_solid failed (%d)" % rc)
+    return out
+
+
+def ref_generate_vtu(d, sol, outdir, filename="ref", nb_ref=(1, 1, 1), flags=(True, True, True)):
+    """YETI's generateVTU (postprocessing/postproc_vtu.f:22) through oracle/ref_shim.f90; sol
+    (nb_cp, 3). Writes outdir/filename.vtu and returns its path."""
+    s, keep = pack_system_inputs(d)
+    S = np.ascontiguousarray(np.asarray(sol, np.float64).reshape(-1, 3))
+    matp = np.asfortranarray(np.asarray(d["MATERIAL_PROPERTIES"], np.float64)[:2])
+    ref_ = np.asarray(nb_ref, np.int32)
+    fl = np.asarray([int(bool(f)) for f in flags], np.int32)
+    fn = filename.encode()
+    op = str(outdir).encode()
+    keep2 = []
+
+    def iv(v):
+        x = C.c_int32(int(v)); keep2.append(x); return C.byref(x)
+
+    f = ref().ref_generate_vtu
+    f.restype = None
+    f(C.c_char_p(fn), iv(len(fn)), C.c_char_p(op), iv(len(op)), _p(fl, _i32p), _p(ref_, _i32p), _p(S, _f64p),
+      s.COORDS3D, s.IEN, iv(s.len_IEN), s.nb_elem_patch, s.Nkv, s.Ukv, iv(s.len_Ukv), s.Nijk, s.weight,
+      iv(s.len_weight), s.Jpqr, s.ELT_TYPE, iv(s.len_ELT_TYPE), matp.ctypes.data_as(_f64p), s.TENSOR,
+      iv(s.len_TENSOR), s.PROPS, iv(max(1, s.len_PROPS)), s.JPROPS, s.NNODE, iv(s.nb_patch), iv(s.nb_elem),
+      iv(s.nb_cp), iv(3))
+    return os.path.join(str(outdir), filename + ".vtu")
+
+
+def parse_vtu(path):
+    """The pieces of a VTU file as written by writePatch (postprocessing/writepatch.f): a list of
+    dicts with NumberOfPoints, NumberOfCells and one array per DataArray (points, connectivity,
+    offsets, types, disp, stress, VM, strain). Tags may span lines (list-directed output wraps
+    long records) and numbers are read whatever their formatting."""
+    import re
+    with open(path) as fh:
+        text = fh.read()
+    pieces = []
+    for m in re.finditer(r"<Piece([^>]*)>(.*?)</Piece", text, re.S):
+        nums = re.findall(r'"\s*(\d+)\s*"', m.group(1))
+        cur = {"NumberOfPoints": int(nums[0]), "NumberOfCells": int(nums[1])}
+        for a in re.finditer(r"<DataArray([^>]*)>(.*?)</DataArray", m.group(2), re.S):
+            nm = re.search(r'Name="(\w+)"', a.group(1))
+            toks = a.group(2).split()
+            cur[nm.group(1) if nm else "points"] = np.array([float(t.replace("D", "E")) for t in toks])
+        pieces.append(cur)
+    return pieces

@@ -262,3 +262,60 @@ subroutine ref_build_cmassmatrix(Mdata, Mrow, Mcol, activeElement, nb_data, COOR
         &   Nkv, Ukv, Nijk, weight, Jpqr, elt_s, PROPS, JPROPS, RHO, tensor_s, ind_dof_free,           &
         &   nb_dof_free, MCRD, NBINT, nb_patch, nb_elem, NNODE, nb_cp, nb_dof_tot)
 end subroutine ref_build_cmassmatrix
+
+! generateVTU (postprocessing/postproc_vtu.f:22) takes assumed-shape Ukv, weight, PROPS, IEN and
+! character arguments; FieldOutput_flag arrives as 0/1 integers.
+subroutine ref_generate_vtu(FILENAME, len_fn, OUTPATH, len_op, iflag, nb_refinement, SOL, COORDS3D, IEN, &
+    &   len_IEN, nb_elem_patch, Nkv, Ukv, len_Ukv, Nijk, weight, len_weight, Jpqr, ELT_TYPE, len_elt,      &
+    &   MATERIAL_PROPERTIES, TENSOR, len_tensor, PROPS, len_PROPS, JPROPS, NNODE, nb_patch, nb_elem,      &
+    &   nb_cp, MCRD) bind(C, name='ref_generate_vtu')
+    use iso_c_binding
+    implicit none
+    integer(c_int), intent(in) :: len_fn, len_op, len_IEN, len_Ukv, len_weight, len_elt, len_tensor,     &
+        &   len_PROPS, nb_patch, nb_elem, nb_cp, MCRD
+    integer(c_int), intent(in) :: iflag(3), nb_refinement(3), IEN(len_IEN), nb_elem_patch(nb_patch),    &
+        &   Nkv(3, nb_patch), Nijk(3, nb_elem), Jpqr(3, nb_patch), JPROPS(nb_patch), NNODE(nb_patch)
+    real(c_double), intent(in) :: SOL(MCRD, nb_cp), COORDS3D(3, nb_cp), Ukv(len_Ukv), weight(len_weight), &
+        &   MATERIAL_PROPERTIES(2, nb_patch), PROPS(len_PROPS)
+    character(kind=c_char), intent(in) :: FILENAME(len_fn), OUTPATH(len_op), ELT_TYPE(len_elt),          &
+        &   TENSOR(len_tensor)
+    character(len=len_fn) :: fn_s
+    character(len=len_op) :: op_s
+    character(len=len_elt) :: elt_s
+    character(len=len_tensor) :: tensor_s
+    logical :: flags(3)
+    integer :: i
+
+    interface
+        subroutine generateVTU(FILENAME, output_path, FieldOutput_flag, nb_refinement, SOL, COORDS3D,   &
+            &   IEN, nb_elem_patch, Nkv, Ukv, Nijk, weight, Jpqr, ELT_TYPE, MATERIAL_PROPERTIES, TENSOR, &
+            &   PROPS, JPROPS, NNODE, nb_patch, nb_elem, nb_cp, MCRD)
+            integer, intent(in) :: nb_cp, MCRD, NNODE(nb_patch), nb_patch, nb_elem, IEN(:),             &
+                &   nb_elem_patch(nb_patch), JPROPS(nb_patch), Nkv(3, nb_patch), Jpqr(3, nb_patch),     &
+                &   Nijk(3, nb_elem), nb_refinement(3)
+            double precision, intent(in) :: COORDS3D(3, nb_cp), Ukv(:), weight(:),                    &
+                &   MATERIAL_PROPERTIES(2, nb_patch), PROPS(:), SOL(MCRD, nb_cp)
+            character(len=*), intent(in) :: FILENAME, output_path, TENSOR, ELT_TYPE
+            logical, intent(in) :: FieldOutput_flag(3)
+        end subroutine
+    end interface
+
+    do i = 1, len_fn
+        fn_s(i:i) = FILENAME(i)
+    enddo
+    do i = 1, len_op
+        op_s(i:i) = OUTPATH(i)
+    enddo
+    do i = 1, len_elt
+        elt_s(i:i) = ELT_TYPE(i)
+    enddo
+    do i = 1, len_tensor
+        tensor_s(i:i) = TENSOR(i)
+    enddo
+    do i = 1, 3
+        flags(i) = iflag(i) /= 0
+    enddo
+    call generateVTU(fn_s, op_s, flags, nb_refinement, SOL, COORDS3D, IEN, nb_elem_patch, Nkv, Ukv,     &
+        &   Nijk, weight, Jpqr, elt_s, MATERIAL_PROPERTIES, tensor_s, PROPS, JPROPS, NNODE, nb_patch,    &
+        &   nb_elem, nb_cp, MCRD)
+end subroutine ref_generate_vtu

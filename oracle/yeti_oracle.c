@@ -769,3 +769,68 @@ int oracle_mass_coo(const yeti_system_inputs *in, double *Md, int32_t *Mr, int32
     }
     return 0;
 }
+
+/* compute_svars_Q1 (postprocessing/compute_svars_solid.f:24-188) for every element of solid
+ * patch ip: at the vertex grid of each element (nb = 2^max(r-1,0)+1 points per direction,
+ * parent coordinate 2/(nb-1) (i-1) - 1, xi fastest; :104-116), shap's R and dR/dx
+ * (main/shap.f90) give the point (sum R x), the displacement (sum R u, flag 1), the strain
+ * (engineering shears 12, 13, 23; :129-145, flag 2 or 3) and the stress D strain
+ * (material_lib, flag 2). svars holds, element after element, nsvint values per vertex:
+ * point(3), [u(3)], [stress(6)], [strain(6)] (:151-173). sol is SOL(3, nb_cp). */
+int oracle_svars_solid(const yeti_system_inputs *in, int ip, const double *sol, const int32_t *nb_ref,
+                       const int32_t *flags, double *svars)
+{
+    if (in->MCRD != 3 || ip < 0 || ip >= in->nb_patch) return YETI_ERR_INVALID_ARGUMENT;
+    opatch P;
+    int rc = build_patch(in, ip, &P);
+    if (rc) return rc;
+    int nb[3];
+    for (int d = 0; d < 3; ++d) nb[d] = (1 << (nb_ref[d] - 1 > 0 ? nb_ref[d] - 1 : 0)) + 1;
+    const int nv = nb[0] * nb[1] * nb[2];
+    const int nsv = 3 + (flags[0] ? 3 : 0) + (flags[1] ? 6 : 0) + (flags[2] ? 6 : 0);
+    double D[6][6];
+    oracle_ddsdde(P.E, P.nu, D);
+    const int n = P.nnode;
+    double *R = (double *)malloc(sizeof(double) * n);
+    double (*dRdx)[3] = (double (*)[3])malloc(sizeof(double) * 3 * n);
+    for (int e = 0; e < P.nelem; ++e) {
+        const int32_t *sctr = P.ien + (int64_t)n * e;
+        for (int iz = 0; iz < nb[2]; ++iz)
+            for (int iy = 0; iy < nb[1]; ++iy)
+                for (int ix = 0; ix < nb[0]; ++ix) {
+                    const int v = (iz * nb[1] + iy) * nb[0] + ix;
+                    const double pt[3] = {2.0 / (double)(nb[0] - 1) * (double)ix - 1.0,
+                                          2.0 / (double)(nb[1] - 1) * (double)iy - 1.0,
+                                          2.0 / (double)(nb[2] - 1) * (double)iz - 1.0};
+                    double AJ[3][3], detJ;
+                    oracle_shap(&P, e, in->COORDS3D, pt, R, dRdx, AJ, &detJ);
+                    double x[3] = {0, 0, 0}, u[3] = {0, 0, 0}, eps[6] = {0, 0, 0, 0, 0, 0}, sig[6];
+                    for (int a = 0; a < n; ++a) {
+                        const int64_t cp = sctr[a] - 1;
+                        for (int c = 0; c < 3; ++c) {
+                            x[c] += R[a] * in->COORDS3D[3 * cp + c];
+                            u[c] += R[a] * sol[3 * cp + c];
+                        }
+                        const double *g = dRdx[a], *s = sol + 3 * cp;
+                        eps[0] += g[0] * s[0];
+                        eps[1] += g[1] * s[1];
+                        eps[3] += g[1] * s[0] + g[0] * s[1];
+                        eps[2] += g[2] * s[2];
+                        eps[4] += g[2] * s[0] + g[0] * s[2];
+                        eps[5] += g[1] * s[2] + g[2] * s[1];
+                    }
+                    for (int i = 0; i < 6; ++i) {
+                        sig[i] = 0.0;
+                        for (int j = 0; j < 6; ++j) sig[i] += D[i][j] * eps[j];
+                    }
+                    double *o = svars + ((int64_t)e * nv + v) * nsv;
+                    for (int c = 0; c < 3; ++c) *o++ = x[c];
+                    if (flags[0]) for (int c = 0; c < 3; ++c) *o++ = u[c];
+                    if (flags[1]) for (int c = 0; c < 6; ++c) *o++ = sig[c];
+                    if (flags[2]) for (int c = 0; c < 6; ++c) *o++ = eps[c];
+                }
+    }
+    free(R);
+    free(dRdx);
+    return 0;
+}

@@ -74,6 +74,9 @@ SIGNATURES = {
     "yeti_evaldisp": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "yeti_reconstruction": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_svars_solid": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_double, C.c_double, C.c_void_p]),
+    "yeti_vtu_write_piece": (C.c_int32, [C.c_char_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "yeti_pcg_csr": (C.c_int32, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                  C.POINTER(C.c_double), C.c_void_p]),
