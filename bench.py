@@ -502,6 +502,21 @@ def main():
     log("mass matrix (COO values + indices): %.2f ms, %.0f GB/s" % (mass_ms, mass_bytes / (mass_ms * 1e-3) / 1e9))
     del K, r, c, F
     torch.cuda.empty_cache()
+    # natural-frequency shape gradient (gradvibration_fullAN.f) of 4 synthetic modes on the same
+    # model (the first call also builds the CP -> element-node map)
+    nmode = 4
+    vect = torch.randn((nmode, 3, bp.nb_cp), dtype=torch.float64, device="cuda")
+    lam = np.linspace(1e6, 4e7, nmode)
+    asm.run_vib_grad(vect, lam)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        gv = asm.run_vib_grad(vect, lam)
+    torch.cuda.synchronize()
+    vib_ms = (time.perf_counter() - t0) / 3 * 1e3
+    log("vibration gradient (%d modes): %.2f ms, %.3e elem/s" % (nmode, vib_ms, bp.nb_elem / (vib_ms * 1e-3)))
+    del vect, gv
+    torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -547,6 +562,7 @@ def main():
                         "roofline": {"bound": "hbm", "achieved": mass_bytes / (mass_ms * 1e-3) / 1e9,
                                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                      "frac": mass_bytes / (mass_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}},
+        "vibration_gradient": {"ms": vib_ms, "modes": nmode, "elements_per_s": bp.nb_elem / (vib_ms * 1e-3)},
     }
     if not args.only_assembly:
         out["c2_solve"] = bench_c2_solve(args.nel, p, args.jacobi_iters)

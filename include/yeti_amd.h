@@ -138,13 +138,21 @@ int32_t yeti_asm_run_coo_matrix(yeti_asm *h, double *Kdata_dev, int32_t *Krow_de
                                 void *stream);
 /* Values only (indices are a pure function of IEN and can be emitted once). */
 int32_t yeti_asm_run_coo_values(yeti_asm *h, double *Kdata_dev, void *stream);
-/* Right-hand side only (concentrated, surface and body loads). */
 /* Consistent mass matrix of the resident model (f2py massmtrx.build_cmassmatrix,
  * src/yeti_iga/main/compute_CMASSMatrix.f:22-215 with UMASSMAT_byCP, UMASSMAT.f:220-330, solid
  * elements): M_ab[k][k] = sum_q RHO w_q DetJac_q R_a R_b, k != l entries 0, emitted in the same
  * COO layout and at the same element offsets as the stiffness (nb_data entries). Mrow / Mcol may
  * both be null (values only). Device pointers. */
 int32_t yeti_asm_run_mass_coo(yeti_asm *h, double *Mdata, int32_t *Mrow, int32_t *Mcol, void *stream);
+/* Natural-frequency shape gradient: gradVibration_AN, optim/gradvibration_fullAN.f:22-189 with
+ * gradUELMASSMAT1 (optim/gradUELMASSMAT_solid.f:22-332), f2py vibration.gradvibration_an (caller
+ * OPT_modelling.py:823-854). vect (device) holds nb_frq eigenvectors as (nb_frq, 3, nb_cp)
+ * column-major, vals (host) their eigenvalues; gradV (device, same layout) receives
+ * d lambda_f / d P over the active U1 elements of the model, divided per mode by the mass
+ * u_f^T M u_f of those elements. */
+int32_t yeti_asm_run_vib_grad(yeti_asm *h, const double *vect, const double *vals, int32_t nb_frq, double *gradV,
+                              void *stream);
+/* Right-hand side only (concentrated, surface and body loads). */
 int32_t yeti_asm_run_rhs(yeti_asm *h, double *F_dev, void *stream);
 /* Replace the control-point coordinates (device, 3 nb_cp values, COORDS3D order) used by later
  * runs; the design-update step of the reference's shape optimisation

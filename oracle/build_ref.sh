@@ -5,11 +5,10 @@
 # The file list is the reference's own `base` library (src/yeti_iga/CMakeLists.txt:21-58),
 # the `optim` library (src/yeti_iga/optim/CMakeLists.txt:20-27), the two assembly drivers,
 # the displacement reconstruction and evaluation, the mass-matrix driver, the VTU writer
-# (generateVTU with its per-element-type field routines and writePatch) and the gradient
-# driver, compiled with the
-# reference's flags (-O2, -fopenmp, -fPIC;
-# CMakeLists.txt:69, src/CMakeLists.txt:42-48) by amdflang (LLVM flang), the only Fortran
-# compiler in this image.
+# (generateVTU with its per-element-type field routines and writePatch), the compliance
+# gradient driver and the natural-frequency gradient (gradVibration_AN, gradUELMASSMAT1),
+# compiled with the reference's flags (-O2, -fopenmp, -fPIC; CMakeLists.txt:69,
+# src/CMakeLists.txt:42-48) by amdflang (LLVM flang), the only Fortran compiler in this image.
 #
 # One compatibility rewrite is applied to a temporary copy (never kept): flang rejects the
 # specification expression `indDLoad(SUM(load_target_nbelem))` because load_target_nbelem is
@@ -41,7 +40,8 @@ main/build_system_elemStorage_OMP.f90 main/reconstruction.f postprocessing/evalu
 main/compute_CMASSMatrix.f postprocessing/compute_svars_solid.f postprocessing/compute_svars_shell.f
 postprocessing/compute_svars_embdedshell.f postprocessing/compute_svars_embdedsolid.f
 postprocessing/writepatch.f postprocessing/postproc_vtu.f optim/gradUELMAT_solid.f90 optim/gradUELMAT_shell.f
-optim/gradUELMAT_embeddedshell.f optim/gradUELMAT_embeddedsolid.f90 optim/gradtotalwork_fullAN.f90"
+optim/gradUELMAT_embeddedshell.f optim/gradUELMAT_embeddedsolid.f90 optim/gradtotalwork_fullAN.f90
+optim/gradUELMASSMAT_solid.f optim/gradvibration_fullAN.f"
 TMP="$(mktemp -d /tmp/yetiref.XXXXXX)"
 trap 'rm -rf "$TMP"' EXIT
 cd "$TMP"

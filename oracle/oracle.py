@@ -5,11 +5,12 @@ only as the checker. Two independent CPU implementations are exposed:
 
 * ``oracle_sys_linmat`` -- liboracle.so, our C restatement (oracle/yeti_oracle.c);
 * ``oracle_reconstruction`` / ``oracle_evaldisp`` / ``oracle_mass_coo`` / ``oracle_svars_solid``
-  -- the same library's restatements of the displacement scatter, the displacement at
-  parametric points, the consistent mass matrix and the VTU vertex fields;
+  / ``oracle_grad_vibration`` -- the same library's restatements of the displacement scatter,
+  the displacement at parametric points, the consistent mass matrix, the VTU vertex fields and
+  the natural-frequency shape gradient;
 * ``ref_sys_linmat_omp`` / ``ref_sys_linmat_serial`` / ``ref_dersbasisfuns`` / ``ref_gauss`` /
   ``ref_grad_linelast_work`` / ``ref_reconstruction`` / ``ref_evaldisp`` / ``ref_build_cmassmatrix`` /
-  ``ref_generate_vtu``
+  ``ref_generate_vtu`` / ``ref_grad_vibration``
   -- oracle/_ref/libyetiref.so, YETI's own Fortran compiled from
   its sources by oracle/build_ref.sh (present only where the YETI sources were available at
   build time; callers must handle ``ref_available() == False``).
@@ -469,3 +470,44 @@ def parse_vtu(path):
             cur[nm.group(1) if nm else "points"] = np.array([float(t.replace("D", "E")) for t in toks])
         pieces.append(cur)
     return pieces
+
+
+def oracle_grad_vibration(d, VECT, VALS):
+    """liboracle's restatement of gradVibration_AN (optim/gradvibration_fullAN.f:22) over the
+    active elements of d; VECT (nb_frq, 3, nb_cp), VALS (nb_frq,) -> gradV (nb_frq, 3, nb_cp)."""
+    s, keep = pack_system_inputs(d)
+    V = np.asfortranarray(np.asarray(VECT, np.float64))
+    nf = V.shape[0]
+    L = np.ascontiguousarray(np.asarray(VALS, np.float64).reshape(nf))
+    out = np.zeros(V.shape, order="F")
+    f = lib().oracle_grad_vibration
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int, _f64p]
+    rc = f(C.byref(s), V.ctypes.data_as(_f64p), _p(L, _f64p), nf, out.ctypes.data_as(_f64p))
+    if rc:
+        raise ValueError("oracle_grad_vibration failed (%d)" % rc)
+    return np.ascontiguousarray(out)
+
+
+def ref_grad_vibration(d, VECT, VALS, activeDir=(1, 1, 1)):
+    """YETI's gradVibration_AN through oracle/ref_shim.f90 over the active elements of d."""
+    s, keep = pack_system_inputs(d)
+    V = np.asfortranarray(np.asarray(VECT, np.float64))
+    nf = V.shape[0]
+    L = np.ascontiguousarray(np.asarray(VALS, np.float64).reshape(nf))
+    out = np.zeros(V.shape, order="F")
+    matp = np.asfortranarray(np.asarray(d["MATERIAL_PROPERTIES"], np.float64)[:2])
+    adir = np.asarray(activeDir, np.int32)
+    keep2 = []
+
+    def iv(v):
+        x = C.c_int32(int(v)); keep2.append(x); return C.byref(x)
+
+    f = ref().ref_grad_vibration
+    f.restype = None
+    f(out.ctypes.data_as(_f64p), V.ctypes.data_as(_f64p), _p(L, _f64p), iv(nf), s.activeElement, _p(adir, _i32p),
+      s.COORDS3D, s.IEN, iv(s.len_IEN), s.nb_elem_patch, s.Nkv, s.Ukv, iv(s.len_Ukv), s.Nijk, s.weight,
+      iv(s.len_weight), s.Jpqr, s.ELT_TYPE, iv(s.len_ELT_TYPE), s.PROPS, iv(max(1, s.len_PROPS)), s.JPROPS,
+      matp.ctypes.data_as(_f64p), s.RHO, s.TENSOR, iv(s.len_TENSOR), iv(3), s.NBINT, iv(s.nb_patch), iv(s.nb_elem),
+      s.NNODE, iv(s.nb_cp))
+    return np.ascontiguousarray(out)

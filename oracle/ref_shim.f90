@@ -319,3 +319,50 @@ subroutine ref_generate_vtu(FILENAME, len_fn, OUTPATH, len_op, iflag, nb_refinem
         &   Nijk, weight, Jpqr, elt_s, MATERIAL_PROPERTIES, tensor_s, PROPS, JPROPS, NNODE, nb_patch,    &
         &   nb_elem, nb_cp, MCRD)
 end subroutine ref_generate_vtu
+
+! gradVibration_AN (optim/gradvibration_fullAN.f:22) takes assumed-shape Ukv, weight, PROPS, IEN.
+subroutine ref_grad_vibration(gradV, VECT, VALS, nb_frq, activeElement, activeDir, COORDS3D, IEN,     &
+    &   len_IEN, nb_elem_patch, Nkv, Ukv, len_Ukv, Nijk, weight, len_weight, Jpqr, ELT_TYPE, len_elt,    &
+    &   PROPS, len_PROPS, JPROPS, MATERIAL_PROPERTIES, RHO, TENSOR, len_tensor, MCRD, NBINT, nb_patch,   &
+    &   nb_elem, NNODE, nb_cp) bind(C, name='ref_grad_vibration')
+    use iso_c_binding
+    implicit none
+    integer(c_int), intent(in) :: nb_frq, len_IEN, len_Ukv, len_weight, len_elt, len_PROPS, len_tensor, &
+        &   MCRD, nb_patch, nb_elem, nb_cp
+    integer(c_int), intent(in) :: activeElement(nb_elem), activeDir(3), IEN(len_IEN),                  &
+        &   nb_elem_patch(nb_patch), Nkv(3, nb_patch), Nijk(3, nb_elem), Jpqr(3, nb_patch),              &
+        &   JPROPS(nb_patch), NBINT(nb_patch), NNODE(nb_patch)
+    real(c_double), intent(in) :: VECT(nb_frq, 3, nb_cp), VALS(nb_frq), COORDS3D(3, nb_cp),            &
+        &   Ukv(len_Ukv), weight(len_weight), PROPS(len_PROPS), MATERIAL_PROPERTIES(2, nb_patch),        &
+        &   RHO(nb_patch)
+    character(kind=c_char), intent(in) :: ELT_TYPE(len_elt), TENSOR(len_tensor)
+    real(c_double), intent(out) :: gradV(nb_frq, 3, nb_cp)
+    character(len=len_elt) :: elt_s
+    character(len=len_tensor) :: tensor_s
+    integer :: i
+
+    interface
+        subroutine gradVibration_AN(gradV, VECT, VALS, nb_frq, activeElement, activeDir, COORDS3D, IEN, &
+            &   nb_elem_patch, Nkv, Ukv, Nijk, weight, Jpqr, ELT_TYPE, PROPS, JPROPS,                    &
+            &   MATERIAL_PROPERTIES, RHO, TENSOR, MCRD, NBINT, nb_patch, nb_elem, nnode, nb_cp)
+            integer, intent(in) :: nb_cp, MCRD, NNODE(nb_patch), nb_patch, nb_elem, IEN(:),             &
+                &   nb_elem_patch(nb_patch), JPROPS(nb_patch), Nkv(3, nb_patch), Jpqr(3, nb_patch),     &
+                &   Nijk(3, nb_elem), NBINT(nb_patch), activeElement(nb_elem), activeDir(3), nb_frq
+            double precision, intent(in) :: COORDS3D(3, nb_cp), Ukv(:), weight(:), PROPS(:),            &
+                &   MATERIAL_PROPERTIES(2, nb_patch), RHO(nb_patch), VECT(nb_frq, 3, nb_cp),             &
+                &   VALS(nb_frq)
+            character(len=*), intent(in) :: TENSOR, ELT_TYPE
+            double precision, intent(out) :: gradV(nb_frq, 3, nb_cp)
+        end subroutine
+    end interface
+
+    do i = 1, len_elt
+        elt_s(i:i) = ELT_TYPE(i)
+    enddo
+    do i = 1, len_tensor
+        tensor_s(i:i) = TENSOR(i)
+    enddo
+    call gradVibration_AN(gradV, VECT, VALS, nb_frq, activeElement, activeDir, COORDS3D, IEN,             &
+        &   nb_elem_patch, Nkv, Ukv, Nijk, weight, Jpqr, elt_s, PROPS, JPROPS, MATERIAL_PROPERTIES, RHO, &
+        &   tensor_s, MCRD, NBINT, nb_patch, nb_elem, NNODE, nb_cp)
+end subroutine ref_grad_vibration

@@ -195,8 +195,18 @@ static int gauss_per_dir(int nbint)
  * element e of patch P. Restates main/shap.f90:21-180 / main/shap_omp.f90:21-180.
  * Outputs: R[n], dRdx[n][3], AJ (parent->physical 3x3, row=physical), *detJ.
  * ------------------------------------------------------------------------------------- */
+static void oracle_shap_xi(const opatch *P, int e, const double *coords, const double *pt,
+                           double *R, double (*dRdx)[3], double AJ[3][3], double *detJ, double (*dRdxi_out)[3]);
 static void oracle_shap(const opatch *P, int e, const double *coords, const double *pt,
                         double *R, double (*dRdx)[3], double AJ[3][3], double *detJ)
+{
+    oracle_shap_xi(P, e, coords, pt, R, dRdx, AJ, detJ, NULL);
+}
+
+/* As oracle_shap, with the parametric derivatives dR/dxi (n x 3) also returned when dRdxi_out
+ * is not NULL (evaluatenurbsfcts.f evalnurbs gives the same R and dR/dxi). */
+static void oracle_shap_xi(const opatch *P, int e, const double *coords, const double *pt,
+                           double *R, double (*dRdx)[3], double AJ[3][3], double *detJ, double (*dRdxi_out)[3])
 {
     const int n = P->nnode;
     const int32_t *nijk = P->nijk + 3 * e;
@@ -262,6 +272,7 @@ static void oracle_shap(const opatch *P, int e, const double *coords, const doub
     *detJ = AJ[0][0] * AJ[1][1] * AJ[2][2] + AJ[0][1] * AJ[1][2] * AJ[2][0]
           + AJ[1][0] * AJ[2][1] * AJ[0][2] - AJ[0][2] * AJ[1][1] * AJ[2][0]
           - AJ[0][1] * AJ[1][0] * AJ[2][2] - AJ[1][2] * AJ[2][1] * AJ[0][0];
+    if (dRdxi_out) memcpy(dRdxi_out, dRdxi, sizeof(double) * 3 * n);
     free(dRdxi);
 }
 
@@ -832,5 +843,171 @@ int oracle_svars_solid(const yeti_system_inputs *in, int ip, const double *sol, 
     }
     free(R);
     free(dRdx);
+    return 0;
+}
+
+/* gradVibration_AN (optim/gradvibration_fullAN.f:22-189) with gradUELMASSMAT1
+ * (optim/gradUELMASSMAT_solid.f:22-332), solid patches: for every active element and Gauss point,
+ * the covariant basis a_i = dx/dxi_i (:120-126), Area = (a1 x a2).a3 (:128-134), the metric
+ * a_i.a_j and its inverse (:141-149), H^{ijkl} = lam g^ij g^kl + mu (g^ik g^jl + g^il g^jk) in
+ * Voigt order 11,22,33,12,13,23 (:152-165), per mode the covariant engineering strain and
+ * H strain (:182-200), work = strain.stress and the mass rho |u|^2 Area detJac (:203-211); per
+ * node the Jacobian term (work - lambda rho |u|^2) dArea/dP (:214-223), the strain term
+ * (:226-243), the metric term (:246-289); the CP gradient is then divided per mode by the total
+ * of the element masses of the active elements (:185-187). VECT is (nb_frq, 3, nb_cp) and gradV
+ * (nb_frq, 3, nb_cp), column-major; detJac is the Gauss weight times the parent-to-parametric
+ * factor (:109-114). */
+int oracle_grad_vibration(const yeti_system_inputs *in, const double *VECT, const double *VALS, int nb_frq,
+                          double *gradV)
+{
+    if (in->MCRD != 3 || nb_frq < 1) return YETI_ERR_INVALID_ARGUMENT;
+    const int64_t ncp = in->nb_cp;
+    memset(gradV, 0, sizeof(double) * nb_frq * 3 * ncp);
+    double *mass = (double *)calloc(nb_frq, sizeof(double));
+    static const int vg[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+    for (int ip = 0; ip < in->nb_patch; ++ip) {
+        opatch P;
+        int rc = build_patch(in, ip, &P);
+        if (rc) { free(mass); return rc; }
+        if (strcmp(P.elt, "U1") != 0) continue;
+        const int n = P.nnode, ng = gauss_per_dir(P.nbint);
+        double gx[MAXG], gw[MAXG];
+        if (oracle_gauss(ng, gx, gw)) { free(mass); return YETI_ERR_UNSUPPORTED_DEGREE; }
+        const double lam = P.E * P.nu / (1.0 + P.nu) / (1.0 - 2.0 * P.nu), mu = P.E / 2.0 / (1.0 + P.nu);
+        double *R = (double *)malloc(sizeof(double) * n);
+        double (*dRdx)[3] = (double (*)[3])malloc(sizeof(double) * 3 * n);
+        double (*dRdxi)[3] = (double (*)[3])malloc(sizeof(double) * 3 * n);
+        double *ge = (double *)malloc(sizeof(double) * nb_frq * 3 * n);
+        double *ue = (double *)malloc(sizeof(double) * nb_frq * 3 * n);
+        double (*X)[3] = (double (*)[3])malloc(sizeof(double) * 3 * n);
+        for (int e = 0; e < P.nelem; ++e) {
+            if (in->activeElement[P.elem0 + e] != 1) continue;
+            const int32_t *sctr = P.ien + (int64_t)n * e;
+            for (int a = 0; a < n; ++a)
+                for (int c = 0; c < 3; ++c) {
+                    X[a][c] = in->COORDS3D[3 * (int64_t)(sctr[a] - 1) + c];
+                    for (int f = 0; f < nb_frq; ++f)
+                        ue[(f * 3 + c) * n + a] = VECT[f + nb_frq * (c + 3 * (int64_t)(sctr[a] - 1))];
+                }
+            memset(ge, 0, sizeof(double) * nb_frq * 3 * n);
+            const int32_t *nijk = P.nijk + 3 * e;
+            double half = 1.0;
+            for (int d = 0; d < 3; ++d) half *= 0.5 * (P.U[d][nijk[d]] - P.U[d][nijk[d] - 1]);
+            for (int gi = 0; gi < ng; ++gi)
+                for (int gj = 0; gj < ng; ++gj)
+                    for (int gk = 0; gk < ng; ++gk) {
+                        const double pt[3] = {gx[gi], gx[gj], gx[gk]};
+                        double AJ[3][3], detJ;
+                        oracle_shap_xi(&P, e, in->COORDS3D, pt, R, dRdx, AJ, &detJ, dRdxi);
+                        const double dJ = gw[gi] * gw[gj] * gw[gk] * half;
+                        double A[3][3] = {{0}};   /* A[c][i]: component c of a_i */
+                        for (int a = 0; a < n; ++a)
+                            for (int i = 0; i < 3; ++i)
+                                for (int c = 0; c < 3; ++c) A[c][i] += dRdxi[a][i] * X[a][c];
+                        double a1[3], a2[3], a3[3], x12[3], x23[3], x31[3];
+                        for (int c = 0; c < 3; ++c) { a1[c] = A[c][0]; a2[c] = A[c][1]; a3[c] = A[c][2]; }
+                        cross(a1, a2, x12);
+                        cross(a2, a3, x23);
+                        cross(a3, a1, x31);
+                        const double area = x12[0] * a3[0] + x12[1] * a3[1] + x12[2] * a3[2];
+                        const double *cx[3] = {x23, x31, x12};
+                        double g[3][3], gi3[3][3];
+                        for (int i = 0; i < 3; ++i)
+                            for (int j = 0; j < 3; ++j)
+                                g[i][j] = A[0][i] * A[0][j] + A[1][i] * A[1][j] + A[2][i] * A[2][j];
+                        const double dg = g[0][0] * (g[1][1] * g[2][2] - g[1][2] * g[2][1])
+                                        - g[0][1] * (g[1][0] * g[2][2] - g[1][2] * g[2][0])
+                                        + g[0][2] * (g[1][0] * g[2][1] - g[1][1] * g[2][0]);
+                        gi3[0][0] = (g[1][1] * g[2][2] - g[1][2] * g[2][1]) / dg;
+                        gi3[0][1] = (g[0][2] * g[2][1] - g[0][1] * g[2][2]) / dg;
+                        gi3[0][2] = (g[0][1] * g[1][2] - g[0][2] * g[1][1]) / dg;
+                        gi3[1][0] = (g[1][2] * g[2][0] - g[1][0] * g[2][2]) / dg;
+                        gi3[1][1] = (g[0][0] * g[2][2] - g[0][2] * g[2][0]) / dg;
+                        gi3[1][2] = (g[0][2] * g[1][0] - g[0][0] * g[1][2]) / dg;
+                        gi3[2][0] = (g[1][0] * g[2][1] - g[1][1] * g[2][0]) / dg;
+                        gi3[2][1] = (g[0][1] * g[2][0] - g[0][0] * g[2][1]) / dg;
+                        gi3[2][2] = (g[0][0] * g[1][1] - g[0][1] * g[1][0]) / dg;
+                        double H[6][6];
+                        for (int kl = 0; kl < 6; ++kl)
+                            for (int ij = 0; ij < 6; ++ij) {
+                                const int i = vg[ij][0], j = vg[ij][1], k = vg[kl][0], l = vg[kl][1];
+                                H[ij][kl] = lam * gi3[i][j] * gi3[k][l] + mu * (gi3[i][k] * gi3[j][l] + gi3[i][l] * gi3[j][k]);
+                            }
+                        for (int f = 0; f < nb_frq; ++f) {
+                            const double *uf = ue + (size_t)f * 3 * n;
+                            double u[3] = {0, 0, 0}, du[3][3] = {{0}};   /* du[c][i] = d u_c / d xi_i */
+                            for (int a = 0; a < n; ++a)
+                                for (int c = 0; c < 3; ++c) {
+                                    u[c] += R[a] * uf[c * n + a];
+                                    for (int i = 0; i < 3; ++i) du[c][i] += dRdxi[a][i] * uf[c * n + a];
+                                }
+                            double eps[6], sig[6], work = 0.0;
+                            for (int ij = 0; ij < 6; ++ij) {
+                                const int i = vg[ij][0], j = vg[ij][1];
+                                if (i == j)
+                                    eps[ij] = A[0][i] * du[0][i] + A[1][i] * du[1][i] + A[2][i] * du[2][i];
+                                else
+                                    eps[ij] = A[0][i] * du[0][j] + A[1][i] * du[1][j] + A[2][i] * du[2][j]
+                                            + A[0][j] * du[0][i] + A[1][j] * du[1][i] + A[2][j] * du[2][i];
+                            }
+                            for (int ij = 0; ij < 6; ++ij) {
+                                sig[ij] = 0.0;
+                                for (int kl = 0; kl < 6; ++kl) sig[ij] += H[ij][kl] * eps[kl];
+                            }
+                            for (int ij = 0; ij < 6; ++ij) work += eps[ij] * sig[ij];
+                            const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+                            mass[f] += P.rho * uu * area * dJ;
+                            double *gf = ge + (size_t)f * 3 * n;
+                            for (int a = 0; a < n; ++a) {
+                                double dA[3] = {0, 0, 0}, dE[3] = {0, 0, 0}, dC[3] = {0, 0, 0};
+                                for (int i = 0; i < 3; ++i)
+                                    for (int c = 0; c < 3; ++c) dA[c] += cx[i][c] * dRdxi[a][i];
+                                for (int ij = 0; ij < 6; ++ij) {
+                                    const int i = vg[ij][0], j = vg[ij][1];
+                                    for (int c = 0; c < 3; ++c) {
+                                        if (i == j) dE[c] += sig[ij] * du[c][i] * dRdxi[a][i];
+                                        else dE[c] += sig[ij] * (du[c][i] * dRdxi[a][j] + du[c][j] * dRdxi[a][i]);
+                                    }
+                                }
+                                /* derivative of the metric and of its inverse, per CP component */
+                                double dgi[3][3][3];   /* dgi[c][i][j] */
+                                for (int c = 0; c < 3; ++c) {
+                                    double dgc[3][3];
+                                    for (int i = 0; i < 3; ++i)
+                                        for (int j = 0; j < 3; ++j) dgc[i][j] = dRdxi[a][i] * A[c][j] + A[c][i] * dRdxi[a][j];
+                                    for (int i = 0; i < 3; ++i)
+                                        for (int j = 0; j < 3; ++j) {
+                                            double t = 0.0;
+                                            for (int k = 0; k < 3; ++k)
+                                                for (int l = 0; l < 3; ++l) t -= gi3[i][k] * gi3[l][j] * dgc[k][l];
+                                            dgi[c][i][j] = t;
+                                        }
+                                }
+                                for (int kl = 0; kl < 6; ++kl)
+                                    for (int ij = 0; ij < 6; ++ij) {
+                                        const int i = vg[ij][0], j = vg[ij][1], k = vg[kl][0], l = vg[kl][1];
+                                        for (int c = 0; c < 3; ++c) {
+                                            const double dh = lam * (dgi[c][i][j] * gi3[k][l] + gi3[i][j] * dgi[c][k][l])
+                                                            + mu * (dgi[c][i][k] * gi3[j][l] + gi3[i][k] * dgi[c][j][l]
+                                                                    + dgi[c][i][l] * gi3[j][k] + gi3[i][l] * dgi[c][j][k]);
+                                            dC[c] += dh * eps[ij] * eps[kl];
+                                        }
+                                    }
+                                for (int c = 0; c < 3; ++c)
+                                    gf[c * n + a] += (work - VALS[f] * P.rho * uu) * dA[c] * dJ
+                                                   + (2.0 * dE[c] + dC[c]) * area * dJ;
+                            }
+                        }
+                    }
+            for (int a = 0; a < n; ++a)
+                for (int c = 0; c < 3; ++c)
+                    for (int f = 0; f < nb_frq; ++f)
+                        gradV[f + nb_frq * (c + 3 * (int64_t)(sctr[a] - 1))] += ge[((size_t)f * 3 + c) * n + a];
+        }
+        free(R); free(dRdx); free(dRdxi); free(ge); free(ue); free(X);
+    }
+    for (int64_t k = 0; k < ncp * 3; ++k)
+        for (int f = 0; f < nb_frq; ++f) gradV[f + nb_frq * k] /= mass[f];
+    free(mass);
     return 0;
 }

@@ -138,6 +138,30 @@ class Assembler:
                                                     _stream_handle(stream)))
         return M, r, c
 
+    def run_vib_grad(self, vect, vals, gradV=None, stream=None):
+        """Natural-frequency shape gradient (gradvibration_fullAN.f:22-189): vect holds nb_frq
+        eigenvectors (nb_frq, 3, nb_cp) (numpy or CUDA; column-major like the f2py argument, or
+        any array of that shape), vals their eigenvalues. Returns the CUDA tensor of
+        d lambda / d P in the same column-major layout, viewed as (nb_frq, 3, nb_cp)."""
+        vals = np.ascontiguousarray(np.asarray(vals, np.float64).reshape(-1))
+        nf = vals.size
+        if isinstance(vect, torch.Tensor):
+            v = vect.detach().to(device="cuda", dtype=torch.float64)
+        else:
+            v = torch.as_tensor(np.asarray(vect, np.float64), device="cuda")
+        if tuple(v.shape) != (nf, 3, self.nb_cp):
+            raise ValueError("vect must be (nb_frq, 3, nb_cp) = (%d, 3, %d), got %s" % (nf, self.nb_cp, tuple(v.shape)))
+        vf = v.permute(2, 1, 0).contiguous()           # column-major (nb_frq, 3, nb_cp) in memory
+        if gradV is None:
+            gradV = torch.empty((self.nb_cp, 3, nf), dtype=torch.float64, device="cuda")
+        if gradV.dtype != torch.float64 or not gradV.is_cuda or gradV.numel() != nf * 3 * self.nb_cp:
+            raise ValueError("gradV must be a CUDA float64 tensor of nb_frq * 3 * nb_cp entries")
+        _lib.check(_lib.lib().yeti_asm_run_vib_grad(self._h, C.c_void_p(vf.data_ptr()), vals.ctypes.data_as(C.c_void_p),
+                                                    nf, C.c_void_p(gradV.data_ptr()), _stream_handle(stream)))
+        if stream is not None:
+            vf.record_stream(stream)   # the temporary stays allocated until that stream has used it
+        return gradV.reshape(self.nb_cp, 3, nf).permute(2, 1, 0)
+
     def _check_out(self, K, r, c, F):
         if K.dtype != torch.float64 or not K.is_cuda or not K.is_contiguous() or K.numel() < self.nb_data:
             raise ValueError("Kdata must be a contiguous CUDA float64 tensor with >= %d entries" % self.nb_data)

@@ -600,6 +600,12 @@ struct yeti_asm {
     std::vector<hipEvent_t> ev;
     int32_t ev_used = 0;
     int32_t n_loaded_total = 0;
+    // natural-frequency gradient (built on first use): CP -> element-node entries of the active
+    // U1 elements in CSR form, per-entry and per-element partials, the per-mode mass
+    yeti::DevBuf<int32_t> vib_ptr;
+    yeti::DevBuf<int64_t> vib_ent;
+    yeti::DevBuf<double> vib_eg, vib_em, vib_mass;
+    int64_t vib_nent = -1, vib_nslot = 0;
     ~yeti_asm() {
         for (auto *p : patches) delete p;
         for (auto e : ev) (void)hipEventDestroy(e);
@@ -1009,6 +1015,388 @@ static int run_mass(yeti_asm *h, double *Md, int32_t *Mr, int32_t *Mc, hipStream
     return 0;
 }
 
+// ---------------------------------------------------------------------------------------
+// Natural-frequency shape gradient (gradvibration_fullAN.f:22-189 with gradUELMASSMAT1,
+// gradUELMASSMAT_solid.f:22-332), solid patches, up to VIB_MAXF modes per launch. For mode f with
+// eigenvalue lambda_f and eigenvector u, element e adds to its node a
+//   g_a[m] = sum_q dv_q [ (eps:sig - lambda_f rho |u|^2) dR_a/dx_m - 2 sum_ik sig_ik du_i/dx_m dR_a/dx_k ],
+// the derivative of u^T (K - lambda_f M) u with respect to control point a (the reference writes
+// the same derivative in the covariant basis as its Jacobian, strain and metric terms), and the
+// mass rho |u|^2 dv; the CP gradient is the sum over elements divided by the total mass per mode.
+// One 256-thread workgroup per active element, Gauss points in chunks of VQC:
+//   1. bases, one thread per (point, direction);
+//   2. W, dW and dx/du of a point by 256 / VQC threads and a shuffle sum, then du/dx and dv;
+//   3. R and dR/dx of every (node, point) of the chunk in LDS;
+//   4. per (point, mode): u, du/dx, sig, the tensor dv (W' I - 2 (du/dx)^T sig) and the mass;
+//   5. per (node, mode): g += tensor . dR/dx, accumulated in LDS over the chunks.
+constexpr int VIB_MAXF = 4, VQC = 16;
+
+struct VibModes {
+    const double *vect;            // (nb_frq, 3, nb_cp) column-major
+    int32_t nf_total, f0, nf;      // modes f0 .. f0 + nf - 1 of nf_total
+    double vals[VIB_MAXF];
+};
+
+template <int P, int NG>
+__global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho, VibModes vm, int64_t ebase,
+                                                       int64_t mbase, double *__restrict__ eg,
+                                                       double *__restrict__ em) {
+    constexpr int NP1 = P + 1, NN = NP1 * NP1 * NP1, NQ = NG * NG * NG, G = 256 / VQC;
+    __shared__ double xw[NN][4];
+    __shared__ double uf[VIB_MAXF][NN][3];
+    __shared__ double Bq[VQC][3][NP1], dBq[VQC][3][NP1];
+    __shared__ double gq[VQC][14];     // du/dx (9, [c][k] = du_c/dx_k), dv, 1/W, dW/du / W (3)
+    __shared__ double Rx[NN][VQC][4];  // R, dR/dx
+    __shared__ double Tq[VQC][VIB_MAXF][9];
+    __shared__ double gacc[VIB_MAXF][NN][3];
+    __shared__ double mpart[VQC][VIB_MAXF];
+    __shared__ double hel[3];
+    const int t = threadIdx.x;
+    const int nf = vm.nf;
+    const int qq = t / G, part = t % G;
+    for (int64_t gi = blockIdx.x; gi < pa.n_active; gi += gridDim.x) {
+        const int e = pa.elems[gi];
+        __syncthreads();   // the previous element is done with the LDS arrays
+        for (int a = t; a < NN; a += 256) {
+            const int64_t cp = (int64_t)pa.ien[(int64_t)NN * e + a] - 1;
+            xw[a][0] = pa.coords[3 * cp];
+            xw[a][1] = pa.coords[3 * cp + 1];
+            xw[a][2] = pa.coords[3 * cp + 2];
+            xw[a][3] = pa.weight[(int64_t)NN * e + a];
+            for (int f = 0; f < nf; ++f)
+                for (int c = 0; c < 3; ++c)
+                    uf[f][a][c] = vm.vect[(vm.f0 + f) + (int64_t)vm.nf_total * (c + 3 * cp)];
+        }
+        for (int x = t; x < VIB_MAXF * NN * 3; x += 256) (&gacc[0][0][0])[x] = 0.0;
+        double macc = 0.0;   // this thread's (point, mode) share of the element mass
+        for (int q0 = 0; q0 < NQ; q0 += VQC) {
+            const int nq = NQ - q0 < VQC ? NQ - q0 : VQC;
+            __syncthreads();
+            // 1. bases of direction `part` at point qq (shap.f90:75-178 for the parametric form)
+            if (part < 3 && qq < nq) {
+                const int q = q0 + qq, d = part;
+                const int gd = d == 0 ? q / (NG * NG) : d == 1 ? (q / NG) % NG : q % NG;
+                const double *U = d == 0 ? pa.U0 : d == 1 ? pa.U1 : pa.U2;
+                const int span1 = pa.nijk[3 * (int64_t)e + d];
+                const double u0 = U[span1 - 1], u1 = U[span1];
+                const double u = ((u1 - u0) * pa.g.x[gd] + (u1 + u0)) * 0.5;
+                double N[NP1], dN[NP1];
+                bspline_ders<P>(span1 - 1, u, U, N, dN);
+#pragma unroll
+                for (int i = 0; i < NP1; ++i) {
+                    Bq[qq][d][i] = N[P - i];
+                    dBq[qq][d][i] = dN[P - i];
+                }
+                if (qq == 0) hel[d] = 0.5 * (u1 - u0);
+            }
+            __syncthreads();
+            // 2. W, dW/du and dx/du of point qq: nodes part, part + G, ... then a shuffle sum
+            {
+                double s[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) s[k] = 0.0;
+                if (qq < nq) {
+#pragma unroll 1
+                    for (int a = part; a < NN; a += G) {
+                        const int i = a % NP1, j = (a / NP1) % NP1, k = a / (NP1 * NP1);
+                        const double w = xw[a][3], x0 = xw[a][0], x1 = xw[a][1], x2 = xw[a][2];
+                        const double B0 = Bq[qq][0][i], B1 = Bq[qq][1][j], B2 = Bq[qq][2][k];
+                        const double b12 = B1 * B2 * w;
+                        const double Na = B0 * b12;
+                        const double d0 = dBq[qq][0][i] * b12;
+                        const double d1 = B0 * dBq[qq][1][j] * B2 * w;
+                        const double d2 = B0 * B1 * dBq[qq][2][k] * w;
+                        s[0] += Na;
+                        s[1] += d0; s[2] += d1; s[3] += d2;
+                        s[4] += x0 * Na; s[5] += x1 * Na; s[6] += x2 * Na;
+                        s[7] += x0 * d0; s[8] += x0 * d1; s[9] += x0 * d2;
+                        s[10] += x1 * d0; s[11] += x1 * d1; s[12] += x1 * d2;
+                        s[13] += x2 * d0; s[14] += x2 * d1; s[15] += x2 * d2;
+                    }
+                }
+#pragma unroll
+                for (int o = 1; o < G; o <<= 1)   // the G threads of a point are adjacent lanes
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) s[k] += __shfl_xor(s[k], o, 64);
+                if (part == 0) {
+                    double *gp = gq[qq];
+                    if (qq < nq) {
+                        const int q = q0 + qq;
+                        const double Wi = 1.0 / s[0];
+                        const double dw[3] = {s[1] * Wi, s[2] * Wi, s[3] * Wi};
+                        double J[3][3];   // dx_r / du_c
+#pragma unroll
+                        for (int r = 0; r < 3; ++r)
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) J[r][c] = (s[7 + 3 * r + c] - s[4 + r] * dw[c]) * Wi;
+                        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) +
+                                           J[0][1] * (J[1][2] * J[2][0] - J[1][0] * J[2][2]) +
+                                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+                        const double di = 1.0 / det;
+                        gp[0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * di;
+                        gp[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * di;
+                        gp[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * di;
+                        gp[3] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * di;
+                        gp[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * di;
+                        gp[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * di;
+                        gp[6] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * di;
+                        gp[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * di;
+                        gp[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * di;
+                        const int g0 = q / (NG * NG), g1 = (q / NG) % NG, g2 = q % NG;
+                        gp[9] = (pa.g.w[g0] * pa.g.w[g1] * pa.g.w[g2]) * (det * hel[0] * hel[1] * hel[2]);
+                        gp[10] = Wi;
+                        gp[11] = dw[0];
+                        gp[12] = dw[1];
+                        gp[13] = dw[2];
+                    } else {
+                        for (int k = 0; k < 14; ++k) gp[k] = 0.0;
+                    }
+                }
+            }
+            __syncthreads();
+            // 3. R = N w / W and dR/dx = (dN w / W - R dW / W) du/dx
+            for (int x = t; x < NN * VQC; x += 256) {
+                const int a = x / VQC, c = x % VQC;
+                double r4[4] = {0.0, 0.0, 0.0, 0.0};
+                if (c < nq) {
+                    const int i = a % NP1, j = (a / NP1) % NP1, k = a / (NP1 * NP1);
+                    const double w = xw[a][3], Wi = gq[c][10];
+                    const double B0 = Bq[c][0][i], B1 = Bq[c][1][j], B2 = Bq[c][2][k];
+                    const double R = B0 * B1 * B2 * w * Wi;
+                    const double du[3] = {dBq[c][0][i] * B1 * B2 * w * Wi - R * gq[c][11],
+                                          B0 * dBq[c][1][j] * B2 * w * Wi - R * gq[c][12],
+                                          B0 * B1 * dBq[c][2][k] * w * Wi - R * gq[c][13]};
+                    r4[0] = R;
+                    for (int kk = 0; kk < 3; ++kk)
+                        r4[1 + kk] = du[0] * gq[c][kk] + du[1] * gq[c][3 + kk] + du[2] * gq[c][6 + kk];
+                }
+                for (int kk = 0; kk < 4; ++kk) Rx[a][c][kk] = r4[kk];
+            }
+            __syncthreads();
+            // 4. per (point, mode): u, du/dx, stress, the tensor and the mass
+            if (t < VQC * nf) {
+                const int c = t % VQC, f = t / VQC;
+                double T[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                if (c < nq) {
+                    double u[3] = {0.0, 0.0, 0.0}, D[3][3] = {};   // D[i][k] = du_i/dx_k
+#pragma unroll 1
+                    for (int a = 0; a < NN; ++a) {
+                        const double R = Rx[a][c][0], g0 = Rx[a][c][1], g1 = Rx[a][c][2], g2 = Rx[a][c][3];
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) {
+                            const double ua = uf[f][a][i];
+                            u[i] += R * ua;
+                            D[i][0] += ua * g0;
+                            D[i][1] += ua * g1;
+                            D[i][2] += ua * g2;
+                        }
+                    }
+                    const double lam = pa.lam, mu = pa.mu, dv = gq[c][9];
+                    const double tr = D[0][0] + D[1][1] + D[2][2];
+                    double sig[3][3], W = 0.0;
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            const double eik = 0.5 * (D[i][k] + D[k][i]);
+                            sig[i][k] = 2.0 * mu * eik + (i == k ? lam * tr : 0.0);
+                            W += eik * sig[i][k];
+                        }
+                    const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+                    const double Wp = W - vm.vals[f] * rho * uu;
+#pragma unroll
+                    for (int m = 0; m < 3; ++m)
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            double v = m == k ? Wp : 0.0;
+                            v -= 2.0 * (D[0][m] * sig[0][k] + D[1][m] * sig[1][k] + D[2][m] * sig[2][k]);
+                            T[3 * m + k] = dv * v;
+                        }
+                    macc += rho * uu * dv;
+                }
+                for (int k = 0; k < 9; ++k) Tq[c][f][k] = T[k];
+            }
+            __syncthreads();
+            // 5. per (node, mode): g[m] += sum_q T[m][k] dR/dx_k
+            for (int x = t; x < NN * nf; x += 256) {
+                const int a = x % NN, f = x / NN;
+                double g[3] = {0.0, 0.0, 0.0};
+                for (int c = 0; c < nq; ++c) {
+                    const double g0 = Rx[a][c][1], g1 = Rx[a][c][2], g2 = Rx[a][c][3];
+#pragma unroll
+                    for (int m = 0; m < 3; ++m)
+                        g[m] += Tq[c][f][3 * m] * g0 + Tq[c][f][3 * m + 1] * g1 + Tq[c][f][3 * m + 2] * g2;
+                }
+                for (int m = 0; m < 3; ++m) gacc[f][a][m] += g[m];
+            }
+        }
+        if (t < VQC * VIB_MAXF) mpart[t % VQC][t / VQC] = t < VQC * nf ? macc : 0.0;
+        __syncthreads();
+        const int64_t eb = ebase + gi * NN;
+        for (int x = t; x < NN * VIB_MAXF * 3; x += 256) {
+            const int a = x / (VIB_MAXF * 3), r = x % (VIB_MAXF * 3), f = r / 3, m = r % 3;
+            eg[(eb + a) * (VIB_MAXF * 3) + r] = f < nf ? gacc[f][a][m] : 0.0;
+        }
+        if (t < VIB_MAXF) {
+            double ms = 0.0;
+            for (int c = 0; c < VQC; ++c) ms += mpart[c][t];
+            em[(mbase + gi) * VIB_MAXF + t] = ms;
+        }
+    }
+}
+
+// Total mass of mode f over every element slot, in a fixed order (one block per mode).
+__global__ void __launch_bounds__(256) vib_mass_kernel(const double *em, int64_t nslot, double *mass) {
+    __shared__ double sh[256];
+    const int f = blockIdx.x;
+    double v = 0.0;
+    for (int64_t i = threadIdx.x; i < nslot; i += 256) v += em[i * VIB_MAXF + f];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mass[f] = sh[0];
+}
+
+// gradV(f0 + f, m, cp) = sum over the element-node entries of cp, in entry order, / mass[f]
+__global__ void __launch_bounds__(256) vib_gather_kernel(int32_t nb_cp, const int32_t *ptr, const int64_t *ent,
+                                                         const double *eg, const double *mass, VibModes vm,
+                                                         double *gradV) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < 3 * (int64_t)nb_cp;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t cp = x / 3;
+        const int m = (int)(x % 3);
+        for (int f = 0; f < vm.nf; ++f) {
+            double s = 0.0;
+            for (int32_t j = ptr[cp]; j < ptr[cp + 1]; ++j) s += eg[ent[j] * (VIB_MAXF * 3) + 3 * f + m];
+            gradV[(vm.f0 + f) + (int64_t)vm.nf_total * (m + 3 * cp)] = s / mass[f];
+        }
+    }
+}
+
+template <int P, int NG>
+static int launch_vib(yeti_asm *h, PatchDev &pd, const VibModes &vm, int64_t ebase, int64_t mbase, hipStream_t st) {
+    PatchArgs pa{};
+    pa.coords = h->coords.p;
+    pa.ien = pd.ien.p;
+    pa.weight = pd.weight.p;
+    pa.nijk = pd.nijk.p;
+    pa.U0 = pd.U[0].p;
+    pa.U1 = pd.U[1].p;
+    pa.U2 = pd.U[2].p;
+    pa.elems = pd.elems.p;
+    pa.n_active = pd.n_active;
+    pa.lam = pd.lam;
+    pa.mu = pd.mu;
+    gauss_legendre(NG, pa.g.x, pa.g.w);
+    const unsigned grid = (unsigned)std::min<int64_t>(pd.n_active, 1 << 20);
+    hipLaunchKernelGGL((vib_grad_kernel<P, NG>), dim3(grid), dim3(256), 0, st, pa, pd.rho, vm, ebase, mbase,
+                       h->vib_eg.p, h->vib_em.p);
+    YETI_HIP(hipGetLastError());
+    return 0;
+}
+
+// CP -> element-node entries of the active U1 elements (entry = patch base + gi nnode + a), in
+// patch, element and node order; buffers for the per-entry and per-element partials.
+static int vib_setup(yeti_asm *h) {
+    if (h->vib_nent >= 0) return 0;
+    std::vector<int32_t> cnt(h->nb_cp + 1, 0);
+    std::vector<std::vector<int32_t>> ien_h(h->patches.size()), el_h(h->patches.size());
+    int64_t nent = 0, nslot = 0;
+    for (size_t k = 0; k < h->patches.size(); ++k) {
+        PatchDev &pd = *h->patches[k];
+        if (!pd.u1 || pd.n_active == 0) continue;
+        ien_h[k].resize((size_t)pd.nnode * pd.nelem);
+        el_h[k].resize(pd.n_active);
+        YETI_HIP(hipMemcpy(ien_h[k].data(), pd.ien.p, ien_h[k].size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        YETI_HIP(hipMemcpy(el_h[k].data(), pd.elems.p, el_h[k].size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (int32_t gi = 0; gi < pd.n_active; ++gi)
+            for (int a = 0; a < pd.nnode; ++a) {
+                const int32_t id = ien_h[k][(size_t)pd.nnode * el_h[k][gi] + a];   // 1-based
+                if (id < 1 || id > h->nb_cp)
+                    return set_error(YETI_ERR_INVALID_ARGUMENT, "vibration gradient: IEN entry %d out of range", id);
+                ++cnt[id];
+            }
+        nent += (int64_t)pd.n_active * pd.nnode;
+        nslot += pd.n_active;
+    }
+    if (nent > INT32_MAX) return set_error(YETI_ERR_SIZE_OVERFLOW, "vibration gradient: %lld element nodes", (long long)nent);
+    std::vector<int32_t> ptr(h->nb_cp + 1, 0);
+    for (int32_t c = 0; c < h->nb_cp; ++c) ptr[c + 1] = ptr[c] + cnt[c + 1];
+    std::vector<int32_t> pos(ptr.begin(), ptr.end() - 1);
+    std::vector<int64_t> ent(std::max<int64_t>(nent, 1));
+    int64_t base = 0;
+    for (size_t k = 0; k < h->patches.size(); ++k) {
+        PatchDev &pd = *h->patches[k];
+        if (!pd.u1 || pd.n_active == 0) continue;
+        for (int32_t gi = 0; gi < pd.n_active; ++gi)
+            for (int a = 0; a < pd.nnode; ++a) {
+                const int32_t cp = ien_h[k][(size_t)pd.nnode * el_h[k][gi] + a] - 1;
+                ent[pos[cp]++] = base + (int64_t)gi * pd.nnode + a;
+            }
+        base += (int64_t)pd.n_active * pd.nnode;
+    }
+    YETI_HIP(h->vib_ptr.upload(ptr.data(), ptr.size()));
+    YETI_HIP(h->vib_ent.upload(ent.data(), ent.size()));
+    YETI_HIP(h->vib_eg.alloc((size_t)std::max<int64_t>(nent, 1) * VIB_MAXF * 3));
+    YETI_HIP(h->vib_em.alloc((size_t)std::max<int64_t>(nslot, 1) * VIB_MAXF));
+    YETI_HIP(h->vib_mass.alloc(VIB_MAXF));
+    h->vib_nent = nent;
+    h->vib_nslot = nslot;
+    return 0;
+}
+
+static int run_vib_grad(yeti_asm *h, const double *vect, const double *vals, int32_t nf_total, double *gradV,
+                        hipStream_t st) {
+    for (auto *pd : h->patches) {
+        if (!pd->u1 || pd->n_active == 0) continue;
+        if (pd->p[1] != pd->p[0] || pd->p[2] != pd->p[0])
+            return set_error(YETI_ERR_UNSUPPORTED_DEGREE, "vibration gradient: degrees must be equal per direction");
+        const int key = pd->p[0] * 16 + pd->ng;
+        if (key != 1 * 16 + 2 && key != 2 * 16 + 3 && key != 3 * 16 + 4 && key != 4 * 16 + 5)
+            return set_error(YETI_ERR_UNSUPPORTED_DEGREE, "vibration gradient: no kernel for p=%d ng=%d", pd->p[0], pd->ng);
+    }
+    int rc = vib_setup(h);
+    if (rc) return rc;
+    for (int32_t f0 = 0; f0 < nf_total; f0 += VIB_MAXF) {
+        VibModes vm{};
+        vm.vect = vect;
+        vm.nf_total = nf_total;
+        vm.f0 = f0;
+        vm.nf = std::min<int32_t>(VIB_MAXF, nf_total - f0);
+        for (int f = 0; f < vm.nf; ++f) vm.vals[f] = vals[f0 + f];
+        int64_t ebase = 0, mbase = 0;
+        for (auto *pd : h->patches) {
+            if (!pd->u1 || pd->n_active == 0) continue;
+            switch (pd->p[0] * 16 + pd->ng) {
+            case 1 * 16 + 2: rc = launch_vib<1, 2>(h, *pd, vm, ebase, mbase, st); break;
+            case 2 * 16 + 3: rc = launch_vib<2, 3>(h, *pd, vm, ebase, mbase, st); break;
+            case 3 * 16 + 4: rc = launch_vib<3, 4>(h, *pd, vm, ebase, mbase, st); break;
+            default: rc = launch_vib<4, 5>(h, *pd, vm, ebase, mbase, st); break;
+            }
+            if (rc) return rc;
+            ebase += (int64_t)pd->n_active * pd->nnode;
+            mbase += pd->n_active;
+        }
+        if (h->vib_nslot > 0) {
+            hipLaunchKernelGGL(vib_mass_kernel, dim3(vm.nf), dim3(256), 0, st, (const double *)h->vib_em.p, h->vib_nslot,
+                               h->vib_mass.p);
+        } else {   // no active element: zero mass, as the reference's empty sum
+            hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(256), 0, st, h->vib_mass.p, (int32_t *)nullptr,
+                               (int32_t *)nullptr, (int64_t)VIB_MAXF);
+        }
+        YETI_HIP(hipGetLastError());
+        const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((3 * (int64_t)h->nb_cp + 255) / 256, 65536));
+        hipLaunchKernelGGL(vib_gather_kernel, dim3(blocks), dim3(256), 0, st, h->nb_cp, (const int32_t *)h->vib_ptr.p,
+                           (const int64_t *)h->vib_ent.p, (const double *)h->vib_eg.p, (const double *)h->vib_mass.p,
+                           vm, gradV);
+        YETI_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
 // u^T F shape derivative of the face loads: g = 2 u^T dF/dX (3 per CP), gathered like F.
 static int run_load_grad(yeti_asm *h, const double *u, double *g, hipStream_t st) {
     if (h->body_loads)
@@ -1370,6 +1758,13 @@ int32_t yeti_asm_run_coo_values(yeti_asm *h, double *Kd, void *stream) {
 int32_t yeti_asm_run_mass_coo(yeti_asm *h, double *Md, int32_t *Mr, int32_t *Mc, void *stream) {
     if (!h || !Md || (!Mr) != (!Mc)) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
     return yeti::run_mass(h, Md, Mr, Mc, (hipStream_t)stream);
+}
+
+int32_t yeti_asm_run_vib_grad(yeti_asm *h, const double *vect, const double *vals, int32_t nb_frq, double *gradV,
+                              void *stream) {
+    if (!h || !vect || !vals || !gradV || nb_frq < 1)
+        return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument or nb_frq < 1");
+    return yeti::run_vib_grad(h, vect, vals, nb_frq, gradV, (hipStream_t)stream);
 }
 
 int32_t yeti_asm_run_rhs(yeti_asm *h, double *F, void *stream) {
