@@ -1,0 +1,157 @@
+"""Model files and k-refinement (yeti_amd.model.IGAparametrization): the reference's own bench
+models end to end.
+
+Fixtures (tests/golden/models/) are the input files of the reference's benchmarks
+benchs/dload/beam-dist.{inp,NB} and benchs/volume_force/volume_force_3D.{inp,NB}, copied as
+data. Pins:
+* refinement is exact: the refined NURBS map equals the coarse one at random parametric points
+  (geometry and weights; a wrong knot multiplicity, Greville point or homogeneous/Cartesian mix-up
+  moves it);
+* the benchmarks' own published checks, with the reference's refinement and tolerances:
+  benchs/dload/test_dload_beam_dist.py (deflexion at CP (20, 0, 0) within 2e-2 of the analytic
+  -1142.857; pressure U40 on the face of one coarse element, mapped to the refined elements on
+  that face) and benchs/volume_force/test_volume_force_3d.py (displacement at xi = (1, 1, 0)
+  within 2e-3 of the Abaqus reference; body force U102 with a direction). On the CPU the
+  assembly is the oracle's; on the GPU it is the drop-in stack (stiffness on the GPU, the
+  callers' scipy solve, reconstruction and evaldisp on the GPU).
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+from oracle import oracle as O
+from yeti_amd.model import IGAparametrization
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MODELS = os.path.join(HERE, "golden", "models")
+VF3D_REF = np.array([-9.96639E-5, -1.01212, -6.42538])   # benchs/volume_force/test_volume_force_3d.py
+
+
+def beam_dist():
+    m = IGAparametrization(filename=os.path.join(MODELS, "beam-dist"))
+    m.refine(np.array([2, 2, 5]), np.array([1, 1, 1]))
+    return m
+
+
+def volume_force_3d():
+    m = IGAparametrization(filename=os.path.join(MODELS, "volume_force_3D"))
+    nb_deg = np.zeros((3, 1), np.intp)
+    nb_ref = np.zeros((3, 1), np.intp)
+    nb_deg[:, 0] = 1
+    nb_ref[:, 0] = (3, 4, 3)
+    m.refine(nb_ref, nb_deg)
+    return m
+
+
+def solve(m, K, r, c, F):
+    n = m.nb_dof_tot
+    Ks = sp.coo_matrix((K, (r, c)), shape=(n, n)).tocsc()
+    Kt = Ks + Ks.T
+    idof = m.ind_dof_free[:m.nb_dof_free] - 1
+    return spl.spsolve(Kt[idof, :][:, idof], F[idof])
+
+
+def full_sol(m, x):
+    U = np.zeros(m.nb_dof_tot)
+    U[m.ind_dof_free[:m.nb_dof_free] - 1] = x
+    return U.reshape(-1, 3)
+
+
+def geometry_at(d, xi):
+    """x(xi) of patch 1 as the NURBS interpolation of the control-point coordinates."""
+    X = np.asarray(d["COORDS3D"], np.float64)
+    return O.oracle_evaldisp(1, xi, X, d["IEN"], d["nb_elem_patch"], d["Nkv"], d["Ukv"], d["Nijk"], d["weight"],
+                             d["Jpqr"], d["NNODE"])
+
+
+def test_reader_coarse_models():
+    m = IGAparametrization(filename=os.path.join(MODELS, "beam-dist"))
+    d = m.get_model_dict()
+    assert m.nb_cp == 8 and m.nb_elem == 1 and d["ELT_TYPE"] == "U1"
+    assert np.array_equal(np.asarray(d["IEN"]), np.arange(8, 0, -1))
+    assert np.allclose(d["MATERIAL_PROPERTIES"].ravel(), [210000.0, 0.3])
+    assert list(d["JDLType"]) == [40] and list(d["ADLMAG"]) == [1000.0]
+    assert d["nb_bc"] == 3 and m.nb_dof_free == 12
+    v = IGAparametrization(filename=os.path.join(MODELS, "volume_force_3D")).get_model_dict()
+    assert list(v["JDLType"]) == [102] and np.allclose(v["load_additionalInfos"], [0.0, 0.0, -1.0])
+    assert np.allclose(v["RHO"], [7.8e-9])
+
+
+@pytest.mark.parametrize("build", [beam_dist, volume_force_3d])
+def test_refinement_is_exact(build):
+    name = build.__name__
+    coarse = IGAparametrization(filename=os.path.join(MODELS, "beam-dist" if name == "beam_dist" else "volume_force_3D"))
+    fine = build()
+    xi = np.random.default_rng(3).uniform(0.0, 1.0, (200, 3))
+    xc = geometry_at(coarse.get_model_dict(), xi)
+    xf = geometry_at(fine.get_model_dict(), xi)
+    assert np.abs(xf - xc).max() <= 1e-12 * np.abs(xc).max()
+    d = fine.get_model_dict()
+    assert d["nb_elem"] == int(np.prod([np.unique(u).size - 1 for u in fine._Ukv[0]]))
+    assert np.asarray(d["NBINT"])[0] == 27
+
+
+def test_bench_dload_beam_dist_oracle():
+    m = beam_dist()
+    K, r, c, F = O.oracle_sys_linmat(m.get_inputs4system_elemStorage())
+    sol = full_sol(m, solve(m, K, r, c, F))
+    idx = np.nonzero(np.all(m.coords == np.array([[20.0], [0.0], [0.0]]), axis=0))[0]
+    assert idx.size == 1
+    assert np.allclose(sol[idx[0], 2], -1142.857, rtol=2e-2)
+
+
+def test_bench_volume_force_3d_oracle():
+    m = volume_force_3d()
+    d = m.get_model_dict()
+    K, r, c, F = O.oracle_sys_linmat(m.get_inputs4system_elemStorage())
+    sol = full_sol(m, solve(m, K, r, c, F))
+    disp = O.oracle_evaldisp(1, np.array([[1.0, 1.0, 0.0]]), sol.T, d["IEN"], d["nb_elem_patch"], d["Nkv"], d["Ukv"],
+                             d["Nijk"], d["weight"], d["Jpqr"], d["NNODE"])[0]
+    assert np.linalg.norm(disp - VF3D_REF) / np.linalg.norm(VF3D_REF) < 2e-3
+
+
+def test_refine_rejects_multipatch():
+    m = IGAparametrization(filename=os.path.join(MODELS, "beam-dist"))
+    m._nb_patch = 2
+    with pytest.raises(NotImplementedError):
+        m.refine(np.array([1, 1, 1]))
+
+
+# --------------------------------------------------------------------------------------- GPU
+
+
+@pytest.mark.gpu
+def test_gpu_bench_dload_beam_dist():
+    """benchs/dload/test_dload_beam_dist.py on the drop-in stack."""
+    from yeti_amd import reconstructionSOL as rsol
+    from yeti_amd.stiffmtrx_elemstorage import sys_linmat_lindef_static as build_stiffmatrix
+    print("[case] bench dload beam-dist", flush=True)
+    m = beam_dist()
+    data, row, col, rhs = build_stiffmatrix(*m.get_inputs4system_elemStorage())
+    x = solve(m, data, row, col, rhs)
+    sol, _ = rsol.reconstruction(**m.get_inputs4solution(x))
+    idx = np.nonzero(np.all(m.coords == np.array([[20.0], [0.0], [0.0]]), axis=0))[0]
+    assert idx.size == 1
+    assert np.allclose(sol[idx[0], 2], -1142.857, rtol=2e-2)
+    # and the same numbers as the oracle's assembly
+    K, r, c, F = O.oracle_sys_linmat(m.get_inputs4system_elemStorage())
+    assert np.array_equal(row, r) and np.array_equal(col, c)
+    assert np.abs(data - K).max() <= 1e-12 * np.abs(K).max()
+
+
+@pytest.mark.gpu
+def test_gpu_bench_volume_force_3d():
+    """benchs/volume_force/test_volume_force_3d.py on the drop-in stack."""
+    from yeti_amd import postproc as pp
+    from yeti_amd import reconstructionSOL as rsol
+    from yeti_amd.stiffmtrx_elemstorage import sys_linmat_lindef_static as build_stiffmatrix
+    print("[case] bench volume_force_3d", flush=True)
+    m = volume_force_3d()
+    data, row, col, rhs = build_stiffmatrix(*m.get_inputs4system_elemStorage())
+    x = solve(m, data, row, col, rhs)
+    sol, _ = rsol.reconstruction(**m.get_inputs4solution(x))
+    disp = pp.evaldisp(*m.get_inputs4evaldisp(np.asarray(sol).transpose(), np.array([1.0, 1.0, 0.0]), numpatch=1))
+    assert np.linalg.norm(disp - VF3D_REF) / np.linalg.norm(VF3D_REF) < 2e-3

@@ -1,0 +1,545 @@
+"""Model files and refinement for solid (U1) models: the part of YETI's pre-processing that turns
+a ``.inp`` + ``.NB`` pair into the flattened f2py argument lists
+(``IGAparametrization(filename=...)``, IGA_parametrization.py; readers
+preprocessing/geometricmodel/nurbsmodel.py and preprocessing/mechanicalmodel/read/abaqus.py) and
+the k-refinement ``IGAparametrization.refine`` (IGA_parametrization.py:2294-2437,
+IGA_refinementFcts.py).
+
+Scope: solid patches; the keywords the solid models use (``*USER ELEMENT``, ``*Node``,
+``*Element``, ``*ELSET`` / ``*NSET`` (lists or ``generate``), ``*UEL PROPERTY``, ``*MATERIAL``
+with ``*Elastic`` / ``*Density``, ``*Boundary`` (first/last dof, value), ``*Dload`` (``U<code>``,
+magnitude, additional values); ``*Cload`` lines are read but raise NotImplementedError when the
+model is flattened); one instance. Refinement is implemented for
+single-patch models: degree elevation, then levels of mid-knot insertion on every non-empty
+interval, both exact. The refined control net is obtained by interpolating the coarse NURBS (in
+homogeneous coordinates) at the Greville abscissae of the refined space, which contains it.
+Node sets that are a whole patch, faces, edges or vertices of the coarse patch are mapped to the
+same entities of the refined one (updateBoundaryConditions, IGA_refinementFcts.py:610-690);
+loaded elements to the refined elements inside them, and for a face load only to those on that
+face (updateLoads, IGA_refinementFcts.py:700-878).
+"""
+from __future__ import annotations
+
+import re
+
+import numpy as np
+
+from ._abi import F2PY_ARGS, F2PY_OPTIONAL
+
+# --------------------------------------------------------------------------------------- NB
+
+
+def _clean_lines(path):
+    """Lines without comments, a definition continued on the next line when it ends with ','
+    (nurbsmodel.py:38-61)."""
+    out, cur = [], ""
+    with open(path) as fh:
+        for raw in fh:
+            line = raw.rstrip()
+            if line.startswith("**"):
+                continue
+            cur += line
+            if line.rstrip().endswith(","):
+                continue
+            if cur.strip():
+                out.append(cur.strip())
+            cur = ""
+    if cur.strip():
+        out.append(cur.strip())
+    return out
+
+
+def _nums(line, dtype=float):
+    return np.array([dtype(float(t)) for t in re.split(r"[,\s]+", line.strip()) if t], dtype=dtype)
+
+
+def read_nb(path):
+    """Geometry of a ``.NB`` file (nurbsmodel.py:24-165)."""
+    lines = _clean_lines(path)
+    low = [l.lower() for l in lines]
+
+    def at(key):
+        for i, l in enumerate(low):
+            if l.startswith(key):
+                return i
+        raise ValueError("%s: keyword %r missing" % (path, key))
+
+    nb_patch = int(_nums(lines[at("*number of patch") + 1], int)[0])
+    dim = _nums(lines[at("*dimension") + 1], int)
+    dim = np.resize(dim, nb_patch) if dim.size == 1 else dim
+    nnode = _nums(lines[at("*number of cp by element") + 1], int)
+    nnode = np.resize(nnode, nb_patch) if nnode.size == 1 else nnode
+    nb_elem = int(_nums(lines[at("*total number of element") + 1], int)[0])
+    nb_elem_patch = _nums(lines[at("*number of element by patch") + 1], int)
+    Ukv, Nkv = [], np.zeros((3, nb_patch), np.int64)
+    for ip in range(nb_patch):
+        i = at("*patch(%d)" % (ip + 1)) + 1
+        kv = []
+        for d in range(dim[ip]):
+            Nkv[d, ip] = int(float(lines[i]))
+            kv.append(_nums(lines[i + 1]))
+            i += 2
+        Ukv.append(kv)
+    Jpqr = np.zeros((3, nb_patch), np.int64)
+    i = at("*jpqr")
+    Jpqr[: dim[0], 0] = _nums(lines[i + 1], int)
+    if nb_patch > 1:
+        if i + 2 >= len(lines) or lines[i + 2].startswith("*"):
+            Jpqr[:, 1:] = Jpqr[:, :1]
+        else:
+            for j in range(1, nb_patch):
+                Jpqr[: dim[j], j] = _nums(lines[i + 1 + j], int)
+    Nijk = np.zeros((3, nb_elem), np.int64)
+    i = at("*nijk")
+    for e in range(nb_elem):
+        v = _nums(lines[i + 1 + e], int)
+        Nijk[: v.size - 1, e] = v[1:]
+    weight = []
+    i = at("*weight")
+    k = 0
+    for ip in range(nb_patch):
+        for e in range(nb_elem_patch[ip]):
+            weight.append(_nums(lines[i + 1 + k])[1:])
+            k += 1
+    return dict(dim=dim, nnode=nnode, nb_patch=nb_patch, nb_elem=nb_elem, nb_elem_patch=nb_elem_patch, Ukv=Ukv,
+                Nkv=Nkv, Jpqr=Jpqr, Nijk=Nijk, weight=weight)
+
+
+# -------------------------------------------------------------------------------------- INP
+
+
+def _kw_opts(line):
+    opts = {}
+    for part in line.split(",")[1:]:
+        if "=" in part:
+            k, v = part.split("=", 1)
+            opts[k.strip().upper()] = v.strip()
+        elif part.strip():
+            opts[part.strip().upper()] = True
+    return line.split(",")[0].strip().upper(), opts
+
+
+def read_inp(path):
+    """Mechanical settings of a ``.inp`` file (abaqus.py: user elements, nodes, elements, sets,
+    UEL properties, materials, boundary conditions, distributed and concentrated loads)."""
+    with open(path) as fh:
+        raw = [l.rstrip("\n") for l in fh]
+    blocks = []   # (keyword, opts, data lines)
+    for line in raw:
+        s = line.strip()
+        if not s or s.startswith("**"):
+            continue
+        if s.startswith("*"):
+            kw, opts = _kw_opts(s)
+            blocks.append([kw, opts, []])
+        elif blocks:
+            blocks[-1][2].append(s)
+    m = dict(uel={}, nodes={}, elements={}, elsets={}, nsets={}, props=[], materials={}, bcs=[], dloads=[],
+             cloads=[])
+    mat = None
+    for kw, opts, data in blocks:
+        if kw == "*USER ELEMENT":
+            m["uel"][opts.get("TYPE", "U1").upper()] = opts
+        elif kw == "*NODE":
+            for d in data:
+                v = _nums(d)
+                m["nodes"][int(v[0])] = v[1:]
+            if "NSET" in opts:
+                m["nsets"][opts["NSET"]] = sorted(int(_nums(d)[0]) for d in data)
+        elif kw == "*ELEMENT":
+            ids = []
+            for d in data:
+                v = _nums(d, int)
+                m["elements"][int(v[0])] = (opts.get("TYPE", "U1").upper(), v[1:])
+                ids.append(int(v[0]))
+            if "ELSET" in opts:
+                m["elsets"][opts["ELSET"]] = ids
+        elif kw in ("*ELSET", "*NSET"):
+            name = opts.get("ELSET" if kw == "*ELSET" else "NSET")
+            vals = []
+            for d in data:
+                vals.extend(int(x) for x in _nums(d, int))
+            if opts.get("GENERATE"):
+                a, b, st = (vals + [1])[:3]
+                vals = list(range(a, b + 1, st))
+            (m["elsets"] if kw == "*ELSET" else m["nsets"])[name] = vals
+        elif kw == "*UEL PROPERTY":
+            vals = []
+            for d in data:
+                vals.extend(_nums(d).tolist())
+            m["props"].append((opts["ELSET"], opts.get("MATERIAL"), np.array(vals, np.float64)))
+        elif kw == "*MATERIAL":
+            mat = opts["NAME"]
+            m["materials"][mat] = {}
+        elif kw == "*ELASTIC":
+            v = _nums(data[0])
+            m["materials"][mat]["E"], m["materials"][mat]["nu"] = float(v[0]), float(v[1])
+        elif kw == "*DENSITY":
+            m["materials"][mat]["rho"] = float(_nums(data[0])[0])
+        elif kw == "*BOUNDARY":
+            for d in data:
+                parts = [x.strip() for x in d.split(",")]
+                first, last = int(parts[1]), int(parts[2])
+                value = float(parts[3]) if len(parts) > 3 and parts[3] else 0.0
+                m["bcs"].append((parts[0], first, last, value))
+        elif kw == "*DLOAD":
+            for d in data:
+                parts = [x.strip() for x in d.split(",")]
+                code = int(parts[1].upper().split("U")[1])
+                m["dloads"].append((parts[0], code, float(parts[2]), np.array([float(x) for x in parts[3:] if x])))
+        elif kw == "*CLOAD":
+            for d in data:
+                parts = [x.strip() for x in d.split(",")]
+                m["cloads"].append((parts[0], int(parts[1]), float(parts[2])))
+    return m
+
+
+def _set_name(target):
+    """'I1.CPonX0' -> 'CPonX0' (one instance)."""
+    return target.split(".", 1)[1] if "." in target else target
+
+
+# ---------------------------------------------------------------------------------- B-splines
+
+
+def _basis_row(U, p, u):
+    """All n basis functions of degree p on knot vector U at u (Cox-de Boor; the last basis
+    function is 1 at the end of the last non-empty span)."""
+    U = np.asarray(U, np.float64)
+    n = U.size - p - 1
+    N = np.zeros(U.size - 1)
+    last = np.max(np.nonzero(U[1:] > U[:-1])[0])
+    for i in range(U.size - 1):
+        if (U[i] <= u < U[i + 1]) or (i == last and u == U[i + 1]):
+            N[i] = 1.0
+    for k in range(1, p + 1):
+        Nk = np.zeros(U.size - 1 - k)
+        for i in range(U.size - 1 - k):
+            a = (u - U[i]) / (U[i + k] - U[i]) * N[i] if U[i + k] > U[i] else 0.0
+            b = (U[i + k + 1] - u) / (U[i + k + 1] - U[i + 1]) * N[i + 1] if U[i + k + 1] > U[i + 1] else 0.0
+            Nk[i] = a + b
+        N = Nk
+    return N[:n]
+
+
+def _elevated_knots(U, p, t):
+    """Knot vector after degree elevation by t: every distinct knot's multiplicity grows by t."""
+    vals, counts = np.unique(np.asarray(U, np.float64), return_counts=True)
+    return np.repeat(vals, counts + t)
+
+
+def _midpoint_knots(U, levels):
+    """`levels` times: insert the middle of every non-empty interval (averageKnot_insertions)."""
+    U = np.asarray(U, np.float64)
+    for _ in range(levels):
+        mids = [(a + b) * 0.5 for a, b in zip(U[:-1], U[1:]) if b > a]
+        U = np.sort(np.concatenate([U, mids]))
+    return U
+
+
+def refinement_matrix(Uc, pc, Uf, pf):
+    """T (n_fine x n_coarse) with N_coarse_j = sum_i T_ij N_fine_i: the coarse space lies in the
+    fine one, so interpolation at the fine Greville abscissae is exact."""
+    nf = Uf.size - pf - 1
+    g = np.array([Uf[i + 1: i + pf + 1].mean() if pf > 0 else 0.5 * (Uf[i] + Uf[i + 1]) for i in range(nf)])
+    Bf = np.array([_basis_row(Uf, pf, u) for u in g])
+    Bc = np.array([_basis_row(Uc, pc, u) for u in g])
+    T = np.linalg.solve(Bf, Bc)
+    T[np.abs(T) < 1e-14 * np.abs(T).max()] = 0.0
+    return T
+
+
+# ------------------------------------------------------------------------------------ model
+
+
+class IGAparametrization:
+    """The flattened model of a ``.inp`` + ``.NB`` pair, with the f2py argument-list builders of
+    the reference's IGAparametrization for the solid path."""
+
+    def __init__(self, filename=None, inp=None, nb=None):
+        if filename is not None:
+            inp, nb = filename + ".inp", filename + ".NB"
+        self._geo = read_nb(nb)
+        self._mech = read_inp(inp)
+        self._build_from_files()
+
+    # -- construction ------------------------------------------------------------------------
+    def _build_from_files(self):
+        g, m = self._geo, self._mech
+        self._nb_patch = int(g["nb_patch"])
+        self._dim = np.asarray(g["dim"])
+        ncp = max(m["nodes"])
+        self._COORDS = np.zeros((3, ncp), order="F")
+        for k, v in m["nodes"].items():
+            self._COORDS[: v.size, k - 1] = v
+        self._mcrd = int(next(iter(m["uel"].values())).get("COORDINATES", 3)) if m["uel"] else 3
+        # patches: the element sets of the UEL PROPERTY lines, in order
+        self._patch_elems = [list(m["elsets"][es]) for es, _, _ in m["props"]]
+        self._elt_type = [m["elements"][els[0]][0] for els in self._patch_elems]
+        self._IEN = [np.array([m["elements"][e][1] for e in els], np.int64) for els in self._patch_elems]
+        self._Nkv = np.asarray(g["Nkv"], np.int64)
+        self._Ukv = [[np.asarray(u, np.float64) for u in kv] for kv in g["Ukv"]]
+        self._Jpqr = np.asarray(g["Jpqr"], np.int64)
+        self._Nijk = np.asarray(g["Nijk"], np.int64)
+        self._weight = [np.asarray(w, np.float64) for w in g["weight"]]
+        self._nnode = np.asarray(g["nnode"], np.int64)
+        self._elementsByPatch = np.asarray(g["nb_elem_patch"], np.int64)
+        self._props = [p for _, _, p in m["props"]]
+        mats = [m["materials"][mn] for _, mn, _ in m["props"]]
+        self._MATERIAL_PROPERTIES = np.array([[mt["E"] for mt in mats], [mt["nu"] for mt in mats],
+                                              [mt.get("rho", 0.0) for mt in mats]])
+        uel = [m["uel"].get(t, {}) for t in self._elt_type]
+        self._TENSOR = [str(u.get("TENSOR", "THREED")) for u in uel]
+        self._NBPINT = np.array([int(u.get("INTEGRATION", 8)) for u in uel], np.int64)
+        # boundary conditions: one per (target set, dof), dof <= 3 (displacements)
+        self._bcs = []
+        for tgt, first, last, value in m["bcs"]:
+            ids = self._node_set(tgt)
+            for dof in range(first, last + 1):
+                if dof <= self._mcrd:
+                    self._bcs.append((np.asarray(ids, np.int64), dof, value))
+        self._dloads = [(np.asarray(self._elem_set(t), np.int64), code, mag, extra) for t, code, mag, extra in
+                        m["dloads"]]
+        self._cloads = [(np.asarray(self._node_set(t), np.int64), dof, mag) for t, dof, mag in m["cloads"]]
+        self._update_dofs()
+
+    def _node_set(self, target):
+        try:
+            return [int(target)]
+        except ValueError:
+            return list(self._mech["nsets"][_set_name(target)])
+
+    def _elem_set(self, target):
+        try:
+            return [int(target)]
+        except ValueError:
+            return list(self._mech["elsets"][_set_name(target)])
+
+    def _update_dofs(self):
+        ncp = self._COORDS.shape[1]
+        self._nb_cp = ncp
+        self._nb_dof_tot = self._mcrd * ncp
+        fixed = np.zeros(self._nb_dof_tot, bool)
+        for ids, dof, _ in self._bcs:
+            fixed[(ids - 1) * self._mcrd + dof - 1] = True
+        free = np.nonzero(~fixed)[0] + 1
+        self._nb_dof_free = int(free.size)
+        self._ind_dof_free = np.zeros(self._nb_dof_tot, np.int64)
+        self._ind_dof_free[: free.size] = free
+
+    # -- properties of the reference's object ------------------------------------------------
+    nb_patch = property(lambda self: self._nb_patch)
+    nb_elem = property(lambda self: int(self._elementsByPatch.sum()))
+    nb_cp = property(lambda self: self._nb_cp)
+    nb_dof_tot = property(lambda self: self._nb_dof_tot)
+    nb_dof_free = property(lambda self: self._nb_dof_free)
+    ind_dof_free = property(lambda self: self._ind_dof_free)
+    coords = property(lambda self: self._COORDS)
+    mcrd = property(lambda self: self._mcrd)
+
+    # -- refinement ----------------------------------------------------------------------------
+    def refine(self, nb_refinementByDirection, nb_degreeElevationByDirection=np.array([0, 0, 0])):
+        """k-refinement of a single solid patch (IGA_parametrization.py:2294-2437): degree
+        elevation by nb_degreeElevationByDirection, then nb_refinementByDirection levels of
+        mid-knot insertion; per-direction arrays of shape (3,) or (3, nb_patch)."""
+        if self._nb_patch != 1 or self._elt_type[0] != "U1" or self._dim[0] != 3:
+            raise NotImplementedError("refine: single-patch 3D solid (U1) models only")
+        nref = np.asarray(nb_refinementByDirection, np.int64).reshape(3, -1)[:, 0]
+        ndeg = np.asarray(nb_degreeElevationByDirection, np.int64).reshape(3, -1)[:, 0]
+        p0 = self._Jpqr[:, 0].copy()
+        Uc = self._Ukv[0]
+        nc = [Uc[d].size - p0[d] - 1 for d in range(3)]
+        grid = self._cp_grid(0)                               # (nc0, nc1, nc2) -> CP id
+        # homogeneous coarse net
+        w = np.zeros(nc)
+        for e in range(self.nb_elem):
+            for l, (i, j, k) in enumerate(self._local_nodes(p0)):
+                s = self._Nijk[:, e]
+                w[s[0] - 1 - i, s[1] - 1 - j, s[2] - 1 - k] = self._weight[e][l]
+        X = self._COORDS[:, grid - 1]                          # (3, nc0, nc1, nc2)
+        H = np.concatenate([X * w[None], w[None]], axis=0)    # (4, ...)
+        pf = p0 + ndeg
+        Uf = [_midpoint_knots(_elevated_knots(Uc[d], p0[d], ndeg[d]), nref[d]) for d in range(3)]
+        T = [refinement_matrix(Uc[d], p0[d], Uf[d], pf[d]) for d in range(3)]
+        Hf = np.einsum("ai,bj,ck,xijk->xabc", T[0], T[1], T[2], H, optimize=True)
+        wf = Hf[3]
+        Xf = Hf[:3] / wf[None]
+        nf = wf.shape
+        # refined CPs, xi fastest
+        self._COORDS = np.asfortranarray(Xf.reshape(3, -1, order="F"))
+        gridf = np.arange(1, int(np.prod(nf)) + 1).reshape(nf, order="F")
+        # elements: every non-empty span, xi fastest
+        spans = [np.nonzero(Uf[d][pf[d]:-pf[d] - 1] < Uf[d][pf[d] + 1:Uf[d].size - pf[d]])[0] + pf[d] + 1
+                 for d in range(3)]                            # 1-based span indices
+        old_nijk, old_ienp = self._Nijk.copy(), self._IEN[0].copy()
+        Nijk, IEN, W = [], [], []
+        loc = self._local_nodes(pf)
+        for s2 in spans[2]:
+            for s1 in spans[1]:
+                for s0 in spans[0]:
+                    Nijk.append((s0, s1, s2))
+                    ids = [gridf[s0 - 1 - i, s1 - 1 - j, s2 - 1 - k] for (i, j, k) in loc]
+                    IEN.append(ids)
+                    W.append([wf[s0 - 1 - i, s1 - 1 - j, s2 - 1 - k] for (i, j, k) in loc])
+        self._Nijk = np.asarray(Nijk, np.int64).T.copy()
+        self._IEN = [np.asarray(IEN, np.int64)]
+        self._weight = [np.asarray(x) for x in W]
+        self._Ukv = [Uf]
+        self._Nkv = np.array([[u.size] for u in Uf], np.int64)
+        self._Jpqr = pf.reshape(3, 1)
+        self._nnode = np.array([int(np.prod(pf + 1))], np.int64)
+        self._elementsByPatch = np.array([len(Nijk)], np.int64)
+        self._NBPINT = np.array([int(np.max(pf) + 1) ** 3], np.int64)
+        # loaded elements: the refined elements inside a loaded coarse element; for a face load
+        # (code 10 f + t, f = 1..6: xi-, xi+, eta-, eta+, zeta-, zeta+) only those on that face
+        # of it (updateLoads, IGA_refinementFcts.py:700-878)
+        def children(coarse_ids, code):
+            face = 0 if code > 99 else code // 10
+            out = []
+            boxes = []
+            for ce in coarse_ids:
+                s = old_nijk[:, ce - 1]
+                boxes.append([(Uc[d][s[d] - 1], Uc[d][s[d]]) for d in range(3)])
+            for fe, s in enumerate(Nijk):
+                lo = [Uf[d][s[d] - 1] for d in range(3)]
+                hi = [Uf[d][s[d]] for d in range(3)]
+                for b in boxes:
+                    if not all(b[d][0] <= lo[d] and hi[d] <= b[d][1] for d in range(3)):
+                        continue
+                    if face:
+                        d, side = (face - 1) // 2, (face - 1) % 2
+                        if (side == 0 and lo[d] != b[d][0]) or (side == 1 and hi[d] != b[d][1]):
+                            continue
+                    out.append(fe + 1)
+                    break
+            return np.asarray(out, np.int64)
+        self._dloads = [(children(ids, code), code, mag, extra) for ids, code, mag, extra in self._dloads]
+        # node sets: whole patch, faces, edges, vertices of the coarse grid -> same on the fine one
+        def map_nodes(ids):
+            ids = set(int(i) for i in ids)
+            mask = np.isin(grid, list(ids))
+            if mask.all():
+                return gridf.ravel(order="F")
+            out = []
+            nce = [np.array([0, n - 1]) for n in nc]
+            for d in range(3):          # faces
+                for side in (0, 1):
+                    sl = [slice(None)] * 3
+                    sl[d] = nce[d][side]
+                    if mask[tuple(sl)].all():
+                        slf = [slice(None)] * 3
+                        slf[d] = 0 if side == 0 else nf[d] - 1
+                        return gridf[tuple(slf)].ravel(order="F")
+            for d in range(3):          # edges along direction d
+                o = [x for x in range(3) if x != d]
+                for s0 in (0, 1):
+                    for s1 in (0, 1):
+                        sl = [slice(None)] * 3
+                        sl[o[0]] = nce[o[0]][s0]
+                        sl[o[1]] = nce[o[1]][s1]
+                        if mask[tuple(sl)].all():
+                            slf = [slice(None)] * 3
+                            slf[o[0]] = 0 if s0 == 0 else nf[o[0]] - 1
+                            slf[o[1]] = 0 if s1 == 0 else nf[o[1]] - 1
+                            out.extend(gridf[tuple(slf)].ravel().tolist())
+            if out:
+                return np.asarray(out, np.int64)
+            for c in np.argwhere(mask):  # vertices
+                if all(c[d] in (0, nc[d] - 1) for d in range(3)):
+                    out.append(gridf[tuple(0 if c[d] == 0 else nf[d] - 1 for d in range(3))])
+            return np.asarray(out, np.int64)
+        self._bcs = [(map_nodes(ids), dof, v) for ids, dof, v in self._bcs]
+        self._cloads = [(map_nodes(ids), dof, v) for ids, dof, v in self._cloads]
+        self._update_dofs()
+
+    @staticmethod
+    def _local_nodes(p):
+        return [(i, j, k) for k in range(p[2] + 1) for j in range(p[1] + 1) for i in range(p[0] + 1)]
+
+    def _cp_grid(self, ip):
+        """CP ids of patch ip on its (n0, n1, n2) grid, from the elements' spans and IEN (local
+        node (i, j, k) of the element with spans s is grid point s - 1 - (i, j, k))."""
+        p = self._Jpqr[:, ip]
+        n = [self._Ukv[ip][d].size - p[d] - 1 for d in range(3)]
+        grid = np.zeros(n, np.int64)
+        loc = self._local_nodes(p)
+        for e in range(self.nb_elem):
+            s = self._Nijk[:, e]
+            for l, (i, j, k) in enumerate(loc):
+                grid[s[0] - 1 - i, s[1] - 1 - j, s[2] - 1 - k] = self._IEN[ip][e][l]
+        if (grid == 0).any():
+            raise ValueError("patch %d: control points not covered by its elements" % (ip + 1))
+        return grid
+
+    # -- f2py argument lists -------------------------------------------------------------------
+    def _flat(self):
+        ne = self.nb_elem
+        bc_target = np.concatenate([ids for ids, _, _ in self._bcs]) if self._bcs else np.zeros(1, np.int64)
+        bc_nb = np.array([ids.size for ids, _, _ in self._bcs], np.int64)
+        bc_values = np.array([[dof for _, dof, _ in self._bcs], [v for _, _, v in self._bcs]], np.float64)
+        loads = list(self._dloads)
+        indDLoad = np.concatenate([ids for ids, _, _, _ in loads]) if loads else np.zeros(1, np.int64)
+        extra = [x for _, _, _, x in loads]
+        nmat = np.full(self._nb_patch, 2, np.int64)
+        ncl = [(ids, dof, mag) for ids, dof, mag in self._cloads]
+        # concentrated loads are code 10 + dof-less entries of the load table: not used here
+        if ncl:
+            raise NotImplementedError("*Cload is not supported by this reader")
+        d = dict(
+            activeElement=np.ones(ne, np.int64),
+            nb_data=float(sum(int(self._elementsByPatch[i]) * (3 * int(self._nnode[i])) * (3 * int(self._nnode[i]) + 1)
+                              // 2 for i in range(self._nb_patch))),
+            COORDS3D=self._COORDS, IEN=np.concatenate([x.ravel() for x in self._IEN]),
+            nb_elem_patch=self._elementsByPatch, Nkv=self._Nkv,
+            Ukv=np.concatenate([np.concatenate(kv) for kv in self._Ukv]), Nijk=self._Nijk,
+            weight=np.concatenate([np.asarray(w, np.float64).ravel() for w in self._weight]), Jpqr=self._Jpqr,
+            ELT_TYPE="".join(self._elt_type), PROPS=np.concatenate(self._props) if self._props else np.zeros(1),
+            JPROPS=np.array([p.size for p in self._props], np.int64),
+            MATERIAL_PROPERTIES=self._MATERIAL_PROPERTIES[:2], n_mat_props=nmat,
+            RHO=self._MATERIAL_PROPERTIES[2], TENSOR="".join("/" + t for t in self._TENSOR),
+            indDLoad=indDLoad, JDLType=np.array([c for _, c, _, _ in loads], np.int64),
+            ADLMAG=np.array([mgn for _, _, mgn, _ in loads], np.float64),
+            load_target_nbelem=np.array([ids.size for ids, _, _, _ in loads], np.int64),
+            load_additionalInfos=np.concatenate(extra) if extra else np.zeros(0),
+            nb_load_additionalInfos=np.array([x.size for x in extra], np.int64),
+            bc_values=bc_values, bc_target=bc_target, bc_target_nbelem=bc_nb,
+            ind_dof_free=self._ind_dof_free, nb_dof_free=self._nb_dof_free, MCRD=self._mcrd, NBINT=self._NBPINT,
+            NNODE=self._nnode, nodal_dist=np.zeros((1, self._nb_cp), order="F"),
+            nb_bc=len(self._bcs), nb_load=len(loads), nb_patch=self._nb_patch, nb_elem=ne, nb_cp=self._nb_cp,
+            nb_dof_tot=self._nb_dof_tot,
+        )
+        return d
+
+    def get_inputs4system_elemStorage(self):
+        """Positional argument list of sys_linmat_lindef_static (IGA_parametrization.py:1705-1716)."""
+        d = self._flat()
+        return [d[k] for k in F2PY_ARGS] + [d[k] for k in F2PY_OPTIONAL if k in d]
+
+    def get_model_dict(self):
+        """The same inputs by name (what yeti_amd.assembly.Assembler also accepts)."""
+        return self._flat()
+
+    def get_inputs4solution(self, U_inv):
+        """Keyword arguments of reconstructionSOL.reconstruction (IGA_parametrization.py:664-682)."""
+        d = self._flat()
+        return {"u_inv": U_inv, "coords": self._COORDS, "ind_dof_free": self._ind_dof_free,
+                "bc_target": d["bc_target"], "bc_target_nbelem": d["bc_target_nbelem"], "bc_values": d["bc_values"],
+                "couplg_flag": False, "nb_bc": d["nb_bc"], "mcrd": self._mcrd, "nb_cp": self._nb_cp,
+                "nb_dof_free": self._nb_dof_free}
+
+    def get_inputs4evaldisp(self, sol, xi, numpatch=1):
+        """Positional argument list of postproc.evaldisp (IGA_parametrization.py:908-938)."""
+        d = self._flat()
+        return [numpatch, xi, sol, self._COORDS, d["IEN"], self._elementsByPatch, self._Nkv, d["Ukv"], self._Nijk,
+                d["weight"], self._Jpqr, d["ELT_TYPE"], self._MATERIAL_PROPERTIES[:2], d["TENSOR"], d["PROPS"],
+                d["JPROPS"], self._nnode, self._nb_patch, self.nb_elem, self._nb_cp, self._mcrd]
+
+    def get_inputs4postprocVTU(self, filename, sol, nb_ref=np.ones(3, dtype=int), Flag=np.array([True, True, True]),
+                               output_path="results"):
+        """Positional argument list of postproc.generatevtu (IGA_parametrization.py:717-752)."""
+        d = self._flat()
+        nb_ref = np.maximum(nb_ref, np.ones(3, dtype=int))
+        return [filename, output_path, Flag, nb_ref, sol, self._COORDS, d["IEN"], self._elementsByPatch, self._Nkv,
+                d["Ukv"], self._Nijk, d["weight"], self._Jpqr, d["ELT_TYPE"], self._MATERIAL_PROPERTIES[:2],
+                d["TENSOR"], d["PROPS"], d["JPROPS"], self._nnode, self._nb_patch, self.nb_elem, self._nb_cp,
+                self._mcrd]
