@@ -1173,14 +1173,15 @@ __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho,
                 for (int kk = 0; kk < 4; ++kk) Rx[a][c][kk] = r4[kk];
             }
             __syncthreads();
-            // 4. per (point, mode): u, du/dx, stress, the tensor and the mass
-            if (t < VQC * nf) {
-                const int c = t % VQC, f = t / VQC;
-                double T[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                if (c < nq) {
-                    double u[3] = {0.0, 0.0, 0.0}, D[3][3] = {};   // D[i][k] = du_i/dx_k
+            // 4. per (point, mode): u, du/dx, stress, the tensor and the mass; the nodes of a
+            // (point, mode) pair are split over 4 adjacent lanes and their sums added by shuffles
+            {
+                const int pr = t >> 2, part4 = t & 3, c = pr % VQC, f = pr / VQC;
+                const bool act = f < nf && c < nq;
+                double u[3] = {0.0, 0.0, 0.0}, D[3][3] = {};   // D[i][k] = du_i/dx_k
+                if (act) {
 #pragma unroll 1
-                    for (int a = 0; a < NN; ++a) {
+                    for (int a = part4; a < NN; a += 4) {
                         const double R = Rx[a][c][0], g0 = Rx[a][c][1], g1 = Rx[a][c][2], g2 = Rx[a][c][3];
 #pragma unroll
                         for (int i = 0; i < 3; ++i) {
@@ -1191,30 +1192,44 @@ __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho,
                             D[i][2] += ua * g2;
                         }
                     }
-                    const double lam = pa.lam, mu = pa.mu, dv = gq[c][9];
-                    const double tr = D[0][0] + D[1][1] + D[2][2];
-                    double sig[3][3], W = 0.0;
-#pragma unroll
-                    for (int i = 0; i < 3; ++i)
-#pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            const double eik = 0.5 * (D[i][k] + D[k][i]);
-                            sig[i][k] = 2.0 * mu * eik + (i == k ? lam * tr : 0.0);
-                            W += eik * sig[i][k];
-                        }
-                    const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-                    const double Wp = W - vm.vals[f] * rho * uu;
-#pragma unroll
-                    for (int m = 0; m < 3; ++m)
-#pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            double v = m == k ? Wp : 0.0;
-                            v -= 2.0 * (D[0][m] * sig[0][k] + D[1][m] * sig[1][k] + D[2][m] * sig[2][k]);
-                            T[3 * m + k] = dv * v;
-                        }
-                    macc += rho * uu * dv;
                 }
-                for (int k = 0; k < 9; ++k) Tq[c][f][k] = T[k];
+#pragma unroll
+                for (int o = 1; o < 4; o <<= 1) {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        u[i] += __shfl_xor(u[i], o, 64);
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) D[i][k] += __shfl_xor(D[i][k], o, 64);
+                    }
+                }
+                if (part4 == 0 && f < nf) {
+                    double T[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                    if (c < nq) {
+                        const double lam = pa.lam, mu = pa.mu, dv = gq[c][9];
+                        const double tr = D[0][0] + D[1][1] + D[2][2];
+                        double sig[3][3], W = 0.0;
+#pragma unroll
+                        for (int i = 0; i < 3; ++i)
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) {
+                                const double eik = 0.5 * (D[i][k] + D[k][i]);
+                                sig[i][k] = 2.0 * mu * eik + (i == k ? lam * tr : 0.0);
+                                W += eik * sig[i][k];
+                            }
+                        const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+                        const double Wp = W - vm.vals[f] * rho * uu;
+#pragma unroll
+                        for (int m = 0; m < 3; ++m)
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) {
+                                double v = m == k ? Wp : 0.0;
+                                v -= 2.0 * (D[0][m] * sig[0][k] + D[1][m] * sig[1][k] + D[2][m] * sig[2][k]);
+                                T[3 * m + k] = dv * v;
+                            }
+                        macc += rho * uu * dv;
+                    }
+                    for (int k = 0; k < 9; ++k) Tq[c][f][k] = T[k];
+                }
             }
             __syncthreads();
             // 5. per (node, mode): g[m] += sum_q T[m][k] dR/dx_k
@@ -1230,7 +1245,7 @@ __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho,
                 for (int m = 0; m < 3; ++m) gacc[f][a][m] += g[m];
             }
         }
-        if (t < VQC * VIB_MAXF) mpart[t % VQC][t / VQC] = t < VQC * nf ? macc : 0.0;
+        if ((t & 3) == 0) mpart[(t >> 2) % VQC][(t >> 2) / VQC] = (t >> 2) / VQC < nf ? macc : 0.0;
         __syncthreads();
         const int64_t eb = ebase + gi * NN;
         for (int x = t; x < NN * VIB_MAXF * 3; x += 256) {
