@@ -2,15 +2,17 @@
 models end to end.
 
 Fixtures (tests/golden/models/) are the input files of the reference's benchmarks
-benchs/dload/beam-dist.{inp,NB} and benchs/volume_force/volume_force_3D.{inp,NB}, copied as
-data. Pins:
+benchs/dload/{beam-dist, beam-dist-field, beam-triangle-1, beam-triangle-2}.{inp,NB} and
+benchs/volume_force/volume_force_3D.{inp,NB}, copied as data. Pins:
 * refinement is exact: the refined NURBS map equals the coarse one at random parametric points
   (geometry and weights; a wrong knot multiplicity, Greville point or homogeneous/Cartesian mix-up
   moves it);
 * the benchmarks' own published checks, with the reference's refinement and tolerances:
-  benchs/dload/test_dload_beam_dist.py (deflexion at CP (20, 0, 0) within 2e-2 of the analytic
-  -1142.857; pressure U40 on the face of one coarse element, mapped to the refined elements on
-  that face) and benchs/volume_force/test_volume_force_3d.py (displacement at xi = (1, 1, 0)
+  benchs/dload/test_dload_beam_{dist, dist_field, triangular_1, triangular_2}.py (deflexion at
+  CP (20, 0, 0) within 2e-2 of the analytic value; pressure U40 on the face of one coarse
+  element, mapped to the refined elements on that face, or U44 with a nodal *DISTRIBUTION
+  refined as a rational field; the uniform field must give exactly the uniform pressure's
+  answer) and benchs/volume_force/test_volume_force_3d.py (displacement at xi = (1, 1, 0)
   within 2e-3 of the Abaqus reference; body force U102 with a direction). On the CPU the
   assembly is the oracle's; on the GPU it is the drop-in stack (stiffness on the GPU, the
   callers' scipy solve, reconstruction and evaldisp on the GPU).
@@ -30,10 +32,19 @@ MODELS = os.path.join(HERE, "golden", "models")
 VF3D_REF = np.array([-9.96639E-5, -1.01212, -6.42538])   # benchs/volume_force/test_volume_force_3d.py
 
 
-def beam_dist():
-    m = IGAparametrization(filename=os.path.join(MODELS, "beam-dist"))
+# benchs/dload: model -> analytic deflexion at CP (20, 0, 0)
+DLOAD = {"beam-dist": -1142.857, "beam-dist-field": -1142.86, "beam-triangle-1": -838.10,
+         "beam-triangle-2": -304.76}
+
+
+def dload_model(name):
+    m = IGAparametrization(filename=os.path.join(MODELS, name))
     m.refine(np.array([2, 2, 5]), np.array([1, 1, 1]))
     return m
+
+
+def beam_dist():
+    return dload_model("beam-dist")
 
 
 def volume_force_3d():
@@ -94,13 +105,24 @@ def test_refinement_is_exact(build):
     assert np.asarray(d["NBINT"])[0] == 27
 
 
-def test_bench_dload_beam_dist_oracle():
-    m = beam_dist()
-    K, r, c, F = O.oracle_sys_linmat(m.get_inputs4system_elemStorage())
-    sol = full_sol(m, solve(m, K, r, c, F))
+def tip_deflexion(m, sol):
     idx = np.nonzero(np.all(m.coords == np.array([[20.0], [0.0], [0.0]]), axis=0))[0]
     assert idx.size == 1
-    assert np.allclose(sol[idx[0], 2], -1142.857, rtol=2e-2)
+    return sol[idx[0], 2]
+
+
+@pytest.mark.parametrize("name", sorted(DLOAD))
+def test_bench_dload_oracle(name):
+    m = dload_model(name)
+    K, r, c, F = O.oracle_sys_linmat(m.get_inputs4system_elemStorage())
+    assert np.allclose(tip_deflexion(m, full_sol(m, solve(m, K, r, c, F))), DLOAD[name], rtol=2e-2)
+
+
+def test_uniform_field_equals_uniform_pressure():
+    a, b = dload_model("beam-dist"), dload_model("beam-dist-field")
+    Fa = O.oracle_sys_linmat(a.get_inputs4system_elemStorage())[3]
+    Fb = O.oracle_sys_linmat(b.get_inputs4system_elemStorage())[3]
+    assert np.abs(Fa - Fb).max() <= 1e-12 * np.abs(Fa).max()
 
 
 def test_bench_volume_force_3d_oracle():
@@ -124,22 +146,22 @@ def test_refine_rejects_multipatch():
 
 
 @pytest.mark.gpu
-def test_gpu_bench_dload_beam_dist():
-    """benchs/dload/test_dload_beam_dist.py on the drop-in stack."""
+@pytest.mark.parametrize("name", sorted(DLOAD))
+def test_gpu_bench_dload(name):
+    """benchs/dload/test_dload_beam_*.py on the drop-in stack."""
     from yeti_amd import reconstructionSOL as rsol
     from yeti_amd.stiffmtrx_elemstorage import sys_linmat_lindef_static as build_stiffmatrix
-    print("[case] bench dload beam-dist", flush=True)
-    m = beam_dist()
+    print("[case] bench dload %s" % name, flush=True)
+    m = dload_model(name)
     data, row, col, rhs = build_stiffmatrix(*m.get_inputs4system_elemStorage())
     x = solve(m, data, row, col, rhs)
     sol, _ = rsol.reconstruction(**m.get_inputs4solution(x))
-    idx = np.nonzero(np.all(m.coords == np.array([[20.0], [0.0], [0.0]]), axis=0))[0]
-    assert idx.size == 1
-    assert np.allclose(sol[idx[0], 2], -1142.857, rtol=2e-2)
+    assert np.allclose(tip_deflexion(m, np.asarray(sol)), DLOAD[name], rtol=2e-2)
     # and the same numbers as the oracle's assembly
     K, r, c, F = O.oracle_sys_linmat(m.get_inputs4system_elemStorage())
     assert np.array_equal(row, r) and np.array_equal(col, c)
     assert np.abs(data - K).max() <= 1e-12 * np.abs(K).max()
+    assert np.abs(rhs - F).max() <= 1e-12 * np.abs(F).max()
 
 
 @pytest.mark.gpu

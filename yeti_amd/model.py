@@ -8,8 +8,8 @@ IGA_refinementFcts.py).
 Scope: solid patches; the keywords the solid models use (``*USER ELEMENT``, ``*Node``,
 ``*Element``, ``*ELSET`` / ``*NSET`` (lists or ``generate``), ``*UEL PROPERTY``, ``*MATERIAL``
 with ``*Elastic`` / ``*Density``, ``*Boundary`` (first/last dof, value), ``*Dload`` (``U<code>``,
-magnitude, additional values); ``*Cload`` lines are read but raise NotImplementedError when the
-model is flattened); one instance. Refinement is implemented for
+magnitude, additional values, or the name of a nodal ``*DISTRIBUTION``); ``*Cload`` lines are
+read but raise NotImplementedError when the model is flattened); one instance. Refinement is implemented for
 single-patch models: degree elevation, then levels of mid-knot insertion on every non-empty
 interval, both exact. The refined control net is obtained by interpolating the coarse NURBS (in
 homogeneous coordinates) at the Greville abscissae of the refined space, which contains it.
@@ -135,7 +135,7 @@ def read_inp(path):
         elif blocks:
             blocks[-1][2].append(s)
     m = dict(uel={}, nodes={}, elements={}, elsets={}, nsets={}, props=[], materials={}, bcs=[], dloads=[],
-             cloads=[])
+             cloads=[], dists=[])
     mat = None
     for kw, opts, data in blocks:
         if kw == "*USER ELEMENT":
@@ -182,11 +182,28 @@ def read_inp(path):
                 first, last = int(parts[1]), int(parts[2])
                 value = float(parts[3]) if len(parts) > 3 and parts[3] else 0.0
                 m["bcs"].append((parts[0], first, last, value))
+        elif kw == "*DISTRIBUTION":
+            # (name, default values, {cp: values}); labels are 1-based in file order
+            if str(opts.get("LOCATION", "")).upper() != "NODE":
+                raise NotImplementedError("*DISTRIBUTION: only LOCATION=NODE is supported")
+            default, table = None, {}
+            for d in data:
+                parts = [x.strip() for x in d.split(",")]
+                if parts[0] == "":
+                    default = np.array([float(x) for x in parts[1:]])
+                else:
+                    table[int(parts[0])] = np.array([float(x) for x in parts[1:]])
+            m["dists"].append((opts["NAME"], default, table))
         elif kw == "*DLOAD":
             for d in data:
                 parts = [x.strip() for x in d.split(",")]
                 code = int(parts[1].upper().split("U")[1])
-                m["dloads"].append((parts[0], code, float(parts[2]), np.array([float(x) for x in parts[3:] if x])))
+                try:
+                    m["dloads"].append((parts[0], code, float(parts[2]),
+                                        np.array([float(x) for x in parts[3:] if x])))
+                except ValueError:   # magnitude given by a nodal distribution: its label
+                    names = [n for n, _, _ in m["dists"]]
+                    m["dloads"].append((parts[0], code, 0.0, np.array([float(names.index(parts[2]) + 1)])))
         elif kw == "*CLOAD":
             for d in data:
                 parts = [x.strip() for x in d.split(",")]
@@ -301,6 +318,18 @@ class IGAparametrization:
         self._dloads = [(np.asarray(self._elem_set(t), np.int64), code, mag, extra) for t, code, mag, extra in
                         m["dloads"]]
         self._cloads = [(np.asarray(self._node_set(t), np.int64), dof, mag) for t, dof, mag in m["cloads"]]
+        # nodal distributions (IGA_parametrization.py:133-155): (nb_cp, n) fields, unlisted CPs at
+        # the default values; those a Dload names form nodal_dist (abaqus.py:1209-1214)
+        self._dists = []
+        used = set(int(x[0]) for _, code, _, x in m["dloads"] if code % 10 == 4 and x.size)
+        for lab, (name, default, table) in enumerate(m["dists"], start=1):
+            ncol = len(next(iter(table.values()))) if table else (default.size if default is not None else 1)
+            f = np.zeros((ncp, ncol))
+            if default is not None:
+                f[:] = default
+            for cp, v in table.items():
+                f[cp - 1] = v
+            self._dists.append((name, f, lab in used))
         self._update_dofs()
 
     def _node_set(self, target):
@@ -357,7 +386,11 @@ class IGAparametrization:
                 s = self._Nijk[:, e]
                 w[s[0] - 1 - i, s[1] - 1 - j, s[2] - 1 - k] = self._weight[e][l]
         X = self._COORDS[:, grid - 1]                          # (3, nc0, nc1, nc2)
-        H = np.concatenate([X * w[None], w[None]], axis=0)    # (4, ...)
+        # nodal fields ride along as further homogeneous coordinates (f w), so each stays the
+        # same rational function (_updateNodalField, IGA_parametrization.py:2474-2497)
+        fields = [f[grid - 1] for _, f, _ in self._dists]     # (nc0, nc1, nc2, n)
+        extra = [np.moveaxis(fd, -1, 0) * w[None] for fd in fields]
+        H = np.concatenate([X * w[None], w[None]] + extra, axis=0)
         pf = p0 + ndeg
         Uf = [_midpoint_knots(_elevated_knots(Uc[d], p0[d], ndeg[d]), nref[d]) for d in range(3)]
         T = [refinement_matrix(Uc[d], p0[d], Uf[d], pf[d]) for d in range(3)]
@@ -365,6 +398,14 @@ class IGAparametrization:
         wf = Hf[3]
         Xf = Hf[:3] / wf[None]
         nf = wf.shape
+        k0 = 4
+        new_dists = []
+        for (name, f, used_), fd in zip(self._dists, fields):
+            n = fd.shape[-1]
+            ff = Hf[k0:k0 + n] / wf[None]
+            k0 += n
+            new_dists.append((name, np.moveaxis(ff, 0, -1).reshape(-1, n, order="F"), used_))
+        self._dists = new_dists
         # refined CPs, xi fastest
         self._COORDS = np.asfortranarray(Xf.reshape(3, -1, order="F"))
         gridf = np.arange(1, int(np.prod(nf)) + 1).reshape(nf, order="F")
@@ -504,11 +545,19 @@ class IGAparametrization:
             nb_load_additionalInfos=np.array([x.size for x in extra], np.int64),
             bc_values=bc_values, bc_target=bc_target, bc_target_nbelem=bc_nb,
             ind_dof_free=self._ind_dof_free, nb_dof_free=self._nb_dof_free, MCRD=self._mcrd, NBINT=self._NBPINT,
-            NNODE=self._nnode, nodal_dist=np.zeros((1, self._nb_cp), order="F"),
+            NNODE=self._nnode, nodal_dist=self._nodal_dist(),
             nb_bc=len(self._bcs), nb_load=len(loads), nb_patch=self._nb_patch, nb_elem=ne, nb_cp=self._nb_cp,
             nb_dof_tot=self._nb_dof_tot,
         )
+        d["nb_n_dist"] = int(d["nodal_dist"].shape[0])
+        d["nb_cp_n_dist"] = int(d["nodal_dist"].shape[1])
         return d
+
+    def _nodal_dist(self):
+        """The distributions named by a Dload, stacked as (sum of columns, nb_cp)
+        (_get_load_info, IGA_parametrization.py:592-611)."""
+        used = [f.T for _, f, u in self._dists if u]
+        return np.asfortranarray(np.vstack(used)) if used else np.zeros((1, self._nb_cp), order="F")
 
     def get_inputs4system_elemStorage(self):
         """Positional argument list of sys_linmat_lindef_static (IGA_parametrization.py:1705-1716)."""
