@@ -2,8 +2,9 @@
 models end to end.
 
 Fixtures (tests/golden/models/) are the input files of the reference's benchmarks
-benchs/dload/{beam-dist, beam-dist-field, beam-triangle-1, beam-triangle-2}.{inp,NB} and
-benchs/volume_force/volume_force_3D.{inp,NB}, copied as data. Pins:
+benchs/dload/{beam-dist, beam-dist-field, beam-triangle-1, beam-triangle-2}.{inp,NB},
+benchs/volume_force/volume_force_3D.{inp,NB} and benchs/platevib/plateVolume.{inp,NB}, copied as
+data. Pins:
 * refinement is exact: the refined NURBS map equals the coarse one at random parametric points
   (geometry and weights; a wrong knot multiplicity, Greville point or homogeneous/Cartesian mix-up
   moves it);
@@ -12,8 +13,10 @@ benchs/volume_force/volume_force_3D.{inp,NB}, copied as data. Pins:
   CP (20, 0, 0) within 2e-2 of the analytic value; pressure U40 on the face of one coarse
   element, mapped to the refined elements on that face, or U44 with a nodal *DISTRIBUTION
   refined as a rational field; the uniform field must give exactly the uniform pressure's
-  answer) and benchs/volume_force/test_volume_force_3d.py (displacement at xi = (1, 1, 0)
-  within 2e-3 of the Abaqus reference; body force U102 with a direction). On the CPU the
+  answer), benchs/volume_force/test_volume_force_3d.py (displacement at xi = (1, 1, 0)
+  within 2e-3 of the Abaqus reference; body force U102 with a direction) and
+  benchs/platevib/test_platevib.py (the 5 lowest eigenfrequencies of K u = lambda M u within
+  1e-3 of the reference's values; stiffness and consistent mass). On the CPU the
   assembly is the oracle's; on the GPU it is the drop-in stack (stiffness on the GPU, the
   callers' scipy solve, reconstruction and evaldisp on the GPU).
 """
@@ -135,6 +138,48 @@ def test_bench_volume_force_3d_oracle():
     assert np.linalg.norm(disp - VF3D_REF) / np.linalg.norm(VF3D_REF) < 2e-3
 
 
+PLATEVIB_REF = np.array([0.2548, 0.6217, 1.9546, 2.3870, 2.8585])   # benchs/platevib/test_platevib.py
+
+
+def platevib():
+    m = IGAparametrization(filename=os.path.join(MODELS, "plateVolume"))
+    m.refine(nb_refinementByDirection=np.array([[2, 2, 1]]).T, nb_degreeElevationByDirection=np.array([[1, 1, 1]]).T)
+    return m
+
+
+def frequencies(m, K, r, c, M, mr, mc, k=5):
+    n = m.nb_dof_tot
+    idof = m.ind_dof_free[:m.nb_dof_free] - 1
+    Ks = sp.coo_matrix((K, (r, c)), shape=(n, n)).tocsc()
+    Ms = sp.coo_matrix((M, (mr, mc)), shape=(n, n)).tocsc()
+    vals, _ = spl.eigsh((Ks + Ks.T)[idof, :][:, idof], k=k, M=(Ms + Ms.T)[idof, :][:, idof], sigma=0.0)
+    return np.sqrt(vals) / 2.0 / np.pi
+
+
+def test_bench_platevib_oracle():
+    m = platevib()
+    K, r, c, _ = O.oracle_sys_linmat(m.get_inputs4system_elemStorage())
+    M, mr, mc = O.oracle_mass_coo(m.get_model_dict())
+    assert np.allclose(frequencies(m, K, r, c, M, mr, mc), PLATEVIB_REF, rtol=1e-3)
+
+
+def test_omp_inputs_match_serial():
+    """benchs/omp/test_build_stifmatrix_omp.py's model: the keyword inputs of the OpenMP routine
+    carry the same model as the positional ones (oracle, both layouts)."""
+    from oracle.oracle import YETI_LAYOUT_PACKED, YETI_LAYOUT_SLOTTED
+    m = IGAparametrization(filename=os.path.join(MODELS, "beam-dist"))
+    m.refine(np.array([2, 2, 2]), np.array([2, 2, 2]))
+    n = m.nb_dof_tot
+    K, r, c, F = O.oracle_sys_linmat(m.get_inputs4system_elemStorage(), layout=YETI_LAYOUT_PACKED)
+    kw = m.get_inputs4system_elemStorage_OMP(num_threads=2)
+    kw.pop("num_threads")
+    K2, r2, c2, F2 = O.oracle_sys_linmat([], layout=YETI_LAYOUT_SLOTTED, kwargs=kw)
+    A = sp.coo_matrix((K, (r, c)), shape=(n, n)).tocsc()
+    B = sp.coo_matrix((K2, (r2, c2)), shape=(n, n)).tocsc()
+    assert spl.norm(A - B) / spl.norm(A) < 1e-15
+    assert np.linalg.norm(F - F2) / np.linalg.norm(F) < 1e-15
+
+
 def test_refine_rejects_multipatch():
     m = IGAparametrization(filename=os.path.join(MODELS, "beam-dist"))
     m._nb_patch = 2
@@ -177,3 +222,33 @@ def test_gpu_bench_volume_force_3d():
     sol, _ = rsol.reconstruction(**m.get_inputs4solution(x))
     disp = pp.evaldisp(*m.get_inputs4evaldisp(np.asarray(sol).transpose(), np.array([1.0, 1.0, 0.0]), numpatch=1))
     assert np.linalg.norm(disp - VF3D_REF) / np.linalg.norm(VF3D_REF) < 2e-3
+
+
+@pytest.mark.gpu
+def test_gpu_bench_platevib():
+    """benchs/platevib/test_platevib.py on the drop-in stack (stiffness and mass on the GPU)."""
+    from yeti_amd.massmtrx import build_cmassmatrix
+    from yeti_amd.stiffmtrx_elemstorage import sys_linmat_lindef_static as build_stiffmatrix
+    print("[case] bench platevib", flush=True)
+    m = platevib()
+    data, row, col, _ = build_stiffmatrix(*m.get_inputs4system_elemStorage())
+    mdata, mrow, mcol = build_cmassmatrix(*m.get_inputs4massmat())
+    assert np.allclose(frequencies(m, data, row, col, mdata, mrow, mcol), PLATEVIB_REF, rtol=1e-3)
+
+
+@pytest.mark.gpu
+def test_gpu_bench_omp():
+    """benchs/omp/test_build_stifmatrix_omp.py on the drop-ins: the serial (packed) and OpenMP
+    (slotted) entry points give the same matrix and vector."""
+    from yeti_amd.stiffmtrx_elemstorage import sys_linmat_lindef_static as build_stiffmatrix
+    from yeti_amd.stiffmtrx_elemstorage_omp import sys_linmat_lindef_static_omp as build_stiffmatrix_omp
+    print("[case] bench omp", flush=True)
+    m = IGAparametrization(filename=os.path.join(MODELS, "beam-dist"))
+    m.refine(np.array([2, 2, 2]), np.array([2, 2, 2]))
+    n = m.nb_dof_tot
+    data, row, col, rhs_seq = build_stiffmatrix(*m.get_inputs4system_elemStorage())
+    A = sp.coo_matrix((data, (row, col)), shape=(n, n)).tocsc()
+    data, row, col, rhs_omp = build_stiffmatrix_omp(**m.get_inputs4system_elemStorage_OMP(num_threads=2))
+    B = sp.coo_matrix((data, (row, col)), shape=(n, n)).tocsc()
+    assert spl.norm(A - B) / spl.norm(A) < 1e-15
+    assert np.linalg.norm(rhs_seq - rhs_omp) / np.linalg.norm(rhs_seq) < 1e-15

@@ -8,15 +8,16 @@ IGA_refinementFcts.py).
 Scope: solid patches; the keywords the solid models use (``*USER ELEMENT``, ``*Node``,
 ``*Element``, ``*ELSET`` / ``*NSET`` (lists or ``generate``), ``*UEL PROPERTY``, ``*MATERIAL``
 with ``*Elastic`` / ``*Density``, ``*Boundary`` (first/last dof, value), ``*Dload`` (``U<code>``,
-magnitude, additional values, or the name of a nodal ``*DISTRIBUTION``); ``*Cload`` lines are
-read but raise NotImplementedError when the model is flattened); one instance. Refinement is implemented for
+magnitude, additional values, or the name of a nodal ``*DISTRIBUTION``), ``*Cload`` (node or
+node set, dof, magnitude); one instance. Refinement is implemented for
 single-patch models: degree elevation, then levels of mid-knot insertion on every non-empty
 interval, both exact. The refined control net is obtained by interpolating the coarse NURBS (in
 homogeneous coordinates) at the Greville abscissae of the refined space, which contains it.
 Node sets that are a whole patch, faces, edges or vertices of the coarse patch are mapped to the
 same entities of the refined one (updateBoundaryConditions, IGA_refinementFcts.py:610-690);
 loaded elements to the refined elements inside them, and for a face load only to those on that
-face (updateLoads, IGA_refinementFcts.py:700-878).
+face; concentrated loads keep the patch vertices among their CPs (updateLoads,
+IGA_refinementFcts.py:700-878).
 """
 from __future__ import annotations
 
@@ -135,7 +136,7 @@ def read_inp(path):
         elif blocks:
             blocks[-1][2].append(s)
     m = dict(uel={}, nodes={}, elements={}, elsets={}, nsets={}, props=[], materials={}, bcs=[], dloads=[],
-             cloads=[], dists=[])
+             cloads=[], dists=[], load_order=[])
     mat = None
     for kw, opts, data in blocks:
         if kw == "*USER ELEMENT":
@@ -204,10 +205,12 @@ def read_inp(path):
                 except ValueError:   # magnitude given by a nodal distribution: its label
                     names = [n for n, _, _ in m["dists"]]
                     m["dloads"].append((parts[0], code, 0.0, np.array([float(names.index(parts[2]) + 1)])))
+                m["load_order"].append(("d", len(m["dloads"]) - 1))
         elif kw == "*CLOAD":
             for d in data:
                 parts = [x.strip() for x in d.split(",")]
                 m["cloads"].append((parts[0], int(parts[1]), float(parts[2])))
+                m["load_order"].append(("c", len(m["cloads"]) - 1))
     return m
 
 
@@ -318,6 +321,7 @@ class IGAparametrization:
         self._dloads = [(np.asarray(self._elem_set(t), np.int64), code, mag, extra) for t, code, mag, extra in
                         m["dloads"]]
         self._cloads = [(np.asarray(self._node_set(t), np.int64), dof, mag) for t, dof, mag in m["cloads"]]
+        self._load_order = list(m["load_order"])
         # nodal distributions (IGA_parametrization.py:133-155): (nb_cp, n) fields, unlisted CPs at
         # the default values; those a Dload names form nodal_dist (abaqus.py:1209-1214)
         self._dists = []
@@ -489,8 +493,15 @@ class IGAparametrization:
                 if all(c[d] in (0, nc[d] - 1) for d in range(3)):
                     out.append(gridf[tuple(0 if c[d] == 0 else nf[d] - 1 for d in range(3))])
             return np.asarray(out, np.int64)
+        def map_vertices(ids):   # concentrated loads keep the patch vertices among their CPs
+            out = []
+            for cp in ids:
+                c = np.argwhere(grid == cp)
+                if c.size and all(c[0][d] in (0, nc[d] - 1) for d in range(3)):
+                    out.append(gridf[tuple(0 if c[0][d] == 0 else nf[d] - 1 for d in range(3))])
+            return np.asarray(out, np.int64)
         self._bcs = [(map_nodes(ids), dof, v) for ids, dof, v in self._bcs]
-        self._cloads = [(map_nodes(ids), dof, v) for ids, dof, v in self._cloads]
+        self._cloads = [(map_vertices(ids), dof, v) for ids, dof, v in self._cloads]
         self._update_dofs()
 
     @staticmethod
@@ -518,14 +529,12 @@ class IGAparametrization:
         bc_target = np.concatenate([ids for ids, _, _ in self._bcs]) if self._bcs else np.zeros(1, np.int64)
         bc_nb = np.array([ids.size for ids, _, _ in self._bcs], np.int64)
         bc_values = np.array([[dof for _, dof, _ in self._bcs], [v for _, _, v in self._bcs]], np.float64)
-        loads = list(self._dloads)
+        # loads in file order; a concentrated load is (CP ids, JDLType = dof, magnitude)
+        loads = [self._dloads[i] if kind == "d" else (self._cloads[i][0], self._cloads[i][1], self._cloads[i][2],
+                                                       np.zeros(0)) for kind, i in self._load_order]
         indDLoad = np.concatenate([ids for ids, _, _, _ in loads]) if loads else np.zeros(1, np.int64)
         extra = [x for _, _, _, x in loads]
         nmat = np.full(self._nb_patch, 2, np.int64)
-        ncl = [(ids, dof, mag) for ids, dof, mag in self._cloads]
-        # concentrated loads are code 10 + dof-less entries of the load table: not used here
-        if ncl:
-            raise NotImplementedError("*Cload is not supported by this reader")
         d = dict(
             activeElement=np.ones(ne, np.int64),
             nb_data=float(sum(int(self._elementsByPatch[i]) * (3 * int(self._nnode[i])) * (3 * int(self._nnode[i]) + 1)
@@ -564,6 +573,18 @@ class IGAparametrization:
         d = self._flat()
         return [d[k] for k in F2PY_ARGS] + [d[k] for k in F2PY_OPTIONAL if k in d]
 
+    def get_inputs4system_elemStorage_OMP(self, num_threads=None, COORDS=None, activeElem=None):
+        """Keyword arguments of sys_linmat_lindef_static_omp (IGA_parametrization.py:1561-1650):
+        the same inputs under their lower-case f2py names, plus num_threads."""
+        d = self._flat()
+        if COORDS is not None and np.shape(COORDS) == np.shape(self._COORDS):
+            d["COORDS3D"] = COORDS
+        if activeElem is not None:
+            d["activeElement"] = np.asarray(activeElem)
+        kw = {k.lower(): d[k] for k in F2PY_ARGS + F2PY_OPTIONAL if k in d}
+        kw["num_threads"] = -1 if num_threads is None else num_threads
+        return kw
+
     def get_model_dict(self):
         """The same inputs by name (what yeti_amd.assembly.Assembler also accepts)."""
         return self._flat()
@@ -575,6 +596,14 @@ class IGAparametrization:
                 "bc_target": d["bc_target"], "bc_target_nbelem": d["bc_target_nbelem"], "bc_values": d["bc_values"],
                 "couplg_flag": False, "nb_bc": d["nb_bc"], "mcrd": self._mcrd, "nb_cp": self._nb_cp,
                 "nb_dof_free": self._nb_dof_free}
+
+    def get_inputs4massmat(self):
+        """Positional argument list of massmtrx.build_cmassmatrix (IGA_parametrization.py:1720-1765)."""
+        d = self._flat()
+        return [d["activeElement"], d["nb_data"], self._COORDS, d["IEN"], self._elementsByPatch, self._Nkv, d["Ukv"],
+                self._Nijk, d["weight"], self._Jpqr, d["ELT_TYPE"], d["PROPS"], d["JPROPS"],
+                self._MATERIAL_PROPERTIES[2], d["TENSOR"], self._ind_dof_free, self._nb_dof_free, self._mcrd,
+                self._NBPINT, self._nnode, self._nb_patch, self.nb_elem, self._nb_cp, self._nb_dof_tot]
 
     def get_inputs4evaldisp(self, sol, xi, numpatch=1):
         """Positional argument list of postproc.evaldisp (IGA_parametrization.py:908-938)."""
