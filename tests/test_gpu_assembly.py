@@ -118,8 +118,8 @@ def test_errors_are_raised(gpu):
     with pytest.raises(YetiError) as ei:
         sys_linmat_lindef_static(**a)
     assert ei.value.code == 2
-    a = dict(bp.args)
-    a["NBINT"] = np.array([64], np.int32)     # p=2 with 4 Gauss points per direction: no kernel
+    a = dict(make_box_patch(nel=(2, 2, 2), p=4).args)
+    a["NBINT"] = np.array([216], np.int32)    # p=4 with 6 points: 125 nodes, beyond the general path
     with pytest.raises(YetiError) as ei:
         sys_linmat_lindef_static(**a)
     assert ei.value.code == 3

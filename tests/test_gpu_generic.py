@@ -1,0 +1,69 @@
+"""The general-degree element path (elem_generic_kernel): degree triples and Gauss orders the
+MFMA element kernel is not built for -- unequal degrees per direction, NBINT other than
+(p + 1)^3 -- against the oracle (COO indices bit-exact, values and F within 1e-12 of their
+maximum), in the serial (packed) and OpenMP (slotted) layouts of the drop-ins.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from yeti_amd.geometry import make_box_patch
+from yeti_amd.model import IGAparametrization
+
+MODELS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "models")
+
+
+def box(p, ng):
+    return make_box_patch(nel=(3, 2, 2), p=p, seed=40 + 10 * p + ng, ngauss=ng).f2py_args()
+
+
+def refined(name, deg, ref):
+    m = IGAparametrization(filename=os.path.join(MODELS, name))
+    m.refine(np.array(ref), np.array(deg))
+    return m.get_inputs4system_elemStorage()
+
+
+CASES = {
+    "box p=2 ng=4": lambda: box(2, 4),
+    "box p=1 ng=3": lambda: box(1, 3),
+    "box p=3 ng=3": lambda: box(3, 3),
+    "beam degrees (2,3,1)": lambda: refined("beam-dist", (1, 2, 0), (1, 1, 2)),
+    "beam degrees (1,1,2)": lambda: refined("load_combination/beam-centrif-dloadF2-dloadF4", (0, 0, 1), (0, 0, 0)),
+    "beam degrees (3,2,2)": lambda: refined("volume_force_3D", (2, 1, 1), (1, 1, 1)),
+}
+
+
+def test_cases_take_the_general_path():
+    """Every case here is one the MFMA kernel is not built for (equal degrees, p + 1 points)."""
+    for name, make in CASES.items():
+        a = make()
+        from yeti_amd._abi import args_to_dict
+        d = args_to_dict(a)
+        p = np.asarray(d["Jpqr"]).reshape(-1)[:3]
+        nbint = int(np.asarray(d["NBINT"]).reshape(-1)[0])
+        ng = int(round(nbint ** (1.0 / 3.0)))
+        assert not (p[0] == p[1] == p[2] and ng == p[0] + 1), name
+        assert np.prod(p + 1) <= 64, name
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_gpu_general_degrees_match_oracle(name):
+    from yeti_amd.stiffmtrx_elemstorage import sys_linmat_lindef_static
+    from yeti_amd.stiffmtrx_elemstorage_omp import sys_linmat_lindef_static_omp
+    from yeti_amd._abi import args_to_dict
+    from oracle.oracle import YETI_LAYOUT_SLOTTED
+    print("[case] general degrees: %s" % name, flush=True)
+    args = CASES[name]()
+    K, r, c, F = sys_linmat_lindef_static(*args)
+    Ko, ro, co, Fo = O.oracle_sys_linmat(args)
+    assert np.array_equal(r, ro) and np.array_equal(c, co)
+    assert np.abs(K - Ko).max() <= 1e-12 * np.abs(Ko).max()
+    assert np.abs(F - Fo).max() <= 1e-12 * max(np.abs(Fo).max(), 1e-300)
+    kw = {k.lower(): v for k, v in args_to_dict(args).items()}
+    K2, r2, c2, F2 = sys_linmat_lindef_static_omp(**kw)
+    Ks, rs, cs, _ = O.oracle_sys_linmat([], layout=YETI_LAYOUT_SLOTTED, kwargs=kw)
+    assert np.array_equal(r2, rs) and np.array_equal(c2, cs)
+    assert np.abs(K2 - Ks).max() <= 1e-12 * np.abs(Ks).max()

@@ -3,8 +3,8 @@ models end to end.
 
 Fixtures (tests/golden/models/) are the input files of the reference's benchmarks
 benchs/dload/{beam-dist, beam-dist-field, beam-triangle-1, beam-triangle-2}.{inp,NB},
-benchs/volume_force/volume_force_3D.{inp,NB} and benchs/platevib/plateVolume.{inp,NB}, copied as
-data. Pins:
+benchs/volume_force/volume_force_3D.{inp,NB}, benchs/platevib/plateVolume.{inp,NB} and
+benchs/load_combination/inputs_beam/* (in load_combination/), copied as data. Pins:
 * refinement is exact: the refined NURBS map equals the coarse one at random parametric points
   (geometry and weights; a wrong knot multiplicity, Greville point or homogeneous/Cartesian mix-up
   moves it);
@@ -16,7 +16,10 @@ data. Pins:
   answer), benchs/volume_force/test_volume_force_3d.py (displacement at xi = (1, 1, 0)
   within 2e-3 of the Abaqus reference; body force U102 with a direction) and
   benchs/platevib/test_platevib.py (the 5 lowest eigenfrequencies of K u = lambda M u within
-  1e-3 of the reference's values; stiffness and consistent mass). On the CPU the
+  1e-3 of the reference's values; stiffness and consistent mass) and
+  benchs/load_combination/test_beam_dload_triangle_centrif.py (the right-hand side of two
+  nodal-distribution pressures and a centrifugal load is the sum of the single loads, whatever
+  the order of declaration, to 1e-12). On the CPU the
   assembly is the oracle's; on the GPU it is the drop-in stack (stiffness on the GPU, the
   callers' scipy solve, reconstruction and evaldisp on the GPU).
 """
@@ -180,6 +183,40 @@ def test_omp_inputs_match_serial():
     assert np.linalg.norm(F - F2) / np.linalg.norm(F) < 1e-15
 
 
+COMBI = os.path.join(MODELS, "load_combination")
+
+
+def combination_rhs(assemble):
+    out = {}
+    for name in sorted(set(f[:-4] for f in os.listdir(COMBI) if f.endswith(".inp"))):
+        m = IGAparametrization(filename=os.path.join(COMBI, name))
+        nb_deg = np.zeros((3, 1), np.intp)
+        nb_deg[:, 0] = (0, 0, 1)
+        m.refine(np.zeros((3, 1), np.intp), nb_deg)
+        out[name] = assemble(m.get_inputs4system_elemStorage())
+    return out
+
+
+def check_combinations(f):
+    """The assertions of benchs/load_combination/test_beam_dload_triangle_centrif.py:128-178."""
+    close = lambda a, b: np.allclose(a, b, rtol=1e-12)
+    assert close(f["beam-dloadF2F4"], f["beam-dloadF2"] + f["beam-dloadF4"])
+    assert close(f["beam-dloadF2F4"], f["beam-dloadF4F2"])
+    assert close(f["beam-dloadF2-centrif"], f["beam-dloadF2"] + f["beam-centrif"])
+    assert close(f["beam-dloadF2-centrif"], f["beam-centrif-dloadF2"])
+    assert close(f["beam-dloadF4-centrif"], f["beam-dloadF4"] + f["beam-centrif"])
+    assert close(f["beam-dloadF4-centrif"], f["beam-centrif-dloadF4"])
+    all3 = f["beam-dloadF2-dloadF4-centrif"]
+    assert close(all3, f["beam-dloadF2"] + f["beam-dloadF4"] + f["beam-centrif"])
+    for other in ("beam-dloadF4-dloadF2-centrif", "beam-dloadF2-centrif-dloadF4", "beam-centrif-dloadF2-dloadF4",
+                  "beam-centrif-dloadF4-dloadF2", "beam-dloadF4-centrif-dloadF2"):
+        assert close(all3, f[other]), other
+
+
+def test_bench_load_combination_oracle():
+    check_combinations(combination_rhs(lambda args: O.oracle_sys_linmat(args)[3]))
+
+
 def test_refine_rejects_multipatch():
     m = IGAparametrization(filename=os.path.join(MODELS, "beam-dist"))
     m._nb_patch = 2
@@ -252,3 +289,16 @@ def test_gpu_bench_omp():
     B = sp.coo_matrix((data, (row, col)), shape=(n, n)).tocsc()
     assert spl.norm(A - B) / spl.norm(A) < 1e-15
     assert np.linalg.norm(rhs_seq - rhs_omp) / np.linalg.norm(rhs_seq) < 1e-15
+
+
+@pytest.mark.gpu
+def test_gpu_bench_load_combination():
+    """benchs/load_combination/test_beam_dload_triangle_centrif.py on the drop-in, and each
+    right-hand side against the oracle's."""
+    from yeti_amd.stiffmtrx_elemstorage import sys_linmat_lindef_static as build_stiffmatrix
+    print("[case] bench load_combination", flush=True)
+    f = combination_rhs(lambda args: build_stiffmatrix(*args)[3])
+    check_combinations(f)
+    fo = combination_rhs(lambda args: O.oracle_sys_linmat(args)[3])
+    for k in f:
+        assert np.abs(f[k] - fo[k]).max() <= 1e-12 * np.abs(fo[k]).max(), k

@@ -640,9 +640,17 @@ static int gauss_per_dir(int nbint) {
     return n;
 }
 
-static bool kernel_available(int p0, int p1, int p2, int ng) {
+// the MFMA element kernel is built for equal degrees with p + 1 Gauss points per direction
+static bool mfma_kernel_available(int p0, int p1, int p2, int ng) {
     if (p0 != p1 || p1 != p2) return false;
     return (p0 == 1 && ng == 2) || (p0 == 2 && ng == 3) || (p0 == 3 && ng == 4) || (p0 == 4 && ng == 5);
+}
+
+// every other case goes to elem_generic_kernel: up to 64 nodes, degrees up to 8, up to 9 points
+static bool kernel_available(int p0, int p1, int p2, int ng) {
+    if (mfma_kernel_available(p0, p1, p2, ng)) return true;
+    return p0 >= 1 && p1 >= 1 && p2 >= 1 && p0 <= 8 && p1 <= 8 && p2 <= 8 && (p0 + 1) * (p1 + 1) * (p2 + 1) <= 64 &&
+           ng >= 1 && ng <= MAX_NG;
 }
 
 static int device_cu_count() {
@@ -721,6 +729,251 @@ static int launch_elem(yeti_asm *h, PatchDev &pd, const double *coords, double *
     return 0;
 }
 
+// ---------------------------------------------------------------------------------------
+// Element stiffness for the degree triples and Gauss orders the MFMA kernel is not built for
+// (unequal degrees per direction, NBINT other than (p + 1)^3), up to GEN_MAXN nodes: the same
+// K_ab[k][l] = lam S^kl_ab + mu S^lk_ab + mu delta_kl tr S_ab with S^kl_ab = sum_q w_q |det J_q|
+// dR_a/dx_k dR_b/dx_l (UELMAT.f90:22-225 with stiffmatrix_byCP, stiffmatrix.F90:130-225), in the
+// same COO layout. One 256-thread workgroup per element, Gauss points in chunks of GEN_QC:
+// bases one thread per (point, direction), W / dW / dx/du of a point by 16 lanes and a shuffle
+// sum, dR/dx of every (node, point) in LDS, then each thread adds the chunk to its node pairs
+// (pair t, t + 256, ... in COO order: b (b + 1) / 2 + a, a <= b) in registers and finally writes
+// their blocks.
+constexpr int GEN_MAXN = 64, GEN_MAXP = 8, GEN_QC = 16, GEN_PPT = (GEN_MAXN * (GEN_MAXN + 1) / 2 + 255) / 256;
+
+struct GenPatch {
+    PatchArgs pa;
+    int p[3];
+    int ng;
+};
+
+__global__ void __launch_bounds__(256) elem_generic_kernel(GenPatch gp, double *__restrict__ Kd,
+                                                           int32_t *__restrict__ Kr, int32_t *__restrict__ Kc) {
+    const PatchArgs &pa = gp.pa;
+    const int n0 = gp.p[0] + 1, n1 = gp.p[1] + 1, n2 = gp.p[2] + 1, NN = n0 * n1 * n2;
+    const int NG = gp.ng, NQ = NG * NG * NG, NPAIR = NN * (NN + 1) / 2;
+    constexpr int G = 256 / GEN_QC;
+    __shared__ double xw[GEN_MAXN][4];
+    __shared__ int32_t dof[GEN_MAXN];
+    __shared__ double Bq[GEN_QC][3][GEN_MAXP + 1], dBq[GEN_QC][3][GEN_MAXP + 1];
+    __shared__ double gq[GEN_QC][14];        // du/dx (9), dv = w |det J|, 1/W, dW/du / W (3)
+    __shared__ double Dx[GEN_MAXN][GEN_QC][3];
+    __shared__ double hel[3];
+    const int t = threadIdx.x, qq = t / G, part = t % G;
+    for (int64_t gi = blockIdx.x; gi < pa.n_active; gi += gridDim.x) {
+        const int e = pa.elems[gi];
+        __syncthreads();
+        for (int a = t; a < NN; a += 256) {
+            const int32_t cp = pa.ien[(int64_t)NN * e + a] - 1;
+            xw[a][0] = pa.coords[3 * (int64_t)cp];
+            xw[a][1] = pa.coords[3 * (int64_t)cp + 1];
+            xw[a][2] = pa.coords[3 * (int64_t)cp + 2];
+            xw[a][3] = pa.weight[(int64_t)NN * e + a];
+            dof[a] = 3 * cp;
+        }
+        double S[GEN_PPT][9];
+#pragma unroll
+        for (int k = 0; k < GEN_PPT; ++k)
+#pragma unroll
+            for (int m = 0; m < 9; ++m) S[k][m] = 0.0;
+        for (int q0 = 0; q0 < NQ; q0 += GEN_QC) {
+            const int nq = NQ - q0 < GEN_QC ? NQ - q0 : GEN_QC;
+            __syncthreads();
+            // 1D bases of direction `part` at point qq (the reference's 3D rule: i slowest,
+            // k fastest, GaussLegendre.f90:175-186), local index reversed (shap.f90:94-119)
+            if (part < 3 && qq < nq) {
+                const int q = q0 + qq, d = part, pd = gp.p[d];
+                const int gd = d == 0 ? q / (NG * NG) : d == 1 ? (q / NG) % NG : q % NG;
+                const double *U = d == 0 ? pa.U0 : d == 1 ? pa.U1 : pa.U2;
+                const int span1 = pa.nijk[3 * (int64_t)e + d], i0 = span1 - 1;
+                const double u0 = U[span1 - 1], u1 = U[span1];
+                const double u = ((u1 - u0) * pa.g.x[gd] + (u1 + u0)) * 0.5;
+                double lf[GEN_MAXP + 1], rt[GEN_MAXP + 1], N[GEN_MAXP + 1], lo[GEN_MAXP + 1];
+                N[0] = 1.0;
+                lo[0] = 1.0;
+                for (int j = 1; j <= pd; ++j) {
+                    lf[j] = u - U[i0 + 1 - j];
+                    rt[j] = U[i0 + j] - u;
+                    double saved = 0.0;
+                    for (int r = 0; r < j; ++r) {
+                        const double tmp = N[r] / (rt[r + 1] + lf[j - r]);
+                        N[r] = saved + rt[r + 1] * tmp;
+                        saved = lf[j - r] * tmp;
+                    }
+                    N[j] = saved;
+                    if (j == pd - 1)
+                        for (int r = 0; r < pd; ++r) lo[r] = N[r];
+                }
+                for (int r = 0; r <= pd; ++r) {
+                    const int k = i0 - pd + r;
+                    double dd = 0.0;
+                    if (r >= 1 && U[k + pd] != U[k]) dd += pd * lo[r - 1] / (U[k + pd] - U[k]);
+                    if (r <= pd - 1 && U[k + pd + 1] != U[k + 1]) dd -= pd * lo[r] / (U[k + pd + 1] - U[k + 1]);
+                    Bq[qq][d][pd - r] = N[r];
+                    dBq[qq][d][pd - r] = dd;
+                }
+                if (qq == 0) hel[d] = 0.5 * (u1 - u0);
+            }
+            __syncthreads();
+            // W, dW/du and dx/du of point qq: nodes part, part + G, ..., then a shuffle sum
+            {
+                double s[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) s[k] = 0.0;
+                if (qq < nq) {
+                    for (int a = part; a < NN; a += G) {
+                        const int i = a % n0, j = (a / n0) % n1, k = a / (n0 * n1);
+                        const double w = xw[a][3], x0 = xw[a][0], x1 = xw[a][1], x2 = xw[a][2];
+                        const double B0 = Bq[qq][0][i], B1 = Bq[qq][1][j], B2 = Bq[qq][2][k];
+                        const double Na = B0 * B1 * B2 * w;
+                        const double d0 = dBq[qq][0][i] * B1 * B2 * w;
+                        const double d1 = B0 * dBq[qq][1][j] * B2 * w;
+                        const double d2 = B0 * B1 * dBq[qq][2][k] * w;
+                        s[0] += Na;
+                        s[1] += d0; s[2] += d1; s[3] += d2;
+                        s[4] += x0 * Na; s[5] += x1 * Na; s[6] += x2 * Na;
+                        s[7] += x0 * d0; s[8] += x0 * d1; s[9] += x0 * d2;
+                        s[10] += x1 * d0; s[11] += x1 * d1; s[12] += x1 * d2;
+                        s[13] += x2 * d0; s[14] += x2 * d1; s[15] += x2 * d2;
+                    }
+                }
+#pragma unroll
+                for (int o = 1; o < G; o <<= 1)
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) s[k] += __shfl_xor(s[k], o, 64);
+                if (part == 0) {
+                    double *g = gq[qq];
+                    if (qq < nq) {
+                        const int q = q0 + qq;
+                        const double Wi = 1.0 / s[0];
+                        const double dw[3] = {s[1] * Wi, s[2] * Wi, s[3] * Wi};
+                        double J[3][3];
+                        for (int r = 0; r < 3; ++r)
+                            for (int c = 0; c < 3; ++c) J[r][c] = (s[7 + 3 * r + c] - s[4 + r] * dw[c]) * Wi;
+                        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) +
+                                           J[0][1] * (J[1][2] * J[2][0] - J[1][0] * J[2][2]) +
+                                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+                        const double di = 1.0 / det;
+                        g[0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * di;
+                        g[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * di;
+                        g[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * di;
+                        g[3] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * di;
+                        g[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * di;
+                        g[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * di;
+                        g[6] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * di;
+                        g[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * di;
+                        g[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * di;
+                        const int g0 = q / (NG * NG), g1 = (q / NG) % NG, g2 = q % NG;
+                        g[9] = (pa.g.w[g0] * pa.g.w[g1] * pa.g.w[g2]) * fabs(det * hel[0] * hel[1] * hel[2]);
+                        g[10] = Wi;
+                        g[11] = dw[0];
+                        g[12] = dw[1];
+                        g[13] = dw[2];
+                    } else {
+                        for (int k = 0; k < 14; ++k) g[k] = 0.0;
+                    }
+                }
+            }
+            __syncthreads();
+            // dR/dx = ((dN w) / W - R dW / W) du/dx of every (node, point)
+            for (int x = t; x < NN * GEN_QC; x += 256) {
+                const int a = x / GEN_QC, c = x % GEN_QC;
+                double r3[3] = {0.0, 0.0, 0.0};
+                if (c < nq) {
+                    const int i = a % n0, j = (a / n0) % n1, k = a / (n0 * n1);
+                    const double w = xw[a][3], Wi = gq[c][10];
+                    const double B0 = Bq[c][0][i], B1 = Bq[c][1][j], B2 = Bq[c][2][k];
+                    const double R = B0 * B1 * B2 * w * Wi;
+                    const double du[3] = {dBq[c][0][i] * B1 * B2 * w * Wi - R * gq[c][11],
+                                          B0 * dBq[c][1][j] * B2 * w * Wi - R * gq[c][12],
+                                          B0 * B1 * dBq[c][2][k] * w * Wi - R * gq[c][13]};
+                    for (int m = 0; m < 3; ++m) r3[m] = du[0] * gq[c][m] + du[1] * gq[c][3 + m] + du[2] * gq[c][6 + m];
+                }
+                for (int m = 0; m < 3; ++m) Dx[a][c][m] = r3[m];
+            }
+            __syncthreads();
+            // pair sums S^kl_ab += dv dR_a/dx_k dR_b/dx_l
+#pragma unroll
+            for (int k = 0; k < GEN_PPT; ++k) {
+                const int pi = t + 256 * k;
+                if (pi < NPAIR) {
+                    int b = (int)((sqrt(8.0 * pi + 1.0) - 1.0) * 0.5);   // pi = b (b + 1) / 2 + a
+                    while (b * (b + 1) / 2 > pi) --b;
+                    while ((b + 1) * (b + 2) / 2 <= pi) ++b;
+                    const int a = pi - b * (b + 1) / 2;
+                    for (int c = 0; c < nq; ++c) {
+                        const double dv = gq[c][9];
+                        const double ga0 = dv * Dx[a][c][0], ga1 = dv * Dx[a][c][1], ga2 = dv * Dx[a][c][2];
+                        const double gb0 = Dx[b][c][0], gb1 = Dx[b][c][1], gb2 = Dx[b][c][2];
+                        S[k][0] += ga0 * gb0; S[k][1] += ga0 * gb1; S[k][2] += ga0 * gb2;
+                        S[k][3] += ga1 * gb0; S[k][4] += ga1 * gb1; S[k][5] += ga1 * gb2;
+                        S[k][6] += ga2 * gb0; S[k][7] += ga2 * gb1; S[k][8] += ga2 * gb2;
+                    }
+                }
+            }
+        }
+        // K_ab[k][l] = lam S^kl + mu S^lk + mu delta_kl tr S, written at the element's COO offset
+        const int64_t base = pa.offs[gi];
+        const double lam = pa.lam, mu = pa.mu;
+#pragma unroll
+        for (int k = 0; k < GEN_PPT; ++k) {
+            const int pi = t + 256 * k;
+            if (pi >= NPAIR) continue;
+            int b = (int)((sqrt(8.0 * pi + 1.0) - 1.0) * 0.5);
+            while (b * (b + 1) / 2 > pi) --b;
+            while ((b + 1) * (b + 2) / 2 <= pi) ++b;
+            const int a = pi - b * (b + 1) / 2;
+            const double tr = mu * (S[k][0] + S[k][4] + S[k][8]);
+            double K[3][3];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) K[r][c] = lam * S[k][3 * r + c] + mu * S[k][3 * c + r] + (r == c ? tr : 0.0);
+            const int64_t o = base + coo_start(b) + (a < b ? 9 * a : 9 * b);
+            if (a < b) {
+                for (int x = 0; x < 9; ++x) {
+                    const int kk = x % 3, ll = x / 3;
+                    Kd[o + x] = K[kk][ll];
+                    if (Kr) {
+                        Kr[o + x] = dof[a] + kk;
+                        Kc[o + x] = dof[b] + ll;
+                    }
+                }
+            } else {
+                for (int x = 0; x < 6; ++x) {   // (0,0) (0,1) (1,1) (0,2) (1,2) (2,2)
+                    const int kk = (0x210100 >> (4 * x)) & 15, ll = (0x222110 >> (4 * x)) & 15;
+                    Kd[o + x] = kk == ll ? 0.5 * K[kk][ll] : K[kk][ll];
+                    if (Kr) {
+                        Kr[o + x] = dof[b] + kk;
+                        Kc[o + x] = dof[b] + ll;
+                    }
+                }
+            }
+        }
+    }
+}
+
+static int launch_generic(yeti_asm *h, PatchDev &pd, double *Kd, int32_t *Kr, int32_t *Kc, hipStream_t st) {
+    GenPatch gp{};
+    gp.pa.coords = h->coords.p;
+    gp.pa.ien = pd.ien.p;
+    gp.pa.weight = pd.weight.p;
+    gp.pa.nijk = pd.nijk.p;
+    gp.pa.U0 = pd.U[0].p;
+    gp.pa.U1 = pd.U[1].p;
+    gp.pa.U2 = pd.U[2].p;
+    gp.pa.elems = pd.elems.p;
+    gp.pa.offs = pd.offs.p;
+    gp.pa.n_active = pd.n_active;
+    gp.pa.lam = pd.lam;
+    gp.pa.mu = pd.mu;
+    gauss_legendre(pd.ng, gp.pa.g.x, gp.pa.g.w);
+    for (int d = 0; d < 3; ++d) gp.p[d] = pd.p[d];
+    gp.ng = pd.ng;
+    const unsigned grid = (unsigned)std::min<int64_t>(pd.n_active, 1 << 20);
+    hipLaunchKernelGGL(elem_generic_kernel, dim3(grid), dim3(256), 0, st, gp, Kd, Kr, Kc);
+    YETI_HIP(hipGetLastError());
+    return 0;
+}
+
 static int run_elements(yeti_asm *h, double *Kd, int32_t *Kr, int32_t *Kc, hipStream_t st) {
     h->ev_used = 0;
     for (auto &zr : h->zero_ranges) {
@@ -733,6 +986,11 @@ static int run_elements(yeti_asm *h, double *Kd, int32_t *Kr, int32_t *Kc, hipSt
     for (auto *pd : h->patches) {
         if (!pd->u1 || pd->n_active == 0) continue;
         int rc = 0;
+        if (!mfma_kernel_available(pd->p[0], pd->p[1], pd->p[2], pd->ng)) {
+            rc = launch_generic(h, *pd, Kd, Kr, Kc, st);
+            if (rc) return rc;
+            continue;
+        }
         int key = pd->p[0] * 16 + pd->ng;
         switch (key) {
         case 1 * 16 + 2: rc = launch_elem<1, 2>(h, *pd, h->coords.p, Kd, Kr, Kc, st); break;
@@ -1668,6 +1926,19 @@ static int create(const yeti_system_inputs *in, int layout, yeti_asm **out) {
     std::vector<int32_t> jdl(in->JDLType, in->JDLType + in->nb_load);
     std::vector<int32_t> add_off(in->nb_load + 1, 0);
     for (int il = 0; il < in->nb_load; ++il) add_off[il + 1] = add_off[il] + in->nb_load_additionalInfos[il];
+    if (in->nb_load > 0 && add_off[in->nb_load] > std::max<int64_t>(0, in->len_load_additionalInfos))
+        return fail(set_error(YETI_ERR_INVALID_ARGUMENT, "load_additionalInfos holds %lld values, the loads name %d",
+                              (long long)in->len_load_additionalInfos, add_off[in->nb_load]));
+    // a nodal-distribution load (type 4) reads row `field` of nodal_dist at every CP of its elements
+    for (int il = 0; il < in->nb_load; ++il) {
+        const int t = in->JDLType[il];
+        if (t < 10 || t > 99 || t % 10 != 4) continue;
+        const int field = in->nb_load_additionalInfos[il] > 0 ? (int)in->load_additionalInfos[add_off[il]] : 0;
+        if (field < 1 || field > std::max(1, in->nb_n_dist) || in->nb_cp_n_dist < in->nb_cp)
+            return fail(set_error(YETI_ERR_INVALID_ARGUMENT,
+                                  "load %d: distribution %d not in nodal_dist (%d x %d, %d control points)", il + 1,
+                                  field, in->nb_n_dist, in->nb_cp_n_dist, in->nb_cp));
+    }
     hipError_t e2 = h->F_conc.upload(Fc.data(), Fc.size());
     if (e2 == hipSuccess) e2 = h->jdl.upload(jdl.data(), jdl.size());
     if (e2 == hipSuccess) e2 = h->adlmag.upload(in->ADLMAG, in->nb_load);

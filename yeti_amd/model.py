@@ -563,10 +563,15 @@ class IGAparametrization:
         return d
 
     def _nodal_dist(self):
-        """The distributions named by a Dload, stacked as (sum of columns, nb_cp)
-        (_get_load_info, IGA_parametrization.py:592-611)."""
-        used = [f.T for _, f, u in self._dists if u]
-        return np.asfortranarray(np.vstack(used)) if used else np.zeros((1, self._nb_cp), order="F")
+        """The nodal distributions as rows of (rows, nb_cp), row k = the distribution with label k
+        (1-based, file order), up to the highest label a Dload names. The reference stacks only
+        the distributions a Dload names (_get_load_info, IGA_parametrization.py:592-611) while
+        the loads address rows by label; both agree whenever the named ones come first, which is
+        when the reference's own lookup stays in range."""
+        last = max([k for k, (_, _, u) in enumerate(self._dists, start=1) if u], default=0)
+        if last == 0:
+            return np.zeros((1, self._nb_cp), order="F")
+        return np.asfortranarray(np.vstack([f.T for _, f, _ in self._dists[:last]]))
 
     def get_inputs4system_elemStorage(self):
         """Positional argument list of sys_linmat_lindef_static (IGA_parametrization.py:1705-1716)."""
