@@ -1,7 +1,8 @@
 """The general-degree element path (elem_generic_kernel): degree triples and Gauss orders the
 MFMA element kernel is not built for -- unequal degrees per direction, NBINT other than
 (p + 1)^3 -- against the oracle (COO indices bit-exact, values and F within 1e-12 of their
-maximum), in the serial (packed) and OpenMP (slotted) layouts of the drop-ins.
+maximum), in the serial (packed) and OpenMP (slotted) layouts of the drop-ins, and the
+consistent mass matrix on the same cases.
 """
 import os
 
@@ -67,3 +68,22 @@ def test_gpu_general_degrees_match_oracle(name):
     Ks, rs, cs, _ = O.oracle_sys_linmat([], layout=YETI_LAYOUT_SLOTTED, kwargs=kw)
     assert np.array_equal(r2, rs) and np.array_equal(c2, cs)
     assert np.abs(K2 - Ks).max() <= 1e-12 * np.abs(Ks).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_gpu_general_degrees_mass_match_oracle(name):
+    from yeti_amd._abi import args_to_dict
+    from yeti_amd.assembly import Assembler
+    print("[case] general degrees mass: %s" % name, flush=True)
+    d = args_to_dict(CASES[name]())
+    d["RHO"] = np.full(int(np.asarray(d["nb_elem_patch"]).size), 7.8)
+    asm = Assembler(d)
+    try:
+        M, r, c = asm.run_mass()
+        Mo, ro, co = O.oracle_mass_coo(d)
+        n = Mo.size
+        assert np.array_equal(r[:n].cpu().numpy(), ro) and np.array_equal(c[:n].cpu().numpy(), co)
+        assert np.abs(M[:n].cpu().numpy() - Mo).max() <= 1e-12 * np.abs(Mo).max()
+    finally:
+        asm.close()

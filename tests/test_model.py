@@ -2,7 +2,7 @@
 models end to end.
 
 Fixtures (tests/golden/models/) are the input files of the reference's benchmarks
-benchs/dload/{beam-dist, beam-dist-field, beam-triangle-1, beam-triangle-2}.{inp,NB},
+benchs/dload/{beam-dist, beam-dist-field, beam-triangle-1, beam-triangle-2, beam-dloadF2F4}.{inp,NB},
 benchs/volume_force/volume_force_3D.{inp,NB}, benchs/platevib/plateVolume.{inp,NB} and
 benchs/load_combination/inputs_beam/* (in load_combination/), copied as data. Pins:
 * refinement is exact: the refined NURBS map equals the coarse one at random parametric points
@@ -217,6 +217,19 @@ def test_bench_load_combination_oracle():
     check_combinations(combination_rhs(lambda args: O.oracle_sys_linmat(args)[3]))
 
 
+def test_unused_distribution_builds():
+    """benchs/dload/test_dload_beam_unused_distrib.py: a distribution declared before the one the
+    Dload names. The load reads row `label` of nodal_dist, so the rows run up to that label
+    (the reference's own stacking of the named distributions only would leave it out of range)."""
+    m = IGAparametrization(filename=os.path.join(MODELS, "beam-dloadF2F4"))
+    m.refine(np.array([2, 2, 5]), np.array([1, 1, 1]))
+    d = m.get_model_dict()
+    label = int(d["load_additionalInfos"][0])
+    assert d["nodal_dist"].shape[0] >= label
+    F = O.oracle_sys_linmat(m.get_inputs4system_elemStorage())[3]
+    assert np.isfinite(F).all() and np.abs(F).max() > 0
+
+
 def test_refine_rejects_multipatch():
     m = IGAparametrization(filename=os.path.join(MODELS, "beam-dist"))
     m._nb_patch = 2
@@ -302,3 +315,14 @@ def test_gpu_bench_load_combination():
     fo = combination_rhs(lambda args: O.oracle_sys_linmat(args)[3])
     for k in f:
         assert np.abs(f[k] - fo[k]).max() <= 1e-12 * np.abs(fo[k]).max(), k
+
+
+@pytest.mark.gpu
+def test_gpu_unused_distribution():
+    from yeti_amd.stiffmtrx_elemstorage import sys_linmat_lindef_static as build_stiffmatrix
+    print("[case] bench dload unused distribution", flush=True)
+    m = IGAparametrization(filename=os.path.join(MODELS, "beam-dloadF2F4"))
+    m.refine(np.array([2, 2, 5]), np.array([1, 1, 1]))
+    data, row, col, rhs = build_stiffmatrix(*m.get_inputs4system_elemStorage())
+    F = O.oracle_sys_linmat(m.get_inputs4system_elemStorage())[3]
+    assert np.abs(rhs - F).max() <= 1e-12 * np.abs(F).max()

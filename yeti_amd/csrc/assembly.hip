@@ -745,8 +745,12 @@ struct GenPatch {
     PatchArgs pa;
     int p[3];
     int ng;
+    double rho;
 };
 
+// MODE 0: stiffness (|det J|); MODE 1: consistent mass rho sum_q w_q det J_q R_a R_b on the
+// diagonal entries, det J signed (compute_CMASSMatrix.f:22-215, UMASSMAT.f:220-330)
+template <int MODE>
 __global__ void __launch_bounds__(256) elem_generic_kernel(GenPatch gp, double *__restrict__ Kd,
                                                            int32_t *__restrict__ Kr, int32_t *__restrict__ Kc) {
     const PatchArgs &pa = gp.pa;
@@ -757,7 +761,7 @@ __global__ void __launch_bounds__(256) elem_generic_kernel(GenPatch gp, double *
     __shared__ int32_t dof[GEN_MAXN];
     __shared__ double Bq[GEN_QC][3][GEN_MAXP + 1], dBq[GEN_QC][3][GEN_MAXP + 1];
     __shared__ double gq[GEN_QC][14];        // du/dx (9), dv = w |det J|, 1/W, dW/du / W (3)
-    __shared__ double Dx[GEN_MAXN][GEN_QC][3];
+    __shared__ double Dx[GEN_MAXN][GEN_QC][MODE == 0 ? 3 : 1];
     __shared__ double hel[3];
     const int t = threadIdx.x, qq = t / G, part = t % G;
     for (int64_t gi = blockIdx.x; gi < pa.n_active; gi += gridDim.x) {
@@ -864,7 +868,9 @@ __global__ void __launch_bounds__(256) elem_generic_kernel(GenPatch gp, double *
                         g[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * di;
                         g[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * di;
                         const int g0 = q / (NG * NG), g1 = (q / NG) % NG, g2 = q % NG;
-                        g[9] = (pa.g.w[g0] * pa.g.w[g1] * pa.g.w[g2]) * fabs(det * hel[0] * hel[1] * hel[2]);
+                        const double dj = det * hel[0] * hel[1] * hel[2];
+                        g[9] = MODE == 0 ? (pa.g.w[g0] * pa.g.w[g1] * pa.g.w[g2]) * fabs(dj)
+                                         : gp.rho * (pa.g.w[g0] * pa.g.w[g1] * pa.g.w[g2] * dj);
                         g[10] = Wi;
                         g[11] = dw[0];
                         g[12] = dw[1];
@@ -875,7 +881,7 @@ __global__ void __launch_bounds__(256) elem_generic_kernel(GenPatch gp, double *
                 }
             }
             __syncthreads();
-            // dR/dx = ((dN w) / W - R dW / W) du/dx of every (node, point)
+            // stiffness: dR/dx = ((dN w) / W - R dW / W) du/dx; mass: R, of every (node, point)
             for (int x = t; x < NN * GEN_QC; x += 256) {
                 const int a = x / GEN_QC, c = x % GEN_QC;
                 double r3[3] = {0.0, 0.0, 0.0};
@@ -884,12 +890,17 @@ __global__ void __launch_bounds__(256) elem_generic_kernel(GenPatch gp, double *
                     const double w = xw[a][3], Wi = gq[c][10];
                     const double B0 = Bq[c][0][i], B1 = Bq[c][1][j], B2 = Bq[c][2][k];
                     const double R = B0 * B1 * B2 * w * Wi;
-                    const double du[3] = {dBq[c][0][i] * B1 * B2 * w * Wi - R * gq[c][11],
-                                          B0 * dBq[c][1][j] * B2 * w * Wi - R * gq[c][12],
-                                          B0 * B1 * dBq[c][2][k] * w * Wi - R * gq[c][13]};
-                    for (int m = 0; m < 3; ++m) r3[m] = du[0] * gq[c][m] + du[1] * gq[c][3 + m] + du[2] * gq[c][6 + m];
+                    if (MODE == 0) {
+                        const double du[3] = {dBq[c][0][i] * B1 * B2 * w * Wi - R * gq[c][11],
+                                              B0 * dBq[c][1][j] * B2 * w * Wi - R * gq[c][12],
+                                              B0 * B1 * dBq[c][2][k] * w * Wi - R * gq[c][13]};
+                        for (int m = 0; m < 3; ++m)
+                            r3[m] = du[0] * gq[c][m] + du[1] * gq[c][3 + m] + du[2] * gq[c][6 + m];
+                    } else {
+                        r3[0] = R;
+                    }
                 }
-                for (int m = 0; m < 3; ++m) Dx[a][c][m] = r3[m];
+                for (int m = 0; m < (MODE == 0 ? 3 : 1); ++m) Dx[a][c][m] = r3[m];
             }
             __syncthreads();
             // pair sums S^kl_ab += dv dR_a/dx_k dR_b/dx_l
@@ -901,10 +912,16 @@ __global__ void __launch_bounds__(256) elem_generic_kernel(GenPatch gp, double *
                     while (b * (b + 1) / 2 > pi) --b;
                     while ((b + 1) * (b + 2) / 2 <= pi) ++b;
                     const int a = pi - b * (b + 1) / 2;
+                    if (MODE == 1) {
+                        for (int c = 0; c < nq; ++c) S[k][0] += gq[c][9] * Dx[b][c][0] * Dx[a][c][0];
+                        continue;
+                    }
                     for (int c = 0; c < nq; ++c) {
                         const double dv = gq[c][9];
-                        const double ga0 = dv * Dx[a][c][0], ga1 = dv * Dx[a][c][1], ga2 = dv * Dx[a][c][2];
-                        const double gb0 = Dx[b][c][0], gb1 = Dx[b][c][1], gb2 = Dx[b][c][2];
+                        const double ga0 = dv * Dx[a][c][0], ga1 = dv * Dx[a][c][MODE == 0 ? 1 : 0],
+                                     ga2 = dv * Dx[a][c][MODE == 0 ? 2 : 0];
+                        const double gb0 = Dx[b][c][0], gb1 = Dx[b][c][MODE == 0 ? 1 : 0],
+                                     gb2 = Dx[b][c][MODE == 0 ? 2 : 0];
                         S[k][0] += ga0 * gb0; S[k][1] += ga0 * gb1; S[k][2] += ga0 * gb2;
                         S[k][3] += ga1 * gb0; S[k][4] += ga1 * gb1; S[k][5] += ga1 * gb2;
                         S[k][6] += ga2 * gb0; S[k][7] += ga2 * gb1; S[k][8] += ga2 * gb2;
@@ -926,7 +943,9 @@ __global__ void __launch_bounds__(256) elem_generic_kernel(GenPatch gp, double *
             const double tr = mu * (S[k][0] + S[k][4] + S[k][8]);
             double K[3][3];
             for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c) K[r][c] = lam * S[k][3 * r + c] + mu * S[k][3 * c + r] + (r == c ? tr : 0.0);
+                for (int c = 0; c < 3; ++c)
+                    K[r][c] = MODE == 0 ? lam * S[k][3 * r + c] + mu * S[k][3 * c + r] + (r == c ? tr : 0.0)
+                                        : (r == c ? S[k][0] : 0.0);
             const int64_t o = base + coo_start(b) + (a < b ? 9 * a : 9 * b);
             if (a < b) {
                 for (int x = 0; x < 9; ++x) {
@@ -951,7 +970,8 @@ __global__ void __launch_bounds__(256) elem_generic_kernel(GenPatch gp, double *
     }
 }
 
-static int launch_generic(yeti_asm *h, PatchDev &pd, double *Kd, int32_t *Kr, int32_t *Kc, hipStream_t st) {
+static int launch_generic(yeti_asm *h, PatchDev &pd, double *Kd, int32_t *Kr, int32_t *Kc, hipStream_t st,
+                          bool mass = false) {
     GenPatch gp{};
     gp.pa.coords = h->coords.p;
     gp.pa.ien = pd.ien.p;
@@ -968,8 +988,12 @@ static int launch_generic(yeti_asm *h, PatchDev &pd, double *Kd, int32_t *Kr, in
     gauss_legendre(pd.ng, gp.pa.g.x, gp.pa.g.w);
     for (int d = 0; d < 3; ++d) gp.p[d] = pd.p[d];
     gp.ng = pd.ng;
+    gp.rho = pd.rho;
     const unsigned grid = (unsigned)std::min<int64_t>(pd.n_active, 1 << 20);
-    hipLaunchKernelGGL(elem_generic_kernel, dim3(grid), dim3(256), 0, st, gp, Kd, Kr, Kc);
+    if (mass)
+        hipLaunchKernelGGL(elem_generic_kernel<1>, dim3(grid), dim3(256), 0, st, gp, Kd, Kr, Kc);
+    else
+        hipLaunchKernelGGL(elem_generic_kernel<0>, dim3(grid), dim3(256), 0, st, gp, Kd, Kr, Kc);
     YETI_HIP(hipGetLastError());
     return 0;
 }
@@ -1258,9 +1282,12 @@ static int run_mass(yeti_asm *h, double *Md, int32_t *Mr, int32_t *Mc, hipStream
     }
     for (auto *pd : h->patches) {
         if (!pd->u1 || pd->n_active == 0) continue;
-        if (pd->p[1] != pd->p[0] || pd->p[2] != pd->p[0])
-            return set_error(YETI_ERR_UNSUPPORTED_DEGREE, "mass: degrees must be equal per direction");
         int rc = 0;
+        if (!mfma_kernel_available(pd->p[0], pd->p[1], pd->p[2], pd->ng)) {   // other degree triples / orders
+            rc = launch_generic(h, *pd, Md, Mr, Mc, st, true);
+            if (rc) return rc;
+            continue;
+        }
         switch (pd->p[0] * 16 + pd->ng) {
         case 1 * 16 + 2: rc = launch_mass<1, 2>(h, *pd, Md, Mr, Mc, st); break;
         case 2 * 16 + 3: rc = launch_mass<2, 3>(h, *pd, Md, Mr, Mc, st); break;
