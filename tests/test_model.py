@@ -326,3 +326,18 @@ def test_gpu_unused_distribution():
     data, row, col, rhs = build_stiffmatrix(*m.get_inputs4system_elemStorage())
     F = O.oracle_sys_linmat(m.get_inputs4system_elemStorage())[3]
     assert np.abs(rhs - F).max() <= 1e-12 * np.abs(F).max()
+
+
+@pytest.mark.gpu
+def test_gpu_rejects_distribution_out_of_range():
+    """A nodal-distribution load whose label has no row in nodal_dist is refused on the host
+    (YETI_ERR_INVALID_ARGUMENT) instead of being read past the array on the device."""
+    from yeti_amd._lib import YetiError
+    from yeti_amd.stiffmtrx_elemstorage import sys_linmat_lindef_static as build_stiffmatrix
+    print("[case] distribution label out of range", flush=True)
+    m = dload_model("beam-triangle-1")
+    d = m.get_model_dict()
+    d["load_additionalInfos"] = np.array([2.0])          # nodal_dist has one row
+    with pytest.raises(YetiError) as ei:
+        build_stiffmatrix(**d)
+    assert ei.value.code == 1

@@ -646,10 +646,10 @@ static bool mfma_kernel_available(int p0, int p1, int p2, int ng) {
     return (p0 == 1 && ng == 2) || (p0 == 2 && ng == 3) || (p0 == 3 && ng == 4) || (p0 == 4 && ng == 5);
 }
 
-// every other case goes to elem_generic_kernel: up to 64 nodes, degrees up to 8, up to 9 points
+// every other case goes to elem_generic_kernel: up to 64 nodes, degrees up to 4, up to 9 points
 static bool kernel_available(int p0, int p1, int p2, int ng) {
     if (mfma_kernel_available(p0, p1, p2, ng)) return true;
-    return p0 >= 1 && p1 >= 1 && p2 >= 1 && p0 <= 8 && p1 <= 8 && p2 <= 8 && (p0 + 1) * (p1 + 1) * (p2 + 1) <= 64 &&
+    return p0 >= 1 && p1 >= 1 && p2 >= 1 && p0 <= 4 && p1 <= 4 && p2 <= 4 && (p0 + 1) * (p1 + 1) * (p2 + 1) <= 64 &&
            ng >= 1 && ng <= MAX_NG;
 }
 
@@ -731,7 +731,8 @@ static int launch_elem(yeti_asm *h, PatchDev &pd, const double *coords, double *
 
 // ---------------------------------------------------------------------------------------
 // Element stiffness for the degree triples and Gauss orders the MFMA kernel is not built for
-// (unequal degrees per direction, NBINT other than (p + 1)^3), up to GEN_MAXN nodes: the same
+// (unequal degrees per direction, NBINT other than (p + 1)^3), up to GEN_MAXN nodes and degree
+// GEN_MAXP per direction: the same
 // K_ab[k][l] = lam S^kl_ab + mu S^lk_ab + mu delta_kl tr S_ab with S^kl_ab = sum_q w_q |det J_q|
 // dR_a/dx_k dR_b/dx_l (UELMAT.f90:22-225 with stiffmatrix_byCP, stiffmatrix.F90:130-225), in the
 // same COO layout. One 256-thread workgroup per element, Gauss points in chunks of GEN_QC:
@@ -739,7 +740,7 @@ static int launch_elem(yeti_asm *h, PatchDev &pd, const double *coords, double *
 // sum, dR/dx of every (node, point) in LDS, then each thread adds the chunk to its node pairs
 // (pair t, t + 256, ... in COO order: b (b + 1) / 2 + a, a <= b) in registers and finally writes
 // their blocks.
-constexpr int GEN_MAXN = 64, GEN_MAXP = 8, GEN_QC = 16, GEN_PPT = (GEN_MAXN * (GEN_MAXN + 1) / 2 + 255) / 256;
+constexpr int GEN_MAXN = 64, GEN_MAXP = 4, GEN_QC = 16, GEN_PPT = (GEN_MAXN * (GEN_MAXN + 1) / 2 + 255) / 256;
 
 struct GenPatch {
     PatchArgs pa;
@@ -750,7 +751,9 @@ struct GenPatch {
 
 // MODE 0: stiffness (|det J|); MODE 1: consistent mass rho sum_q w_q det J_q R_a R_b on the
 // diagonal entries, det J signed (compute_CMASSMatrix.f:22-215, UMASSMAT.f:220-330)
-template <int MODE>
+// PPT: node pairs per thread (the launch picks the smallest of 1, 2, 3, 5, 9 that covers the
+// element's pairs, so small elements keep few accumulators live)
+template <int MODE, int PPT>
 __global__ void __launch_bounds__(256) elem_generic_kernel(GenPatch gp, double *__restrict__ Kd,
                                                            int32_t *__restrict__ Kr, int32_t *__restrict__ Kc) {
     const PatchArgs &pa = gp.pa;
@@ -775,9 +778,9 @@ __global__ void __launch_bounds__(256) elem_generic_kernel(GenPatch gp, double *
             xw[a][3] = pa.weight[(int64_t)NN * e + a];
             dof[a] = 3 * cp;
         }
-        double S[GEN_PPT][9];
+        double S[PPT][9];
 #pragma unroll
-        for (int k = 0; k < GEN_PPT; ++k)
+        for (int k = 0; k < PPT; ++k)
 #pragma unroll
             for (int m = 0; m < 9; ++m) S[k][m] = 0.0;
         for (int q0 = 0; q0 < NQ; q0 += GEN_QC) {
@@ -905,7 +908,7 @@ __global__ void __launch_bounds__(256) elem_generic_kernel(GenPatch gp, double *
             __syncthreads();
             // pair sums S^kl_ab += dv dR_a/dx_k dR_b/dx_l
 #pragma unroll
-            for (int k = 0; k < GEN_PPT; ++k) {
+            for (int k = 0; k < PPT; ++k) {
                 const int pi = t + 256 * k;
                 if (pi < NPAIR) {
                     int b = (int)((sqrt(8.0 * pi + 1.0) - 1.0) * 0.5);   // pi = b (b + 1) / 2 + a
@@ -933,7 +936,7 @@ __global__ void __launch_bounds__(256) elem_generic_kernel(GenPatch gp, double *
         const int64_t base = pa.offs[gi];
         const double lam = pa.lam, mu = pa.mu;
 #pragma unroll
-        for (int k = 0; k < GEN_PPT; ++k) {
+        for (int k = 0; k < PPT; ++k) {
             const int pi = t + 256 * k;
             if (pi >= NPAIR) continue;
             int b = (int)((sqrt(8.0 * pi + 1.0) - 1.0) * 0.5);
@@ -990,10 +993,22 @@ static int launch_generic(yeti_asm *h, PatchDev &pd, double *Kd, int32_t *Kr, in
     gp.ng = pd.ng;
     gp.rho = pd.rho;
     const unsigned grid = (unsigned)std::min<int64_t>(pd.n_active, 1 << 20);
-    if (mass)
-        hipLaunchKernelGGL(elem_generic_kernel<1>, dim3(grid), dim3(256), 0, st, gp, Kd, Kr, Kc);
-    else
-        hipLaunchKernelGGL(elem_generic_kernel<0>, dim3(grid), dim3(256), 0, st, gp, Kd, Kr, Kc);
+    const int nn = (pd.p[0] + 1) * (pd.p[1] + 1) * (pd.p[2] + 1), need = (nn * (nn + 1) / 2 + 255) / 256;
+    if (need > GEN_PPT) return set_error(YETI_ERR_UNSUPPORTED_DEGREE, "general element kernel: %d nodes", nn);
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, gp, Kd, Kr, Kc); };
+    if (mass) {
+        if (need <= 1) go(elem_generic_kernel<1, 1>);
+        else if (need <= 2) go(elem_generic_kernel<1, 2>);
+        else if (need <= 3) go(elem_generic_kernel<1, 3>);
+        else if (need <= 5) go(elem_generic_kernel<1, 5>);
+        else go(elem_generic_kernel<1, GEN_PPT>);
+    } else {
+        if (need <= 1) go(elem_generic_kernel<0, 1>);
+        else if (need <= 2) go(elem_generic_kernel<0, 2>);
+        else if (need <= 3) go(elem_generic_kernel<0, 3>);
+        else if (need <= 5) go(elem_generic_kernel<0, 5>);
+        else go(elem_generic_kernel<0, GEN_PPT>);
+    }
     YETI_HIP(hipGetLastError());
     return 0;
 }
