@@ -2,7 +2,8 @@
 MFMA element kernel is not built for -- unequal degrees per direction, NBINT other than
 (p + 1)^3 -- against the oracle (COO indices bit-exact, values and F within 1e-12 of their
 maximum), in the serial (packed) and OpenMP (slotted) layouts of the drop-ins, and the
-consistent mass matrix on the same cases.
+consistent mass matrix and the natural-frequency gradient (run-time-degree instance of the
+gradient kernel) on the same cases.
 """
 import os
 
@@ -87,3 +88,23 @@ def test_gpu_general_degrees_mass_match_oracle(name):
         assert np.abs(M[:n].cpu().numpy() - Mo).max() <= 1e-12 * np.abs(Mo).max()
     finally:
         asm.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_gpu_general_degrees_vib_grad_match_oracle(name):
+    from yeti_amd._abi import args_to_dict
+    from yeti_amd.assembly import Assembler
+    print("[case] general degrees vibration gradient: %s" % name, flush=True)
+    d = args_to_dict(CASES[name]())
+    d["RHO"] = np.full(int(np.asarray(d["nb_elem_patch"]).size), 7.8e-9)
+    ncp = int(np.asarray(d["COORDS3D"]).shape[1])
+    V = np.random.default_rng(5).standard_normal((5, 3, ncp))
+    L = np.linspace(1e6, 4e7, 5)
+    asm = Assembler(d)
+    try:
+        g = asm.run_vib_grad(V, L).cpu().numpy()
+    finally:
+        asm.close()
+    go = O.oracle_grad_vibration(d, V, L)
+    assert np.abs(g - go).max() <= 1e-11 * np.abs(go).max()

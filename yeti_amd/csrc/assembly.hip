@@ -1315,6 +1315,36 @@ static int run_mass(yeti_asm *h, double *Md, int32_t *Mr, int32_t *Mc, hipStream
     return 0;
 }
 
+// B-spline values and first derivatives at u for a run-time degree p <= PMAX, span i (0-based):
+// the triangular recurrence with the degree p - 1 values kept for the derivatives (the contract
+// of dersbasisfuns.f90:30-131, first derivative only)
+template <int PMAX>
+__device__ inline void bspline_ders_rt(int i, int p, double u, const double *__restrict__ U, double *N, double *dN) {
+    double lf[PMAX + 1], rt[PMAX + 1], lo[PMAX + 1];
+    N[0] = 1.0;
+    lo[0] = 1.0;
+    for (int j = 1; j <= p; ++j) {
+        lf[j] = u - U[i + 1 - j];
+        rt[j] = U[i + j] - u;
+        double saved = 0.0;
+        for (int r = 0; r < j; ++r) {
+            const double tmp = N[r] / (rt[r + 1] + lf[j - r]);
+            N[r] = saved + rt[r + 1] * tmp;
+            saved = lf[j - r] * tmp;
+        }
+        N[j] = saved;
+        if (j == p - 1)
+            for (int r = 0; r < p; ++r) lo[r] = N[r];
+    }
+    for (int r = 0; r <= p; ++r) {
+        const int k = i - p + r;
+        double dd = 0.0;
+        if (r >= 1 && U[k + p] != U[k]) dd += p * lo[r - 1] / (U[k + p] - U[k]);
+        if (r <= p - 1 && U[k + p + 1] != U[k + 1]) dd -= p * lo[r] / (U[k + p + 1] - U[k + 1]);
+        dN[r] = dd;
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Natural-frequency shape gradient (gradvibration_fullAN.f:22-189 with gradUELMASSMAT1,
 // gradUELMASSMAT_solid.f:22-332), solid patches, up to VIB_MAXF modes per launch. For mode f with
@@ -1337,11 +1367,21 @@ struct VibModes {
     double vals[VIB_MAXF];
 };
 
-template <int P, int NG>
+// RT = false: degree P in every direction and NG points (compile-time); RT = true: the degrees and
+// Gauss order of `dg` at run time, P and GEN_MAXN bounding the arrays.
+struct VibDegrees {
+    int p[3];
+    int ng;
+};
+
+template <int P, int NG, bool RT = false>
 __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho, VibModes vm, int64_t ebase,
                                                        int64_t mbase, double *__restrict__ eg,
-                                                       double *__restrict__ em) {
-    constexpr int NP1 = P + 1, NN = NP1 * NP1 * NP1, NQ = NG * NG * NG, G = 256 / VQC;
+                                                       double *__restrict__ em, VibDegrees dg = VibDegrees{}) {
+    constexpr int NP1 = P + 1, NN = RT ? GEN_MAXN : NP1 * NP1 * NP1, G = 256 / VQC;
+    const int p0 = RT ? dg.p[0] : P, p1 = RT ? dg.p[1] : P, p2 = RT ? dg.p[2] : P;
+    const int n0 = p0 + 1, n1 = p1 + 1, nn = RT ? n0 * n1 * (p2 + 1) : NN;
+    const int ng = RT ? dg.ng : NG, NQ = ng * ng * ng;
     __shared__ double xw[NN][4];
     __shared__ double uf[VIB_MAXF][NN][3];
     __shared__ double Bq[VQC][3][NP1], dBq[VQC][3][NP1];
@@ -1357,12 +1397,12 @@ __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho,
     for (int64_t gi = blockIdx.x; gi < pa.n_active; gi += gridDim.x) {
         const int e = pa.elems[gi];
         __syncthreads();   // the previous element is done with the LDS arrays
-        for (int a = t; a < NN; a += 256) {
-            const int64_t cp = (int64_t)pa.ien[(int64_t)NN * e + a] - 1;
+        for (int a = t; a < nn; a += 256) {
+            const int64_t cp = (int64_t)pa.ien[(int64_t)nn * e + a] - 1;
             xw[a][0] = pa.coords[3 * cp];
             xw[a][1] = pa.coords[3 * cp + 1];
             xw[a][2] = pa.coords[3 * cp + 2];
-            xw[a][3] = pa.weight[(int64_t)NN * e + a];
+            xw[a][3] = pa.weight[(int64_t)nn * e + a];
             for (int f = 0; f < nf; ++f)
                 for (int c = 0; c < 3; ++c)
                     uf[f][a][c] = vm.vect[(vm.f0 + f) + (int64_t)vm.nf_total * (c + 3 * cp)];
@@ -1375,17 +1415,26 @@ __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho,
             // 1. bases of direction `part` at point qq (shap.f90:75-178 for the parametric form)
             if (part < 3 && qq < nq) {
                 const int q = q0 + qq, d = part;
-                const int gd = d == 0 ? q / (NG * NG) : d == 1 ? (q / NG) % NG : q % NG;
+                const int gd = d == 0 ? q / (ng * ng) : d == 1 ? (q / ng) % ng : q % ng;
                 const double *U = d == 0 ? pa.U0 : d == 1 ? pa.U1 : pa.U2;
                 const int span1 = pa.nijk[3 * (int64_t)e + d];
                 const double u0 = U[span1 - 1], u1 = U[span1];
                 const double u = ((u1 - u0) * pa.g.x[gd] + (u1 + u0)) * 0.5;
                 double N[NP1], dN[NP1];
-                bspline_ders<P>(span1 - 1, u, U, N, dN);
+                if constexpr (RT) {
+                    const int pd = d == 0 ? p0 : d == 1 ? p1 : p2;
+                    bspline_ders_rt<P>(span1 - 1, pd, u, U, N, dN);
+                    for (int i = 0; i <= pd; ++i) {
+                        Bq[qq][d][i] = N[pd - i];
+                        dBq[qq][d][i] = dN[pd - i];
+                    }
+                } else {
+                    bspline_ders<P>(span1 - 1, u, U, N, dN);
 #pragma unroll
-                for (int i = 0; i < NP1; ++i) {
-                    Bq[qq][d][i] = N[P - i];
-                    dBq[qq][d][i] = dN[P - i];
+                    for (int i = 0; i < NP1; ++i) {
+                        Bq[qq][d][i] = N[P - i];
+                        dBq[qq][d][i] = dN[P - i];
+                    }
                 }
                 if (qq == 0) hel[d] = 0.5 * (u1 - u0);
             }
@@ -1397,8 +1446,8 @@ __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho,
                 for (int k = 0; k < 16; ++k) s[k] = 0.0;
                 if (qq < nq) {
 #pragma unroll 1
-                    for (int a = part; a < NN; a += G) {
-                        const int i = a % NP1, j = (a / NP1) % NP1, k = a / (NP1 * NP1);
+                    for (int a = part; a < nn; a += G) {
+                        const int i = a % n0, j = (a / n0) % n1, k = a / (n0 * n1);
                         const double w = xw[a][3], x0 = xw[a][0], x1 = xw[a][1], x2 = xw[a][2];
                         const double B0 = Bq[qq][0][i], B1 = Bq[qq][1][j], B2 = Bq[qq][2][k];
                         const double b12 = B1 * B2 * w;
@@ -1442,7 +1491,7 @@ __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho,
                         gp[6] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * di;
                         gp[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * di;
                         gp[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * di;
-                        const int g0 = q / (NG * NG), g1 = (q / NG) % NG, g2 = q % NG;
+                        const int g0 = q / (ng * ng), g1 = (q / ng) % ng, g2 = q % ng;
                         gp[9] = (pa.g.w[g0] * pa.g.w[g1] * pa.g.w[g2]) * (det * hel[0] * hel[1] * hel[2]);
                         gp[10] = Wi;
                         gp[11] = dw[0];
@@ -1455,11 +1504,11 @@ __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho,
             }
             __syncthreads();
             // 3. R = N w / W and dR/dx = (dN w / W - R dW / W) du/dx
-            for (int x = t; x < NN * VQC; x += 256) {
+            for (int x = t; x < nn * VQC; x += 256) {
                 const int a = x / VQC, c = x % VQC;
                 double r4[4] = {0.0, 0.0, 0.0, 0.0};
                 if (c < nq) {
-                    const int i = a % NP1, j = (a / NP1) % NP1, k = a / (NP1 * NP1);
+                    const int i = a % n0, j = (a / n0) % n1, k = a / (n0 * n1);
                     const double w = xw[a][3], Wi = gq[c][10];
                     const double B0 = Bq[c][0][i], B1 = Bq[c][1][j], B2 = Bq[c][2][k];
                     const double R = B0 * B1 * B2 * w * Wi;
@@ -1481,7 +1530,7 @@ __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho,
                 double u[3] = {0.0, 0.0, 0.0}, D[3][3] = {};   // D[i][k] = du_i/dx_k
                 if (act) {
 #pragma unroll 1
-                    for (int a = part4; a < NN; a += 4) {
+                    for (int a = part4; a < nn; a += 4) {
                         const double R = Rx[a][c][0], g0 = Rx[a][c][1], g1 = Rx[a][c][2], g2 = Rx[a][c][3];
 #pragma unroll
                         for (int i = 0; i < 3; ++i) {
@@ -1533,8 +1582,8 @@ __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho,
             }
             __syncthreads();
             // 5. per (node, mode): g[m] += sum_q T[m][k] dR/dx_k
-            for (int x = t; x < NN * nf; x += 256) {
-                const int a = x % NN, f = x / NN;
+            for (int x = t; x < nn * nf; x += 256) {
+                const int a = x % nn, f = x / nn;
                 double g[3] = {0.0, 0.0, 0.0};
                 for (int c = 0; c < nq; ++c) {
                     const double g0 = Rx[a][c][1], g1 = Rx[a][c][2], g2 = Rx[a][c][3];
@@ -1547,8 +1596,8 @@ __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho,
         }
         if ((t & 3) == 0) mpart[(t >> 2) % VQC][(t >> 2) / VQC] = (t >> 2) / VQC < nf ? macc : 0.0;
         __syncthreads();
-        const int64_t eb = ebase + gi * NN;
-        for (int x = t; x < NN * VIB_MAXF * 3; x += 256) {
+        const int64_t eb = ebase + gi * nn;
+        for (int x = t; x < nn * VIB_MAXF * 3; x += 256) {
             const int a = x / (VIB_MAXF * 3), r = x % (VIB_MAXF * 3), f = r / 3, m = r % 3;
             eg[(eb + a) * (VIB_MAXF * 3) + r] = f < nf ? gacc[f][a][m] : 0.0;
         }
@@ -1591,7 +1640,7 @@ __global__ void __launch_bounds__(256) vib_gather_kernel(int32_t nb_cp, const in
     }
 }
 
-template <int P, int NG>
+template <int P, int NG, bool RT = false>
 static int launch_vib(yeti_asm *h, PatchDev &pd, const VibModes &vm, int64_t ebase, int64_t mbase, hipStream_t st) {
     PatchArgs pa{};
     pa.coords = h->coords.p;
@@ -1605,10 +1654,11 @@ static int launch_vib(yeti_asm *h, PatchDev &pd, const VibModes &vm, int64_t eba
     pa.n_active = pd.n_active;
     pa.lam = pd.lam;
     pa.mu = pd.mu;
-    gauss_legendre(NG, pa.g.x, pa.g.w);
+    gauss_legendre(RT ? pd.ng : NG, pa.g.x, pa.g.w);
+    VibDegrees dg{{pd.p[0], pd.p[1], pd.p[2]}, pd.ng};
     const unsigned grid = (unsigned)std::min<int64_t>(pd.n_active, 1 << 20);
-    hipLaunchKernelGGL((vib_grad_kernel<P, NG>), dim3(grid), dim3(256), 0, st, pa, pd.rho, vm, ebase, mbase,
-                       h->vib_eg.p, h->vib_em.p);
+    hipLaunchKernelGGL((vib_grad_kernel<P, NG, RT>), dim3(grid), dim3(256), 0, st, pa, pd.rho, vm, ebase, mbase,
+                       h->vib_eg.p, h->vib_em.p, dg);
     YETI_HIP(hipGetLastError());
     return 0;
 }
@@ -1667,11 +1717,13 @@ static int run_vib_grad(yeti_asm *h, const double *vect, const double *vals, int
                         hipStream_t st) {
     for (auto *pd : h->patches) {
         if (!pd->u1 || pd->n_active == 0) continue;
-        if (pd->p[1] != pd->p[0] || pd->p[2] != pd->p[0])
-            return set_error(YETI_ERR_UNSUPPORTED_DEGREE, "vibration gradient: degrees must be equal per direction");
-        const int key = pd->p[0] * 16 + pd->ng;
-        if (key != 1 * 16 + 2 && key != 2 * 16 + 3 && key != 3 * 16 + 4 && key != 4 * 16 + 5)
-            return set_error(YETI_ERR_UNSUPPORTED_DEGREE, "vibration gradient: no kernel for p=%d ng=%d", pd->p[0], pd->ng);
+        const bool fixed = mfma_kernel_available(pd->p[0], pd->p[1], pd->p[2], pd->ng);
+        const bool general = pd->p[0] >= 1 && pd->p[1] >= 1 && pd->p[2] >= 1 && pd->p[0] <= GEN_MAXP &&
+                             pd->p[1] <= GEN_MAXP && pd->p[2] <= GEN_MAXP && pd->nnode <= GEN_MAXN && pd->ng >= 1 &&
+                             pd->ng <= MAX_NG;
+        if (!fixed && !general)
+            return set_error(YETI_ERR_UNSUPPORTED_DEGREE, "vibration gradient: no kernel for degrees (%d,%d,%d), ng=%d",
+                             pd->p[0], pd->p[1], pd->p[2], pd->ng);
     }
     int rc = vib_setup(h);
     if (rc) return rc;
@@ -1685,7 +1737,9 @@ static int run_vib_grad(yeti_asm *h, const double *vect, const double *vals, int
         int64_t ebase = 0, mbase = 0;
         for (auto *pd : h->patches) {
             if (!pd->u1 || pd->n_active == 0) continue;
-            switch (pd->p[0] * 16 + pd->ng) {
+            if (!mfma_kernel_available(pd->p[0], pd->p[1], pd->p[2], pd->ng))
+                rc = launch_vib<GEN_MAXP, MAX_NG, true>(h, *pd, vm, ebase, mbase, st);
+            else switch (pd->p[0] * 16 + pd->ng) {
             case 1 * 16 + 2: rc = launch_vib<1, 2>(h, *pd, vm, ebase, mbase, st); break;
             case 2 * 16 + 3: rc = launch_vib<2, 3>(h, *pd, vm, ebase, mbase, st); break;
             case 3 * 16 + 4: rc = launch_vib<3, 4>(h, *pd, vm, ebase, mbase, st); break;
