@@ -2,8 +2,9 @@
 MFMA element kernel is not built for -- unequal degrees per direction, NBINT other than
 (p + 1)^3 -- against the oracle (COO indices bit-exact, values and F within 1e-12 of their
 maximum), in the serial (packed) and OpenMP (slotted) layouts of the drop-ins, and the
-consistent mass matrix and the natural-frequency gradient (run-time-degree instance of the
-gradient kernel) on the same cases.
+consistent mass matrix, the natural-frequency gradient (run-time-degree instance of the
+gradient kernel), the VTU vertex fields and the displacement at parametric points on the same
+cases.
 """
 import os
 
@@ -108,3 +109,26 @@ def test_gpu_general_degrees_vib_grad_match_oracle(name):
         asm.close()
     go = O.oracle_grad_vibration(d, V, L)
     assert np.abs(g - go).max() <= 1e-11 * np.abs(go).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_gpu_general_degrees_postprocessing_match_oracle(name):
+    from yeti_amd._abi import args_to_dict
+    from yeti_amd.postproc import evaldisp_points, svars_solid
+    print("[case] general degrees post-processing: %s" % name, flush=True)
+    d = args_to_dict(CASES[name]())
+    ncp = int(np.asarray(d["COORDS3D"]).shape[1])
+    sol = np.random.default_rng(6).standard_normal((ncp, 3)) * 1e-3
+    kw = dict(IEN=d["IEN"], elementsByPatch=d["nb_elem_patch"], Nkv=d["Nkv"], Ukv=d["Ukv"], Nijk=d["Nijk"],
+              weight=d["weight"], Jpqr=d["Jpqr"], NNODE=d["NNODE"])
+    sv = svars_solid(1, (2, 2, 2), (True, True, True), sol.T, d["COORDS3D"], MATERIAL_PROPERTIES=d["MATERIAL_PROPERTIES"],
+                     **kw)
+    so = O.oracle_svars_solid(d, 0, sol, (2, 2, 2), (True, True, True))
+    assert sv.shape == so.shape
+    assert np.abs(sv[:, :, :6] - so[:, :, :6]).max() <= 1e-13 * np.abs(so[:, :, :6]).max()
+    assert np.abs(sv[:, :, 6:] - so[:, :, 6:]).max() <= 1e-11 * np.abs(so[:, :, 6:]).max()
+    xi = np.random.default_rng(7).uniform(0.0, 1.0, (256, 3))
+    u = evaldisp_points(1, xi, sol.T, **kw)
+    uo = O.oracle_evaldisp(1, xi, sol.T, **kw)
+    assert np.abs(u - uo).max() <= 1e-13 * np.abs(uo).max()
