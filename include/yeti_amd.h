@@ -152,6 +152,12 @@ int32_t yeti_asm_run_mass_coo(yeti_asm *h, double *Mdata, int32_t *Mrow, int32_t
  * u_f^T M u_f of those elements. */
 int32_t yeti_asm_run_vib_grad(yeti_asm *h, const double *vect, const double *vals, int32_t nb_frq, double *gradV,
                               void *stream);
+/* Strain-energy shape derivative on the resident model: g = -d(u^T K u)/dX (3 per control
+ * point, COORDS3D order) over its active solid elements, for any degree triple the stiffness
+ * accepts; the internal-work term of gradLinElastWork_AN (optim/gradtotalwork_fullAN.f90:17-300,
+ * gradUELMAT1adj) for SOL = ADJ = u, in Cartesian form. u and g are device vectors of 3 nb_cp
+ * values (dof-interleaved). */
+int32_t yeti_asm_strain_energy_grad(yeti_asm *h, const double *u, double *g, void *stream);
 /* Right-hand side only (concentrated, surface and body loads). */
 int32_t yeti_asm_run_rhs(yeti_asm *h, double *F_dev, void *stream);
 /* Replace the control-point coordinates (device, 3 nb_cp values, COORDS3D order) used by later

@@ -111,6 +111,51 @@ def test_gpu_general_degrees_vib_grad_match_oracle(name):
     assert np.abs(g - go).max() <= 1e-11 * np.abs(go).max()
 
 
+def _energy(d, u):
+    """u^T K u from the oracle's packed COO (upper blocks, diagonal halved: K = C + C^T)."""
+    K, r, c, _ = O.oracle_sys_linmat([], kwargs={k.lower(): v for k, v in d.items()})
+    return 2.0 * float(np.sum(K * u[r] * u[c]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_gpu_general_degrees_strain_energy_grad(name):
+    """-d(u^T K u)/dX of Assembler.strain_energy_grad against (1) the oracle's natural-frequency
+    gradient with eigenvalue 0 times the mode mass u^T M u (that gradient divides by it), within
+    1e-11 of the maximum, and (2) central differences of the oracle's u^T K(X) u along random
+    directions (step 1e-5 of the model size, within 1e-6 of |g| |d|)."""
+    import torch
+    from yeti_amd._abi import args_to_dict
+    from yeti_amd.assembly import Assembler
+    print("[case] general degrees strain energy gradient: %s" % name, flush=True)
+    d = args_to_dict(CASES[name]())
+    d["RHO"] = np.full(int(np.asarray(d["nb_elem_patch"]).size), 7.8)
+    X = np.asarray(d["COORDS3D"], dtype=np.float64)
+    ncp = X.shape[1]
+    V = np.random.default_rng(6).standard_normal((1, 3, ncp))
+    u = np.ascontiguousarray(V[0].T.reshape(-1))
+    asm = Assembler(d)
+    try:
+        g = asm.strain_energy_grad(torch.from_numpy(u).cuda()).cpu().numpy()
+    finally:
+        asm.close()
+    M, rm, cm = O.oracle_mass_coo(d)
+    mass = 2.0 * float(np.sum(M * u[rm] * u[cm]))
+    go = -O.oracle_grad_vibration(d, V, np.zeros(1))[0].T.reshape(-1) * mass
+    assert np.abs(g - go).max() <= 1e-11 * np.abs(go).max()
+    size = float(np.ptp(X, axis=1).max())
+    rng = np.random.default_rng(7)
+    for k in range(2):
+        dX = rng.standard_normal(X.shape)
+        h = 1e-5 * size
+        dp, dm = dict(d), dict(d)
+        dp["COORDS3D"] = np.asfortranarray(X + h * dX)
+        dm["COORDS3D"] = np.asfortranarray(X - h * dX)
+        fd = -(_energy(dp, u) - _energy(dm, u)) / (2 * h)
+        an = float(g @ dX.T.reshape(-1))
+        assert abs(fd - an) <= 1e-6 * np.linalg.norm(g) * np.linalg.norm(dX), (fd, an)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_gpu_general_degrees_postprocessing_match_oracle(name):

@@ -40,10 +40,12 @@ def q_grad(mf, v):
     return mf.strain_energy_grad(t).cpu().numpy().reshape(-1, 3).T
 
 
+@pytest.mark.parametrize("impl", ["matrix-free", "assembler"])
 @pytest.mark.parametrize("name", ["grad_4x2x2_p2", "unit_cube_p1"])
-def test_strain_energy_grad_matches_reference(gpu, name):
+def test_strain_energy_grad_matches_reference(gpu, name, impl):
+    from yeti_amd.assembly import Assembler
     from yeti_amd.matfree import F64, MatrixFree
-    print("[case] strain energy gradient vs gradLinElastWork_AN", name, flush=True)
+    print("[case] strain energy gradient vs gradLinElastWork_AN", name, impl, flush=True)
     if name == "unit_cube_p1":
         from tests.golden.make_golden import unit_cube_args
         args = unit_cube_args()
@@ -51,7 +53,7 @@ def test_strain_energy_grad_matches_reference(gpu, name):
     else:
         d, bp = fixture()
         args = bp.args
-    mf = MatrixFree(args, precisions=F64)
+    mf = MatrixFree(args, precisions=F64) if impl == "matrix-free" else Assembler(args)
     SOL, ADJ, gi = d["SOL"], d["ADJ"], d["gradWint"]
     for k in range(ADJ.shape[0]):
         g = 0.25 * (q_grad(mf, SOL + ADJ[k]) - q_grad(mf, SOL - ADJ[k]))
@@ -88,6 +90,42 @@ def test_compliance_gradient_matches_finite_differences(gpu):
         assert np.abs(g.sum(axis=1)).max() <= 1e-9 * np.abs(g).max() * g.shape[1]
         hom = float((X0 * g).sum())
         assert abs(hom - 3 * J.item()) <= 1e-8 * abs(J.item())
+
+
+def test_compliance_gradient_general_degrees(gpu):
+    """A model the matrix-free operator does not cover (degrees (2, 3, 1), pressure on a face):
+    CompliancePatch takes the assembler's element-wise strain-energy gradient. Checked against
+    central differences of its own J along random directions (within 1e-6 of |g| |d|, step 1e-5
+    of the model size), translation invariance and homogeneity (sum X . dJ/dX = 3 J)."""
+    import os
+    import torch
+    from yeti_amd.gradient import CompliancePatch, compliance
+    from yeti_amd.model import IGAparametrization
+    print("[case] compliance gradient, degrees (2,3,1)", flush=True)
+    m = IGAparametrization(filename=os.path.join(os.path.dirname(__file__), "golden", "models", "beam-dist"))
+    m.refine(np.array((1, 1, 2)), np.array((1, 2, 0)))
+    patch = CompliancePatch(m.get_inputs4system_elemStorage(), solver="csr", rtol=1e-14)
+    assert patch.mf is None
+    X0 = patch.coords0.clone().cuda()
+    X = X0.clone().requires_grad_(True)
+    J = compliance(X, patch)
+    J.backward()
+    g = X.grad.detach().cpu().numpy()
+    size = float(np.ptp(X0.cpu().numpy().reshape(-1, 3), axis=0).max())
+    rng = np.random.default_rng(3)
+    for k in range(3):
+        dX = rng.standard_normal(g.size)
+        h = 1e-5 * size
+        t = torch.from_numpy(dX).cuda()
+        Jp = compliance(X0 + h * t, patch).item()
+        Jm = compliance(X0 - h * t, patch).item()
+        fd, an = (Jp - Jm) / (2 * h), float(g @ dX)
+        print("  direction %d: fd %.12e, gradient %.12e" % (k, fd, an))
+        assert abs(fd - an) <= 1e-6 * np.linalg.norm(g) * np.linalg.norm(dX)
+    G = g.reshape(-1, 3)
+    assert np.abs(G.sum(axis=0)).max() <= 1e-9 * np.abs(G).max() * G.shape[0]
+    hom = float((X0.cpu().numpy() * g).sum())
+    assert abs(hom - 3 * J.item()) <= 1e-8 * abs(J.item())
 
 
 def test_coordinate_update_round_trip(gpu):

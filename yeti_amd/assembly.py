@@ -162,6 +162,20 @@ class Assembler:
             vf.record_stream(stream)   # the temporary stays allocated until that stream has used it
         return gradV.reshape(self.nb_cp, 3, nf).permute(2, 1, 0)
 
+    def strain_energy_grad(self, u, g=None, stream=None):
+        """-d(u^T K u)/dX (3 nb_cp, COORDS3D order) for the dof-interleaved CUDA vector u, over the
+        active solid elements (any degree triple the stiffness accepts)."""
+        u = u.detach().to(device="cuda", dtype=torch.float64).contiguous()
+        if u.numel() != 3 * self.nb_cp:
+            raise ValueError("u must hold 3 nb_cp = %d values" % (3 * self.nb_cp))
+        if g is None:
+            g = torch.empty(3 * self.nb_cp, dtype=torch.float64, device="cuda")
+        _lib.check(_lib.lib().yeti_asm_strain_energy_grad(self._h, C.c_void_p(u.data_ptr()), C.c_void_p(g.data_ptr()),
+                                                          _stream_handle(stream)))
+        if stream is not None:
+            u.record_stream(stream)
+        return g
+
     def _check_out(self, K, r, c, F):
         if K.dtype != torch.float64 or not K.is_cuda or not K.is_contiguous() or K.numel() < self.nb_data:
             raise ValueError("Kdata must be a contiguous CUDA float64 tensor with >= %d entries" % self.nb_data)

@@ -1365,6 +1365,8 @@ struct VibModes {
     const double *vect;            // (nb_frq, 3, nb_cp) column-major
     int32_t nf_total, f0, nf;      // modes f0 .. f0 + nf - 1 of nf_total
     double vals[VIB_MAXF];
+    double scale;                  // the CP sums are multiplied by scale ...
+    int32_t normalize;             // ... and divided by each mode's mass when set
 };
 
 // RT = false: degree P in every direction and NG points (compile-time); RT = true: the degrees and
@@ -1635,7 +1637,8 @@ __global__ void __launch_bounds__(256) vib_gather_kernel(int32_t nb_cp, const in
         for (int f = 0; f < vm.nf; ++f) {
             double s = 0.0;
             for (int32_t j = ptr[cp]; j < ptr[cp + 1]; ++j) s += eg[ent[j] * (VIB_MAXF * 3) + 3 * f + m];
-            gradV[(vm.f0 + f) + (int64_t)vm.nf_total * (m + 3 * cp)] = s / mass[f];
+            const double v = s * vm.scale;
+            gradV[(vm.f0 + f) + (int64_t)vm.nf_total * (m + 3 * cp)] = vm.normalize ? v / mass[f] : v;
         }
     }
 }
@@ -1714,7 +1717,7 @@ static int vib_setup(yeti_asm *h) {
 }
 
 static int run_vib_grad(yeti_asm *h, const double *vect, const double *vals, int32_t nf_total, double *gradV,
-                        hipStream_t st) {
+                        hipStream_t st, double scale = 1.0, bool normalize = true) {
     for (auto *pd : h->patches) {
         if (!pd->u1 || pd->n_active == 0) continue;
         const bool fixed = mfma_kernel_available(pd->p[0], pd->p[1], pd->p[2], pd->ng);
@@ -1734,6 +1737,8 @@ static int run_vib_grad(yeti_asm *h, const double *vect, const double *vals, int
         vm.f0 = f0;
         vm.nf = std::min<int32_t>(VIB_MAXF, nf_total - f0);
         for (int f = 0; f < vm.nf; ++f) vm.vals[f] = vals[f0 + f];
+        vm.scale = scale;
+        vm.normalize = normalize ? 1 : 0;
         int64_t ebase = 0, mbase = 0;
         for (auto *pd : h->patches) {
             if (!pd->u1 || pd->n_active == 0) continue;
@@ -1764,6 +1769,14 @@ static int run_vib_grad(yeti_asm *h, const double *vect, const double *vals, int
         YETI_HIP(hipGetLastError());
     }
     return 0;
+}
+
+// -d(u^T K u)/dX (3 per CP, COORDS3D order) over the active solid elements: the gradient kernel
+// with eigenvalue 0 (no mass term), the sums scaled by -1 and not normalised. u is dof-interleaved
+// (3 cp + c), which is the (1, 3, nb_cp) column-major layout of one mode.
+static int run_strain_energy_grad(yeti_asm *h, const double *u, double *g, hipStream_t st) {
+    const double zero = 0.0;
+    return run_vib_grad(h, u, &zero, 1, g, st, -1.0, false);
 }
 
 // u^T F shape derivative of the face loads: g = 2 u^T dF/dX (3 per CP), gathered like F.
@@ -2147,6 +2160,11 @@ int32_t yeti_asm_run_vib_grad(yeti_asm *h, const double *vect, const double *val
     if (!h || !vect || !vals || !gradV || nb_frq < 1)
         return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument or nb_frq < 1");
     return yeti::run_vib_grad(h, vect, vals, nb_frq, gradV, (hipStream_t)stream);
+}
+
+int32_t yeti_asm_strain_energy_grad(yeti_asm *h, const double *u, double *g, void *stream) {
+    if (!h || !u || !g) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    return yeti::run_strain_energy_grad(h, u, g, (hipStream_t)stream);
 }
 
 int32_t yeti_asm_run_rhs(yeti_asm *h, double *F, void *stream) {

@@ -20,6 +20,7 @@ import numpy as np
 import torch
 
 from ._abi import args_to_dict
+from ._lib import YetiError
 from .assembly import Assembler
 from .matfree import F64, MatrixFree
 from .solver import StiffnessCSR, free_mask_from_args
@@ -40,7 +41,14 @@ class CompliancePatch:
         self.solver = solver
         self.rtol, self.maxit = float(rtol), int(maxit)
         self.asm = Assembler(d)
-        self.mf = MatrixFree(d, precisions=F64)
+        # the matrix-free operator covers single-patch equal-degree bricks; other models take the
+        # element-wise strain-energy gradient of the assembler and the CSR solve
+        try:
+            self.mf = MatrixFree(d, precisions=F64)
+        except YetiError as e:
+            if solver == "mf" or e.code not in (1, 2, 3):   # device and budget errors propagate
+                raise
+            self.mf = None
         self.csr = StiffnessCSR(self.asm) if solver == "csr" else None
         self.mask = free_mask_from_args(d)
         self.nb_cp = self.asm.nb_cp
@@ -51,7 +59,8 @@ class CompliancePatch:
     def set_coords(self, coords):
         c = coords.detach().reshape(-1).to(device="cuda", dtype=torch.float64).contiguous()
         self.asm.update_coords(c)
-        self.mf.update_coords(c)
+        if self.mf is not None:
+            self.mf.update_coords(c)
 
     def solve(self):
         """(u, F) at the current coordinates; u is 0 on the fixed dofs."""
@@ -68,7 +77,8 @@ class CompliancePatch:
 
     def shape_gradient(self, u):
         """dJ/dX (3 nb_cp, COORDS3D order) for the state u at the current coordinates."""
-        return self.mf.strain_energy_grad(u) + self.asm.load_shape_grad(u)
+        gint = self.mf.strain_energy_grad(u) if self.mf is not None else self.asm.strain_energy_grad(u)
+        return gint + self.asm.load_shape_grad(u)
 
 
 class ComplianceFunction(torch.autograd.Function):
