@@ -230,11 +230,55 @@ def test_unused_distribution_builds():
     assert np.isfinite(F).all() and np.abs(F).max() > 0
 
 
-def test_refine_rejects_multipatch():
-    m = IGAparametrization(filename=os.path.join(MODELS, "beam-dist"))
-    m._nb_patch = 2
+CENTRIF2 = os.path.join(MODELS, "centrif_two_patches")
+CENTRIF_REF = np.array([1.60785E-6, -1.94110E-8, 3.15129E-7])   # benchs/centrif/centrif_U1_U1_*/test_*.py
+
+
+def centrif_two_patches(name):
+    """benchs/centrif/centrif_U1_U1_{1load,2loads}: two single-element trilinear patches stacked
+    in z, joined by the face of CPs 5-8, centrifugal body load; each patch degree +1 and two
+    levels of mid-knot insertion per direction."""
+    m = IGAparametrization(filename=os.path.join(CENTRIF2, name))
+    nb_ref = np.full((3, m.nb_patch), 2, np.intp)
+    nb_deg = np.full((3, m.nb_patch), 1, np.intp)
+    m.refine(nb_ref, nb_deg)
+    return m
+
+
+def centrif_point_sol(m, sol):
+    idx = np.nonzero(np.all(m.coords == np.array([[12.0], [1.5], [0.0]]), axis=0))[0]
+    assert idx.size == 1
+    return sol[idx[0]]
+
+
+def test_refine_two_patches():
+    """Two patches joined by a face: 6^3 refined CPs per patch, the 36 of the shared face counted
+    once (396 CPs, all at distinct points); each refined patch is the coarse one (geometry at
+    random parametric points within 1e-12); the reference's known answer (Abaqus, x within 2 %,
+    the bench's own tolerance); the load given on each patch separately (2loads) gives the same F
+    as on both at once."""
+    coarse = IGAparametrization(filename=os.path.join(CENTRIF2, "centrif_U1_U1_1load"))
+    m = centrif_two_patches("centrif_U1_U1_1load")
+    assert m.nb_patch == 2 and m.nb_elem == 128 and m.nb_cp == 396
+    assert np.unique(m.coords.T, axis=0).shape[0] == 396
+    d, dc = m.get_model_dict(), coarse.get_model_dict()
+    xi = np.random.default_rng(4).uniform(0.0, 1.0, (100, 3))
+    for ip in (1, 2):
+        args = lambda g: (ip, xi, np.asarray(g["COORDS3D"], np.float64), g["IEN"], g["nb_elem_patch"], g["Nkv"],
+                          g["Ukv"], g["Nijk"], g["weight"], g["Jpqr"], g["NNODE"])
+        xc, xf = O.oracle_evaldisp(*args(dc)), O.oracle_evaldisp(*args(d))
+        assert np.abs(xf - xc).max() <= 1e-12 * np.abs(xc).max()
+    K, r, c, F = O.oracle_sys_linmat(m.get_inputs4system_elemStorage())
+    u = centrif_point_sol(m, full_sol(m, solve(m, K, r, c, F)))
+    assert abs((u[0] - CENTRIF_REF[0]) / CENTRIF_REF[0]) < 0.02
+    F2 = O.oracle_sys_linmat(centrif_two_patches("centrif_U1_U1_2loads").get_inputs4system_elemStorage())[3]
+    assert np.abs(F2 - F).max() <= 1e-14 * np.abs(F).max()
+
+
+def test_refine_rejects_nonconforming_faces():
+    m = IGAparametrization(filename=os.path.join(CENTRIF2, "centrif_U1_U1_1load"))
     with pytest.raises(NotImplementedError):
-        m.refine(np.array([1, 1, 1]))
+        m.refine(np.array([[2, 1], [2, 1], [2, 1]]), np.array([[1, 1], [1, 1], [1, 1]]))
 
 
 # --------------------------------------------------------------------------------------- GPU
@@ -326,6 +370,25 @@ def test_gpu_unused_distribution():
     data, row, col, rhs = build_stiffmatrix(*m.get_inputs4system_elemStorage())
     F = O.oracle_sys_linmat(m.get_inputs4system_elemStorage())[3]
     assert np.abs(rhs - F).max() <= 1e-12 * np.abs(F).max()
+
+
+@pytest.mark.gpu
+def test_gpu_bench_centrif_two_patches():
+    """benchs/centrif/centrif_U1_U1_1load/test_centrif_u1_u1_1load.py on the drop-in stack, with
+    the oracle's assembly of the same refined model."""
+    from yeti_amd import reconstructionSOL as rsol
+    from yeti_amd.stiffmtrx_elemstorage import sys_linmat_lindef_static as build_stiffmatrix
+    print("[case] bench centrif, two patches", flush=True)
+    m = centrif_two_patches("centrif_U1_U1_1load")
+    data, row, col, rhs = build_stiffmatrix(*m.get_inputs4system_elemStorage())
+    K, r, c, F = O.oracle_sys_linmat(m.get_inputs4system_elemStorage())
+    assert np.array_equal(row, r) and np.array_equal(col, c)
+    assert np.abs(data - K).max() <= 1e-12 * np.abs(K).max()
+    assert np.abs(rhs - F).max() <= 1e-12 * np.abs(F).max()
+    x = solve(m, data, row, col, rhs)
+    sol, _ = rsol.reconstruction(**m.get_inputs4solution(x))
+    u = centrif_point_sol(m, np.asarray(sol))
+    assert abs((u[0] - CENTRIF_REF[0]) / CENTRIF_REF[0]) < 0.02
 
 
 @pytest.mark.gpu

@@ -257,6 +257,30 @@ def _midpoint_knots(U, levels):
     return U
 
 
+def _face_slices(shape):
+    """Index tuples of the six faces of a 3D grid: xi-, xi+, eta-, eta+, zeta-, zeta+."""
+    out = []
+    for d in range(3):
+        for side in (0, 1):
+            sl = [slice(None)] * 3
+            sl[d] = 0 if side == 0 else shape[d] - 1
+            out.append(tuple(sl))
+    return out
+
+
+def _face_orientation(A, B):
+    """The map t (transpose and/or reversal of the two axes) with t(B) == A for two 2D arrays of
+    CP ids, or None when B is no reorientation of A."""
+    for tr in (False, True):
+        B1 = B.T if tr else B
+        for f0 in (1, -1):
+            for f1 in (1, -1):
+                B2 = B1[::f0, ::f1]
+                if B2.shape == A.shape and np.array_equal(B2, A):
+                    return lambda Y, tr=tr, f0=f0, f1=f1: (Y.T if tr else Y)[::f0, ::f1]
+    return None
+
+
 def refinement_matrix(Uc, pc, Uf, pf):
     """T (n_fine x n_coarse) with N_coarse_j = sum_i T_ij N_fine_i: the coarse space lies in the
     fine one, so interpolation at the fine Greville abscissae is exact."""
@@ -372,100 +396,171 @@ class IGAparametrization:
 
     # -- refinement ----------------------------------------------------------------------------
     def refine(self, nb_refinementByDirection, nb_degreeElevationByDirection=np.array([0, 0, 0])):
-        """k-refinement of a single solid patch (IGA_parametrization.py:2294-2437): degree
-        elevation by nb_degreeElevationByDirection, then nb_refinementByDirection levels of
-        mid-knot insertion; per-direction arrays of shape (3,) or (3, nb_patch)."""
-        if self._nb_patch != 1 or self._elt_type[0] != "U1" or self._dim[0] != 3:
-            raise NotImplementedError("refine: single-patch 3D solid (U1) models only")
-        nref = np.asarray(nb_refinementByDirection, np.int64).reshape(3, -1)[:, 0]
-        ndeg = np.asarray(nb_degreeElevationByDirection, np.int64).reshape(3, -1)[:, 0]
-        p0 = self._Jpqr[:, 0].copy()
-        Uc = self._Ukv[0]
-        nc = [Uc[d].size - p0[d] - 1 for d in range(3)]
-        grid = self._cp_grid(0)                               # (nc0, nc1, nc2) -> CP id
-        # homogeneous coarse net
-        w = np.zeros(nc)
-        for e in range(self.nb_elem):
-            for l, (i, j, k) in enumerate(self._local_nodes(p0)):
+        """k-refinement of 3D solid patches (IGA_parametrization.py:2294-2437, iga_refinement in
+        IGA_refinementFcts.py:880-1015): per patch, degree elevation by
+        nb_degreeElevationByDirection, then nb_refinementByDirection levels of mid-knot insertion;
+        arrays of shape (3,) (every patch alike) or (3, k) (patch by patch; patches beyond k are
+        not refined).
+
+        Patches that share a whole face of control points stay joined: the refined CPs of that
+        face keep the numbers the earlier patch gave them and the other CPs are numbered patch by
+        patch, xi fastest (build_coordsFine / updateIndCPbyPatch, IGA_refinementFcts.py:385-552).
+        Sharing only edges or vertices, or faces refined differently on the two sides, raises
+        NotImplementedError."""
+        if any(t != "U1" for t in self._elt_type) or np.any(np.asarray(self._dim).reshape(-1) != 3):
+            raise NotImplementedError("refine: 3D solid (U1) patches only")
+        npch = self._nb_patch
+
+        def per_patch(a):
+            a = np.asarray(a, np.int64)
+            if a.ndim == 1:
+                a = np.concatenate([a, np.zeros(3 - a.size, np.int64)])
+                return np.tile(a.reshape(3, 1), (1, npch))
+            if a.shape[1] < npch:
+                a = np.concatenate([a, np.zeros((3, npch - a.shape[1]), np.int64)], axis=1)
+            return a[:, :npch]
+
+        nref, ndeg = per_patch(nb_refinementByDirection), per_patch(nb_degreeElevationByDirection)
+        eoff = np.concatenate([[0], np.cumsum(self._elementsByPatch)]).astype(np.int64)
+        pts = []
+        for ip in range(npch):
+            p0 = self._Jpqr[:, ip].copy()
+            Uc = self._Ukv[ip]
+            nc = [Uc[d].size - p0[d] - 1 for d in range(3)]
+            grid = self._cp_grid(ip)                               # (nc0, nc1, nc2) -> CP id
+            w = np.zeros(nc)
+            for e in range(eoff[ip], eoff[ip + 1]):
                 s = self._Nijk[:, e]
-                w[s[0] - 1 - i, s[1] - 1 - j, s[2] - 1 - k] = self._weight[e][l]
-        X = self._COORDS[:, grid - 1]                          # (3, nc0, nc1, nc2)
-        # nodal fields ride along as further homogeneous coordinates (f w), so each stays the
-        # same rational function (_updateNodalField, IGA_parametrization.py:2474-2497)
-        fields = [f[grid - 1] for _, f, _ in self._dists]     # (nc0, nc1, nc2, n)
-        extra = [np.moveaxis(fd, -1, 0) * w[None] for fd in fields]
-        H = np.concatenate([X * w[None], w[None]] + extra, axis=0)
-        pf = p0 + ndeg
-        Uf = [_midpoint_knots(_elevated_knots(Uc[d], p0[d], ndeg[d]), nref[d]) for d in range(3)]
-        T = [refinement_matrix(Uc[d], p0[d], Uf[d], pf[d]) for d in range(3)]
-        Hf = np.einsum("ai,bj,ck,xijk->xabc", T[0], T[1], T[2], H, optimize=True)
-        wf = Hf[3]
-        Xf = Hf[:3] / wf[None]
-        nf = wf.shape
-        k0 = 4
-        new_dists = []
-        for (name, f, used_), fd in zip(self._dists, fields):
-            n = fd.shape[-1]
-            ff = Hf[k0:k0 + n] / wf[None]
-            k0 += n
-            new_dists.append((name, np.moveaxis(ff, 0, -1).reshape(-1, n, order="F"), used_))
-        self._dists = new_dists
-        # refined CPs, xi fastest
-        self._COORDS = np.asfortranarray(Xf.reshape(3, -1, order="F"))
-        gridf = np.arange(1, int(np.prod(nf)) + 1).reshape(nf, order="F")
-        # elements: every non-empty span, xi fastest
-        spans = [np.nonzero(Uf[d][pf[d]:-pf[d] - 1] < Uf[d][pf[d] + 1:Uf[d].size - pf[d]])[0] + pf[d] + 1
-                 for d in range(3)]                            # 1-based span indices
-        old_nijk, old_ienp = self._Nijk.copy(), self._IEN[0].copy()
-        Nijk, IEN, W = [], [], []
-        loc = self._local_nodes(pf)
-        for s2 in spans[2]:
-            for s1 in spans[1]:
-                for s0 in spans[0]:
-                    Nijk.append((s0, s1, s2))
-                    ids = [gridf[s0 - 1 - i, s1 - 1 - j, s2 - 1 - k] for (i, j, k) in loc]
-                    IEN.append(ids)
-                    W.append([wf[s0 - 1 - i, s1 - 1 - j, s2 - 1 - k] for (i, j, k) in loc])
+                for l, (i, j, k) in enumerate(self._local_nodes(p0)):
+                    w[s[0] - 1 - i, s[1] - 1 - j, s[2] - 1 - k] = self._weight[e][l]
+            X = self._COORDS[:, grid - 1]
+            # nodal fields ride along as further homogeneous coordinates (f w), so each stays the
+            # same rational function (_updateNodalField, IGA_parametrization.py:2474-2497)
+            extra = [np.moveaxis(f[grid - 1], -1, 0) * w[None] for _, f, _ in self._dists]
+            H = np.concatenate([X * w[None], w[None]] + extra, axis=0)
+            pf = p0 + ndeg[:, ip]
+            Uf = [_midpoint_knots(_elevated_knots(Uc[d], p0[d], ndeg[d, ip]), nref[d, ip]) for d in range(3)]
+            T = [refinement_matrix(Uc[d], p0[d], Uf[d], pf[d]) for d in range(3)]
+            Hf = np.einsum("ai,bj,ck,xijk->xabc", T[0], T[1], T[2], H, optimize=True)
+            spans = [np.nonzero(Uf[d][pf[d]:-pf[d] - 1] < Uf[d][pf[d] + 1:Uf[d].size - pf[d]])[0] + pf[d] + 1
+                     for d in range(3)]                            # 1-based span indices
+            nijk = [(s0, s1, s2) for s2 in spans[2] for s1 in spans[1] for s0 in spans[0]]
+            pts.append(dict(p0=p0, Uc=Uc, nc=nc, grid=grid, pf=pf, Uf=Uf, Hf=Hf, nf=Hf.shape[1:], nijk=nijk))
+
+        # numbering of the refined CPs
+        count = 0
+        for ip, q in enumerate(pts):
+            gridf = np.zeros(q["nf"], np.int64)
+            for jp in range(ip):
+                o = pts[jp]
+                common = set(np.intersect1d(q["grid"], o["grid"]).tolist())
+                if not common:
+                    continue
+                covered = set()
+                for fa in _face_slices(q["grid"].shape):
+                    A = q["grid"][fa]
+                    for fb in _face_slices(o["grid"].shape):
+                        B = o["grid"][fb]
+                        t = _face_orientation(A, B)
+                        if t is None:
+                            continue
+                        fa_f = _face_slices(q["nf"])[_face_slices(q["grid"].shape).index(fa)]
+                        fb_f = _face_slices(o["nf"])[_face_slices(o["grid"].shape).index(fb)]
+                        Bf = t(o["gridf"][fb_f])
+                        if Bf.shape != gridf[fa_f].shape:
+                            raise NotImplementedError("refine: patches %d and %d are refined differently on "
+                                                      "their shared face" % (jp + 1, ip + 1))
+                        gridf[fa_f] = Bf
+                        covered |= set(A.ravel().tolist())
+                if common - covered:
+                    raise NotImplementedError("refine: patches %d and %d share control points outside a whole "
+                                              "face" % (jp + 1, ip + 1))
+            flat = gridf.ravel(order="F")
+            new = np.flatnonzero(flat == 0)
+            flat[new] = count + 1 + np.arange(new.size)
+            q["gridf"] = flat.reshape(q["nf"], order="F")
+            q["new"] = new
+            count = max(count, int(flat.max()))
+        nb_cp = count
+
+        # coordinates from the patch that numbered the CP; weights and nodal fields patch by
+        # patch, a later patch overwriting a shared CP (as the reference's vectWeight_fine)
+        COORDS = np.zeros((3, nb_cp), order="F")
+        vw = np.zeros(nb_cp)
+        nfield = [f.shape[1] for _, f, _ in self._dists]
+        hfields = [np.zeros((nb_cp, n)) for n in nfield]
+        for q in pts:
+            ids = q["gridf"].ravel(order="F") - 1
+            Hf = q["Hf"].reshape(q["Hf"].shape[0], -1, order="F")
+            wf = Hf[3]
+            COORDS[:, ids[q["new"]]] = Hf[:3, q["new"]] / wf[q["new"]][None]
+            vw[ids] = wf
+            k0 = 4
+            for hfd, n in zip(hfields, nfield):
+                hfd[ids] = Hf[k0:k0 + n].T
+                k0 += n
+        self._dists = [(name, hfd / vw[:, None], used_) for (name, _, used_), hfd in zip(self._dists, hfields)]
+        self._COORDS = COORDS
+
+        # elements: every non-empty span of each patch, xi fastest, patch by patch
+        old_nijk = self._Nijk.copy()
+        Nijk, IEN, W, feoff = [], [], [], [0]
+        for q in pts:
+            loc = self._local_nodes(q["pf"])
+            rows = []
+            for (s0, s1, s2) in q["nijk"]:
+                Nijk.append((s0, s1, s2))
+                ids = [q["gridf"][s0 - 1 - i, s1 - 1 - j, s2 - 1 - k] for (i, j, k) in loc]
+                rows.append(ids)
+                W.append(vw[np.asarray(ids) - 1])
+            IEN.append(np.asarray(rows, np.int64))
+            feoff.append(feoff[-1] + len(rows))
         self._Nijk = np.asarray(Nijk, np.int64).T.copy()
-        self._IEN = [np.asarray(IEN, np.int64)]
+        self._IEN = IEN
         self._weight = [np.asarray(x) for x in W]
-        self._Ukv = [Uf]
-        self._Nkv = np.array([[u.size] for u in Uf], np.int64)
-        self._Jpqr = pf.reshape(3, 1)
-        self._nnode = np.array([int(np.prod(pf + 1))], np.int64)
-        self._elementsByPatch = np.array([len(Nijk)], np.int64)
-        self._NBPINT = np.array([int(np.max(pf) + 1) ** 3], np.int64)
+        self._Ukv = [q["Uf"] for q in pts]
+        self._Nkv = np.array([[q["Uf"][d].size for q in pts] for d in range(3)], np.int64)
+        self._Jpqr = np.array([[q["pf"][d] for q in pts] for d in range(3)], np.int64)
+        self._nnode = np.array([int(np.prod(q["pf"] + 1)) for q in pts], np.int64)
+        self._elementsByPatch = np.array([len(q["nijk"]) for q in pts], np.int64)
+        self._NBPINT = np.array([int(np.max(q["pf"]) + 1) ** 3 for q in pts], np.int64)
+
         # loaded elements: the refined elements inside a loaded coarse element; for a face load
         # (code 10 f + t, f = 1..6: xi-, xi+, eta-, eta+, zeta-, zeta+) only those on that face
         # of it (updateLoads, IGA_refinementFcts.py:700-878)
         def children(coarse_ids, code):
             face = 0 if code > 99 else code // 10
             out = []
-            boxes = []
-            for ce in coarse_ids:
-                s = old_nijk[:, ce - 1]
-                boxes.append([(Uc[d][s[d] - 1], Uc[d][s[d]]) for d in range(3)])
-            for fe, s in enumerate(Nijk):
-                lo = [Uf[d][s[d] - 1] for d in range(3)]
-                hi = [Uf[d][s[d]] for d in range(3)]
-                for b in boxes:
-                    if not all(b[d][0] <= lo[d] and hi[d] <= b[d][1] for d in range(3)):
-                        continue
-                    if face:
-                        d, side = (face - 1) // 2, (face - 1) % 2
-                        if (side == 0 and lo[d] != b[d][0]) or (side == 1 and hi[d] != b[d][1]):
+            for ip, q in enumerate(pts):
+                boxes = []
+                for ce in coarse_ids:
+                    if eoff[ip] < ce <= eoff[ip + 1]:
+                        s = old_nijk[:, ce - 1]
+                        boxes.append([(q["Uc"][d][s[d] - 1], q["Uc"][d][s[d]]) for d in range(3)])
+                if not boxes:
+                    continue
+                Uf = q["Uf"]
+                for fe, s in enumerate(q["nijk"]):
+                    lo = [Uf[d][s[d] - 1] for d in range(3)]
+                    hi = [Uf[d][s[d]] for d in range(3)]
+                    for b in boxes:
+                        if not all(b[d][0] <= lo[d] and hi[d] <= b[d][1] for d in range(3)):
                             continue
-                    out.append(fe + 1)
-                    break
+                        if face:
+                            d, side = (face - 1) // 2, (face - 1) % 2
+                            if (side == 0 and lo[d] != b[d][0]) or (side == 1 and hi[d] != b[d][1]):
+                                continue
+                        out.append(feoff[ip] + fe + 1)
+                        break
             return np.asarray(out, np.int64)
         self._dloads = [(children(ids, code), code, mag, extra) for ids, code, mag, extra in self._dloads]
-        # node sets: whole patch, faces, edges, vertices of the coarse grid -> same on the fine one
-        def map_nodes(ids):
-            ids = set(int(i) for i in ids)
-            mask = np.isin(grid, list(ids))
+
+        # node sets: per patch, the whole patch, a face, edges or vertices of its coarse grid ->
+        # the same on its refined grid; the union over the patches
+        def map_patch(q, mask):
+            grid, gridf, nc, nf = q["grid"], q["gridf"], q["nc"], q["nf"]
             if mask.all():
-                return gridf.ravel(order="F")
-            out = []
+                return gridf.ravel(order="F").tolist()
             nce = [np.array([0, n - 1]) for n in nc]
             for d in range(3):          # faces
                 for side in (0, 1):
@@ -474,7 +569,8 @@ class IGAparametrization:
                     if mask[tuple(sl)].all():
                         slf = [slice(None)] * 3
                         slf[d] = 0 if side == 0 else nf[d] - 1
-                        return gridf[tuple(slf)].ravel(order="F")
+                        return gridf[tuple(slf)].ravel(order="F").tolist()
+            out = []
             for d in range(3):          # edges along direction d
                 o = [x for x in range(3) if x != d]
                 for s0 in (0, 1):
@@ -488,17 +584,29 @@ class IGAparametrization:
                             slf[o[1]] = 0 if s1 == 0 else nf[o[1]] - 1
                             out.extend(gridf[tuple(slf)].ravel().tolist())
             if out:
-                return np.asarray(out, np.int64)
+                return out
             for c in np.argwhere(mask):  # vertices
                 if all(c[d] in (0, nc[d] - 1) for d in range(3)):
-                    out.append(gridf[tuple(0 if c[d] == 0 else nf[d] - 1 for d in range(3))])
-            return np.asarray(out, np.int64)
+                    out.append(int(gridf[tuple(0 if c[d] == 0 else nf[d] - 1 for d in range(3))]))
+            return out
+
+        def map_nodes(ids):
+            ids = list(set(int(i) for i in ids))
+            out = []
+            for q in pts:
+                mask = np.isin(q["grid"], ids)
+                if mask.any():
+                    out.extend(map_patch(q, mask))
+            return np.asarray(list(dict.fromkeys(out)), np.int64)
+
         def map_vertices(ids):   # concentrated loads keep the patch vertices among their CPs
             out = []
             for cp in ids:
-                c = np.argwhere(grid == cp)
-                if c.size and all(c[0][d] in (0, nc[d] - 1) for d in range(3)):
-                    out.append(gridf[tuple(0 if c[0][d] == 0 else nf[d] - 1 for d in range(3))])
+                for q in pts:
+                    c = np.argwhere(q["grid"] == cp)
+                    if c.size and all(c[0][d] in (0, q["nc"][d] - 1) for d in range(3)):
+                        out.append(int(q["gridf"][tuple(0 if c[0][d] == 0 else q["nf"][d] - 1 for d in range(3))]))
+                        break
             return np.asarray(out, np.int64)
         self._bcs = [(map_nodes(ids), dof, v) for ids, dof, v in self._bcs]
         self._cloads = [(map_vertices(ids), dof, v) for ids, dof, v in self._cloads]
@@ -515,10 +623,11 @@ class IGAparametrization:
         n = [self._Ukv[ip][d].size - p[d] - 1 for d in range(3)]
         grid = np.zeros(n, np.int64)
         loc = self._local_nodes(p)
-        for e in range(self.nb_elem):
-            s = self._Nijk[:, e]
+        e0 = int(np.sum(self._elementsByPatch[:ip]))
+        for el in range(int(self._elementsByPatch[ip])):
+            s = self._Nijk[:, e0 + el]
             for l, (i, j, k) in enumerate(loc):
-                grid[s[0] - 1 - i, s[1] - 1 - j, s[2] - 1 - k] = self._IEN[ip][e][l]
+                grid[s[0] - 1 - i, s[1] - 1 - j, s[2] - 1 - k] = self._IEN[ip][el][l]
         if (grid == 0).any():
             raise ValueError("patch %d: control points not covered by its elements" % (ip + 1))
         return grid
