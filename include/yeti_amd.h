@@ -164,9 +164,10 @@ int32_t yeti_asm_run_rhs(yeti_asm *h, double *F_dev, void *stream);
  * runs; the design-update step of the reference's shape optimisation
  * (OPT_modelling.py:268-312 re-marshals COORDS3D before each assembly). */
 int32_t yeti_asm_update_coords(yeti_asm *h, const double *coords_dev, void *stream);
-/* g[3 I + j] = 2 u^T dF/dX_Ij: shape derivative of the face-load work for a displacement u
- * (device, 3 nb_cp). Face loads 10..99 only (YETI's own gradient leaves this term as a stub,
- * optim/gradUELMAT_solid.f90:463-477); body loads return YETI_ERR_UNSUPPORTED_ELEMENT. */
+/* g[3 I + j] = 2 u^T dF/dX_Ij: shape derivative of the load work for a displacement u (device,
+ * 3 nb_cp). Face loads 10..99 (YETI's own gradient leaves this term as a stub,
+ * optim/gradUELMAT_solid.f90:463-477) and body loads 101 / 102 (the centrifugal term of
+ * gradUELMAT1adj, optim/gradUELMAT_solid.f90:369-455, plus the constant volume force). */
 int32_t yeti_asm_load_shape_grad(yeti_asm *h, const double *u_dev, double *g_dev, void *stream);
 /* Measurement: with timing on, each element-kernel launch of the next runs is bracketed by HIP
  * events on its stream; yeti_asm_kernel_times waits for them and returns the durations (ms) of

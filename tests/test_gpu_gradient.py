@@ -128,6 +128,105 @@ def test_compliance_gradient_general_degrees(gpu):
     assert abs(hom - 3 * J.item()) <= 1e-8 * abs(J.item())
 
 
+MODELS = "tests/golden/models"
+
+
+def _body_load_models():
+    from yeti_amd.model import IGAparametrization
+
+    def unit_cube():
+        return IGAparametrization(filename=MODELS + "/grad_solid/unit_cube_U1")
+
+    def volume_force():
+        m = IGAparametrization(filename=MODELS + "/volume_force_3D")
+        m.refine(np.array([1, 1, 1]), np.array([1, 1, 1]))
+        return m
+
+    def centrif_beam():
+        m = IGAparametrization(filename=MODELS + "/load_combination/beam-centrif-dloadF2-dloadF4")
+        m.refine(np.array([1, 1, 2]), np.array([1, 1, 1]))
+        return m
+
+    def two_patches():
+        m = IGAparametrization(filename=MODELS + "/centrif_two_patches/centrif_U1_U1_1load")
+        m.refine(np.full((3, 2), 1, np.intp), np.full((3, 2), 1, np.intp))
+        return m
+    return {"unit cube, centrifugal": unit_cube, "volume force (102)": volume_force,
+            "beam, centrifugal + pressure": centrif_beam, "two patches, centrifugal": two_patches}
+
+
+@pytest.mark.parametrize("name", ["unit cube, centrifugal", "volume force (102)", "beam, centrifugal + pressure",
+                                  "two patches, centrifugal"])
+def test_load_shape_grad_with_body_loads(gpu, name):
+    """g = 2 u^T dF/dX with body loads (101 centrifugal, 102 constant; with face loads on the
+    beam) against central differences of u . F(X) from the oracle's assembly along random
+    directions (step 1e-5 of the model size, within 1e-6 of |g| |d|)."""
+    import torch
+    from oracle import oracle as O
+    from yeti_amd._abi import args_to_dict
+    from yeti_amd.assembly import Assembler
+    print("[case] load shape gradient with body loads:", name, flush=True)
+    d = args_to_dict(_body_load_models()[name]().get_inputs4system_elemStorage())
+    X = np.asarray(d["COORDS3D"], np.float64)
+    rng = np.random.default_rng(8)
+    u = rng.standard_normal(X.size)
+    asm = Assembler(d)
+    try:
+        g = asm.load_shape_grad(torch.from_numpy(u).cuda()).cpu().numpy()
+    finally:
+        asm.close()
+    assert np.abs(g).max() > 0
+
+    def uF(Xs):
+        dd = dict(d)
+        dd["COORDS3D"] = np.asfortranarray(Xs)
+        return float(u @ O.oracle_sys_linmat([], kwargs={k.lower(): v for k, v in dd.items()})[3])
+    h = 1e-5 * float(np.ptp(X, axis=1).max())
+    for k in range(2):
+        dX = rng.standard_normal(X.shape)
+        fd = 2.0 * (uF(X + h * dX) - uF(X - h * dX)) / (2 * h)
+        an = float(g @ dX.T.reshape(-1))
+        print("  direction %d: fd %.12e, gradient %.12e" % (k, fd, an))
+        assert abs(fd - an) <= 1e-6 * np.linalg.norm(g) * np.linalg.norm(dX)
+
+
+def test_compliance_gradient_centrifugal_unit_cube(gpu):
+    """benchs/grad_solid/test_grad_solid_unit_cube_centrif.py: the analytical compliance gradient
+    of the centrifugally loaded unit cube against finite differences (the bench's rtol 1e-5),
+    here of the oracle's J = F . K^-1 F (scipy solve, step 1e-6) for every coordinate of every
+    control point; J itself within 1e-10 of the oracle's."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    from oracle import oracle as O
+    from yeti_amd._abi import args_to_dict
+    from yeti_amd.gradient import CompliancePatch, compliance
+    print("[case] compliance gradient, centrifugal unit cube", flush=True)
+    m = _body_load_models()["unit cube, centrifugal"]()
+    d = args_to_dict(m.get_inputs4system_elemStorage())
+    patch = CompliancePatch(d, solver="csr", rtol=1e-14)
+    X = patch.coords0.clone().cuda().requires_grad_(True)
+    J = compliance(X, patch)
+    J.backward()
+    g = X.grad.detach().cpu().numpy()
+    idof = m.ind_dof_free[:m.nb_dof_free] - 1
+    n = m.nb_dof_tot
+
+    def J_oracle(Xflat):
+        dd = dict(d)
+        dd["COORDS3D"] = np.asfortranarray(Xflat.reshape(-1, 3).T)
+        K, r, c, F = O.oracle_sys_linmat([], kwargs={k.lower(): v for k, v in dd.items()})
+        Ks = sp.coo_matrix((K, (r, c)), shape=(n, n)).tocsc()
+        Kt = (Ks + Ks.T)[idof, :][:, idof]
+        return float(F[idof] @ spl.spsolve(Kt, F[idof]))
+    X0 = patch.coords0.numpy()
+    J0 = J_oracle(X0)
+    assert abs(J.item() - J0) <= 1e-10 * abs(J0)
+    h = 1e-6
+    fd = np.array([(J_oracle(X0 + h * e) - J_oracle(X0 - h * e)) / (2 * h) for e in np.eye(X0.size)])
+    print("  max |g - fd| %.3e, max |g| %.3e" % (np.abs(g - fd).max(), np.abs(g).max()))
+    assert np.all(np.abs(g - fd) <= 1e-5 * np.abs(fd) + 1e-6 * np.abs(fd).max())
+
+
 def test_coordinate_update_round_trip(gpu):
     """Moving the control points and moving them back reproduces J and the gradient bit for bit."""
     import torch

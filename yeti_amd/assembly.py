@@ -71,7 +71,7 @@ class Assembler:
         return [float(buf[i]) for i in range(min(n.value, cap))]
 
     def load_shape_grad(self, u, out=None, stream=None):
-        """g[3 I + j] = 2 u^T dF/dX_Ij for the face loads (FP64)."""
+        """g[3 I + j] = 2 u^T dF/dX_Ij for the face and body loads (FP64)."""
         if u.dtype != torch.float64 or u.numel() != 3 * self.nb_cp or not u.is_cuda:
             raise ValueError("u must be a CUDA float64 tensor of %d entries" % (3 * self.nb_cp))
         if out is None:

@@ -408,7 +408,8 @@ __global__ void __launch_bounds__(64) elem_rhs_kernel(RhsPatch rp, LoadTable lt,
 //     d dS/dX_cj = dR_c/dxi_A (cB x n)_j + dR_c/dxi_B (n x cA)_j,  n = (cA x cB)/dS
 //   normal load (types 0 and 4), F_a = R_a (A x B) w f:
 //     u . d(A x B)/dX_cj = dR_c/dxi_A (B x u)_j + dR_c/dxi_B (u x A)_j
-// with u = u(xi_q) = sum_a R_a u_a. Body loads are not handled here (the caller rejects them).
+// with u = u(xi_q) = sum_a R_a u_a. Body loads (101 centrifugal, 102 constant) at the volume
+// Gauss points as elem_rhs_kernel, derivative below.
 __global__ void __launch_bounds__(64) elem_load_grad_kernel(RhsPatch rp, LoadTable lt, const double *__restrict__ u,
                                                             double *grad_buf) {
     const int le = blockIdx.x;
@@ -437,6 +438,71 @@ __global__ void __launch_bounds__(64) elem_load_grad_kernel(RhsPatch rp, LoadTab
     const int ng = rp.ng;
     const int gle = rp.slot0 + le;
     const int l0 = lt.elem_loads_ptr[gle], l1 = lt.elem_loads_ptr[gle + 1];
+    bool any_body = false;
+    for (int li = l0; li < l1; ++li) {
+        int t = lt.jdltype[lt.elem_loads[li]];
+        any_body |= (t == 101 || t == 102);
+    }
+    // body loads, u . F = sum_q w |J| u(x_q) . f(x_q):
+    //   d/dX_cj = w |J| [ (u . f) dR_c/dx_j + R_c (u . df/dx)_j ],
+    // d|J|/dX_cj = |J| dR_c/dx_j; f = rho omega^2 (I - d d^T)(x - A) for 101 (so
+    // u . df/dx = rho omega^2 (I - d d^T) u), a constant vector for 102.
+    if (any_body) {
+        for (int gi = 0; gi < ng; ++gi)
+            for (int gj = 0; gj < ng; ++gj)
+                for (int gk = 0; gk < ng; ++gk) {
+                    double pt[3] = {rp.g.x[gi], rp.g.x[gj], rp.g.x[gk]};
+                    double AJ[3][3], detJ;
+                    rhs_basis<RHS_MAXN>(rp, e, pt, nn, sW, sX, R, dR, AJ, &detJ);
+                    const double dvol = rp.g.w[gi] * rp.g.w[gj] * rp.g.w[gk] * fabs(detJ);
+                    // rows of the inverse Jacobian: dxi_m/dx = (col_{m+1} x col_{m+2}) / detJ
+                    double col[3][3], cof[3][3];
+                    for (int m = 0; m < 3; ++m)
+                        for (int d = 0; d < 3; ++d) col[m][d] = AJ[d][m];
+                    for (int m = 0; m < 3; ++m) cross3(col[(m + 1) % 3], col[(m + 2) % 3], cof[m]);
+                    double uq[3] = {0.0, 0.0, 0.0}, xq[3] = {0.0, 0.0, 0.0};
+                    int c = 0;
+                    for (int a = lane; a < nn; a += 64, ++c)
+                        for (int d = 0; d < 3; ++d) {
+                            uq[d] += R[c] * sU[a][d];
+                            xq[d] += R[c] * sX[a][d];
+                        }
+                    for (int o = 32; o > 0; o >>= 1)
+                        for (int d = 0; d < 3; ++d) {
+                            uq[d] += __shfl_xor(uq[d], o, 64);
+                            xq[d] += __shfl_xor(xq[d], o, 64);
+                        }
+                    double uf = 0.0, hv[3] = {0.0, 0.0, 0.0};
+                    for (int li = l0; li < l1; ++li) {
+                        const int il = lt.elem_loads[li];
+                        const int t = lt.jdltype[il];
+                        const double *A = lt.add_infos + lt.add_off[il];
+                        if (t == 101) {
+                            double vd[3] = {A[3] - A[0], A[4] - A[1], A[5] - A[2]};
+                            const double nd = sqrt(vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2]);
+                            for (int d = 0; d < 3; ++d) vd[d] /= nd;
+                            const double ag[3] = {xq[0] - A[0], xq[1] - A[1], xq[2] - A[2]};
+                            const double sc = ag[0] * vd[0] + ag[1] * vd[1] + ag[2] * vd[2];
+                            const double ud = uq[0] * vd[0] + uq[1] * vd[1] + uq[2] * vd[2];
+                            const double mag = rp.rho * lt.adlmag[il] * lt.adlmag[il];
+                            for (int d = 0; d < 3; ++d) {
+                                uf += mag * (ag[d] - sc * vd[d]) * uq[d];
+                                hv[d] += mag * (uq[d] - ud * vd[d]);
+                            }
+                        } else if (t == 102) {
+                            uf += lt.adlmag[il] * (A[0] * uq[0] + A[1] * uq[1] + A[2] * uq[2]);
+                        }
+                    }
+                    const double wq = 2.0 * dvol;
+                    c = 0;
+                    for (int a = lane; a < nn; a += 64, ++c)
+                        for (int d = 0; d < 3; ++d) {
+                            const double dRdx =
+                                (dR[c][0] * cof[0][d] + dR[c][1] * cof[1][d] + dR[c][2] * cof[2][d]) / detJ;
+                            g[c][d] += wq * (uf * dRdx + R[c] * hv[d]);
+                        }
+                }
+    }
     for (int li = l0; li < l1; ++li) {
         int il = lt.elem_loads[li];
         int t = lt.jdltype[il];
@@ -594,7 +660,6 @@ struct yeti_asm {
     yeti::DevBuf<int32_t> jdl, add_off, elem_loads, elem_loads_ptr, cp_ptr, cp_ent;
     yeti::DevBuf<double> adlmag, add_infos, nodal_dist, rhs_buf, grad_buf;
     int32_t nb_n_dist = 1;
-    bool body_loads = false;   // any element carries a body load (101 / 102)
     // optional per-launch timing of the element kernel (HIP events on the launch stream)
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -1779,10 +1844,8 @@ static int run_strain_energy_grad(yeti_asm *h, const double *u, double *g, hipSt
     return run_vib_grad(h, u, &zero, 1, g, st, -1.0, false);
 }
 
-// u^T F shape derivative of the face loads: g = 2 u^T dF/dX (3 per CP), gathered like F.
+// u^T F shape derivative of the face and body loads: g = 2 u^T dF/dX (3 per CP), gathered like F.
 static int run_load_grad(yeti_asm *h, const double *u, double *g, hipStream_t st) {
-    if (h->body_loads)
-        return set_error(YETI_ERR_UNSUPPORTED_ELEMENT, "load shape gradient: body loads (101/102) are not supported");
     if (h->n_loaded_total > 0 && !h->grad_buf.p) YETI_HIP(h->grad_buf.alloc((size_t)h->n_loaded_total * 3 * RHS_MAXN));
     LoadTable lt;
     lt.jdltype = h->jdl.p;
@@ -1895,7 +1958,6 @@ static int create(const yeti_system_inputs *in, int layout, yeti_asm **out) {
         for (int64_t k = load_off[il]; k < load_off[il + 1]; ++k) {
             int64_t el = in->indDLoad[k];
             if (el >= 1 && el <= in->nb_elem) {
-                if (t == 101 || t == 102) h->body_loads = true;
                 auto &v = loads_of_elem[el - 1];
                 if (v.empty() || v.back() != il) v.push_back(il);
             }
