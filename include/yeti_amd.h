@@ -158,6 +158,12 @@ int32_t yeti_asm_run_vib_grad(yeti_asm *h, const double *vect, const double *val
  * gradUELMAT1adj) for SOL = ADJ = u, in Cartesian form. u and g are device vectors of 3 nb_cp
  * values (dof-interleaved). */
 int32_t yeti_asm_strain_energy_grad(yeti_asm *h, const double *u, double *g, void *stream);
+/* Volume of the active solid elements and its shape derivative dV/dX (3 per control point,
+ * COORDS3D order), for any degree triple the stiffness accepts: f2py volume.computevolume /
+ * computegradvolume (optim/volume.f90:19-210, optim/gradvolume.f90:16-355; callers
+ * OPT_modelling.py:508-530, all patches active). vol is a device pointer to one double, g a
+ * device vector of 3 nb_cp values. */
+int32_t yeti_asm_volume_grad(yeti_asm *h, double *vol, double *g, void *stream);
 /* Right-hand side only (concentrated, surface and body loads). */
 int32_t yeti_asm_run_rhs(yeti_asm *h, double *F_dev, void *stream);
 /* Replace the control-point coordinates (device, 3 nb_cp values, COORDS3D order) used by later

@@ -32,6 +32,7 @@ SIGNATURES = {
     "yeti_asm_run_rhs": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_asm_run_mass_coo": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_asm_strain_energy_grad": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_asm_volume_grad": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_asm_run_vib_grad": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "yeti_asm_update_coords": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_asm_load_shape_grad": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

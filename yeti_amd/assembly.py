@@ -176,6 +176,15 @@ class Assembler:
             u.record_stream(stream)
         return g
 
+    def volume_grad(self, stream=None):
+        """(V, dV/dX): volume of the active solid elements (0-d CUDA tensor) and its derivative
+        with respect to the control points (3 nb_cp, COORDS3D order)."""
+        vol = torch.empty((), dtype=torch.float64, device="cuda")
+        g = torch.empty(3 * self.nb_cp, dtype=torch.float64, device="cuda")
+        _lib.check(_lib.lib().yeti_asm_volume_grad(self._h, C.c_void_p(vol.data_ptr()), C.c_void_p(g.data_ptr()),
+                                                   _stream_handle(stream)))
+        return vol, g
+
     def _check_out(self, K, r, c, F):
         if K.dtype != torch.float64 or not K.is_cuda or not K.is_contiguous() or K.numel() < self.nb_data:
             raise ValueError("Kdata must be a contiguous CUDA float64 tensor with >= %d entries" % self.nb_data)

@@ -670,6 +670,7 @@ struct yeti_asm {
     yeti::DevBuf<int32_t> vib_ptr;
     yeti::DevBuf<int64_t> vib_ent;
     yeti::DevBuf<double> vib_eg, vib_em, vib_mass;
+    yeti::DevBuf<double> vol_u;    // translation field of the volume gradient
     int64_t vib_nent = -1, vib_nslot = 0;
     ~yeti_asm() {
         for (auto *p : patches) delete p;
@@ -1432,6 +1433,7 @@ struct VibModes {
     double vals[VIB_MAXF];
     double scale;                  // the CP sums are multiplied by scale ...
     int32_t normalize;             // ... and divided by each mode's mass when set
+    int32_t unit_density;          // density 1 in place of the patches' (volume)
 };
 
 // RT = false: degree P in every direction and NG points (compile-time); RT = true: the degrees and
@@ -1725,7 +1727,8 @@ static int launch_vib(yeti_asm *h, PatchDev &pd, const VibModes &vm, int64_t eba
     gauss_legendre(RT ? pd.ng : NG, pa.g.x, pa.g.w);
     VibDegrees dg{{pd.p[0], pd.p[1], pd.p[2]}, pd.ng};
     const unsigned grid = (unsigned)std::min<int64_t>(pd.n_active, 1 << 20);
-    hipLaunchKernelGGL((vib_grad_kernel<P, NG, RT>), dim3(grid), dim3(256), 0, st, pa, pd.rho, vm, ebase, mbase,
+    const double rho = vm.unit_density ? 1.0 : pd.rho;
+    hipLaunchKernelGGL((vib_grad_kernel<P, NG, RT>), dim3(grid), dim3(256), 0, st, pa, rho, vm, ebase, mbase,
                        h->vib_eg.p, h->vib_em.p, dg);
     YETI_HIP(hipGetLastError());
     return 0;
@@ -1782,7 +1785,7 @@ static int vib_setup(yeti_asm *h) {
 }
 
 static int run_vib_grad(yeti_asm *h, const double *vect, const double *vals, int32_t nf_total, double *gradV,
-                        hipStream_t st, double scale = 1.0, bool normalize = true) {
+                        hipStream_t st, double scale = 1.0, bool normalize = true, bool unit_density = false) {
     for (auto *pd : h->patches) {
         if (!pd->u1 || pd->n_active == 0) continue;
         const bool fixed = mfma_kernel_available(pd->p[0], pd->p[1], pd->p[2], pd->ng);
@@ -1804,6 +1807,7 @@ static int run_vib_grad(yeti_asm *h, const double *vect, const double *vals, int
         for (int f = 0; f < vm.nf; ++f) vm.vals[f] = vals[f0 + f];
         vm.scale = scale;
         vm.normalize = normalize ? 1 : 0;
+        vm.unit_density = unit_density ? 1 : 0;
         int64_t ebase = 0, mbase = 0;
         for (auto *pd : h->patches) {
             if (!pd->u1 || pd->n_active == 0) continue;
@@ -1842,6 +1846,28 @@ static int run_vib_grad(yeti_asm *h, const double *vect, const double *vals, int
 static int run_strain_energy_grad(yeti_asm *h, const double *u, double *g, hipStream_t st) {
     const double zero = 0.0;
     return run_vib_grad(h, u, &zero, 1, g, st, -1.0, false);
+}
+
+// u = e_x at every control point (a rigid translation)
+__global__ void translation_kernel(double *u, int64_t n3) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) u[i] = i % 3 == 0 ? 1.0 : 0.0;
+}
+
+// Volume of the active solid elements and its derivative dV/dX (3 per CP): the gradient kernel
+// for the translation u = e_x with density 1 and eigenvalue -1. Then du/dx = 0, |u| = 1, the
+// per-point tensor is w|J| I, so the CP sums are sum_q w|J| dR_a/dx_m = dV/dX_am, and the
+// "mass" u^T M u is sum_q w|J| = V. vol (device, 1 value) and g (device, 3 nb_cp).
+static int run_volume_grad(yeti_asm *h, double *vol, double *g, hipStream_t st) {
+    const int64_t n3 = 3 * (int64_t)h->nb_cp;
+    if (!h->vol_u.p) YETI_HIP(h->vol_u.alloc((size_t)std::max<int64_t>(n3, 1)));
+    hipLaunchKernelGGL(translation_kernel, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st, h->vol_u.p, n3);
+    YETI_HIP(hipGetLastError());
+    const double eig = -1.0;
+    int rc = run_vib_grad(h, h->vol_u.p, &eig, 1, g, st, 1.0, false, true);
+    if (rc) return rc;
+    YETI_HIP(hipMemcpyAsync(vol, h->vib_mass.p, sizeof(double), hipMemcpyDeviceToDevice, st));
+    return 0;
 }
 
 // u^T F shape derivative of the face and body loads: g = 2 u^T dF/dX (3 per CP), gathered like F.
@@ -2227,6 +2253,11 @@ int32_t yeti_asm_run_vib_grad(yeti_asm *h, const double *vect, const double *val
 int32_t yeti_asm_strain_energy_grad(yeti_asm *h, const double *u, double *g, void *stream) {
     if (!h || !u || !g) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
     return yeti::run_strain_energy_grad(h, u, g, (hipStream_t)stream);
+}
+
+int32_t yeti_asm_volume_grad(yeti_asm *h, double *vol, double *g, void *stream) {
+    if (!h || !vol || !g) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    return yeti::run_volume_grad(h, vol, g, (hipStream_t)stream);
 }
 
 int32_t yeti_asm_run_rhs(yeti_asm *h, double *F, void *stream) {
