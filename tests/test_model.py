@@ -275,6 +275,28 @@ def test_refine_two_patches():
     assert np.abs(F2 - F).max() <= 1e-14 * np.abs(F).max()
 
 
+@pytest.mark.parametrize("perm", ["flip xi", "swap xi eta"])
+def test_refine_two_patches_any_orientation(perm):
+    """The second patch re-parametrised (xi reversed, or xi and eta exchanged: the shared face is
+    then met transposed or reversed): same number of CPs, every CP at a distinct point, and the
+    displacement at the reference point within 1e-9 of the unchanged model's."""
+    base = centrif_two_patches("centrif_U1_U1_1load")
+    m = IGAparametrization(filename=os.path.join(CENTRIF2, "centrif_U1_U1_1load"))
+    loc = m._local_nodes(np.array([1, 1, 1]))
+    old = m._IEN[1][0].copy()
+    for l, (i, j, k) in enumerate(loc):
+        src = (1 - i, j, k) if perm == "flip xi" else (j, i, k)
+        m._IEN[1][0][l] = old[loc.index(src)]
+    m.refine(np.full((3, 2), 2, np.intp), np.full((3, 2), 1, np.intp))
+    assert m.nb_cp == base.nb_cp == 396
+    assert np.unique(m.coords.T, axis=0).shape[0] == 396
+    sols = []
+    for mm in (base, m):
+        K, r, c, F = O.oracle_sys_linmat(mm.get_inputs4system_elemStorage())
+        sols.append(centrif_point_sol(mm, full_sol(mm, solve(mm, K, r, c, F))))
+    assert np.abs(sols[1] - sols[0]).max() <= 1e-9 * np.abs(sols[0]).max()
+
+
 def test_refine_rejects_nonconforming_faces():
     m = IGAparametrization(filename=os.path.join(CENTRIF2, "centrif_U1_U1_1load"))
     with pytest.raises(NotImplementedError):
