@@ -20,6 +20,10 @@ Each fixture is data only (inputs and expected outputs):
   compliance J = F . u, and central finite differences (steps 1e-6 and 1e-4) of J at 24
   seeded (control point, direction) pairs, each J evaluated with the compiled serial assembly
   and a direct solve.
+* grad_two_patches_p2.npz: benchs/centrif/centrif_U1_U1_1load (two patches joined by a face,
+  centrifugal load) refined by yeti_amd.model (degree +1, one mid-knot level per direction per
+  patch; the sha256 of its COORDS3D, weights and IEN pin that refinement): gradLinElastWork_AN
+  (internal and external terms) for a seeded state and two seeded adjoints.
 """
 from __future__ import annotations
 
@@ -170,5 +174,22 @@ def grad_fixture():
                         geometry_sha=geo)
 
 
+def two_patch_grad_fixture():
+    from yeti_amd.model import IGAparametrization
+    m = IGAparametrization(filename=os.path.join(HERE, "models", "centrif_two_patches", "centrif_U1_U1_1load"))
+    m.refine(np.full((3, 2), 1, np.intp), np.full((3, 2), 1, np.intp))
+    args = m.get_inputs4system_elemStorage()
+    d = m.get_model_dict()
+    rng = np.random.default_rng(21)
+    SOL = rng.uniform(-1e-3, 1e-3, size=(3, m.nb_cp))
+    ADJ = rng.uniform(-1e-3, 1e-3, size=(2, 3, m.nb_cp))
+    gi, ge = O.ref_grad_linelast_work(args, SOL, ADJ)
+    geo = sha(np.concatenate([np.asarray(d["COORDS3D"]).ravel(order="F"), np.asarray(d["weight"], np.float64)])) + \
+        sha(np.asarray(d["IEN"], np.int64))
+    np.savez_compressed(os.path.join(HERE, "grad_two_patches_p2.npz"), SOL=SOL, ADJ=ADJ, gradWint=gi,
+                        gradWext=ge, geometry_sha=geo)
+
+
 if __name__ == "__main__":
     main()
+    two_patch_grad_fixture()

@@ -279,6 +279,51 @@ def test_volume_and_gradient(gpu, name):
         assert abs(fd - an) <= 1e-7 * np.linalg.norm(g) * np.linalg.norm(dX), (fd, an)
 
 
+@pytest.mark.parametrize("name", ["unit_cube_p1", "grad_two_patches_p2"])
+def test_gradients_match_reference_fixtures(gpu, name):
+    """gradLinElastWork_AN (optim/gradtotalwork_fullAN.f90:17-300) on the centrifugally loaded
+    unit cube and on the two-patch model: its external term for the adjoint a is a^T dF/dX =
+    load_shape_grad(a) / 2 (the centrifugal term of gradUELMAT1adj), its internal term the
+    polarisation (q(u + a) - q(u - a)) / 4 of the assembler's strain-energy gradient; both within
+    1e-10 of their maximum."""
+    import hashlib
+    import torch
+    from yeti_amd._abi import args_to_dict
+    from yeti_amd.assembly import Assembler
+    print("[case] gradients vs gradLinElastWork_AN:", name, flush=True)
+    f = np.load("tests/golden/%s.npz" % name)
+    if name == "unit_cube_p1":
+        from tests.golden.make_golden import unit_cube_args
+        d = unit_cube_args()
+    else:
+        from yeti_amd.model import IGAparametrization
+        m = IGAparametrization(filename=MODELS + "/centrif_two_patches/centrif_U1_U1_1load")
+        m.refine(np.full((3, 2), 1, np.intp), np.full((3, 2), 1, np.intp))
+        d = args_to_dict(m.get_inputs4system_elemStorage())
+        md = m.get_model_dict()
+        geo = hashlib.sha256(np.ascontiguousarray(np.concatenate([np.asarray(md["COORDS3D"]).ravel(order="F"),
+                                                                  np.asarray(md["weight"], np.float64)])).tobytes())
+        geo = geo.hexdigest() + hashlib.sha256(np.ascontiguousarray(np.asarray(md["IEN"], np.int64)).tobytes()).hexdigest()
+        assert geo == str(f["geometry_sha"]), "refinement changed: regenerate the fixture"
+    asm = Assembler(d)
+    try:
+        def dev(v):
+            return torch.from_numpy(np.ascontiguousarray(v.T.reshape(-1))).cuda()
+
+        def host(t):
+            return t.cpu().numpy().reshape(-1, 3).T
+        SOL, ADJ = f["SOL"], f["ADJ"]
+        for k in range(ADJ.shape[0]):
+            ge = 0.5 * host(asm.load_shape_grad(dev(ADJ[k])))
+            gi = 0.25 * (host(asm.strain_energy_grad(dev(SOL + ADJ[k]))) - host(asm.strain_energy_grad(dev(SOL - ADJ[k]))))
+            ee = np.abs(ge - f["gradWext"][k]).max() / np.abs(f["gradWext"][k]).max()
+            ei = np.abs(gi - f["gradWint"][k]).max() / np.abs(f["gradWint"][k]).max()
+            print("  adjoint %d: relative error external %.2e, internal %.2e" % (k, ee, ei))
+            assert ee <= 1e-10 and ei <= 1e-10
+    finally:
+        asm.close()
+
+
 def test_coordinate_update_round_trip(gpu):
     """Moving the control points and moving them back reproduces J and the gradient bit for bit."""
     import torch
