@@ -92,20 +92,28 @@ def test_compliance_gradient_matches_finite_differences(gpu):
         assert abs(hom - 3 * J.item()) <= 1e-8 * abs(J.item())
 
 
-def test_compliance_gradient_general_degrees(gpu):
-    """A model the matrix-free operator does not cover (degrees (2, 3, 1), pressure on a face):
-    CompliancePatch takes the assembler's element-wise strain-energy gradient. Checked against
-    central differences of its own J along random directions (within 1e-6 of |g| |d|, step 1e-5
-    of the model size), translation invariance and homogeneity (sum X . dJ/dX = 3 J)."""
+@pytest.mark.parametrize("name", ["degrees (2,3,1), pressure", "two patches, centrifugal"])
+def test_compliance_gradient_general_degrees(gpu, name):
+    """Models the matrix-free operator does not cover (degrees (2, 3, 1) with pressure on a face;
+    two patches with a centrifugal load): CompliancePatch takes the assembler's element-wise
+    strain-energy gradient. Checked against central differences of its own J along random
+    directions (within 1e-6 of |g| |d|, step 1e-5 of the model size) and homogeneity about the
+    origin, which the rotation axis passes through: sum X . dJ/dX = 3 J for the pressure
+    (K ~ s, F ~ s^2) and 7 J for the centrifugal load (F ~ s^4); translation invariance for the
+    pressure."""
     import os
     import torch
     from yeti_amd.gradient import CompliancePatch, compliance
     from yeti_amd.model import IGAparametrization
-    print("[case] compliance gradient, degrees (2,3,1)", flush=True)
-    m = IGAparametrization(filename=os.path.join(os.path.dirname(__file__), "golden", "models", "beam-dist"))
-    m.refine(np.array((1, 1, 2)), np.array((1, 2, 0)))
+    print("[case] compliance gradient,", name, flush=True)
+    if name.startswith("degrees"):
+        m = IGAparametrization(filename=os.path.join(os.path.dirname(__file__), "golden", "models", "beam-dist"))
+        m.refine(np.array((1, 1, 2)), np.array((1, 2, 0)))
+    else:
+        m = _body_load_models()["two patches, centrifugal"]()
     patch = CompliancePatch(m.get_inputs4system_elemStorage(), solver="csr", rtol=1e-14)
-    assert patch.mf is None
+    if name.startswith("degrees"):
+        assert patch.mf is None
     X0 = patch.coords0.clone().cuda()
     X = X0.clone().requires_grad_(True)
     J = compliance(X, patch)
@@ -123,9 +131,10 @@ def test_compliance_gradient_general_degrees(gpu):
         print("  direction %d: fd %.12e, gradient %.12e" % (k, fd, an))
         assert abs(fd - an) <= 1e-6 * np.linalg.norm(g) * np.linalg.norm(dX)
     G = g.reshape(-1, 3)
-    assert np.abs(G.sum(axis=0)).max() <= 1e-9 * np.abs(G).max() * G.shape[0]
+    if name.startswith("degrees"):
+        assert np.abs(G.sum(axis=0)).max() <= 1e-9 * np.abs(G).max() * G.shape[0]
     hom = float((X0.cpu().numpy() * g).sum())
-    assert abs(hom - 3 * J.item()) <= 1e-8 * abs(J.item())
+    assert abs(hom - (3 if name.startswith("degrees") else 7) * J.item()) <= 1e-8 * abs(J.item())
 
 
 MODELS = "tests/golden/models"

@@ -50,6 +50,45 @@ def test_mf_apply_matches_assembled(gpu, nel, p):
             assert np.abs(D[cp] - blk).max() <= 1e-12 * np.abs(Ad).max()
 
 
+def test_mf_two_patches(gpu):
+    """Two patches joined by a face (the refined centrifugal two-patch bench, p = 2): the apply,
+    the diagonal blocks and the strain-energy gradient sum both patches' contributions on the
+    shared CPs and start from zero on the others. Against the oracle's assembled K (1e-12), its
+    3x3 diagonal blocks, and the assembler's element-wise strain-energy gradient (1e-12),
+    starting from output buffers filled with NaN."""
+    import os
+    import torch
+    from yeti_amd._abi import args_to_dict
+    from yeti_amd.assembly import Assembler
+    from yeti_amd.matfree import MatrixFree
+    from yeti_amd.model import IGAparametrization
+    print("[case] mf two patches", flush=True)
+    m = IGAparametrization(filename=os.path.join(os.path.dirname(__file__), "golden", "models",
+                                                 "centrif_two_patches", "centrif_U1_U1_1load"))
+    m.refine(np.full((3, 2), 1, np.intp), np.full((3, 2), 1, np.intp))
+    d = args_to_dict(m.get_inputs4system_elemStorage())
+    A, _ = oracle_K(d)
+    mf = MatrixFree(d)
+    x = np.random.default_rng(4).standard_normal(m.nb_dof_tot)
+    xt = torch.from_numpy(x).cuda()
+    y = torch.full_like(xt, float("nan"))
+    mf.apply(xt, y)
+    y = y.cpu().numpy()
+    y0 = A @ x
+    assert np.abs(y - y0).max() <= 1e-12 * np.abs(y0).max()
+    D = mf.diag_blocks().cpu().numpy()
+    Ad = A.toarray()
+    for cp in range(m.nb_cp):
+        assert np.abs(D[cp] - Ad[3 * cp:3 * cp + 3, 3 * cp:3 * cp + 3]).max() <= 1e-12 * np.abs(Ad).max()
+    asm = Assembler(d)
+    try:
+        g0 = asm.strain_energy_grad(xt).cpu().numpy()
+    finally:
+        asm.close()
+    g = mf.strain_energy_grad(xt).cpu().numpy()
+    assert np.abs(g - g0).max() <= 1e-12 * np.abs(g0).max()
+
+
 def test_mf_solves_cantilever(gpu):
     import torch
     from yeti_amd.matfree import MatrixFree

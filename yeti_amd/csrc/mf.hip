@@ -494,6 +494,13 @@ __global__ void mf_gather_kernel(MfPatch mp, int32_t br, int32_t bn, int32_t nb0
 }
 
 template <typename T>
+__global__ void fill_zero_pred_kernel(T *p, int64_t n, const int32_t *pred) {
+    if (pred && *pred == 0) return;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = T(0);
+}
+
+template <typename T>
 __global__ void fill_kernel(T *p, int64_t n, T v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -523,6 +530,7 @@ struct MfPatchDev {
 struct yeti_mf {
     int32_t nb_cp;
     int precisions;     // bit 0: FP64, bit 1: FP32
+    bool zero_first = false;   // several patches, or CPs in no patch: y is cleared, then every patch adds
     std::vector<yeti::MfPatchDev *> patches;
     yeti::DevBuf<double> coords, w64;
     yeti::DevBuf<float> w32;
@@ -560,6 +568,11 @@ static int dispatch_brick(const MfPatchDev &pd, const T *geo, const T *x, const 
 template <typename T, int MODE>
 static int mf_run(const yeti_mf *m, const T *x, T *y, hipStream_t st, const int32_t *pred = nullptr) {
     constexpr int NC = MODE == 1 ? 9 : 3;
+    if (m->zero_first) {
+        const int64_t n = (int64_t)NC * m->nb_cp;
+        hipLaunchKernelGGL((fill_zero_pred_kernel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, y, n, pred);
+        YETI_HIP(hipGetLastError());
+    }
     for (size_t ip = 0; ip < m->patches.size(); ++ip) {
         const MfPatchDev &pd = *m->patches[ip];
         const T *geo;
@@ -581,7 +594,7 @@ static int mf_run(const yeti_mf *m, const T *x, T *y, hipStream_t st, const int3
         const int32_t *nbk = pd.nbrick[MODE == 1 ? 1 : 0];
         int64_t ncp = (int64_t)pd.mp.ncp[0] * pd.mp.ncp[1] * pd.mp.ncp[2];
         hipLaunchKernelGGL((mf_gather_kernel<T, NC>), dim3((unsigned)((ncp + 255) / 256)), dim3(256), 0, st, pd.mp, br,
-                           bn, nbk[0], nbk[1], nbk[2], part, y, ip > 0 ? 1 : 0, pred);
+                           bn, nbk[0], nbk[1], nbk[2], part, y, (m->zero_first || ip > 0) ? 1 : 0, pred);
         YETI_HIP(hipGetLastError());
     }
     return 0;
@@ -1270,6 +1283,13 @@ static int32_t mf_create(const yeti_system_inputs *in, int32_t precisions, hipSt
         ieoff += (int64_t)nn * ne;
         eloff += ne;
     }
+    // a CP shared by two patches gets both patches' sums; a CP in no patch gets 0
+    std::vector<char> covered((size_t)m->nb_cp, 0);
+    for (auto *pd : m->patches)
+        for (int32_t v : pd->hcpmap)
+            if (v >= 0 && v < m->nb_cp) covered[(size_t)v] = 1;
+    m->zero_first = m->patches.size() > 1;
+    for (char c : covered) m->zero_first |= c == 0;
     *out = m;
     return 0;
 }
