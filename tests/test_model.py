@@ -297,6 +297,27 @@ def test_refine_two_patches_any_orientation(perm):
     assert np.abs(sols[1] - sols[0]).max() <= 1e-9 * np.abs(sols[0]).max()
 
 
+def test_refine_additional_knots():
+    """additional_knots go in after the degree elevation and before the mid-knot levels
+    (patchRefinement, IGA_refinementFcts.py:185-269): xi = 0.3 then two levels gives the breaks
+    0, .075, .15, .225, .3, .475, .65, .825, 1 (mid-knots first would give .25/.5/.75 too); the
+    geometry is unchanged (1e-12) and the benchs/dload beam-dist deflexion stays within its 2 %."""
+    coarse = IGAparametrization(filename=os.path.join(MODELS, "beam-dist"))
+    m = IGAparametrization(filename=os.path.join(MODELS, "beam-dist"))
+    m.refine(np.array([2, 2, 5]), np.array([1, 1, 1]),
+             additional_knots={"patches": np.array([0]), "1": np.array([0.3]), "2": np.array([]),
+                               "3": np.array([0.55, 0.8])})
+    brk = np.unique(m._Ukv[0][0])
+    assert brk.size == 9 and np.abs(brk - [0.0, 0.075, 0.15, 0.225, 0.3, 0.475, 0.65, 0.825, 1.0]).max() <= 1e-15
+    assert {0.55, 0.8} <= set(np.unique(m._Ukv[0][2]).tolist())
+    xi = np.random.default_rng(5).uniform(0.0, 1.0, (200, 3))
+    xc = geometry_at(coarse.get_model_dict(), xi)
+    xf = geometry_at(m.get_model_dict(), xi)
+    assert np.abs(xf - xc).max() <= 1e-12 * np.abs(xc).max()
+    K, r, c, F = O.oracle_sys_linmat(m.get_inputs4system_elemStorage())
+    assert np.allclose(tip_deflexion(m, full_sol(m, solve(m, K, r, c, F))), DLOAD["beam-dist"], rtol=2e-2)
+
+
 def test_refine_rejects_nonconforming_faces():
     m = IGAparametrization(filename=os.path.join(CENTRIF2, "centrif_U1_U1_1load"))
     with pytest.raises(NotImplementedError):

@@ -395,12 +395,15 @@ class IGAparametrization:
     mcrd = property(lambda self: self._mcrd)
 
     # -- refinement ----------------------------------------------------------------------------
-    def refine(self, nb_refinementByDirection, nb_degreeElevationByDirection=np.array([0, 0, 0])):
+    def refine(self, nb_refinementByDirection, nb_degreeElevationByDirection=np.array([0, 0, 0]),
+               additional_knots=None):
         """k-refinement of 3D solid patches (IGA_parametrization.py:2294-2437, iga_refinement in
         IGA_refinementFcts.py:880-1015): per patch, degree elevation by
         nb_degreeElevationByDirection, then nb_refinementByDirection levels of mid-knot insertion;
         arrays of shape (3,) (every patch alike) or (3, k) (patch by patch; patches beyond k are
-        not refined).
+        not refined). additional_knots = {"patches": 0-based patch indices, "1": xi knots, "2":
+        eta knots, "3": zeta knots} are inserted into those patches after the degree elevation
+        and before the mid-knot levels (patchRefinement, IGA_refinementFcts.py:185-269).
 
         Patches that share a whole face of control points stay joined: the refined CPs of that
         face keep the numbers the earlier patch gave them and the other CPs are numbered patch by
@@ -439,7 +442,17 @@ class IGAparametrization:
             extra = [np.moveaxis(f[grid - 1], -1, 0) * w[None] for _, f, _ in self._dists]
             H = np.concatenate([X * w[None], w[None]] + extra, axis=0)
             pf = p0 + ndeg[:, ip]
-            Uf = [_midpoint_knots(_elevated_knots(Uc[d], p0[d], ndeg[d, ip]), nref[d, ip]) for d in range(3)]
+            Uf = []
+            for d in range(3):
+                Ue = _elevated_knots(Uc[d], p0[d], ndeg[d, ip])
+                extra_k = np.asarray(additional_knots.get(str(d + 1), []), np.float64).reshape(-1) \
+                    if additional_knots is not None and ip in np.asarray(additional_knots.get("patches", [])) \
+                    else np.zeros(0)
+                if extra_k.size:
+                    if np.any(extra_k <= Ue[0]) or np.any(extra_k >= Ue[-1]):
+                        raise ValueError("refine: additional knots must lie inside the knot vector")
+                    Ue = np.sort(np.concatenate([Ue, extra_k]))
+                Uf.append(_midpoint_knots(Ue, nref[d, ip]))
             T = [refinement_matrix(Uc[d], p0[d], Uf[d], pf[d]) for d in range(3)]
             Hf = np.einsum("ai,bj,ck,xijk->xabc", T[0], T[1], T[2], H, optimize=True)
             spans = [np.nonzero(Uf[d][pf[d]:-pf[d] - 1] < Uf[d][pf[d] + 1:Uf[d].size - pf[d]])[0] + pf[d] + 1
