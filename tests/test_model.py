@@ -318,6 +318,41 @@ def test_refine_additional_knots():
     assert np.allclose(tip_deflexion(m, full_sol(m, solve(m, K, r, c, F))), DLOAD["beam-dist"], rtol=2e-2)
 
 
+def _boehm(U, p, u):
+    """Single knot insertion (Boehm; Piegl & Tiller, The NURBS Book, eq. 5.15): Q = A P."""
+    n = U.size - p - 1
+    k = int(np.searchsorted(U, u, side="right")) - 1
+    A = np.zeros((n + 1, n))
+    for i in range(n + 1):
+        if i <= k - p:
+            A[i, i] = 1.0
+        elif i <= k:
+            a = (u - U[i]) / (U[i + p] - U[i])
+            A[i, i] = a
+            A[i, i - 1] = 1.0 - a
+        else:
+            A[i, i - 1] = 1.0
+    return A
+
+
+def test_refine_transformation_matrices():
+    """refine_and_getTransformationMatrices: for one extra knot on a p = 2 patch the xi matrix is
+    Boehm's insertion matrix and the others are identities; for a full k-refinement each matrix
+    maps the coarse homogeneous net onto the refined one (rows sum to 1)."""
+    m = IGAparametrization(filename=os.path.join(MODELS, "beam-dist"))
+    T = m.refine_and_getTransformationMatrices(np.array([1, 1, 1]), np.array([1, 1, 1]))
+    assert len(T) == 1 and len(T[0]) == 3
+    for t in T[0]:
+        assert np.abs(np.asarray(t.sum(axis=1)).ravel() - 1.0).max() <= 1e-14
+    U0 = m._Ukv[0][0].copy()
+    T = m.refine_and_getTransformationMatrices(np.array([0, 0, 0]), np.array([0, 0, 0]),
+                                               additional_knots={"patches": np.array([0]), "1": np.array([0.3]),
+                                                                 "2": np.array([]), "3": np.array([])})
+    assert np.abs(T[0][0].toarray() - _boehm(U0, 2, 0.3)).max() <= 1e-14
+    for t in T[0][1:]:
+        assert np.abs(t.toarray() - np.eye(t.shape[0])).max() == 0.0
+
+
 def test_refine_rejects_nonconforming_faces():
     m = IGAparametrization(filename=os.path.join(CENTRIF2, "centrif_U1_U1_1load"))
     with pytest.raises(NotImplementedError):

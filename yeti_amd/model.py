@@ -24,6 +24,7 @@ from __future__ import annotations
 import re
 
 import numpy as np
+from scipy import sparse
 
 from ._abi import F2PY_ARGS, F2PY_OPTIONAL
 
@@ -458,7 +459,7 @@ class IGAparametrization:
             spans = [np.nonzero(Uf[d][pf[d]:-pf[d] - 1] < Uf[d][pf[d] + 1:Uf[d].size - pf[d]])[0] + pf[d] + 1
                      for d in range(3)]                            # 1-based span indices
             nijk = [(s0, s1, s2) for s2 in spans[2] for s1 in spans[1] for s0 in spans[0]]
-            pts.append(dict(p0=p0, Uc=Uc, nc=nc, grid=grid, pf=pf, Uf=Uf, Hf=Hf, nf=Hf.shape[1:], nijk=nijk))
+            pts.append(dict(p0=p0, Uc=Uc, nc=nc, grid=grid, pf=pf, Uf=Uf, Hf=Hf, nf=Hf.shape[1:], nijk=nijk, T=T))
 
         # numbering of the refined CPs
         count = 0
@@ -624,6 +625,15 @@ class IGAparametrization:
         self._bcs = [(map_nodes(ids), dof, v) for ids, dof, v in self._bcs]
         self._cloads = [(map_vertices(ids), dof, v) for ids, dof, v in self._cloads]
         self._update_dofs()
+        return [[sparse.csc_matrix(t) for t in q["T"]] for q in pts]
+
+    def refine_and_getTransformationMatrices(self, nb_refinementByDirection,
+                                             nb_degreeElevationByDirection=np.array([0, 0, 0]),
+                                             additional_knots=None):
+        """refine, returning per patch the three 1D matrices (scipy CSC) that map the coarse
+        homogeneous control points (x w, w) of each direction to the refined ones
+        (IGA_parametrization.py:2294-2423)."""
+        return self.refine(nb_refinementByDirection, nb_degreeElevationByDirection, additional_knots)
 
     @staticmethod
     def _local_nodes(p):
