@@ -470,6 +470,32 @@ def test_gpu_bench_centrif_two_patches():
 
 
 @pytest.mark.gpu
+def test_gpu_volume_dropin():
+    """volume.computevolume / computegradvolume on get_inputs4area: the two stacked boxes of the
+    centrifugal bench hold 2 x 3 x 3 = 18 and 2 x 3 x 2 = 12; listpatch selects them; the
+    gradient of both patches equals Assembler.volume_grad, that of the unit cube is +-1/4."""
+    from yeti_amd import volume
+    from yeti_amd._abi import args_to_dict
+    from yeti_amd.assembly import Assembler
+    print("[case] volume drop-in", flush=True)
+    m = centrif_two_patches("centrif_U1_U1_1load")
+    for flags, ref in (((1, 0), 18.0), ((0, 1), 12.0), ((1, 1), 30.0)):
+        V = volume.computevolume(*m.get_inputs4area(activepatch=np.array(flags)))
+        assert abs(V - ref) <= 1e-12 * ref, (flags, V)
+    g = volume.computegradvolume(*m.get_inputs4area())
+    assert g.shape == (3, m.nb_cp) and g.flags.f_contiguous
+    asm = Assembler(args_to_dict(m.get_inputs4system_elemStorage()))
+    try:
+        g0 = asm.volume_grad()[1].cpu().numpy().reshape(-1, 3).T
+    finally:
+        asm.close()
+    assert np.abs(g - g0).max() <= 1e-13 * np.abs(g0).max()
+    cube = IGAparametrization(filename=os.path.join(MODELS, "grad_solid", "unit_cube_U1"))
+    gc = volume.computegradvolume(*cube.get_inputs4area())
+    assert np.abs(np.abs(gc) - 0.25).max() <= 1e-14
+
+
+@pytest.mark.gpu
 def test_gpu_rejects_distribution_out_of_range():
     """A nodal-distribution load whose label has no row in nodal_dist is refused on the host
     (YETI_ERR_INVALID_ARGUMENT) instead of being read past the array on the device."""

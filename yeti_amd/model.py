@@ -742,6 +742,18 @@ class IGAparametrization:
                 self._MATERIAL_PROPERTIES[2], d["TENSOR"], self._ind_dof_free, self._nb_dof_free, self._mcrd,
                 self._NBPINT, self._nnode, self._nb_patch, self.nb_elem, self._nb_cp, self._nb_dof_tot]
 
+    def get_inputs4area(self, COORDS=None, activepatch=None):
+        """Positional argument list of volume.computevolume / computegradvolume
+        (IGA_parametrization.py:1286-1312): every patch active unless activepatch (one flag per
+        patch) is given."""
+        d = self._flat()
+        X = self._COORDS if COORDS is None or np.shape(COORDS) != np.shape(self._COORDS) else COORDS
+        if activepatch is None or np.size(activepatch) != self._nb_patch:
+            activepatch = np.ones(self._nb_patch, np.int64)
+        return [np.asarray(activepatch), X, d["IEN"], self._elementsByPatch, self._Nkv, d["Ukv"], self._Nijk,
+                d["weight"], self._Jpqr, d["ELT_TYPE"], d["PROPS"], d["JPROPS"], d["TENSOR"], self._mcrd,
+                self._NBPINT, self._nnode, self._nb_patch, self.nb_elem, self._nb_cp]
+
     def get_inputs4evaldisp(self, sol, xi, numpatch=1):
         """Positional argument list of postproc.evaldisp (IGA_parametrization.py:908-938)."""
         d = self._flat()
