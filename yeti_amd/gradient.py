@@ -19,7 +19,8 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from ._abi import args_to_dict
+from ._abi import (YETI_ERR_INVALID_ARGUMENT, YETI_ERR_UNSUPPORTED_DEGREE, YETI_ERR_UNSUPPORTED_ELEMENT,
+                   args_to_dict)
 from ._lib import YetiError
 from .assembly import Assembler
 from .matfree import F64, MatrixFree
@@ -46,7 +47,8 @@ class CompliancePatch:
         try:
             self.mf = MatrixFree(d, precisions=F64)
         except YetiError as e:
-            if solver == "mf" or e.code not in (1, 2, 3):   # device and budget errors propagate
+            model_errors = (YETI_ERR_INVALID_ARGUMENT, YETI_ERR_UNSUPPORTED_ELEMENT, YETI_ERR_UNSUPPORTED_DEGREE)
+            if solver == "mf" or e.code not in model_errors:   # device and budget errors propagate
                 raise
             self.mf = None
         self.csr = StiffnessCSR(self.asm) if solver == "csr" else None
