@@ -248,3 +248,39 @@ def test_gpu_generatevtu_dropin(tmp_path):
     assert len(pcs) == 1
     so = O.oracle_svars_solid(d, 0, sol, tuple(nb_ref), tuple(flags))
     check_piece(pcs[0], so, tuple(flags), tuple(nb_ref), tol_pts=1e-13, tol_f=1e-11)
+
+
+@pytest.mark.gpu
+def test_gpu_generatevtu_two_patches(tmp_path):
+    """generatevtu on the refined two-patch centrifugal bench (get_inputs4postprocVTU): one piece
+    per patch, each equal to the oracle's vertex fields of that patch; the vertices of the shared
+    face z = 3 carry the same displacement in both pieces."""
+    from yeti_amd.model import IGAparametrization
+    from yeti_amd.postproc import generatevtu
+    print("[case] generatevtu two patches", flush=True)
+    m = IGAparametrization(filename=os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "models",
+                                                 "centrif_two_patches", "centrif_U1_U1_1load"))
+    m.refine(np.full((3, 2), 1, np.intp), np.full((3, 2), 1, np.intp))
+    d = m.get_model_dict()
+    sol = random_sol(d, seed=6)
+    nb_ref, flags = np.array([2, 2, 2]), np.array([True, True, True])
+    path = generatevtu(*m.get_inputs4postprocVTU("two", sol.transpose(), nb_ref=nb_ref, Flag=flags,
+                                                 output_path=str(tmp_path)))
+    pcs = O.parse_vtu(path)
+    assert len(pcs) == 2
+    for ip in range(2):
+        so = O.oracle_svars_solid(d, ip, sol, tuple(nb_ref), tuple(flags))
+        check_piece(pcs[ip], so, tuple(flags), tuple(nb_ref), tol_pts=1e-13, tol_f=1e-11)
+    face = []
+    for pc in pcs:
+        x = pc["points"].reshape(-1, 3)
+        u = pc["disp"].reshape(-1, 3)
+        on = np.abs(x[:, 2] - 3.0) <= 1e-12
+        key = np.round(x[on], 9)
+        order = np.lexsort(key.T)
+        face.append((key[order], u[on][order]))
+    (xa, ua), (xb, ub) = face
+    uniq_a, ia = np.unique(xa, axis=0, return_index=True)
+    uniq_b, ib = np.unique(xb, axis=0, return_index=True)
+    assert uniq_a.shape[0] > 0 and np.array_equal(uniq_a, uniq_b)
+    assert np.abs(ua[ia] - ub[ib]).max() <= 1e-12 * np.abs(sol).max()
