@@ -516,6 +516,23 @@ def main():
     vib_ms = (time.perf_counter() - t0) / 3 * 1e3
     log("vibration gradient (%d modes): %.2f ms, %.3e elem/s" % (nmode, vib_ms, bp.nb_elem / (vib_ms * 1e-3)))
     del vect, gv
+    # element-wise strain-energy gradient -d(u^T K u)/dX (the compliance gradient's stiffness term
+    # for models the matrix-free operator does not cover) and the volume gradient, same model
+    u = torch.randn(3 * bp.nb_cp, dtype=torch.float64, device="cuda")
+    asm.strain_energy_grad(u)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        gs = asm.strain_energy_grad(u)
+    torch.cuda.synchronize()
+    seg_ms = (time.perf_counter() - t0) / 3 * 1e3
+    t0 = time.perf_counter()
+    for _ in range(3):
+        vol, gvol = asm.volume_grad()
+    torch.cuda.synchronize()
+    vol_ms = (time.perf_counter() - t0) / 3 * 1e3
+    log("strain-energy gradient: %.2f ms; volume + gradient: %.2f ms" % (seg_ms, vol_ms))
+    del u, gs, vol, gvol
     torch.cuda.empty_cache()
 
     cpu = None
@@ -563,6 +580,8 @@ def main():
                                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                      "frac": mass_bytes / (mass_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}},
         "vibration_gradient": {"ms": vib_ms, "modes": nmode, "elements_per_s": bp.nb_elem / (vib_ms * 1e-3)},
+        "strain_energy_gradient": {"ms": seg_ms, "elements_per_s": bp.nb_elem / (seg_ms * 1e-3)},
+        "volume_gradient": {"ms": vol_ms, "elements_per_s": bp.nb_elem / (vol_ms * 1e-3)},
     }
     if not args.only_assembly:
         out["c2_solve"] = bench_c2_solve(args.nel, p, args.jacobi_iters)
