@@ -42,8 +42,9 @@ class CompliancePatch:
         self.solver = solver
         self.rtol, self.maxit = float(rtol), int(maxit)
         self.asm = Assembler(d)
-        # the matrix-free operator covers single-patch equal-degree bricks; other models take the
-        # element-wise strain-energy gradient of the assembler and the CSR solve
+        # the matrix-free operator covers full element grids of equal degrees with p + 1 points;
+        # other models take the element-wise strain-energy gradient of the assembler and the CSR
+        # solve
         try:
             self.mf = MatrixFree(d, precisions=F64)
         except YetiError as e:
