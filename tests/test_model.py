@@ -451,13 +451,14 @@ def test_gpu_unused_distribution():
 
 
 @pytest.mark.gpu
-def test_gpu_bench_centrif_two_patches():
-    """benchs/centrif/centrif_U1_U1_1load/test_centrif_u1_u1_1load.py on the drop-in stack, with
-    the oracle's assembly of the same refined model."""
+@pytest.mark.parametrize("name", ["centrif_U1_U1_1load", "centrif_U1_U1_2loads"])
+def test_gpu_bench_centrif_two_patches(name):
+    """benchs/centrif/centrif_U1_U1_{1load,2loads}/test_*.py on the drop-in stack, with the
+    oracle's assembly of the same refined model."""
     from yeti_amd import reconstructionSOL as rsol
     from yeti_amd.stiffmtrx_elemstorage import sys_linmat_lindef_static as build_stiffmatrix
-    print("[case] bench centrif, two patches", flush=True)
-    m = centrif_two_patches("centrif_U1_U1_1load")
+    print("[case] bench centrif, two patches:", name, flush=True)
+    m = centrif_two_patches(name)
     data, row, col, rhs = build_stiffmatrix(*m.get_inputs4system_elemStorage())
     K, r, c, F = O.oracle_sys_linmat(m.get_inputs4system_elemStorage())
     assert np.array_equal(row, r) and np.array_equal(col, c)
