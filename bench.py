@@ -410,8 +410,9 @@ def main():
                     help="run only a 2 GiB device copy and one element-kernel run (for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the assembly shards by element "
-              "ranges with no communication; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: one GPU only (the assembly would shard by contiguous element ranges, "
+              "each GPU writing its own slice of the COO with no communication). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
