@@ -93,3 +93,70 @@ def test_generator_conventions():
     assert np.isclose(X[0].max(), 10.0) and np.isclose(X[1].max(), 1.0)
     # traction on the 4 elements at x = 10, code 23
     assert a["JDLType"][0] == 23 and a["indDLoad"].size == n2 * n3 // (n2 * n3) * 4
+
+
+def _counting_create(fn, d, extra):
+    """Run a *_create_in entry point in counting mode (null arena: no device is touched)."""
+    L = _lib.lib()
+    s, keep = _abi.pack_system_inputs(d)
+    out = ctypes.c_void_p()
+    used = ctypes.c_size_t(0)
+    rc = getattr(L, fn)(ctypes.byref(s), extra, None, 0, None, ctypes.byref(out), ctypes.byref(used))
+    msg = L.yeti_last_error().decode()
+    return rc, msg, used.value, out.value
+
+
+@pytest.mark.parametrize("fn,extra", [("yeti_asm_create_in", 0), ("yeti_mf_create_in", 1)])
+def test_set_up_rejects_out_of_range_ids_before_any_device_call(fn, extra):
+    bp = make_box_patch(nel=(3, 2, 2), p=2)
+    good = dict(bp.args)
+    rc, msg, used, out = _counting_create(fn, good, extra)
+    assert rc == 0 and used > 0 and out is None, msg
+
+    def bad(key, idx, value):
+        d = dict(good)
+        arr = np.array(d[key], copy=True, order="F")
+        arr.reshape(-1, order="F")[idx] = value
+        d[key] = arr
+        return d
+
+    n1 = bp.n_cp_dir[0]
+    cases = [
+        (bad("IEN", 5, bp.nb_cp + 1), "IEN"),
+        (bad("IEN", 0, 0), "IEN"),
+        (bad("Nijk", 3, 2), "span"),                        # below p + 1 in xi
+        (bad("Nijk", 4, bp.nel[1] + 3), "span"),            # beyond nk - p - 1 in eta
+        (bad("indDLoad", 0, bp.nb_elem + 1), "load"),       # face load on a missing element
+        (bad("bc_target", 0, bp.nb_cp + 7), "bc_target"),
+        (bad("Ukv", n1, -1.0), "knot"),                     # decreasing knot vector
+    ]
+    for d, word in cases:
+        rc, msg, _, out = _counting_create(fn, d, extra)
+        assert rc == _abi.YETI_ERR_INVALID_ARGUMENT, (word, rc, msg)
+        assert word in msg and out is None, msg
+    d = dict(good)
+    d["nb_elem_patch"] = np.array([bp.nb_elem + 1], np.int32)
+    rc, msg, _, _ = _counting_create(fn, d, extra)
+    assert rc == _abi.YETI_ERR_INVALID_ARGUMENT and "nb_elem" in msg
+
+
+def test_host_drop_in_rejects_bad_connectivity_without_a_device():
+    """The host-buffer drop-in validates before it uploads anything."""
+    L = _lib.lib()
+    bp = make_box_patch(nel=(2, 1, 1), p=2)
+    d = dict(bp.args)
+    ien = np.array(d["IEN"], copy=True)
+    ien[7] = bp.nb_cp + 100
+    d["IEN"] = ien
+    s, keep = _abi.pack_system_inputs(d)
+    nd = int(bp.nb_data)
+    K = np.zeros(nd)
+    r = np.zeros(nd, np.int32)
+    c = np.zeros(nd, np.int32)
+    F = np.zeros(bp.nb_dof_tot)
+    p64 = ctypes.POINTER(ctypes.c_double)
+    p32 = ctypes.POINTER(ctypes.c_int32)
+    rc = L.yeti_sys_linmat_lindef_static(ctypes.byref(s), 0, K.ctypes.data_as(p64), r.ctypes.data_as(p32),
+                                         c.ctypes.data_as(p32), F.ctypes.data_as(p64))
+    assert rc == _abi.YETI_ERR_INVALID_ARGUMENT
+    assert "IEN" in L.yeti_last_error().decode()

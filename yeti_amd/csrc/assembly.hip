@@ -1951,9 +1951,8 @@ static int create(const yeti_system_inputs *in, int layout, yeti_asm **out) {
     if (!in) return set_error(YETI_ERR_INVALID_ARGUMENT, "null inputs");
     if (layout != YETI_LAYOUT_PACKED && layout != YETI_LAYOUT_SLOTTED)
         return set_error(YETI_ERR_INVALID_ARGUMENT, "unknown layout %d", layout);
-    if (in->MCRD != 3) return set_error(YETI_ERR_INVALID_ARGUMENT, "MCRD=%d, only 3D solids are supported", in->MCRD);
-    if (in->nb_patch < 1 || in->nb_cp < 1 || in->nb_elem < 0)
-        return set_error(YETI_ERR_INVALID_ARGUMENT, "empty model");
+    // every id the set-up and the kernels gather through is checked here, before any device call
+    if (int rc0 = validate_system_inputs(in, false)) return rc0;
     if ((int64_t)in->nb_cp * 3 > INT32_MAX)
         return set_error(YETI_ERR_SIZE_OVERFLOW, "dof ids exceed int32");
     std::vector<std::string> types;

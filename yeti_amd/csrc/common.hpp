@@ -17,6 +17,11 @@ std::string &last_error();
 
 int set_error(int code, const char *fmt, ...);
 
+// Host-side range checks of a model (ids, spans, array lengths, load and condition targets);
+// 0 or YETI_ERR_INVALID_ARGUMENT / YETI_ERR_UNSUPPORTED_ELEMENT with the message set. With
+// require_solid, every patch must be a solid (U1 / U0) patch.
+int validate_system_inputs(const yeti_system_inputs *in, bool require_solid);
+
 #define YETI_HIP(call)                                                                    \
     do {                                                                                  \
         hipError_t e_ = (call);                                                           \

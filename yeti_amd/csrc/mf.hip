@@ -1137,8 +1137,9 @@ extern "C" {
 static int32_t mf_create(const yeti_system_inputs *in, int32_t precisions, hipStream_t st, yeti_mf **out) {
     if (!in || !out) return set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
-    if (in->MCRD != 3) return set_error(YETI_ERR_INVALID_ARGUMENT, "MCRD must be 3");
     if (!(precisions & 3)) return set_error(YETI_ERR_INVALID_ARGUMENT, "no precision requested");
+    // ids, spans and lengths checked on the host before any device call
+    if (int rc0 = validate_system_inputs(in, true)) return rc0;
     auto *m = new yeti_mf();
     auto fail = [&](int code) {
         delete m;
