@@ -335,20 +335,51 @@ def _boehm(U, p, u):
     return A
 
 
+def _homogeneous_net(m, ip=0):
+    """Homogeneous control points (x w, y w, z w, w) of patch ip, xi fastest, as rows."""
+    grid = m._cp_grid(ip)
+    p = m._Jpqr[:, ip]
+    w = np.zeros(grid.shape)
+    e0 = int(np.sum(m._elementsByPatch[:ip]))
+    for e in range(e0, e0 + int(m._elementsByPatch[ip])):
+        s = m._Nijk[:, e]
+        for l, (i, j, k) in enumerate(m._local_nodes(p)):
+            w[s[0] - 1 - i, s[1] - 1 - j, s[2] - 1 - k] = m._weight[e][l]
+    ids = grid.ravel(order="F") - 1
+    wf = w.ravel(order="F")
+    return np.column_stack([m._COORDS[:, ids].T * wf[:, None], wf])
+
+
 def test_refine_transformation_matrices():
-    """refine_and_getTransformationMatrices: for one extra knot on a p = 2 patch the xi matrix is
-    Boehm's insertion matrix and the others are identities; for a full k-refinement each matrix
-    maps the coarse homogeneous net onto the refined one (rows sum to 1)."""
+    """refine_and_getTransformationMatrices returns per patch the whole-patch matrices
+    [S_xi, S_eta, S_zeta] of the reference (IGA_refinementFcts.py:340-355): S_zeta S_eta S_xi maps
+    the coarse homogeneous net onto the refined one (the product _updateRefinementMatHistory
+    forms, IGA_parametrization.py:2456-2461); for one extra knot on a p = 2 patch S_xi is
+    I (x) I (x) Boehm's insertion matrix and the other two are identities."""
     m = IGAparametrization(filename=os.path.join(MODELS, "beam-dist"))
+    Hc = _homogeneous_net(m)
+    nc = [m._Ukv[0][d].size - m._Jpqr[d, 0] - 1 for d in range(3)]
     T = m.refine_and_getTransformationMatrices(np.array([1, 1, 1]), np.array([1, 1, 1]))
     assert len(T) == 1 and len(T[0]) == 3
+    Hf = _homogeneous_net(m)
+    nf = [m._Ukv[0][d].size - m._Jpqr[d, 0] - 1 for d in range(3)]
+    S_xi, S_eta, S_zeta = T[0]
+    assert S_xi.shape == (nf[0] * nc[1] * nc[2], nc[0] * nc[1] * nc[2])
+    assert S_eta.shape == (nf[0] * nf[1] * nc[2], nf[0] * nc[1] * nc[2])
+    assert S_zeta.shape == (nf[0] * nf[1] * nf[2], nf[0] * nf[1] * nc[2])
     for t in T[0]:
         assert np.abs(np.asarray(t.sum(axis=1)).ravel() - 1.0).max() <= 1e-14
+    assert np.abs(S_zeta @ (S_eta @ (S_xi @ Hc)) - Hf).max() <= 1e-12 * np.abs(Hf).max()
+    # a single knot in xi of the refined (now p = 2) patch: Boehm's matrix on every (eta, zeta)
+    # line, identities elsewhere
+    assert m._Jpqr[0, 0] == 2
     U0 = m._Ukv[0][0].copy()
+    nc = [m._Ukv[0][d].size - m._Jpqr[d, 0] - 1 for d in range(3)]
     T = m.refine_and_getTransformationMatrices(np.array([0, 0, 0]), np.array([0, 0, 0]),
                                                additional_knots={"patches": np.array([0]), "1": np.array([0.3]),
                                                                  "2": np.array([]), "3": np.array([])})
-    assert np.abs(T[0][0].toarray() - _boehm(U0, 2, 0.3)).max() <= 1e-14
+    B = _boehm(U0, 2, 0.3)
+    assert np.abs(T[0][0].toarray() - np.kron(np.eye(nc[1] * nc[2]), B)).max() <= 1e-14
     for t in T[0][1:]:
         assert np.abs(t.toarray() - np.eye(t.shape[0])).max() == 0.0
 
@@ -357,6 +388,33 @@ def test_refine_rejects_nonconforming_faces():
     m = IGAparametrization(filename=os.path.join(CENTRIF2, "centrif_U1_U1_1load"))
     with pytest.raises(NotImplementedError):
         m.refine(np.array([[2, 1], [2, 1], [2, 1]]), np.array([[1, 1], [1, 1], [1, 1]]))
+
+
+def test_refine_rejects_knots_that_meet_reversed():
+    """The same additional knot (xi = 0.3) on both patches, with the second patch's xi reversed:
+    the two sides of the shared face get knots 0.3 and 0.7, the same face grid size but a
+    different refinement, which refine must reject rather than merge the CPs by index. With xi
+    unchanged, or knots along directions not reversed, the patches stay joined."""
+    def model(flip):
+        m = IGAparametrization(filename=os.path.join(CENTRIF2, "centrif_U1_U1_1load"))
+        if flip:
+            loc = m._local_nodes(np.array([1, 1, 1]))
+            old = m._IEN[1][0].copy()
+            for l, (i, j, k) in enumerate(loc):
+                m._IEN[1][0][l] = old[loc.index((1 - i, j, k))]
+        return m
+
+    def knots(d):
+        ak = {"patches": np.array([0, 1]), "1": np.array([]), "2": np.array([]), "3": np.array([])}
+        ak[d] = np.array([0.3])
+        return ak
+
+    zero = np.zeros((3, 2), np.intp)
+    for d in "123":
+        model(False).refine(zero, zero, additional_knots=knots(d))
+    model(True).refine(zero, zero, additional_knots=knots("2"))
+    with pytest.raises(NotImplementedError):
+        model(True).refine(zero, zero, additional_knots=knots("1"))
 
 
 # --------------------------------------------------------------------------------------- GPU

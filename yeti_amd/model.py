@@ -282,6 +282,33 @@ def _face_orientation(A, B):
     return None
 
 
+def _patch_refinement_matrices(T):
+    """[S_xi, S_eta, S_zeta] of a patch from its 1D refinement matrices T[d] (n_fine x n_coarse):
+    S_xi = I_c2 (x) I_c1 (x) T0, S_eta = I_c2 (x) T1 (x) I_f0, S_zeta = T2 (x) I_f1 (x) I_f0 on
+    control-point vectors ordered xi fastest."""
+    (f0, c0), (f1, c1), (f2, c2) = (t.shape for t in T)
+    Ts = [sparse.csc_matrix(t) for t in T]
+    eye = lambda n: sparse.identity(n, format="csc")
+    S_xi = sparse.kron(eye(c2), sparse.kron(eye(c1), Ts[0]), format="csc")
+    S_eta = sparse.kron(eye(c2), sparse.kron(Ts[1], eye(f0)), format="csc")
+    S_zeta = sparse.kron(Ts[2], sparse.kron(eye(f1), eye(f0)), format="csc")
+    return [S_xi, S_eta, S_zeta]
+
+
+def _same_face_net(q, fa, o, fb, t):
+    """Whether two refined patches give the same control points on a shared face: q's face fa
+    and o's face fb seen through the orientation t. Refining the two sides with different knots
+    (say 0.3 on one side and 0.7 on the other) moves the interior face points apart."""
+    A = q["Hf"][(slice(None),) + fa]
+    B = o["Hf"][(slice(None),) + fb]
+    B = np.stack([t(B[c]) for c in range(B.shape[0])])
+    if A.shape != B.shape:
+        return False
+    xa, xb = A[:3] / A[3], B[:3] / B[3]
+    scale = max(1.0, float(np.abs(xa).max()))
+    return bool(np.abs(xa - xb).max() <= 1e-10 * scale)
+
+
 def refinement_matrix(Uc, pc, Uf, pf):
     """T (n_fine x n_coarse) with N_coarse_j = sum_i T_ij N_fine_i: the coarse space lies in the
     fine one, so interpolation at the fine Greville abscissae is exact."""
@@ -481,7 +508,7 @@ class IGAparametrization:
                         fa_f = _face_slices(q["nf"])[_face_slices(q["grid"].shape).index(fa)]
                         fb_f = _face_slices(o["nf"])[_face_slices(o["grid"].shape).index(fb)]
                         Bf = t(o["gridf"][fb_f])
-                        if Bf.shape != gridf[fa_f].shape:
+                        if Bf.shape != gridf[fa_f].shape or not _same_face_net(q, fa_f, o, fb_f, t):
                             raise NotImplementedError("refine: patches %d and %d are refined differently on "
                                                       "their shared face" % (jp + 1, ip + 1))
                         gridf[fa_f] = Bf
@@ -625,14 +652,17 @@ class IGAparametrization:
         self._bcs = [(map_nodes(ids), dof, v) for ids, dof, v in self._bcs]
         self._cloads = [(map_vertices(ids), dof, v) for ids, dof, v in self._cloads]
         self._update_dofs()
-        return [[sparse.csc_matrix(t) for t in q["T"]] for q in pts]
+        return [_patch_refinement_matrices(q["T"]) for q in pts]
 
     def refine_and_getTransformationMatrices(self, nb_refinementByDirection,
                                              nb_degreeElevationByDirection=np.array([0, 0, 0]),
                                              additional_knots=None):
-        """refine, returning per patch the three 1D matrices (scipy CSC) that map the coarse
-        homogeneous control points (x w, w) of each direction to the refined ones
-        (IGA_parametrization.py:2294-2423)."""
+        """refine, returning per patch [S_xi, S_eta, S_zeta] (scipy CSC): whole-patch matrices
+        that refine one direction each, so that S_zeta S_eta S_xi maps the coarse homogeneous
+        control points (x w, w) of the patch, xi fastest, to the refined ones (the product the
+        reference forms in parameterizationRefinement, IGA_refinementFcts.py:340-355, and applies
+        in _updateRefinementMatHistory, IGA_parametrization.py:2456-2461; returned by
+        IGA_parametrization.py:2294-2423)."""
         return self.refine(nb_refinementByDirection, nb_degreeElevationByDirection, additional_knots)
 
     @staticmethod
