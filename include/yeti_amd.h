@@ -19,7 +19,7 @@
  *   yeti_pcg_*                      <- scipy spsolve / the matrix-free CG iga_mf_cg_3d,
  *                                      src/yeti_iga/iga_wq_mf/fsrc/mf_iga_solver.f90:642-958
  *   yeti_mf_*                       <- mf_iga_get_ku_3d, src/yeti_iga/iga_wq_mf/fsrc/mf_iga_solver.f90:292-555
- *   yeti_grad_compliance            <- optim.gradadjointwork.gradlinelastwork_an,
+ *   yeti_grad_lin_elast_work        <- optim.gradadjointwork.gradlinelastwork_an,
  *                                      src/yeti_iga/optim/gradtotalwork_fullAN.f90:17-300
  */
 #ifndef YETI_AMD_H
@@ -164,6 +164,20 @@ int32_t yeti_asm_strain_energy_grad(yeti_asm *h, const double *u, double *g, voi
  * OPT_modelling.py:508-530, all patches active). vol is a device pointer to one double, g a
  * device vector of 3 nb_cp values. */
 int32_t yeti_asm_volume_grad(yeti_asm *h, double *vol, double *g, void *stream);
+/* Work gradients of gradLinElastWork_AN (f2py optim.gradadjointwork.gradlinelastwork_an,
+ * src/yeti_iga/optim/gradtotalwork_fullAN.f90:17-300 with gradUELMAT1adj,
+ * optim/gradUELMAT_solid.f90:18-479; caller OPT_modelling.py:447-481) over the active solid
+ * elements of the resident model, for a state SOL (3, nb_cp) and nadj adjoint fields ADJ
+ * (nadj, 3, nb_cp), both column-major device arrays:
+ *   gradWint (nadj, 3, nb_cp) = -ADJ_k^T (dK/dX) SOL   (computeWint != 0, else zeros)
+ *   gradWext (nadj, 3, nb_cp) =  ADJ_k^T (dF/dX)       (computeWext != 0, else zeros)
+ * with the external term restricted to what the reference computes: the centrifugal body loads
+ * (type 101), whose axis points the k-th such load reads at values 6k+1..6k+6 of
+ * load_additionalInfos as the reference does; its surface-load term is a stub (zero) there and
+ * here. activeDir plays no part (the reference's solid routine does not read it). Synchronises
+ * `stream` once (the polarisation scales). */
+int32_t yeti_grad_lin_elast_work(yeti_asm *h, const double *SOL, const double *ADJ, int32_t nadj, int32_t computeWint,
+                                 int32_t computeWext, double *gradWint, double *gradWext, void *stream);
 /* Right-hand side only (concentrated, surface and body loads). */
 int32_t yeti_asm_run_rhs(yeti_asm *h, double *F_dev, void *stream);
 /* Replace the control-point coordinates (device, 3 nb_cp values, COORDS3D order) used by later

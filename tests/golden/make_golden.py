@@ -190,6 +190,58 @@ def two_patch_grad_fixture():
                         gradWext=ge, geometry_sha=geo)
 
 
+def load_combination_grad_fixture():
+    """gradLinElastWork_AN on two refined beams of benchs/load_combination carrying a nodal
+    pressure field on face 2 (U24, one additional value) and a centrifugal load (U101, six
+    values), in both orders, with half of the elements active: the reference reads the
+    centrifugal axis points from the start of load_additionalInfos whatever precedes them."""
+    from yeti_amd.model import IGAparametrization
+    out = {}
+    rng = np.random.default_rng(31)
+    for tag, name in (("pf", "beam-dloadF2-centrif"), ("fp", "beam-centrif-dloadF2")):
+        m = IGAparametrization(filename=os.path.join(HERE, "models", "load_combination", name))
+        m.refine(np.array([1, 1, 1]), np.array([1, 1, 1]))
+        args = list(m.get_inputs4system_elemStorage())
+        act = (rng.uniform(size=m.nb_elem) < 0.5).astype(np.int64)
+        act[0] = 1
+        args[0] = act                                     # activeElement
+        SOL = rng.uniform(-1e-3, 1e-3, size=(3, m.nb_cp))
+        ADJ = rng.uniform(-1e-3, 1e-3, size=(2, 3, m.nb_cp))
+        gi, ge = O.ref_grad_linelast_work(args, SOL, ADJ)
+        out.update({tag + "_active": act, tag + "_SOL": SOL, tag + "_ADJ": ADJ, tag + "_gradWint": gi,
+                    tag + "_gradWext": ge})
+    np.savez_compressed(os.path.join(HERE, "grad_load_combination_p2.npz"), **out)
+
+
+def c5_fields(X):
+    """The state and adjoint fields of the 32^3 p=2 gradient fixture: closed-form functions of the
+    control-point coordinates X (3, nb_cp), so that the fixture needs no stored inputs.
+    SOL (3, nb_cp); ADJ (2, 3, nb_cp) = (SOL / 2, a second field)."""
+    ph = np.arange(3)[:, None]
+    SOL = 1e-3 * np.sin(0.7 * X[0][None] + 1.3 * X[1][None] + 0.4 * X[2][None] + ph)
+    A1 = 1e-3 * np.cos(0.3 * X[0][None] - 2.1 * X[1][None] + 1.7 * X[2][None] + 0.5 * ph)
+    return SOL, np.stack([0.5 * SOL, A1])
+
+
+def c5_grad_fixture():
+    """gradLinElastWork_AN on the 32^3 p=2 cantilever (39,304 control points): gradWint for the
+    closed-form fields of c5_fields at 8,192 seeded control points (all three directions, both
+    adjoints) plus the Frobenius norm and the sum of each adjoint's full array. With ADJ_0 =
+    SOL / 2 the first adjoint is the compliance's (gradWint_0 = -1/2 u^T dK/dX u)."""
+    bp = make_box_patch(nel=(32, 32, 32), p=2)
+    X = np.asarray(bp.args["COORDS3D"], np.float64)
+    SOL, ADJ = c5_fields(X)
+    gi, ge = O.ref_grad_linelast_work(bp.f2py_args(), SOL, ADJ)
+    rng = np.random.default_rng(32)
+    cps = np.sort(rng.choice(bp.nb_cp, 8192, replace=False))
+    geo = sha(np.concatenate([bp.args["COORDS3D"].ravel(order="F"), bp.args["weight"]])) + sha(bp.args["IEN"])
+    np.savez_compressed(os.path.join(HERE, "grad_32cube_p2.npz"), sample_cp=cps, gradWint_sample=gi[:, :, cps],
+                        fro=np.sqrt((gi ** 2).sum(axis=(1, 2))), total=gi.sum(axis=(1, 2)),
+                        max_abs=np.abs(gi).max(axis=(1, 2)), gradWext_max=np.abs(ge).max(), geometry_sha=geo)
+
+
 if __name__ == "__main__":
     main()
     two_patch_grad_fixture()
+    load_combination_grad_fixture()
+    c5_grad_fixture()

@@ -160,3 +160,25 @@ def test_host_drop_in_rejects_bad_connectivity_without_a_device():
                                          c.ctypes.data_as(p32), F.ctypes.data_as(p64))
     assert rc == _abi.YETI_ERR_INVALID_ARGUMENT
     assert "IEN" in L.yeti_last_error().decode()
+
+
+def test_gradient_argument_list_follows_the_reference_order():
+    """inputs4gradtotalwork builds gradlinelastwork_an's positional list in the order of
+    IGA_parametrization._get_inputs4gradTotalwork (IGA_parametrization.py:1516-1525), which is the
+    drop-in's parameter order; the trailing sizes are the f2py optional arguments."""
+    import inspect
+    from yeti_amd.gradadjointwork import gradlinelastwork_an, inputs4gradtotalwork
+    bp = make_box_patch(nel=(2, 1, 1), p=2)
+    SOL = np.zeros((3, bp.nb_cp))
+    ADJ = np.zeros((2, 3, bp.nb_cp))
+    lst = inputs4gradtotalwork(dict(bp.args), SOL, ADJ, computeWint=False)
+    names = list(inspect.signature(gradlinelastwork_an).parameters)
+    assert names[:7] == ["computewint", "computewext", "sol", "adj", "activeelement", "activedir", "coords3d"]
+    assert names[-6:] == ["nadj", "mcrd", "nb_load", "nb_patch", "nb_elem", "nb_cp"]
+    assert len(lst) == len(names) == 33
+    assert lst[0] is False and lst[2] is SOL and lst[3] is ADJ
+    assert np.array_equal(lst[4], np.ones(bp.nb_elem)) and np.array_equal(lst[5], np.ones(3))
+    assert lst[-6:] == [2, 3, 1, 1, bp.nb_elem, bp.nb_cp]
+    # a wrong-sized activeElem or COORDS falls back to the model's, as the reference's marshaller
+    lst = inputs4gradtotalwork(dict(bp.args), SOL, ADJ, COORDS=np.zeros((3, 2)), activeElem=np.ones(3))
+    assert lst[6] is bp.args["COORDS3D"] and lst[4].size == bp.nb_elem

@@ -353,3 +353,151 @@ def test_coordinate_update_round_trip(gpu):
     J1.backward()
     assert J1.item() == J0.item()
     assert torch.equal(X1.grad, g0)
+
+
+def test_backward_after_a_later_forward(gpu):
+    """J1 = J(X1), then J2 = J(X2), then J1.backward(): the gradient is taken at X1 (the
+    coordinates J1 was computed at), bit-identical to a fresh forward + backward at X1, and the
+    patch is left at X2 (a following backward of J2 equals a fresh one at X2)."""
+    import torch
+    from yeti_amd.gradient import CompliancePatch, compliance
+    print("[case] backward after a later forward", flush=True)
+    d, bp = fixture()
+    patch = CompliancePatch(bp.args, solver="csr", rtol=1e-13)
+    X0 = patch.coords0.clone().cuda()
+    rng = np.random.default_rng(3)
+    X2v = X0 + torch.from_numpy(rng.uniform(-0.01, 0.01, X0.numel())).cuda()
+    Xa = X0.clone().requires_grad_(True)
+    Ja = compliance(Xa, patch)
+    Jb = compliance(X2v.clone().requires_grad_(True), patch)
+    Ja.backward()
+    assert patch.converged
+    ref = CompliancePatch(bp.args, solver="csr", rtol=1e-13)
+    Xr = X0.clone().requires_grad_(True)
+    Jr = compliance(Xr, ref)
+    Jr.backward()
+    assert Ja.item() == Jr.item()
+    assert torch.equal(Xa.grad, Xr.grad)
+    assert torch.equal(patch.coords, X2v)
+    Xc = X2v.clone().requires_grad_(True)
+    Jc = compliance(Xc, ref)
+    Jc.backward()
+    assert Jb.item() == Jc.item()
+
+
+def _dropin_cases():
+    return ["grad_4x2x2_p2", "unit_cube_p1", "grad_two_patches_p2", "load_combination_pf", "load_combination_fp"]
+
+
+@pytest.mark.parametrize("name", _dropin_cases())
+def test_gradlinelastwork_an_dropin_matches_reference(gpu, name):
+    """gradadjointwork.gradlinelastwork_an, called with the reference's argument list
+    (_get_inputs4gradTotalwork order), returns the compiled gradLinElastWork_AN's (gradWint,
+    gradWext) within 1e-10 of their maximum: traction only (external term 0 in the reference),
+    the centrifugal unit cube, the refined two-patch model, and two beams with a nodal pressure
+    field and a centrifugal load in both orders with half of the elements active (the reference
+    reads the centrifugal axis from the start of load_additionalInfos)."""
+    from yeti_amd._abi import args_to_dict
+    from yeti_amd.gradadjointwork import gradlinelastwork_an, inputs4gradtotalwork
+    print("[case] gradlinelastwork_an drop-in:", name, flush=True)
+    act = None
+    if name == "grad_4x2x2_p2":
+        f, bp = fixture()
+        d = dict(bp.args)
+        SOL, ADJ, gi_ref, ge_ref = f["SOL"], f["ADJ"], f["gradWint"], f["gradWext"]
+    elif name == "unit_cube_p1":
+        from tests.golden.make_golden import unit_cube_args
+        f = np.load("tests/golden/unit_cube_p1.npz")
+        d = unit_cube_args()
+        SOL, ADJ, gi_ref, ge_ref = f["SOL"], f["ADJ"], f["gradWint"], f["gradWext"]
+    else:
+        from yeti_amd.model import IGAparametrization
+        if name == "grad_two_patches_p2":
+            m = IGAparametrization(filename=MODELS + "/centrif_two_patches/centrif_U1_U1_1load")
+            m.refine(np.full((3, 2), 1, np.intp), np.full((3, 2), 1, np.intp))
+            f = np.load("tests/golden/grad_two_patches_p2.npz")
+            SOL, ADJ, gi_ref, ge_ref = f["SOL"], f["ADJ"], f["gradWint"], f["gradWext"]
+        else:
+            tag = name.split("_")[-1]
+            model = {"pf": "beam-dloadF2-centrif", "fp": "beam-centrif-dloadF2"}[tag]
+            m = IGAparametrization(filename=MODELS + "/load_combination/" + model)
+            m.refine(np.array([1, 1, 1]), np.array([1, 1, 1]))
+            f = np.load("tests/golden/grad_load_combination_p2.npz")
+            SOL, ADJ = f[tag + "_SOL"], f[tag + "_ADJ"]
+            gi_ref, ge_ref, act = f[tag + "_gradWint"], f[tag + "_gradWext"], f[tag + "_active"]
+        d = args_to_dict(m.get_inputs4system_elemStorage())
+    gi, ge = gradlinelastwork_an(*inputs4gradtotalwork(d, SOL, ADJ, activeElem=act))
+    assert gi.shape == gi_ref.shape and ge.shape == ge_ref.shape and gi.flags.f_contiguous
+    ei = np.abs(gi - gi_ref).max() / np.abs(gi_ref).max()
+    print("  internal: max %.3e, relative error %.2e" % (np.abs(gi_ref).max(), ei))
+    assert ei <= 1e-10
+    if np.abs(ge_ref).max() == 0.0:
+        assert np.abs(ge).max() == 0.0
+    else:
+        ee = np.abs(ge - ge_ref).max() / np.abs(ge_ref).max()
+        print("  external: max %.3e, relative error %.2e" % (np.abs(ge_ref).max(), ee))
+        assert ee <= 1e-10
+    # the flags switch either term off (zeros, as the reference's initialisation)
+    gi0, ge0 = gradlinelastwork_an(*inputs4gradtotalwork(d, SOL, ADJ, activeElem=act, computeWint=False))
+    assert np.abs(gi0).max() == 0.0 and np.array_equal(ge0, ge)
+    gi1, ge1 = gradlinelastwork_an(*inputs4gradtotalwork(d, SOL, ADJ, activeElem=act, computeWext=False))
+    assert np.abs(ge1).max() == 0.0 and np.array_equal(gi1, gi)
+
+
+def test_c5_32cube_gradient(gpu):
+    """The differentiation workload at its size: 32^3 elements, p = 2 (39,304 control points).
+    * gradlinelastwork_an (element-wise gradient kernel, polarised) and the matrix-free MODE 2
+      strain-energy gradient against the compiled gradLinElastWork_AN (tests/golden/
+      grad_32cube_p2.npz: 8,192 sampled control points x 3 directions x 2 adjoints, and the full
+      arrays' Frobenius norms): within 1e-10 of the maximum;
+    * translation invariance of the reference's own array (its sum is ~1e-15 of its norm);
+    * the compliance gradient dJ/dX (J = F . u, CSR PCG to 1e-12) against central differences of
+      J at 8 seeded coordinates: step 1e-4 within 1e-5 max|g|, step 1e-6 within 1e-4 max|g|."""
+    import torch
+    from tests.golden.make_golden import c5_fields
+    from yeti_amd.gradadjointwork import gradlinelastwork_an, inputs4gradtotalwork
+    from yeti_amd.gradient import CompliancePatch, compliance
+    from yeti_amd.matfree import F64, MatrixFree
+    print("[case] C5 32^3 p=2 gradient", flush=True)
+    f = np.load("tests/golden/grad_32cube_p2.npz")
+    bp = make_box_patch(nel=(32, 32, 32), p=2)
+    geo = sha(np.concatenate([bp.args["COORDS3D"].ravel(order="F"), bp.args["weight"]])) + sha(bp.args["IEN"])
+    assert geo == str(f["geometry_sha"]), "generator changed: regenerate the fixture"
+    X = np.asarray(bp.args["COORDS3D"], np.float64)
+    SOL, ADJ = c5_fields(X)
+    cps = f["sample_cp"]
+    ref = f["gradWint_sample"]
+    scale = f["max_abs"][:, None, None]
+    assert np.all(np.abs(f["total"]) <= 1e-12 * f["fro"])
+    gi, ge = gradlinelastwork_an(*inputs4gradtotalwork(dict(bp.args), SOL, ADJ))
+    err = (np.abs(gi[:, :, cps] - ref) / scale).max()
+    fro = np.sqrt((gi ** 2).sum(axis=(1, 2)))
+    print("  drop-in: sampled relative error %.2e, Frobenius %s vs %s" % (err, fro, f["fro"]), flush=True)
+    assert err <= 1e-10 and np.all(np.abs(fro - f["fro"]) <= 1e-10 * f["fro"])
+    assert np.abs(ge).max() == 0.0 == float(f["gradWext_max"])
+    mf = MatrixFree(bp.args, precisions=F64)
+    q = q_grad(mf, SOL)                       # -u^T dK/dX u = 2 gradWint_0 for ADJ_0 = u / 2
+    err_mf = (np.abs(0.5 * q[:, cps] - ref[0]) / scale[0]).max()
+    print("  matrix-free MODE 2: sampled relative error %.2e" % err_mf, flush=True)
+    assert err_mf <= 1e-10
+    mf = None
+    patch = CompliancePatch(bp.args, solver="csr", rtol=1e-12)
+    X0 = patch.coords0.clone().cuda()
+    Xg = X0.clone().requires_grad_(True)
+    J = compliance(Xg, patch)
+    J.backward()
+    assert patch.converged
+    g = Xg.grad.detach().cpu().numpy()
+    gmax = np.abs(g).max()
+    rng = np.random.default_rng(50)
+    interior = np.flatnonzero(np.abs(g) > 1e-3 * gmax)
+    for h, tol in ((1e-4, 1e-5), (1e-6, 1e-4)):
+        worst = 0.0
+        for i in rng.choice(interior, 4, replace=False):
+            Xp, Xm = X0.clone(), X0.clone()
+            Xp[i] += h
+            Xm[i] -= h
+            fd = (compliance(Xp, patch).item() - compliance(Xm, patch).item()) / (2 * h)
+            worst = max(worst, abs(fd - g[i]) / gmax)
+        print("  central differences, step %g: max error %.2e of max|g|" % (h, worst), flush=True)
+        assert worst <= tol

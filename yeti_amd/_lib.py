@@ -36,6 +36,8 @@ SIGNATURES = {
     "yeti_asm_run_vib_grad": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "yeti_asm_update_coords": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_asm_load_shape_grad": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_grad_lin_elast_work": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                             C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_asm_set_timing": (C.c_int32, [C.c_void_p, C.c_int32]),
     "yeti_asm_kernel_times": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
     "yeti_asm_destroy": (None, [C.c_void_p]),

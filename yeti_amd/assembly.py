@@ -162,6 +162,27 @@ class Assembler:
             vf.record_stream(stream)   # the temporary stays allocated until that stream has used it
         return gradV.reshape(self.nb_cp, 3, nf).permute(2, 1, 0)
 
+    def grad_lin_elast_work(self, sol, adj, compute_wint=True, compute_wext=True, stream=None):
+        """(gradWint, gradWext), CUDA float64 tensors of nadj x 3 nb_cp (adjoint-major, dof
+        3 I + d fastest), for the dof-interleaved state ``sol`` (3 nb_cp) and adjoints ``adj``
+        (nadj x 3 nb_cp): -adj_k^T dK/dX sol and adj_k^T dF/dX with the reference's external terms
+        (centrifugal loads). ``yeti_grad_lin_elast_work`` works on the (nadj, 3, nb_cp)
+        column-major layout, which is the transpose of these tensors."""
+        n3 = 3 * self.nb_cp
+        sol = sol.detach().to(device="cuda", dtype=torch.float64).reshape(-1).contiguous()
+        adj = adj.detach().to(device="cuda", dtype=torch.float64).reshape(-1, n3)
+        if sol.numel() != n3:
+            raise ValueError("sol must hold 3 nb_cp = %d values" % n3)
+        nadj = adj.shape[0]
+        adj_cm = adj.t().contiguous()                       # (3 nb_cp, nadj) = (nadj, 3, nb_cp) col-major
+        gi = torch.empty(n3, nadj, dtype=torch.float64, device="cuda")
+        ge = torch.empty(n3, nadj, dtype=torch.float64, device="cuda")
+        _lib.check(_lib.lib().yeti_grad_lin_elast_work(self._h, C.c_void_p(sol.data_ptr()), C.c_void_p(adj_cm.data_ptr()),
+                                                       int(nadj), int(bool(compute_wint)), int(bool(compute_wext)),
+                                                       C.c_void_p(gi.data_ptr()), C.c_void_p(ge.data_ptr()),
+                                                       _stream_handle(stream)))
+        return gi.t(), ge.t()
+
     def strain_energy_grad(self, u, g=None, stream=None):
         """-d(u^T K u)/dX (3 nb_cp, COORDS3D order) for the dof-interleaved CUDA vector u, over the
         active solid elements (any degree triple the stiffness accepts)."""

@@ -149,6 +149,7 @@ struct LoadTable {
     const int32_t *elem_loads_ptr; // [n_loaded + 1]
     const double *nodal_dist;      // (nb_n_dist, nb_cp)
     int32_t nb_n_dist;
+    int32_t only_type = 0;         // load-gradient runs: 0 = every load, else only this JDLType
 };
 
 struct RhsPatch {
@@ -476,6 +477,7 @@ __global__ void __launch_bounds__(64) elem_load_grad_kernel(RhsPatch rp, LoadTab
                     for (int li = l0; li < l1; ++li) {
                         const int il = lt.elem_loads[li];
                         const int t = lt.jdltype[il];
+                        if (lt.only_type && t != lt.only_type) continue;
                         const double *A = lt.add_infos + lt.add_off[il];
                         if (t == 101) {
                             double vd[3] = {A[3] - A[0], A[4] - A[1], A[5] - A[2]};
@@ -506,7 +508,7 @@ __global__ void __launch_bounds__(64) elem_load_grad_kernel(RhsPatch rp, LoadTab
     for (int li = l0; li < l1; ++li) {
         int il = lt.elem_loads[li];
         int t = lt.jdltype[il];
-        if (!(t > 9 && t < 100)) continue;
+        if (!(t > 9 && t < 100) || (lt.only_type && t != lt.only_type)) continue;
         int face = t / 10, typ = t % 10;
         int field = typ == 4 ? (int)lt.add_infos[lt.add_off[il]] : 0;
         // columns spanning the face, in the order of elem_rhs_kernel's normal
@@ -671,6 +673,11 @@ struct yeti_asm {
     yeti::DevBuf<int64_t> vib_ent;
     yeti::DevBuf<double> vib_eg, vib_em, vib_mass;
     yeti::DevBuf<double> vol_u;    // translation field of the volume gradient
+    // gradLinElastWork_AN: offsets of the centrifugal loads' axis points in load_additionalInfos
+    // as the reference reads them (the k-th load of type 101 at 6k), and the work buffers
+    yeti::DevBuf<int32_t> add_off_ref;
+    bool ref_infos_ok = true;
+    yeti::DevBuf<double> gw_fields, gw_grads, gw_vec, gw_vec2, gw_sums;
     int64_t vib_nent = -1, vib_nslot = 0;
     ~yeti_asm() {
         for (auto *p : patches) delete p;
@@ -1871,12 +1878,15 @@ static int run_volume_grad(yeti_asm *h, double *vol, double *g, hipStream_t st) 
 }
 
 // u^T F shape derivative of the face and body loads: g = 2 u^T dF/dX (3 per CP), gathered like F.
-static int run_load_grad(yeti_asm *h, const double *u, double *g, hipStream_t st) {
+// reference_terms: only the terms gradLinElastWork_AN computes (centrifugal loads, their axis
+// points read as it reads them; its face-load term is a stub, optim/gradUELMAT_solid.f90:463-477).
+static int run_load_grad(yeti_asm *h, const double *u, double *g, hipStream_t st, bool reference_terms = false) {
     if (h->n_loaded_total > 0 && !h->grad_buf.p) YETI_HIP(h->grad_buf.alloc((size_t)h->n_loaded_total * 3 * RHS_MAXN));
     LoadTable lt;
     lt.jdltype = h->jdl.p;
     lt.adlmag = h->adlmag.p;
-    lt.add_off = h->add_off.p;
+    lt.add_off = reference_terms ? h->add_off_ref.p : h->add_off.p;
+    lt.only_type = reference_terms ? 101 : 0;
     lt.add_infos = h->add_infos.p;
     lt.elem_loads = h->elem_loads.p;
     lt.elem_loads_ptr = h->elem_loads_ptr.p;
@@ -1906,6 +1916,127 @@ static int run_load_grad(yeti_asm *h, const double *u, double *g, hipStream_t st
     hipLaunchKernelGGL(rhs_gather_kernel, dim3(blocks), dim3(256), 0, st, h->nb_cp, (const double *)nullptr,
                        h->cp_ptr.p, h->cp_ent.p, (const double *)h->grad_buf.p, g);
     YETI_HIP(hipGetLastError());
+    return 0;
+}
+
+// ---- gradLinElastWork_AN: internal and external work gradients for given SOL and ADJ --------
+// Fields f of an (nf, 3, nb_cp) column-major array sit at f + nf (3 cp + d).
+
+// sums[f] = sum over (d, cp) of V[f, d, cp]^2, one block per field, fixed order (deterministic)
+__global__ void __launch_bounds__(256) field_sumsq_kernel(const double *V, int32_t nf, int64_t n3, double *sums) {
+    const int f = blockIdx.x;
+    double s = 0.0;
+    for (int64_t j = threadIdx.x; j < n3; j += blockDim.x) {
+        const double v = V[f + (int64_t)nf * j];
+        s += v * v;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[f] = red[0];
+}
+
+// V[2k] = u + s_k a_k, V[2k + 1] = u - s_k a_k (fields of an (2 nadj, 3, nb_cp) array)
+__global__ void polar_fields_kernel(const double *u, const double *adj, int32_t nadj, int64_t n3, const double *scale,
+                                    double *V) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n3 * nadj) return;
+    const int64_t j = x / nadj;
+    const int k = (int)(x % nadj);
+    const double a = scale[k] * adj[k + (int64_t)nadj * j];
+    V[2 * k + (int64_t)(2 * nadj) * j] = u[j] + a;
+    V[2 * k + 1 + (int64_t)(2 * nadj) * j] = u[j] - a;
+}
+
+// out[k, j] = (G[2k, j] - G[2k + 1, j]) / (4 s_k): the polarisation of the quadratic form
+__global__ void polar_combine_kernel(const double *G, int32_t nadj, int64_t n3, const double *scale, double *out) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n3 * nadj) return;
+    const int64_t j = x / nadj;
+    const int k = (int)(x % nadj);
+    out[k + (int64_t)nadj * j] =
+        (G[2 * k + (int64_t)(2 * nadj) * j] - G[2 * k + 1 + (int64_t)(2 * nadj) * j]) / (4.0 * scale[k]);
+}
+
+// a[j] = ADJ[k, j] and back: out[k, j] = c g[j]
+__global__ void field_extract_kernel(const double *adj, int32_t nadj, int32_t k, int64_t n3, double *a) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n3) a[j] = adj[k + (int64_t)nadj * j];
+}
+__global__ void field_insert_kernel(const double *g, double c, int32_t nadj, int32_t k, int64_t n3, double *out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n3) out[k + (int64_t)nadj * j] = c * g[j];
+}
+
+// gradWint[k] = -ADJ_k^T (dK/dX) SOL and gradWext[k] = ADJ_k^T (dF/dX) over the active solid
+// elements (gradtotalwork_fullAN.f90:17-300). The internal term is bilinear and symmetric, and
+// the gradient kernel gives the quadratic form q(v) = -v^T (dK/dX) v, so
+//   -a^T (dK/dX) u = (q(u + s a) - q(u - s a)) / (4 s),
+// with s = |u| / |a| so that both fields have the norm of u (no cancellation beyond rounding);
+// the 2 nadj fields run through the kernel together (its field-independent work is shared).
+static int run_grad_lin_elast_work(yeti_asm *h, const double *sol, const double *adj, int32_t nadj, bool wint,
+                                   bool wext, double *gint, double *gext, hipStream_t st) {
+    const int64_t n3 = 3 * (int64_t)h->nb_cp;
+    const unsigned blk = (unsigned)((n3 * nadj + 255) / 256), blk1 = (unsigned)((n3 + 255) / 256);
+    if (h->gw_sums.n < (size_t)(2 * nadj + 1)) YETI_HIP(h->gw_sums.alloc(2 * (size_t)nadj + 1));
+    if (wint) {
+        if (h->gw_fields.n < (size_t)(2 * nadj * n3)) {
+            YETI_HIP(h->gw_fields.alloc((size_t)(2 * nadj * n3)));
+            YETI_HIP(h->gw_grads.alloc((size_t)(2 * nadj * n3)));
+        }
+        double *sums = h->gw_sums.p, *scale = h->gw_sums.p + nadj + 1;
+        hipLaunchKernelGGL(field_sumsq_kernel, dim3(1), dim3(256), 0, st, sol, 1, n3, sums);
+        hipLaunchKernelGGL(field_sumsq_kernel, dim3(nadj), dim3(256), 0, st, adj, nadj, n3, sums + 1);
+        YETI_HIP(hipGetLastError());
+        std::vector<double> hs(nadj + 1), sc(nadj);
+        YETI_HIP(hipMemcpyAsync(hs.data(), sums, (nadj + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+        YETI_HIP(hipStreamSynchronize(st));
+        for (int k = 0; k < nadj; ++k)
+            sc[k] = (hs[0] > 0.0 && hs[k + 1] > 0.0) ? std::sqrt(hs[0] / hs[k + 1]) : 1.0;
+        YETI_HIP(hipMemcpyAsync(scale, sc.data(), nadj * sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(polar_fields_kernel, dim3(blk), dim3(256), 0, st, sol, adj, nadj, n3, (const double *)scale,
+                           h->gw_fields.p);
+        YETI_HIP(hipGetLastError());
+        std::vector<double> zeros(2 * nadj, 0.0);
+        int rc = run_vib_grad(h, h->gw_fields.p, zeros.data(), 2 * nadj, h->gw_grads.p, st, -1.0, false);
+        if (rc) return rc;
+        hipLaunchKernelGGL(polar_combine_kernel, dim3(blk), dim3(256), 0, st, (const double *)h->gw_grads.p, nadj, n3,
+                           (const double *)scale, gint);
+        YETI_HIP(hipGetLastError());
+        // the host copy of the scales must outlive the copy above
+        YETI_HIP(hipStreamSynchronize(st));
+    } else {
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(blk), dim3(256), 0, st, gint, (int32_t *)nullptr, (int32_t *)nullptr,
+                           n3 * nadj);
+        YETI_HIP(hipGetLastError());
+    }
+    if (wext) {
+        if (!h->ref_infos_ok)
+            return set_error(YETI_ERR_INVALID_ARGUMENT,
+                             "load_additionalInfos holds fewer than 6 values per centrifugal load");
+        if (h->gw_vec.n < (size_t)n3) {
+            YETI_HIP(h->gw_vec.alloc(n3));
+            YETI_HIP(h->gw_vec2.alloc(n3));
+        }
+        for (int k = 0; k < nadj; ++k) {
+            hipLaunchKernelGGL(field_extract_kernel, dim3(blk1), dim3(256), 0, st, adj, nadj, k, n3, h->gw_vec.p);
+            YETI_HIP(hipGetLastError());
+            int rc = run_load_grad(h, h->gw_vec.p, h->gw_vec2.p, st, true);
+            if (rc) return rc;
+            // run_load_grad gives 2 a^T dF/dX
+            hipLaunchKernelGGL(field_insert_kernel, dim3(blk1), dim3(256), 0, st, (const double *)h->gw_vec2.p, 0.5, nadj,
+                               k, n3, gext);
+            YETI_HIP(hipGetLastError());
+        }
+    } else {
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(blk), dim3(256), 0, st, gext, (int32_t *)nullptr, (int32_t *)nullptr,
+                           n3 * nadj);
+        YETI_HIP(hipGetLastError());
+    }
     return 0;
 }
 
@@ -2122,6 +2253,14 @@ static int create(const yeti_system_inputs *in, int layout, yeti_asm **out) {
     std::vector<int32_t> jdl(in->JDLType, in->JDLType + in->nb_load);
     std::vector<int32_t> add_off(in->nb_load + 1, 0);
     for (int il = 0; il < in->nb_load; ++il) add_off[il + 1] = add_off[il] + in->nb_load_additionalInfos[il];
+    // gradLinElastWork_AN reads the k-th centrifugal load's axis points A, B at values 6k+1..6k+6
+    // of load_additionalInfos, whatever the other loads hold (its loadcount advances for type 101
+    // only, optim/gradUELMAT_solid.f90:378-398)
+    std::vector<int32_t> add_off_ref(in->nb_load + 1, 0);
+    int64_t n101 = 0;
+    for (int il = 0; il < in->nb_load; ++il)
+        if (in->JDLType[il] == 101) add_off_ref[il] = (int32_t)(6 * n101++);
+    h->ref_infos_ok = 6 * n101 <= std::max<int64_t>(0, in->len_load_additionalInfos);
     if (in->nb_load > 0 && add_off[in->nb_load] > std::max<int64_t>(0, in->len_load_additionalInfos))
         return fail(set_error(YETI_ERR_INVALID_ARGUMENT, "load_additionalInfos holds %lld values, the loads name %d",
                               (long long)in->len_load_additionalInfos, add_off[in->nb_load]));
@@ -2139,6 +2278,7 @@ static int create(const yeti_system_inputs *in, int layout, yeti_asm **out) {
     if (e2 == hipSuccess) e2 = h->jdl.upload(jdl.data(), jdl.size());
     if (e2 == hipSuccess) e2 = h->adlmag.upload(in->ADLMAG, in->nb_load);
     if (e2 == hipSuccess) e2 = h->add_off.upload(add_off.data(), add_off.size());
+    if (e2 == hipSuccess) e2 = h->add_off_ref.upload(add_off_ref.data(), add_off_ref.size());
     if (e2 == hipSuccess) e2 = h->add_infos.upload(in->load_additionalInfos, std::max<int64_t>(1, in->len_load_additionalInfos));
     if (e2 == hipSuccess) e2 = h->elem_loads.upload(elem_loads.data(), elem_loads.size());
     if (e2 == hipSuccess) e2 = h->elem_loads_ptr.upload(elem_loads_ptr.data(), elem_loads_ptr.size());
@@ -2257,6 +2397,14 @@ int32_t yeti_asm_strain_energy_grad(yeti_asm *h, const double *u, double *g, voi
 int32_t yeti_asm_volume_grad(yeti_asm *h, double *vol, double *g, void *stream) {
     if (!h || !vol || !g) return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
     return yeti::run_volume_grad(h, vol, g, (hipStream_t)stream);
+}
+
+int32_t yeti_grad_lin_elast_work(yeti_asm *h, const double *SOL, const double *ADJ, int32_t nadj, int32_t computeWint,
+                                 int32_t computeWext, double *gradWint, double *gradWext, void *stream) {
+    if (!h || !SOL || !ADJ || !gradWint || !gradWext || nadj < 1)
+        return yeti::set_error(YETI_ERR_INVALID_ARGUMENT, "null argument or nadj < 1");
+    return yeti::run_grad_lin_elast_work(h, SOL, ADJ, nadj, computeWint != 0, computeWext != 0, gradWint, gradWext,
+                                         (hipStream_t)stream);
 }
 
 int32_t yeti_asm_run_rhs(yeti_asm *h, double *F, void *stream) {

@@ -16,6 +16,8 @@ _propagateSensitivityAnalysis, OPT_modelling.py:439-445). Homogeneous Dirichlet 
 """
 from __future__ import annotations
 
+import warnings
+
 import numpy as np
 import torch
 
@@ -56,17 +58,25 @@ class CompliancePatch:
         self.mask = free_mask_from_args(d)
         self.nb_cp = self.asm.nb_cp
         self.coords0 = torch.from_numpy(np.asarray(d["COORDS3D"], dtype=np.float64).reshape(-1, order="F").copy())
+        # coordinates the resident operator was last built at (device, 3 nb_cp)
+        self.coords = self.coords0.to("cuda")
         self.iters = 0
         self.relres = 0.0
+        self.converged = True
 
     def set_coords(self, coords):
         c = coords.detach().reshape(-1).to(device="cuda", dtype=torch.float64).contiguous()
+        if c.numel() != 3 * self.nb_cp:
+            raise ValueError("expected %d coordinates, got %d" % (3 * self.nb_cp, c.numel()))
         self.asm.update_coords(c)
         if self.mf is not None:
             self.mf.update_coords(c)
+        self.coords = c.clone()
 
     def solve(self):
-        """(u, F) at the current coordinates; u is 0 on the fixed dofs."""
+        """(u, F) at the current coordinates; u is 0 on the fixed dofs. A solve that stops at
+        maxit above rtol sets ``converged`` False and warns: J and its gradient then come from an
+        unconverged u."""
         if self.solver == "csr":
             K, r, c, F = self.asm.run_coo()
             vals = self.csr.values(K)
@@ -76,6 +86,10 @@ class CompliancePatch:
             self.asm.run_rhs(F)
             u, it, rel = self.mf.pcg_f64(F, self.mask, rtol=self.rtol, maxit=self.maxit)
         self.iters, self.relres = it, rel
+        self.converged = bool(rel <= self.rtol)
+        if not self.converged:
+            warnings.warn("compliance solve stopped after %d iterations at relative residual %.3g > rtol %.3g"
+                          % (it, rel, self.rtol), RuntimeWarning, stacklevel=2)
         return u, F
 
     def shape_gradient(self, u):
@@ -93,14 +107,26 @@ class ComplianceFunction(torch.autograd.Function):
         u, F = patch.solve()
         J = torch.dot(F, u)
         ctx.patch = patch
-        ctx.save_for_backward(u)
+        ctx.save_for_backward(u, patch.coords.clone())
         ctx.coords_shape = coords.shape
         return J
 
     @staticmethod
     def backward(ctx, gJ):
-        (u,) = ctx.saved_tensors
-        g = ctx.patch.shape_gradient(u)
+        # the patch may have moved since this forward (a later forward, as in a line search):
+        # the gradient is taken at the coordinates this J was computed at, then the patch is put
+        # back where it was
+        u, X = ctx.saved_tensors
+        patch = ctx.patch
+        moved = not torch.equal(patch.coords, X)
+        if moved:
+            current = patch.coords
+            patch.set_coords(X)
+        try:
+            g = patch.shape_gradient(u)
+        finally:
+            if moved:
+                patch.set_coords(current)
         return (gJ * g).reshape(ctx.coords_shape).to(gJ.device), None
 
 

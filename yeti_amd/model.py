@@ -752,6 +752,14 @@ class IGAparametrization:
         kw["num_threads"] = -1 if num_threads is None else num_threads
         return kw
 
+    def _get_inputs4gradTotalwork(self, sol, ADJ, COORDS=None, activeElem=None, activeDir=np.ones(3, np.intp),
+                                  computeWint=True, computeWext=True):
+        """Positional argument list of gradadjointwork.gradlinelastwork_an
+        (IGA_parametrization.py:1507-1527)."""
+        from .gradadjointwork import inputs4gradtotalwork
+        return inputs4gradtotalwork(self._flat(), sol, ADJ, COORDS=COORDS, activeElem=activeElem, activeDir=activeDir,
+                                    computeWint=computeWint, computeWext=computeWext)
+
     def get_model_dict(self):
         """The same inputs by name (what yeti_amd.assembly.Assembler also accepts)."""
         return self._flat()
