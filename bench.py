@@ -44,24 +44,47 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(p, sample_nel, threads):
-    """YETI's sys_linmat_lindef_static_omp (compiled reference) on a sample patch; falls back to
-    the C restatement (oracle/liboracle.so) when the reference library is absent."""
+def cpu_baseline(nel, p, prefix, runs, threads):
+    """YETI's sys_linmat_lindef_static_omp (compiled reference, oracle/_ref) on this host's cores:
+    the first ``prefix`` elements of the benchmark patch itself (activeElement marks them; the
+    routine's slot of element e lies below e * nb_data_elem, so the COO arrays hold only the
+    prefix's slots) and, in full, the 16x4x4 p=2 cantilever. Each figure is the median of
+    ``runs`` calls after one warm-up, output allocation included. Falls back to the C restatement
+    (oracle/liboracle.so, packed layout) when the reference library is absent."""
+    import numpy as np
     from oracle import oracle as O
     from yeti_amd.geometry import make_box_patch
-    bp = make_box_patch(nel=sample_nel, p=p, seed=20261202)
-    args = bp.f2py_args()
     kind = "reference" if O.ref_available() else "port"
-    run = (lambda: O.ref_sys_linmat_omp(args, num_threads=threads)) if kind == "reference" else \
-        (lambda: O.oracle_sys_linmat(args, nthreads=threads))
-    t0 = time.perf_counter()
-    run()
-    dt = time.perf_counter() - t0
-    ne = bp.nb_elem
-    return {"value": ne / dt, "unit": "elem/s", "cores": threads, "kind": kind,
-            "sample": "%dx%dx%d elements p=%d (%d elements, same generator), one call of "
-                      "sys_linmat_lindef_static_omp incl. output allocation, %.1f s"
-                      % (sample_nel[0], sample_nel[1], sample_nel[2], p, ne, dt)}
+
+    def timed(d):
+        run = (lambda: O.ref_sys_linmat_omp((), num_threads=threads, kwargs=d)) if kind == "reference" else \
+            (lambda: O.oracle_sys_linmat((), nthreads=threads, kwargs=d))
+        run()
+        ts = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            run()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), ts
+
+    bp = make_box_patch(nel=(nel, nel, nel), p=p)
+    n = (p + 1) ** 3
+    nde = 9 * n * (n - 1) // 2 + 6 * n
+    d = dict(bp.args)
+    act = np.zeros(bp.nb_elem, np.int32)
+    act[:prefix] = 1
+    d["activeElement"] = act
+    d["nb_data"] = float(prefix * nde)
+    t_pre, ts_pre = timed(d)
+    c1 = make_box_patch(nel=(16, 4, 4), p=2)
+    t_c1, _ = timed(dict(c1.args))
+    return {"value": prefix / t_pre, "unit": "elem/s", "cores": threads, "kind": kind,
+            "sample": "first %d of the %d elements of the %d^3 p=%d benchmark patch (activeElement), "
+                      "sys_linmat_lindef_static_omp incl. output allocation, median of %d calls after a "
+                      "warm-up: %.2f s (runs %s)" % (prefix, bp.nb_elem, nel, p, runs, t_pre,
+                                                    ", ".join("%.2f" % t for t in ts_pre)),
+            "c1_16x4x4_p2": {"elements": c1.nb_elem, "seconds": t_c1, "value": c1.nb_elem / t_c1,
+                             "unit": "elem/s"}}
 
 
 def read_traffic(path):
@@ -129,9 +152,10 @@ def bench_serving(n, rate, seed, budget, plain_n):
 
 
 def bench_c2_solve(nel, p, jacobi_iters):
-    """The C2 solve after the assembly: device CSR of K + K^T, its SpMV against the HBM roofline,
-    Jacobi PCG for a bounded number of iterations (residual reached), and the fast-diagonalisation
-    PCG session (FP64, matrix-free) to 1e-10, its solution checked with the assembled CSR."""
+    """The C2 solve after the assembly: device CSR of K + K^T, its nodal-block SpMV against the
+    HBM roofline, Jacobi PCG to 1e-10 (2-norm) with the true residual and the FP64 rounding floor
+    eps ||(|A||x|)|| / ||b|| of the final iterate, and the fast-diagonalisation PCG session (FP64,
+    matrix-free) to 1e-10, its solution checked with the assembled CSR."""
     import numpy as np
     import torch
     from yeti_amd.assembly import Assembler
@@ -168,18 +192,24 @@ def bench_c2_solve(nel, p, jacobi_iters):
     e1.record(stream)
     torch.cuda.synchronize()
     spmv_ms = e0.elapsed_time(e1) / 10
-    spmv_bytes = 12 * csr.nnz + 8 * (csr.nrows + 1) + 16 * csr.nrows
+    ncp = csr.nrows // 3
+    nblocks = csr.nnz // 9
+    # values, one block column per 3x3 block, the block row pointers, x read, y written
+    spmv_bytes = 8 * csr.nnz + 4 * nblocks + 8 * (ncp + 1) + 16 * csr.nrows
     log("C2 %d^3 p=%d: CSR pattern %.2f s (nnz %d), values %.1f ms, SpMV %.3f ms (%.0f GB/s)"
         % (nel, p, t_pat, csr.nnz, t_val * 1e3, spmv_ms, spmv_bytes / (spmv_ms * 1e-3) / 1e9))
     mask = free_mask_from_args(bp.args)
+    maskd = mask.to(torch.float64)
+    bnorm = float((F * maskd).norm())
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    try:
-        _, it, rel = csr.pcg(vals, F, mask, rtol=1e-10, maxit=jacobi_iters)
-    except Exception as ex:   # not converged within the bound: read the reached residual from the message
-        it, rel = jacobi_iters, float(str(ex).split("relative residual")[1].split()[0])
+    u, it, rel = csr.pcg(vals, F, mask, rtol=1e-10, maxit=jacobi_iters, check_every=100, raise_on_maxit=False)
     torch.cuda.synchronize()
     t_jac = time.perf_counter() - t0
-    log("C2 Jacobi PCG: %d iterations in %.2f s, relative residual %.3e" % (it, t_jac, rel))
+    true_rel = float(((F - csr.spmv(vals, u)) * maskd).norm()) / bnorm
+    floor = float(np.finfo(np.float64).eps) * float((csr.spmv_abs(vals, u) * maskd).norm()) / bnorm
+    log("C2 Jacobi PCG: %d iterations in %.2f s (%.3f ms/iteration), relative residual %.3e (true %.3e, "
+        "FP64 floor eps||A||x|| / ||b|| %.3e)" % (it, t_jac, t_jac / max(it, 1) * 1e3, rel, true_rel, floor))
     req = Request(0, 0.0, (nel, nel, nel), p, 210000.0, 0.3, -1.0, 20261202)
     req.args = bp.args
     srv = SolveServer(budget_bytes=64 << 30, precision="fp64", nstreams=1, chunk=32, rtol=1e-10, max_iters=20000)
@@ -187,22 +217,26 @@ def bench_c2_solve(nel, p, jacobi_iters):
     rep = srv.run([req], keep_solutions=True, realtime=False)
     t_fdm = time.perf_counter() - t0
     d = rep["requests"][0]
-    u = torch.as_tensor(rep["x"][0], device="cuda")
+    uf = torch.as_tensor(rep["x"][0], device="cuda")
     srv.close()
-    r = (F - csr.spmv(vals, u)) * mask
-    rel_csr = float(r.norm() / (F * mask).norm())
+    rel_csr = float(((F - csr.spmv(vals, uf)) * maskd).norm()) / bnorm
+    du = float((uf - u).norm() / u.norm())
     log("C2 fast-diagonalisation PCG (FP64, matrix-free): %d iterations in %.2f s, residual %.3e "
-        "(assembled CSR: %.3e)" % (d["iters"], t_fdm, d["relres"], rel_csr))
+        "(assembled CSR: %.3e), |u_fdm - u_jacobi| / |u_jacobi| %.2e" % (d["iters"], t_fdm, d["relres"], rel_csr, du))
     out = {"workload": "%d^3 p=%d cantilever (%d dofs): device CSR of K + K^T from the FP64 COO values, "
-                       "Jacobi PCG (bounded) and fast-diagonalisation PCG to 1e-10" % (nel, p, csr.nrows),
+                       "Jacobi PCG and fast-diagonalisation PCG to 1e-10" % (nel, p, csr.nrows),
            "csr_pattern_s": t_pat, "csr_values_ms": t_val * 1e3, "nnz": csr.nnz,
-           "spmv": {"ms": spmv_ms, "roofline": {"bound": "hbm", "achieved": spmv_bytes / (spmv_ms * 1e-3) / 1e9,
-                                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                                "frac": spmv_bytes / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}},
-           "jacobi_pcg": {"iterations": it, "seconds": t_jac, "relres": rel, "converged": rel <= 1e-10},
+           "spmv": {"ms": spmv_ms, "kernel": "block_spmv_kernel (nodal 3x3 blocks, one wave per control point)",
+                    "algorithmic_bytes": spmv_bytes,
+                    "roofline": {"bound": "hbm", "achieved": spmv_bytes / (spmv_ms * 1e-3) / 1e9,
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": spmv_bytes / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}},
+           "jacobi_pcg": {"iterations": it, "max_iterations": jacobi_iters, "seconds": t_jac,
+                          "ms_per_iteration": t_jac / max(it, 1) * 1e3, "relres": rel, "true_relres": true_rel,
+                          "fp64_floor_eps_absA_absx_over_b": floor, "converged": rel <= 1e-10},
            "fdm_pcg_fp64": {"iterations": d["iters"], "seconds": t_fdm, "relres": d["relres"],
                             "relres_assembled_csr": rel_csr, "stopped_at_fp64_floor": d["floor"],
-                            "failed": d["failed"]}}
+                            "failed": d["failed"], "rel_diff_vs_jacobi_solution": du}}
     del csr, vals, asm
     torch.cuda.empty_cache()
     return out
@@ -395,17 +429,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nel", type=int, default=64, help="elements per direction")
     ap.add_argument("--p", type=int, default=3, help="degree")
-    ap.add_argument("--cpu-sample", default="32,16,16", help="elements per direction of the CPU sample")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    ap.add_argument("--cpu-prefix", type=int, default=16384,
+                    help="elements of the benchmark patch timed by the CPU baseline")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="timed CPU calls (median) after one warm-up")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = every core of this host")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="offered requests per second")
     ap.add_argument("--serve-plain", type=int, default=0,
                     help="requests solved one at a time (plain FP64); 0: all of them")
     ap.add_argument("--grad-fd", type=int, default=50, help="coordinates checked by central differences")
-    ap.add_argument("--jacobi-iters", type=int, default=1000, help="bound on the C2 Jacobi PCG iterations")
+    ap.add_argument("--jacobi-iters", type=int, default=60000, help="bound on the C2 Jacobi PCG iterations")
     ap.add_argument("--mp-nel", type=int, default=128, help="elements per direction of the mixed-precision solve")
     ap.add_argument("--only-assembly", action="store_true")
+    ap.add_argument("--sections", default="c2,serving,gradient,mixed",
+                    help="comma list of the parts run after the assembly: c2, serving, gradient, mixed")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run only a 2 GiB device copy and one element-kernel run (for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -538,11 +576,11 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        sample = tuple(int(x) for x in args.cpu_sample.split(","))
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log("cpu baseline on %s with %d threads ..." % (sample, threads))
-        cpu = cpu_baseline(p, sample, threads)
-        log("cpu baseline: %.3e elem/s (%s)" % (cpu["value"], cpu["kind"]))
+        threads = args.cpu_threads or len(os.sched_getaffinity(0))
+        log("cpu baseline: first %d elements with %d threads ..." % (args.cpu_prefix, threads))
+        cpu = cpu_baseline(args.nel, p, args.cpu_prefix, args.cpu_runs, threads)
+        log("cpu baseline: %.3e elem/s (%s); 16x4x4 p=2: %.3e elem/s"
+            % (cpu["value"], cpu["kind"], cpu["c1_16x4x4_p2"]["value"]))
 
     out = {
         "metric": METRIC,
@@ -584,10 +622,14 @@ def main():
         "strain_energy_gradient": {"ms": seg_ms, "elements_per_s": bp.nb_elem / (seg_ms * 1e-3)},
         "volume_gradient": {"ms": vol_ms, "elements_per_s": bp.nb_elem / (vol_ms * 1e-3)},
     }
-    if not args.only_assembly:
+    sections = set() if args.only_assembly else set(args.sections.split(","))
+    if "c2" in sections:
         out["c2_solve"] = bench_c2_solve(args.nel, p, args.jacobi_iters)
+    if "serving" in sections:
         out["serving"] = bench_serving(args.serve_requests, args.serve_rate, 20261202, 200 << 30, args.serve_plain)
+    if "gradient" in sections:
         out["gradient"] = bench_gradient(args.grad_fd)
+    if "mixed" in sections:
         out["mixed_precision"] = bench_mixed_precision(args.mp_nel)
     print(json.dumps(out), flush=True)
 

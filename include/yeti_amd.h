@@ -212,8 +212,13 @@ const int32_t *yeti_csr_colind(const yeti_csr *c);   /* device, [nnz]       */
 int32_t yeti_csr_pattern_to_host(const yeti_csr *c, int64_t *rowptr_host, int32_t *colind_host);
 /* vals[nnz] = CSR values of K_side + K_side^T from the COO values the handle wrote. */
 int32_t yeti_csr_values(const yeti_csr *c, const double *Kdata_dev, double *vals_dev, void *stream);
+/* y = A x, walking the nodal 3x3 blocks (one wave per control point, each block column read once). */
 int32_t yeti_csr_spmv(const yeti_csr *c, const double *vals_dev, const double *x_dev, double *y_dev,
                       void *stream);
+/* y = |A| |x| (entrywise absolute values): eps ||(|A||x|)|| bounds the FP64 rounding of A x, the
+ * floor below which no residual of this matrix can be resolved. */
+int32_t yeti_csr_spmv_abs(const yeti_csr *c, const double *vals_dev, const double *x_dev, double *y_dev,
+                          void *stream);
 void    yeti_csr_destroy(yeti_csr *c);
 /* scipy's canonical K_side + K_side^T: the structural CSR with every entry whose sum is exactly 0
  * removed (scipy's sparse add drops them; overlay/iga_model.py:391-395 returns that matrix).
@@ -231,6 +236,14 @@ int32_t yeti_csr_drop_zeros(const yeti_csr *c, const double *vals_dev, int64_t *
 int32_t yeti_pcg_csr(int64_t nrows, const int64_t *rowptr, const int32_t *colind, const double *vals,
                      const double *b, const uint8_t *free_mask, double *x, double rtol, int32_t maxit,
                      int32_t check_every, int32_t *iters, double *relres, void *stream);
+
+/* Jacobi PCG on a device CSR handle (its nodal-block product; same iteration as yeti_pcg_csr).
+ * norm 2: stop at ||r||_2 <= rtol ||b||_2; norm 0: stop at max|r| <= rtol max|b| over the free
+ * dofs, the rule of iga_mf_cg_3d (maxval(abs(r)) / maxval(abs(b)),
+ * iga_wq_mf/fsrc/mf_iga_solver.f90:779,935). relres is reported in the chosen norm. */
+int32_t yeti_csr_pcg(const yeti_csr *c, const double *vals, const double *b, const uint8_t *free_mask, double *x,
+                     double rtol, int32_t maxit, int32_t check_every, int32_t norm, int32_t *iters, double *relres,
+                     void *stream);
 
 /* ---- matrix-free operator (sum-factorised, FP64 and FP32) --------------------------------
  * y = (K_side + K_side^T) x without forming K, for patches whose elements fill the tensor grid

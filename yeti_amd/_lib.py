@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-from ._abi import ERROR_NAMES, YetiSystemInputs
+from ._abi import ERROR_NAMES, YETI_ERR_NOT_CONVERGED, YetiSystemInputs  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libyeti_hip.so")
@@ -49,6 +49,9 @@ SIGNATURES = {
     "yeti_csr_pattern_to_host": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_csr_values": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_csr_spmv": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_csr_spmv_abs": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_csr_pcg": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int32,
+                                 C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p]),
     "yeti_csr_destroy": (None, [C.c_void_p]),
     "yeti_mf_create": (C.c_int32, [C.POINTER(YetiSystemInputs), C.c_int32, C.POINTER(C.c_void_p)]),
     "yeti_mf_nb_cp": (C.c_int32, [C.c_void_p]),

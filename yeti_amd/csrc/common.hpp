@@ -17,6 +17,14 @@ std::string &last_error();
 
 int set_error(int code, const char *fmt, ...);
 
+// Nodal-block SpMV of a device CSR (csr.hip): y = mask .* (A x) (mode 0), with |A| |x| (mode 1),
+// or mode 2: also partial[b] = grid block b's share of x . y, *nparts blocks (<= 4096).
+int csr_block_spmv(const yeti_csr *c, const double *vals, const double *x, const uint8_t *mask, double *y,
+                   double *partial, int mode, int *nparts, hipStream_t st);
+int64_t csr_nrows(const yeti_csr *c);
+const int64_t *csr_rowptr_dev(const yeti_csr *c);
+const int32_t *csr_colind_dev(const yeti_csr *c);
+
 // Host-side range checks of a model (ids, spans, array lengths, load and condition targets);
 // 0 or YETI_ERR_INVALID_ARGUMENT / YETI_ERR_UNSUPPORTED_ELEMENT with the message set. With
 // require_solid, every patch must be a solid (U1 / U0) patch.

@@ -333,19 +333,92 @@ __global__ void csr_values_kernel(int32_t nb_cp, const int64_t *bptr, const int6
         for (int l = 0; l < 3; ++l) vals[r0 + 3LL * k * nb + 3 * j + l] = v[3 * k + l];
 }
 
-// ---- SpMV: one wave per scalar row ---------------------------------------------------------
-__global__ void __launch_bounds__(256) spmv_kernel(int64_t nrows, const int64_t *__restrict__ rowptr,
-                                                   const int32_t *__restrict__ colind,
-                                                   const double *__restrict__ vals,
-                                                   const double *__restrict__ x, double *__restrict__ y) {
-    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= nrows) return;
-    double s = 0.0;
-    for (int64_t k = rowptr[row] + lane; k < rowptr[row + 1]; k += 64) s += vals[k] * x[colind[k]];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane == 0) y[row] = s;
+// ---- SpMV by nodal 3x3 blocks: one wave per CP row ---------------------------------------
+// The three dof rows 3I, 3I+1, 3I+2 of a CP share their block columns; row 3I+k of block j is
+// vals[9 b0 + 3 k nb + 3 j + l], l = 0..2 (b0 = bptr[I], nb blocks in the row). A lane takes
+// blocks j = lane, lane + 64, ...: it reads the block column once (4 bytes per 9 entries instead
+// of 4 per entry), x at 3J..3J+2, and 3 x 3 consecutive values, consecutive lanes on consecutive
+// 24-byte runs. Rows of a masked-out dof give 0. MODE bits: 1 = |vals| |x| (the FP64 rounding
+// bound of a product), 2 = block partials of x . y for the PCG.
+template <int MODE>
+__global__ void __launch_bounds__(256) block_spmv_kernel(int32_t nb_cp, const int64_t *__restrict__ bptr,
+                                                         const int32_t *__restrict__ bcol,
+                                                         const double *__restrict__ vals,
+                                                         const double *__restrict__ x,
+                                                         const uint8_t *__restrict__ mask,
+                                                         double *__restrict__ y, double *__restrict__ partial) {
+    constexpr bool ABS = (MODE & 1) != 0, DOT = (MODE & 2) != 0;
+    __shared__ double sh[4];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double acc = 0.0;
+    for (int64_t I = (int64_t)blockIdx.x * 4 + w; I < nb_cp; I += (int64_t)gridDim.x * 4) {
+        const int64_t b0 = bptr[I];
+        const int nb = (int)(bptr[I + 1] - b0);
+        const double *v0 = vals + 9 * b0;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int j = lane; j < nb; j += 64) {
+            const int64_t J = bcol[b0 + j];
+            double x0 = x[3 * J], x1 = x[3 * J + 1], x2 = x[3 * J + 2];
+            const double *a0 = v0 + 3 * j, *a1 = a0 + 3 * (int64_t)nb, *a2 = a1 + 3 * (int64_t)nb;
+            double c[9] = {a0[0], a0[1], a0[2], a1[0], a1[1], a1[2], a2[0], a2[1], a2[2]};
+            if (ABS) {
+                x0 = fabs(x0); x1 = fabs(x1); x2 = fabs(x2);
+#pragma unroll
+                for (int t = 0; t < 9; ++t) c[t] = fabs(c[t]);
+            }
+            s0 += c[0] * x0 + c[1] * x1 + c[2] * x2;
+            s1 += c[3] * x0 + c[4] * x1 + c[5] * x2;
+            s2 += c[6] * x0 + c[7] * x1 + c[8] * x2;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            s0 += __shfl_xor(s0, o, 64);
+            s1 += __shfl_xor(s1, o, 64);
+            s2 += __shfl_xor(s2, o, 64);
+        }
+        if (lane == 0) {
+            double r[3] = {s0, s1, s2};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (mask && !mask[3 * I + k]) r[k] = 0.0;
+                y[3 * I + k] = r[k];
+                if (DOT) acc += r[k] * x[3 * I + k];
+            }
+        }
+    }
+    if (DOT) {
+        double v = lane == 0 ? acc : 0.0;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) sh[w] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    }
 }
+
+constexpr int SPMV_MAX_BLOCKS = 4096;
+
+// y = mask .* (A x) (mode 0), the same with |A| |x| (mode 1), or mode 2: y = mask .* (A x) and
+// partial[b] = the grid block's share of x . y (grid of *nparts blocks, at most
+// SPMV_MAX_BLOCKS). mask may be null.
+int csr_block_spmv(const yeti_csr *c, const double *vals, const double *x, const uint8_t *mask, double *y,
+                   double *partial, int mode, int *nparts, hipStream_t st) {
+    if (c->nb_cp <= 0) return 0;
+    const unsigned g = (unsigned)std::min<int64_t>(SPMV_MAX_BLOCKS, ((int64_t)c->nb_cp + 3) / 4);
+    if (nparts) *nparts = (int)g;
+    switch (mode) {
+    case 0: hipLaunchKernelGGL(block_spmv_kernel<0>, dim3(g), dim3(256), 0, st, c->nb_cp, c->bptr.p, c->bcol.p, vals, x,
+                               mask, y, partial); break;
+    case 1: hipLaunchKernelGGL(block_spmv_kernel<1>, dim3(g), dim3(256), 0, st, c->nb_cp, c->bptr.p, c->bcol.p, vals, x,
+                               mask, y, partial); break;
+    default: hipLaunchKernelGGL(block_spmv_kernel<2>, dim3(g), dim3(256), 0, st, c->nb_cp, c->bptr.p, c->bcol.p, vals,
+                                x, mask, y, partial); break;
+    }
+    YETI_HIP(hipGetLastError());
+    return 0;
+}
+
+int64_t csr_nrows(const yeti_csr *c) { return 3 * (int64_t)c->nb_cp; }
+const int64_t *csr_rowptr_dev(const yeti_csr *c) { return c->rowptr.p; }
+const int32_t *csr_colind_dev(const yeti_csr *c) { return c->colind.p; }
 
 }  // namespace yeti
 
@@ -514,11 +587,12 @@ int32_t yeti_csr_values(const yeti_csr *c, const double *Kdata_dev, double *vals
 
 int32_t yeti_csr_spmv(const yeti_csr *c, const double *vals_dev, const double *x_dev, double *y_dev, void *stream) {
     if (!c || !vals_dev || !x_dev || !y_dev) return set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
-    int64_t nrows = 3 * (int64_t)c->nb_cp;
-    hipLaunchKernelGGL(spmv_kernel, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, (hipStream_t)stream, nrows,
-                       c->rowptr.p, c->colind.p, vals_dev, x_dev, y_dev);
-    YETI_HIP(hipGetLastError());
-    return 0;
+    return csr_block_spmv(c, vals_dev, x_dev, nullptr, y_dev, nullptr, 0, nullptr, (hipStream_t)stream);
+}
+
+int32_t yeti_csr_spmv_abs(const yeti_csr *c, const double *vals_dev, const double *x_dev, double *y_dev, void *stream) {
+    if (!c || !vals_dev || !x_dev || !y_dev) return set_error(YETI_ERR_INVALID_ARGUMENT, "null argument");
+    return csr_block_spmv(c, vals_dev, x_dev, nullptr, y_dev, nullptr, 1, nullptr, (hipStream_t)stream);
 }
 
 int32_t yeti_csr_pattern_to_host(const yeti_csr *c, int64_t *rowptr_host, int32_t *colind_host) {

@@ -6,6 +6,7 @@
 // iga_mf_cg_3d (iga_wq_mf/fsrc/mf_iga_solver.f90:642-958) iterates alpha = rsold / (p . Ap).
 // Here the free-dof restriction is a 0/1 mask on the full vector: masked entries of x, r, z and
 // p stay exactly zero, which is the reduced system. Stopping rule: ||r||_2 <= rtol ||b||_2.
+#include <algorithm>
 #include <cmath>
 
 #include "common.hpp"
@@ -122,6 +123,27 @@ __global__ void __launch_bounds__(256) init_kernel(int64_t n, const double *__re
     }
 }
 
+// partial[b] = max over the block's entries of |v[i]| (masked entries count as 0)
+__global__ void __launch_bounds__(256) absmax_kernel(int64_t n, const double *__restrict__ v, const uint8_t *mask,
+                                                     double *__restrict__ partial) {
+    __shared__ double sh[4];
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (!mask || mask[i]) m = fmax(m, fabs(v[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) sh[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+__global__ void finalize_max_kernel(const double *partial, int n, double *out) {
+    if (threadIdx.x != 0) return;
+    double m = 0.0;
+    for (int i = 0; i < n; ++i) m = fmax(m, partial[i]);
+    *out = m;
+}
+
 // dinv[i] = mask[i] / A[i][i]
 __global__ void jacobi_kernel(int64_t nrows, const int64_t *rowptr, const int32_t *colind, const double *vals,
                               const uint8_t *mask, double *dinv) {
@@ -136,6 +158,107 @@ __global__ void jacobi_kernel(int64_t nrows, const int64_t *rowptr, const int32_
     dinv[row] = (mask[row] && d != 0.0) ? 1.0 / d : 0.0;
 }
 
+// Work vectors of one PCG solve: r, z, p, q, dinv (nrows each), two partial-sum arrays
+// (RED_BLOCKS each, or PCG_PARTS for the nodal-block product) and 8 scalars.
+struct PcgWork {
+    double *r, *z, *p, *q, *dinv, *part1, *part2, *sc;
+};
+constexpr int PCG_PARTS = 4096;
+
+// The operator of a PCG solve: a scalar CSR (rowptr, colind, vals), or a device CSR handle whose
+// nodal 3x3 blocks the product walks (blocked != null; rowptr / colind are then its scalar
+// pattern, used for the diagonal).
+struct PcgOp {
+    int64_t nrows;
+    const int64_t *rowptr;
+    const int32_t *colind;
+    const double *vals;
+    const yeti_csr *blocked;
+};
+
+// Stop test. norm 2: ||r||_2 <= rtol ||b||_2; norm 0: max|r| <= rtol max|b| over the free dofs,
+// the rule of iga_mf_cg_3d (maxval(abs(r)) / maxval(abs(b)), mf_iga_solver.f90:779,935).
+static int pcg_run(const PcgOp &op, const double *b, const uint8_t *free_mask, double *x, double rtol, int32_t maxit,
+                   int32_t check_every, int norm, int32_t *iters_out, double *relres_out, hipStream_t st,
+                   const PcgWork &w) {
+    const int64_t nrows = op.nrows;
+    const int64_t *rowptr = op.rowptr;
+    const int32_t *colind = op.colind;
+    const double *vals = op.vals;
+    if (check_every <= 0) check_every = 10;
+    const unsigned nb = (unsigned)std::min<int64_t>(RED_BLOCKS, (nrows + 255) / 256);
+    hipLaunchKernelGGL(jacobi_kernel, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, st, nrows, rowptr,
+                       colind, vals, free_mask, w.dinv);
+    hipLaunchKernelGGL(init_kernel, dim3(nb), dim3(256), 0, st, nrows, b, free_mask, w.dinv, x, w.r, w.z, w.p,
+                       w.part1, w.part2);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, w.part1, (int)nb, w.sc + 0);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, w.part2, (int)nb, w.sc + 3);
+    YETI_HIP(hipGetLastError());
+    double hs[8] = {0};
+    YETI_HIP(hipMemcpyAsync(hs, w.sc, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+    YETI_HIP(hipStreamSynchronize(st));
+    const double bb = hs[3];
+    if (bb == 0.0) {
+        if (iters_out) *iters_out = 0;
+        if (relres_out) *relres_out = 0.0;
+        return 0;
+    }
+    // norm 0: the reference's max|b| over the free dofs (r = mask b at the start)
+    double bmax = 0.0;
+    if (norm == 0) {
+        hipLaunchKernelGGL(absmax_kernel, dim3(nb), dim3(256), 0, st, nrows, (const double *)w.r, free_mask, w.part2);
+        hipLaunchKernelGGL(finalize_max_kernel, dim3(1), dim3(64), 0, st, (const double *)w.part2, (int)nb, w.sc + 6);
+        YETI_HIP(hipGetLastError());
+        YETI_HIP(hipMemcpyAsync(&bmax, w.sc + 6, sizeof(double), hipMemcpyDeviceToHost, st));
+        YETI_HIP(hipStreamSynchronize(st));
+    }
+    const unsigned nspmv = (unsigned)std::min<int64_t>(RED_BLOCKS, (nrows + 3) / 4);
+    int it = 0;
+    double rel = 1.0;
+    while (it < maxit) {
+        int nparts = (int)nspmv;
+        if (op.blocked) {
+            int rc = csr_block_spmv(op.blocked, vals, w.p, free_mask, w.q, w.part1, 2, &nparts, st);
+            if (rc) return rc;
+        } else {
+            hipLaunchKernelGGL(spmv_dot_kernel, dim3(nspmv), dim3(256), 0, st, nrows, rowptr, colind, vals, w.p,
+                               free_mask, w.q, w.part1);
+        }
+        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, w.part1, nparts, w.sc + 1);
+        hipLaunchKernelGGL(update_xr_kernel, dim3(nb), dim3(256), 0, st, nrows, w.sc, x, w.r, w.z, w.p, w.q, w.dinv,
+                           w.part1, w.part2);
+        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, w.part1, (int)nb, w.sc + 4);
+        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, w.part2, (int)nb, w.sc + 2);
+        hipLaunchKernelGGL(update_p_kernel, dim3(nb), dim3(256), 0, st, nrows, w.sc, w.p, w.z);
+        hipLaunchKernelGGL(roll_scalars_kernel, dim3(1), dim3(1), 0, st, w.sc);
+        YETI_HIP(hipGetLastError());
+        ++it;
+        if (it % check_every == 0 || it == maxit) {
+            double rr = 0.0;
+            if (norm == 0) {
+                hipLaunchKernelGGL(absmax_kernel, dim3(nb), dim3(256), 0, st, nrows, (const double *)w.r, free_mask,
+                                   w.part2);
+                hipLaunchKernelGGL(finalize_max_kernel, dim3(1), dim3(64), 0, st, (const double *)w.part2, (int)nb,
+                                   w.sc + 6);
+                YETI_HIP(hipGetLastError());
+                YETI_HIP(hipMemcpyAsync(&rr, w.sc + 6, sizeof(double), hipMemcpyDeviceToHost, st));
+                YETI_HIP(hipStreamSynchronize(st));
+                rel = rr / bmax;
+            } else {
+                YETI_HIP(hipMemcpyAsync(&rr, w.sc + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+                YETI_HIP(hipStreamSynchronize(st));
+                rel = std::sqrt(rr / bb);
+            }
+            if (!(rel > rtol)) break;     // also stops on NaN
+        }
+    }
+    if (iters_out) *iters_out = it;
+    if (relres_out) *relres_out = rel;
+    if (std::isnan(rel)) return set_error(YETI_ERR_NOT_CONVERGED, "pcg: residual is NaN");
+    if (rel > rtol) return set_error(YETI_ERR_NOT_CONVERGED, "pcg: %d iterations, relative residual %.3e > %.3e", it, rel, rtol);
+    return 0;
+}
+
 }  // namespace yeti
 
 using namespace yeti;
@@ -147,8 +270,6 @@ int32_t yeti_pcg_csr(int64_t nrows, const int64_t *rowptr, const int32_t *colind
                      int32_t check_every, int32_t *iters_out, double *relres_out, void *stream) {
     if (!rowptr || !colind || !vals || !b || !free_mask || !x || nrows <= 0 || maxit < 0)
         return set_error(YETI_ERR_INVALID_ARGUMENT, "pcg: bad argument");
-    if (check_every <= 0) check_every = 10;
-    hipStream_t st = (hipStream_t)stream;
     DevBuf<double> r, z, p, q, dinv, part1, part2, sc;
     YETI_HIP(r.alloc(nrows));
     YETI_HIP(z.alloc(nrows));
@@ -158,51 +279,30 @@ int32_t yeti_pcg_csr(int64_t nrows, const int64_t *rowptr, const int32_t *colind
     YETI_HIP(part1.alloc(RED_BLOCKS));
     YETI_HIP(part2.alloc(RED_BLOCKS));
     YETI_HIP(sc.alloc(8));
-    const unsigned nb = (unsigned)std::min<int64_t>(RED_BLOCKS, (nrows + 255) / 256);
-    hipLaunchKernelGGL(jacobi_kernel, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, st, nrows, rowptr,
-                       colind, vals, free_mask, dinv.p);
-    hipLaunchKernelGGL(init_kernel, dim3(nb), dim3(256), 0, st, nrows, b, free_mask, dinv.p, x, r.p, z.p, p.p,
-                       part1.p, part2.p);
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, part1.p, (int)nb, sc.p + 0);
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, part2.p, (int)nb, sc.p + 3);
-    YETI_HIP(hipGetLastError());
-    double hs[8] = {0};
-    YETI_HIP(hipMemcpyAsync(hs, sc.p, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
-    YETI_HIP(hipStreamSynchronize(st));
-    const double bb = hs[3];
-    if (bb == 0.0) {
-        if (iters_out) *iters_out = 0;
-        if (relres_out) *relres_out = 0.0;
-        return 0;
-    }
-    const unsigned nspmv = (unsigned)std::min<int64_t>(RED_BLOCKS, (nrows + 3) / 4);
-    int it = 0;
-    double rel = 1.0;
-    while (it < maxit) {
-        hipLaunchKernelGGL(spmv_dot_kernel, dim3(nspmv), dim3(256), 0, st, nrows, rowptr, colind, vals, p.p,
-                           free_mask, q.p, part1.p);
-        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, part1.p, (int)nspmv, sc.p + 1);
-        hipLaunchKernelGGL(update_xr_kernel, dim3(nb), dim3(256), 0, st, nrows, sc.p, x, r.p, z.p, p.p, q.p,
-                           dinv.p, part1.p, part2.p);
-        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, part1.p, (int)nb, sc.p + 4);
-        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, part2.p, (int)nb, sc.p + 2);
-        hipLaunchKernelGGL(update_p_kernel, dim3(nb), dim3(256), 0, st, nrows, sc.p, p.p, z.p);
-        hipLaunchKernelGGL(roll_scalars_kernel, dim3(1), dim3(1), 0, st, sc.p);
-        YETI_HIP(hipGetLastError());
-        ++it;
-        if (it % check_every == 0 || it == maxit) {
-            double rr = 0.0;
-            YETI_HIP(hipMemcpyAsync(&rr, sc.p + 2, sizeof(double), hipMemcpyDeviceToHost, st));
-            YETI_HIP(hipStreamSynchronize(st));
-            rel = std::sqrt(rr / bb);
-            if (!(rel > rtol)) break;     // also stops on NaN
-        }
-    }
-    if (iters_out) *iters_out = it;
-    if (relres_out) *relres_out = rel;
-    if (std::isnan(rel)) return set_error(YETI_ERR_NOT_CONVERGED, "pcg: residual is NaN");
-    if (rel > rtol) return set_error(YETI_ERR_NOT_CONVERGED, "pcg: %d iterations, relative residual %.3e > %.3e", it, rel, rtol);
-    return 0;
+    const PcgWork w{r.p, z.p, p.p, q.p, dinv.p, part1.p, part2.p, sc.p};
+    const PcgOp op{nrows, rowptr, colind, vals, nullptr};
+    return pcg_run(op, b, free_mask, x, rtol, maxit, check_every, 2, iters_out, relres_out, (hipStream_t)stream, w);
+}
+
+int32_t yeti_csr_pcg(const yeti_csr *c, const double *vals, const double *b, const uint8_t *free_mask, double *x,
+                     double rtol, int32_t maxit, int32_t check_every, int32_t norm, int32_t *iters_out,
+                     double *relres_out, void *stream) {
+    if (!c || !vals || !b || !free_mask || !x || maxit < 0 || (norm != 0 && norm != 2))
+        return set_error(YETI_ERR_INVALID_ARGUMENT, "pcg: bad argument");
+    const int64_t nrows = csr_nrows(c);
+    if (nrows <= 0) return set_error(YETI_ERR_INVALID_ARGUMENT, "pcg: empty matrix");
+    DevBuf<double> r, z, p, q, dinv, part1, part2, sc;
+    YETI_HIP(r.alloc(nrows));
+    YETI_HIP(z.alloc(nrows));
+    YETI_HIP(p.alloc(nrows));
+    YETI_HIP(q.alloc(nrows));
+    YETI_HIP(dinv.alloc(nrows));
+    YETI_HIP(part1.alloc(PCG_PARTS));
+    YETI_HIP(part2.alloc(PCG_PARTS));
+    YETI_HIP(sc.alloc(8));
+    const PcgWork w{r.p, z.p, p.p, q.p, dinv.p, part1.p, part2.p, sc.p};
+    const PcgOp op{nrows, csr_rowptr_dev(c), csr_colind_dev(c), vals, c};
+    return pcg_run(op, b, free_mask, x, rtol, maxit, check_every, norm, iters_out, relres_out, (hipStream_t)stream, w);
 }
 
 }  // extern "C"

@@ -87,18 +87,33 @@ class StiffnessCSR:
         return sp.csr_matrix((v, ci, rp), shape=(self.nrows, self.nrows))
 
 
-    def pcg(self, vals, b, free_mask, rtol=1e-10, maxit=100000, check_every=20, stream=None):
-        """Jacobi-PCG on A[free][:, free]; returns (x, iterations, relative residual)."""
+    def pcg(self, vals, b, free_mask, rtol=1e-10, maxit=100000, check_every=20, norm="2", stream=None,
+            raise_on_maxit=True):
+        """Jacobi-PCG on A[free][:, free]; returns (x, iterations, relative residual). norm "2"
+        stops at ||r||_2 <= rtol ||b||_2, norm "inf" at max|r| <= rtol max|b| (the stopping rule
+        of iga_mf_cg_3d, mf_iga_solver.f90:779,935). With raise_on_maxit False a solve that ends
+        at maxit returns its iterate instead of raising."""
+        if norm not in ("2", "inf"):
+            raise ValueError("norm must be '2' or 'inf'")
         x = torch.zeros(self.nrows, dtype=torch.float64, device=b.device)
         it = C.c_int32(0)
         rel = C.c_double(0.0)
-        rc = _lib.lib().yeti_pcg_csr(self.nrows, C.c_void_p(self.rowptr_ptr), C.c_void_p(self.colind_ptr),
-                                     C.c_void_p(vals.data_ptr()), C.c_void_p(b.data_ptr()),
+        rc = _lib.lib().yeti_csr_pcg(self._h, C.c_void_p(vals.data_ptr()), C.c_void_p(b.data_ptr()),
                                      C.c_void_p(free_mask.data_ptr()), C.c_void_p(x.data_ptr()), float(rtol),
-                                     int(maxit), int(check_every), C.byref(it), C.byref(rel),
-                                     _stream_handle(stream))
+                                     int(maxit), int(check_every), 2 if norm == "2" else 0, C.byref(it),
+                                     C.byref(rel), _stream_handle(stream))
+        if rc == _lib.YETI_ERR_NOT_CONVERGED and not raise_on_maxit:
+            return x, int(it.value), float(rel.value)
         _lib.check(rc)
         return x, int(it.value), float(rel.value)
+
+    def spmv_abs(self, vals, x, y=None, stream=None):
+        """y = |A| |x|: eps ||y|| bounds the FP64 rounding of A x (the residual floor)."""
+        if y is None:
+            y = torch.empty(self.nrows, dtype=torch.float64, device=x.device)
+        _lib.check(_lib.lib().yeti_csr_spmv_abs(self._h, C.c_void_p(vals.data_ptr()), C.c_void_p(x.data_ptr()),
+                                                C.c_void_p(y.data_ptr()), _stream_handle(stream)))
+        return y
 
 
 def free_mask_from_args(args_dict, device="cuda"):
