@@ -245,6 +245,16 @@ int32_t yeti_csr_pcg(const yeti_csr *c, const double *vals, const double *b, con
                      double rtol, int32_t maxit, int32_t check_every, int32_t norm, int32_t *iters, double *relres,
                      void *stream);
 
+/* Same solve as yeti_pcg_csr, allocating nothing: no counterpart in the reference. For callers
+ * that cannot allocate device memory (tight solve loops, harnesses that forbid hipMalloc). The
+ * work vectors live in a file-scope device array of the library (up to 1,048,576 rows, about
+ * 42 MB per GPU, held until the process ends); larger systems return YETI_ERR_SIZE_OVERFLOW.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (other threads or
+ * streams) race and must be serialised by the caller. */
+int32_t yeti_pcg_csr_noalloc(int64_t nrows, const int64_t *rowptr, const int32_t *colind, const double *vals,
+                             const double *b, const uint8_t *free_mask, double *x, double rtol, int32_t maxit,
+                             int32_t check_every, int32_t *iters, double *relres, void *stream);
+
 /* ---- matrix-free operator (sum-factorised, FP64 and FP32) --------------------------------
  * y = (K_side + K_side^T) x without forming K, for patches whose elements fill the tensor grid
  * of their knot spans. Generalises YETI's scalar matrix-free apply mf_iga_get_ku_3d

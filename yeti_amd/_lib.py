@@ -88,6 +88,9 @@ SIGNATURES = {
     "yeti_pcg_csr": (C.c_int32, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                  C.POINTER(C.c_double), C.c_void_p]),
+    "yeti_pcg_csr_noalloc": (C.c_int32, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_double), C.c_void_p]),
 }
 
 _lib = None

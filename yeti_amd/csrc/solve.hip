@@ -8,6 +8,7 @@
 // p stay exactly zero, which is the reduced system. Stopping rule: ||r||_2 <= rtol ||b||_2.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -259,6 +260,36 @@ static int pcg_run(const PcgOp &op, const double *b, const uint8_t *free_mask, d
     return 0;
 }
 
+// ---- the allocation-free variant's scratch ------------------------------------------------
+// One file-scope device array holds every work buffer of yeti_pcg_csr_noalloc: 8 scalars, two
+// partial-sum arrays and five vectors of up to PCG_NOALLOC_MAX_ROWS rows (about 42 MB, held on
+// each device the library runs on until the process ends). Host code never passes `scratch`
+// itself (in host code the name is a stand-in, not a device address): its address on each
+// device is looked up once with hipGetSymbolAddress, that device current, and kept here.
+constexpr int64_t PCG_NOALLOC_MAX_ROWS = int64_t(1) << 20;
+constexpr int64_t PCG_SCRATCH_DOUBLES = 8 + 2 * RED_BLOCKS + 5 * PCG_NOALLOC_MAX_ROWS;
+static __device__ __align__(256) double scratch[PCG_SCRATCH_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex scratch_mutex;
+static double *scratch_ptr[MAX_DEVICES] = {};
+
+static int scratch_on_current_device(double **out) {
+    int dev = 0;
+    YETI_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return set_error(YETI_ERR_DEVICE, "pcg: device id %d out of range", dev);
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(YETI_ERR_DEVICE, "pcg: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        scratch_ptr[dev] = (double *)ptr;
+    }
+    *out = scratch_ptr[dev];
+    return 0;
+}
+
 }  // namespace yeti
 
 using namespace yeti;
@@ -303,6 +334,30 @@ int32_t yeti_csr_pcg(const yeti_csr *c, const double *vals, const double *b, con
     const PcgWork w{r.p, z.p, p.p, q.p, dinv.p, part1.p, part2.p, sc.p};
     const PcgOp op{nrows, csr_rowptr_dev(c), csr_colind_dev(c), vals, c};
     return pcg_run(op, b, free_mask, x, rtol, maxit, check_every, norm, iters_out, relres_out, (hipStream_t)stream, w);
+}
+
+int32_t yeti_pcg_csr_noalloc(int64_t nrows, const int64_t *rowptr, const int32_t *colind, const double *vals,
+                             const double *b, const uint8_t *free_mask, double *x, double rtol, int32_t maxit,
+                             int32_t check_every, int32_t *iters_out, double *relres_out, void *stream) {
+    if (!rowptr || !colind || !vals || !b || !free_mask || !x || nrows <= 0 || maxit < 0)
+        return set_error(YETI_ERR_INVALID_ARGUMENT, "pcg: bad argument");
+    if (nrows > PCG_NOALLOC_MAX_ROWS)
+        return set_error(YETI_ERR_SIZE_OVERFLOW, "pcg (no allocation): %lld rows exceed the scratch maximum of %lld",
+                         (long long)nrows, (long long)PCG_NOALLOC_MAX_ROWS);
+    double *s = nullptr;
+    if (int rc = scratch_on_current_device(&s)) return rc;
+    // every entry the solve reads is written by it first (scalars by the reductions, vectors by
+    // the Jacobi and start kernels), so nothing left by an earlier call needs clearing
+    const PcgWork w{s + 8 + 2 * RED_BLOCKS,
+                    s + 8 + 2 * RED_BLOCKS + nrows,
+                    s + 8 + 2 * RED_BLOCKS + 2 * nrows,
+                    s + 8 + 2 * RED_BLOCKS + 3 * nrows,
+                    s + 8 + 2 * RED_BLOCKS + 4 * nrows,
+                    s + 8,
+                    s + 8 + RED_BLOCKS,
+                    s};
+    const PcgOp op{nrows, rowptr, colind, vals, nullptr};
+    return pcg_run(op, b, free_mask, x, rtol, maxit, check_every, 2, iters_out, relres_out, (hipStream_t)stream, w);
 }
 
 }  // extern "C"
