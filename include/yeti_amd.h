@@ -278,9 +278,13 @@ int32_t yeti_mf_update_coords(yeti_mf *m, const double *coords_dev, void *stream
 void    yeti_mf_destroy(yeti_mf *m);
 
 /* FP64 PCG with the FP64 matrix-free apply and a 3x3 nodal block-Jacobi preconditioner on the
- * free dofs (the CG loop of iga_mf_cg_3d, mf_iga_solver.f90:642-958). x = 0 on fixed dofs. */
+ * free dofs (the CG loop of iga_mf_cg_3d, mf_iga_solver.f90:642-958). x = 0 on fixed dofs.
+ * norm 2: stop at ||r||_2 <= rtol ||b||_2; norm 0: stop at max|r| <= rtol max|b| over the free
+ * dofs, iga_mf_cg_3d's own rule (maxval(abs(r)) / maxval(abs(b)), mf_iga_solver.f90:779,935);
+ * relres is reported in that norm. */
 int32_t yeti_mf_pcg_f64(const yeti_mf *m, const double *b, const uint8_t *free_mask, double *x, double rtol,
-                        int32_t maxit, int32_t check_every, int32_t *iters, double *relres, void *stream);
+                        int32_t maxit, int32_t check_every, int32_t norm, int32_t *iters, double *relres,
+                        void *stream);
 /* FP64 iterative refinement with FP32 inner PCG (FP32 apply + FP32 block-Jacobi, FP64 dots):
  * stops when ||b - K x||_2 <= rtol ||b||_2 over the free dofs. */
 int32_t yeti_mf_solve_ir(const yeti_mf *m, const double *b, const uint8_t *free_mask, double *x, double rtol,

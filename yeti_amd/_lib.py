@@ -62,7 +62,7 @@ SIGNATURES = {
     "yeti_mf_update_coords": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_mf_destroy": (None, [C.c_void_p]),
     "yeti_mf_pcg_f64": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int32,
-                                    C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p]),
+                                    C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p]),
     "yeti_mf_solve_ir": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                      C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                      C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p]),

@@ -175,17 +175,52 @@ __global__ void x_update_kernel(int64_t n, double *x, const float *d) {
         x[i] += (double)d[i];
 }
 
+// partial[b] = max |v[i]| over the block's masked-in entries; out = max of the partials
+template <typename T>
+__global__ void __launch_bounds__(256) absmax_partial_kernel(int64_t n, const T *__restrict__ v, const uint8_t *mask,
+                                                             double *__restrict__ partial) {
+    __shared__ double sh[4];
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (mask[i]) m = fmax(m, fabs((double)v[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) sh[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+__global__ void max_partials_kernel(const double *part, int n, double *out) {
+    if (threadIdx.x != 0) return;
+    double m = 0.0;
+    for (int i = 0; i < n; ++i) m = fmax(m, part[i]);
+    *out = m;
+}
+
+template <typename T>
+static int absmax_host(int64_t n, const T *v, const uint8_t *mask, double *part, double *slot, double *out,
+                       hipStream_t st) {
+    const unsigned nb = (unsigned)std::min<int64_t>(NRED, (n + 255) / 256);
+    hipLaunchKernelGGL(absmax_partial_kernel<T>, dim3(nb), dim3(256), 0, st, n, v, mask, part);
+    hipLaunchKernelGGL(max_partials_kernel, dim3(1), dim3(64), 0, st, (const double *)part, (int)nb, slot);
+    YETI_HIP(hipGetLastError());
+    YETI_HIP(hipMemcpyAsync(out, slot, sizeof(double), hipMemcpyDeviceToHost, st));
+    YETI_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
 template <typename T>
 static int apply_op(const yeti_mf *m, const T *x, T *y, hipStream_t st) {
     if constexpr (sizeof(T) == 8) return yeti_mf_apply_f64(m, (const double *)x, (double *)y, st);
     else return yeti_mf_apply_f32(m, (const float *)x, (float *)y, st);
 }
 
-// PCG on the masked system from x = 0. r must hold mask*b on entry; returns iterations.
+// PCG on the masked system from x = 0. r must hold mask*b on entry; returns iterations. Stops
+// when r.r <= rtol_abs2, or, with inf_tol > 0, when max|r| <= inf_tol (then *rr_out is max|r|).
 template <typename T>
 static int pcg(const yeti_mf *m, const T *inv, const uint8_t *mask, T *x, T *r, double rtol_abs2, int maxit,
                int check_every, int *iters, double *rr_out, hipStream_t st, DevBuf<T> &z, DevBuf<T> &p,
-               DevBuf<T> &q, DevBuf<double> &part, DevBuf<double> &sc) {
+               DevBuf<T> &q, DevBuf<double> &part, DevBuf<double> &sc, double inf_tol = 0.0) {
     const int32_t ncp = yeti_mf_nb_cp(m);
     const int64_t n = 3 * (int64_t)ncp;
     const unsigned nb = (unsigned)std::min<int64_t>(NRED, (n + 255) / 256);
@@ -211,9 +246,15 @@ static int pcg(const yeti_mf *m, const T *inv, const uint8_t *mask, T *x, T *r, 
         YETI_HIP(hipGetLastError());
         ++it;
         if (it % check_every == 0 || it == maxit) {
-            YETI_HIP(hipMemcpyAsync(&rr, sc.p + 2, sizeof(double), hipMemcpyDeviceToHost, st));
-            YETI_HIP(hipStreamSynchronize(st));
-            if (!(rr > rtol_abs2)) break;
+            if (inf_tol > 0.0) {
+                int rc2 = absmax_host<T>(n, r, mask, part.p, sc.p + 6, &rr, st);
+                if (rc2) return rc2;
+                if (!(rr > inf_tol)) break;
+            } else {
+                YETI_HIP(hipMemcpyAsync(&rr, sc.p + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+                YETI_HIP(hipStreamSynchronize(st));
+                if (!(rr > rtol_abs2)) break;
+            }
         }
     }
     *iters = it;
@@ -246,8 +287,10 @@ using namespace yeti;
 extern "C" {
 
 int32_t yeti_mf_pcg_f64(const yeti_mf *m, const double *b, const uint8_t *free_mask, double *x, double rtol,
-                        int32_t maxit, int32_t check_every, int32_t *iters, double *relres, void *stream) {
-    if (!m || !b || !free_mask || !x || maxit < 0) return set_error(YETI_ERR_INVALID_ARGUMENT, "mf pcg: bad argument");
+                        int32_t maxit, int32_t check_every, int32_t norm, int32_t *iters, double *relres,
+                        void *stream) {
+    if (!m || !b || !free_mask || !x || maxit < 0 || (norm != 0 && norm != 2))
+        return set_error(YETI_ERR_INVALID_ARGUMENT, "mf pcg: bad argument");
     if (check_every <= 0) check_every = 10;
     hipStream_t st = (hipStream_t)stream;
     const int64_t n = 3 * (int64_t)yeti_mf_nb_cp(m);
@@ -268,15 +311,19 @@ int32_t yeti_mf_pcg_f64(const yeti_mf *m, const double *b, const uint8_t *free_m
     YETI_HIP(hipMemcpyAsync(&bb, sc.p + 3, sizeof(double), hipMemcpyDeviceToHost, st));
     YETI_HIP(hipStreamSynchronize(st));
     int it = 0;
-    double rr = 0.0;
+    double rr = 0.0, bmax = 0.0;
+    if (bb > 0.0 && norm == 0) {   // the reference's max|b| (r = mask b here)
+        rc = absmax_host<double>(n, r.p, free_mask, part.p, sc.p + 6, &bmax, st);
+        if (rc) return rc;
+    }
     if (bb > 0.0) {
         rc = pcg<double>(m, S.inv64.p, free_mask, x, r.p, rtol * rtol * bb, maxit, check_every, &it, &rr, st, z, p, q,
-                         part, sc);
+                         part, sc, norm == 0 ? rtol * bmax : 0.0);
         if (rc) return rc;
     } else {
         hipLaunchKernelGGL(zero_kernel<double>, dim3(nb), dim3(256), 0, st, n, x);
     }
-    double rel = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+    double rel = bb > 0.0 ? (norm == 0 ? rr / bmax : std::sqrt(rr / bb)) : 0.0;
     if (iters) *iters = it;
     if (relres) *relres = rel;
     if (!(rel <= rtol)) return set_error(YETI_ERR_NOT_CONVERGED, "mf pcg: %d iterations, relative residual %.3e", it, rel);

@@ -79,12 +79,18 @@ class MatrixFree:
             raise ValueError("coords must be a contiguous CUDA float64 tensor of %d entries" % self.ndof)
         _lib.check(_lib.lib().yeti_mf_update_coords(self._h, C.c_void_p(c.data_ptr()), _stream_handle(stream)))
 
-    def pcg_f64(self, b, free_mask, rtol=1e-10, maxit=200000, check_every=20, stream=None):
+    def pcg_f64(self, b, free_mask, rtol=1e-10, maxit=200000, check_every=20, norm="2", stream=None):
+        """FP64 PCG (block-Jacobi) on the free dofs: (x, iterations, relative residual). norm "2"
+        stops at ||r||_2 <= rtol ||b||_2, norm "inf" at max|r| <= rtol max|b|, the rule of
+        iga_mf_cg_3d (mf_iga_solver.f90:779,935)."""
+        if norm not in ("2", "inf"):
+            raise ValueError("norm must be '2' or 'inf'")
         x = torch.empty(self.ndof, dtype=torch.float64, device=b.device)
         it, rel = C.c_int32(0), C.c_double(0.0)
         _lib.check(_lib.lib().yeti_mf_pcg_f64(self._h, C.c_void_p(b.data_ptr()), C.c_void_p(free_mask.data_ptr()),
                                               C.c_void_p(x.data_ptr()), float(rtol), int(maxit), int(check_every),
-                                              C.byref(it), C.byref(rel), _stream_handle(stream)))
+                                              2 if norm == "2" else 0, C.byref(it), C.byref(rel),
+                                              _stream_handle(stream)))
         return x, int(it.value), float(rel.value)
 
     def solve_ir(self, b, free_mask, rtol=1e-10, inner_rtol=1e-4, max_outer=50, max_inner=100000,
