@@ -335,6 +335,12 @@ typedef struct yeti_ir_status {
 int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask_dev, int32_t precision, int32_t precond, double rtol,
                        double delta, int32_t chunk, int32_t max_iters, int32_t max_updates, void *arena,
                        size_t arena_bytes, void *stream, yeti_ir **out, size_t *used);
+/* Make the session's FP64 products (every product of a precision-64 session, and the residual
+ * replacement of either precision) go through an assembled device CSR of K + K^T (values vals,
+ * which must stay valid) instead of the matrix-free operator: the solve is then the FP64 solve of
+ * the assembled matrix. c == NULL restores the matrix-free operator. Call before the first step
+ * of a graph-stepped session. */
+int32_t yeti_ir_set_csr(yeti_ir *s, const yeti_csr *c, const double *vals);
 /* x = 0 and the first direction set up for right-hand side b (device buffers kept by the session). */
 int32_t yeti_ir_begin(yeti_ir *s, const double *b_dev, double *x_dev, void *stream);
 int32_t yeti_ir_step(yeti_ir *s, int32_t use_graph, void *stream);

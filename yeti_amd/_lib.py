@@ -73,6 +73,7 @@ SIGNATURES = {
     "yeti_ir_create": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
                                    C.c_int32, C.c_int32, C.c_void_p, C.c_size_t, C.c_void_p, C.POINTER(C.c_void_p),
                                    C.POINTER(C.c_size_t)]),
+    "yeti_ir_set_csr": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_ir_begin": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_ir_step": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p]),
     "yeti_ir_status_async": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -19,8 +19,9 @@ int set_error(int code, const char *fmt, ...);
 
 // Nodal-block SpMV of a device CSR (csr.hip): y = mask .* (A x) (mode 0), with |A| |x| (mode 1),
 // or mode 2: also partial[b] = grid block b's share of x . y, *nparts blocks (<= 4096).
+// With pred non-null the launch does nothing while *pred == 0 (device-side solver control).
 int csr_block_spmv(const yeti_csr *c, const double *vals, const double *x, const uint8_t *mask, double *y,
-                   double *partial, int mode, int *nparts, hipStream_t st);
+                   double *partial, int mode, int *nparts, hipStream_t st, const int32_t *pred = nullptr);
 int64_t csr_nrows(const yeti_csr *c);
 const int64_t *csr_rowptr_dev(const yeti_csr *c);
 const int32_t *csr_colind_dev(const yeti_csr *c);

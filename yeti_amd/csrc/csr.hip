@@ -346,7 +346,9 @@ __global__ void __launch_bounds__(256) block_spmv_kernel(int32_t nb_cp, const in
                                                          const double *__restrict__ vals,
                                                          const double *__restrict__ x,
                                                          const uint8_t *__restrict__ mask,
-                                                         double *__restrict__ y, double *__restrict__ partial) {
+                                                         double *__restrict__ y, double *__restrict__ partial,
+                                                         const int32_t *__restrict__ pred) {
+    if (pred && *pred == 0) return;   // predicated launch (device-side solver control)
     constexpr bool ABS = (MODE & 1) != 0, DOT = (MODE & 2) != 0;
     __shared__ double sh[4];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -400,17 +402,17 @@ constexpr int SPMV_MAX_BLOCKS = 4096;
 // partial[b] = the grid block's share of x . y (grid of *nparts blocks, at most
 // SPMV_MAX_BLOCKS). mask may be null.
 int csr_block_spmv(const yeti_csr *c, const double *vals, const double *x, const uint8_t *mask, double *y,
-                   double *partial, int mode, int *nparts, hipStream_t st) {
+                   double *partial, int mode, int *nparts, hipStream_t st, const int32_t *pred) {
     if (c->nb_cp <= 0) return 0;
     const unsigned g = (unsigned)std::min<int64_t>(SPMV_MAX_BLOCKS, ((int64_t)c->nb_cp + 3) / 4);
     if (nparts) *nparts = (int)g;
     switch (mode) {
     case 0: hipLaunchKernelGGL(block_spmv_kernel<0>, dim3(g), dim3(256), 0, st, c->nb_cp, c->bptr.p, c->bcol.p, vals, x,
-                               mask, y, partial); break;
+                               mask, y, partial, pred); break;
     case 1: hipLaunchKernelGGL(block_spmv_kernel<1>, dim3(g), dim3(256), 0, st, c->nb_cp, c->bptr.p, c->bcol.p, vals, x,
-                               mask, y, partial); break;
+                               mask, y, partial, pred); break;
     default: hipLaunchKernelGGL(block_spmv_kernel<2>, dim3(g), dim3(256), 0, st, c->nb_cp, c->bptr.p, c->bcol.p, vals,
-                                x, mask, y, partial); break;
+                                x, mask, y, partial, pred); break;
     }
     YETI_HIP(hipGetLastError());
     return 0;

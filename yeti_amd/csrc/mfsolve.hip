@@ -432,6 +432,10 @@ int fdm_apply(const yeti_fdm *f, const float *r, const uint8_t *mask, float *z, 
 // iterations after which a residual replacement is forced (so a solve at the FP64 rounding
 // floor is recognised within a few such periods)
 constexpr int32_t IR_FORCE_EVERY = 256;
+// after a replacement that did not halve the true residual, the next one is forced this soon:
+// the solve is at its FP64 floor, and stopping there should not cost IR_FORCE_EVERY iterations
+// per stagnant replacement
+constexpr int32_t IR_FORCE_STAGNANT = 32;
 
 __global__ void pred_sum_partials_kernel(const double *part, int n, double *out, const int32_t *pred) {
     if (pred && *pred == 0) return;
@@ -504,9 +508,12 @@ __global__ void __launch_bounds__(256) pred_xr_kernel(int64_t n, double *sc, T *
     if (last_block_total(block_sum256(acc, sh), part, flags + F_COUNTER, sh, rr) && threadIdx.x == 0) {
         sc[S_RR] = rr;
         if (rr > sc[S_RRMAX]) sc[S_RRMAX] = rr;
-        if (!(rr > delta2 * sc[S_RRMAX]) || !(rr > sc[S_TOL2]) ||
-            flags[F_ITERS] + 1 - (int32_t)sc[S_LASTUPD] >= IR_FORCE_EVERY)
-            flags[F_UPDATE] = 1;
+        const int32_t since = flags[F_ITERS] + 1 - (int32_t)sc[S_LASTUPD];
+        if (!(rr > delta2 * sc[S_RRMAX]) || !(rr > sc[S_TOL2])) {
+            if (flags[F_UPDATE] == 0) flags[F_UPDATE] = 1;
+        } else if (since >= (flags[F_STAGNANT] && flags[F_DEPARTED] ? IR_FORCE_STAGNANT : IR_FORCE_EVERY)) {
+            if (flags[F_UPDATE] == 0) flags[F_UPDATE] = 2;
+        }
     }
 }
 
@@ -547,6 +554,10 @@ __global__ void __launch_bounds__(256) ir_true_residual_kernel(int64_t n, const 
     }
     double rrt;
     if (!last_block_total(block_sum256(acc, sh), part, flags + F_COUNTER, sh, rrt) || threadIdx.x != 0) return;
+    const bool forced = flags[F_UPDATE] == 2;
+    // the iterated residual has left the true one behind (by more than 4x in norm): the mark of
+    // a rounding floor, as opposed to a solve that is merely converging slowly
+    const bool departed = rrt > 16.0 * sc[S_RR];
     sc[S_RRT] = rrt;
     flags[F_UPDATE] = 0;
     flags[F_UPDATES] += 1;
@@ -558,18 +569,52 @@ __global__ void __launch_bounds__(256) ir_true_residual_kernel(int64_t n, const 
         flags[F_ACTIVE] = 0;
         return;
     }
+    flags[F_DEPARTED] = departed ? 1 : 0;
     if (!(rrt < 0.25 * sc[S_RRT_PREV])) flags[F_STAGNANT] += 1;
     else flags[F_STAGNANT] = 0;
-    sc[S_RRT_PREV] = rrt < sc[S_RRT_PREV] ? rrt : sc[S_RRT_PREV];
-    if (flags[F_STAGNANT] >= max_stagnant) {   // FP64 rounding floor above rtol: stop, flagged
-        flags[F_DONE] = 1;
-        flags[F_FLOOR] = 1;
-        flags[F_ACTIVE] = 0;
-    } else if (flags[F_UPDATES] >= max_updates) {
-        flags[F_DONE] = 1;
-        flags[F_FAILED] = 1;
-        flags[F_ACTIVE] = 0;
+    // far above the best residual so far: the iterate is drifting away at the rounding floor
+    const bool diverging = rrt > 1e4 * sc[S_RRT_PREV];
+    // S_RRT_PREV is the lowest true residual so far: save this iterate when it is the new best
+    if (rrt < sc[S_RRT_PREV]) {
+        sc[S_RRT_PREV] = rrt;
+        flags[F_BEST] = 1;
     }
+    // a forced replacement whose true residual has departed from the iterated one: the old
+    // direction carries no information about the replaced residual, so the next direction starts
+    // afresh (beta = 0 through an infinite previous r.z)
+    if (forced && departed) sc[S_RZ] = __builtin_inf();
+    bool stop = false;
+    if (flags[F_STAGNANT] >= max_stagnant || diverging) {   // FP64 rounding floor above rtol: stop, flagged
+        flags[F_FLOOR] = 1;
+        stop = true;
+    } else if (flags[F_UPDATES] >= max_updates) {
+        flags[F_FAILED] = 1;
+        stop = true;
+    }
+    if (stop) {
+        flags[F_DONE] = 1;
+        flags[F_ACTIVE] = 0;
+        if (sc[S_RRT_PREV] < rrt) {   // return the best iterate, and report its residual
+            flags[F_RESTORE] = 1;
+            sc[S_RRT] = sc[S_RRT_PREV];
+        }
+    }
+}
+
+// After a replacement: keep the iterate as the best one (F_BEST), or put the best one back at
+// the end of a solve that stopped above its best residual (F_RESTORE).
+__global__ void __launch_bounds__(256) ir_best_kernel(int64_t n, double *x, double *xbest, const int32_t *flags) {
+    const int32_t best = flags[F_BEST], restore = flags[F_RESTORE];
+    if (!best && !restore) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (best) xbest[i] = x[i];
+        else x[i] = xbest[i];
+    }
+}
+
+__global__ void ir_best_clear_kernel(int32_t *flags) {
+    flags[F_BEST] = 0;
+    flags[F_RESTORE] = 0;
 }
 
 template <typename T>
@@ -621,12 +666,15 @@ struct yeti_ir {
     double rtol = 0, delta = 0;
     int32_t chunk = 8, max_iters = 0, max_updates = 0;
     unsigned nb = 1, nbc = 1;
-    yeti::DevBuf<double> blk, inv64, y64, part, sc;
+    yeti::DevBuf<double> blk, inv64, y64, part, sc, xbest;
     yeti::DevBuf<float> inv32;
     yeti::DevBuf<double> vr64, vy64, vz64, vp64, vq64;   // iteration vectors, FP64
     yeti::DevBuf<float> vr32, vy32, vz32, vp32, vq32;    // iteration vectors, FP32
     yeti::DevBuf<int32_t> flags;
     yeti_fdm *fdm = nullptr;  // fast diagonalisation (else 3x3 block-Jacobi)
+    // FP64 operator: the matrix-free apply, or an assembled device CSR (yeti_ir_set_csr)
+    const yeti_csr *csr = nullptr;
+    const double *csr_vals = nullptr;
     const double *b = nullptr;
     double *x = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -641,6 +689,16 @@ struct yeti_ir {
 namespace yeti {
 
 constexpr int32_t IR_MAX_STAGNANT = 4;
+
+// The session's products: FP64 ones through the assembled CSR when one is set, FP32 ones always
+// matrix-free.
+static int ir_apply(yeti_ir *s, const double *x, double *y, hipStream_t st, const int32_t *pred) {
+    if (s->csr) return csr_block_spmv(s->csr, s->csr_vals, x, nullptr, y, nullptr, 0, nullptr, st, pred);
+    return mf_apply_pred(s->m, x, y, st, pred);
+}
+static int ir_apply(yeti_ir *s, const float *x, float *y, hipStream_t st, const int32_t *pred) {
+    return mf_apply_pred(s->m, x, y, st, pred);
+}
 
 // z = M^-1 r; r.z goes to rz_tail in the same launch sequence (mode 0: first direction, 1: end
 // of an iteration), predicated
@@ -669,7 +727,7 @@ static int ir_step_launches(yeti_ir *s, T *r, T *y, T *z, T *p, T *q, const T *i
     const int32_t *upd = s->flags.p + F_UPDATE;
     const double delta2 = s->delta * s->delta;
     for (int k = 0; k < s->chunk; ++k) {
-        int rc = mf_apply_pred(s->m, p, q, st, act);
+        int rc = ir_apply(s, p, q, st, act);
         if (rc) return rc;
         hipLaunchKernelGGL(pred_mask_dot_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->mask, p, q, s->part.p,
                            s->sc.p, s->flags.p);
@@ -678,10 +736,13 @@ static int ir_step_launches(yeti_ir *s, T *r, T *y, T *z, T *p, T *q, const T *i
         if (k == s->chunk - 1) {
             // residual replacement (predicated on F_UPDATE)
             hipLaunchKernelGGL(ir_accumulate_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->x, y, s->flags.p);
-            rc = mf_apply_pred(s->m, (const double *)s->x, s->y64.p, st, upd);
+            rc = ir_apply(s, (const double *)s->x, s->y64.p, st, upd);
             if (rc) return rc;
             hipLaunchKernelGGL(ir_true_residual_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->b, s->y64.p, s->mask,
                                r, s->part.p, s->sc.p, s->flags.p, s->rtol * s->rtol, s->max_updates, IR_MAX_STAGNANT);
+            hipLaunchKernelGGL(ir_best_kernel, dim3(s->nb), dim3(256), 0, st, s->n, s->x, s->xbest.p,
+                               (const int32_t *)s->flags.p);
+            hipLaunchKernelGGL(ir_best_clear_kernel, dim3(1), dim3(1), 0, st, s->flags.p);
         }
         // preconditioner and direction
         rc = ir_precond<T>(s, inv, r, z, 1, st, act);
@@ -751,6 +812,7 @@ int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask, int32_t preci
     const size_t n = (size_t)s->n;
     hipError_t e = s->blk.alloc(9 * (size_t)s->ncp);
     if (e == hipSuccess) e = s->y64.alloc(n);
+    if (e == hipSuccess) e = s->xbest.alloc(n);
     if (e == hipSuccess) e = s->part.alloc(NRED);
     if (e == hipSuccess) e = s->sc.alloc(S_NSC);
     if (e == hipSuccess) e = s->flags.alloc(F_ALLOC);
@@ -791,6 +853,16 @@ int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask, int32_t preci
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
         return fail(set_error(YETI_ERR_DEVICE, "ir: preconditioner set-up failed"));
     *out = s;
+    return 0;
+}
+
+int32_t yeti_ir_set_csr(yeti_ir *s, const yeti_csr *c, const double *vals) {
+    if (!s || (c && !vals)) return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: null argument");
+    if (s->gexec) return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: set the operator before the first graph step");
+    if (c && csr_nrows(c) != s->n) return set_error(YETI_ERR_INVALID_ARGUMENT, "ir: CSR has %lld rows, session %lld",
+                                                    (long long)csr_nrows(c), (long long)s->n);
+    s->csr = c;
+    s->csr_vals = c ? vals : nullptr;
     return 0;
 }
 

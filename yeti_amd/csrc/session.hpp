@@ -12,10 +12,13 @@ enum : int {
     S_RZ = 0, S_PQ, S_RR, S_RZN, S_BB, S_RRT, S_RRMAX, S_TOL2, S_RRT_PREV, S_LASTUPD, S_BETA, S_NSC = 16
 };
 // flags[] slots (int32), in the order of yeti_ir_status; F_COUNTER (not reported) is the
-// arrival counter of the grid reductions
+// arrival counter of the grid reductions; F_BEST / F_RESTORE (not reported) ask the step to save
+// the iterate as the best one so far / to put the best one back; F_DEPARTED (not reported): at the
+// last replacement the true residual stood far above the iterated one. F_UPDATE is 1 for a replacement
+// the residual's fall (or the tolerance) called for, 2 for a forced one.
 enum : int {
     F_ACTIVE = 0, F_UPDATE, F_STAGNANT, F_DONE, F_FAILED, F_ITERS, F_FLOOR, F_UPDATES, F_NFLAGS = 8,
-    F_COUNTER = 8, F_ALLOC = 16
+    F_COUNTER = 8, F_BEST = 9, F_RESTORE = 10, F_DEPARTED = 11, F_ALLOC = 16
 };
 
 // Sum of v over a 256-thread block, returned in thread 0.

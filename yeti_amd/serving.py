@@ -98,7 +98,7 @@ def ir_session_bytes(ncp_dims, precision, precond="fdm"):
     nb_cp = int(np.prod(ncp_dims))
     n = 3 * nb_cp
     w = 8 if precision == 64 else 4
-    sizes = [9 * nb_cp * 8, n * 8, NRED * 8, 16 * 8, 8 * 4, 9 * nb_cp * w] + [n * w] * 5
+    sizes = [9 * nb_cp * 8, n * 8, n * 8, NRED * 8, 16 * 8, 8 * 4, 9 * nb_cp * w] + [n * w] * 5
     total = sum(_align(s) for s in sizes)
     if precond == "fdm":
         al = lambda b: _align(b) + ALIGN   # noqa: E731
@@ -164,6 +164,57 @@ def latency_summary(lat):
         k = max(0, min(a.size - 1, int(math.ceil(q / 100.0 * a.size)) - 1))
         return float(a[k])
     return {"p50": pct(50), "p99": pct(99), "mean": float(a.mean()), "max": float(a[-1])}
+
+
+def solve_session(mf, F, mask, precision=64, precond="fdm", rtol=1e-10, delta=0.1, chunk=32, max_iters=100000,
+                  max_updates=2000, csr=None, csr_vals=None, step_log=None, step_every=0):
+    """One graph-stepped solve session (the serving loop's solver) on a resident MatrixFree
+    operator ``mf``: K[free][:, free] x = F[free] to rtol on the FP64 residual. precision 64 or 32
+    (FP32 vectors, operator and preconditioner with FP64 residual replacement). With ``csr`` (a
+    StiffnessCSR) and its values, the FP64 products use the assembled matrix: the FP64 solve of
+    the assembled system. Returns (x, status dict)."""
+    import torch
+    L = _lib.lib()
+    n = int(F.numel())
+    ncp = n // 3
+    used = C.c_size_t(0)
+    h = C.c_void_p()
+    mask_p = C.c_void_p(mask.data_ptr())
+    _lib.check(L.yeti_ir_create(mf._h, mask_p, int(precision), 1 if precond == "fdm" else 0, float(rtol), float(delta),
+                                int(chunk), int(max_iters), int(max_updates), None, 0, None, C.byref(h),
+                                C.byref(used)))
+    arena = torch.empty(int(used.value) + (1 << 20), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.Stream()
+    sh = C.c_void_p(stream.cuda_stream)
+    torch.cuda.synchronize()
+    _lib.check(L.yeti_ir_create(mf._h, mask_p, int(precision), 1 if precond == "fdm" else 0, float(rtol), float(delta),
+                                int(chunk), int(max_iters), int(max_updates), C.c_void_p(arena.data_ptr()),
+                                int(arena.numel()), sh, C.byref(h), C.byref(used)))
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    status = torch.empty(STATUS_BYTES, dtype=torch.uint8, pin_memory=True)
+    try:
+        if csr is not None:
+            _lib.check(L.yeti_ir_set_csr(h, csr._h, C.c_void_p(csr_vals.data_ptr())))
+        _lib.check(L.yeti_ir_begin(h, C.c_void_p(F.data_ptr()), C.c_void_p(x.data_ptr()), sh))
+        steps = 0
+        while True:
+            _lib.check(L.yeti_ir_step(h, 1, sh))
+            _lib.check(L.yeti_ir_status_async(h, C.c_void_p(status.data_ptr()), sh))
+            stream.synchronize()
+            steps += 1
+            buf = status.numpy()
+            fl = np.frombuffer(buf[:32].tobytes(), dtype=np.int32)
+            sc = np.frombuffer(buf[32:].tobytes(), dtype=np.float64)
+            st = {"done": int(fl[3]), "failed": int(fl[4]), "floor": int(fl[6]), "iters": int(fl[5]),
+                  "updates": int(fl[7]), "relres": math.sqrt(sc[5] / sc[4]) if sc[4] > 0 else 0.0}
+            if step_log and step_every and steps % step_every == 0:
+                step_log(st)
+            if st["done"]:
+                return x, st
+    finally:
+        stream.synchronize()
+        L.yeti_ir_destroy(h)
+        del arena
 
 
 # ------------------------------------------------------------------------------------------
