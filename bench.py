@@ -44,45 +44,41 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(nel, p, prefix, runs, threads):
+def cpu_baseline(nel, p, slab, runs, threads):
     """YETI's sys_linmat_lindef_static_omp (compiled reference, oracle/_ref) on this host's cores:
-    the first ``prefix`` elements of the benchmark patch itself (activeElement marks them; the
-    routine's slot of element e lies below e * nb_data_elem, so the COO arrays hold only the
-    prefix's slots) and, in full, the 16x4x4 p=2 cantilever. Each figure is the median of
-    ``runs`` calls after one warm-up, output allocation included. Falls back to the C restatement
-    (oracle/liboracle.so, packed layout) when the reference library is absent."""
+    an nel x nel x slab patch of the benchmark's generator (the same element type, degree,
+    quadrature and jitter as the benchmark patch, so the same work per element; a prefix of the
+    benchmark patch itself would leave all but one thread idle, as the routine's
+    `!$omp parallel do` hands out contiguous element ranges) and, in full, the 16x4x4 p=2
+    cantilever. Each figure is the median of ``runs`` calls after one warm-up, output allocation
+    included. Falls back to the C restatement (oracle/liboracle.so) when the reference library is
+    absent."""
     import numpy as np
     from oracle import oracle as O
     from yeti_amd.geometry import make_box_patch
     kind = "reference" if O.ref_available() else "port"
 
-    def timed(d):
-        run = (lambda: O.ref_sys_linmat_omp((), num_threads=threads, kwargs=d)) if kind == "reference" else \
-            (lambda: O.oracle_sys_linmat((), nthreads=threads, kwargs=d))
+    def timed(args, tag):
+        run = (lambda: O.ref_sys_linmat_omp(args, num_threads=threads)) if kind == "reference" else \
+            (lambda: O.oracle_sys_linmat(args, nthreads=threads))
         run()
         ts = []
-        for _ in range(runs):
+        for k in range(runs):
             t0 = time.perf_counter()
             run()
             ts.append(time.perf_counter() - t0)
+            log("cpu baseline %s: call %d of %d, %.2f s" % (tag, k + 1, runs, ts[-1]))
         return float(np.median(ts)), ts
 
-    bp = make_box_patch(nel=(nel, nel, nel), p=p)
-    n = (p + 1) ** 3
-    nde = 9 * n * (n - 1) // 2 + 6 * n
-    d = dict(bp.args)
-    act = np.zeros(bp.nb_elem, np.int32)
-    act[:prefix] = 1
-    d["activeElement"] = act
-    d["nb_data"] = float(prefix * nde)
-    t_pre, ts_pre = timed(d)
+    bp = make_box_patch(nel=(nel, nel, slab), p=p)
+    t_s, ts_s = timed(bp.f2py_args(), "%dx%dx%d p=%d" % (nel, nel, slab, p))
     c1 = make_box_patch(nel=(16, 4, 4), p=2)
-    t_c1, _ = timed(dict(c1.args))
-    return {"value": prefix / t_pre, "unit": "elem/s", "cores": threads, "kind": kind,
-            "sample": "first %d of the %d elements of the %d^3 p=%d benchmark patch (activeElement), "
+    t_c1, _ = timed(c1.f2py_args(), "16x4x4 p=2")
+    return {"value": bp.nb_elem / t_s, "unit": "elem/s", "cores": threads, "kind": kind,
+            "sample": "%dx%dx%d p=%d patch of the benchmark's generator (%d elements), "
                       "sys_linmat_lindef_static_omp incl. output allocation, median of %d calls after a "
-                      "warm-up: %.2f s (runs %s)" % (prefix, bp.nb_elem, nel, p, runs, t_pre,
-                                                    ", ".join("%.2f" % t for t in ts_pre)),
+                      "warm-up: %.2f s (runs %s)" % (nel, nel, slab, p, bp.nb_elem, runs, t_s,
+                                                    ", ".join("%.2f" % t for t in ts_s)),
             "c1_16x4x4_p2": {"elements": c1.nb_elem, "seconds": t_c1, "value": c1.nb_elem / t_c1,
                              "unit": "elem/s"}}
 
@@ -473,8 +469,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nel", type=int, default=64, help="elements per direction")
     ap.add_argument("--p", type=int, default=3, help="degree")
-    ap.add_argument("--cpu-prefix", type=int, default=16384,
-                    help="elements of the benchmark patch timed by the CPU baseline")
+    ap.add_argument("--cpu-slab", type=int, default=4,
+                    help="elements in z of the nel x nel x slab patch timed by the CPU baseline")
     ap.add_argument("--cpu-runs", type=int, default=3, help="timed CPU calls (median) after one warm-up")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = every core of this host")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -623,8 +619,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
-        log("cpu baseline: first %d elements with %d threads ..." % (args.cpu_prefix, threads))
-        cpu = cpu_baseline(args.nel, p, args.cpu_prefix, args.cpu_runs, threads)
+        log("cpu baseline: %dx%dx%d elements with %d threads ..." % (args.nel, args.nel, args.cpu_slab, threads))
+        cpu = cpu_baseline(args.nel, p, args.cpu_slab, args.cpu_runs, threads)
         log("cpu baseline: %.3e elem/s (%s); 16x4x4 p=2: %.3e elem/s"
             % (cpu["value"], cpu["kind"], cpu["c1_16x4x4_p2"]["value"]))
 
