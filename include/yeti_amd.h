@@ -349,6 +349,28 @@ int32_t yeti_ir_step(yeti_ir *s, int32_t use_graph, void *stream);
 int32_t yeti_ir_status_async(const yeti_ir *s, yeti_ir_status *host_status, void *stream);
 void    yeti_ir_destroy(yeti_ir *s);
 
+/* ---- batches of sessions (serving) ----------------------------------------------------------
+ * A batch advances many sessions of one precision (32 or 64) with the same launches: each launch
+ * of a step walks device task lists naming a member and a piece of its work (a brick of elements,
+ * a block of control points, a chunk of a vector, a tile of a preconditioner product), so one
+ * launch moves every active member forward. Members must be single-patch sessions with the fast
+ * diagonalisation preconditioner, created and begun (yeti_ir_begin) by the caller; their status
+ * flags and solutions stay in the sessions. A member's reductions are split into chunks that
+ * depend only on its own size, so its iterates are the same whatever else shares the batch.
+ * yeti_batch_step enqueues `chunk` iterations of every member (captured once into a hipGraph
+ * with use_graph; membership changes need no new capture). task_capacity bounds each task list.
+ * No reference counterpart (the reference solves one system per call). */
+typedef struct yeti_batch yeti_batch;
+int32_t yeti_batch_create(int32_t precision, int32_t chunk, int64_t task_capacity, yeti_batch **out);
+int32_t yeti_batch_add(yeti_batch *b, yeti_ir *s);
+int32_t yeti_batch_remove(yeti_batch *b, yeti_ir *s);
+int32_t yeti_batch_size(const yeti_batch *b);
+int32_t yeti_batch_step(yeti_batch *b, int32_t use_graph, void *stream);
+/* statuses of the members in the order they were added (removed ones leave the order), into
+ * host memory (pinned for an asynchronous copy); read it once the stream has reached this point */
+int32_t yeti_batch_status_async(yeti_batch *b, yeti_ir_status *host_status, void *stream);
+void    yeti_batch_destroy(yeti_batch *b);
+
 /* ---- Displacement reconstruction ------------------------------------------------------------
  * Replaces f2py reconstructionSOL.reconstruction (src/yeti_iga/main/reconstruction.f:22-135,
  * called as rsol.reconstruction(**get_inputs4solution(x)) at overlay/iga_model.py:504 and

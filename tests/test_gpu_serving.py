@@ -124,3 +124,53 @@ def test_session_footprint_formula(gpu, precond):
         bound = ir_session_bytes(dims, prec, "fdm" if precond else "jacobi")
         assert used.value <= counted.value <= bound
         _lib.lib().yeti_ir_destroy(h)
+
+
+# ---- batched serving: every active request advanced by the same launches -----------------
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_batched_solutions_match_direct_solve(gpu, precision):
+    """The batched server (shared launches walking device task lists) solves each request of a
+    mix of degrees 2-4 to rtol 1e-10 on its FP64 residual, within 1e-8 of the direct solve."""
+    from yeti_amd.serving import SolveServer
+    print("[case] batched solutions vs direct solve", precision, flush=True)
+    reqs = small_requests()
+    srv = SolveServer(budget_bytes=1 << 30, precision=precision, chunk=6, rtol=1e-10, batched=True)
+    rep = srv.run(reqs, keep_solutions=True, realtime=False)
+    srv.close()
+    assert rep["completed"] == len(reqs) and rep["failed"] == 0
+    for r in reqs:
+        u = direct_u(r.model())
+        err = np.linalg.norm(rep["x"][r.rid] - u) / np.linalg.norm(u)
+        info = rep["requests"][r.rid]
+        print("  req %d nel %s p %d: %d iterations, %d replacements, relres %.2e err %.2e" % (
+            r.rid, r.nel, r.p, info["iters"], info["updates"], info["relres"], err))
+        assert info["relres"] <= 1e-10 or info["floor"]
+        assert err <= 1e-8
+    assert rep["peak_arena_bytes"] <= srv.budget
+
+
+def test_batched_result_identical_alone_and_in_mix(gpu):
+    """A request's batched solution is bit-identical alone and inside a batch with eleven
+    others of other sizes and degrees, admitted together or arriving later (its reductions are
+    chunked by its own size only), and equal with and without graph capture."""
+    from yeti_amd.serving import SolveServer
+    print("[case] batched alone vs mix", flush=True)
+    reqs = small_requests()
+    mix = SolveServer(budget_bytes=1 << 30, precision="mixed", chunk=8, batched=True).run(
+        reqs, keep_solutions=True, realtime=False)
+    # staggered admissions: members join while others are mid-solve
+    staggered = [type(r)(r.rid, 0.002 * i, r.nel, r.p, r.E, r.nu, r.traction, r.seed) for i, r in enumerate(reqs)]
+    for a, b in zip(staggered, reqs):
+        a.args = b.args
+    late = SolveServer(budget_bytes=1 << 30, precision="mixed", chunk=8, batched=True).run(
+        staggered, keep_solutions=True, realtime=True)
+    for k in (0, 5, 11):
+        alone = SolveServer(budget_bytes=1 << 30, precision="mixed", chunk=8, batched=True).run(
+            [reqs[k]], keep_solutions=True, realtime=False)
+        eager = SolveServer(budget_bytes=1 << 30, precision="mixed", chunk=8, batched=True, use_graph=False).run(
+            [reqs[k]], keep_solutions=True, realtime=False)
+        assert np.array_equal(alone["x"][k], mix["x"][k])
+        assert np.array_equal(alone["x"][k], late["x"][k])
+        assert np.array_equal(alone["x"][k], eager["x"][k])

@@ -78,6 +78,13 @@ SIGNATURES = {
     "yeti_ir_step": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p]),
     "yeti_ir_status_async": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "yeti_ir_destroy": (None, [C.c_void_p]),
+    "yeti_batch_create": (C.c_int32, [C.c_int32, C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]),
+    "yeti_batch_add": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "yeti_batch_remove": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "yeti_batch_size": (C.c_int32, [C.c_void_p]),
+    "yeti_batch_step": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "yeti_batch_status_async": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "yeti_batch_destroy": (None, [C.c_void_p]),
     "yeti_csr_drop_zeros": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p]),
     "yeti_evaldisp": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),

@@ -24,7 +24,7 @@
 #include <cmath>
 #include <vector>
 
-#include "session.hpp"
+#include "batch.hpp"
 
 namespace yeti {
 
@@ -82,15 +82,6 @@ struct MfCfg {
     static_assert(EPB >= 1, "element too large for one workgroup");
 };
 
-struct MfPatch {
-    int32_t N[3];            // elements per direction
-    int32_t ncp[3];          // control points per direction
-    const int32_t *cpmap;    // patch grid CP -> global CP (0-based)
-    const double *Btab;      // [d][e_d][NG][NP1] values (basis function span - p + r)
-    const double *dBtab;     // same, derivatives
-    int32_t boff[3];         // offset of direction d in the tables (in elements)
-    double lam, mu;
-};
 
 template <typename T, int P, int NG, int NC, int BN>
 struct MfSmem {
@@ -209,20 +200,19 @@ struct MfBrick {
 // For MODE 2 the quadrature-point tensor is M = 2 H^T sigma - (sigma : H) I in place of sigma:
 // with d(dR_a/dx_k)/dX_cj = -dR_a/dx_j dR_c/dx_k and d(det J)/dX_cj = det J dR_c/dx_j,
 // -x^T dK_e/dX_cj x = sum_q w|det J| sum_k M_jk dR_c/dx_k, the same backward contraction as K x.
+// The work of one brick (`brick`, in the nb0 x nb1 x . brick grid of the patch): its elements'
+// contributions summed into partial[brick]. Every thread of the workgroup calls it.
 template <typename T, int P, int NG, int MODE>
-__global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__restrict__ geo, const T *__restrict__ x,
-                                                       const T *__restrict__ wcp, const int32_t *__restrict__ nbrick,
-                                                       T *__restrict__ partial, const int32_t *__restrict__ pred) {
-    if (pred && *pred == 0) return;   // predicated launch (device-side solver control)
+__device__ __forceinline__ void mf_brick_body(MfSmem<T, P, NG, MODE == 1 ? 9 : 3, MfBrick<P, MODE>::BN> &sm,
+                                              const MfPatch &mp, const T *__restrict__ geo, const T *__restrict__ x,
+                                              const T *__restrict__ wcp, int32_t nb0, int32_t nb1,
+                                              T *__restrict__ partial, int64_t brick) {
     using C = MfCfg<P, NG>;
     using Bk = MfBrick<P, MODE>;
     constexpr int NP1 = C::NP1, NQ = C::NQ, NN = C::NN, BN = Bk::BN, BR = Bk::BR, NC = MODE == 1 ? 9 : 3;
-    __shared__ MfSmem<T, P, NG, NC, BN> sm;
     const int tid = threadIdx.x;
     const int slot = tid / C::TPE, r = tid % C::TPE;
-    const int64_t brick = blockIdx.x;
-    const int b1 = (int)(brick % nbrick[0]), b2 = (int)((brick / nbrick[0]) % nbrick[1]),
-              b3 = (int)(brick / ((int64_t)nbrick[0] * nbrick[1]));
+    const int b1 = (int)(brick % nb0), b2 = (int)((brick / nb0) % nb1), b3 = (int)(brick / ((int64_t)nb0 * nb1));
     for (int i = tid; i < BN * BN * BN * NC; i += 256) sm.acc[i] = T(0);
     const T lam = (T)mp.lam, mu = (T)mp.mu;
     // elements of colour c: e_d = b_d BR + c_d + NP1 t_d, t_d in [0, K): EPC per colour
@@ -455,15 +445,20 @@ __global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__re
     for (int i = tid; i < BN * BN * BN * NC; i += 256) out[i] = sm.acc[i];
 }
 
+template <typename T, int P, int NG, int MODE>
+__global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__restrict__ geo, const T *__restrict__ x,
+                                                       const T *__restrict__ wcp, const int32_t *__restrict__ nbrick,
+                                                       T *__restrict__ partial, const int32_t *__restrict__ pred) {
+    if (pred && *pred == 0) return;   // predicated launch (device-side solver control)
+    __shared__ MfSmem<T, P, NG, MODE == 1 ? 9 : 3, MfBrick<P, MODE>::BN> sm;
+    mf_brick_body<T, P, NG, MODE>(sm, mp, geo, x, wcp, nbrick[0], nbrick[1], partial, blockIdx.x);
+}
+
 // y[global cp] (+)= sum over the bricks covering patch CP c (fixed order)
 template <typename T, int NC>
-__global__ void mf_gather_kernel(MfPatch mp, int32_t br, int32_t bn, int32_t nb0, int32_t nb1, int32_t nb2,
-                                 const T *__restrict__ partial, T *__restrict__ y, int accumulate,
-                                 const int32_t *__restrict__ pred) {
-    if (pred && *pred == 0) return;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t ncp = (int64_t)mp.ncp[0] * mp.ncp[1] * mp.ncp[2];
-    if (t >= ncp) return;
+__device__ __forceinline__ void mf_gather_point(const MfPatch &mp, int32_t br, int32_t bn, int32_t nb0, int32_t nb1,
+                                                int32_t nb2, const T *__restrict__ partial, T *__restrict__ y,
+                                                int accumulate, int64_t t) {
     const int c1 = (int)(t % mp.ncp[0]), c2 = (int)((t / mp.ncp[0]) % mp.ncp[1]),
               c3 = (int)(t / ((int64_t)mp.ncp[0] * mp.ncp[1]));
     const int cc[3] = {c1, c2, c3};
@@ -491,6 +486,17 @@ __global__ void mf_gather_kernel(MfPatch mp, int32_t br, int32_t bn, int32_t nb0
         if (accumulate) y[(int64_t)NC * cp + c] += s[c];
         else y[(int64_t)NC * cp + c] = s[c];
     }
+}
+
+template <typename T, int NC>
+__global__ void mf_gather_kernel(MfPatch mp, int32_t br, int32_t bn, int32_t nb0, int32_t nb1, int32_t nb2,
+                                 const T *__restrict__ partial, T *__restrict__ y, int accumulate,
+                                 const int32_t *__restrict__ pred) {
+    if (pred && *pred == 0) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ncp = (int64_t)mp.ncp[0] * mp.ncp[1] * mp.ncp[2];
+    if (t >= ncp) return;
+    mf_gather_point<T, NC>(mp, br, bn, nb0, nb1, nb2, partial, y, accumulate, t);
 }
 
 template <typename T>
@@ -643,16 +649,6 @@ static int mf_geometry(const yeti_mf *m, const MfPatchDev &pd, hipStream_t st) {
 // must be whole patch faces per component (the clamped face of the solids here).
 namespace yeti {
 
-struct FdmArgs {
-    int32_t n[3];            // patch CP grid
-    int32_t lo[3][3];        // [component][direction] first free index
-    int32_t m[3][3];         // [component][direction] free count
-    int64_t tot[3];          // m0 m1 m2 per component
-    int64_t maxtot;
-    int64_t uoff[3][3];      // offset of U_cd (m x m, row-major) in the matrix buffer
-    int64_t doff[3];         // offset of 1/D_c in the scaling buffer
-    const int32_t *cpmap;
-};
 
 }  // namespace yeti
 
@@ -758,31 +754,18 @@ static bool gen_eigen(int n, const std::vector<double> &K, const std::vector<dou
 // slice (m0 x m1) and R = A^T; along axis 2, L = the tensor seen as (m0 m1) x m2 and R = A^T.
 // Element (row, col) of operand X of component c, batch z, is at X[off[c] + z bs[c] + row rs[c]
 // + col cs[c]].
-struct FdmGemm {
-    int32_t M[3], N[3], K[3], nbatch[3];
-    int64_t loff[3], lrs[3], lcs[3], lbs[3];
-    int64_t roff[3], rrs[3], rcs[3], rbs[3];
-    int64_t coff[3], crs[3], ccs[3], cbs[3];
-    int32_t gather;   // R is read from the DOF vector r through the CP map (first forward product)
-    int32_t scale;    // C is multiplied by 1 / D_c (last forward product)
-};
 
 // 64 x 64 tile of C per 256-thread workgroup, K in steps of 16 through LDS, 4 x 4 outputs per
 // thread (rows tx + 16 i, columns ty + 16 j), accumulated in T. grid = (N tiles, M tiles,
 // 3 x batches).
+// One 64 x 64 tile (rows from mt, columns from nt) of component c, batch z; every thread of the
+// workgroup calls it.
 template <typename T>
-__global__ void __launch_bounds__(256) fdm_gemm_kernel(FdmGemm g, FdmArgs fa, const T *__restrict__ L,
-                                                       const T *__restrict__ R, T *__restrict__ Cout,
-                                                       const T *__restrict__ invD, const T *__restrict__ r,
-                                                       const int32_t *__restrict__ pred) {
-    if (pred && *pred == 0) return;
-    const int c = blockIdx.z % 3, z = blockIdx.z / 3;
-    if (z >= g.nbatch[c]) return;
+__device__ __forceinline__ void fdm_gemm_tile(const FdmGemm &g, const FdmArgs &fa, int c, int z, int mt, int nt,
+                                              const T *__restrict__ L, const T *__restrict__ R, T *__restrict__ Cout,
+                                              const T *__restrict__ invD, const T *__restrict__ r, T (*Ls)[65],
+                                              T (*Rs)[65]) {
     const int M = g.M[c], N = g.N[c], K = g.K[c];
-    const int mt = blockIdx.y * 64, nt = blockIdx.x * 64;
-    if (mt >= M || nt >= N) return;
-    __shared__ T Ls[16][65];
-    __shared__ T Rs[16][65];
     const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
     const T *Lb = L + g.loff[c] + z * g.lbs[c];
     const T *Rb = R + g.roff[c] + z * g.rbs[c];
@@ -843,6 +826,21 @@ __global__ void __launch_bounds__(256) fdm_gemm_kernel(FdmGemm g, FdmArgs fa, co
                 Cout[g.coff[c] + lin] = v;
             }
         }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) fdm_gemm_kernel(FdmGemm g, FdmArgs fa, const T *__restrict__ L,
+                                                       const T *__restrict__ R, T *__restrict__ Cout,
+                                                       const T *__restrict__ invD, const T *__restrict__ r,
+                                                       const int32_t *__restrict__ pred) {
+    if (pred && *pred == 0) return;
+    const int c = blockIdx.z % 3, z = blockIdx.z / 3;
+    if (z >= g.nbatch[c]) return;
+    const int mt = blockIdx.y * 64, nt = blockIdx.x * 64;
+    if (mt >= g.M[c] || nt >= g.N[c]) return;
+    __shared__ T Ls[16][65];
+    __shared__ T Rs[16][65];
+    fdm_gemm_tile<T>(g, fa, c, z, mt, nt, L, R, Cout, invD, r, Ls, Rs);
 }
 
 // GEMM descriptor of the mode product along `axis` with A[a][b] = P[b m + a] (transposed_store)
@@ -1127,6 +1125,187 @@ int fdm_create(const yeti_mf *m, const uint8_t *mask_host, int precision, hipStr
 }
 
 void fdm_destroy(yeti_fdm *f) { delete f; }
+
+// ---- batches: the operator and the preconditioner of many sessions per launch -------------
+// Bricks of degree P of every member: the iteration apply p -> brick partials in the batch
+// precision T (active members), or (RPL) the FP64 apply of the replacement x -> FP64 partials
+// (members with a replacement due).
+template <typename T, int P, int NG, bool RPL>
+__global__ void __launch_bounds__(256) bt_brick_kernel(BatchDev bd, int list) {
+    __shared__ MfSmem<T, P, NG, 3, MfBrick<P, 0>::BN> sm;
+    constexpr int pi = sizeof(T) == 8 ? 0 : 1;
+    const int32_t cnt = bd.count[list];
+    const BTask *tl = bd.list[list];
+    for (int32_t t = blockIdx.x; t < cnt; t += gridDim.x) {
+        const BTask tk = tl[t];
+        const BReq &R = bd.req[tk.req];
+        if (R.flags[RPL ? F_UPDATE : F_ACTIVE] == 0) continue;   // the same for every thread
+        __syncthreads();   // the previous task's copy-out has read the accumulator
+        const T *x = RPL ? (const T *)R.x : (const T *)R.vp;
+        mf_brick_body<T, P, NG, 0>(sm, R.mp, (const T *)R.geo[pi], x, (const T *)R.w[pi], R.nbrick[0], R.nbrick[1],
+                                   (T *)R.part[pi], tk.a);
+    }
+}
+
+// Node sums of the bricks: q (iteration) or y64 (RPL), BATCH_GATHER_BLOCK control points a task.
+template <typename T, bool RPL>
+__global__ void __launch_bounds__(256) bt_gather_kernel(BatchDev bd) {
+    constexpr int pi = sizeof(T) == 8 ? 0 : 1;
+    const int32_t cnt = bd.count[BT_GATHER];
+    const BTask *tl = bd.list[BT_GATHER];
+    for (int32_t t = blockIdx.x; t < cnt; t += gridDim.x) {
+        const BTask tk = tl[t];
+        const BReq &R = bd.req[tk.req];
+        if (R.flags[RPL ? F_UPDATE : F_ACTIVE] == 0) continue;
+        const int64_t cp = (int64_t)tk.a * BATCH_GATHER_BLOCK + threadIdx.x;
+        if (cp >= R.ncp) continue;
+        T *y = RPL ? (T *)R.y64 : (T *)R.vq;
+        mf_gather_point<T, 3>(R.mp, R.br, R.bn, R.nbrick[0], R.nbrick[1], R.nbrick[2], (const T *)R.part[pi], y, 0, cp);
+    }
+}
+
+// Tiles of the preconditioner's mode product `prod` of every active member.
+template <typename T>
+__global__ void __launch_bounds__(256) bt_fdm_gemm_kernel(BatchDev bd, int prod) {
+    __shared__ T Ls[16][65];
+    __shared__ T Rs[16][65];
+    const int list = BT_GEMM0 + prod;
+    const int32_t cnt = bd.count[list];
+    const BTask *tl = bd.list[list];
+    for (int32_t t = blockIdx.x; t < cnt; t += gridDim.x) {
+        const BTask tk = tl[t];
+        const BReq &R = bd.req[tk.req];
+        if (R.flags[F_ACTIVE] == 0) continue;
+        const FdmGemm &g = R.g[prod];
+        const int c = tk.a, z = tk.b;
+        const int ntn = (g.N[c] + 63) / 64;
+        __syncthreads();   // the previous tile's reads of Ls / Rs are done
+        fdm_gemm_tile<T>(g, R.fa, c, z, (tk.c / ntn) * 64, (tk.c % ntn) * 64, (const T *)R.L[prod],
+                         (const T *)R.R[prod], (T *)R.C[prod], (const T *)R.invD, (const T *)R.vr, Ls, Rs);
+    }
+}
+
+// z from the last product (0 outside the free boxes and on masked dofs) and r.z, handed to
+// rz_tail by each member's last chunk (mode 0: first direction; 1: end of an iteration).
+template <typename T>
+__global__ void __launch_bounds__(256) bt_fdm_scatter_kernel(BatchDev bd, int mode) {
+    __shared__ double sh[4];
+    const int32_t cnt = bd.count[BT_CP];
+    const BTask *tl = bd.list[BT_CP];
+    for (int32_t t = blockIdx.x; t < cnt; t += gridDim.x) {
+        const BTask tk = tl[t];
+        const BReq &R = bd.req[tk.req];
+        if (R.flags[F_ACTIVE] == 0) continue;
+        const FdmArgs &fa = R.fa;
+        const T *w = (const T *)R.wout;
+        const T *r = (const T *)R.vr;
+        T *z = (T *)R.vz;
+        const int64_t g0 = (int64_t)tk.a * R.cchunk, g1 = std::min<int64_t>(R.ncp, g0 + R.cchunk);
+        double acc = 0.0;
+        for (int64_t g = g0 + threadIdx.x; g < g1; g += blockDim.x) {
+            const int i = (int)(g % fa.n[0]), j = (int)((g / fa.n[0]) % fa.n[1]), k = (int)(g / ((int64_t)fa.n[0] * fa.n[1]));
+            const int64_t cp = fa.cpmap[g];
+            for (int c = 0; c < 3; ++c) {
+                const int ii = i - fa.lo[c][0], jj = j - fa.lo[c][1], kk = k - fa.lo[c][2];
+                T v = T(0);
+                if (ii >= 0 && ii < fa.m[c][0] && jj >= 0 && jj < fa.m[c][1] && kk >= 0 && kk < fa.m[c][2] &&
+                    R.mask[3 * cp + c])
+                    v = w[c * fa.maxtot + ii + (int64_t)fa.m[c][0] * (jj + (int64_t)fa.m[c][1] * kk)];
+                z[3 * cp + c] = v;
+                acc += (double)r[3 * cp + c] * (double)v;
+            }
+        }
+        double tot;
+        if (member_total(block_sum256(acc, sh), R.red, R.flags + F_COUNTER, R.ncchunk, tk.a, sh, tot) &&
+            threadIdx.x == 0) {
+            RzTail tail;
+            tail.sc = R.sc;
+            tail.flags = R.flags;
+            tail.mode = mode;
+            tail.max_iters = R.max_iters;
+            rz_tail(tail, tot);
+        }
+    }
+}
+
+constexpr unsigned BT_GRID_BRICK = 1024, BT_GRID = 2048;
+
+template <typename T, bool RPL>
+static int batch_apply_t(const BatchDev &bd, hipStream_t st) {
+    hipLaunchKernelGGL((bt_brick_kernel<T, 1, 2, RPL>), dim3(BT_GRID_BRICK), dim3(256), 0, st, bd, (int)BT_BRICK1);
+    hipLaunchKernelGGL((bt_brick_kernel<T, 2, 3, RPL>), dim3(BT_GRID_BRICK), dim3(256), 0, st, bd, (int)BT_BRICK2);
+    hipLaunchKernelGGL((bt_brick_kernel<T, 3, 4, RPL>), dim3(BT_GRID_BRICK), dim3(256), 0, st, bd, (int)BT_BRICK3);
+    hipLaunchKernelGGL((bt_brick_kernel<T, 4, 5, RPL>), dim3(BT_GRID_BRICK), dim3(256), 0, st, bd, (int)BT_BRICK4);
+    hipLaunchKernelGGL((bt_gather_kernel<T, RPL>), dim3(BT_GRID), dim3(BATCH_GATHER_BLOCK), 0, st, bd);
+    YETI_HIP(hipGetLastError());
+    return 0;
+}
+
+int batch_mf_apply(const BatchDev &bd, int prec64, bool rpl, hipStream_t st) {
+    if (rpl) return batch_apply_t<double, true>(bd, st);
+    return prec64 ? batch_apply_t<double, false>(bd, st) : batch_apply_t<float, false>(bd, st);
+}
+
+template <typename T>
+static int batch_fdm_t(const BatchDev &bd, int mode, hipStream_t st) {
+    for (int prod = 0; prod < 6; ++prod)
+        hipLaunchKernelGGL(bt_fdm_gemm_kernel<T>, dim3(BT_GRID), dim3(256), 0, st, bd, prod);
+    hipLaunchKernelGGL(bt_fdm_scatter_kernel<T>, dim3(BT_GRID), dim3(256), 0, st, bd, mode);
+    YETI_HIP(hipGetLastError());
+    return 0;
+}
+
+int batch_fdm_apply(const BatchDev &bd, int prec64, int mode, hipStream_t st) {
+    return prec64 ? batch_fdm_t<double>(bd, mode, st) : batch_fdm_t<float>(bd, mode, st);
+}
+
+// The six mode products of fdm_apply_t: (axis, matrix, transposed store, input, output).
+int batch_fill_operator(const yeti_mf *m, const yeti_fdm *f, int prec64, BReq *R) {
+    if (!m || !f) return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: session without fast diagonalisation");
+    if (m->patches.size() != 1 || m->zero_first)
+        return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: single-patch full-grid operators only");
+    const MfPatchDev &pd = *m->patches[0];
+    if (pd.p < 1 || pd.p > 4 || pd.ng != pd.p + 1)
+        return set_error(YETI_ERR_UNSUPPORTED_DEGREE, "batch: degree %d with %d points", pd.p, pd.ng);
+    if (!prec64 && (!pd.geo32.p || !pd.part32.p || !m->w32.p))
+        return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: FP32 operator not prepared");
+    R->mp = pd.mp;
+    R->p = pd.p;
+    R->ng = pd.ng;
+    for (int d = 0; d < 3; ++d) R->nbrick[d] = pd.nbrick[0][d];
+    R->br = 2 * (pd.p + 1);
+    R->bn = R->br + pd.p;
+    R->ncp = m->nb_cp;
+    R->n = 3 * (int64_t)m->nb_cp;
+    R->geo[0] = pd.geo64.p;
+    R->geo[1] = pd.geo32.p;
+    R->w[0] = m->w64.p;
+    R->w[1] = m->w32.p;
+    R->part[0] = pd.part64.p;
+    R->part[1] = pd.part32.p;
+    R->fa = f->fa;
+    const void *U = prec64 ? (const void *)f->U64.p : (const void *)f->U32.p;
+    const void *Ut = prec64 ? (const void *)f->Ut64.p : (const void *)f->Ut32.p;
+    void *wa = prec64 ? (void *)f->wa64.p : (void *)f->wa32.p;
+    void *wb = prec64 ? (void *)f->wb64.p : (void *)f->wb32.p;
+    R->invD = prec64 ? (const void *)f->invD64.p : (const void *)f->invD32.p;
+    if (!U || !Ut || !wa || !wb || !R->invD) return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: preconditioner precision");
+    struct Prod { int axis; const void *P; bool tr; const void *in; void *out; bool gather, scale; };
+    const Prod prods[6] = {{0, U, true, nullptr, wb, true, false},  {1, Ut, false, wb, wa, false, false},
+                           {2, Ut, false, wa, wb, false, true},     {0, Ut, true, wb, wa, false, false},
+                           {1, U, false, wa, wb, false, false},     {2, U, false, wb, wa, false, false}};
+    for (int k = 0; k < 6; ++k) {
+        FdmGemm g = fdm_gemm_op(f->fa, prods[k].axis, prods[k].tr);
+        g.gather = prods[k].gather ? 1 : 0;
+        g.scale = prods[k].scale ? 1 : 0;
+        R->g[k] = g;
+        R->L[k] = prods[k].axis == 0 ? prods[k].P : prods[k].in;
+        R->R[k] = prods[k].axis == 0 ? prods[k].in : prods[k].P;
+        R->C[k] = prods[k].out;
+    }
+    R->wout = wa;
+    return 0;
+}
 
 }  // namespace yeti
 

@@ -13,7 +13,7 @@
 #include <cstddef>
 #include <vector>
 
-#include "session.hpp"
+#include "batch.hpp"
 
 extern "C" {
 int32_t yeti_mf_apply_f64(const yeti_mf *m, const double *x, double *y, void *stream);
@@ -775,6 +775,318 @@ static int ir_begin_launches(yeti_ir *s, T *r, T *y, T *z, T *p, const T *inv, h
 
 }  // namespace yeti
 
+namespace yeti {
+
+// ---- batches: many sessions advanced by the same launches --------------------------------
+// Member vector chunk [a vchunk, (a + 1) vchunk) of task (req, a) in the BT_VEC list.
+#define BT_VEC_LOOP(bd, PRED)                                                          \
+    const int32_t cnt_ = bd.count[BT_VEC];                                             \
+    const BTask *tl_ = bd.list[BT_VEC];                                                \
+    for (int32_t t_ = blockIdx.x; t_ < cnt_; t_ += gridDim.x) {                         \
+        const BTask tk = tl_[t_];                                                      \
+        const BReq &R = bd.req[tk.req];                                                \
+        if (R.flags[PRED] == 0) continue;                                              \
+        const int64_t i0 = (int64_t)tk.a * R.vchunk, i1 = std::min<int64_t>(R.n, i0 + R.vchunk);
+
+// q = mask q; S_PQ = p.q
+template <typename T>
+__global__ void __launch_bounds__(256) bt_mask_dot_kernel(BatchDev bd) {
+    __shared__ double sh[4];
+    BT_VEC_LOOP(bd, F_ACTIVE)
+        const T *p = (const T *)R.vp;
+        T *q = (T *)R.vq;
+        double acc = 0.0;
+        for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            T qi = R.mask[i] ? q[i] : T(0);
+            q[i] = qi;
+            acc += (double)p[i] * (double)qi;
+        }
+        double tot;
+        if (member_total(block_sum256(acc, sh), R.red, R.flags + F_COUNTER, R.nvchunk, tk.a, sh, tot) &&
+            threadIdx.x == 0)
+            R.sc[S_PQ] = tot;
+    }
+}
+
+// y += alpha p, r -= alpha q; S_RR = r.r and the replacement test (as pred_xr_kernel)
+template <typename T>
+__global__ void __launch_bounds__(256) bt_xr_kernel(BatchDev bd) {
+    __shared__ double sh[4];
+    BT_VEC_LOOP(bd, F_ACTIVE)
+        const double alpha = R.sc[S_RZ] / R.sc[S_PQ];
+        T *y = (T *)R.vy;
+        T *r = (T *)R.vr;
+        const T *p = (const T *)R.vp;
+        const T *q = (const T *)R.vq;
+        double acc = 0.0;
+        for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            y[i] = (T)((double)y[i] + alpha * (double)p[i]);
+            T ri = (T)((double)r[i] - alpha * (double)q[i]);
+            r[i] = ri;
+            acc += (double)ri * (double)ri;
+        }
+        double rr;
+        if (member_total(block_sum256(acc, sh), R.red, R.flags + F_COUNTER, R.nvchunk, tk.a, sh, rr) &&
+            threadIdx.x == 0) {
+            double *sc = R.sc;
+            int32_t *flags = R.flags;
+            sc[S_RR] = rr;
+            if (rr > sc[S_RRMAX]) sc[S_RRMAX] = rr;
+            const int32_t since = flags[F_ITERS] + 1 - (int32_t)sc[S_LASTUPD];
+            if (!(rr > R.delta2 * sc[S_RRMAX]) || !(rr > sc[S_TOL2])) {
+                if (flags[F_UPDATE] == 0) flags[F_UPDATE] = 1;
+            } else if (since >= (flags[F_STAGNANT] && flags[F_DEPARTED] ? IR_FORCE_STAGNANT : IR_FORCE_EVERY)) {
+                if (flags[F_UPDATE] == 0) flags[F_UPDATE] = 2;
+            }
+        }
+    }
+}
+
+// p = z + beta p
+template <typename T>
+__global__ void __launch_bounds__(256) bt_p_kernel(BatchDev bd) {
+    BT_VEC_LOOP(bd, F_ACTIVE)
+        const double beta = R.sc[S_BETA];
+        T *p = (T *)R.vp;
+        const T *z = (const T *)R.vz;
+        for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) p[i] = (T)((double)z[i] + beta * (double)p[i]);
+    }
+}
+
+// x += y, y = 0 (members with a replacement due)
+template <typename T>
+__global__ void __launch_bounds__(256) bt_accumulate_kernel(BatchDev bd) {
+    BT_VEC_LOOP(bd, F_UPDATE)
+        T *y = (T *)R.vy;
+        for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            R.x[i] += (double)y[i];
+            y[i] = T(0);
+        }
+    }
+}
+
+// r = mask (b - K x) from the FP64 apply, and the session's replacement logic (as
+// ir_true_residual_kernel)
+template <typename T>
+__global__ void __launch_bounds__(256) bt_true_residual_kernel(BatchDev bd) {
+    __shared__ double sh[4];
+    BT_VEC_LOOP(bd, F_UPDATE)
+        T *r = (T *)R.vr;
+        double acc = 0.0;
+        for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            double v = R.mask[i] ? R.b[i] - R.y64[i] : 0.0;
+            r[i] = (T)v;
+            acc += v * v;
+        }
+        double rrt;
+        if (!member_total(block_sum256(acc, sh), R.red, R.flags + F_COUNTER, R.nvchunk, tk.a, sh, rrt) ||
+            threadIdx.x != 0)
+            continue;
+        double *sc = R.sc;
+        int32_t *flags = R.flags;
+        const bool forced = flags[F_UPDATE] == 2;
+        const bool departed = rrt > 16.0 * sc[S_RR];
+        sc[S_RRT] = rrt;
+        flags[F_UPDATE] = 0;
+        flags[F_UPDATES] += 1;
+        sc[S_LASTUPD] = (double)(flags[F_ITERS] + 1);
+        sc[S_RR] = rrt;
+        sc[S_RRMAX] = rrt;
+        if (!(rrt > R.rtol2 * sc[S_BB])) {
+            flags[F_DONE] = 1;
+            flags[F_ACTIVE] = 0;
+            continue;
+        }
+        flags[F_DEPARTED] = departed ? 1 : 0;
+        if (!(rrt < 0.25 * sc[S_RRT_PREV])) flags[F_STAGNANT] += 1;
+        else flags[F_STAGNANT] = 0;
+        const bool diverging = rrt > 1e4 * sc[S_RRT_PREV];
+        if (rrt < sc[S_RRT_PREV]) {
+            sc[S_RRT_PREV] = rrt;
+            flags[F_BEST] = 1;
+        }
+        if (forced && departed) sc[S_RZ] = __builtin_inf();
+        bool stop = false;
+        if (flags[F_STAGNANT] >= IR_MAX_STAGNANT || diverging) {
+            flags[F_FLOOR] = 1;
+            stop = true;
+        } else if (flags[F_UPDATES] >= R.max_updates) {
+            flags[F_FAILED] = 1;
+            stop = true;
+        }
+        if (stop) {
+            flags[F_DONE] = 1;
+            flags[F_ACTIVE] = 0;
+            if (sc[S_RRT_PREV] < rrt) {
+                flags[F_RESTORE] = 1;
+                sc[S_RRT] = sc[S_RRT_PREV];
+            }
+        }
+    }
+}
+
+// keep the best iterate / put it back (as ir_best_kernel); the flags are cleared afterwards by
+// bt_best_clear_kernel
+__global__ void __launch_bounds__(256) bt_best_kernel(BatchDev bd) {
+    const int32_t cnt = bd.count[BT_VEC];
+    const BTask *tl = bd.list[BT_VEC];
+    for (int32_t t = blockIdx.x; t < cnt; t += gridDim.x) {
+        const BTask tk = tl[t];
+        const BReq &R = bd.req[tk.req];
+        const int32_t best = R.flags[F_BEST], restore = R.flags[F_RESTORE];
+        if (!best && !restore) continue;
+        const int64_t i0 = (int64_t)tk.a * R.vchunk, i1 = std::min<int64_t>(R.n, i0 + R.vchunk);
+        for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            if (best) R.xbest[i] = R.x[i];
+            else R.x[i] = R.xbest[i];
+        }
+    }
+}
+
+__global__ void bt_best_clear_kernel(BatchDev bd) {
+    for (int32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < bd.count[BT_NLISTS]; m += gridDim.x * blockDim.x) {
+        bd.req[m].flags[F_BEST] = 0;
+        bd.req[m].flags[F_RESTORE] = 0;
+    }
+}
+
+// statuses of the members, in member order, in the layout of yeti_ir_status
+__global__ void bt_status_kernel(BatchDev bd, yeti_ir_status *out) {
+    const int32_t nm = bd.count[BT_NLISTS];
+    for (int32_t m = blockIdx.x; m < nm; m += gridDim.x) {
+        const BReq &R = bd.req[m];
+        int32_t *fl = &out[m].active;
+        double *sc = &out[m].rz;
+        if (threadIdx.x < F_NFLAGS) fl[threadIdx.x] = R.flags[threadIdx.x];
+        if (threadIdx.x < S_NSC) sc[threadIdx.x] = R.sc[threadIdx.x];
+    }
+}
+
+constexpr unsigned BT_VGRID = 2048;
+
+template <typename T>
+static int batch_step_launches(const BatchDev &bd, int prec64, int chunk, hipStream_t st) {
+    for (int k = 0; k < chunk; ++k) {
+        int rc = batch_mf_apply(bd, prec64, false, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(bt_mask_dot_kernel<T>, dim3(BT_VGRID), dim3(256), 0, st, bd);
+        hipLaunchKernelGGL(bt_xr_kernel<T>, dim3(BT_VGRID), dim3(256), 0, st, bd);
+        if (k == chunk - 1) {
+            hipLaunchKernelGGL(bt_accumulate_kernel<T>, dim3(BT_VGRID), dim3(256), 0, st, bd);
+            rc = batch_mf_apply(bd, prec64, true, st);
+            if (rc) return rc;
+            hipLaunchKernelGGL(bt_true_residual_kernel<T>, dim3(BT_VGRID), dim3(256), 0, st, bd);
+            hipLaunchKernelGGL(bt_best_kernel, dim3(BT_VGRID), dim3(256), 0, st, bd);
+            hipLaunchKernelGGL(bt_best_clear_kernel, dim3(1), dim3(256), 0, st, bd);
+        }
+        rc = batch_fdm_apply(bd, prec64, 1, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(bt_p_kernel<T>, dim3(BT_VGRID), dim3(256), 0, st, bd);
+        YETI_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+}  // namespace yeti
+
+struct yeti_batch {
+    int prec64 = 0;
+    int32_t chunk = 16;
+    std::vector<yeti_ir *> members;
+    std::vector<yeti::BReq> hreq;
+    yeti::DevBuf<yeti::BReq> dreq;
+    yeti::DevBuf<yeti::BTask> dlist[yeti::BT_NLISTS];
+    yeti::DevBuf<int32_t> dcount;
+    yeti::DevBuf<yeti_ir_status> dstat;
+    int64_t cap = 0;             // tasks per list
+    bool dirty = true;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    ~yeti_batch() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+    yeti::BatchDev dev() const {
+        yeti::BatchDev bd;
+        bd.req = dreq.p;
+        for (int l = 0; l < yeti::BT_NLISTS; ++l) bd.list[l] = dlist[l].p;
+        bd.count = dcount.p;
+        return bd;
+    }
+};
+
+namespace yeti {
+
+// session fields of a member (the operator's come from batch_fill_operator)
+static int batch_fill_session(const yeti_ir *s, int prec64, BReq *R) {
+    if (!s->b || !s->x) return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: session not begun");
+    if (s->csr) return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: sessions with an assembled operator are not batched");
+    R->mask = s->mask;
+    R->b = s->b;
+    R->x = s->x;
+    R->xbest = s->xbest.p;
+    R->y64 = s->y64.p;
+    if (prec64) {
+        R->vr = s->vr64.p; R->vy = s->vy64.p; R->vz = s->vz64.p; R->vp = s->vp64.p; R->vq = s->vq64.p;
+    } else {
+        R->vr = s->vr32.p; R->vy = s->vy32.p; R->vz = s->vz32.p; R->vp = s->vp32.p; R->vq = s->vq32.p;
+    }
+    R->sc = s->sc.p;
+    R->flags = s->flags.p;
+    R->red = s->part.p;
+    const int64_t n = s->n, ncp = s->ncp;
+    R->vchunk = (int32_t)std::max<int64_t>(BATCH_CHUNK_MIN, (n + BATCH_RED_MAX - 1) / BATCH_RED_MAX);
+    R->nvchunk = (int32_t)((n + R->vchunk - 1) / R->vchunk);
+    R->cchunk = (int32_t)std::max<int64_t>(BATCH_CHUNK_MIN / 3, (ncp + BATCH_RED_MAX - 1) / BATCH_RED_MAX);
+    R->ncchunk = (int32_t)((ncp + R->cchunk - 1) / R->cchunk);
+    R->rtol2 = s->rtol * s->rtol;
+    R->delta2 = s->delta * s->delta;
+    R->max_iters = s->max_iters;
+    R->max_updates = s->max_updates;
+    return 0;
+}
+
+// task lists of the current members, uploaded to the device (on `st`, which is synchronised)
+static int batch_upload(yeti_batch *b, hipStream_t st) {
+    std::vector<BTask> lists[BT_NLISTS];
+    for (int32_t mi = 0; mi < (int32_t)b->hreq.size(); ++mi) {
+        const BReq &R = b->hreq[mi];
+        const int64_t nbr = (int64_t)R.nbrick[0] * R.nbrick[1] * R.nbrick[2];
+        for (int64_t k = 0; k < nbr; ++k) lists[BT_BRICK1 + R.p - 1].push_back({mi, (int32_t)k, 0, 0});
+        for (int64_t k = 0; k < (R.ncp + BATCH_GATHER_BLOCK - 1) / BATCH_GATHER_BLOCK; ++k)
+            lists[BT_GATHER].push_back({mi, (int32_t)k, 0, 0});
+        for (int32_t k = 0; k < R.nvchunk; ++k) lists[BT_VEC].push_back({mi, k, 0, 0});
+        for (int32_t k = 0; k < R.ncchunk; ++k) lists[BT_CP].push_back({mi, k, 0, 0});
+        for (int prod = 0; prod < 6; ++prod) {
+            const FdmGemm &g = R.g[prod];
+            for (int c = 0; c < 3; ++c) {
+                const int tiles = ((g.M[c] + 63) / 64) * ((g.N[c] + 63) / 64);
+                for (int z = 0; z < g.nbatch[c]; ++z)
+                    for (int tt = 0; tt < tiles; ++tt) lists[BT_GEMM0 + prod].push_back({mi, c, z, tt});
+            }
+        }
+    }
+    std::vector<int32_t> counts(BT_NLISTS + 1);
+    for (int l = 0; l < BT_NLISTS; ++l) {
+        if ((int64_t)lists[l].size() > b->cap)
+            return set_error(YETI_ERR_BUDGET, "batch: %zu tasks in list %d, capacity %lld", lists[l].size(), l,
+                             (long long)b->cap);
+        counts[l] = (int32_t)lists[l].size();
+        if (!lists[l].empty())
+            YETI_HIP(hipMemcpyAsync(b->dlist[l].p, lists[l].data(), lists[l].size() * sizeof(BTask),
+                                    hipMemcpyHostToDevice, st));
+    }
+    counts[BT_NLISTS] = (int32_t)b->hreq.size();
+    if (!b->hreq.empty())
+        YETI_HIP(hipMemcpyAsync(b->dreq.p, b->hreq.data(), b->hreq.size() * sizeof(BReq), hipMemcpyHostToDevice, st));
+    YETI_HIP(hipMemcpyAsync(b->dcount.p, counts.data(), counts.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    YETI_HIP(hipStreamSynchronize(st));   // the host vectors go out of scope
+    b->dirty = false;
+    return 0;
+}
+
+}  // namespace yeti
+
 extern "C" {
 
 int32_t yeti_ir_create(const yeti_mf *m, const uint8_t *free_mask, int32_t precision, int32_t precond, double rtol,
@@ -922,5 +1234,118 @@ int32_t yeti_ir_status_async(const yeti_ir *s, yeti_ir_status *host_status, void
 }
 
 void yeti_ir_destroy(yeti_ir *s) { delete s; }
+
+int32_t yeti_batch_create(int32_t precision, int32_t chunk, int64_t task_capacity, yeti_batch **out) {
+    if (!out) return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: null handle pointer");
+    *out = nullptr;
+    if ((precision != 32 && precision != 64) || chunk < 1 || task_capacity < 1 || task_capacity > INT32_MAX)
+        return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: bad precision, chunk or capacity");
+    auto *b = new yeti_batch();
+    b->prec64 = precision == 64;
+    b->chunk = chunk;
+    b->cap = task_capacity;
+    hipError_t e = b->dreq.alloc(BATCH_MAX_MEMBERS);
+    for (int l = 0; l < BT_NLISTS && e == hipSuccess; ++l) e = b->dlist[l].alloc((size_t)task_capacity);
+    if (e == hipSuccess) e = b->dcount.alloc(BT_NLISTS + 1);
+    if (e == hipSuccess) e = b->dstat.alloc(BATCH_MAX_MEMBERS);
+    if (e == hipSuccess) {
+        std::vector<int32_t> zero(BT_NLISTS + 1, 0);
+        e = hipMemcpy(b->dcount.p, zero.data(), zero.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        delete b;
+        return set_error(YETI_ERR_DEVICE, "batch: device tables: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return 0;
+}
+
+int32_t yeti_batch_add(yeti_batch *b, yeti_ir *s) {
+    if (!b || !s) return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: null argument");
+    if ((int)b->members.size() >= BATCH_MAX_MEMBERS) return set_error(YETI_ERR_BUDGET, "batch: full");
+    if (s->inner64 != b->prec64) return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: session precision differs");
+    for (auto *m : b->members)
+        if (m == s) return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: session already a member");
+    BReq R{};
+    int rc = batch_fill_operator(s->m, s->fdm, b->prec64, &R);
+    if (rc) return rc;
+    rc = batch_fill_session(s, b->prec64, &R);
+    if (rc) return rc;
+    b->members.push_back(s);
+    b->hreq.push_back(R);
+    b->dirty = true;
+    return 0;
+}
+
+int32_t yeti_batch_remove(yeti_batch *b, yeti_ir *s) {
+    if (!b || !s) return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: null argument");
+    for (size_t k = 0; k < b->members.size(); ++k)
+        if (b->members[k] == s) {
+            b->members.erase(b->members.begin() + k);
+            b->hreq.erase(b->hreq.begin() + k);
+            b->dirty = true;
+            return 0;
+        }
+    return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: not a member");
+}
+
+int32_t yeti_batch_size(const yeti_batch *b) { return b ? (int32_t)b->members.size() : -1; }
+
+int32_t yeti_batch_step(yeti_batch *b, int32_t use_graph, void *stream) {
+    if (!b) return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: null handle");
+    hipStream_t st = (hipStream_t)stream;
+    if (b->dirty) {
+        int rc = batch_upload(b, st);
+        if (rc) return rc;
+    }
+    if (b->members.empty()) return 0;
+    const BatchDev bd = b->dev();
+    auto launches = [&]() {
+        return b->prec64 ? batch_step_launches<double>(bd, 1, b->chunk, st)
+                         : batch_step_launches<float>(bd, 0, b->chunk, st);
+    };
+    if (!use_graph) return launches();
+    if (!st) return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: graph steps need a non-null stream");
+    if (!b->gexec) {
+        // the step reads every member and count from the device tables, so one capture serves
+        // every later membership
+        YETI_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = launches();
+        hipGraph_t g = nullptr;
+        hipError_t ec = hipStreamEndCapture(st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess) return set_error(YETI_ERR_DEVICE, "batch: capture failed: %s", hipGetErrorString(ec));
+        hipGraphExec_t ex = nullptr;
+        hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return set_error(YETI_ERR_DEVICE, "batch: graph instantiation failed: %s", hipGetErrorString(ei));
+        }
+        b->graph = g;
+        b->gexec = ex;
+    }
+    YETI_HIP(hipGraphLaunch(b->gexec, st));
+    return 0;
+}
+
+int32_t yeti_batch_status_async(yeti_batch *b, yeti_ir_status *host_status, void *stream) {
+    if (!b || !host_status) return set_error(YETI_ERR_INVALID_ARGUMENT, "batch: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    if (b->dirty) {
+        int rc = batch_upload(b, st);
+        if (rc) return rc;
+    }
+    if (b->members.empty()) return 0;
+    hipLaunchKernelGGL(bt_status_kernel, dim3(64), dim3(64), 0, st, b->dev(), b->dstat.p);
+    YETI_HIP(hipGetLastError());
+    YETI_HIP(hipMemcpyAsync(host_status, b->dstat.p, b->members.size() * sizeof(yeti_ir_status), hipMemcpyDeviceToHost,
+                            st));
+    return 0;
+}
+
+void yeti_batch_destroy(yeti_batch *b) { delete b; }
 
 }  // extern "C"

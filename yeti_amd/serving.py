@@ -39,6 +39,7 @@ from .geometry import make_box_patch
 
 ALIGN = 256
 NRED = 1024
+BATCH_MAX_MEMBERS = 256   # members of one batch (the library's table size)
 STATUS_BYTES = 8 * 4 + 16 * 8     # yeti_ir_status
 
 
@@ -220,7 +221,7 @@ def solve_session(mf, F, mask, precision=64, precond="fdm", rtol=1e-10, delta=0.
 # ------------------------------------------------------------------------------------------
 class _Active:
     __slots__ = ("req", "off", "nbytes", "asm", "mf", "ir", "F", "x", "mask", "stream", "event", "status",
-                 "t_admit", "steps", "keep")
+                 "t_admit", "steps", "keep", "in_batch")
 
 
 class SolveServer:
@@ -231,9 +232,13 @@ class SolveServer:
     to about `nstreams` before the HIP runtime starts for the sessions to overlap."""
 
     def __init__(self, budget_bytes, precision="mixed", nstreams=16, chunk=16, rtol=1e-8, delta=0.1,
-                 max_iters=100000, max_updates=400, use_graph=True, prep_threads=2, precond="fdm"):
+                 max_iters=100000, max_updates=400, use_graph=True, prep_threads=2, precond="fdm", batched=False,
+                 task_capacity=1 << 20):
         import torch
         _lib.require_device()
+        self.batched = bool(batched)
+        if self.batched and precond != "fdm":
+            raise ValueError("batched serving runs the fast-diagonalisation preconditioner")
         if precision not in ("mixed", "fp64"):
             raise ValueError("precision must be 'mixed' or 'fp64'")
         self.torch = torch
@@ -251,6 +256,13 @@ class SolveServer:
         self.alloc = ArenaAllocator(self.budget)
         self.streams = [torch.cuda.Stream() for _ in range(int(nstreams))]
         self.prep_threads = int(prep_threads)
+        self.batch = None
+        if self.batched:
+            h = C.c_void_p()
+            _lib.check(_lib.lib().yeti_batch_create(self.inner, self.chunk, int(task_capacity), C.byref(h)))
+            self.batch = h
+            self.batch_status = torch.empty((BATCH_MAX_MEMBERS, STATUS_BYTES), dtype=torch.uint8, pin_memory=True)
+            self.batch_event = torch.cuda.Event()
 
     # -- footprint ---------------------------------------------------------------------------
     def footprint(self, req, inputs):
@@ -321,6 +333,9 @@ class SolveServer:
         except Exception:
             self._release(a)
             raise
+        if self.batched:
+            _lib.check(L.yeti_batch_add(self.batch, a.ir))
+            return a
         a.status = torch.empty(STATUS_BYTES, dtype=torch.uint8, pin_memory=True)
         a.event = torch.cuda.Event()
         self._step(a)
@@ -334,6 +349,13 @@ class SolveServer:
         a.event.record(a.stream)
         a.steps += 1
 
+    @staticmethod
+    def _parse_status(buf):
+        fl = np.frombuffer(buf[:32].tobytes(), dtype=np.int32)
+        sc = np.frombuffer(buf[32:].tobytes(), dtype=np.float64)
+        return {"done": int(fl[3]), "failed": int(fl[4]), "floor": int(fl[6]), "iters": int(fl[5]),
+                "updates": int(fl[7]), "relres": math.sqrt(sc[5] / sc[4]) if sc[4] > 0 else 0.0}
+
     def _read_status(self, a):
         buf = a.status.numpy()
         fl = np.frombuffer(buf[:32].tobytes(), dtype=np.int32)
@@ -343,6 +365,9 @@ class SolveServer:
 
     def _release(self, a):
         L = _lib.lib()
+        if self.batched and getattr(a, "ir", None) is not None and getattr(a, "in_batch", False):
+            _lib.check(L.yeti_batch_remove(self.batch, a.ir))
+            a.in_batch = False
         # the handles live in the arena: deleting them frees no device memory
         if getattr(a, "ir", None) is not None:
             L.yeti_ir_destroy(a.ir)
@@ -361,6 +386,8 @@ class SolveServer:
         (closed-loop throughput). Returns a report dict; with keep_solutions, report["x"][rid]
         holds each solution on the host. progress(done, total) is called as requests complete;
         step_log(rid, status) every `step_every` steps of a request."""
+        if self.batched:
+            return self._run_batched(requests, keep_solutions, realtime, progress)
         torch = self.torch
         reqs = sorted(requests, key=lambda r: r.arrival)
         ready = {}
@@ -455,5 +482,107 @@ class SolveServer:
             rep["x"] = xs
         return rep
 
+    def _run_batched(self, requests, keep_solutions, realtime, progress):
+        """The loop of a batched server: admitted requests join one batch of sessions, each step
+        advances every member by `chunk` iterations with shared launches (one graph), one status
+        copy for the whole batch, then completed members leave and newly arrived ones join."""
+        torch = self.torch
+        L = _lib.lib()
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        ready = {}
+        lock = threading.Lock()
+        stop = [False]
+
+        def prep(chunk):
+            for r in chunk:
+                if stop[0]:
+                    return
+                r.model()
+                with lock:
+                    ready[r.rid] = True
+
+        threads = []
+        for k in range(max(1, self.prep_threads)):
+            th = threading.Thread(target=prep, args=(reqs[k::max(1, self.prep_threads)],), daemon=True)
+            th.start()
+            threads.append(th)
+        stream = self.streams[0]
+        sh = C.c_void_p(stream.cuda_stream)
+        pending = deque(reqs)
+        members = []            # in batch order
+        done = {}
+        xs = {}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        try:
+            while pending or members:
+                now = time.perf_counter() - t0
+                # admission, first come first served, while the arena has room
+                while pending and (not realtime or pending[0].arrival <= now) and len(members) < BATCH_MAX_MEMBERS:
+                    r = pending[0]
+                    with lock:
+                        ok = ready.get(r.rid, False)
+                    if not ok:
+                        break
+                    a = self._admit(r, stream, keep_solutions)
+                    if a is None:
+                        if not members:
+                            raise RuntimeError("request %d needs more than the whole budget" % r.rid)
+                        break
+                    a.in_batch = True
+                    a.steps = 0
+                    pending.popleft()
+                    members.append(a)
+                if not members:
+                    if pending and realtime:
+                        time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.01)))
+                    else:
+                        time.sleep(20e-6)
+                    continue
+                _lib.check(L.yeti_batch_step(self.batch, 1 if self.use_graph else 0, sh))
+                _lib.check(L.yeti_batch_status_async(self.batch, C.c_void_p(self.batch_status.data_ptr()), sh))
+                self.batch_event.record(stream)
+                self.batch_event.synchronize()
+                buf = self.batch_status.numpy()
+                still = []
+                for k, a in enumerate(members):
+                    a.steps += 1
+                    st = self._parse_status(buf[k])
+                    if st["done"]:
+                        t_end = time.perf_counter() - t0
+                        arr = a.req.arrival if realtime else 0.0
+                        done[a.req.rid] = dict(latency=t_end - arr, finish=t_end, steps=a.steps, p=a.req.p,
+                                               nel=a.req.nel, ndof=3 * a.req.nb_cp, **st)
+                        if a.keep:
+                            xs[a.req.rid] = a.x.cpu().numpy().copy()
+                        self._release(a)
+                        if progress:
+                            progress(len(done), len(reqs))
+                    else:
+                        still.append(a)
+                members = still
+            torch.cuda.synchronize()
+            makespan = time.perf_counter() - t0
+        finally:
+            stop[0] = True
+            for a in members:
+                try:
+                    stream.synchronize()
+                    self._release(a)
+                except Exception:
+                    pass
+        lat = [d["latency"] for d in done.values()]
+        rep = {"completed": len(done), "makespan_s": makespan,
+               "solves_per_s": len(done) / makespan if makespan > 0 else None,
+               "latency_s": latency_summary(lat), "failed": sum(1 for d in done.values() if d["failed"]),
+               "at_fp64_floor": sum(1 for d in done.values() if d["floor"]),
+               "peak_arena_bytes": self.alloc.peak, "budget_bytes": self.budget, "requests": done}
+        if keep_solutions:
+            rep["x"] = xs
+        return rep
+
     def close(self):
+        if self.batch is not None:
+            _lib.lib().yeti_batch_destroy(self.batch)
+            self.batch = None
         self.arena_buf = None
