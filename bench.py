@@ -97,6 +97,11 @@ def read_traffic(path):
 
 
 def bench_serving(n, rate, seed, budget, plain_n):
+    """The serving workload: a Poisson stream of n requests (seeded random patches, 4-48
+    elements per direction, degree 2-4) at `rate` per second, served by the batched server
+    (every active request advanced by shared launches) in mixed precision and in FP64, both to
+    1e-8 on the FP64 residual; the plain FP64 GPU path (each request alone, eager launches, one
+    after the other) on the first plain_n requests."""
     import numpy as np
     import torch
     from yeti_amd.serving import SolveServer, request_stream
@@ -109,21 +114,27 @@ def bench_serving(n, rate, seed, budget, plain_n):
     out = {}
     xs = {}
     for prec in ("mixed", "fp64"):
-        srv = SolveServer(budget_bytes=budget, precision=prec, nstreams=16, chunk=16, rtol=1e-8)
-        rep = srv.run(reqs, keep_solutions=True, realtime=True)
+        srv = SolveServer(budget_bytes=budget, precision=prec, chunk=16, rtol=1e-8, batched=True)
+        t1 = time.perf_counter()
+        rep = srv.run(reqs, keep_solutions=True, realtime=True,
+                      progress=lambda d, t: (d % 250 == 0) and log("  serving %s: %d / %d done, %.1f s"
+                                                                   % (prec, d, t, time.perf_counter() - t1)))
         xs[prec] = rep.pop("x")
+        its = [d["iters"] for d in rep["requests"].values()]
         out[prec] = {"solves_per_s": rep["solves_per_s"], "latency_s": rep["latency_s"],
-                     "completed": rep["completed"], "failed": rep["failed"],
-                     "mean_iterations": float(np.mean([d["iters"] for d in rep["requests"].values()])),
-                     "peak_arena_gb": rep["peak_arena_bytes"] / 1e9}
-        log("serving %s: %.2f solves/s, p50 %.3f s, p99 %.3f s, failed %d"
-            % (prec, rep["solves_per_s"], rep["latency_s"]["p50"], rep["latency_s"]["p99"], rep["failed"]))
+                     "completed": rep["completed"], "failed": rep["failed"], "at_fp64_floor": rep["at_fp64_floor"],
+                     "mean_iterations": float(np.mean(its)), "makespan_s": rep["makespan_s"],
+                     "peak_arena_gb": rep["peak_arena_bytes"] / 1e9, "batch_steps": rep["steps"],
+                     "host_admission_s": rep["host_admission_s"], "step_wait_s": rep["step_wait_s"]}
+        log("serving %s: %.2f solves/s, p50 %.3f s, p99 %.3f s, failed %d, mean iterations %.0f"
+            % (prec, rep["solves_per_s"], rep["latency_s"]["p50"], rep["latency_s"]["p99"], rep["failed"],
+               np.mean(its)))
         srv.close()
         del srv
         torch.cuda.empty_cache()
     err = max(float(np.linalg.norm(xs["mixed"][k] - xs["fp64"][k]) / np.linalg.norm(xs["fp64"][k])) for k in xs["fp64"])
     # plain FP64 GPU path: each request alone, one after the other, eager launches, one stream
-    sub = [r for r in reqs[:plain_n or len(reqs)]]
+    sub = reqs[:plain_n]
     lat = []
     t0 = time.perf_counter()
     for r in sub:
@@ -135,15 +146,17 @@ def bench_serving(n, rate, seed, budget, plain_n):
         del srv
     dt = time.perf_counter() - t0
     out["plain_fp64"] = {"solves_per_s": len(sub) / dt, "requests": len(sub),
-                         "latency_s": {"p50": float(np.median(lat)), "max": float(max(lat))}}
+                         "latency_s": {"p50": float(np.median(lat)), "p99": float(np.percentile(lat, 99)),
+                                       "max": float(max(lat))}}
     log("plain fp64 (one at a time, %d requests): %.2f solves/s" % (len(sub), len(sub) / dt))
     out["max_rel_err_mixed_vs_fp64"] = err
     out["speedup_mixed_vs_plain_fp64"] = out["mixed"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]
     out["speedup_mixed_vs_served_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
     out["workload"] = ("%d requests, Poisson arrivals at %.1f/s, elements per direction log-uniform in [4,48], "
                        "p in {2,3,4}, E~U[1e3,1e5], nu~U[0.2,0.4], traction~U[0.1,10], solved to 1e-8 "
-                       "(FP64 residual), fast-diagonalisation PCG, arena %.0f GB, 16 streams; plain FP64: the first %d "
-                       "requests solved alone, one after the other, eager launches" % (n, rate, budget / 1e9, len(sub)))
+                       "(FP64 residual), fast-diagonalisation PCG, arena %.0f GB; served: one batch of sessions "
+                       "advanced by shared launches (graph per 16 iterations); plain FP64: the first %d requests "
+                       "solved alone, one after the other, eager launches" % (n, rate, budget / 1e9, len(sub)))
     return out
 
 
@@ -474,10 +487,10 @@ def main():
     ap.add_argument("--cpu-runs", type=int, default=3, help="timed CPU calls (median) after one warm-up")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = every core of this host")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=48)
-    ap.add_argument("--serve-rate", type=float, default=20.0, help="offered requests per second")
-    ap.add_argument("--serve-plain", type=int, default=0,
-                    help="requests solved one at a time (plain FP64); 0: all of them")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=25.0, help="offered requests per second")
+    ap.add_argument("--serve-plain", type=int, default=24,
+                    help="requests solved one at a time (plain FP64), from the start of the stream")
     ap.add_argument("--grad-fd", type=int, default=50, help="coordinates checked by central differences")
     ap.add_argument("--jacobi-iters", type=int, default=60000, help="bound on the C2 Jacobi PCG iterations")
     ap.add_argument("--mp-nel", type=int, default=128, help="elements per direction of the mixed-precision solve")

@@ -280,7 +280,7 @@ class SolveServer:
         return parts
 
     # -- admission -----------------------------------------------------------------------
-    def _admit(self, req, stream, keep):
+    def _admit(self, req, stream, keep, join=True):
         torch = self.torch
         L = _lib.lib()
         args = req.model()
@@ -334,7 +334,9 @@ class SolveServer:
             self._release(a)
             raise
         if self.batched:
-            _lib.check(L.yeti_batch_add(self.batch, a.ir))
+            if join:
+                _lib.check(L.yeti_batch_add(self.batch, a.ir))
+                a.in_batch = True
             return a
         a.status = torch.empty(STATUS_BYTES, dtype=torch.uint8, pin_memory=True)
         a.event = torch.cuda.Event()
@@ -483,9 +485,11 @@ class SolveServer:
         return rep
 
     def _run_batched(self, requests, keep_solutions, realtime, progress):
-        """The loop of a batched server: admitted requests join one batch of sessions, each step
-        advances every member by `chunk` iterations with shared launches (one graph), one status
-        copy for the whole batch, then completed members leave and newly arrived ones join."""
+        """The loop of a batched server: admitted requests join one batch of sessions; each step
+        advances every member by `chunk` iterations with shared launches (one graph) and copies
+        the whole batch's status once. While a step runs on the GPU the host admits arrived
+        requests on a second stream (set-up, right-hand side, first direction); they join the
+        batch after the step, when completed members leave it."""
         torch = self.torch
         L = _lib.lib()
         reqs = sorted(requests, key=lambda r: r.arrival)
@@ -507,42 +511,69 @@ class SolveServer:
             th.start()
             threads.append(th)
         stream = self.streams[0]
+        adm_stream = self.streams[1] if len(self.streams) > 1 else torch.cuda.Stream()
         sh = C.c_void_p(stream.cuda_stream)
         pending = deque(reqs)
         members = []            # in batch order
+        joining = []            # admitted during the running step
         done = {}
         xs = {}
+        t_admit = t_wait = 0.0
+        nsteps = 0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+
+        def admit_arrived(deadline_event=None, limit=BATCH_MAX_MEMBERS):
+            # admit while requests have arrived, fit, and (when a step runs) until it ends
+            nonlocal t_admit
+            n_admitted = 0
+            while pending and (not realtime or pending[0].arrival <= time.perf_counter() - t0) and \
+                    len(members) + len(joining) < BATCH_MAX_MEMBERS and n_admitted < limit:
+                if deadline_event is not None and deadline_event.query() and (members or joining):
+                    return
+                n_admitted += 1
+                r = pending[0]
+                with lock:
+                    ok = ready.get(r.rid, False)
+                if not ok:
+                    return
+                ta = time.perf_counter()
+                a = self._admit(r, adm_stream, keep_solutions, join=False)
+                t_admit += time.perf_counter() - ta
+                if a is None:
+                    if not members and not joining:
+                        raise RuntimeError("request %d needs more than the whole budget" % r.rid)
+                    return
+                a.steps = 0
+                pending.popleft()
+                joining.append(a)
+
         try:
-            while pending or members:
-                now = time.perf_counter() - t0
-                # admission, first come first served, while the arena has room
-                while pending and (not realtime or pending[0].arrival <= now) and len(members) < BATCH_MAX_MEMBERS:
-                    r = pending[0]
-                    with lock:
-                        ok = ready.get(r.rid, False)
-                    if not ok:
-                        break
-                    a = self._admit(r, stream, keep_solutions)
-                    if a is None:
-                        if not members:
-                            raise RuntimeError("request %d needs more than the whole budget" % r.rid)
-                        break
-                    a.in_batch = True
-                    a.steps = 0
-                    pending.popleft()
-                    members.append(a)
+            while pending or members or joining:
+                if joining:
+                    adm_stream.synchronize()
+                    for a in joining:
+                        _lib.check(L.yeti_batch_add(self.batch, a.ir))
+                        a.in_batch = True
+                    members.extend(joining)
+                    joining = []
                 if not members:
-                    if pending and realtime:
-                        time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.01)))
-                    else:
-                        time.sleep(20e-6)
+                    # a first group, then the rest while the GPU steps
+                    admit_arrived(limit=8)
+                    if not joining:
+                        if pending and realtime:
+                            time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.01)))
+                        else:
+                            time.sleep(20e-6)
                     continue
+                tw = time.perf_counter()
                 _lib.check(L.yeti_batch_step(self.batch, 1 if self.use_graph else 0, sh))
                 _lib.check(L.yeti_batch_status_async(self.batch, C.c_void_p(self.batch_status.data_ptr()), sh))
                 self.batch_event.record(stream)
+                admit_arrived(self.batch_event)
                 self.batch_event.synchronize()
+                t_wait += time.perf_counter() - tw
+                nsteps += 1
                 buf = self.batch_status.numpy()
                 still = []
                 for k, a in enumerate(members):
@@ -565,9 +596,9 @@ class SolveServer:
             makespan = time.perf_counter() - t0
         finally:
             stop[0] = True
-            for a in members:
+            torch.cuda.synchronize()
+            for a in members + joining:
                 try:
-                    stream.synchronize()
                     self._release(a)
                 except Exception:
                     pass
@@ -576,7 +607,8 @@ class SolveServer:
                "solves_per_s": len(done) / makespan if makespan > 0 else None,
                "latency_s": latency_summary(lat), "failed": sum(1 for d in done.values() if d["failed"]),
                "at_fp64_floor": sum(1 for d in done.values() if d["floor"]),
-               "peak_arena_bytes": self.alloc.peak, "budget_bytes": self.budget, "requests": done}
+               "peak_arena_bytes": self.alloc.peak, "budget_bytes": self.budget, "requests": done,
+               "host_admission_s": t_admit, "step_wait_s": t_wait, "steps": nsteps}
         if keep_solutions:
             rep["x"] = xs
         return rep
