@@ -28,8 +28,9 @@ ROOT = os.path.dirname(HERE)
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-from yeti_amd._abi import (YETI_LAYOUT_PACKED, YETI_LAYOUT_SLOTTED,  # noqa: E402
-                           args_to_dict, pack_system_inputs)
+from oracle.inputs import YETI_LAYOUT_PACKED, YETI_LAYOUT_SLOTTED  # noqa: E402
+from oracle.inputs import pack as pack_system_inputs  # noqa: E402
+from oracle.inputs import to_dict as args_to_dict  # noqa: E402
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
@@ -324,16 +325,15 @@ def ref_reconstruction(u_inv, coords, ind_dof_free, bc_target, bc_target_nbelem,
 def oracle_evaldisp(numpatch, xi_points, sol, IEN, elementsByPatch, Nkv, Ukv, Nijk, weight, Jpqr, NNODE, mcrd=3):
     """liboracle's restatement of evalDisp (evaluate_disp.f:20-103) at many points:
     xi_points (npts, 3) -> (npts, 3); sol (mcrd, nb_cp)."""
-    from yeti_amd.postproc import _patch
-    s, keep = _patch(numpatch, IEN, elementsByPatch, Nkv, Ukv, Nijk, weight, Jpqr, NNODE)
-    p = np.array(list(s.p), np.int32)
-    nkv = np.array(list(s.nkv), np.int32)
+    from oracle.inputs import patch_slice
+    keep = patch_slice(numpatch, IEN, elementsByPatch, Nkv, Ukv, Nijk, weight, Jpqr, NNODE)
+    p, nkv = keep["p"], keep["nkv"]
     S = np.asfortranarray(np.asarray(sol, np.float64).reshape(mcrd, -1, order="F"))
     X = np.ascontiguousarray(np.asarray(xi_points, np.float64).reshape(-1, 3))
     out = np.zeros((X.shape[0], 3))
     U = keep["U"]
     rc = lib().oracle_evaldisp(_p(p, _i32p), _p(nkv, _i32p), _p(U[0], _f64p), _p(U[1], _f64p), _p(U[2], _f64p),
-                               int(s.nelem), int(s.nnode), _p(keep["nijk"], _i32p), _p(keep["ien"], _i32p),
+                               int(keep["nelem"]), int(keep["nnode"]), _p(keep["nijk"], _i32p), _p(keep["ien"], _i32p),
                                _p(keep["w"], _f64p), int(mcrd), _p(S, _f64p), X.shape[0], _p(X, _f64p),
                                _p(out, _f64p))
     if rc:
