@@ -288,6 +288,66 @@ def bench_gradient(nfd, seed=7):
             "max_abs_err_vs_fd_rel_to_max_grad": max(errs), "pcg_iterations": patch.iters}
 
 
+def bench_mf_apply(mf, nel, nb_elem, nrep=20):
+    """The matrix-free apply alone at degree 2: DOF-applies/s and HBM roofline (algorithmic bytes:
+    the 13 stored geometry values per Gauss point + x read + y written, in the apply's precision)."""
+    import torch
+    out = {}
+    nqp = nb_elem * 27
+    stream = torch.cuda.current_stream()
+    for name, dtype, sz in (("fp32", torch.float32, 4), ("fp64", torch.float64, 8)):
+        x = torch.randn(mf.ndof, dtype=dtype, device="cuda")
+        y = torch.empty_like(x)
+        mf.apply(x, y)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(nrep):
+            mf.apply(x, y)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / nrep
+        byts = 13 * sz * nqp + 2 * sz * mf.ndof
+        gbs = byts / (ms * 1e-3) / 1e9
+        out["apply_" + name] = {"ms": ms, "dof_applies_per_s": mf.ndof / (ms * 1e-3),
+                                "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                             "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": byts}}
+        log("%d^3 p=2 apply %s: %.3f ms, %.3e DOF-applies/s, %.0f GB/s algorithmic"
+            % (nel, name, ms, mf.ndof / (ms * 1e-3), gbs))
+        del x, y
+    return out
+
+
+def bench_apply(nel):
+    """The matrix-free apply at nel^3, degree 2, and a bounded mixed-precision solve on the same
+    operator (apply + fast-diagonalisation preconditioner per iteration)."""
+    import torch
+    from yeti_amd.geometry import make_box_patch
+    from yeti_amd.matfree import MatrixFree
+    from yeti_amd.serving import solve_session
+    from yeti_amd.solver import free_mask_from_args
+    from yeti_amd.assembly import Assembler
+    bp = make_box_patch(nel=(nel, nel, nel), p=2)
+    asm = Assembler(bp.args)
+    F = asm.run_rhs()
+    asm.close()
+    mask = free_mask_from_args(bp.args)
+    mf = MatrixFree(bp.args)
+    out = bench_mf_apply(mf, nel, bp.nb_elem)
+    for prec in (32, 64):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        x, st = solve_session(mf, F, mask, precision=prec, rtol=1e-10, max_iters=200)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        out["solve_%d_200" % prec] = {"seconds": dt, "iterations": st["iters"],
+                                      "ms_per_iteration": dt * 1e3 / max(1, st["iters"])}
+        log("%d^3 p=2 %s solve, %d iterations: %.2f ms per iteration"
+            % (nel, "mixed" if prec == 32 else "fp64", st["iters"], dt * 1e3 / max(1, st["iters"])))
+    mf.close()
+    return out
+
+
 def bench_mixed_precision(nel, jacobi_cap):
     """The precision workload (128^3 p=2): the FP64 oracle is the assembled matrix (device COO ->
     CSR of K + K^T) solved in FP64 with the fast-diagonalisation preconditioner to its FP64 floor;
@@ -367,34 +427,11 @@ def bench_mixed_precision(nel, jacobi_cap):
     csr.close()
     del csr
     torch.cuda.empty_cache()
-    # the matrix-free apply alone: DOF-applies/s and HBM roofline (algorithmic bytes: the 13
-    # stored geometry values per Gauss point + x read + y written, in the apply's precision)
-    nqp = bp.nb_elem * 27
-    stream = torch.cuda.current_stream()
-    for name, dtype, sz in (("fp32", torch.float32, 4), ("fp64", torch.float64, 8)):
-        x = torch.randn(mf.ndof, dtype=dtype, device="cuda")
-        y = torch.empty_like(x)
-        mf.apply(x, y)
-        torch.cuda.synchronize()
-        nrep = 20
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(nrep):
-            mf.apply(x, y)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / nrep
-        byts = 13 * sz * nqp + 2 * sz * mf.ndof
-        gbs = byts / (ms * 1e-3) / 1e9
-        out["apply_" + name] = {"ms": ms, "dof_applies_per_s": mf.ndof / (ms * 1e-3),
-                                "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                             "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": byts}}
-        log("%d^3 p=2 apply %s: %.3f ms, %.3e DOF-applies/s, %.0f GB/s algorithmic"
-            % (nel, name, ms, mf.ndof / (ms * 1e-3), gbs))
-        del x, y
+    out.update(bench_mf_apply(mf, nel, bp.nb_elem))
     mf.close()
     del mf
     torch.cuda.empty_cache()
+    stream = torch.cuda.current_stream()
     # displacement reconstruction (reconstruction.f:22) of the mixed solution, on the device
     import ctypes as C
     from yeti_amd import _lib
@@ -498,9 +535,11 @@ def main():
                     help="bound on the FP32 block-Jacobi iterations of the mixed-precision section")
     ap.add_argument("--only-assembly", action="store_true")
     ap.add_argument("--sections", default="c2,serving,gradient,mixed",
-                    help="comma list of the parts run after the assembly: c2, serving, gradient, mixed")
-    ap.add_argument("--pmc-probe", action="store_true",
-                    help="run only a 2 GiB device copy and one element-kernel run (for rocprofv3 --pmc)")
+                    help="comma list of the parts run after the assembly: c2, serving, gradient, mixed, apply "
+                         "(the matrix-free apply and bounded solves alone, at --mp-nel)")
+    ap.add_argument("--pmc-probe", nargs="?", const="elem", choices=("elem", "mf"),
+                    help="run only a 2 GiB device copy, then one element-kernel run (elem) or one FP32 and one "
+                         "FP64 matrix-free apply at --mp-nel, degree 2 (mf), for rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: one GPU only (the assembly would shard by contiguous element ranges, "
@@ -520,11 +559,22 @@ def main():
     nel = (args.nel, args.nel, args.nel)
     if args.pmc_probe:
         # calibration copy of known size (2 GiB read + 2 GiB written), then one run of the
-        # element kernels on the bench workload; counters per dispatch come from rocprofv3
+        # measured kernels; counters per dispatch come from rocprofv3
         a = torch.ones(2 << 30, dtype=torch.uint8, device="cuda")
         b = torch.empty_like(a)
         b.copy_(a)
         torch.cuda.synchronize()
+        if args.pmc_probe == "mf":
+            from yeti_amd.matfree import MatrixFree
+            bp = make_box_patch(nel=(args.mp_nel,) * 3, p=2)
+            mf = MatrixFree(bp.args)
+            for dt in (torch.float32, torch.float64):
+                x = torch.randn(mf.ndof, dtype=dt, device="cuda")
+                mf.apply(x, torch.empty_like(x))
+            torch.cuda.synchronize()
+            log("pmc probe done: copy of %d bytes, matrix-free apply of %d elements" % (a.numel(), bp.nb_elem))
+            mf.close()
+            return
         bp = make_box_patch(nel=nel, p=p)
         asm = Assembler(bp.f2py_args())
         K, r, c = asm.alloc_coo(True)
@@ -684,6 +734,8 @@ def main():
         out["serving"] = bench_serving(args.serve_requests, args.serve_rate, 20261202, 200 << 30, args.serve_plain)
     if "gradient" in sections:
         out["gradient"] = bench_gradient(args.grad_fd)
+    if "apply" in sections:
+        out["matfree_apply"] = bench_apply(args.mp_nel)
     if "mixed" in sections:
         out["mixed_precision"] = bench_mixed_precision(args.mp_nel, args.mp_jacobi_iters)
     print(json.dumps(out), flush=True)

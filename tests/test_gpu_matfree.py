@@ -24,7 +24,8 @@ def oracle_K(args):
     return (A + A.T).tocsr(), F
 
 
-@pytest.mark.parametrize("nel,p", [((5, 4, 3), 2), ((4, 3, 3), 3), ((3, 2, 2), 4), ((4, 3, 2), 1), ((13, 7, 9), 2)])
+@pytest.mark.parametrize("nel,p", [((5, 4, 3), 2), ((4, 3, 3), 3), ((3, 2, 2), 4), ((4, 3, 2), 1), ((13, 7, 9), 2),
+                                   ((3, 5, 10), 4), ((5, 3, 11), 3), ((9, 10, 17), 1)])
 def test_mf_apply_matches_assembled(gpu, nel, p):
     import torch
     from yeti_amd.matfree import MatrixFree

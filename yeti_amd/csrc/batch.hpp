@@ -14,6 +14,7 @@ struct MfPatch {
     const double *Btab;      // [d][e_d][NG][NP1] values (basis function span - p + r)
     const double *dBtab;     // same, derivatives
     int32_t boff[3];         // offset of direction d in the tables (in elements)
+    int32_t gnb[3];          // brick grid of the sweep apply, which also orders the stored geometry
     double lam, mu;
 };
 
@@ -61,8 +62,8 @@ struct BReq {
     // operator
     MfPatch mp;
     int32_t p, ng;
-    int32_t nbrick[3];       // apply brick grid
-    int32_t br, bn;          // apply brick edge in elements / nodes
+    int32_t nbrick[3];       // apply brick grid (the sweep's)
+    int32_t br[3], bn[3];    // apply brick edge per direction in elements / nodes
     int32_t ncp;
     int64_t n;               // 3 ncp
     const void *geo[2];      // quadrature geometry, [0] FP64, [1] FP32

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "mf_sweep.hpp"
 
 namespace yeti {
 
@@ -94,7 +95,7 @@ struct MfSmem {
     T w[C::EPB][C::NN];
 };
 
-// geometry of one patch: geo[e][c][q], e = e1 + N1 (e2 + N2 e3), q = g0 + NG (g1 + NG g2)
+// geometry of one patch, in the sweep apply's order (sweep_geo_off)
 template <int P, int NG>
 __global__ void mf_geometry_kernel(MfPatch mp, const double *__restrict__ coords, const double *__restrict__ wcp,
                                    const double *__restrict__ half, GaussRule g, double *__restrict__ geo64,
@@ -165,8 +166,9 @@ __global__ void mf_geometry_kernel(MfPatch mp, const double *__restrict__ coords
     out[10] = dw[1];
     out[11] = dw[2];
     out[12] = om;
+    const int64_t o0 = sweep_geo_off<P>(mp.gnb, e1, e2, e3, 0, g0, g1, g2);
     for (int c = 0; c < 13; ++c) {
-        int64_t o = (e * 13 + c) * C::NQ + q;
+        const int64_t o = o0 + (int64_t)c * Sweep<P>::NTQ;
         if (geo64) geo64[o] = out[c];
         if (geo32) geo32[o] = (float)out[c];
     }
@@ -311,13 +313,14 @@ __device__ __forceinline__ void mf_brick_body(MfSmem<T, P, NG, MODE == 1 ? 9 : 3
                         G[1][f] = d1;
                         G[2][f] = d2;
                     }
-                    const T *gq = geo + (e * 13) * NQ + r;
+                    constexpr int CS = Sweep<P>::NTQ;   // stride between geometry values
+                    const T *gq = geo + sweep_geo_off<P>(mp.gnb, e1, e2, e3, 0, g0, g1, g2);
                     T A[9], gw[3];
-                    for (int c = 0; c < 9; ++c) A[c] = gq[c * NQ];
-                    gw[0] = gq[9 * NQ];
-                    gw[1] = gq[10 * NQ];
-                    gw[2] = gq[11 * NQ];
-                    const T om = gq[12 * NQ];
+                    for (int c = 0; c < 9; ++c) A[c] = gq[c * CS];
+                    gw[0] = gq[9 * CS];
+                    gw[1] = gq[10 * CS];
+                    gw[2] = gq[11 * CS];
+                    const T om = gq[12 * CS];
                     // H[l][k] = d u_l / d x_k = sum_m (G[m][l] - gw[m] V[l]) A[m][k]
                     T H[3][3];
                     for (int l = 0; l < 3; ++l)
@@ -411,19 +414,20 @@ __device__ __forceinline__ void mf_brick_body(MfSmem<T, P, NG, MODE == 1 ? 9 : 3
                 if (live && r < NN) {
                     const int i = r % NP1, j = (r / NP1) % NP1, k = r / (NP1 * NP1);
                     const T w = sm.w[slot][r];
+                    constexpr int CS = Sweep<P>::NTQ;
                     for (int q = 0; q < NQ; ++q) {
                         const int g0 = q % NG, g1 = (q / NG) % NG, g2 = q / (NG * NG);
-                        const T *gq = geo + (e * 13) * NQ + q;
+                        const T *gq = geo + sweep_geo_off<P>(mp.gnb, e1, e2, e3, 0, g0, g1, g2);
                         const T Bq = B[0][g0][i] * B[1][g1][j] * B[2][g2][k];
                         const T dq[3] = {dB[0][g0][i] * B[1][g1][j] * B[2][g2][k], B[0][g0][i] * dB[1][g1][j] * B[2][g2][k],
                                          B[0][g0][i] * B[1][g1][j] * dB[2][g2][k]};
                         T gx[3];
                         for (int kk = 0; kk < 3; ++kk) {
                             T s = 0;
-                            for (int m = 0; m < 3; ++m) s += (dq[m] - Bq * gq[(9 + m) * NQ]) * gq[(3 * m + kk) * NQ];
+                            for (int m = 0; m < 3; ++m) s += (dq[m] - Bq * gq[(9 + m) * CS]) * gq[(3 * m + kk) * CS];
                             gx[kk] = w * s;
                         }
-                        const T om = gq[12 * NQ];
+                        const T om = gq[12 * CS];
                         const T nn2 = gx[0] * gx[0] + gx[1] * gx[1] + gx[2] * gx[2];
                         for (int kk = 0; kk < 3; ++kk)
                             for (int ll = 0; ll < 3; ++ll)
@@ -454,31 +458,32 @@ __global__ void __launch_bounds__(256) mf_brick_kernel(MfPatch mp, const T *__re
     mf_brick_body<T, P, NG, MODE>(sm, mp, geo, x, wcp, nbrick[0], nbrick[1], partial, blockIdx.x);
 }
 
-// y[global cp] (+)= sum over the bricks covering patch CP c (fixed order)
+// y[global cp] (+)= sum over the bricks covering patch CP t (fixed order); bricks of br[d]
+// elements cover bn[d] = br[d] + p control points along direction d, in a grid of nb[d].
 template <typename T, int NC>
-__device__ __forceinline__ void mf_gather_point(const MfPatch &mp, int32_t br, int32_t bn, int32_t nb0, int32_t nb1,
-                                                int32_t nb2, const T *__restrict__ partial, T *__restrict__ y,
+__device__ __forceinline__ void mf_gather_point(const MfPatch &mp, const int32_t br[3], const int32_t bn[3],
+                                                const int32_t nb[3], const T *__restrict__ partial, T *__restrict__ y,
                                                 int accumulate, int64_t t) {
     const int c1 = (int)(t % mp.ncp[0]), c2 = (int)((t / mp.ncp[0]) % mp.ncp[1]),
               c3 = (int)(t / ((int64_t)mp.ncp[0] * mp.ncp[1]));
     const int cc[3] = {c1, c2, c3};
-    const int nb[3] = {nb0, nb1, nb2};
     int lo[3], hi[3];
     for (int d = 0; d < 3; ++d) {
         // brick b covers CPs [b br, b br + bn)
-        hi[d] = std::min(nb[d] - 1, cc[d] / br);
-        int l = (cc[d] - bn) / br + 1;
-        if (cc[d] - bn < 0) l = 0;
+        hi[d] = std::min(nb[d] - 1, cc[d] / br[d]);
+        int l = (cc[d] - bn[d]) / br[d] + 1;
+        if (cc[d] - bn[d] < 0) l = 0;
         lo[d] = std::max(0, l);
     }
+    const int64_t bvol = (int64_t)bn[0] * bn[1] * bn[2] * NC;
     T s[NC];
     for (int c = 0; c < NC; ++c) s[c] = T(0);
     for (int b3 = lo[2]; b3 <= hi[2]; ++b3)
         for (int b2 = lo[1]; b2 <= hi[1]; ++b2)
             for (int b1 = lo[0]; b1 <= hi[0]; ++b1) {
-                const int64_t brick = b1 + (int64_t)nb0 * (b2 + (int64_t)nb1 * b3);
-                const int n1 = c1 - b1 * br, n2 = c2 - b2 * br, n3 = c3 - b3 * br;
-                const T *src = partial + brick * (int64_t)(bn * bn * bn * NC) + ((n3 * bn + n2) * bn + n1) * NC;
+                const int64_t brick = b1 + (int64_t)nb[0] * (b2 + (int64_t)nb[1] * b3);
+                const int n1 = c1 - b1 * br[0], n2 = c2 - b2 * br[1], n3 = c3 - b3 * br[2];
+                const T *src = partial + brick * bvol + ((n3 * bn[1] + n2) * bn[0] + n1) * NC;
                 for (int c = 0; c < NC; ++c) s[c] += src[c];
             }
     const int cp = mp.cpmap[t];
@@ -488,15 +493,29 @@ __device__ __forceinline__ void mf_gather_point(const MfPatch &mp, int32_t br, i
     }
 }
 
+struct BrickGrid {
+    int32_t br[3], bn[3], nb[3];
+};
+
 template <typename T, int NC>
-__global__ void mf_gather_kernel(MfPatch mp, int32_t br, int32_t bn, int32_t nb0, int32_t nb1, int32_t nb2,
-                                 const T *__restrict__ partial, T *__restrict__ y, int accumulate,
-                                 const int32_t *__restrict__ pred) {
+__global__ void mf_gather_kernel(MfPatch mp, BrickGrid bg, const T *__restrict__ partial, T *__restrict__ y,
+                                 int accumulate, const int32_t *__restrict__ pred) {
     if (pred && *pred == 0) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t ncp = (int64_t)mp.ncp[0] * mp.ncp[1] * mp.ncp[2];
     if (t >= ncp) return;
-    mf_gather_point<T, NC>(mp, br, bn, nb0, nb1, nb2, partial, y, accumulate, t);
+    mf_gather_point<T, NC>(mp, bg.br, bg.bn, bg.nb, partial, y, accumulate, t);
+}
+
+// The apply as brick sweeps: one workgroup of Sweep<P>::NT threads per brick.
+template <typename T, int P>
+__global__ void __launch_bounds__(Sweep<P>::NT) mf_sweep_kernel(MfPatch mp, const T *__restrict__ geo,
+                                                                const T *__restrict__ x, const T *__restrict__ wcp,
+                                                                T *__restrict__ partial,
+                                                                const int32_t *__restrict__ pred) {
+    if (pred && *pred == 0) return;   // predicated launch (device-side solver control)
+    __shared__ SweepSmem<T, P> sm;
+    mf_sweep_body<T, P>(sm, mp, geo, x, wcp, partial, blockIdx.x);
 }
 
 template <typename T>
@@ -519,7 +538,8 @@ namespace yeti {
 struct MfPatchDev {
     int p, ng;
     MfPatch mp;
-    int32_t nbrick[2][3];                 // per mode (apply, diagonal)
+    int32_t nbrick[2][3];                 // coloured bricks: [0] strain-energy gradient, [1] diagonal blocks
+    int32_t sweep_br[3];                  // sweep brick edge per direction (the grid is mp.gnb)
     DevBuf<int32_t> cpmap, nbrick_dev;    // [2][3]
     DevBuf<double> Btab, dBtab, half, geo64;
     DevBuf<float> geo32;
@@ -559,16 +579,50 @@ static int launch_brick(const MfPatchDev &pd, const T *geo, const T *x, const T 
     return 0;
 }
 
+template <typename T, int P>
+static int launch_sweep(const MfPatchDev &pd, const T *geo, const T *x, const T *w, T *part, hipStream_t st,
+                        const int32_t *pred) {
+    const int64_t nb = (int64_t)pd.mp.gnb[0] * pd.mp.gnb[1] * pd.mp.gnb[2];
+    hipLaunchKernelGGL((mf_sweep_kernel<T, P>), dim3((unsigned)nb), dim3(Sweep<P>::NT), 0, st, pd.mp, geo, x, w, part,
+                       pred);
+    YETI_HIP(hipGetLastError());
+    return 0;
+}
+
+// MODE 0 (K x): brick sweeps; MODE 1 and 2: the element-wise coloured bricks.
+template <typename T, int MODE, int P>
+static int launch_mode(const MfPatchDev &pd, const T *geo, const T *x, const T *w, T *part, hipStream_t st,
+                       const int32_t *pred) {
+    if constexpr (MODE == 0) return launch_sweep<T, P>(pd, geo, x, w, part, st, pred);
+    else return launch_brick<T, P, P + 1, MODE>(pd, geo, x, w, part, st, pred);
+}
+
 template <typename T, int MODE>
 static int dispatch_brick(const MfPatchDev &pd, const T *geo, const T *x, const T *w, T *part, hipStream_t st,
                           const int32_t *pred) {
     switch (pd.p * 16 + pd.ng) {
-    case 1 * 16 + 2: return launch_brick<T, 1, 2, MODE>(pd, geo, x, w, part, st, pred);
-    case 2 * 16 + 3: return launch_brick<T, 2, 3, MODE>(pd, geo, x, w, part, st, pred);
-    case 3 * 16 + 4: return launch_brick<T, 3, 4, MODE>(pd, geo, x, w, part, st, pred);
-    case 4 * 16 + 5: return launch_brick<T, 4, 5, MODE>(pd, geo, x, w, part, st, pred);
+    case 1 * 16 + 2: return launch_mode<T, MODE, 1>(pd, geo, x, w, part, st, pred);
+    case 2 * 16 + 3: return launch_mode<T, MODE, 2>(pd, geo, x, w, part, st, pred);
+    case 3 * 16 + 4: return launch_mode<T, MODE, 3>(pd, geo, x, w, part, st, pred);
+    case 4 * 16 + 5: return launch_mode<T, MODE, 4>(pd, geo, x, w, part, st, pred);
     default: return set_error(YETI_ERR_UNSUPPORTED_DEGREE, "matrix-free: no kernel for p=%d ng=%d", pd.p, pd.ng);
     }
+}
+
+// The bricks whose partial sums MODE's kernel writes.
+static BrickGrid brick_grid(const MfPatchDev &pd, int mode) {
+    BrickGrid bg;
+    for (int d = 0; d < 3; ++d) {
+        if (mode == 0) {
+            bg.br[d] = pd.sweep_br[d];
+            bg.nb[d] = pd.mp.gnb[d];
+        } else {
+            bg.br[d] = mode == 1 ? pd.p + 1 : 2 * (pd.p + 1);
+            bg.nb[d] = pd.nbrick[mode == 1 ? 1 : 0][d];
+        }
+        bg.bn[d] = bg.br[d] + pd.p;
+    }
+    return bg;
 }
 
 template <typename T, int MODE>
@@ -596,11 +650,10 @@ static int mf_run(const yeti_mf *m, const T *x, T *y, hipStream_t st, const int3
         if (!geo || !part) return set_error(YETI_ERR_INVALID_ARGUMENT, "matrix-free: precision not prepared");
         int rc = dispatch_brick<T, MODE>(pd, geo, x, w, part, st, pred);
         if (rc) return rc;
-        const int br = MODE == 1 ? pd.p + 1 : 2 * (pd.p + 1), bn = br + pd.p;
-        const int32_t *nbk = pd.nbrick[MODE == 1 ? 1 : 0];
+        const BrickGrid bg = brick_grid(pd, MODE);
         int64_t ncp = (int64_t)pd.mp.ncp[0] * pd.mp.ncp[1] * pd.mp.ncp[2];
-        hipLaunchKernelGGL((mf_gather_kernel<T, NC>), dim3((unsigned)((ncp + 255) / 256)), dim3(256), 0, st, pd.mp, br,
-                           bn, nbk[0], nbk[1], nbk[2], part, y, (m->zero_first || ip > 0) ? 1 : 0, pred);
+        hipLaunchKernelGGL((mf_gather_kernel<T, NC>), dim3((unsigned)((ncp + 255) / 256)), dim3(256), 0, st, pd.mp, bg,
+                           part, y, (m->zero_first || ip > 0) ? 1 : 0, pred);
         YETI_HIP(hipGetLastError());
     }
     return 0;
@@ -1130,9 +1183,9 @@ void fdm_destroy(yeti_fdm *f) { delete f; }
 // Bricks of degree P of every member: the iteration apply p -> brick partials in the batch
 // precision T (active members), or (RPL) the FP64 apply of the replacement x -> FP64 partials
 // (members with a replacement due).
-template <typename T, int P, int NG, bool RPL>
-__global__ void __launch_bounds__(256) bt_brick_kernel(BatchDev bd, int list) {
-    __shared__ MfSmem<T, P, NG, 3, MfBrick<P, 0>::BN> sm;
+template <typename T, int P, bool RPL>
+__global__ void __launch_bounds__(Sweep<P>::NT) bt_sweep_kernel(BatchDev bd, int list) {
+    __shared__ SweepSmem<T, P> sm;
     constexpr int pi = sizeof(T) == 8 ? 0 : 1;
     const int32_t cnt = bd.count[list];
     const BTask *tl = bd.list[list];
@@ -1140,10 +1193,9 @@ __global__ void __launch_bounds__(256) bt_brick_kernel(BatchDev bd, int list) {
         const BTask tk = tl[t];
         const BReq &R = bd.req[tk.req];
         if (R.flags[RPL ? F_UPDATE : F_ACTIVE] == 0) continue;   // the same for every thread
-        __syncthreads();   // the previous task's copy-out has read the accumulator
+        __syncthreads();   // the previous task's last reads of the shared tables are done
         const T *x = RPL ? (const T *)R.x : (const T *)R.vp;
-        mf_brick_body<T, P, NG, 0>(sm, R.mp, (const T *)R.geo[pi], x, (const T *)R.w[pi], R.nbrick[0], R.nbrick[1],
-                                   (T *)R.part[pi], tk.a);
+        mf_sweep_body<T, P>(sm, R.mp, (const T *)R.geo[pi], x, (const T *)R.w[pi], (T *)R.part[pi], tk.a);
     }
 }
 
@@ -1160,7 +1212,7 @@ __global__ void __launch_bounds__(256) bt_gather_kernel(BatchDev bd) {
         const int64_t cp = (int64_t)tk.a * BATCH_GATHER_BLOCK + threadIdx.x;
         if (cp >= R.ncp) continue;
         T *y = RPL ? (T *)R.y64 : (T *)R.vq;
-        mf_gather_point<T, 3>(R.mp, R.br, R.bn, R.nbrick[0], R.nbrick[1], R.nbrick[2], (const T *)R.part[pi], y, 0, cp);
+        mf_gather_point<T, 3>(R.mp, R.br, R.bn, R.nbrick, (const T *)R.part[pi], y, 0, cp);
     }
 }
 
@@ -1232,10 +1284,10 @@ constexpr unsigned BT_GRID_BRICK = 1024, BT_GRID = 2048;
 
 template <typename T, bool RPL>
 static int batch_apply_t(const BatchDev &bd, hipStream_t st) {
-    hipLaunchKernelGGL((bt_brick_kernel<T, 1, 2, RPL>), dim3(BT_GRID_BRICK), dim3(256), 0, st, bd, (int)BT_BRICK1);
-    hipLaunchKernelGGL((bt_brick_kernel<T, 2, 3, RPL>), dim3(BT_GRID_BRICK), dim3(256), 0, st, bd, (int)BT_BRICK2);
-    hipLaunchKernelGGL((bt_brick_kernel<T, 3, 4, RPL>), dim3(BT_GRID_BRICK), dim3(256), 0, st, bd, (int)BT_BRICK3);
-    hipLaunchKernelGGL((bt_brick_kernel<T, 4, 5, RPL>), dim3(BT_GRID_BRICK), dim3(256), 0, st, bd, (int)BT_BRICK4);
+    hipLaunchKernelGGL((bt_sweep_kernel<T, 1, RPL>), dim3(BT_GRID_BRICK), dim3(Sweep<1>::NT), 0, st, bd, (int)BT_BRICK1);
+    hipLaunchKernelGGL((bt_sweep_kernel<T, 2, RPL>), dim3(BT_GRID_BRICK), dim3(Sweep<2>::NT), 0, st, bd, (int)BT_BRICK2);
+    hipLaunchKernelGGL((bt_sweep_kernel<T, 3, RPL>), dim3(BT_GRID_BRICK), dim3(Sweep<3>::NT), 0, st, bd, (int)BT_BRICK3);
+    hipLaunchKernelGGL((bt_sweep_kernel<T, 4, RPL>), dim3(BT_GRID_BRICK), dim3(Sweep<4>::NT), 0, st, bd, (int)BT_BRICK4);
     hipLaunchKernelGGL((bt_gather_kernel<T, RPL>), dim3(BT_GRID), dim3(BATCH_GATHER_BLOCK), 0, st, bd);
     YETI_HIP(hipGetLastError());
     return 0;
@@ -1272,9 +1324,11 @@ int batch_fill_operator(const yeti_mf *m, const yeti_fdm *f, int prec64, BReq *R
     R->mp = pd.mp;
     R->p = pd.p;
     R->ng = pd.ng;
-    for (int d = 0; d < 3; ++d) R->nbrick[d] = pd.nbrick[0][d];
-    R->br = 2 * (pd.p + 1);
-    R->bn = R->br + pd.p;
+    for (int d = 0; d < 3; ++d) {
+        R->nbrick[d] = pd.mp.gnb[d];
+        R->br[d] = pd.sweep_br[d];
+        R->bn[d] = pd.sweep_br[d] + pd.p;
+    }
     R->ncp = m->nb_cp;
     R->n = 3 * (int64_t)m->nb_cp;
     R->geo[0] = pd.geo64.p;
@@ -1305,6 +1359,24 @@ int batch_fill_operator(const yeti_mf *m, const yeti_fdm *f, int prec64, BReq *R
     }
     R->wout = wa;
     return 0;
+}
+
+// Sweep brick grid, brick edges, geometry size and partial-sum size of a degree-p patch.
+template <int P>
+static void sweep_dims_t(const int32_t N[3], int32_t gnb[3], int32_t br[3], int64_t &geo_n, int64_t &part_n) {
+    sweep_grid<P>(N, gnb);
+    br[0] = br[1] = Sweep<P>::BR;
+    br[2] = Sweep<P>::BZ;
+    geo_n = sweep_geo_size<P>(gnb);
+    part_n = (int64_t)gnb[0] * gnb[1] * gnb[2] * Sweep<P>::PART;
+}
+static void sweep_dims(int p, const int32_t N[3], int32_t gnb[3], int32_t br[3], int64_t &geo_n, int64_t &part_n) {
+    switch (p) {
+    case 1: sweep_dims_t<1>(N, gnb, br, geo_n, part_n); break;
+    case 2: sweep_dims_t<2>(N, gnb, br, geo_n, part_n); break;
+    case 3: sweep_dims_t<3>(N, gnb, br, geo_n, part_n); break;
+    default: sweep_dims_t<4>(N, gnb, br, geo_n, part_n); break;
+    }
 }
 
 }  // namespace yeti
@@ -1426,6 +1498,8 @@ static int32_t mf_create(const yeti_system_inputs *in, int32_t precisions, hipSt
             pd->nbrick[0][d] = (N[d] + 2 * np1 - 1) / (2 * np1);
             pd->nbrick[1][d] = (N[d] + np1 - 1) / np1;
         }
+        int64_t geo_n = 0, sweep_part_n = 0;
+        sweep_dims(p0, N, pd->mp.gnb, pd->sweep_br, geo_n, sweep_part_n);
         double E = in->MATERIAL_PROPERTIES[(int64_t)in->n_mat_props_max * ip];
         double nu = in->MATERIAL_PROPERTIES[(int64_t)in->n_mat_props_max * ip + 1];
         pd->mp.lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
@@ -1444,13 +1518,16 @@ static int32_t mf_create(const yeti_system_inputs *in, int32_t precisions, hipSt
         const int bn0 = 2 * np1 + p0, bn1 = np1 + p0;
         const int64_t nb0 = (int64_t)pd->nbrick[0][0] * pd->nbrick[0][1] * pd->nbrick[0][2];
         const int64_t nb1 = (int64_t)pd->nbrick[1][0] * pd->nbrick[1][1] * pd->nbrick[1][2];
-        const int64_t part_n = std::max(nb0 * bn0 * bn0 * bn0 * 3, nb1 * bn1 * bn1 * bn1 * 9);
-        // geometry in FP64 always (it seeds the FP32 copy), brick partials per precision
-        er = pd->geo64.alloc(nel * 13 * nq);
-        if (er == hipSuccess && (precisions & 2)) er = pd->geo32.alloc(nel * 13 * nq);
-        // the FP64 partials also serve the (FP64) diagonal-block pass
+        const int64_t part_n = std::max(sweep_part_n, std::max(nb0 * bn0 * bn0 * bn0 * 3, nb1 * bn1 * bn1 * bn1 * 9));
+        // geometry in FP64 always (it seeds the FP32 copy), in whole bricks of the sweep; brick
+        // partials per precision
+        (void)nel;
+        (void)nq;
+        er = pd->geo64.alloc(geo_n);
+        if (er == hipSuccess && (precisions & 2)) er = pd->geo32.alloc(geo_n);
+        // the FP64 partials also serve the (FP64) diagonal-block and strain-energy gradient passes
         if (er == hipSuccess) er = pd->part64.alloc(part_n);
-        if (er == hipSuccess && (precisions & 2)) er = pd->part32.alloc(nb0 * bn0 * bn0 * bn0 * 3);
+        if (er == hipSuccess && (precisions & 2)) er = pd->part32.alloc(sweep_part_n);
         if (er != hipSuccess) return fail(set_error(YETI_ERR_DEVICE, "matrix-free: allocation failed (out of memory?)"));
         if (!counting()) {
             const bool arena = alloc_ctx().arena != nullptr;

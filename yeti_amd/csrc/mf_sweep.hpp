@@ -1,0 +1,350 @@
+// mf_sweep.hpp -- the matrix-free apply y = K x as sum-factorised sweeps over bricks of elements
+// (included by mf.hip, after MfPatch).
+//
+// Same operator as the element-wise contraction of mf.hip (mf_iga_get_ku_3d,
+// iga_wq_mf/fsrc/mf_iga_solver.f90:292-555, generalised to 3-component NURBS elasticity), but the
+// 1D contractions run over a whole brick of BR x BR x BZ elements instead of one element at a
+// time: control points shared by neighbouring elements of the brick are contracted once, and the
+// sums over the elements that share a control point happen inside the backward contractions
+// instead of through a coloured scatter.
+//
+// A brick is swept through its node layers in z. For node layer n3 the brick's control points are
+// read once (w x, 3 components), contracted along x (stage A: nodes -> Gauss positions
+// a = e1 NG + g0, value and x-derivative) and along y (stage B: -> Gauss-point columns
+// (a, b = e2 NG + g1): value, x- and y-derivative of the 3 components, 9 numbers per column) into
+// a ring of the last P+1 node layers in LDS. Once the P+1 node layers of element layer e3 are in
+// the ring, each thread takes one Gauss point (column, g2) of the layer: the z contraction
+// (stage C), the quadrature-point work (the same 13 stored geometry numbers per point as the
+// element-wise kernel) and the transposed z contraction into its P+1 back-contraction registers
+// (one per node layer e3 .. e3+P, shifted down after every element layer). Node layer e3 is then
+// complete: its back-contractions are summed over g2 (lanes of one column are adjacent in a wave),
+// contracted back along y (B^T) and x (A^T), scaled by the weights and written to the brick's
+// partial sums; bricks overlap by P node planes, and a gather sums the bricks covering each
+// control point in a fixed order. No floating-point atomics: the apply is bit-reproducible.
+//
+// Threads: the NG Gauss points g2 of a column sit in adjacent lanes, CPW = floor(64 / NG) columns
+// per wave. The stored geometry is laid out per (brick, element layer, value) in that thread
+// order, so each value is one contiguous read per wave.
+#pragma once
+
+namespace yeti {
+
+template <int P>
+struct Sweep {
+    static constexpr int NP1 = P + 1, NG = P + 1;
+    static constexpr int BR = P == 1 ? 8 : 2;                  // brick edge in x and y (elements)
+    static constexpr int BZ = 8;                               // brick depth in z (elements)
+    static constexpr bool PREFETCH_GEO = false;                // next element layer's geometry held in registers (costs occupancy)
+    static constexpr int BN = BR + P;                          // control points per brick row
+    static constexpr int NZ = BZ + P;                          // node layers per brick
+    static constexpr int NA = BR * NG;                         // Gauss positions per brick row
+    static constexpr int NCOL = NA * NA;                       // Gauss-point columns per brick
+    static constexpr int NTQ = NG * NCOL;                      // Gauss points per element layer
+    static constexpr int CPW = 64 / NG;                        // columns per wave
+    static constexpr int NWAVE = (NCOL + CPW - 1) / CPW;
+    static constexpr int NT = 64 * NWAVE;                      // threads per brick
+    static constexpr int NGEO = 13;                            // stored geometry values per point
+    static constexpr int PART = NZ * BN * BN * 3;              // partial sums per brick
+    static_assert(BZ >= BR, "one table size for all directions");
+    static_assert(NT <= 1024, "brick too large for one workgroup");
+};
+
+// Brick grid of a patch for the sweep (elements per direction over the brick edges).
+template <int P>
+__host__ __device__ inline void sweep_grid(const int32_t N[3], int32_t nb[3]) {
+    nb[0] = (N[0] + Sweep<P>::BR - 1) / Sweep<P>::BR;
+    nb[1] = (N[1] + Sweep<P>::BR - 1) / Sweep<P>::BR;
+    nb[2] = (N[2] + Sweep<P>::BZ - 1) / Sweep<P>::BZ;
+}
+
+// Offset of geometry value c of Gauss point (g0, g1, g2) of element (e1, e2, e3):
+// [brick][element layer in the brick][c][column][g2], column = a + NA b.
+template <int P>
+__host__ __device__ inline int64_t sweep_geo_off(const int32_t nb[3], int e1, int e2, int e3, int c, int g0, int g1,
+                                                 int g2) {
+    using S = Sweep<P>;
+    const int b1 = e1 / S::BR, b2 = e2 / S::BR, b3 = e3 / S::BZ;
+    const int64_t brick = b1 + (int64_t)nb[0] * (b2 + (int64_t)nb[1] * b3);
+    const int col = (e1 - b1 * S::BR) * S::NG + g0 + S::NA * ((e2 - b2 * S::BR) * S::NG + g1);
+    return (((brick * S::BZ + (e3 - b3 * S::BZ)) * S::NGEO + c) * S::NCOL + col) * S::NG + g2;
+}
+
+template <int P>
+inline int64_t sweep_geo_size(const int32_t nb[3]) {
+    return (int64_t)nb[0] * nb[1] * nb[2] * Sweep<P>::BZ * Sweep<P>::NGEO * Sweep<P>::NTQ;
+}
+
+template <typename T, int P>
+struct SweepSmem {
+    using S = Sweep<P>;
+    T tab[3][2][S::BZ][S::NG][S::NP1];   // [direction][B | dB][element of the brick][g][i]
+    T X[3][S::BN][S::BN];                // node layer: [component][n2][n1] = w x
+    T Wn[S::NP1][S::BN][S::BN];          // weights of the ring's node layers: [slot][n2][n1]
+    T Y[3][2][S::BN][S::NA];             // stage A: [component][value | d/dxi0][n2][a]
+    T Z[S::NP1][9][S::NCOL];             // ring of node layers after stage B: [slot][3 f + var][column]
+    T Ws[9][S::NCOL];                    // back-contractions of the completed node layer, summed over g2
+    T Pb[3][2][S::BN][S::NA];            // stage B^T: [component][P' | R0'][n2][a]
+};
+
+// One brick: partial[brick] = its elements' contribution to K x at its (NZ x BN x BN) nodes.
+// Every thread of the workgroup (Sweep<P>::NT threads) calls it.
+template <typename T, int P>
+__device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch &mp, const T *__restrict__ geo,
+                                              const T *__restrict__ x, const T *__restrict__ wcp,
+                                              T *__restrict__ partial, int64_t brick) {
+    using S = Sweep<P>;
+    constexpr int NP1 = S::NP1, NG = S::NG, BR = S::BR, BN = S::BN, NA = S::NA, NCOL = S::NCOL;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int col = wave * S::CPW + lane / NG, g2 = lane % NG;
+    const bool valid = lane < S::CPW * NG && col < NCOL;
+    const int a = col % NA, b = col / NA;          // Gauss positions (x, y) of the column
+    const int e1l = a / NG, g0 = a % NG, e2l = b / NG, g1 = b % NG;
+    const int b1 = (int)(brick % mp.gnb[0]), b2 = (int)((brick / mp.gnb[0]) % mp.gnb[1]),
+              b3 = (int)(brick / ((int64_t)mp.gnb[0] * mp.gnb[1]));
+    const int E1 = b1 * BR, E2 = b2 * BR, E3 = b3 * S::BZ;
+    const int ne1 = min(BR, mp.N[0] - E1), ne2 = min(BR, mp.N[1] - E2), ne3 = min(S::BZ, mp.N[2] - E3);
+    const int nz = ne3 + P;
+    const bool inside = valid && e1l < ne1 && e2l < ne2;
+    const T lam = (T)mp.lam, mu = (T)mp.mu;
+    T *out = partial + brick * (int64_t)S::PART;
+    const T *geo_b = geo + brick * (int64_t)S::BZ * S::NGEO * S::NTQ + (int64_t)col * NG + g2;
+
+    // 1D tables of the brick's elements; zero past the patch, so contractions there give 0
+    {
+        const int Eb[3] = {E1, E2, E3}, nb[3] = {ne1, ne2, ne3};
+        constexpr int NTAB = 3 * 2 * S::BZ * NG * NP1;
+        for (int i = tid; i < NTAB; i += S::NT) {
+            const int r = i % NP1, g = (i / NP1) % NG, le = (i / (NP1 * NG)) % S::BZ, wh = (i / (NP1 * NG * S::BZ)) % 2,
+                      d = i / (NP1 * NG * S::BZ * 2);
+            T v = T(0);
+            if (le < nb[d]) {
+                const int64_t o = ((int64_t)(mp.boff[d] + Eb[d] + le) * NG + g) * NP1 + r;
+                v = (T)(wh ? mp.dBtab[o] : mp.Btab[o]);
+            }
+            sm.tab[d][wh][le][g][r] = v;
+        }
+    }
+    T W[NP1][9];   // back-contractions for node layers e3 .. e3 + P of the thread's Gauss point
+#pragma unroll
+    for (int k = 0; k < NP1; ++k)
+#pragma unroll
+        for (int o = 0; o < 9; ++o) W[k][o] = T(0);
+
+    // back-contract node layer n3 from W[0] (all threads), write it, shift W down
+    auto backward = [&](int n3) {
+        // sum over the NG lanes of the column (adjacent lanes of one wave)
+        T s[9];
+#pragma unroll
+        for (int o = 0; o < 9; ++o) {
+            T v = W[0][o];
+#pragma unroll
+            for (int k = 1; k < NG; ++k) v += __shfl(W[0][o], lane + k, 64);
+            s[o] = v;
+        }
+        if (valid && g2 == 0)
+#pragma unroll
+            for (int o = 0; o < 9; ++o) sm.Ws[o][col] = s[o];
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+#pragma unroll
+            for (int o = 0; o < 9; ++o) W[k][o] = W[k + 1][o];
+#pragma unroll
+        for (int o = 0; o < 9; ++o) W[P][o] = T(0);
+        __syncthreads();
+        // B^T: (a, n2, f) <- columns (a, e2 NG + g1) with n2 = e2 + j
+        for (int it = tid; it < NA * BN * 3; it += S::NT) {
+            const int aa = it % NA, n2 = (it / NA) % BN, f = it / (NA * BN);
+            T pv = T(0), r0 = T(0);
+            for (int j = 0; j < NP1; ++j) {
+                const int e2 = n2 - j;
+                if (e2 < 0 || e2 >= BR) continue;
+#pragma unroll
+                for (int g = 0; g < NG; ++g) {
+                    const int cc = aa + NA * (e2 * NG + g);
+                    const T bv = sm.tab[1][0][e2][g][j], dv = sm.tab[1][1][e2][g][j];
+                    pv += sm.Ws[f * 3 + 0][cc] * bv + sm.Ws[f * 3 + 2][cc] * dv;
+                    r0 += sm.Ws[f * 3 + 1][cc] * bv;
+                }
+            }
+            sm.Pb[f][0][n2][aa] = pv;
+            sm.Pb[f][1][n2][aa] = r0;
+        }
+        __syncthreads();
+        // A^T: node (n1, n2) of layer n3 <- positions a = e1 NG + g0 with n1 = e1 + i
+        for (int it = tid; it < BN * BN; it += S::NT) {
+            const int n1 = it % BN, n2 = it / BN;
+            if (n1 >= ne1 + P || n2 >= ne2 + P) continue;
+            T y[3] = {T(0), T(0), T(0)};
+            for (int i = 0; i < NP1; ++i) {
+                const int e1 = n1 - i;
+                if (e1 < 0 || e1 >= BR) continue;
+#pragma unroll
+                for (int g = 0; g < NG; ++g) {
+                    const T bv = sm.tab[0][0][e1][g][i], dv = sm.tab[0][1][e1][g][i];
+#pragma unroll
+                    for (int f = 0; f < 3; ++f) y[f] += sm.Pb[f][0][n2][e1 * NG + g] * bv + sm.Pb[f][1][n2][e1 * NG + g] * dv;
+                }
+            }
+            const T w = sm.Wn[n3 % NP1][n2][n1];
+            T *dst = out + ((int64_t)(n3 * BN + n2) * BN + n1) * 3;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) dst[f] = y[f] * w;
+        }
+    };
+
+    // Node-layer loads run one layer ahead (the CP ids two layers ahead), the geometry of the
+    // next element layer one layer ahead, so their latency overlaps the contractions in between.
+    // Thread t < BN^2 loads node (t mod BN, t / BN) of each layer.
+    static_assert(BN * BN <= S::NT, "one thread per node of a layer");
+    const int ln1 = tid % BN, ln2 = tid / BN;
+    const bool loader = tid < BN * BN && ln1 < ne1 + P && ln2 < ne2 + P;
+    const int64_t cpl0 = (E1 + ln1) + (int64_t)mp.ncp[0] * ((E2 + ln2) + (int64_t)mp.ncp[1] * E3);
+    const int64_t cpl_dz = (int64_t)mp.ncp[0] * mp.ncp[1];
+    int cp_next = 0;          // CP of the loader's node in layer n3 + 1
+    T xr[3] = {T(0), T(0), T(0)}, wr = T(0);   // its w and x in layer n3
+    if (loader) {
+        const int cp0 = mp.cpmap[cpl0];
+        wr = wcp[cp0];
+        xr[0] = x[3 * (int64_t)cp0];
+        xr[1] = x[3 * (int64_t)cp0 + 1];
+        xr[2] = x[3 * (int64_t)cp0 + 2];
+        if (nz > 1) cp_next = mp.cpmap[cpl0 + cpl_dz];
+    }
+    T gv[S::NGEO];            // geometry of the thread's Gauss point in the next element layer
+    auto load_geo = [&](int e3) {
+        const T *gq = geo_b + (int64_t)e3 * S::NGEO * S::NTQ;
+#pragma unroll
+        for (int c = 0; c < S::NGEO; ++c) gv[c] = inside ? gq[c * S::NTQ] : T(0);
+    };
+    if constexpr (S::PREFETCH_GEO) load_geo(0);
+
+    for (int n3 = 0; n3 < nz; ++n3) {
+        const int slot = n3 % NP1;
+        // ---- forward: node layer n3 into ring slot n3 mod (P+1) ----
+        if (tid < BN * BN) {
+            sm.X[0][ln2][ln1] = wr * xr[0];
+            sm.X[1][ln2][ln1] = wr * xr[1];
+            sm.X[2][ln2][ln1] = wr * xr[2];
+            sm.Wn[slot][ln2][ln1] = wr;
+        }
+        if (loader && n3 + 1 < nz) {
+            const int cp = cp_next;
+            wr = wcp[cp];
+            xr[0] = x[3 * (int64_t)cp];
+            xr[1] = x[3 * (int64_t)cp + 1];
+            xr[2] = x[3 * (int64_t)cp + 2];
+            if (n3 + 2 < nz) cp_next = mp.cpmap[cpl0 + (int64_t)(n3 + 2) * cpl_dz];
+        }
+        __syncthreads();
+        // stage A: (a, n2, f) = sum_i X[f][n2][e1 + i] (B | dB)[e1][g0][i]
+        for (int it = tid; it < NA * BN * 3; it += S::NT) {
+            const int aa = it % NA, n2 = (it / NA) % BN, f = it / (NA * BN);
+            const int e1 = aa / NG, g = aa % NG;
+            T v = T(0), d = T(0);
+#pragma unroll
+            for (int i = 0; i < NP1; ++i) {
+                const T xv = sm.X[f][n2][e1 + i];
+                v += xv * sm.tab[0][0][e1][g][i];
+                d += xv * sm.tab[0][1][e1][g][i];
+            }
+            sm.Y[f][0][n2][aa] = v;
+            sm.Y[f][1][n2][aa] = d;
+        }
+        __syncthreads();
+        // stage B: column (a, b), outputs o = 3 f + var (value, d/dxi0, d/dxi1) dealt over the
+        // column's NG lanes
+        if (valid) {
+            for (int o = g2; o < 9; o += NG) {
+                const int f = o / 3, var = o % 3;
+                const int src = var == 1 ? 1 : 0;
+                T s = T(0);
+#pragma unroll
+                for (int j = 0; j < NP1; ++j)
+                    s += sm.Y[f][src][e2l + j][a] * sm.tab[1][var == 2 ? 1 : 0][e2l][g1][j];
+                sm.Z[slot][o][col] = s;
+            }
+        }
+        __syncthreads();
+        const int e3 = n3 - P;
+        if (e3 < 0) continue;
+        // ---- element layer e3: the thread's Gauss point (column, g2) ----
+        if (inside) {
+            T V[3], G[3][3];   // G[m][f]: d/dxi_m of component f
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                T v = T(0), d0 = T(0), d1 = T(0), d2 = T(0);
+#pragma unroll
+                for (int k = 0; k < NP1; ++k) {
+                    const int sk = (e3 + k) % NP1;
+                    const T bz = sm.tab[2][0][e3][g2][k], dz = sm.tab[2][1][e3][g2][k];
+                    const T zv = sm.Z[sk][f * 3 + 0][col];
+                    v += zv * bz;
+                    d2 += zv * dz;
+                    d0 += sm.Z[sk][f * 3 + 1][col] * bz;
+                    d1 += sm.Z[sk][f * 3 + 2][col] * bz;
+                }
+                V[f] = v;
+                G[0][f] = d0;
+                G[1][f] = d1;
+                G[2][f] = d2;
+            }
+            if constexpr (!S::PREFETCH_GEO) load_geo(e3);
+            T A[9], gw[3];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) A[c] = gv[c];
+            gw[0] = gv[9];
+            gw[1] = gv[10];
+            gw[2] = gv[11];
+            const T om = gv[12];
+            if constexpr (S::PREFETCH_GEO)
+                if (e3 + 1 < ne3) load_geo(e3 + 1);
+            // H[l][k] = d u_l / d x_k = sum_m (G[m][l] - gw[m] V[l]) A[m][k]
+            T H[3][3];
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    T s = T(0);
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) s += (G[m][l] - gw[m] * V[l]) * A[3 * m + k];
+                    H[l][k] = s;
+                }
+            const T tr = lam * (H[0][0] + H[1][1] + H[2][2]);
+            // coefficients of dB/dxi_m (Tm) and of B (tb) per component, into the z back-contraction
+            T Tm[3][3], tb[3];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                T sg[3];
+#pragma unroll
+                for (int l = 0; l < 3; ++l) sg[l] = mu * (H[f][l] + H[l][f]) + (f == l ? tr : T(0));
+                T t0 = T(0);
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    T s = T(0);
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) s += sg[l] * A[3 * m + l];
+                    s *= om;
+                    Tm[f][m] = s;
+                    t0 -= s * gw[m];
+                }
+                tb[f] = t0;
+            }
+#pragma unroll
+            for (int k = 0; k < NP1; ++k) {
+                const T bz = sm.tab[2][0][e3][g2][k], dz = sm.tab[2][1][e3][g2][k];
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    W[k][f * 3 + 0] += tb[f] * bz + Tm[f][2] * dz;
+                    W[k][f * 3 + 1] += Tm[f][0] * bz;
+                    W[k][f * 3 + 2] += Tm[f][1] * bz;
+                }
+            }
+        }
+        backward(e3);
+    }
+    // the last P node layers have all their contributions
+    for (int n3 = ne3; n3 < nz; ++n3) backward(n3);
+}
+
+}  // namespace yeti

@@ -536,24 +536,18 @@ __global__ void __launch_bounds__(256) ir_accumulate_kernel(int64_t n, double *x
     }
 }
 
-// Residual replacement, predicated on F_UPDATE: r64 = mask (b - K x); r = (T) r64; then, in the
-// last block, the end of the replacement: stop at the tolerance, and count replacements that do
-// not improve the true residual (stagnation at the FP64 rounding floor of ill-conditioned systems).
-template <typename T>
-__global__ void __launch_bounds__(256) ir_true_residual_kernel(int64_t n, const double *b, const double *Kx,
-                                                               const uint8_t *mask, T *r, double *part, double *sc,
-                                                               int32_t *flags, double rtol2, int32_t max_updates,
-                                                               int32_t max_stagnant) {
-    if (flags[F_UPDATE] == 0) return;
-    __shared__ double sh[4];
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double v = mask[i] ? b[i] - Kx[i] : 0.0;
-        r[i] = (T)v;
-        acc += v * v;
-    }
-    double rrt;
-    if (!last_block_total(block_sum256(acc, sh), part, flags + F_COUNTER, sh, rrt) || threadIdx.x != 0) return;
+// The end of a residual replacement whose true residual is rrt = |mask (b - K x)|^2 (one thread):
+// stop at the tolerance; count replacements that do not improve the true residual; keep track of
+// the best iterate (F_BEST) and of whether the iterated residual had departed from the true one.
+// A solve that stagnates (max_stagnant replacements without a 4x gain) or diverges (true
+// residual 1e4 x above its best) stops: at its FP64 rounding floor (F_FLOOR) when its best
+// relative residual is at most IR_FLOOR_MAX, else as failed (F_FAILED), which is what an
+// iteration that breaks down in its working precision gives. The best iterate is restored
+// (F_RESTORE) when the solve stops above it.
+constexpr double IR_FLOOR_MAX = 1e-6;
+
+__device__ __forceinline__ void ir_replacement_end(double rrt, double *sc, int32_t *flags, double rtol2,
+                                                   int32_t max_updates, int32_t max_stagnant) {
     const bool forced = flags[F_UPDATE] == 2;
     // the iterated residual has left the true one behind (by more than 4x in norm): the mark of
     // a rounding floor, as opposed to a solve that is merely converging slowly
@@ -572,7 +566,7 @@ __global__ void __launch_bounds__(256) ir_true_residual_kernel(int64_t n, const 
     flags[F_DEPARTED] = departed ? 1 : 0;
     if (!(rrt < 0.25 * sc[S_RRT_PREV])) flags[F_STAGNANT] += 1;
     else flags[F_STAGNANT] = 0;
-    // far above the best residual so far: the iterate is drifting away at the rounding floor
+    // far above the best residual so far: the iterate is drifting away
     const bool diverging = rrt > 1e4 * sc[S_RRT_PREV];
     // S_RRT_PREV is the lowest true residual so far: save this iterate when it is the new best
     if (rrt < sc[S_RRT_PREV]) {
@@ -584,8 +578,9 @@ __global__ void __launch_bounds__(256) ir_true_residual_kernel(int64_t n, const 
     // afresh (beta = 0 through an infinite previous r.z)
     if (forced && departed) sc[S_RZ] = __builtin_inf();
     bool stop = false;
-    if (flags[F_STAGNANT] >= max_stagnant || diverging) {   // FP64 rounding floor above rtol: stop, flagged
-        flags[F_FLOOR] = 1;
+    if (flags[F_STAGNANT] >= max_stagnant || diverging) {
+        if (sc[S_RRT_PREV] <= IR_FLOOR_MAX * IR_FLOOR_MAX * sc[S_BB]) flags[F_FLOOR] = 1;
+        else flags[F_FAILED] = 1;
         stop = true;
     } else if (flags[F_UPDATES] >= max_updates) {
         flags[F_FAILED] = 1;
@@ -599,6 +594,27 @@ __global__ void __launch_bounds__(256) ir_true_residual_kernel(int64_t n, const 
             sc[S_RRT] = sc[S_RRT_PREV];
         }
     }
+}
+
+// Residual replacement, predicated on F_UPDATE: r64 = mask (b - K x); r = (T) r64; then, in the
+// last block, the end of the replacement: stop at the tolerance, and count replacements that do
+// not improve the true residual (stagnation at the FP64 rounding floor of ill-conditioned systems).
+template <typename T>
+__global__ void __launch_bounds__(256) ir_true_residual_kernel(int64_t n, const double *b, const double *Kx,
+                                                               const uint8_t *mask, T *r, double *part, double *sc,
+                                                               int32_t *flags, double rtol2, int32_t max_updates,
+                                                               int32_t max_stagnant) {
+    if (flags[F_UPDATE] == 0) return;
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double v = mask[i] ? b[i] - Kx[i] : 0.0;
+        r[i] = (T)v;
+        acc += v * v;
+    }
+    double rrt;
+    if (!last_block_total(block_sum256(acc, sh), part, flags + F_COUNTER, sh, rrt) || threadIdx.x != 0) return;
+    ir_replacement_end(rrt, sc, flags, rtol2, max_updates, max_stagnant);
 }
 
 // After a replacement: keep the iterate as the best one (F_BEST), or put the best one back at
@@ -882,46 +898,7 @@ __global__ void __launch_bounds__(256) bt_true_residual_kernel(BatchDev bd) {
         if (!member_total(block_sum256(acc, sh), R.red, R.flags + F_COUNTER, R.nvchunk, tk.a, sh, rrt) ||
             threadIdx.x != 0)
             continue;
-        double *sc = R.sc;
-        int32_t *flags = R.flags;
-        const bool forced = flags[F_UPDATE] == 2;
-        const bool departed = rrt > 16.0 * sc[S_RR];
-        sc[S_RRT] = rrt;
-        flags[F_UPDATE] = 0;
-        flags[F_UPDATES] += 1;
-        sc[S_LASTUPD] = (double)(flags[F_ITERS] + 1);
-        sc[S_RR] = rrt;
-        sc[S_RRMAX] = rrt;
-        if (!(rrt > R.rtol2 * sc[S_BB])) {
-            flags[F_DONE] = 1;
-            flags[F_ACTIVE] = 0;
-            continue;
-        }
-        flags[F_DEPARTED] = departed ? 1 : 0;
-        if (!(rrt < 0.25 * sc[S_RRT_PREV])) flags[F_STAGNANT] += 1;
-        else flags[F_STAGNANT] = 0;
-        const bool diverging = rrt > 1e4 * sc[S_RRT_PREV];
-        if (rrt < sc[S_RRT_PREV]) {
-            sc[S_RRT_PREV] = rrt;
-            flags[F_BEST] = 1;
-        }
-        if (forced && departed) sc[S_RZ] = __builtin_inf();
-        bool stop = false;
-        if (flags[F_STAGNANT] >= IR_MAX_STAGNANT || diverging) {
-            flags[F_FLOOR] = 1;
-            stop = true;
-        } else if (flags[F_UPDATES] >= R.max_updates) {
-            flags[F_FAILED] = 1;
-            stop = true;
-        }
-        if (stop) {
-            flags[F_DONE] = 1;
-            flags[F_ACTIVE] = 0;
-            if (sc[S_RRT_PREV] < rrt) {
-                flags[F_RESTORE] = 1;
-                sc[S_RRT] = sc[S_RRT_PREV];
-            }
-        }
+        ir_replacement_end(rrt, R.sc, R.flags, R.rtol2, R.max_updates, IR_MAX_STAGNANT);
     }
 }
 
