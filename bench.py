@@ -519,14 +519,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nel", type=int, default=64, help="elements per direction")
     ap.add_argument("--p", type=int, default=3, help="degree")
-    ap.add_argument("--cpu-slab", type=int, default=4,
+    ap.add_argument("--cpu-slab", type=int, default=16,
                     help="elements in z of the nel x nel x slab patch timed by the CPU baseline")
-    ap.add_argument("--cpu-runs", type=int, default=3, help="timed CPU calls (median) after one warm-up")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed CPU calls (median) after one warm-up")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = every core of this host")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--serve-rate", type=float, default=25.0, help="offered requests per second")
-    ap.add_argument("--serve-plain", type=int, default=24,
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="offered requests per second")
+    ap.add_argument("--serve-plain", type=int, default=100,
                     help="requests solved one at a time (plain FP64), from the start of the stream")
     ap.add_argument("--grad-fd", type=int, default=50, help="coordinates checked by central differences")
     ap.add_argument("--jacobi-iters", type=int, default=60000, help="bound on the C2 Jacobi PCG iterations")

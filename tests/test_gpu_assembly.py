@@ -128,3 +128,37 @@ def test_errors_are_raised(gpu):
     with pytest.raises(YetiError) as ei:
         sys_linmat_lindef_static(**a)
     assert ei.value.code == 4
+
+
+@pytest.mark.parametrize("p,nel", [(4, (16, 16, 2)), (3, (16, 16, 3)), (2, (16, 16, 4)), (1, (16, 16, 6))])
+def test_several_elements_per_workgroup(gpu, p, nel):
+    """Patches with more elements than workgroups (the element kernel is persistent, one
+    workgroup per CU): elements that are the first, second and last of their workgroup, with
+    indices (run_matrix) and values only (run_values), against the oracle run on those elements
+    alone. Indices bit-exact, values within 1e-12 of the largest magnitude."""
+    import torch
+    from yeti_amd.assembly import Assembler
+    print("[case] several elements per workgroup", p, nel, flush=True)
+    bp = make_box_patch(nel=nel, p=p, seed=31 + p)
+    args = dict(bp.args)
+    n = (p + 1) ** 3
+    nde = 9 * n * (n - 1) // 2 + 6 * n
+    ne = bp.nb_elem
+    sel = np.array(sorted({0, 1, 255, 256, 257, 300, ne // 2, ne - 1}))
+    act = np.zeros(ne, np.int32)
+    act[sel] = 1
+    sub = dict(args)
+    sub["activeElement"] = act
+    sub["nb_data"] = float(sel.size * nde)
+    Ko, ro, co, _ = O.oracle_sys_linmat((), kwargs=sub)
+    asm = Assembler(args)
+    K, r, c = asm.alloc_coo(True)
+    asm.run_matrix(K, r, c)
+    Kv = torch.full((asm.nb_data,), float("nan"), dtype=torch.float64, device="cuda")
+    asm.run_values(Kv)
+    idx = torch.as_tensor((sel[:, None] * nde + np.arange(nde)[None, :]).ravel(), device="cuda")
+    scale = np.abs(Ko).max()
+    assert np.array_equal(r[idx].cpu().numpy(), ro) and np.array_equal(c[idx].cpu().numpy(), co)
+    assert np.abs(K[idx].cpu().numpy() - Ko).max() <= 1e-12 * scale
+    assert np.abs(Kv[idx].cpu().numpy() - Ko).max() <= 1e-12 * scale
+    asm.close()

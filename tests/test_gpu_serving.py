@@ -174,3 +174,46 @@ def test_batched_result_identical_alone_and_in_mix(gpu):
         assert np.array_equal(alone["x"][k], mix["x"][k])
         assert np.array_equal(alone["x"][k], late["x"][k])
         assert np.array_equal(alone["x"][k], eager["x"][k])
+
+
+def test_batched_stream_requests_match_assembled_fp64(gpu):
+    """Requests of the serving stream itself (seeded generator: 4-48 elements per direction,
+    degrees 2-4, the bench's seed), the first eight whose patches have at most 24^3 elements,
+    served by the batched server in mixed precision to rtol 1e-10: each displacement within 1e-8
+    of an FP64 solve whose every product is the assembled CSR of K + K^T (device COO of the
+    oracle-pinned assembly), solved to 1e-12 or its FP64 floor."""
+    import torch
+    from yeti_amd.assembly import Assembler
+    from yeti_amd.matfree import MatrixFree
+    from yeti_amd.serving import SolveServer, request_stream, solve_session
+    from yeti_amd.solver import StiffnessCSR, free_mask_from_args
+    print("[case] batched stream requests vs assembled FP64", flush=True)
+    reqs = [r for r in request_stream(n=200, rate=40.0, seed=20261202) if np.prod(r.nel) <= 24 ** 3][:8]
+    reqs = [type(r)(k, 0.0, r.nel, r.p, r.E, r.nu, r.traction, r.seed) for k, r in enumerate(reqs)]
+    assert sorted({r.p for r in reqs}) == [2, 3, 4]
+    srv = SolveServer(budget_bytes=8 << 30, precision="mixed", rtol=1e-10, batched=True)
+    rep = srv.run(reqs, keep_solutions=True, realtime=False)
+    srv.close()
+    assert rep["completed"] == len(reqs) and rep["failed"] == 0
+    for r in reqs:
+        args = r.model()
+        asm = Assembler(args)
+        F = asm.run_rhs()
+        K = torch.empty(asm.nb_data, dtype=torch.float64, device="cuda")
+        asm.run_values(K)
+        csr = StiffnessCSR(asm)
+        vals = csr.values(K)
+        del K
+        mask = free_mask_from_args(args)
+        mf = MatrixFree(args)
+        u0, st0 = solve_session(mf, F, mask, precision=64, rtol=1e-12, csr=csr, csr_vals=vals, max_iters=50000)
+        assert not st0["failed"]
+        u0 = u0.cpu().numpy()
+        err = np.linalg.norm(rep["x"][r.rid] - u0) / np.linalg.norm(u0)
+        info = rep["requests"][r.rid]
+        print("  req %d nel %s p %d: %d iterations, relres %.2e, |du|/|u| %.2e (FP64 %d its, relres %.2e)" % (
+            r.rid, r.nel, r.p, info["iters"], info["relres"], err, st0["iters"], st0["relres"]), flush=True)
+        assert err <= 1e-8
+        mf.close()
+        csr.close()
+        asm.close()

@@ -790,7 +790,7 @@ static int launch_elem(yeti_asm *h, PatchDev &pd, const double *coords, double *
             if (rc) return rc;
             YETI_HIP(hipEventRecord(ea, st));
         }
-        if (Kr)
+        if (Kr || C::VALUES_VIA_IDX)
             hipLaunchKernelGGL((elem_stiffness_kernel<P, NG, true>), dim3(grid), dim3(C::NTH), 0, st, pa,
                                (const double *)pd.geo.p, Kd, Kr, Kc);
         else

@@ -65,6 +65,11 @@ struct ElemCfg {
     static constexpr bool PROD_COPY = false;
     static constexpr int CPW = PROD_COPY ? CW + PW : CW;
     static constexpr int MAXTILE = NNP >= 128 ? 9 : 12; // accumulator tiles per consumer wave per pass
+    // Values-only runs use the instantiation that writes indices, with null index pointers:
+    // the values-only instantiation at p = 4 (NNP = 128, ~620 bytes of register spills per lane)
+    // wrote wrong values for the second and later elements of a workgroup (a defect not traced
+    // to the source); the with-indices instantiation agrees with the oracle in both uses.
+    static constexpr bool VALUES_VIA_IDX = NNP >= 128;
     static constexpr int MAXU = 8;                      // units per consumer wave per pass
     static constexpr int MAXF = 12;                     // distinct A (or B) fragments per wave
     static constexpr int MAXPASS = 8;
@@ -702,7 +707,7 @@ __device__ __forceinline__ void copy_round(const ElemSmem<P, NG> &sm, int64_t ba
                 const int j = j0 + 64 * h + lane;
                 if (j < len) {
                     Kdata[gb + j] = v[h];
-                    if (WRITE_IDX) {
+                    if (WRITE_IDX && Krow) {   // null: a values-only run of this instantiation
                         int a, kk, ll;
                         if (j < 9 * b) {
                             a = j / 9;

@@ -558,7 +558,15 @@ __device__ __forceinline__ void ir_replacement_end(double rrt, double *sc, int32
     sc[S_LASTUPD] = (double)(flags[F_ITERS] + 1);
     sc[S_RR] = rrt;
     sc[S_RRMAX] = rrt;
-    if (!(rrt > rtol2 * sc[S_BB])) {
+    if (!(rrt == rrt)) {   // NaN: the iteration broke down; return the best iterate, failed
+        flags[F_FAILED] = 1;
+        flags[F_DONE] = 1;
+        flags[F_ACTIVE] = 0;
+        flags[F_RESTORE] = 1;
+        sc[S_RRT] = sc[S_RRT_PREV];
+        return;
+    }
+    if (rrt <= rtol2 * sc[S_BB]) {
         flags[F_DONE] = 1;
         flags[F_ACTIVE] = 0;
         return;
@@ -635,13 +643,14 @@ __global__ void ir_best_clear_kernel(int32_t *flags) {
 
 template <typename T>
 __global__ void __launch_bounds__(256) ir_begin_copy_kernel(int64_t n, const double *b, const uint8_t *mask, double *x,
-                                                            T *r, T *y, double *part) {
+                                                            double *xbest, T *r, T *y, double *part) {
     __shared__ double sh[4];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         double v = mask[i] ? b[i] : 0.0;
         r[i] = (T)v;
         x[i] = 0.0;
+        xbest[i] = 0.0;   // the best iterate so far is the start, whose residual is b
         y[i] = T(0);
         acc += v * v;
     }
@@ -778,7 +787,8 @@ static int ir_step_any(yeti_ir *s, hipStream_t st) {
 template <typename T>
 static int ir_begin_launches(yeti_ir *s, T *r, T *y, T *z, T *p, const T *inv, hipStream_t st) {
     const int32_t *act = s->flags.p + F_ACTIVE;
-    hipLaunchKernelGGL(ir_begin_copy_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->b, s->mask, s->x, r, y, s->part.p);
+    hipLaunchKernelGGL(ir_begin_copy_kernel<T>, dim3(s->nb), dim3(256), 0, st, s->n, s->b, s->mask, s->x, s->xbest.p, r,
+                       y, s->part.p);
     hipLaunchKernelGGL(pred_sum_partials_kernel, dim3(1), dim3(256), 0, st, s->part.p, (int)s->nb, s->sc.p + S_BB,
                        (const int32_t *)nullptr);
     hipLaunchKernelGGL(ir_begin_flags_kernel, dim3(1), dim3(1), 0, st, s->sc.p, s->flags.p, s->rtol * s->rtol);
