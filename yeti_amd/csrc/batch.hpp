@@ -14,7 +14,8 @@ struct MfPatch {
     const double *Btab;      // [d][e_d][NG][NP1] values (basis function span - p + r)
     const double *dBtab;     // same, derivatives
     int32_t boff[3];         // offset of direction d in the tables (in elements)
-    int32_t gnb[3];          // brick grid of the sweep apply, which also orders the stored geometry
+    int32_t gnb[2][3];       // brick grids of the sweep apply, [0] FP64, [1] FP32; each orders that
+                             // precision's stored geometry
     double lam, mu;
 };
 
@@ -62,8 +63,8 @@ struct BReq {
     // operator
     MfPatch mp;
     int32_t p, ng;
-    int32_t nbrick[3];       // apply brick grid (the sweep's)
-    int32_t br[3], bn[3];    // apply brick edge per direction in elements / nodes
+    int32_t nbrick[2][3];    // apply brick grids (the sweep's), [0] FP64, [1] FP32
+    int32_t br[2][3], bn[2][3];  // apply brick edges per direction in elements / nodes, same index
     int32_t ncp;
     int64_t n;               // 3 ncp
     const void *geo[2];      // quadrature geometry, [0] FP64, [1] FP32
@@ -97,7 +98,8 @@ struct BTask {
 };
 
 enum : int {
-    BT_BRICK1 = 0, BT_BRICK2, BT_BRICK3, BT_BRICK4,   // apply bricks per degree 1..4
+    BT_BRICK1 = 0, BT_BRICK2, BT_BRICK3, BT_BRICK4,   // apply bricks per degree 1..4 (batch precision)
+    BT_RBRICK1, BT_RBRICK2, BT_RBRICK3, BT_RBRICK4,   // FP64 bricks of an FP32 batch (replacement applies)
     BT_GATHER, BT_VEC, BT_CP, BT_GEMM0, BT_NLISTS = BT_GEMM0 + 6
 };
 

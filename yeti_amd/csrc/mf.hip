@@ -166,11 +166,11 @@ __global__ void mf_geometry_kernel(MfPatch mp, const double *__restrict__ coords
     out[10] = dw[1];
     out[11] = dw[2];
     out[12] = om;
-    const int64_t o0 = sweep_geo_off<P>(mp.gnb, e1, e2, e3, 0, g0, g1, g2);
+    const int64_t o64 = sweep_geo_off<P, 8>(mp.gnb[0], e1, e2, e3, 0, g0, g1, g2);
+    const int64_t o32 = sweep_geo_off<P, 4>(mp.gnb[1], e1, e2, e3, 0, g0, g1, g2);
     for (int c = 0; c < 13; ++c) {
-        const int64_t o = o0 + (int64_t)c * Sweep<P>::NTQ;
-        if (geo64) geo64[o] = out[c];
-        if (geo32) geo32[o] = (float)out[c];
+        if (geo64) geo64[o64 + (int64_t)c * Sweep<P, 8>::NTQ] = out[c];
+        if (geo32) geo32[o32 + (int64_t)c * Sweep<P, 4>::NTQ] = (float)out[c];
     }
 }
 
@@ -313,8 +313,9 @@ __device__ __forceinline__ void mf_brick_body(MfSmem<T, P, NG, MODE == 1 ? 9 : 3
                         G[1][f] = d1;
                         G[2][f] = d2;
                     }
-                    constexpr int CS = Sweep<P>::NTQ;   // stride between geometry values
-                    const T *gq = geo + sweep_geo_off<P>(mp.gnb, e1, e2, e3, 0, g0, g1, g2);
+                    constexpr int CS = Sweep<P, sizeof(T)>::NTQ;   // stride between geometry values
+                    const T *gq = geo + sweep_geo_off<P, sizeof(T)>(mp.gnb[sizeof(T) == 8 ? 0 : 1], e1, e2, e3, 0, g0,
+                                                                     g1, g2);
                     T A[9], gw[3];
                     for (int c = 0; c < 9; ++c) A[c] = gq[c * CS];
                     gw[0] = gq[9 * CS];
@@ -414,10 +415,11 @@ __device__ __forceinline__ void mf_brick_body(MfSmem<T, P, NG, MODE == 1 ? 9 : 3
                 if (live && r < NN) {
                     const int i = r % NP1, j = (r / NP1) % NP1, k = r / (NP1 * NP1);
                     const T w = sm.w[slot][r];
-                    constexpr int CS = Sweep<P>::NTQ;
+                    constexpr int CS = Sweep<P, sizeof(T)>::NTQ;
                     for (int q = 0; q < NQ; ++q) {
                         const int g0 = q % NG, g1 = (q / NG) % NG, g2 = q / (NG * NG);
-                        const T *gq = geo + sweep_geo_off<P>(mp.gnb, e1, e2, e3, 0, g0, g1, g2);
+                        const T *gq = geo + sweep_geo_off<P, sizeof(T)>(mp.gnb[sizeof(T) == 8 ? 0 : 1], e1, e2, e3, 0,
+                                                                         g0, g1, g2);
                         const T Bq = B[0][g0][i] * B[1][g1][j] * B[2][g2][k];
                         const T dq[3] = {dB[0][g0][i] * B[1][g1][j] * B[2][g2][k], B[0][g0][i] * dB[1][g1][j] * B[2][g2][k],
                                          B[0][g0][i] * B[1][g1][j] * dB[2][g2][k]};
@@ -507,9 +509,9 @@ __global__ void mf_gather_kernel(MfPatch mp, BrickGrid bg, const T *__restrict__
     mf_gather_point<T, NC>(mp, bg.br, bg.bn, bg.nb, partial, y, accumulate, t);
 }
 
-// The apply as brick sweeps: one workgroup of Sweep<P>::NT threads per brick.
+// The apply as brick sweeps: one workgroup of Sweep<P, sizeof(T)>::NT threads per brick.
 template <typename T, int P>
-__global__ void __launch_bounds__(Sweep<P>::NT) mf_sweep_kernel(MfPatch mp, const T *__restrict__ geo,
+__global__ void __launch_bounds__((Sweep<P, sizeof(T)>::NT)) mf_sweep_kernel(MfPatch mp, const T *__restrict__ geo,
                                                                 const T *__restrict__ x, const T *__restrict__ wcp,
                                                                 T *__restrict__ partial,
                                                                 const int32_t *__restrict__ pred) {
@@ -539,7 +541,7 @@ struct MfPatchDev {
     int p, ng;
     MfPatch mp;
     int32_t nbrick[2][3];                 // coloured bricks: [0] strain-energy gradient, [1] diagonal blocks
-    int32_t sweep_br[3];                  // sweep brick edge per direction (the grid is mp.gnb)
+    int32_t sweep_br[2][3];               // sweep brick edge per direction, [0] FP64, [1] FP32 (grids: mp.gnb)
     DevBuf<int32_t> cpmap, nbrick_dev;    // [2][3]
     DevBuf<double> Btab, dBtab, half, geo64;
     DevBuf<float> geo32;
@@ -582,8 +584,10 @@ static int launch_brick(const MfPatchDev &pd, const T *geo, const T *x, const T 
 template <typename T, int P>
 static int launch_sweep(const MfPatchDev &pd, const T *geo, const T *x, const T *w, T *part, hipStream_t st,
                         const int32_t *pred) {
-    const int64_t nb = (int64_t)pd.mp.gnb[0] * pd.mp.gnb[1] * pd.mp.gnb[2];
-    hipLaunchKernelGGL((mf_sweep_kernel<T, P>), dim3((unsigned)nb), dim3(Sweep<P>::NT), 0, st, pd.mp, geo, x, w, part,
+    const int32_t *g = pd.mp.gnb[sizeof(T) == 8 ? 0 : 1];
+    const int64_t nb = (int64_t)g[0] * g[1] * g[2];
+    hipLaunchKernelGGL((mf_sweep_kernel<T, P>), dim3((unsigned)nb), dim3(Sweep<P, sizeof(T)>::NT), 0, st, pd.mp, geo, x, w,
+                       part,
                        pred);
     YETI_HIP(hipGetLastError());
     return 0;
@@ -609,13 +613,13 @@ static int dispatch_brick(const MfPatchDev &pd, const T *geo, const T *x, const 
     }
 }
 
-// The bricks whose partial sums MODE's kernel writes.
-static BrickGrid brick_grid(const MfPatchDev &pd, int mode) {
+// The bricks whose partial sums MODE's kernel writes (MODE 0 in precision pi: 0 FP64, 1 FP32).
+static BrickGrid brick_grid(const MfPatchDev &pd, int mode, int pi) {
     BrickGrid bg;
     for (int d = 0; d < 3; ++d) {
         if (mode == 0) {
-            bg.br[d] = pd.sweep_br[d];
-            bg.nb[d] = pd.mp.gnb[d];
+            bg.br[d] = pd.sweep_br[pi][d];
+            bg.nb[d] = pd.mp.gnb[pi][d];
         } else {
             bg.br[d] = mode == 1 ? pd.p + 1 : 2 * (pd.p + 1);
             bg.nb[d] = pd.nbrick[mode == 1 ? 1 : 0][d];
@@ -650,7 +654,7 @@ static int mf_run(const yeti_mf *m, const T *x, T *y, hipStream_t st, const int3
         if (!geo || !part) return set_error(YETI_ERR_INVALID_ARGUMENT, "matrix-free: precision not prepared");
         int rc = dispatch_brick<T, MODE>(pd, geo, x, w, part, st, pred);
         if (rc) return rc;
-        const BrickGrid bg = brick_grid(pd, MODE);
+        const BrickGrid bg = brick_grid(pd, MODE, sizeof(T) == 8 ? 0 : 1);
         int64_t ncp = (int64_t)pd.mp.ncp[0] * pd.mp.ncp[1] * pd.mp.ncp[2];
         hipLaunchKernelGGL((mf_gather_kernel<T, NC>), dim3((unsigned)((ncp + 255) / 256)), dim3(256), 0, st, pd.mp, bg,
                            part, y, (m->zero_first || ip > 0) ? 1 : 0, pred);
@@ -1184,7 +1188,7 @@ void fdm_destroy(yeti_fdm *f) { delete f; }
 // precision T (active members), or (RPL) the FP64 apply of the replacement x -> FP64 partials
 // (members with a replacement due).
 template <typename T, int P, bool RPL>
-__global__ void __launch_bounds__(Sweep<P>::NT) bt_sweep_kernel(BatchDev bd, int list) {
+__global__ void __launch_bounds__((Sweep<P, sizeof(T)>::NT)) bt_sweep_kernel(BatchDev bd, int list) {
     __shared__ SweepSmem<T, P> sm;
     constexpr int pi = sizeof(T) == 8 ? 0 : 1;
     const int32_t cnt = bd.count[list];
@@ -1212,7 +1216,7 @@ __global__ void __launch_bounds__(256) bt_gather_kernel(BatchDev bd) {
         const int64_t cp = (int64_t)tk.a * BATCH_GATHER_BLOCK + threadIdx.x;
         if (cp >= R.ncp) continue;
         T *y = RPL ? (T *)R.y64 : (T *)R.vq;
-        mf_gather_point<T, 3>(R.mp, R.br, R.bn, R.nbrick, (const T *)R.part[pi], y, 0, cp);
+        mf_gather_point<T, 3>(R.mp, R.br[pi], R.bn[pi], R.nbrick[pi], (const T *)R.part[pi], y, 0, cp);
     }
 }
 
@@ -1282,20 +1286,24 @@ __global__ void __launch_bounds__(256) bt_fdm_scatter_kernel(BatchDev bd, int mo
 
 constexpr unsigned BT_GRID_BRICK = 1024, BT_GRID = 2048;
 
+// `lists`: BT_BRICK1 (bricks of the batch precision) or BT_RBRICK1 (FP64 bricks of an FP32 batch)
 template <typename T, bool RPL>
-static int batch_apply_t(const BatchDev &bd, hipStream_t st) {
-    hipLaunchKernelGGL((bt_sweep_kernel<T, 1, RPL>), dim3(BT_GRID_BRICK), dim3(Sweep<1>::NT), 0, st, bd, (int)BT_BRICK1);
-    hipLaunchKernelGGL((bt_sweep_kernel<T, 2, RPL>), dim3(BT_GRID_BRICK), dim3(Sweep<2>::NT), 0, st, bd, (int)BT_BRICK2);
-    hipLaunchKernelGGL((bt_sweep_kernel<T, 3, RPL>), dim3(BT_GRID_BRICK), dim3(Sweep<3>::NT), 0, st, bd, (int)BT_BRICK3);
-    hipLaunchKernelGGL((bt_sweep_kernel<T, 4, RPL>), dim3(BT_GRID_BRICK), dim3(Sweep<4>::NT), 0, st, bd, (int)BT_BRICK4);
+static int batch_apply_t(const BatchDev &bd, int lists, hipStream_t st) {
+    hipLaunchKernelGGL((bt_sweep_kernel<T, 1, RPL>), dim3(BT_GRID_BRICK), dim3(Sweep<1, sizeof(T)>::NT), 0, st, bd, lists);
+    hipLaunchKernelGGL((bt_sweep_kernel<T, 2, RPL>), dim3(BT_GRID_BRICK), dim3(Sweep<2, sizeof(T)>::NT), 0, st, bd,
+                       lists + 1);
+    hipLaunchKernelGGL((bt_sweep_kernel<T, 3, RPL>), dim3(BT_GRID_BRICK), dim3(Sweep<3, sizeof(T)>::NT), 0, st, bd,
+                       lists + 2);
+    hipLaunchKernelGGL((bt_sweep_kernel<T, 4, RPL>), dim3(BT_GRID_BRICK), dim3(Sweep<4, sizeof(T)>::NT), 0, st, bd,
+                       lists + 3);
     hipLaunchKernelGGL((bt_gather_kernel<T, RPL>), dim3(BT_GRID), dim3(BATCH_GATHER_BLOCK), 0, st, bd);
     YETI_HIP(hipGetLastError());
     return 0;
 }
 
 int batch_mf_apply(const BatchDev &bd, int prec64, bool rpl, hipStream_t st) {
-    if (rpl) return batch_apply_t<double, true>(bd, st);
-    return prec64 ? batch_apply_t<double, false>(bd, st) : batch_apply_t<float, false>(bd, st);
+    if (rpl) return batch_apply_t<double, true>(bd, prec64 ? (int)BT_BRICK1 : (int)BT_RBRICK1, st);
+    return prec64 ? batch_apply_t<double, false>(bd, BT_BRICK1, st) : batch_apply_t<float, false>(bd, BT_BRICK1, st);
 }
 
 template <typename T>
@@ -1324,11 +1332,12 @@ int batch_fill_operator(const yeti_mf *m, const yeti_fdm *f, int prec64, BReq *R
     R->mp = pd.mp;
     R->p = pd.p;
     R->ng = pd.ng;
-    for (int d = 0; d < 3; ++d) {
-        R->nbrick[d] = pd.mp.gnb[d];
-        R->br[d] = pd.sweep_br[d];
-        R->bn[d] = pd.sweep_br[d] + pd.p;
-    }
+    for (int k = 0; k < 2; ++k)
+        for (int d = 0; d < 3; ++d) {
+            R->nbrick[k][d] = pd.mp.gnb[k][d];
+            R->br[k][d] = pd.sweep_br[k][d];
+            R->bn[k][d] = pd.sweep_br[k][d] + pd.p;
+        }
     R->ncp = m->nb_cp;
     R->n = 3 * (int64_t)m->nb_cp;
     R->geo[0] = pd.geo64.p;
@@ -1362,20 +1371,28 @@ int batch_fill_operator(const yeti_mf *m, const yeti_fdm *f, int prec64, BReq *R
 }
 
 // Sweep brick grid, brick edges, geometry size and partial-sum size of a degree-p patch.
-template <int P>
+template <int P, int W>
 static void sweep_dims_t(const int32_t N[3], int32_t gnb[3], int32_t br[3], int64_t &geo_n, int64_t &part_n) {
-    sweep_grid<P>(N, gnb);
-    br[0] = br[1] = Sweep<P>::BR;
-    br[2] = Sweep<P>::BZ;
-    geo_n = sweep_geo_size<P>(gnb);
-    part_n = (int64_t)gnb[0] * gnb[1] * gnb[2] * Sweep<P>::PART;
+    using S = Sweep<P, W>;
+    sweep_grid<P, W>(N, gnb);
+    br[0] = S::BRX;
+    br[1] = S::BRY;
+    br[2] = S::BZ;
+    geo_n = sweep_geo_size<P, W>(gnb);
+    part_n = (int64_t)gnb[0] * gnb[1] * gnb[2] * S::PART;
 }
-static void sweep_dims(int p, const int32_t N[3], int32_t gnb[3], int32_t br[3], int64_t &geo_n, int64_t &part_n) {
-    switch (p) {
-    case 1: sweep_dims_t<1>(N, gnb, br, geo_n, part_n); break;
-    case 2: sweep_dims_t<2>(N, gnb, br, geo_n, part_n); break;
-    case 3: sweep_dims_t<3>(N, gnb, br, geo_n, part_n); break;
-    default: sweep_dims_t<4>(N, gnb, br, geo_n, part_n); break;
+// W: bytes per value (8 FP64, 4 FP32)
+static void sweep_dims(int p, int W, const int32_t N[3], int32_t gnb[3], int32_t br[3], int64_t &geo_n,
+                       int64_t &part_n) {
+    switch (p * 16 + W) {
+    case 1 * 16 + 8: sweep_dims_t<1, 8>(N, gnb, br, geo_n, part_n); break;
+    case 2 * 16 + 8: sweep_dims_t<2, 8>(N, gnb, br, geo_n, part_n); break;
+    case 3 * 16 + 8: sweep_dims_t<3, 8>(N, gnb, br, geo_n, part_n); break;
+    case 4 * 16 + 8: sweep_dims_t<4, 8>(N, gnb, br, geo_n, part_n); break;
+    case 1 * 16 + 4: sweep_dims_t<1, 4>(N, gnb, br, geo_n, part_n); break;
+    case 2 * 16 + 4: sweep_dims_t<2, 4>(N, gnb, br, geo_n, part_n); break;
+    case 3 * 16 + 4: sweep_dims_t<3, 4>(N, gnb, br, geo_n, part_n); break;
+    default: sweep_dims_t<4, 4>(N, gnb, br, geo_n, part_n); break;
     }
 }
 
@@ -1498,8 +1515,9 @@ static int32_t mf_create(const yeti_system_inputs *in, int32_t precisions, hipSt
             pd->nbrick[0][d] = (N[d] + 2 * np1 - 1) / (2 * np1);
             pd->nbrick[1][d] = (N[d] + np1 - 1) / np1;
         }
-        int64_t geo_n = 0, sweep_part_n = 0;
-        sweep_dims(p0, N, pd->mp.gnb, pd->sweep_br, geo_n, sweep_part_n);
+        int64_t geo_n64 = 0, part_n64 = 0, geo_n32 = 0, part_n32 = 0;
+        sweep_dims(p0, 8, N, pd->mp.gnb[0], pd->sweep_br[0], geo_n64, part_n64);
+        sweep_dims(p0, 4, N, pd->mp.gnb[1], pd->sweep_br[1], geo_n32, part_n32);
         double E = in->MATERIAL_PROPERTIES[(int64_t)in->n_mat_props_max * ip];
         double nu = in->MATERIAL_PROPERTIES[(int64_t)in->n_mat_props_max * ip + 1];
         pd->mp.lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
@@ -1518,16 +1536,16 @@ static int32_t mf_create(const yeti_system_inputs *in, int32_t precisions, hipSt
         const int bn0 = 2 * np1 + p0, bn1 = np1 + p0;
         const int64_t nb0 = (int64_t)pd->nbrick[0][0] * pd->nbrick[0][1] * pd->nbrick[0][2];
         const int64_t nb1 = (int64_t)pd->nbrick[1][0] * pd->nbrick[1][1] * pd->nbrick[1][2];
-        const int64_t part_n = std::max(sweep_part_n, std::max(nb0 * bn0 * bn0 * bn0 * 3, nb1 * bn1 * bn1 * bn1 * 9));
+        const int64_t part_n = std::max(part_n64, std::max(nb0 * bn0 * bn0 * bn0 * 3, nb1 * bn1 * bn1 * bn1 * 9));
         // geometry in FP64 always (it seeds the FP32 copy), in whole bricks of the sweep; brick
         // partials per precision
         (void)nel;
         (void)nq;
-        er = pd->geo64.alloc(geo_n);
-        if (er == hipSuccess && (precisions & 2)) er = pd->geo32.alloc(geo_n);
+        er = pd->geo64.alloc(geo_n64);
+        if (er == hipSuccess && (precisions & 2)) er = pd->geo32.alloc(geo_n32);
         // the FP64 partials also serve the (FP64) diagonal-block and strain-energy gradient passes
         if (er == hipSuccess) er = pd->part64.alloc(part_n);
-        if (er == hipSuccess && (precisions & 2)) er = pd->part32.alloc(sweep_part_n);
+        if (er == hipSuccess && (precisions & 2)) er = pd->part32.alloc(part_n32);
         if (er != hipSuccess) return fail(set_error(YETI_ERR_DEVICE, "matrix-free: allocation failed (out of memory?)"));
         if (!counting()) {
             const bool arena = alloc_ctx().arena != nullptr;

@@ -3,7 +3,7 @@
 //
 // Same operator as the element-wise contraction of mf.hip (mf_iga_get_ku_3d,
 // iga_wq_mf/fsrc/mf_iga_solver.f90:292-555, generalised to 3-component NURBS elasticity), but the
-// 1D contractions run over a whole brick of BR x BR x BZ elements instead of one element at a
+// 1D contractions run over a whole brick of BRX x BRY x BZ elements instead of one element at a
 // time: control points shared by neighbouring elements of the brick are contracted once, and the
 // sums over the elements that share a control point happen inside the backward contractions
 // instead of through a coloured scatter.
@@ -29,86 +29,100 @@
 
 namespace yeti {
 
-template <int P>
+// Brick shape and thread mapping per degree P and value width W (4: FP32, 8: FP64 bytes).
+// P = 1: the NG Gauss points g2 of a column in adjacent lanes (CPW = floor(64 / NG) columns per
+// wave, back-contractions summed over g2 with lane shuffles). P >= 2: one thread per column,
+// looping over g2; bricks sized so that the columns fill whole waves (P = 2: 7 x 3 elements,
+// 189 columns in 3 waves; P = 3: 2 x 2, one wave; P = 4: 2 x 2, 100 columns in 2 waves), so
+// several bricks share a CU. (With lanes per g2 a P = 4 brick needs 9 waves, one workgroup per
+// CU, and every barrier of a node layer stalls the CU.)
+template <int P, int W>
 struct Sweep {
     static constexpr int NP1 = P + 1, NG = P + 1;
-    static constexpr int BR = P == 1 ? 8 : 2;                  // brick edge in x and y (elements)
+    static constexpr bool PER_COLUMN = P >= 2;
+    static constexpr int BRX = P == 1 ? 8 : P == 2 ? 7 : 2;    // brick edge in x (elements)
+    static constexpr int BRY = P == 1 ? 8 : P == 2 ? 3 : 2;    // brick edge in y (elements)
     static constexpr int BZ = 8;                               // brick depth in z (elements)
-    static constexpr bool PREFETCH_GEO = false;                // next element layer's geometry held in registers (costs occupancy)
-    static constexpr int BN = BR + P;                          // control points per brick row
+    static constexpr int BNX = BRX + P, BNY = BRY + P;         // control points per brick row in x, y
     static constexpr int NZ = BZ + P;                          // node layers per brick
-    static constexpr int NA = BR * NG;                         // Gauss positions per brick row
-    static constexpr int NCOL = NA * NA;                       // Gauss-point columns per brick
+    static constexpr int NAX = BRX * NG, NAY = BRY * NG;       // Gauss positions per brick row in x, y
+    static constexpr int NCOL = NAX * NAY;                     // Gauss-point columns per brick
     static constexpr int NTQ = NG * NCOL;                      // Gauss points per element layer
-    static constexpr int CPW = 64 / NG;                        // columns per wave
+    static constexpr int CPW = PER_COLUMN ? 64 : 64 / NG;      // columns per wave
     static constexpr int NWAVE = (NCOL + CPW - 1) / CPW;
     static constexpr int NT = 64 * NWAVE;                      // threads per brick
     static constexpr int NGEO = 13;                            // stored geometry values per point
-    static constexpr int PART = NZ * BN * BN * 3;              // partial sums per brick
-    static_assert(BZ >= BR, "one table size for all directions");
+    static constexpr int PART = NZ * BNY * BNX * 3;            // partial sums per brick
+    static_assert(BZ >= BRX && BZ >= BRY, "one table size for all directions");
     static_assert(NT <= 1024, "brick too large for one workgroup");
 };
 
 // Brick grid of a patch for the sweep (elements per direction over the brick edges).
-template <int P>
+template <int P, int W>
 __host__ __device__ inline void sweep_grid(const int32_t N[3], int32_t nb[3]) {
-    nb[0] = (N[0] + Sweep<P>::BR - 1) / Sweep<P>::BR;
-    nb[1] = (N[1] + Sweep<P>::BR - 1) / Sweep<P>::BR;
-    nb[2] = (N[2] + Sweep<P>::BZ - 1) / Sweep<P>::BZ;
+    using S = Sweep<P, W>;
+    nb[0] = (N[0] + S::BRX - 1) / S::BRX;
+    nb[1] = (N[1] + S::BRY - 1) / S::BRY;
+    nb[2] = (N[2] + S::BZ - 1) / S::BZ;
 }
 
 // Offset of geometry value c of Gauss point (g0, g1, g2) of element (e1, e2, e3):
-// [brick][element layer in the brick][c][column][g2], column = a + NA b.
-template <int P>
+// [brick][element layer in the brick][c][column][g2], column = a + NAX b.
+template <int P, int W>
 __host__ __device__ inline int64_t sweep_geo_off(const int32_t nb[3], int e1, int e2, int e3, int c, int g0, int g1,
                                                  int g2) {
-    using S = Sweep<P>;
-    const int b1 = e1 / S::BR, b2 = e2 / S::BR, b3 = e3 / S::BZ;
+    using S = Sweep<P, W>;
+    const int b1 = e1 / S::BRX, b2 = e2 / S::BRY, b3 = e3 / S::BZ;
     const int64_t brick = b1 + (int64_t)nb[0] * (b2 + (int64_t)nb[1] * b3);
-    const int col = (e1 - b1 * S::BR) * S::NG + g0 + S::NA * ((e2 - b2 * S::BR) * S::NG + g1);
+    const int col = (e1 - b1 * S::BRX) * S::NG + g0 + S::NAX * ((e2 - b2 * S::BRY) * S::NG + g1);
     return (((brick * S::BZ + (e3 - b3 * S::BZ)) * S::NGEO + c) * S::NCOL + col) * S::NG + g2;
 }
 
-template <int P>
+template <int P, int W>
 inline int64_t sweep_geo_size(const int32_t nb[3]) {
-    return (int64_t)nb[0] * nb[1] * nb[2] * Sweep<P>::BZ * Sweep<P>::NGEO * Sweep<P>::NTQ;
+    using S = Sweep<P, W>;
+    return (int64_t)nb[0] * nb[1] * nb[2] * S::BZ * S::NGEO * S::NTQ;
 }
 
 template <typename T, int P>
 struct SweepSmem {
-    using S = Sweep<P>;
+    using S = Sweep<P, sizeof(T)>;
     T tab[3][2][S::BZ][S::NG][S::NP1];   // [direction][B | dB][element of the brick][g][i]
-    T X[3][S::BN][S::BN];                // node layer: [component][n2][n1] = w x
-    T Wn[S::NP1][S::BN][S::BN];          // weights of the ring's node layers: [slot][n2][n1]
-    T Y[3][2][S::BN][S::NA];             // stage A: [component][value | d/dxi0][n2][a]
+    T X[3][S::BNY][S::BNX];              // node layer: [component][n2][n1] = w x
+    T Wn[S::NP1][S::BNY][S::BNX];        // weights of the ring's node layers: [slot][n2][n1]
+    T Y[3][2][S::BNY][S::NAX];           // stage A: [component][value | d/dxi0][n2][a]
     T Z[S::NP1][9][S::NCOL];             // ring of node layers after stage B: [slot][3 f + var][column]
     T Ws[9][S::NCOL];                    // back-contractions of the completed node layer, summed over g2
-    T Pb[3][2][S::BN][S::NA];            // stage B^T: [component][P' | R0'][n2][a]
+    T Pb[3][2][S::BNY][S::NAX];          // stage B^T: [component][P' | R0'][n2][a]
 };
 
-// One brick: partial[brick] = its elements' contribution to K x at its (NZ x BN x BN) nodes.
-// Every thread of the workgroup (Sweep<P>::NT threads) calls it.
+// One brick: partial[brick] = its elements' contribution to K x at its (NZ x BNY x BNX) nodes.
+// Every thread of the workgroup (Sweep<P, sizeof(T)>::NT threads) calls it.
 template <typename T, int P>
 __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch &mp, const T *__restrict__ geo,
                                               const T *__restrict__ x, const T *__restrict__ wcp,
                                               T *__restrict__ partial, int64_t brick) {
-    using S = Sweep<P>;
-    constexpr int NP1 = S::NP1, NG = S::NG, BR = S::BR, BN = S::BN, NA = S::NA, NCOL = S::NCOL;
+    using S = Sweep<P, sizeof(T)>;
+    const int32_t *gnb = mp.gnb[sizeof(T) == 8 ? 0 : 1];
+    constexpr int NP1 = S::NP1, NG = S::NG, BRX = S::BRX, BRY = S::BRY, BNX = S::BNX, BNY = S::BNY, NAX = S::NAX,
+                  NCOL = S::NCOL;
+    constexpr bool PC = S::PER_COLUMN;
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int col = wave * S::CPW + lane / NG, g2 = lane % NG;
-    const bool valid = lane < S::CPW * NG && col < NCOL;
-    const int a = col % NA, b = col / NA;          // Gauss positions (x, y) of the column
+    const int col = PC ? tid : wave * S::CPW + lane / NG;
+    const int g2l = PC ? 0 : lane % NG;               // the thread's g2 (lanes per g2 only)
+    const bool valid = PC ? col < NCOL : (lane < S::CPW * NG && col < NCOL);
+    const int a = col % NAX, b = col / NAX;        // Gauss positions (x, y) of the column
     const int e1l = a / NG, g0 = a % NG, e2l = b / NG, g1 = b % NG;
-    const int b1 = (int)(brick % mp.gnb[0]), b2 = (int)((brick / mp.gnb[0]) % mp.gnb[1]),
-              b3 = (int)(brick / ((int64_t)mp.gnb[0] * mp.gnb[1]));
-    const int E1 = b1 * BR, E2 = b2 * BR, E3 = b3 * S::BZ;
-    const int ne1 = min(BR, mp.N[0] - E1), ne2 = min(BR, mp.N[1] - E2), ne3 = min(S::BZ, mp.N[2] - E3);
+    const int b1 = (int)(brick % gnb[0]), b2 = (int)((brick / gnb[0]) % gnb[1]),
+              b3 = (int)(brick / ((int64_t)gnb[0] * gnb[1]));
+    const int E1 = b1 * BRX, E2 = b2 * BRY, E3 = b3 * S::BZ;
+    const int ne1 = min(BRX, mp.N[0] - E1), ne2 = min(BRY, mp.N[1] - E2), ne3 = min(S::BZ, mp.N[2] - E3);
     const int nz = ne3 + P;
     const bool inside = valid && e1l < ne1 && e2l < ne2;
     const T lam = (T)mp.lam, mu = (T)mp.mu;
     T *out = partial + brick * (int64_t)S::PART;
-    const T *geo_b = geo + brick * (int64_t)S::BZ * S::NGEO * S::NTQ + (int64_t)col * NG + g2;
+    const T *geo_b = geo + brick * (int64_t)S::BZ * S::NGEO * S::NTQ + (int64_t)col * NG + g2l;
 
     // 1D tables of the brick's elements; zero past the patch, so contractions there give 0
     {
@@ -133,16 +147,19 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
 
     // back-contract node layer n3 from W[0] (all threads), write it, shift W down
     auto backward = [&](int n3) {
-        // sum over the NG lanes of the column (adjacent lanes of one wave)
+        // sum over the NG lanes of the column (adjacent lanes of one wave), or the thread's own
+        // sum over g2 (one thread per column)
         T s[9];
 #pragma unroll
         for (int o = 0; o < 9; ++o) {
             T v = W[0][o];
+            if constexpr (!PC) {
 #pragma unroll
-            for (int k = 1; k < NG; ++k) v += __shfl(W[0][o], lane + k, 64);
+                for (int k = 1; k < NG; ++k) v += __shfl(W[0][o], lane + k, 64);
+            }
             s[o] = v;
         }
-        if (valid && g2 == 0)
+        if (valid && g2l == 0)
 #pragma unroll
             for (int o = 0; o < 9; ++o) sm.Ws[o][col] = s[o];
 #pragma unroll
@@ -153,15 +170,15 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
         for (int o = 0; o < 9; ++o) W[P][o] = T(0);
         __syncthreads();
         // B^T: (a, n2, f) <- columns (a, e2 NG + g1) with n2 = e2 + j
-        for (int it = tid; it < NA * BN * 3; it += S::NT) {
-            const int aa = it % NA, n2 = (it / NA) % BN, f = it / (NA * BN);
+        for (int it = tid; it < NAX * BNY * 3; it += S::NT) {
+            const int aa = it % NAX, n2 = (it / NAX) % BNY, f = it / (NAX * BNY);
             T pv = T(0), r0 = T(0);
             for (int j = 0; j < NP1; ++j) {
                 const int e2 = n2 - j;
-                if (e2 < 0 || e2 >= BR) continue;
+                if (e2 < 0 || e2 >= BRY) continue;
 #pragma unroll
                 for (int g = 0; g < NG; ++g) {
-                    const int cc = aa + NA * (e2 * NG + g);
+                    const int cc = aa + NAX * (e2 * NG + g);
                     const T bv = sm.tab[1][0][e2][g][j], dv = sm.tab[1][1][e2][g][j];
                     pv += sm.Ws[f * 3 + 0][cc] * bv + sm.Ws[f * 3 + 2][cc] * dv;
                     r0 += sm.Ws[f * 3 + 1][cc] * bv;
@@ -172,13 +189,13 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
         }
         __syncthreads();
         // A^T: node (n1, n2) of layer n3 <- positions a = e1 NG + g0 with n1 = e1 + i
-        for (int it = tid; it < BN * BN; it += S::NT) {
-            const int n1 = it % BN, n2 = it / BN;
+        for (int it = tid; it < BNX * BNY; it += S::NT) {
+            const int n1 = it % BNX, n2 = it / BNX;
             if (n1 >= ne1 + P || n2 >= ne2 + P) continue;
             T y[3] = {T(0), T(0), T(0)};
             for (int i = 0; i < NP1; ++i) {
                 const int e1 = n1 - i;
-                if (e1 < 0 || e1 >= BR) continue;
+                if (e1 < 0 || e1 >= BRX) continue;
 #pragma unroll
                 for (int g = 0; g < NG; ++g) {
                     const T bv = sm.tab[0][0][e1][g][i], dv = sm.tab[0][1][e1][g][i];
@@ -187,7 +204,7 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
                 }
             }
             const T w = sm.Wn[n3 % NP1][n2][n1];
-            T *dst = out + ((int64_t)(n3 * BN + n2) * BN + n1) * 3;
+            T *dst = out + ((int64_t)(n3 * BNY + n2) * BNX + n1) * 3;
 #pragma unroll
             for (int f = 0; f < 3; ++f) dst[f] = y[f] * w;
         }
@@ -195,10 +212,10 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
 
     // Node-layer loads run one layer ahead (the CP ids two layers ahead), the geometry of the
     // next element layer one layer ahead, so their latency overlaps the contractions in between.
-    // Thread t < BN^2 loads node (t mod BN, t / BN) of each layer.
-    static_assert(BN * BN <= S::NT, "one thread per node of a layer");
-    const int ln1 = tid % BN, ln2 = tid / BN;
-    const bool loader = tid < BN * BN && ln1 < ne1 + P && ln2 < ne2 + P;
+    // Thread t < BNX BNY loads node (t mod BNX, t / BNX) of each layer.
+    static_assert(BNX * BNY <= S::NT, "one thread per node of a layer");
+    const int ln1 = tid % BNX, ln2 = tid / BNX;
+    const bool loader = tid < BNX * BNY && ln1 < ne1 + P && ln2 < ne2 + P;
     const int64_t cpl0 = (E1 + ln1) + (int64_t)mp.ncp[0] * ((E2 + ln2) + (int64_t)mp.ncp[1] * E3);
     const int64_t cpl_dz = (int64_t)mp.ncp[0] * mp.ncp[1];
     int cp_next = 0;          // CP of the loader's node in layer n3 + 1
@@ -211,18 +228,21 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
         xr[2] = x[3 * (int64_t)cp0 + 2];
         if (nz > 1) cp_next = mp.cpmap[cpl0 + cpl_dz];
     }
-    T gv[S::NGEO];            // geometry of the thread's Gauss point in the next element layer
-    auto load_geo = [&](int e3) {
-        const T *gq = geo_b + (int64_t)e3 * S::NGEO * S::NTQ;
-#pragma unroll
-        for (int c = 0; c < S::NGEO; ++c) gv[c] = inside ? gq[c * S::NTQ] : T(0);
-    };
-    if constexpr (S::PREFETCH_GEO) load_geo(0);
+    // Geometry of the thread's Gauss point in element layer n3 - P: loaded at the start of node
+    // layer n3, so the loads have the node-layer stages to arrive; the registers are live only
+    // where the quadrature-point work needs them anyway (loading the next layer's instead
+    // would keep 13 more registers live there and cost a wave per SIMD).
+    T gv[S::NGEO];
 
     for (int n3 = 0; n3 < nz; ++n3) {
         const int slot = n3 % NP1;
+        if (!PC && n3 >= P && inside) {
+            const T *gq = geo_b + (int64_t)(n3 - P) * S::NGEO * S::NTQ;
+#pragma unroll
+            for (int c = 0; c < S::NGEO; ++c) gv[c] = gq[c * S::NTQ];
+        }
         // ---- forward: node layer n3 into ring slot n3 mod (P+1) ----
-        if (tid < BN * BN) {
+        if (tid < BNX * BNY) {
             sm.X[0][ln2][ln1] = wr * xr[0];
             sm.X[1][ln2][ln1] = wr * xr[1];
             sm.X[2][ln2][ln1] = wr * xr[2];
@@ -238,8 +258,8 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
         }
         __syncthreads();
         // stage A: (a, n2, f) = sum_i X[f][n2][e1 + i] (B | dB)[e1][g0][i]
-        for (int it = tid; it < NA * BN * 3; it += S::NT) {
-            const int aa = it % NA, n2 = (it / NA) % BN, f = it / (NA * BN);
+        for (int it = tid; it < NAX * BNY * 3; it += S::NT) {
+            const int aa = it % NAX, n2 = (it / NAX) % BNY, f = it / (NAX * BNY);
             const int e1 = aa / NG, g = aa % NG;
             T v = T(0), d = T(0);
 #pragma unroll
@@ -255,7 +275,7 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
         // stage B: column (a, b), outputs o = 3 f + var (value, d/dxi0, d/dxi1) dealt over the
         // column's NG lanes
         if (valid) {
-            for (int o = g2; o < 9; o += NG) {
+            for (int o = g2l; o < 9; o += PC ? 1 : NG) {
                 const int f = o / 3, var = o % 3;
                 const int src = var == 1 ? 1 : 0;
                 T s = T(0);
@@ -268,8 +288,15 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
         __syncthreads();
         const int e3 = n3 - P;
         if (e3 < 0) continue;
-        // ---- element layer e3: the thread's Gauss point (column, g2) ----
+        // ---- element layer e3: the thread's Gauss point (column, g2), or each g2 of its column ----
         if (inside) {
+#pragma unroll 1
+          for (int g2 = g2l; g2 < (PC ? NG : g2l + 1); ++g2) {
+            if constexpr (PC) {
+                const T *gq = geo_b + (int64_t)e3 * S::NGEO * S::NTQ + g2;
+#pragma unroll
+                for (int c = 0; c < S::NGEO; ++c) gv[c] = gq[c * S::NTQ];
+            }
             T V[3], G[3][3];   // G[m][f]: d/dxi_m of component f
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
@@ -289,7 +316,6 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
                 G[1][f] = d1;
                 G[2][f] = d2;
             }
-            if constexpr (!S::PREFETCH_GEO) load_geo(e3);
             T A[9], gw[3];
 #pragma unroll
             for (int c = 0; c < 9; ++c) A[c] = gv[c];
@@ -297,8 +323,6 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
             gw[1] = gv[10];
             gw[2] = gv[11];
             const T om = gv[12];
-            if constexpr (S::PREFETCH_GEO)
-                if (e3 + 1 < ne3) load_geo(e3 + 1);
             // H[l][k] = d u_l / d x_k = sum_m (G[m][l] - gw[m] V[l]) A[m][k]
             T H[3][3];
 #pragma unroll
@@ -340,6 +364,7 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
                     W[k][f * 3 + 2] += Tm[f][1] * bz;
                 }
             }
+          }
         }
         backward(e3);
     }

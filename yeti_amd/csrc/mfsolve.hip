@@ -1038,8 +1038,13 @@ static int batch_upload(yeti_batch *b, hipStream_t st) {
     std::vector<BTask> lists[BT_NLISTS];
     for (int32_t mi = 0; mi < (int32_t)b->hreq.size(); ++mi) {
         const BReq &R = b->hreq[mi];
-        const int64_t nbr = (int64_t)R.nbrick[0] * R.nbrick[1] * R.nbrick[2];
+        const int pi = b->prec64 ? 0 : 1;
+        const int64_t nbr = (int64_t)R.nbrick[pi][0] * R.nbrick[pi][1] * R.nbrick[pi][2];
         for (int64_t k = 0; k < nbr; ++k) lists[BT_BRICK1 + R.p - 1].push_back({mi, (int32_t)k, 0, 0});
+        if (!b->prec64) {   // the FP64 replacement apply sweeps the FP64 bricks
+            const int64_t nb64 = (int64_t)R.nbrick[0][0] * R.nbrick[0][1] * R.nbrick[0][2];
+            for (int64_t k = 0; k < nb64; ++k) lists[BT_RBRICK1 + R.p - 1].push_back({mi, (int32_t)k, 0, 0});
+        }
         for (int64_t k = 0; k < (R.ncp + BATCH_GATHER_BLOCK - 1) / BATCH_GATHER_BLOCK; ++k)
             lists[BT_GATHER].push_back({mi, (int32_t)k, 0, 0});
         for (int32_t k = 0; k < R.nvchunk; ++k) lists[BT_VEC].push_back({mi, k, 0, 0});
