@@ -525,7 +525,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = every core of this host")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--serve-rate", type=float, default=40.0, help="offered requests per second")
+    ap.add_argument("--serve-rate", type=float, default=60.0, help="offered requests per second")
     ap.add_argument("--serve-plain", type=int, default=100,
                     help="requests solved one at a time (plain FP64), from the start of the stream")
     ap.add_argument("--grad-fd", type=int, default=50, help="coordinates checked by central differences")
