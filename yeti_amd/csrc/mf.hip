@@ -812,9 +812,26 @@ static bool gen_eigen(int n, const std::vector<double> &K, const std::vector<dou
 // Element (row, col) of operand X of component c, batch z, is at X[off[c] + z bs[c] + row rs[c]
 // + col cs[c]].
 
-// 64 x 64 tile of C per 256-thread workgroup, K in steps of 16 through LDS, 4 x 4 outputs per
-// thread (rows tx + 16 i, columns ty + 16 j), accumulated in T. grid = (N tiles, M tiles,
-// 3 x batches).
+// K depth of the preconditioner GEMM's LDS tiles: fewer, larger staging rounds, each a memory
+// round trip before its matrix-core steps.
+constexpr int FDM_KC = 16;
+
+// One 16 x 16 x 4 matrix-core step, D = A B + D, in FP32 or FP64: lane l holds A[l & 15][l >> 4]
+// and B[l >> 4][l & 15]; the 4 outputs of a lane are D[i][l & 15] with i = 4 (l >> 4) + r in
+// FP32 and i = 4 r + (l >> 4) in FP64 (both checked on the device).
+template <typename T> struct MfmaAcc;
+template <> struct MfmaAcc<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct MfmaAcc<double> { typedef double type __attribute__((ext_vector_type(4))); };
+__device__ __forceinline__ MfmaAcc<float>::type mfma_16x16x4(float a, float b, MfmaAcc<float>::type c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ MfmaAcc<double>::type mfma_16x16x4(double a, double b, MfmaAcc<double>::type c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// 64 x 64 tile of C per 256-thread workgroup, K in steps of FDM_KC through LDS, each wave a 32 x 32
+// quarter on the matrix cores (2 x 2 blocks of 16 x 16), accumulated in T. grid = (N tiles,
+// M tiles, 3 x batches).
 // One 64 x 64 tile (rows from mt, columns from nt) of component c, batch z; every thread of the
 // workgroup calls it.
 template <typename T>
@@ -823,25 +840,26 @@ __device__ __forceinline__ void fdm_gemm_tile(const FdmGemm &g, const FdmArgs &f
                                               const T *__restrict__ invD, const T *__restrict__ r, T (*Ls)[65],
                                               T (*Rs)[65]) {
     const int M = g.M[c], N = g.N[c], K = g.K[c];
-    const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int rbase = 32 * (wave >> 1), cbase = 32 * (wave & 1);
     const T *Lb = L + g.loff[c] + z * g.lbs[c];
     const T *Rb = R + g.roff[c] + z * g.rbs[c];
-    T acc[4][4];
+    typename MfmaAcc<T>::type acc[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+        for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = typename MfmaAcc<T>::type{T(0), T(0), T(0), T(0)};
     const int m1c = fa.m[c][1];
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int k0 = 0; k0 < K; k0 += FDM_KC) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {   // L tile, 64 rows x 16: lanes along the rows
+        for (int q = 0; q < FDM_KC / 4; ++q) {   // L tile, 64 rows x FDM_KC: lanes along the rows
             const int mm = t & 63, kk = (t >> 6) + 4 * q;
             const int row = mt + mm, col = k0 + kk;
             Ls[kk][mm] = (row < M && col < K) ? Lb[row * g.lrs[c] + col * g.lcs[c]] : T(0);
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {   // R tile, 16 x 64 columns: lanes along K
-            const int kk = t & 15, nn = (t >> 4) + 16 * q;
+        for (int q = 0; q < FDM_KC / 4; ++q) {   // R tile, FDM_KC x 64 columns: lanes along K
+            const int kk = t % FDM_KC, nn = t / FDM_KC + (256 / FDM_KC) * q;
             const int row = k0 + kk, col = nt + nn;
             T v = T(0);
             if (row < K && col < N) {
@@ -857,32 +875,40 @@ __device__ __forceinline__ void fdm_gemm_tile(const FdmGemm &g, const FdmArgs &f
             Rs[kk][nn] = v;
         }
         __syncthreads();
+        // wave w: rows 32 (w / 2) + 16 rb, columns 32 (w % 2) + 16 cb, MFMA K steps of 4
 #pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-            T a[4], b[4];
+        for (int k4 = 0; k4 < FDM_KC; k4 += 4) {
+            T a[2], b[2];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = Ls[kk][tx + 16 * i];
+            for (int rb = 0; rb < 2; ++rb) a[rb] = Ls[k4 + (lane >> 4)][rbase + 16 * rb + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Rs[kk][ty + 16 * j];
+            for (int cb = 0; cb < 2; ++cb) b[cb] = Rs[k4 + (lane >> 4)][cbase + 16 * cb + (lane & 15)];
+            // C^T = R^T L^T on the matrix cores, so that a lane's outputs share a row of C and
+            // consecutive lanes hold consecutive rows (C's rows are its contiguous index)
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+                for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = mfma_16x16x4(b[cb], a[rb], acc[rb][cb]);
         }
         __syncthreads();
     }
+    // accumulator (rb, cb), register r of lane l: tile row 16 rb + (l & 15), column 16 cb + i, with
+    // i = 4 (l >> 4) + r (FP32) or 4 r + (l >> 4) (FP64)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int row = mt + tx + 16 * i, col = nt + ty + 16 * j;
-            if (row < M && col < N) {
-                const int64_t lin = z * g.cbs[c] + row * g.crs[c] + col * g.ccs[c];   // index in the component
-                T v = acc[i][j];
-                if (g.scale) v *= invD[fa.doff[c] + lin];
-                Cout[g.coff[c] + lin] = v;
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = sizeof(T) == 4 ? 4 * (lane >> 4) + r : 4 * r + (lane >> 4);
+                const int row = mt + rbase + 16 * rb + (lane & 15), col = nt + cbase + 16 * cb + i;
+                if (row < M && col < N) {
+                    const int64_t lin = z * g.cbs[c] + row * g.crs[c] + col * g.ccs[c];   // index in the component
+                    T v = acc[rb][cb][r];
+                    if (g.scale) v *= invD[fa.doff[c] + lin];
+                    Cout[g.coff[c] + lin] = v;
+                }
             }
-        }
 }
 
 template <typename T>
@@ -895,8 +921,8 @@ __global__ void __launch_bounds__(256) fdm_gemm_kernel(FdmGemm g, FdmArgs fa, co
     if (z >= g.nbatch[c]) return;
     const int mt = blockIdx.y * 64, nt = blockIdx.x * 64;
     if (mt >= g.M[c] || nt >= g.N[c]) return;
-    __shared__ T Ls[16][65];
-    __shared__ T Rs[16][65];
+    __shared__ T Ls[FDM_KC][65];
+    __shared__ T Rs[FDM_KC][65];
     fdm_gemm_tile<T>(g, fa, c, z, mt, nt, L, R, Cout, invD, r, Ls, Rs);
 }
 
@@ -1223,8 +1249,8 @@ __global__ void __launch_bounds__(256) bt_gather_kernel(BatchDev bd) {
 // Tiles of the preconditioner's mode product `prod` of every active member.
 template <typename T>
 __global__ void __launch_bounds__(256) bt_fdm_gemm_kernel(BatchDev bd, int prod) {
-    __shared__ T Ls[16][65];
-    __shared__ T Rs[16][65];
+    __shared__ T Ls[FDM_KC][65];
+    __shared__ T Rs[FDM_KC][65];
     const int list = BT_GEMM0 + prod;
     const int32_t cnt = bd.count[list];
     const BTask *tl = bd.list[list];
