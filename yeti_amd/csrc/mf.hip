@@ -832,9 +832,77 @@ __device__ __forceinline__ MfmaAcc<double>::type mfma_16x16x4(double a, double b
 // 64 x 64 tile of C per 256-thread workgroup, K in steps of FDM_KC through LDS, each wave a 32 x 32
 // quarter on the matrix cores (2 x 2 blocks of 16 x 16), accumulated in T. grid = (N tiles,
 // M tiles, 3 x batches).
-// One 64 x 64 tile (rows from mt, columns from nt) of component c, batch z; every thread of the
-// workgroup calls it.
+// The operand loads of a K step are issued together from clamped addresses (no load behind a
+// branch; what falls outside the operands is replaced by zero when it is written to LDS), and
+// the next step's loads are in flight while the matrix cores work on the current one, so a tile
+// costs about one memory round trip per K step rather than one per load.
 template <typename T>
+__device__ __forceinline__ void fdm_load_L(const FdmGemm &g, int c, int mt, int k0, const T *__restrict__ Lb,
+                                           T (&lv)[FDM_KC / 4]) {
+    const int M = g.M[c], K = g.K[c], t = threadIdx.x;
+    const int64_t lrs = g.lrs[c], lcs = g.lcs[c];
+#pragma unroll
+    for (int q = 0; q < FDM_KC / 4; ++q) {   // L, 64 rows x FDM_KC: lanes along the rows
+        const int row = mt + (t & 63), col = k0 + (t >> 6) + 4 * q;
+        lv[q] = Lb[min(row, M - 1) * lrs + min(col, K - 1) * lcs];
+    }
+}
+
+// R, FDM_KC x 64 columns, lanes along K: read as stored
+template <typename T>
+__device__ __forceinline__ void fdm_load_R(const FdmGemm &g, int c, int nt, int k0, const T *__restrict__ Rb,
+                                           T (&rv)[FDM_KC / 4]) {
+    const int N = g.N[c], K = g.K[c], t = threadIdx.x;
+    const int rk = min(k0 + t % FDM_KC, K - 1);
+    const int64_t rrs = g.rrs[c], rcs = g.rcs[c];
+#pragma unroll
+    for (int q = 0; q < FDM_KC / 4; ++q) rv[q] = Rb[rk * rrs + min(nt + t / FDM_KC + (256 / FDM_KC) * q, N - 1) * rcs];
+}
+
+// ... or gathered from the DOF vector r: the CP ids of a step are loaded one step ahead of its
+// values, so the dependent load does not stall the step.
+__device__ __forceinline__ void fdm_load_ids(const FdmGemm &g, const FdmArgs &fa, int c, int nt, int k0,
+                                             int32_t (&id)[FDM_KC / 4]) {
+    const int N = g.N[c], K = g.K[c], t = threadIdx.x;
+    const int rk = min(k0 + t % FDM_KC, K - 1), m1c = fa.m[c][1];
+#pragma unroll
+    for (int q = 0; q < FDM_KC / 4; ++q) {
+        const int col = min(nt + t / FDM_KC + (256 / FDM_KC) * q, N - 1);
+        const int j = col % m1c, k = col / m1c;
+        const int64_t gp = (fa.lo[c][0] + rk) +
+                           (int64_t)fa.n[0] * ((fa.lo[c][1] + j) + (int64_t)fa.n[1] * (fa.lo[c][2] + k));
+        id[q] = fa.cpmap[gp];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void fdm_load_gathered(int c, const T *__restrict__ r, const int32_t (&id)[FDM_KC / 4],
+                                                  T (&rv)[FDM_KC / 4]) {
+#pragma unroll
+    for (int q = 0; q < FDM_KC / 4; ++q) rv[q] = r[3 * (int64_t)id[q] + c];
+}
+
+// The step's operands into LDS, zero outside L (M x K) and R (K x N).
+template <typename T>
+__device__ __forceinline__ void fdm_store_step(int M, int N, int K, int mt, int nt, int k0, const T (&lv)[FDM_KC / 4],
+                                               const T (&rv)[FDM_KC / 4], T (*Ls)[65], T (*Rs)[65]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < FDM_KC / 4; ++q) {
+        const int kk = (t >> 6) + 4 * q;
+        Ls[kk][t & 63] = (mt + (t & 63) < M && k0 + kk < K) ? lv[q] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < FDM_KC / 4; ++q) {
+        const int kk = t % FDM_KC, nn = t / FDM_KC + (256 / FDM_KC) * q;
+        Rs[kk][nn] = (k0 + kk < K && nt + nn < N) ? rv[q] : T(0);
+    }
+}
+
+// One 64 x 64 tile (rows from mt, columns from nt) of component c, batch z; every thread of the
+// workgroup calls it. GATHER = g.gather, a template argument so that the loads in flight across
+// a K step are not merged from two code paths (which would wait for them).
+template <typename T, bool GATHER>
 __device__ __forceinline__ void fdm_gemm_tile(const FdmGemm &g, const FdmArgs &fa, int c, int z, int mt, int nt,
                                               const T *__restrict__ L, const T *__restrict__ R, T *__restrict__ Cout,
                                               const T *__restrict__ invD, const T *__restrict__ r, T (*Ls)[65],
@@ -849,32 +917,30 @@ __device__ __forceinline__ void fdm_gemm_tile(const FdmGemm &g, const FdmArgs &f
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
         for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = typename MfmaAcc<T>::type{T(0), T(0), T(0), T(0)};
-    const int m1c = fa.m[c][1];
+    T lv[FDM_KC / 4], rv[FDM_KC / 4];
+    int32_t idn[FDM_KC / 4];
+    if (K > 0) {
+        fdm_load_L<T>(g, c, mt, 0, Lb, lv);
+        if (GATHER) {
+            fdm_load_ids(g, fa, c, nt, 0, idn);
+            fdm_load_gathered<T>(c, r, idn, rv);
+            if (FDM_KC < K) fdm_load_ids(g, fa, c, nt, FDM_KC, idn);
+        } else {
+            fdm_load_R<T>(g, c, nt, 0, Rb, rv);
+        }
+    }
     for (int k0 = 0; k0 < K; k0 += FDM_KC) {
-#pragma unroll
-        for (int q = 0; q < FDM_KC / 4; ++q) {   // L tile, 64 rows x FDM_KC: lanes along the rows
-            const int mm = t & 63, kk = (t >> 6) + 4 * q;
-            const int row = mt + mm, col = k0 + kk;
-            Ls[kk][mm] = (row < M && col < K) ? Lb[row * g.lrs[c] + col * g.lcs[c]] : T(0);
-        }
-#pragma unroll
-        for (int q = 0; q < FDM_KC / 4; ++q) {   // R tile, FDM_KC x 64 columns: lanes along K
-            const int kk = t % FDM_KC, nn = t / FDM_KC + (256 / FDM_KC) * q;
-            const int row = k0 + kk, col = nt + nn;
-            T v = T(0);
-            if (row < K && col < N) {
-                if (g.gather) {
-                    const int j = col % m1c, k = col / m1c;
-                    const int64_t gp = (fa.lo[c][0] + row) +
-                                       (int64_t)fa.n[0] * ((fa.lo[c][1] + j) + (int64_t)fa.n[1] * (fa.lo[c][2] + k));
-                    v = r[3 * (int64_t)fa.cpmap[gp] + c];
-                } else {
-                    v = Rb[row * g.rrs[c] + col * g.rcs[c]];
-                }
-            }
-            Rs[kk][nn] = v;
-        }
+        fdm_store_step<T>(M, N, K, mt, nt, k0, lv, rv, Ls, Rs);
         __syncthreads();
+        if (k0 + FDM_KC < K) {   // the next step's operands
+            fdm_load_L<T>(g, c, mt, k0 + FDM_KC, Lb, lv);
+            if (GATHER) {
+                fdm_load_gathered<T>(c, r, idn, rv);
+                if (k0 + 2 * FDM_KC < K) fdm_load_ids(g, fa, c, nt, k0 + 2 * FDM_KC, idn);
+            } else {
+                fdm_load_R<T>(g, c, nt, k0 + FDM_KC, Rb, rv);
+            }
+        }
         // wave w: rows 32 (w / 2) + 16 rb, columns 32 (w % 2) + 16 cb, MFMA K steps of 4
 #pragma unroll
         for (int k4 = 0; k4 < FDM_KC; k4 += 4) {
@@ -893,7 +959,23 @@ __device__ __forceinline__ void fdm_gemm_tile(const FdmGemm &g, const FdmArgs &f
         __syncthreads();
     }
     // accumulator (rb, cb), register r of lane l: tile row 16 rb + (l & 15), column 16 cb + i, with
-    // i = 4 (l >> 4) + r (FP32) or 4 r + (l >> 4) (FP64)
+    // i = 4 (l >> 4) + r (FP32) or 4 r + (l >> 4) (FP64); the scaling loads are issued together
+    // (clamped addresses) before the stores
+    const int64_t cbs = g.cbs[c], crs = g.crs[c], ccs = g.ccs[c];
+    T sc[2][2][4];
+    const bool scale = g.scale;
+    if (scale) {
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = sizeof(T) == 4 ? 4 * (lane >> 4) + r : 4 * r + (lane >> 4);
+                    const int row = mt + rbase + 16 * rb + (lane & 15), col = nt + cbase + 16 * cb + i;
+                    sc[rb][cb][r] = invD[fa.doff[c] + z * cbs + min(row, M - 1) * crs + min(col, N - 1) * ccs];
+                }
+    }
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -902,12 +984,9 @@ __device__ __forceinline__ void fdm_gemm_tile(const FdmGemm &g, const FdmArgs &f
             for (int r = 0; r < 4; ++r) {
                 const int i = sizeof(T) == 4 ? 4 * (lane >> 4) + r : 4 * r + (lane >> 4);
                 const int row = mt + rbase + 16 * rb + (lane & 15), col = nt + cbase + 16 * cb + i;
-                if (row < M && col < N) {
-                    const int64_t lin = z * g.cbs[c] + row * g.crs[c] + col * g.ccs[c];   // index in the component
-                    T v = acc[rb][cb][r];
-                    if (g.scale) v *= invD[fa.doff[c] + lin];
-                    Cout[g.coff[c] + lin] = v;
-                }
+                if (row < M && col < N)
+                    Cout[g.coff[c] + z * cbs + row * crs + col * ccs] =
+                        scale ? acc[rb][cb][r] * sc[rb][cb][r] : acc[rb][cb][r];
             }
 }
 
@@ -923,7 +1002,10 @@ __global__ void __launch_bounds__(256) fdm_gemm_kernel(FdmGemm g, FdmArgs fa, co
     if (mt >= g.M[c] || nt >= g.N[c]) return;
     __shared__ T Ls[FDM_KC][65];
     __shared__ T Rs[FDM_KC][65];
-    fdm_gemm_tile<T>(g, fa, c, z, mt, nt, L, R, Cout, invD, r, Ls, Rs);
+    if (g.gather)
+        fdm_gemm_tile<T, true>(g, fa, c, z, mt, nt, L, R, Cout, invD, r, Ls, Rs);
+    else
+        fdm_gemm_tile<T, false>(g, fa, c, z, mt, nt, L, R, Cout, invD, r, Ls, Rs);
 }
 
 // GEMM descriptor of the mode product along `axis` with A[a][b] = P[b m + a] (transposed_store)
@@ -974,6 +1056,38 @@ static void fdm_product(const FdmArgs &fa, int axis, const T *P, bool transposed
     hipLaunchKernelGGL(fdm_gemm_kernel<T>, grid, dim3(256), 0, st, g, fa, Lp, Rp, out, invD, r, pred);
 }
 
+// z at grid point g (0 outside the component's free box and on masked dofs), its terms of r.z
+// added to acc in component order. The grid has fewer than 2^31 points (the set-up checks), so its index arithmetic is
+// 32-bit; every load is issued from a clamped address and the value selected afterwards, so the
+// three components' loads are in flight together.
+template <typename T>
+__device__ __forceinline__ void fdm_scatter_point(const FdmArgs &fa, int32_t g, const T *__restrict__ w,
+                                                  const T *__restrict__ r, const uint8_t *__restrict__ mask,
+                                                  T *__restrict__ z, double &acc) {
+    const uint32_t n0 = fa.n[0], n1 = fa.n[1], q = (uint32_t)g / n0;
+    const int i = (int)((uint32_t)g - q * n0), j = (int)(q % n1), k = (int)(q / n1);
+    const int64_t cp = fa.cpmap[g];
+    T wv[3], rv[3];
+    uint8_t mk[3];
+    bool in[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int ii = i - fa.lo[c][0], jj = j - fa.lo[c][1], kk = k - fa.lo[c][2];
+        in[c] = ii >= 0 && ii < fa.m[c][0] && jj >= 0 && jj < fa.m[c][1] && kk >= 0 && kk < fa.m[c][2];
+        const int ic = max(0, min(ii, fa.m[c][0] - 1)), jc = max(0, min(jj, fa.m[c][1] - 1)),
+                  kc = max(0, min(kk, fa.m[c][2] - 1));
+        wv[c] = w[c * fa.maxtot + ic + (int64_t)fa.m[c][0] * (jc + (int64_t)fa.m[c][1] * kc)];
+        mk[c] = mask[3 * cp + c];
+        rv[c] = r[3 * cp + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const T v = (in[c] && mk[c]) ? wv[c] : T(0);
+        z[3 * cp + c] = v;
+        acc += (double)rv[c] * (double)v;
+    }
+}
+
 // z on every CP (0 outside the free boxes and on masked dofs); r.z reduced in the same launch
 // and handed to rz_tail (the session's scalars: first direction, or beta and the iteration count)
 template <typename T>
@@ -984,18 +1098,8 @@ __global__ void __launch_bounds__(256) fdm_scatter_kernel(FdmArgs fa, int64_t nc
     if (pred && *pred == 0) return;
     __shared__ double sh[4];
     double acc = 0.0;
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ncp; g += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(g % fa.n[0]), j = (int)((g / fa.n[0]) % fa.n[1]), k = (int)(g / ((int64_t)fa.n[0] * fa.n[1]));
-        const int64_t cp = fa.cpmap[g];
-        for (int c = 0; c < 3; ++c) {
-            const int ii = i - fa.lo[c][0], jj = j - fa.lo[c][1], kk = k - fa.lo[c][2];
-            T v = T(0);
-            if (ii >= 0 && ii < fa.m[c][0] && jj >= 0 && jj < fa.m[c][1] && kk >= 0 && kk < fa.m[c][2] && mask[3 * cp + c])
-                v = w[c * fa.maxtot + ii + (int64_t)fa.m[c][0] * (jj + (int64_t)fa.m[c][1] * kk)];
-            z[3 * cp + c] = v;
-            acc += (double)r[3 * cp + c] * (double)v;
-        }
-    }
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ncp; g += (int64_t)gridDim.x * blockDim.x)
+        fdm_scatter_point<T>(fa, (int32_t)g, w, r, mask, z, acc);
     double tot;
     if (last_block_total(block_sum256(acc, sh), part, tail.flags + F_COUNTER, sh, tot) && threadIdx.x == 0)
         rz_tail(tail, tot);
@@ -1061,6 +1165,8 @@ int fdm_create(const yeti_mf *m, const uint8_t *mask_host, int precision, hipStr
     FdmArgs &fa = f->fa;
     for (int d = 0; d < 3; ++d) fa.n[d] = pd.mp.ncp[d];
     const int64_t ngrid = (int64_t)fa.n[0] * fa.n[1] * fa.n[2];
+    if (ngrid > INT32_MAX)
+        return fail(set_error(YETI_ERR_INVALID_ARGUMENT, "fast diagonalisation: more than 2^31 - 1 control points"));
     f->ncp = (int32_t)ngrid;
     // fixed dofs: whole faces per component
     for (int c = 0; c < 3; ++c) {
@@ -1262,8 +1368,13 @@ __global__ void __launch_bounds__(256) bt_fdm_gemm_kernel(BatchDev bd, int prod)
         const int c = tk.a, z = tk.b;
         const int ntn = (g.N[c] + 63) / 64;
         __syncthreads();   // the previous tile's reads of Ls / Rs are done
-        fdm_gemm_tile<T>(g, R.fa, c, z, (tk.c / ntn) * 64, (tk.c % ntn) * 64, (const T *)R.L[prod],
-                         (const T *)R.R[prod], (T *)R.C[prod], (const T *)R.invD, (const T *)R.vr, Ls, Rs);
+        const int mt = (tk.c / ntn) * 64, nt = (tk.c % ntn) * 64;
+        if (g.gather)
+            fdm_gemm_tile<T, true>(g, R.fa, c, z, mt, nt, (const T *)R.L[prod], (const T *)R.R[prod],
+                                   (T *)R.C[prod], (const T *)R.invD, (const T *)R.vr, Ls, Rs);
+        else
+            fdm_gemm_tile<T, false>(g, R.fa, c, z, mt, nt, (const T *)R.L[prod], (const T *)R.R[prod],
+                                    (T *)R.C[prod], (const T *)R.invD, (const T *)R.vr, Ls, Rs);
     }
 }
 
@@ -1284,19 +1395,8 @@ __global__ void __launch_bounds__(256) bt_fdm_scatter_kernel(BatchDev bd, int mo
         T *z = (T *)R.vz;
         const int64_t g0 = (int64_t)tk.a * R.cchunk, g1 = std::min<int64_t>(R.ncp, g0 + R.cchunk);
         double acc = 0.0;
-        for (int64_t g = g0 + threadIdx.x; g < g1; g += blockDim.x) {
-            const int i = (int)(g % fa.n[0]), j = (int)((g / fa.n[0]) % fa.n[1]), k = (int)(g / ((int64_t)fa.n[0] * fa.n[1]));
-            const int64_t cp = fa.cpmap[g];
-            for (int c = 0; c < 3; ++c) {
-                const int ii = i - fa.lo[c][0], jj = j - fa.lo[c][1], kk = k - fa.lo[c][2];
-                T v = T(0);
-                if (ii >= 0 && ii < fa.m[c][0] && jj >= 0 && jj < fa.m[c][1] && kk >= 0 && kk < fa.m[c][2] &&
-                    R.mask[3 * cp + c])
-                    v = w[c * fa.maxtot + ii + (int64_t)fa.m[c][0] * (jj + (int64_t)fa.m[c][1] * kk)];
-                z[3 * cp + c] = v;
-                acc += (double)r[3 * cp + c] * (double)v;
-            }
-        }
+        for (int64_t g = g0 + threadIdx.x; g < g1; g += blockDim.x)
+            fdm_scatter_point<T>(fa, (int32_t)g, w, r, R.mask, z, acc);
         double tot;
         if (member_total(block_sum256(acc, sh), R.red, R.flags + F_COUNTER, R.ncchunk, tk.a, sh, tot) &&
             threadIdx.x == 0) {
