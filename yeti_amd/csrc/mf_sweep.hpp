@@ -13,18 +13,19 @@
 // a = e1 NG + g0, value and x-derivative) and along y (stage B: -> Gauss-point columns
 // (a, b = e2 NG + g1): value, x- and y-derivative of the 3 components, 9 numbers per column) into
 // a ring of the last P+1 node layers in LDS. Once the P+1 node layers of element layer e3 are in
-// the ring, each thread takes one Gauss point (column, g2) of the layer: the z contraction
-// (stage C), the quadrature-point work (the same 13 stored geometry numbers per point as the
-// element-wise kernel) and the transposed z contraction into its P+1 back-contraction registers
-// (one per node layer e3 .. e3+P, shifted down after every element layer). Node layer e3 is then
-// complete: its back-contractions are summed over g2 (lanes of one column are adjacent in a wave),
-// contracted back along y (B^T) and x (A^T), scaled by the weights and written to the brick's
-// partial sums; bricks overlap by P node planes, and a gather sums the bricks covering each
-// control point in a fixed order. No floating-point atomics: the apply is bit-reproducible.
+// the ring, the Gauss points of the layer are visited: the z contraction (stage C), the
+// quadrature-point work (the same 13 stored geometry numbers per point as the element-wise
+// kernel) and the transposed z contraction into P+1 back-contraction registers (one per node
+// layer e3 .. e3+P, shifted down after every element layer). Node layer e3 is then complete: its
+// back-contractions, summed over g2, are contracted back along y (B^T) and x (A^T), scaled by
+// the weights and written to the brick's partial sums; bricks overlap by P node planes, and a
+// gather sums the bricks covering each control point in a fixed order. No floating-point
+// atomics: the apply is bit-reproducible.
 //
-// Threads: the NG Gauss points g2 of a column sit in adjacent lanes, CPW = floor(64 / NG) columns
-// per wave. The stored geometry is laid out per (brick, element layer, value) in that thread
-// order, so each value is one contiguous read per wave.
+// Threads (Sweep below): at P = 1 one thread per Gauss point, the NG points g2 of a column in
+// adjacent lanes (the sum over g2 by lane shuffles); at P >= 2 one thread per column, looping
+// over g2 (the sum over g2 in registers). The stored geometry is laid out per (brick, element
+// layer, value) as [column][g2], so each value is one contiguous read per wave either way.
 #pragma once
 
 namespace yeti {
