@@ -511,7 +511,8 @@ __global__ void mf_gather_kernel(MfPatch mp, BrickGrid bg, const T *__restrict__
 
 // The apply as brick sweeps: one workgroup of Sweep<P, sizeof(T)>::NT threads per brick.
 template <typename T, int P>
-__global__ void __launch_bounds__((Sweep<P, sizeof(T)>::NT)) mf_sweep_kernel(MfPatch mp, const T *__restrict__ geo,
+__global__ void __launch_bounds__((Sweep<P, sizeof(T)>::NT)) __attribute__((amdgpu_waves_per_eu(Sweep<P, sizeof(T)>::WPE)))
+mf_sweep_kernel(MfPatch mp, const T *__restrict__ geo,
                                                                 const T *__restrict__ x, const T *__restrict__ wcp,
                                                                 T *__restrict__ partial,
                                                                 const int32_t *__restrict__ pred) {
@@ -1320,7 +1321,8 @@ void fdm_destroy(yeti_fdm *f) { delete f; }
 // precision T (active members), or (RPL) the FP64 apply of the replacement x -> FP64 partials
 // (members with a replacement due).
 template <typename T, int P, bool RPL>
-__global__ void __launch_bounds__((Sweep<P, sizeof(T)>::NT)) bt_sweep_kernel(BatchDev bd, int list) {
+__global__ void __launch_bounds__((Sweep<P, sizeof(T)>::NT)) __attribute__((amdgpu_waves_per_eu(Sweep<P, sizeof(T)>::WPE)))
+bt_sweep_kernel(BatchDev bd, int list) {
     __shared__ SweepSmem<T, P> sm;
     constexpr int pi = sizeof(T) == 8 ? 0 : 1;
     const int32_t cnt = bd.count[list];

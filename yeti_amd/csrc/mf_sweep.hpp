@@ -54,6 +54,9 @@ struct Sweep {
     static constexpr int NT = 64 * NWAVE;                      // threads per brick
     static constexpr int NGEO = 13;                            // stored geometry values per point
     static constexpr int PART = NZ * BNY * BNX * 3;            // partial sums per brick
+    // waves per SIMD the register allocation is held to (the bricks' LDS allows more)
+    static constexpr int WPE = P == 1 ? (W == 4 ? 4 : 3) : P == 2 ? (W == 4 ? 4 : 2) : P == 3 ? (W == 4 ? 3 : 2)
+                                                                                               : (W == 4 ? 3 : 2);
     static_assert(BZ >= BRX && BZ >= BRY, "one table size for all directions");
     static_assert(NT <= 1024, "brick too large for one workgroup");
 };
@@ -92,7 +95,9 @@ struct SweepSmem {
     T X[3][S::BNY][S::BNX];              // node layer: [component][n2][n1] = w x
     T Wn[S::NP1][S::BNY][S::BNX];        // weights of the ring's node layers: [slot][n2][n1]
     T Y[3][2][S::BNY][S::NAX];           // stage A: [component][value | d/dxi0][n2][a]
-    T Z[S::NP1][9][S::NCOL];             // ring of node layers after stage B: [slot][3 f + var][column]
+    // ring of node layers after stage B, [slot][3 f + var][column]: shared only at P = 1, where a
+    // column's g2 points are separate threads; with one thread per column it lives in registers
+    T Z[S::PER_COLUMN ? 1 : S::NP1][9][S::PER_COLUMN ? 1 : S::NCOL];
     T Ws[9][S::NCOL];                    // back-contractions of the completed node layer, summed over g2
     T Pb[3][2][S::BNY][S::NAX];          // stage B^T: [component][P' | R0'][n2][a]
 };
@@ -141,10 +146,17 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
         }
     }
     T W[NP1][9];   // back-contractions for node layers e3 .. e3 + P of the thread's Gauss point
+    // one thread per column: its stage-B outputs of node layers n3 - P .. n3 (Zr[k] = layer
+    // n3 - P + k), shifted down as a layer comes in
+    T Zr[PC ? NP1 : 1][9];
 #pragma unroll
     for (int k = 0; k < NP1; ++k)
 #pragma unroll
         for (int o = 0; o < 9; ++o) W[k][o] = T(0);
+#pragma unroll
+    for (int k = 0; k < (PC ? NP1 : 1); ++k)
+#pragma unroll
+        for (int o = 0; o < 9; ++o) Zr[k][o] = T(0);
 
     // back-contract node layer n3 from W[0] (all threads), write it, shift W down
     auto backward = [&](int n3) {
@@ -273,20 +285,41 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
             sm.Y[f][1][n2][aa] = d;
         }
         __syncthreads();
-        // stage B: column (a, b), outputs o = 3 f + var (value, d/dxi0, d/dxi1) dealt over the
-        // column's NG lanes
-        if (valid) {
-            for (int o = g2l; o < 9; o += PC ? 1 : NG) {
-                const int f = o / 3, var = o % 3;
-                const int src = var == 1 ? 1 : 0;
-                T s = T(0);
+        // stage B: column (a, b), outputs o = 3 f + var (value, d/dxi0, d/dxi1): into the
+        // thread's registers (one thread per column), or dealt over the column's NG lanes
+        if constexpr (PC) {
 #pragma unroll
-                for (int j = 0; j < NP1; ++j)
-                    s += sm.Y[f][src][e2l + j][a] * sm.tab[1][var == 2 ? 1 : 0][e2l][g1][j];
-                sm.Z[slot][o][col] = s;
+            for (int k = 0; k < P; ++k)
+#pragma unroll
+                for (int o = 0; o < 9; ++o) Zr[k][o] = Zr[k + 1][o];
+            if (valid) {
+#pragma unroll
+                for (int o = 0; o < 9; ++o) {
+                    const int f = o / 3, var = o % 3;
+                    const int src = var == 1 ? 1 : 0;
+                    T s = T(0);
+#pragma unroll
+                    for (int j = 0; j < NP1; ++j)
+                        s += sm.Y[f][src][e2l + j][a] * sm.tab[1][var == 2 ? 1 : 0][e2l][g1][j];
+                    Zr[P][o] = s;
+                }
             }
+            // no barrier: the ring is the thread's own, and Y is rewritten only after the next
+            // layer's first barrier
+        } else {
+            if (valid) {
+                for (int o = g2l; o < 9; o += NG) {
+                    const int f = o / 3, var = o % 3;
+                    const int src = var == 1 ? 1 : 0;
+                    T s = T(0);
+#pragma unroll
+                    for (int j = 0; j < NP1; ++j)
+                        s += sm.Y[f][src][e2l + j][a] * sm.tab[1][var == 2 ? 1 : 0][e2l][g1][j];
+                    sm.Z[slot][o][col] = s;
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
         const int e3 = n3 - P;
         if (e3 < 0) continue;
         // ---- element layer e3: the thread's Gauss point (column, g2), or each g2 of its column ----
@@ -306,11 +339,11 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
                 for (int k = 0; k < NP1; ++k) {
                     const int sk = (e3 + k) % NP1;
                     const T bz = sm.tab[2][0][e3][g2][k], dz = sm.tab[2][1][e3][g2][k];
-                    const T zv = sm.Z[sk][f * 3 + 0][col];
+                    const T zv = PC ? Zr[k][f * 3 + 0] : sm.Z[sk][f * 3 + 0][col];
                     v += zv * bz;
                     d2 += zv * dz;
-                    d0 += sm.Z[sk][f * 3 + 1][col] * bz;
-                    d1 += sm.Z[sk][f * 3 + 2][col] * bz;
+                    d0 += (PC ? Zr[k][f * 3 + 1] : sm.Z[sk][f * 3 + 1][col]) * bz;
+                    d1 += (PC ? Zr[k][f * 3 + 2] : sm.Z[sk][f * 3 + 2][col]) * bz;
                 }
                 V[f] = v;
                 G[0][f] = d0;
