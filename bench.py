@@ -133,6 +133,19 @@ def bench_serving(n, rate, seed, budget, plain_n):
         del srv
         torch.cuda.empty_cache()
     err = max(float(np.linalg.norm(xs["mixed"][k] - xs["fp64"][k]) / np.linalg.norm(xs["fp64"][k])) for k in xs["fp64"])
+    # capacity: the first cap_n requests of the stream all admitted at t = 0, served to the end
+    # (solves/s at saturation, where the offered rate no longer bounds the throughput)
+    cap_n = min(n, 400)
+    for prec in ("mixed", "fp64"):
+        srv = SolveServer(budget_bytes=budget, precision=prec, chunk=16, rtol=1e-8, batched=True)
+        rep = srv.run(reqs[:cap_n], realtime=False)
+        out[prec]["capacity"] = {"requests": cap_n, "solves_per_s": rep["solves_per_s"],
+                                 "makespan_s": rep["makespan_s"], "failed": rep["failed"]}
+        log("serving %s capacity: %d requests at t = 0 in %.2f s, %.2f solves/s"
+            % (prec, cap_n, rep["makespan_s"], rep["solves_per_s"]))
+        srv.close()
+        del srv
+        torch.cuda.empty_cache()
     # plain FP64 GPU path: each request alone, one after the other, eager launches, one stream
     sub = reqs[:plain_n]
     lat = []
@@ -152,11 +165,15 @@ def bench_serving(n, rate, seed, budget, plain_n):
     out["max_rel_err_mixed_vs_fp64"] = err
     out["speedup_mixed_vs_plain_fp64"] = out["mixed"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]
     out["speedup_mixed_vs_served_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    out["capacity_mixed_vs_served_fp64"] = (out["mixed"]["capacity"]["solves_per_s"] /
+                                            out["fp64"]["capacity"]["solves_per_s"])
+    out["capacity_mixed_vs_plain_fp64"] = out["mixed"]["capacity"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]
     out["workload"] = ("%d requests, Poisson arrivals at %.1f/s, elements per direction log-uniform in [4,48], "
                        "p in {2,3,4}, E~U[1e3,1e5], nu~U[0.2,0.4], traction~U[0.1,10], solved to 1e-8 "
                        "(FP64 residual), fast-diagonalisation PCG, arena %.0f GB; served: one batch of sessions "
                        "advanced by shared launches (graph per 16 iterations); plain FP64: the first %d requests "
-                       "solved alone, one after the other, eager launches" % (n, rate, budget / 1e9, len(sub)))
+                       "solved alone, one after the other, eager launches; capacity: the first %d requests "
+                       "admitted at t = 0" % (n, rate, budget / 1e9, len(sub), cap_n))
     return out
 
 
@@ -679,6 +696,21 @@ def main():
     del u, gs, vol, gvol
     torch.cuda.empty_cache()
 
+    def grad_work(ms, nf):
+        """Algorithmic work of one gradient call with nf fields (modes) against the HBM and FP64
+        peaks. Per element: IEN, weights, coordinates, spans and the fields' nodal values read,
+        per-node partials and the mass written, then gathered per CP; flops: R and dR/dx per
+        (point, node) (~26), u and du/dx per (point, field) (24 per node), the tensor contraction
+        per (point, node, field) (18)."""
+        n = (p + 1) ** 3
+        nq = (p + 1) ** 3
+        byts = bp.nb_elem * (4 * n + 8 * n + 24 * n + 12 + nf * (24 * n + 24 * n + 8)) + nf * 24 * bp.nb_cp
+        flops = bp.nb_elem * nq * n * (26 + nf * (24 + 18))
+        s = ms * 1e-3
+        return {"algorithmic_bytes": byts, "achieved_gbs": byts / s / 1e9, "hbm_frac": byts / s / 1e9 / HBM_PEAK_GBS,
+                "flops": flops, "achieved_tflops": flops / s / 1e12, "fp64_frac": flops / s / 1e12 / 78.6,
+                "bound": "latency (one workgroup per element, 5 barriers per Gauss-point chunk)"}
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
@@ -723,9 +755,11 @@ def main():
                         "roofline": {"bound": "hbm", "achieved": mass_bytes / (mass_ms * 1e-3) / 1e9,
                                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                      "frac": mass_bytes / (mass_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}},
-        "vibration_gradient": {"ms": vib_ms, "modes": nmode, "elements_per_s": bp.nb_elem / (vib_ms * 1e-3)},
-        "strain_energy_gradient": {"ms": seg_ms, "elements_per_s": bp.nb_elem / (seg_ms * 1e-3)},
-        "volume_gradient": {"ms": vol_ms, "elements_per_s": bp.nb_elem / (vol_ms * 1e-3)},
+        "vibration_gradient": {"ms": vib_ms, "modes": nmode, "elements_per_s": bp.nb_elem / (vib_ms * 1e-3),
+                               "work": grad_work(vib_ms, nmode)},
+        "strain_energy_gradient": {"ms": seg_ms, "elements_per_s": bp.nb_elem / (seg_ms * 1e-3),
+                                   "work": grad_work(seg_ms, 1)},
+        "volume_gradient": {"ms": vol_ms, "elements_per_s": bp.nb_elem / (vol_ms * 1e-3), "work": grad_work(vol_ms, 1)},
     }
     sections = set() if args.only_assembly else set(args.sections.split(","))
     if "c2" in sections:
