@@ -71,6 +71,9 @@ struct ElemCfg {
     // wrote wrong values for the second and later elements of a workgroup (a defect not traced
     // to the source); the with-indices instantiation agrees with the oracle in both uses.
     static constexpr bool VALUES_VIA_IDX = NNP >= 128;
+    // consumers read one MFMA K step of their fragments per LDS read instead of two (the A and B
+    // fragments then take half the registers), where the accumulators leave no room for more
+    static constexpr bool SPLIT_K = NNP >= 128;
     static constexpr int MAXU = 8;                      // units per consumer wave per pass
     static constexpr int MAXF = 12;                     // distinct A (or B) fragments per wave
     static constexpr int MAXPASS = 8;
@@ -564,6 +567,30 @@ __device__ __forceinline__ void mfma_chunk(const ElemSmem<P, NG> &sm, int buf, C
     if constexpr (nt > 0) {
         const int lane = threadIdx.x & 63;
         const int nl = lane & 15, kg = lane >> 4;
+        if constexpr (C::SPLIT_K) {
+            // one MFMA K step per LDS read: half the fragment registers
+#pragma unroll 1
+            for (int t = 0; t < C::QC / 4; ++t) {
+                const int qo = 8 * (t >> 1) + 2 * kg + (t & 1);
+                double Af[na], Bf[nb];
+                static_for<na>([&](auto ic) {
+                    constexpr int x = decltype(ic)::value;
+                    constexpr int key = Sched<P, NG>::S.afr[PASS][W][x];
+                    Af[x] = sm.G[buf][key % 3][16 * (key / 3) + nl][qo];
+                });
+                static_for<nb>([&](auto ic) {
+                    constexpr int x = decltype(ic)::value;
+                    constexpr int key = Sched<P, NG>::S.bfr[PASS][W][x];
+                    Bf[x] = sm.G[buf][key % 3][16 * (key / 3) + nl][qo];
+                });
+                static_for<nt>([&](auto ic) {
+                    constexpr int u = decltype(ic)::value;
+                    constexpr int ai = Sched<P, NG>::S.aidx[PASS][W][u], bi = Sched<P, NG>::S.bidx[PASS][W][u];
+                    acc.S[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(Af[ai], Bf[bi], acc.S[u], 0, 0, 0);
+                });
+            }
+            return;
+        }
 #pragma unroll 1
         for (int t = 0; t < C::QC / 8; ++t) {
             const int qo = 8 * t + 2 * kg;
