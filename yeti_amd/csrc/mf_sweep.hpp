@@ -93,7 +93,9 @@ struct SweepSmem {
     using S = Sweep<P, sizeof(T)>;
     T tab[3][2][S::BZ][S::NG][S::NP1];   // [direction][B | dB][element of the brick][g][i]
     T X[3][S::BNY][S::BNX];              // node layer: [component][n2][n1] = w x
-    T Wn[S::NP1][S::BNY][S::BNX];        // weights of the ring's node layers: [slot][n2][n1]
+    // weights of the node layers, [slot][n2][n1]: a ring of P + 2 slots, so that the loader's
+    // write of layer n3 + 1 cannot meet another thread's read of layer n3 - P in A^T
+    T Wn[S::NP1 + 1][S::BNY][S::BNX];
     T Y[3][2][S::BNY][S::NAX];           // stage A: [component][value | d/dxi0][n2][a]
     // ring of node layers after stage B, [slot][3 f + var][column]: shared only at P = 1, where a
     // column's g2 points are separate threads; with one thread per column it lives in registers
@@ -201,25 +203,24 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
             sm.Pb[f][1][n2][aa] = r0;
         }
         __syncthreads();
-        // A^T: node (n1, n2) of layer n3 <- positions a = e1 NG + g0 with n1 = e1 + i
-        for (int it = tid; it < BNX * BNY; it += S::NT) {
-            const int n1 = it % BNX, n2 = it / BNX;
+        // A^T: component f of node (n1, n2) of layer n3 <- positions a = e1 NG + g0 with
+        // n1 = e1 + i (one thread per node and component)
+        for (int it = tid; it < BNX * BNY * 3; it += S::NT) {
+            const int f = it / (BNX * BNY), nd = it - f * (BNX * BNY);
+            const int n1 = nd % BNX, n2 = nd / BNX;
             if (n1 >= ne1 + P || n2 >= ne2 + P) continue;
-            T y[3] = {T(0), T(0), T(0)};
+            T y = T(0);
             for (int i = 0; i < NP1; ++i) {
                 const int e1 = n1 - i;
                 if (e1 < 0 || e1 >= BRX) continue;
 #pragma unroll
                 for (int g = 0; g < NG; ++g) {
                     const T bv = sm.tab[0][0][e1][g][i], dv = sm.tab[0][1][e1][g][i];
-#pragma unroll
-                    for (int f = 0; f < 3; ++f) y[f] += sm.Pb[f][0][n2][e1 * NG + g] * bv + sm.Pb[f][1][n2][e1 * NG + g] * dv;
+                    y += sm.Pb[f][0][n2][e1 * NG + g] * bv + sm.Pb[f][1][n2][e1 * NG + g] * dv;
                 }
             }
-            const T w = sm.Wn[n3 % NP1][n2][n1];
-            T *dst = out + ((int64_t)(n3 * BNY + n2) * BNX + n1) * 3;
-#pragma unroll
-            for (int f = 0; f < 3; ++f) dst[f] = y[f] * w;
+            const T w = sm.Wn[n3 % (NP1 + 1)][n2][n1];
+            out[((int64_t)(n3 * BNY + n2) * BNX + n1) * 3 + f] = y * w;
         }
     };
 
@@ -259,7 +260,7 @@ __device__ __forceinline__ void mf_sweep_body(SweepSmem<T, P> &sm, const MfPatch
             sm.X[0][ln2][ln1] = wr * xr[0];
             sm.X[1][ln2][ln1] = wr * xr[1];
             sm.X[2][ln2][ln1] = wr * xr[2];
-            sm.Wn[slot][ln2][ln1] = wr;
+            sm.Wn[n3 % (NP1 + 1)][ln2][ln1] = wr;
         }
         if (loader && n3 + 1 < nz) {
             const int cp = cp_next;
