@@ -65,6 +65,12 @@ struct ElemCfg {
     // with indices, values only unchanged.)
     static constexpr bool PROD_COPY = true;
     static constexpr int CPW = PROD_COPY ? CW + PW : CW;
+    // The deferred slices of the last round, copied beside the next element's steps, are left to
+    // the consumers: in those steps the producers' chunk building is the longer task (measured
+    // per wave with the shader clock at p = 3: producers busy, consumers waiting at the step
+    // barrier), while the blocking rounds keep all 12 waves.
+    static constexpr bool PROD_DEFER_COPY = false;
+    static constexpr int DPW = PROD_DEFER_COPY ? CW + PW : CW;
     static constexpr int MAXTILE = NNP >= 128 ? 9 : 12; // accumulator tiles per consumer wave per pass
     // Values-only runs use the instantiation that writes indices, with null index pointers:
     // the values-only instantiation at p = 4 (NNP = 128, ~620 bytes of register spills per lane)
@@ -815,7 +821,7 @@ __device__ __forceinline__ void deferred_copy(const ElemSmem<P, NG> &sm, int pas
                                               double *__restrict__ Kdata, int32_t *__restrict__ Krow,
                                               int32_t *__restrict__ Kcol) {
     using C = ElemCfg<P, NG>;
-    constexpr int NW = C::CPW;
+    constexpr int NW = C::DPW;
     if (pass == PASS) {
         constexpr int R = Sched<P, NG>::S.nround[PASS] - 1;
         if (part < 0)
@@ -947,18 +953,18 @@ elem_stiffness_kernel(PatchArgs pa, const double *__restrict__ geo, double *__re
             const int rem = s % SPE, ch = rem % C::NCH;
             if (ch == C::NCH - 1) {
                 producer_epilogue<P, NG, WRITE_IDX>(sm, rem / C::NCH, base, Kdata, Krow, Kcol);
-                if constexpr (elem_defer_last<P, NG>() && C::PROD_COPY) {
+                if constexpr (elem_defer_last<P, NG>() && C::PROD_DEFER_COPY) {
                     pend_pass = rem / C::NCH;
                     pend_base = base;
                 }
-            } else if (elem_defer_last<P, NG>() && C::PROD_COPY && pend_pass >= 0) {
+            } else if (elem_defer_last<P, NG>() && C::PROD_DEFER_COPY && pend_pass >= 0) {
                 deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, ch, pend_base, Kdata, Krow, Kcol);
             }
             if (s1 % SPE == 0) base = next_base;
             __syncthreads();
         }
         // an unfinished deferred round (the stage is stable after the last barrier)
-        if constexpr (elem_defer_last<P, NG>() && C::PROD_COPY)
+        if constexpr (elem_defer_last<P, NG>() && C::PROD_DEFER_COPY)
             if (pend_pass >= 0) deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, -1, pend_base, Kdata, Krow, Kcol);
     } else {
         __syncthreads();
