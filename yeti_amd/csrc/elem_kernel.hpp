@@ -300,93 +300,114 @@ struct PatchArgs {
 // independent loads. J is the
 // derivative with respect to the knot parameters (shap.f90:133-153); w |det J| includes the
 // parent-to-parameter factors (shap.f90:156-178) and the Gauss weights (UELMAT.f90:118-119).
-// A workgroup of 4 waves takes 4 GEO_EPW elements: their control points (x, y, z, w) are
-// gathered once into LDS with one load per node, then each lane evaluates one Gauss point
-// (several when NQ > 64) from LDS broadcasts.
+// One wave per element, 4 per workgroup. The 16 sums a Gauss point needs (W, dW/dxi_m, and
+// sum w x_c R and its xi derivatives, before the division by W) are tensor-product contractions
+// of the four nodal quantities (w, w x, w y, w z), evaluated by sum factorisation through LDS:
+// along xi_0 (values and derivatives at the NG abscissae), then xi_1, then xi_2, one lane per
+// Gauss point in the last stage (NP1^3 NG + ... multiply-adds per quantity instead of NP1^3 NG^3).
 template <int NQ>
-constexpr int geo_epw() { return NQ >= 64 ? 1 : 64 / NQ; }   // elements per wave
+constexpr int geo_epw() { return 1; }   // elements per wave
 
 template <int P, int NG>
 __global__ void __launch_bounds__(256) elem_geo_kernel(PatchArgs pa, double *__restrict__ geo) {
     using C = ElemCfg<P, NG>;
     constexpr int NP1 = C::NP1, NN = C::NN, NQ = C::NQ;
-    constexpr int EPW = geo_epw<NQ>(), EPB = 4 * EPW;
-    constexpr int QPL = (NQ + 63) / 64;              // Gauss points per lane
-    __shared__ double cp[EPB][NN][4];
-    const int64_t gb0 = (int64_t)blockIdx.x * EPB;
-    for (int x = threadIdx.x; x < EPB * NN; x += 256) {
-        const int le = x / NN, a = x - le * NN;
-        const int64_t gi = gb0 + le;
-        if (gi < pa.n_active) {
-            const int e = pa.elems[gi];
-            const double *xp = pa.coords + 3 * (int64_t)(pa.ien[(int64_t)NN * e + a] - 1);
-            cp[le][a][0] = xp[0];
-            cp[le][a][1] = xp[1];
-            cp[le][a][2] = xp[2];
-            cp[le][a][3] = pa.weight[(int64_t)NN * e + a];
-        }
-    }
-    __syncthreads();
+    constexpr int EPB = 4;                              // elements per workgroup, one per wave
+    constexpr int NT1 = 4 * 2 * NG * NP1 * NP1;         // stage 0: [quantity][B | dB][g0][k][j]
+    constexpr int NT2 = 4 * 3 * NG * NG * NP1;          // stage 1: [quantity][v][g0][g1][k]
+    __shared__ double cp[EPB][4][NN];                   // (w, w x, w y, w z) per node
+    __shared__ double tab[EPB][3][2][NG][NP1];          // [d][B | dB][g][i], i <-> basis P - i
+    __shared__ double hh[EPB][3];                       // half span per direction
+    __shared__ double T1[EPB][NT1];
+    __shared__ double T2[EPB][NT2];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int le = wv * EPW + (EPW > 1 ? lane / NQ : 0);
-    const int64_t gi = gb0 + le;
-    if ((EPW > 1 && lane >= EPW * NQ) || gi >= pa.n_active) return;
-    const int e = pa.elems[gi];
-    for (int r = 0; r < QPL; ++r) {
-        const int q = EPW > 1 ? lane % NQ : lane + 64 * r;
-        if (q >= NQ) break;
-        const int gq[3] = {q % NG, (q / NG) % NG, q / (NG * NG)};
-        double B[3][NP1], dB[3][NP1], half[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
+    const int64_t gi = (int64_t)blockIdx.x * EPB + wv;
+    const bool act = gi < pa.n_active;   // wave-uniform; inactive waves still meet the barriers
+    const int e = act ? pa.elems[gi] : 0;
+    double *rec = geo + gi * C::REC;
+    if (act) {
+        for (int a = lane; a < NN; a += 64) {
+            const double *xp = pa.coords + 3 * (int64_t)(pa.ien[(int64_t)NN * e + a] - 1);
+            const double w = pa.weight[(int64_t)NN * e + a];
+            cp[wv][0][a] = w;
+            cp[wv][1][a] = w * xp[0];
+            cp[wv][2][a] = w * xp[1];
+            cp[wv][3][a] = w * xp[2];
+            rec[ELEM_NGEO * NQ + C::NTAB + a] = w;
+        }
+        if (lane < 3 * NG) {   // 1D bases of direction d at abscissa g
+            const int d = lane / NG, g = lane - d * NG;
             const double *U = d == 0 ? pa.U0 : d == 1 ? pa.U1 : pa.U2;
             const int span1 = pa.nijk[3 * (int64_t)e + d];
             const double u0 = U[span1 - 1], u1 = U[span1];
-            const double u = ((u1 - u0) * pa.g.x[gq[d]] + (u1 + u0)) * 0.5;
+            const double u = ((u1 - u0) * pa.g.x[g] + (u1 + u0)) * 0.5;
             double N[NP1], dN[NP1];
             bspline_ders<P>(span1 - 1, u, U, N, dN);
+            double *tb = rec + ELEM_NGEO * NQ + (d * NG + g) * NP1;
 #pragma unroll
             for (int i = 0; i < NP1; ++i) {   // local index i <-> basis function P - i
-                B[d][i] = N[P - i];
-                dB[d][i] = dN[P - i];
+                tab[wv][d][0][g][i] = N[P - i];
+                tab[wv][d][1][g][i] = dN[P - i];
+                tb[i] = N[P - i];
+                tb[3 * NG * NP1 + i] = dN[P - i];
             }
-            half[d] = 0.5 * (u1 - u0);
+            if (g == 0) hh[wv][d] = 0.5 * (u1 - u0);
         }
-        double *rec = geo + gi * C::REC;
+    }
+    __syncthreads();
+    if (act) {   // stage 0: contract i (xi_0)
+        for (int o = lane; o < NT1; o += 64) {
+            const int j = o % NP1, k = (o / NP1) % NP1, g0 = (o / (NP1 * NP1)) % NG, v = (o / (NP1 * NP1 * NG)) % 2,
+                      qn = o / (NP1 * NP1 * NG * 2);
+            double s = 0.0;
 #pragma unroll
-        for (int d = 0; d < 3; ++d)   // each direction's table from the lane whose other two indices are 0
-            if (gq[(d + 1) % 3] == 0 && gq[(d + 2) % 3] == 0) {
-                double *tb = rec + ELEM_NGEO * NQ + (d * NG + gq[d]) * NP1;
+            for (int i = 0; i < NP1; ++i) s += tab[wv][0][v][g0][i] * cp[wv][qn][i + NP1 * (j + NP1 * k)];
+            T1[wv][o] = s;
+        }
+    }
+    __syncthreads();
+    if (act) {   // stage 1: contract j (xi_1); v = 0: B0 B1, 1: dB0 B1, 2: B0 dB1
+        for (int o = lane; o < NT2; o += 64) {
+            const int k = o % NP1, g1 = (o / NP1) % NG, g0 = (o / (NP1 * NG)) % NG, v = (o / (NP1 * NG * NG)) % 3,
+                      qn = o / (NP1 * NG * NG * 3);
+            const double *t1 = &T1[wv][(((qn * 2 + (v == 1 ? 1 : 0)) * NG + g0) * NP1 + k) * NP1];
+            double s = 0.0;
 #pragma unroll
-                for (int i = 0; i < NP1; ++i) {
-                    tb[i] = B[d][i];
-                    tb[3 * NG * NP1 + i] = dB[d][i];
-                }
+            for (int j = 0; j < NP1; ++j) s += t1[j] * tab[wv][1][v == 2 ? 1 : 0][g1][j];
+            T2[wv][o] = s;
+        }
+    }
+    __syncthreads();
+    if (!act) return;
+    // stage 2: contract k (xi_2), one lane per Gauss point, and the point's Jacobian work
+    for (int q = lane; q < NQ; q += 64) {
+        const int g0 = q % NG, g1 = (q / NG) % NG, g2 = q / (NG * NG);
+        double s[16];   // 0: W, 1-3: dW/dxi_m, 4 + c: sum w x_c R, 7 + 3 c + m: its xi_m derivative
+#pragma unroll
+        for (int qn = 0; qn < 4; ++qn) {
+            double val = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
+            const double *t2 = &T2[wv][((qn * 3) * NG * NG + g0 * NG + g1) * NP1];
+#pragma unroll
+            for (int k = 0; k < NP1; ++k) {
+                const double bz = tab[wv][2][0][g2][k], dz = tab[wv][2][1][g2][k];
+                const double t0 = t2[k];
+                val += t0 * bz;
+                d2 += t0 * dz;
+                d0 += t2[NG * NG * NP1 + k] * bz;
+                d1 += t2[2 * NG * NG * NP1 + k] * bz;
             }
-        for (int a = q; a < NN; a += NQ) rec[ELEM_NGEO * NQ + C::NTAB + a] = cp[le][a][3];
-        double s[16];
-#pragma unroll
-        for (int m = 0; m < 16; ++m) s[m] = 0.0;
-#pragma unroll
-        for (int k = 0; k < NP1; ++k)
-#pragma unroll
-            for (int j = 0; j < NP1; ++j)
-#pragma unroll
-                for (int i = 0; i < NP1; ++i) {
-                    const int a = i + NP1 * (j + NP1 * k);
-                    const double x0 = cp[le][a][0], x1 = cp[le][a][1], x2 = cp[le][a][2], w = cp[le][a][3];
-                    const double b12 = B[1][j] * B[2][k] * w;
-                    const double Na = B[0][i] * b12;
-                    const double d0 = dB[0][i] * b12;
-                    const double d1 = B[0][i] * dB[1][j] * B[2][k] * w;
-                    const double d2 = B[0][i] * B[1][j] * dB[2][k] * w;
-                    s[0] += Na;
-                    s[1] += d0; s[2] += d1; s[3] += d2;
-                    s[4] += x0 * Na; s[5] += x1 * Na; s[6] += x2 * Na;
-                    s[7] += x0 * d0; s[8] += x0 * d1; s[9] += x0 * d2;
-                    s[10] += x1 * d0; s[11] += x1 * d1; s[12] += x1 * d2;
-                    s[13] += x2 * d0; s[14] += x2 * d1; s[15] += x2 * d2;
-                }
+            if (qn == 0) {
+                s[0] = val;
+                s[1] = d0;
+                s[2] = d1;
+                s[3] = d2;
+            } else {
+                s[3 + qn] = val;
+                s[7 + 3 * (qn - 1) + 0] = d0;
+                s[7 + 3 * (qn - 1) + 1] = d1;
+                s[7 + 3 * (qn - 1) + 2] = d2;
+            }
+        }
         const double Wi = 1.0 / s[0];
         const double dw[3] = {s[1] * Wi, s[2] * Wi, s[3] * Wi};
         double J[3][3];
@@ -398,7 +419,7 @@ __global__ void __launch_bounds__(256) elem_geo_kernel(PatchArgs pa, double *__r
         const double m01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
         const double m02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
         const double det = J[0][0] * m00 + J[0][1] * m01 + J[0][2] * m02;
-        const double om = pa.g.w[gq[0]] * pa.g.w[gq[1]] * pa.g.w[gq[2]] * fabs(det * half[0] * half[1] * half[2]);
+        const double om = pa.g.w[g0] * pa.g.w[g1] * pa.g.w[g2] * fabs(det * hh[wv][0] * hh[wv][1] * hh[wv][2]);
         const double f = sqrt(om) * Wi / det;
         double A[9];   // A[3 m + c] = f * (adj J)[m][c]
         A[0] = m00 * f;
