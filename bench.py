@@ -72,14 +72,14 @@ def cpu_baseline(nel, p, slab, runs, threads):
 
     bp = make_box_patch(nel=(nel, nel, slab), p=p)
     t_s, ts_s = timed(bp.f2py_args(), "%dx%dx%d p=%d" % (nel, nel, slab, p))
-    c1 = make_box_patch(nel=(16, 4, 4), p=2)
-    t_c1, _ = timed(c1.f2py_args(), "16x4x4 p=2")
+    small = make_box_patch(nel=(16, 4, 4), p=2)
+    t_small, _ = timed(small.f2py_args(), "16x4x4 p=2")
     return {"value": bp.nb_elem / t_s, "unit": "elem/s", "cores": threads, "kind": kind,
             "sample": "%dx%dx%d p=%d patch of the benchmark's generator (%d elements), "
                       "sys_linmat_lindef_static_omp incl. output allocation, median of %d calls after a "
                       "warm-up: %.2f s (runs %s)" % (nel, nel, slab, p, bp.nb_elem, runs, t_s,
                                                     ", ".join("%.2f" % t for t in ts_s)),
-            "c1_16x4x4_p2": {"elements": c1.nb_elem, "seconds": t_c1, "value": c1.nb_elem / t_c1,
+            "cantilever_16x4x4_p2": {"elements": small.nb_elem, "seconds": t_small, "value": small.nb_elem / t_small,
                              "unit": "elem/s"}}
 
 
@@ -177,8 +177,8 @@ def bench_serving(n, rate, seed, budget, plain_n):
     return out
 
 
-def bench_c2_solve(nel, p, jacobi_iters):
-    """The C2 solve after the assembly: device CSR of K + K^T, its nodal-block SpMV against the
+def bench_solve_64cube(nel, p, jacobi_iters):
+    """The solve of the assembled patch (64^3 p=3 by default): device CSR of K + K^T, its nodal-block SpMV against the
     HBM roofline, Jacobi PCG to 1e-10 (2-norm) with the true residual and the FP64 rounding floor
     eps ||(|A||x|)|| / ||b|| of the final iterate, and the fast-diagonalisation PCG session (FP64,
     matrix-free) to 1e-10, its solution checked with the assembled CSR."""
@@ -222,7 +222,7 @@ def bench_c2_solve(nel, p, jacobi_iters):
     nblocks = csr.nnz // 9
     # values, one block column per 3x3 block, the block row pointers, x read, y written
     spmv_bytes = 8 * csr.nnz + 4 * nblocks + 8 * (ncp + 1) + 16 * csr.nrows
-    log("C2 %d^3 p=%d: CSR pattern %.2f s (nnz %d), values %.1f ms, SpMV %.3f ms (%.0f GB/s)"
+    log("%d^3 p=%d: CSR pattern %.2f s (nnz %d), values %.1f ms, SpMV %.3f ms (%.0f GB/s)"
         % (nel, p, t_pat, csr.nnz, t_val * 1e3, spmv_ms, spmv_bytes / (spmv_ms * 1e-3) / 1e9))
     mask = free_mask_from_args(bp.args)
     maskd = mask.to(torch.float64)
@@ -234,7 +234,7 @@ def bench_c2_solve(nel, p, jacobi_iters):
     t_jac = time.perf_counter() - t0
     true_rel = float(((F - csr.spmv(vals, u)) * maskd).norm()) / bnorm
     floor = float(np.finfo(np.float64).eps) * float((csr.spmv_abs(vals, u) * maskd).norm()) / bnorm
-    log("C2 Jacobi PCG: %d iterations in %.2f s (%.3f ms/iteration), relative residual %.3e (true %.3e, "
+    log("assembled patch, Jacobi PCG: %d iterations in %.2f s (%.3f ms/iteration), relative residual %.3e (true %.3e, "
         "FP64 floor eps||A||x|| / ||b|| %.3e)" % (it, t_jac, t_jac / max(it, 1) * 1e3, rel, true_rel, floor))
     req = Request(0, 0.0, (nel, nel, nel), p, 210000.0, 0.3, -1.0, 20261202)
     req.args = bp.args
@@ -247,7 +247,7 @@ def bench_c2_solve(nel, p, jacobi_iters):
     srv.close()
     rel_csr = float(((F - csr.spmv(vals, uf)) * maskd).norm()) / bnorm
     du = float((uf - u).norm() / u.norm())
-    log("C2 fast-diagonalisation PCG (FP64, matrix-free): %d iterations in %.2f s, residual %.3e "
+    log("assembled patch, fast-diagonalisation PCG (FP64, matrix-free): %d iterations in %.2f s, residual %.3e "
         "(assembled CSR: %.3e), |u_fdm - u_jacobi| / |u_jacobi| %.2e" % (d["iters"], t_fdm, d["relres"], rel_csr, du))
     out = {"workload": "%d^3 p=%d cantilever (%d dofs): device CSR of K + K^T from the FP64 COO values, "
                        "Jacobi PCG and fast-diagonalisation PCG to 1e-10" % (nel, p, csr.nrows),
@@ -546,13 +546,13 @@ def main():
     ap.add_argument("--serve-plain", type=int, default=100,
                     help="requests solved one at a time (plain FP64), from the start of the stream")
     ap.add_argument("--grad-fd", type=int, default=50, help="coordinates checked by central differences")
-    ap.add_argument("--jacobi-iters", type=int, default=60000, help="bound on the C2 Jacobi PCG iterations")
+    ap.add_argument("--jacobi-iters", type=int, default=60000, help="bound on the Jacobi PCG iterations of the assembled-patch solve")
     ap.add_argument("--mp-nel", type=int, default=128, help="elements per direction of the mixed-precision solve")
     ap.add_argument("--mp-jacobi-iters", type=int, default=6000,
                     help="bound on the FP32 block-Jacobi iterations of the mixed-precision section")
     ap.add_argument("--only-assembly", action="store_true")
-    ap.add_argument("--sections", default="c2,serving,gradient,mixed",
-                    help="comma list of the parts run after the assembly: c2, serving, gradient, mixed, apply "
+    ap.add_argument("--sections", default="solve,serving,gradient,mixed",
+                    help="comma list of the parts run after the assembly: solve, serving, gradient, mixed, apply "
                          "(the matrix-free apply and bounded solves alone, at --mp-nel)")
     ap.add_argument("--pmc-probe", nargs="?", const="elem", choices=("elem", "mf"),
                     help="run only a 2 GiB device copy, then one element-kernel run (elem) or one FP32 and one "
@@ -717,7 +717,7 @@ def main():
         log("cpu baseline: %dx%dx%d elements with %d threads ..." % (args.nel, args.nel, args.cpu_slab, threads))
         cpu = cpu_baseline(args.nel, p, args.cpu_slab, args.cpu_runs, threads)
         log("cpu baseline: %.3e elem/s (%s); 16x4x4 p=2: %.3e elem/s"
-            % (cpu["value"], cpu["kind"], cpu["c1_16x4x4_p2"]["value"]))
+            % (cpu["value"], cpu["kind"], cpu["cantilever_16x4x4_p2"]["value"]))
 
     out = {
         "metric": METRIC,
@@ -762,8 +762,8 @@ def main():
         "volume_gradient": {"ms": vol_ms, "elements_per_s": bp.nb_elem / (vol_ms * 1e-3), "work": grad_work(vol_ms, 1)},
     }
     sections = set() if args.only_assembly else set(args.sections.split(","))
-    if "c2" in sections:
-        out["c2_solve"] = bench_c2_solve(args.nel, p, args.jacobi_iters)
+    if "solve" in sections:
+        out["solve_64cube_p3"] = bench_solve_64cube(args.nel, p, args.jacobi_iters)
     if "serving" in sections:
         out["serving"] = bench_serving(args.serve_requests, args.serve_rate, 20261202, 200 << 30, args.serve_plain)
     if "gradient" in sections:

@@ -16,13 +16,13 @@ from yeti_amd.geometry import make_box_patch
 pytestmark = pytest.mark.gpu
 
 
-def test_c3_mixed_precision_against_assembled_fp64(gpu):
+def test_128cube_p2_mixed_precision_against_assembled_fp64(gpu):
     import torch
     from yeti_amd.assembly import Assembler
     from yeti_amd.matfree import MatrixFree
     from yeti_amd.serving import solve_session
     from yeti_amd.solver import StiffnessCSR, free_mask_from_args
-    print("[case] C3 128^3 p=2: mixed solve against the assembled FP64 solve", flush=True)
+    print("[case] 128^3 p=2: mixed solve against the assembled FP64 solve", flush=True)
     bp = make_box_patch(nel=(128, 128, 128), p=2)
     asm = Assembler(bp.args)
     F = asm.run_rhs()

@@ -30,12 +30,12 @@ def _slices():
     return [np.arange(0, 256), np.arange(mid - 128, mid + 128), np.arange(ne - 256, ne)]
 
 
-def test_c2_full_size_assembly_csr_spmv(gpu):
+def test_64cube_p3_full_size_assembly_csr_spmv(gpu):
     import torch
     from oracle import oracle as O
     from yeti_amd.assembly import Assembler
     from yeti_amd.solver import StiffnessCSR
-    print("[case] C2 64^3 p=3 full-size assembly, CSR pattern and SpMV", flush=True)
+    print("[case] 64^3 p=3 full-size assembly, CSR pattern and SpMV", flush=True)
     bp = make_box_patch(nel=(NEL, NEL, NEL), p=P)
     ne = bp.nb_elem
     asm = Assembler(bp.args)

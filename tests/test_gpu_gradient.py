@@ -444,7 +444,7 @@ def test_gradlinelastwork_an_dropin_matches_reference(gpu, name):
     assert np.abs(ge1).max() == 0.0 and np.array_equal(gi1, gi)
 
 
-def test_c5_32cube_gradient(gpu):
+def test_32cube_p2_gradient(gpu):
     """The differentiation workload at its size: 32^3 elements, p = 2 (39,304 control points).
     * gradlinelastwork_an (element-wise gradient kernel, polarised) and the matrix-free MODE 2
       strain-energy gradient against the compiled gradLinElastWork_AN (tests/golden/
@@ -458,7 +458,7 @@ def test_c5_32cube_gradient(gpu):
     from yeti_amd.gradadjointwork import gradlinelastwork_an, inputs4gradtotalwork
     from yeti_amd.gradient import CompliancePatch, compliance
     from yeti_amd.matfree import F64, MatrixFree
-    print("[case] C5 32^3 p=2 gradient", flush=True)
+    print("[case] 32^3 p=2 gradient", flush=True)
     f = np.load("tests/golden/grad_32cube_p2.npz")
     bp = make_box_patch(nel=(32, 32, 32), p=2)
     geo = sha(np.concatenate([bp.args["COORDS3D"].ravel(order="F"), bp.args["weight"]])) + sha(bp.args["IEN"])
