@@ -269,15 +269,24 @@ static __device__ inline void cross3(const double *a, const double *b, double *c
 
 constexpr int RHS_MAXN = 729;   // (8+1)^3
 
+// Dynamic LDS of elem_rhs_kernel: the element's node coordinates, weights and CP ids (sized by
+// the patch's node count instead of the largest degree, so that many elements share a CU).
+static size_t rhs_smem_bytes(const RhsPatch &rp) {
+    const size_t nn = (size_t)(rp.p[0] + 1) * (rp.p[1] + 1) * (rp.p[2] + 1);
+    return nn * (4 * sizeof(double) + sizeof(int32_t));
+}
+
 __global__ void __launch_bounds__(64) elem_rhs_kernel(RhsPatch rp, LoadTable lt, double *rhs_buf) {
     const int le = blockIdx.x;
     if (le >= rp.n_loaded) return;
     const int e = rp.elems[le];
     const int lane = threadIdx.x;
     const int nn = (rp.p[0] + 1) * (rp.p[1] + 1) * (rp.p[2] + 1);
-    __shared__ double sX[RHS_MAXN][3];
-    __shared__ double sW[RHS_MAXN];
-    __shared__ int32_t sIen[RHS_MAXN];
+    // the element's nodes, sized at launch (rhs_smem_bytes): coordinates, weights, CP ids
+    extern __shared__ double rhs_smem[];
+    double (*sX)[3] = reinterpret_cast<double (*)[3]>(rhs_smem);
+    double *sW = rhs_smem + 3 * nn;
+    int32_t *sIen = reinterpret_cast<int32_t *>(rhs_smem + 4 * nn);
     for (int a = lane; a < nn; a += 64) {
         int cp = rp.ien[(int64_t)nn * e + a];
         sIen[a] = cp;
@@ -2067,7 +2076,7 @@ static int run_rhs(yeti_asm *h, double *F, hipStream_t st) {
         rp.ng = pd->ng;
         rp.rho = pd->rho;
         gauss_legendre(pd->ng, rp.g.x, rp.g.w);
-        hipLaunchKernelGGL(elem_rhs_kernel, dim3(pd->n_loaded), dim3(64), 0, st, rp, lt, h->rhs_buf.p);
+        hipLaunchKernelGGL(elem_rhs_kernel, dim3(pd->n_loaded), dim3(64), rhs_smem_bytes(rp), st, rp, lt, h->rhs_buf.p);
         YETI_HIP(hipGetLastError());
     }
     int blocks = (h->nb_cp + 255) / 256;
