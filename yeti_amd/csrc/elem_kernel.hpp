@@ -177,13 +177,16 @@ constexpr UnitSchedule<NT, WAVES, MAXU, MAXTILE, MAXF, MAXPASS, MAXR> make_sched
             const int nbq = hi0 > lo0 ? hi0 - lo0 : 0;
             return unit_tiles(kind, pi) * nbq * (kind != 0 && pair_I(pi) == Ju ? 3 : 1);
         };
-        // Waves w and w + 4 share a SIMD's matrix core, so the units are first cut into 4 runs of
-        // at most T tiles (one per SIMD, smallest T that places all). With one staging round each
-        // run is split between its two waves at the unit boundary that balances their tiles; with
-        // several, its units are dealt to the two waves so that in every round the busier wave
-        // stages as little as possible (each round waits for its slowest wave), largest first:
-        // at p = 3 the pair-major halves left one wave per round with most of the diagonal-pair
-        // staging (64^3: 6.16 -> 5.99 ms per launch with indices, same output).
+        // Waves w and w + 4 share a SIMD's matrix core, so every SIMD gets at most T tiles (the
+        // smallest T that places all). With one staging round the units are cut into 4 pair-major
+        // runs, one per SIMD, each split between its two waves at the unit boundary that balances
+        // their tiles (neighbouring units share fragments). With several rounds every unit is
+        // dealt, largest staging cost first, to the wave that keeps the sum over rounds of the
+        // slowest wave's staging smallest (each round waits for its slowest wave), then the one
+        // that adds the fewest fragments: pair-major runs left the diagonal pairs' staging (one-
+        // tile units that pass a wave barrier and add into LDS) on a few waves, one of them busy
+        // 6.7k cycles in a round where others had nothing (64^3 p = 3: 6.16 -> 5.82 ms per launch
+        // with indices, 5.04 -> 4.59 ms values only; 32^3 p = 4: 41.3 -> 36.9 ms; same output).
         bool placed = false;
         for (int T = (tiles + 3) / 4; T <= 2 * MAXTILE && !placed; ++T) {
             int r0[4] = {}, r1[4] = {}, sidx = 0, fill = 0;
@@ -236,59 +239,82 @@ constexpr UnitSchedule<NT, WAVES, MAXU, MAXTILE, MAXF, MAXPASS, MAXR> make_sched
                     s.nt[pass][wv[h]] = f;
                 }
             }
-            for (int sm = 0; sm < 4 && fits && r > 1; ++sm) {   // several rounds: dealt by staging cost
-                const int wv[2] = {sm, sm + 4};
-                int cost[2][MAXR] = {}, cnt[2] = {0, 0}, f[2] = {0, 0};
-                bool taken[4 * MAXU] = {};
-                const int nrun = r1[sm] - r0[sm];
-                if (nrun > 2 * MAXU) {
-                    fits = false;
-                    break;
-                }
-                for (int step = 0; step < nrun && fits; ++step) {
-                    // the untaken unit with the largest staging cost (its largest round), ties to the
-                    // earlier unit
-                    int pick = -1, pc = -1;
-                    for (int x = 0; x < nrun; ++x) {
+            if (fits && r > 1) {   // several rounds: every unit dealt to a wave by staging cost
+                int cost[WAVES][MAXR] = {}, cnt[WAVES] = {}, f[WAVES] = {}, simd[4] = {};
+                int akey[WAVES][MAXF] = {}, bkey[WAVES][MAXF] = {}, na[WAVES] = {}, nb[WAVES] = {};
+                bool taken[4 * MAXU * WAVES] = {};
+                const int nun = u1 - u0;
+                if (nun > 4 * MAXU * WAVES) fits = false;
+                for (int step = 0; step < nun && fits; ++step) {
+                    int pick = -1, pc = -1, pt = -1;   // largest staging cost first, then most tiles
+                    for (int x = 0; x < nun; ++x) {
                         if (taken[x]) continue;
                         int c = 0;
-                        for (int q = 0; q < r; ++q) c = stage_cost(r0[sm] + x, q) > c ? stage_cost(r0[sm] + x, q) : c;
-                        if (c > pc) {
+                        for (int q = 0; q < r; ++q) c = stage_cost(u0 + x, q) > c ? stage_cost(u0 + x, q) : c;
+                        const int tx = unit_tiles((u0 + x) % 4, (u0 + x) / 4);
+                        if (c > pc || (c == pc && tx > pt)) {
                             pc = c;
+                            pt = tx;
                             pick = x;
                         }
                     }
-                    const int u = r0[sm] + pick, nt = unit_tiles(u % 4, u / 4);
-                    int hbest = -1, score = 1 << 30;
-                    for (int h = 0; h < 2; ++h) {
-                        if (cnt[h] >= MAXU || f[h] + nt > MAXTILE) continue;
-                        int sc = 0;   // sum over rounds of the busier wave's cost, then the tile spread
-                        for (int q = 0; q < r; ++q) {
-                            const int ca = cost[0][q] + (h == 0 ? stage_cost(u, q) : 0);
-                            const int cb = cost[1][q] + (h == 1 ? stage_cost(u, q) : 0);
-                            sc += ca > cb ? ca : cb;
+                    const int u = u0 + pick, pi = u / 4, kind = u % 4, nt = unit_tiles(kind, pi);
+                    int wbest = -1;
+                    long long best = 0;
+                    for (int w = 0; w < WAVES; ++w) {
+                        if (cnt[w] >= MAXU || f[w] + nt > MAXTILE || simd[w % 4] + nt > T) continue;
+                        int newa = 0, newb = 0;   // fragments the unit adds to the wave
+                        for (int tt = 0; tt < nt; ++tt) {
+                            const int ak = pair_I(pi) * 3 + unit_k(kind, tt), bk = pair_J(pi) * 3 + unit_l(kind, tt);
+                            bool ha = false, hb = false;
+                            for (int y = 0; y < na[w]; ++y) ha |= akey[w][y] == ak;
+                            for (int y = 0; y < nb[w]; ++y) hb |= bkey[w][y] == bk;
+                            for (int t2 = 0; t2 < tt; ++t2) {
+                                ha |= pair_I(pi) * 3 + unit_k(kind, t2) == ak;
+                                hb |= pair_J(pi) * 3 + unit_l(kind, t2) == bk;
+                            }
+                            newa += ha ? 0 : 1;
+                            newb += hb ? 0 : 1;
                         }
-                        const int fa = f[0] + (h == 0 ? nt : 0), fb = f[1] + (h == 1 ? nt : 0);
-                        sc = sc * 64 + (fa > fb ? fa - fb : fb - fa);
-                        if (sc < score) {
-                            score = sc;
-                            hbest = h;
+                        if (na[w] + newa > MAXF || nb[w] + newb > MAXF) continue;
+                        long long crit = 0;   // sum over rounds of the slowest wave's staging
+                        for (int q = 0; q < r; ++q) {
+                            int mx = 0;
+                            for (int w2 = 0; w2 < WAVES; ++w2) {
+                                const int c2 = cost[w2][q] + (w2 == w ? stage_cost(u, q) : 0);
+                                mx = c2 > mx ? c2 : mx;
+                            }
+                            crit += mx;
+                        }
+                        const long long sc = crit * 4096 + (newa + newb) * 64 + f[w];
+                        if (wbest < 0 || sc < best) {
+                            best = sc;
+                            wbest = w;
                         }
                     }
-                    if (hbest < 0) {
+                    if (wbest < 0) {
                         fits = false;
                         break;
                     }
                     taken[pick] = true;
-                    for (int q = 0; q < r; ++q) cost[hbest][q] += stage_cost(u, q);
-                    s.unit[pass][wv[hbest]][cnt[hbest]] = u;
-                    s.toff[pass][wv[hbest]][cnt[hbest]] = f[hbest];
-                    ++cnt[hbest];
-                    f[hbest] += nt;
+                    for (int q = 0; q < r; ++q) cost[wbest][q] += stage_cost(u, q);
+                    for (int tt = 0; tt < nt; ++tt) {
+                        const int ak = pair_I(pi) * 3 + unit_k(kind, tt), bk = pair_J(pi) * 3 + unit_l(kind, tt);
+                        bool ha = false, hb = false;
+                        for (int y = 0; y < na[wbest]; ++y) ha |= akey[wbest][y] == ak;
+                        for (int y = 0; y < nb[wbest]; ++y) hb |= bkey[wbest][y] == bk;
+                        if (!ha) akey[wbest][na[wbest]++] = ak;
+                        if (!hb) bkey[wbest][nb[wbest]++] = bk;
+                    }
+                    s.unit[pass][wbest][cnt[wbest]] = u;
+                    s.toff[pass][wbest][cnt[wbest]] = f[wbest];
+                    ++cnt[wbest];
+                    f[wbest] += nt;
+                    simd[wbest % 4] += nt;
                 }
-                for (int h = 0; h < 2; ++h) {
-                    s.nu[pass][wv[h]] = cnt[h];
-                    s.nt[pass][wv[h]] = f[h];
+                for (int w = 0; w < WAVES; ++w) {
+                    s.nu[pass][w] = cnt[w];
+                    s.nt[pass][w] = f[w];
                 }
             }
             placed = fits;
