@@ -533,13 +533,6 @@ __global__ void __launch_bounds__(256) elem_geo_kernel(PatchArgs pa, double *__r
     }
 }
 
-__device__ __forceinline__ double bcast(double v, int src) {
-    // wave-uniform source lane -> scalar broadcast
-    int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
-    int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
-    return __hiloint2double(hi, lo);
-}
-
 // ---------------------------------------------------------------------------------------
 template <int P, int NG>
 struct ElemSmem {
@@ -611,34 +604,49 @@ __device__ __forceinline__ void producer_store_element(ElemSmem<P, NG> &sm, int 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Producer: the geometry of this wave's Gauss points of a chunk, one value per lane (lane
-// 12 t + v holds value v of Gauss point q0 + qw + t), loaded one step before it is used.
+// Producer: the geometry of this wave's Gauss points of a chunk, loaded one step before it is
+// used. g[t] holds Gauss point q0 + qw + t: lane 16 r + v of every 16-lane row r holds its value
+// v, so one DPP row broadcast per value (no lane reads into scalar registers) hands it to all
+// 64 node lanes.
 template <int P, int NG>
-__device__ __forceinline__ double producer_prefetch(const double *__restrict__ geo_e, int pw, int q0) {
+struct ProducerGeo {
+    double g[ElemCfg<P, NG>::GPW];
+};
+
+template <int P, int NG>
+__device__ __forceinline__ ProducerGeo<P, NG> producer_prefetch(const double *__restrict__ geo_e, int pw, int q0) {
     using C = ElemCfg<P, NG>;
-    const int lane = threadIdx.x & 63;
-    const int t = lane / ELEM_NGEO, v = lane - ELEM_NGEO * t;
-    const int q = q0 + pw * C::GPW + t;
-    return (t < C::GPW && q < C::NQ) ? geo_e[v * C::NQ + q] : 0.0;
+    const int v = threadIdx.x & 15;
+    ProducerGeo<P, NG> r;
+#pragma unroll
+    for (int t = 0; t < C::GPW; ++t) {
+        const int q = q0 + pw * C::GPW + t;
+        r.g[t] = (v < ELEM_NGEO && q < C::NQ) ? geo_e[v * C::NQ + q] : 0.0;
+    }
+    return r;
+}
+
+// value of lane N of each 16-lane row, in every lane of that row (DPP row_newbcast)
+template <int N>
+__device__ __forceinline__ double row_bcast(double v) {
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + N, 0xF, 0xF, true);
 }
 
 // Producer: Gauss points [q0 + qw, q0 + qw + GPW) of the chunk into buffer `buf`; lane = node.
 template <int P, int NG>
-__device__ __forceinline__ void producer_chunk(ElemSmem<P, NG> &sm, double gv, int pw, int buf, int q0,
-                                               const ProducerElem<P, NG> &pe) {
+__device__ __forceinline__ void producer_chunk(ElemSmem<P, NG> &sm, const ProducerGeo<P, NG> &gv, int pw, int buf,
+                                               int q0, const ProducerElem<P, NG> &pe) {
     using C = ElemCfg<P, NG>;
     constexpr int NP1 = C::NP1, NN = C::NN, NNP = C::NNP, NQ = C::NQ;
-    static_assert(ELEM_NGEO * C::GPW <= 64, "geometry of a wave's Gauss points must fit a wave");
+    static_assert(ELEM_NGEO <= 16, "geometry of a Gauss point must fit a 16-lane row");
     const int lane = threadIdx.x & 63;
     const int qw = pw * C::GPW;
 #pragma unroll
     for (int t = 0; t < C::GPW; ++t) {
         const int qt = q0 + qw + t;   // wave-uniform
         double A[9], dw[3];
-#pragma unroll
-        for (int v = 0; v < 9; ++v) A[v] = bcast(gv, ELEM_NGEO * t + v);
-#pragma unroll
-        for (int m = 0; m < 3; ++m) dw[m] = bcast(gv, ELEM_NGEO * t + 9 + m);
+        static_for<9>([&](auto v) { A[v] = row_bcast<decltype(v)::value>(gv.g[t]); });
+        static_for<3>([&](auto m) { dw[m] = row_bcast<9 + decltype(m)::value>(gv.g[t]); });
         const int h0 = qt % NG, h1 = (qt / NG) % NG, h2 = (qt / (NG * NG)) % NG;
 #pragma unroll
         for (int m = 0; m < (NNP + 63) / 64; ++m) {
@@ -1038,7 +1046,7 @@ elem_stiffness_kernel(PatchArgs pa, const double *__restrict__ geo, double *__re
             producer_fetch_element<P, NG>(geo_b + (int64_t)el * rec_stride, pa.offs + blockIdx.x + (int64_t)el * gridDim.x,
                                           fe);
         };
-        double gv = 0.0;         // geometry of the chunk built next
+        ProducerGeo<P, NG> gv{};   // geometry of the chunk built next
         int64_t base = 0;        // COO offset of the element of step s
         int64_t next_base = 0;
         if (nsteps > 0) {
@@ -1057,7 +1065,7 @@ elem_stiffness_kernel(PatchArgs pa, const double *__restrict__ geo, double *__re
         int64_t pend_base = 0;
         for (int s = 0; s < nsteps; ++s) {
             const int s1 = s + 1, s2 = s + 2;
-            const double gcur = gv;
+            const ProducerGeo<P, NG> gcur = gv;
             if (s1 < nsteps && s1 % SPE == 0) {   // the element of step s1 starts
                 producer_store_element<P, NG>(sm, pw, fe, pe);
                 next_base = fe.base;
