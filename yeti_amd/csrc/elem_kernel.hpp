@@ -86,9 +86,11 @@ struct ElemCfg {
     static constexpr int MAXR = 8;                      // staging rounds per pass
     static constexpr int GPW = QC / PW;                 // Gauss points per producer wave per chunk
     // per-element record written by the geometry pass: 12 geometry values per Gauss point
-    // ([v][q]), the 1D basis tables B, dB ([which][d][g][i]) and the node weights
+    // ([v][q]), the 1D basis tables B, dB ([which][d][g][i]), the node weights and the nodes'
+    // dof bases 3 (IEN - 1) (int32, from double offset DOFOFF)
     static constexpr int NTAB = 2 * 3 * NG * NP1;
-    static constexpr int64_t REC = (int64_t)ELEM_NGEO * NQ + NTAB + NN;
+    static constexpr int64_t DOFOFF = (int64_t)ELEM_NGEO * NQ + NTAB + NN;
+    static constexpr int64_t REC = DOFOFF + (NN + 1) / 2;
     static constexpr int64_t NDE = 9LL * NN * (NN - 1) / 2 + 6LL * NN;   // COO entries per element
     // LDS stage of one round of COO values: what is left of ~150 KiB after G
     static constexpr int64_t GBYTES = 2LL * 3 * NNP * QCP * 8;
@@ -421,8 +423,10 @@ __global__ void __launch_bounds__(256) elem_geo_kernel(PatchArgs pa, double *__r
     double *rec = geo + gi * C::REC;
     if (act) {
         for (int a = lane; a < NN; a += 64) {
-            const double *xp = pa.coords + 3 * (int64_t)(pa.ien[(int64_t)NN * e + a] - 1);
+            const int32_t cp1 = pa.ien[(int64_t)NN * e + a];
+            const double *xp = pa.coords + 3 * (int64_t)(cp1 - 1);
             const double w = pa.weight[(int64_t)NN * e + a];
+            reinterpret_cast<int32_t *>(rec + C::DOFOFF)[a] = 3 * (cp1 - 1);
             cp[wv][0][a] = w;
             cp[wv][1][a] = w * xp[0];
             cp[wv][2][a] = w * xp[1];
@@ -542,7 +546,10 @@ struct ElemSmem {
     double B[C::PW][3][NG][C::NP1];     // per producer wave: B(reversed local index) at the abscissae
     double dB[C::PW][3][NG][C::NP1];
     double stage[C::STG];               // COO values of one round, element-local order
-    int32_t dof[C::NN];                 // 3 (IEN - 1) of the element being emitted
+    // dof bases 3 (IEN - 1) and COO offset of the elements being emitted, by element parity
+    // (written by the producers one step before the element's first step)
+    int32_t dof[2][C::NN];
+    int64_t ebase[2];
 };
 
 // Producer: per-element set-up (1D bases in the wave's own LDS table, node weights in registers),
@@ -559,6 +566,7 @@ struct ProducerFetch {
     using C = ElemCfg<P, NG>;
     double tab[(C::NTAB + 63) / 64];   // table entry lane + 64 m
     double w[(C::NNP + 63) / 64];
+    int32_t dof[(C::NNP + 63) / 64];
     int64_t base;                      // COO offset of the element
 };
 
@@ -576,13 +584,14 @@ __device__ __forceinline__ void producer_fetch_element(const double *__restrict_
     for (int m = 0; m < (C::NNP + 63) / 64; ++m) {
         const int a = lane + 64 * m;
         f.w[m] = a < C::NN ? rec[ELEM_NGEO * C::NQ + C::NTAB + a] : 0.0;
+        f.dof[m] = a < C::NN ? reinterpret_cast<const int32_t *>(rec + C::DOFOFF)[a] : 0;
     }
     f.base = *offs_gi;
 }
 
 template <int P, int NG>
-__device__ __forceinline__ void producer_store_element(ElemSmem<P, NG> &sm, int pw, const ProducerFetch<P, NG> &f,
-                                                       ProducerElem<P, NG> &pe) {
+__device__ __forceinline__ void producer_store_element(ElemSmem<P, NG> &sm, int pw, int slot,
+                                                       const ProducerFetch<P, NG> &f, ProducerElem<P, NG> &pe) {
     using C = ElemCfg<P, NG>;
     constexpr int HALF = 3 * NG * C::NP1;
     const int lane = threadIdx.x & 63;
@@ -598,6 +607,12 @@ __device__ __forceinline__ void producer_store_element(ElemSmem<P, NG> &sm, int 
     }
 #pragma unroll
     for (int m = 0; m < (C::NNP + 63) / 64; ++m) pe.wb[m] = f.w[m];
+    if (pw == 0) {   // the element's dof bases and COO offset, for the copy-out of its values
+#pragma unroll
+        for (int m = 0; m < (C::NNP + 63) / 64; ++m)
+            if (lane + 64 * m < C::NN) sm.dof[slot][lane + 64 * m] = f.dof[m];
+        if (lane == 0) sm.ebase[slot] = f.base;
+    }
     // the table is read back by this wave only: order the LDS writes before the reads
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -842,7 +857,7 @@ __device__ __forceinline__ void stage_round(ElemSmem<P, NG> &sm, double lam, dou
 // a = j / 9, (kk, ll) = (j mod 3, j mod 9 / 3); the last 6 are the diagonal block.
 // row = 3 (IEN_a - 1) + kk, col = 3 (IEN_b - 1) + ll.
 template <int P, int NG, bool WRITE_IDX, int PASS, int R, int NW, int NPART = 1>
-__device__ __forceinline__ void copy_round(const ElemSmem<P, NG> &sm, int64_t base, double *__restrict__ Kdata,
+__device__ __forceinline__ void copy_round(const ElemSmem<P, NG> &sm, int slot, double *__restrict__ Kdata,
                                            int32_t *__restrict__ Krow, int32_t *__restrict__ Kcol, int part = 0) {
     constexpr auto &S = Sched<P, NG>::S;
     constexpr int b0 = S.rb[PASS][R], b1 = S.rb[PASS][R + 1];
@@ -850,11 +865,13 @@ __device__ __forceinline__ void copy_round(const ElemSmem<P, NG> &sm, int64_t ba
     (void)S;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
+    const int64_t base = sm.ebase[slot];
+    const int32_t *dof = sm.dof[slot];
     for (int b = b0 + wave + NW * part; b < b1; b += NW * NPART) {
         const int len = 9 * b + 6;
         const int64_t gb = base + coo_start(b);
         const double *st_b = sm.stage + ((int)coo_start(b) - c0);
-        const int32_t dofb = WRITE_IDX ? sm.dof[b] : 0;
+        const int32_t dofb = WRITE_IDX ? dof[b] : 0;
         for (int j0 = 0; j0 < len; j0 += 128) {
             double v[2];
 #pragma unroll
@@ -877,7 +894,7 @@ __device__ __forceinline__ void copy_round(const ElemSmem<P, NG> &sm, int64_t ba
                             kk = (0x210100 >> (4 * d)) & 15;
                             ll = (0x222110 >> (4 * d)) & 15;
                         }
-                        Krow[gb + j] = sm.dof[a] + kk;
+                        Krow[gb + j] = dof[a] + kk;
                         Kcol[gb + j] = dofb + ll;
                     }
                 }
@@ -895,7 +912,7 @@ constexpr bool elem_defer_last() { return ElemCfg<P, NG>::NCH >= 2; }
 // Rounds of pass PASS for consumer wave W: stage, barrier, copy (all waves), barrier before the
 // next round; the last round is only staged when its copy is deferred.
 template <int P, int NG, bool WRITE_IDX, int PASS, int W, int R = 0>
-__device__ __forceinline__ void consumer_epilogue(ElemSmem<P, NG> &sm, double lam, double mu, int64_t base,
+__device__ __forceinline__ void consumer_epilogue(ElemSmem<P, NG> &sm, double lam, double mu, int slot,
                                                   const ConsumerAcc<P, NG> &acc, double *__restrict__ Kdata,
                                                   int32_t *__restrict__ Krow, int32_t *__restrict__ Kcol) {
     using C = ElemCfg<P, NG>;
@@ -903,44 +920,44 @@ __device__ __forceinline__ void consumer_epilogue(ElemSmem<P, NG> &sm, double la
     stage_round<P, NG, PASS, W, R>(sm, lam, mu, acc);
     if constexpr (R + 1 == NR && elem_defer_last<P, NG>()) return;
     __syncthreads();
-    copy_round<P, NG, WRITE_IDX, PASS, R, C::CPW>(sm, base, Kdata, Krow, Kcol);
+    copy_round<P, NG, WRITE_IDX, PASS, R, C::CPW>(sm, slot, Kdata, Krow, Kcol);
     if constexpr (R + 1 < NR) {
         __syncthreads();
-        consumer_epilogue<P, NG, WRITE_IDX, PASS, W, R + 1>(sm, lam, mu, base, acc, Kdata, Krow, Kcol);
+        consumer_epilogue<P, NG, WRITE_IDX, PASS, W, R + 1>(sm, lam, mu, slot, acc, Kdata, Krow, Kcol);
     }
 }
 
 // The producers' side of the same rounds (same barriers; they help with the copies when
 // PROD_COPY).
 template <int P, int NG, bool WRITE_IDX, int PASS, int R = 0>
-__device__ __forceinline__ void producer_epilogue_pass(ElemSmem<P, NG> &sm, int64_t base, double *__restrict__ Kdata,
+__device__ __forceinline__ void producer_epilogue_pass(ElemSmem<P, NG> &sm, int slot, double *__restrict__ Kdata,
                                                        int32_t *__restrict__ Krow, int32_t *__restrict__ Kcol) {
     using C = ElemCfg<P, NG>;
     constexpr int NR = Sched<P, NG>::S.nround[PASS];
     if constexpr (R + 1 == NR && elem_defer_last<P, NG>()) return;
     __syncthreads();
-    if constexpr (C::PROD_COPY) copy_round<P, NG, WRITE_IDX, PASS, R, C::CPW>(sm, base, Kdata, Krow, Kcol);
+    if constexpr (C::PROD_COPY) copy_round<P, NG, WRITE_IDX, PASS, R, C::CPW>(sm, slot, Kdata, Krow, Kcol);
     if constexpr (R + 1 < NR) {
         __syncthreads();
-        producer_epilogue_pass<P, NG, WRITE_IDX, PASS, R + 1>(sm, base, Kdata, Krow, Kcol);
+        producer_epilogue_pass<P, NG, WRITE_IDX, PASS, R + 1>(sm, slot, Kdata, Krow, Kcol);
     }
 }
 
 template <int P, int NG, bool WRITE_IDX, int PASS = 0>
-__device__ __forceinline__ void producer_epilogue(ElemSmem<P, NG> &sm, int pass, int64_t base,
+__device__ __forceinline__ void producer_epilogue(ElemSmem<P, NG> &sm, int pass, int slot,
                                                   double *__restrict__ Kdata, int32_t *__restrict__ Krow,
                                                   int32_t *__restrict__ Kcol) {
     if (pass == PASS) {
-        producer_epilogue_pass<P, NG, WRITE_IDX, PASS>(sm, base, Kdata, Krow, Kcol);
+        producer_epilogue_pass<P, NG, WRITE_IDX, PASS>(sm, slot, Kdata, Krow, Kcol);
     } else if constexpr (PASS + 1 < Sched<P, NG>::NPASS) {
-        producer_epilogue<P, NG, WRITE_IDX, PASS + 1>(sm, pass, base, Kdata, Krow, Kcol);
+        producer_epilogue<P, NG, WRITE_IDX, PASS + 1>(sm, pass, slot, Kdata, Krow, Kcol);
     }
 }
 
 // Slice `part` of the deferred copy-out of the last round of pass `pass` (by the copying waves;
 // part = -1 copies the whole round).
 template <int P, int NG, bool WRITE_IDX, int PASS = 0>
-__device__ __forceinline__ void deferred_copy(const ElemSmem<P, NG> &sm, int pass, int part, int64_t base,
+__device__ __forceinline__ void deferred_copy(const ElemSmem<P, NG> &sm, int pass, int part, int slot,
                                               double *__restrict__ Kdata, int32_t *__restrict__ Krow,
                                               int32_t *__restrict__ Kcol) {
     using C = ElemCfg<P, NG>;
@@ -948,18 +965,18 @@ __device__ __forceinline__ void deferred_copy(const ElemSmem<P, NG> &sm, int pas
     if (pass == PASS) {
         constexpr int R = Sched<P, NG>::S.nround[PASS] - 1;
         if (part < 0)
-            copy_round<P, NG, WRITE_IDX, PASS, R, NW>(sm, base, Kdata, Krow, Kcol);
+            copy_round<P, NG, WRITE_IDX, PASS, R, NW>(sm, slot, Kdata, Krow, Kcol);
         else
-            copy_round<P, NG, WRITE_IDX, PASS, R, NW, C::NCH - 1>(sm, base, Kdata, Krow, Kcol, part);
+            copy_round<P, NG, WRITE_IDX, PASS, R, NW, C::NCH - 1>(sm, slot, Kdata, Krow, Kcol, part);
     } else if constexpr (PASS + 1 < Sched<P, NG>::NPASS) {
-        deferred_copy<P, NG, WRITE_IDX, PASS + 1>(sm, pass, part, base, Kdata, Krow, Kcol);
+        deferred_copy<P, NG, WRITE_IDX, PASS + 1>(sm, pass, part, slot, Kdata, Krow, Kcol);
     }
 }
 
 // One consumer step for wave W: the pass is dispatched to compile-time code.
 template <int P, int NG, bool WRITE_IDX, int W, int PASS = 0>
 __device__ __forceinline__ void consumer_step(ElemSmem<P, NG> &sm, const PatchArgs &pa, int buf, int pass, int ch,
-                                              int e, int64_t base, ConsumerAcc<P, NG> &acc,
+                                              int slot, ConsumerAcc<P, NG> &acc,
                                               double *__restrict__ Kdata, int32_t *__restrict__ Krow,
                                               int32_t *__restrict__ Kcol) {
     using C = ElemCfg<P, NG>;
@@ -967,14 +984,10 @@ __device__ __forceinline__ void consumer_step(ElemSmem<P, NG> &sm, const PatchAr
         if (ch == 0) acc.zero();
         mfma_chunk<P, NG, PASS, W>(sm, buf, acc);
         if (ch == C::NCH - 1) {
-            if (WRITE_IDX && W < 2) {
-                const int t = threadIdx.x;
-                if (t < C::NN) sm.dof[t] = 3 * (pa.ien[(int64_t)C::NN * e + t] - 1);
-            }
-            consumer_epilogue<P, NG, WRITE_IDX, PASS, W>(sm, pa.lam, pa.mu, base, acc, Kdata, Krow, Kcol);
+            consumer_epilogue<P, NG, WRITE_IDX, PASS, W>(sm, pa.lam, pa.mu, slot, acc, Kdata, Krow, Kcol);
         }
     } else if constexpr (PASS + 1 < Sched<P, NG>::NPASS) {
-        consumer_step<P, NG, WRITE_IDX, W, PASS + 1>(sm, pa, buf, pass, ch, e, base, acc, Kdata, Krow, Kcol);
+        consumer_step<P, NG, WRITE_IDX, W, PASS + 1>(sm, pa, buf, pass, ch, slot, acc, Kdata, Krow, Kcol);
     }
 }
 
@@ -988,27 +1001,27 @@ __device__ __forceinline__ void consumer_loop(ElemSmem<P, NG> &sm, const PatchAr
     constexpr int SPE = Sched<P, NG>::NPASS * C::NCH;
     ConsumerAcc<P, NG> acc;
     int pend_pass = -1;     // pass whose last round waits in the stage for its copy-out
-    int64_t pend_base = 0;
+    int pend_slot = 0;
+    // No global loads in this loop (the element's COO offset and dof bases come from the
+    // producers through LDS): a wait for a load would also wait for this wave's stores.
     for (int s = 0; s < nsteps; ++s) {
         const int el = s / SPE, rem = s % SPE;
         const int pass = rem / C::NCH, ch = rem % C::NCH;
-        const int64_t gi = blockIdx.x + (int64_t)el * gridDim.x;
-        const int e = pa.elems[gi];
-        const int64_t base = pa.offs[gi];
-        consumer_step<P, NG, WRITE_IDX, W>(sm, pa, s & 1, pass, ch, e, base, acc, Kdata, Krow, Kcol);
+        const int slot = el & 1;
+        consumer_step<P, NG, WRITE_IDX, W>(sm, pa, s & 1, pass, ch, slot, acc, Kdata, Krow, Kcol);
         if constexpr (elem_defer_last<P, NG>()) {
             if (ch == C::NCH - 1) {
                 pend_pass = pass;
-                pend_base = base;
+                pend_slot = slot;
             } else if (pend_pass >= 0) {
-                deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, ch, pend_base, Kdata, Krow, Kcol);
+                deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, ch, pend_slot, Kdata, Krow, Kcol);
             }
         }
         __syncthreads();
     }
     // an unfinished deferred round (the stage is stable after the last barrier)
     if constexpr (elem_defer_last<P, NG>())
-        if (pend_pass >= 0) deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, -1, pend_base, Kdata, Krow, Kcol);
+        if (pend_pass >= 0) deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, -1, pend_slot, Kdata, Krow, Kcol);
 }
 
 template <int P, int NG, bool WRITE_IDX>
@@ -1047,13 +1060,10 @@ elem_stiffness_kernel(PatchArgs pa, const double *__restrict__ geo, double *__re
                                           fe);
         };
         ProducerGeo<P, NG> gv{};   // geometry of the chunk built next
-        int64_t base = 0;        // COO offset of the element of step s
-        int64_t next_base = 0;
         if (nsteps > 0) {
             fetch(0);
             gv = step_geo(0);
-            producer_store_element<P, NG>(sm, pw, fe, pe);
-            base = next_base = fe.base;
+            producer_store_element<P, NG>(sm, pw, 0, fe, pe);
             producer_chunk<P, NG>(sm, gv, pw, 0, 0, pe);
             if (nsteps > 1) {
                 gv = step_geo(1);
@@ -1062,33 +1072,32 @@ elem_stiffness_kernel(PatchArgs pa, const double *__restrict__ geo, double *__re
         }
         __syncthreads();
         int pend_pass = -1;     // pass whose last round waits in the stage for its copy-out
-        int64_t pend_base = 0;
+        int pend_slot = 0;
         for (int s = 0; s < nsteps; ++s) {
             const int s1 = s + 1, s2 = s + 2;
+            const int slot = (s / SPE) & 1;
             const ProducerGeo<P, NG> gcur = gv;
-            if (s1 < nsteps && s1 % SPE == 0) {   // the element of step s1 starts
-                producer_store_element<P, NG>(sm, pw, fe, pe);
-                next_base = fe.base;
-            }
+            // the element of step s1 starts: its slot's last reader, the element two back, was
+            // copied out by the end of the previous step
+            if (s1 < nsteps && s1 % SPE == 0) producer_store_element<P, NG>(sm, pw, (s1 / SPE) & 1, fe, pe);
             if (s2 < nsteps) gv = step_geo(s2);   // geometry for step s2, before this step's chunk
             if (s1 < nsteps) producer_chunk<P, NG>(sm, gcur, pw, s1 & 1, ((s1 % SPE) % C::NCH) * C::QC, pe);
             if (s2 < nsteps && s2 % SPE == 0) fetch(s2 / SPE);   // after the chunk's wait on gcur
             const int rem = s % SPE, ch = rem % C::NCH;
             if (ch == C::NCH - 1) {
-                producer_epilogue<P, NG, WRITE_IDX>(sm, rem / C::NCH, base, Kdata, Krow, Kcol);
+                producer_epilogue<P, NG, WRITE_IDX>(sm, rem / C::NCH, slot, Kdata, Krow, Kcol);
                 if constexpr (elem_defer_last<P, NG>() && C::PROD_DEFER_COPY) {
                     pend_pass = rem / C::NCH;
-                    pend_base = base;
+                    pend_slot = slot;
                 }
             } else if (elem_defer_last<P, NG>() && C::PROD_DEFER_COPY && pend_pass >= 0) {
-                deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, ch, pend_base, Kdata, Krow, Kcol);
+                deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, ch, pend_slot, Kdata, Krow, Kcol);
             }
-            if (s1 % SPE == 0) base = next_base;
             __syncthreads();
         }
         // an unfinished deferred round (the stage is stable after the last barrier)
         if constexpr (elem_defer_last<P, NG>() && C::PROD_DEFER_COPY)
-            if (pend_pass >= 0) deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, -1, pend_base, Kdata, Krow, Kcol);
+            if (pend_pass >= 0) deferred_copy<P, NG, WRITE_IDX>(sm, pend_pass, -1, pend_slot, Kdata, Krow, Kcol);
     } else {
         __syncthreads();
         switch (wave) {
