@@ -1459,8 +1459,14 @@ struct VibDegrees {
     int ng;
 };
 
+// waves per SIMD the kernel is compiled for: at p = 3 three (its LDS allows three workgroups per
+// CU once the node sums are in registers), at p = 2 four, elsewhere what the registers give
+template <int P, bool RT>
+constexpr int vib_wpe() { return RT ? 1 : P == 3 ? 3 : P == 2 ? 4 : 1; }
+
 template <int P, int NG, bool RT = false>
-__global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho, VibModes vm, int64_t ebase,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(vib_wpe<P, RT>())))
+vib_grad_kernel(PatchArgs pa, double rho, VibModes vm, int64_t ebase,
                                                        int64_t mbase, double *__restrict__ eg,
                                                        double *__restrict__ em, VibDegrees dg = VibDegrees{}) {
     constexpr int NP1 = P + 1, NN = RT ? GEN_MAXN : NP1 * NP1 * NP1, G = 256 / VQC;
@@ -1473,7 +1479,6 @@ __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho,
     __shared__ double gq[VQC][14];     // du/dx (9, [c][k] = du_c/dx_k), dv, 1/W, dW/du / W (3)
     __shared__ double Rx[NN][VQC][4];  // R, dR/dx
     __shared__ double Tq[VQC][VIB_MAXF][9];
-    __shared__ double gacc[VIB_MAXF][NN][3];
     __shared__ double mpart[VQC][VIB_MAXF];
     __shared__ double hel[3];
     const int t = threadIdx.x;
@@ -1492,7 +1497,11 @@ __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho,
                 for (int c = 0; c < 3; ++c)
                     uf[f][a][c] = vm.vect[(vm.f0 + f) + (int64_t)vm.nf_total * (c + 3 * cp)];
         }
-        for (int x = t; x < VIB_MAXF * NN * 3; x += 256) (&gacc[0][0][0])[x] = 0.0;
+        // this thread's node sums: (node, mode) x = t + 256 s, a = x mod nn, f = x / nn
+        constexpr int GSL = (NN * VIB_MAXF + 255) / 256;
+        double gr[GSL][3];
+#pragma unroll
+        for (int s = 0; s < GSL; ++s) gr[s][0] = gr[s][1] = gr[s][2] = 0.0;
         double macc = 0.0;   // this thread's (point, mode) share of the element mass
         for (int q0 = 0; q0 < NQ; q0 += VQC) {
             const int nq = NQ - q0 < VQC ? NQ - q0 : VQC;
@@ -1667,7 +1676,10 @@ __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho,
             }
             __syncthreads();
             // 5. per (node, mode): g[m] += sum_q T[m][k] dR/dx_k
-            for (int x = t; x < nn * nf; x += 256) {
+#pragma unroll
+            for (int s = 0; s < GSL; ++s) {
+                const int x = t + 256 * s;
+                if (x >= nn * nf) continue;
                 const int a = x % nn, f = x / nn;
                 double g[3] = {0.0, 0.0, 0.0};
                 for (int c = 0; c < nq; ++c) {
@@ -1676,15 +1688,20 @@ __global__ void __launch_bounds__(256) vib_grad_kernel(PatchArgs pa, double rho,
                     for (int m = 0; m < 3; ++m)
                         g[m] += Tq[c][f][3 * m] * g0 + Tq[c][f][3 * m + 1] * g1 + Tq[c][f][3 * m + 2] * g2;
                 }
-                for (int m = 0; m < 3; ++m) gacc[f][a][m] += g[m];
+#pragma unroll
+                for (int m = 0; m < 3; ++m) gr[s][m] += g[m];
             }
         }
         if ((t & 3) == 0) mpart[(t >> 2) % VQC][(t >> 2) / VQC] = (t >> 2) / VQC < nf ? macc : 0.0;
         __syncthreads();
         const int64_t eb = ebase + gi * nn;
-        for (int x = t; x < nn * VIB_MAXF * 3; x += 256) {
-            const int a = x / (VIB_MAXF * 3), r = x % (VIB_MAXF * 3), f = r / 3, m = r % 3;
-            eg[(eb + a) * (VIB_MAXF * 3) + r] = f < nf ? gacc[f][a][m] : 0.0;
+#pragma unroll
+        for (int s = 0; s < GSL; ++s) {
+            const int x = t + 256 * s;
+            if (x >= nn * VIB_MAXF) continue;
+            const int a = x % nn, f = x / nn;
+#pragma unroll
+            for (int m = 0; m < 3; ++m) eg[(eb + a) * (VIB_MAXF * 3) + 3 * f + m] = f < nf ? gr[s][m] : 0.0;
         }
         if (t < VIB_MAXF) {
             double ms = 0.0;
