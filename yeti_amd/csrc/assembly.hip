@@ -1440,7 +1440,8 @@ __device__ inline void bspline_ders_rt(int i, int p, double u, const double *__r
 //   2. W, dW and dx/du of a point by 256 / VQC threads and a shuffle sum, then du/dx and dv;
 //   3. R and dR/dx of every (node, point) of the chunk in LDS;
 //   4. per (point, mode): u, du/dx, sig, the tensor dv (W' I - 2 (du/dx)^T sig) and the mass;
-//   5. per (node, mode): g += tensor . dR/dx, accumulated in LDS over the chunks.
+//   5. per (node, mode): g += tensor . dR/dx, accumulated over the chunks in the registers of the
+//      (node, mode) pair's thread.
 constexpr int VIB_MAXF = 4, VQC = 16;
 
 struct VibModes {
