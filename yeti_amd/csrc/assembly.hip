@@ -1145,7 +1145,10 @@ struct MassCfg {
     static constexpr int NP1 = P + 1, NN = NP1 * NP1 * NP1, NQ = NG * NG * NG;
     static constexpr int NNP = (NN + 15) / 16 * 16, NT = NNP / 16, NTILE = NT * (NT + 1) / 2;
     static constexpr int MAXT = (NTILE + 3) / 4;                       // tiles per wave
-    static constexpr int MQC = NQ <= 64 ? (NQ + 7) / 8 * 8 : 32;       // points per chunk
+    // points per chunk: at most 32, so that at p = 3 the LDS (26 KB) and registers (compiled for
+    // 4 waves per SIMD) allow 4 workgroups per CU (64-point chunks: 49 KB, 3 per CU, 16.6 ms
+    // against 15.5 ms at 64^3)
+    static constexpr int MQC = NQ <= 32 ? (NQ + 7) / 8 * 8 : 32;
     static constexpr int G = 256 / MQC;                                // threads per point
     static constexpr int RS = MQC + 2;                                 // R row stride (doubles)
     static constexpr int NPAIR = NN * (NN + 1) / 2;
@@ -1159,7 +1162,8 @@ struct MassCfg {
 };
 
 template <int P, int NG, bool WRITE_IDX>
-__global__ void __launch_bounds__(256) mass_kernel(PatchArgs pa, double rho, double *__restrict__ Md,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P == 3 ? 4 : 1)))
+mass_kernel(PatchArgs pa, double rho, double *__restrict__ Md,
                                                    int32_t *__restrict__ Mr, int32_t *__restrict__ Mc) {
     using C = MassCfg<P, NG>;
     constexpr int NP1 = C::NP1, NN = C::NN, NQ = C::NQ, MQC = C::MQC, G = C::G, RS = C::RS;
